@@ -1,0 +1,209 @@
+"""ctypes front end for the test oracles (test infrastructure only).
+
+* ``restatement()`` -- oracle/_oracle/libvoro_oracle.so, the project's CPU
+  restatement of the reference algorithm (oracle/voro_oracle.cc).
+* ``reference()``   -- oracle/_ref/libvororef.so, the unmodified reference
+  sources compiled by oracle/Makefile (absent where the reference checkout is
+  not available and nothing was prebuilt).
+
+Only tests, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg use
+this module, as the checker.  The product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "_oracle", "libvoro_oracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libvororef.so")
+REF_ROOT = "/root/reference"
+
+KIND = {"container": 0, "container_poly": 1, "container_periodic": 2, "container_periodic_poly": 3}
+
+
+class Spec(C.Structure):
+    _fields_ = [("kind", C.c_int), ("geom", C.c_double * 6), ("grid", C.c_int * 3),
+                ("periodic", C.c_int * 3), ("init_mem", C.c_int), ("n_spheres", C.c_int),
+                ("spheres", C.POINTER(C.c_double))]
+
+
+class Out(C.Structure):
+    _fields_ = [("id", C.POINTER(C.c_int)), ("ijk", C.POINTER(C.c_int)), ("q", C.POINTER(C.c_int)),
+                ("exists", C.POINTER(C.c_int)), ("vol", C.POINTER(C.c_double)),
+                ("cent", C.POINTER(C.c_double)), ("nfaces", C.POINTER(C.c_int)),
+                ("nbr_off", C.POINTER(C.c_longlong)), ("nbr", C.POINTER(C.c_int)),
+                ("nbr_cap", C.c_longlong), ("pos", C.POINTER(C.c_double))]
+
+
+class Stats(C.Structure):
+    _fields_ = [("n_planes", C.POINTER(C.c_int)), ("wl_exit", C.POINTER(C.c_int)),
+                ("phase", C.POINTER(C.c_int)), ("max_p", C.POINTER(C.c_int)),
+                ("max_order", C.POINTER(C.c_int))]
+
+
+def build(reference=True, quiet=True):
+    """Builds the restatement and, when the reference checkout is present,
+    the reference library.  Never needed on hosts that got prebuilt files."""
+    targets = ["oracle"]
+    if reference and os.path.isdir(os.path.join(REF_ROOT, "src")):
+        targets.append("ref")
+    subprocess.run(["make", "-s", "-j8", "REF=" + REF_ROOT] + targets, cwd=HERE, check=True,
+                   stdout=subprocess.DEVNULL if quiet else None)
+
+
+_LIBS = {}
+
+
+def _load(path, fn):
+    if path not in _LIBS:
+        if not os.path.exists(path):
+            return None
+        lib = C.CDLL(path)
+        f = getattr(lib, fn)
+        f.restype = C.c_longlong
+        _LIBS[path] = (lib, f)
+    return _LIBS[path][1]
+
+
+def restatement():
+    f = _load(ORACLE_SO, "orc_compute_neighbor")
+    if f is None:
+        raise RuntimeError("oracle restatement not built: run oracle.build()")
+    return f
+
+
+def reference():
+    """The reference-compiled oracle, or None where it is unavailable."""
+    return _load(REF_SO, "ref_compute_neighbor")
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def make_spec(kind, geom, grid, periodic=(0, 0, 0), spheres=None, init_mem=8):
+    s = Spec()
+    s.kind = KIND[kind] if isinstance(kind, str) else int(kind)
+    for t in range(6):
+        s.geom[t] = float(geom[t])
+    for t in range(3):
+        s.grid[t] = int(grid[t])
+        s.periodic[t] = int(periodic[t])
+    s.init_mem = init_mem
+    keep = None
+    if spheres is not None and len(spheres):
+        keep = np.ascontiguousarray(np.asarray(spheres, dtype=np.float64).reshape(-1, 5))
+        s.n_spheres = keep.shape[0]
+        s.spheres = _ptr(keep, C.c_double)
+    else:
+        s.n_spheres = 0
+    s._keep = keep
+    return s
+
+
+def run(which, spec, ids, xyz, r=None, stats=False, nbr_per_cell=40):
+    """Runs one oracle; returns a dict of numpy arrays in c_loop_all order."""
+    f = restatement() if which == "restatement" else reference()
+    if f is None:
+        raise RuntimeError("reference oracle not available")
+    n = len(ids)
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
+    cap = max(1, n * nbr_per_cell)
+    while True:
+        res = {k: np.zeros(n, np.int32) for k in ("id", "ijk", "q", "exists", "nfaces")}
+        res["vol"] = np.zeros(n)
+        res["cent"] = np.zeros((n, 3))
+        res["pos"] = np.zeros((n, 3))
+        res["nbr_off"] = np.zeros(n + 1, np.int64)
+        res["nbr"] = np.zeros(cap, np.int32)
+        o = Out()
+        for k in ("id", "ijk", "q", "exists", "nfaces"):
+            setattr(o, k, _ptr(res[k], C.c_int))
+        o.vol = _ptr(res["vol"], C.c_double)
+        o.cent = _ptr(res["cent"], C.c_double)
+        o.pos = _ptr(res["pos"], C.c_double)
+        o.nbr_off = _ptr(res["nbr_off"], C.c_longlong)
+        o.nbr = _ptr(res["nbr"], C.c_int)
+        o.nbr_cap = cap
+        st = None
+        if stats and which == "restatement":
+            sres = {k: np.zeros(n, np.int32) for k in ("n_planes", "wl_exit", "phase", "max_p", "max_order")}
+            st = Stats()
+            for k, v in sres.items():
+                setattr(st, k, _ptr(v, C.c_int))
+        args = [C.byref(spec), C.c_int(n), _ptr(ids, C.c_int), _ptr(xyz, C.c_double),
+                None if rr is None else _ptr(rr, C.c_double), C.byref(o)]
+        if which == "restatement":
+            args.append(C.byref(st) if st is not None else None)
+        else:
+            secs = C.c_double(0)
+            args.append(C.byref(secs))
+        m = f(*args)
+        if m < 0:
+            raise RuntimeError("oracle reported an internal error (%d)" % m)
+        if res["nbr_off"][m] > cap:
+            cap = int(res["nbr_off"][m]) + 16
+            continue
+        break
+    m = int(m)
+    out = {k: v[:m] for k, v in res.items() if k not in ("nbr_off", "nbr")}
+    out["nbr_off"] = res["nbr_off"][:m + 1]
+    out["nbr"] = res["nbr"][:int(res["nbr_off"][m])]
+    if st is not None:
+        for k, v in sres.items():
+            out["st_" + k] = v[:m]
+    return out
+
+
+def time_reference(spec, ids, xyz, r=None, loop="compute_all_cells"):
+    """Seconds the reference takes for compute_all_cells() (or the
+    voronoicell_neighbor output loop), steady_clock around the call only."""
+    lib = _load(REF_SO, "ref_compute_neighbor") and _LIBS[REF_SO][0]
+    if not lib:
+        return None
+    fn = lib.ref_time_compute_all_cells if loop == "compute_all_cells" else lib.ref_time_neighbor_loop
+    fn.restype = C.c_double
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
+    return fn(C.byref(spec), C.c_int(len(ids)), _ptr(ids, C.c_int), _ptr(xyz, C.c_double),
+              None if rr is None else _ptr(rr, C.c_double))
+
+
+def compare(a, b, vol_rtol=1e-12):
+    """Returns a list of human-readable mismatches between two result dicts
+    (empty when identical to the stated tolerance)."""
+    errs = []
+    for k in ("id", "ijk", "q", "exists", "nfaces"):
+        if len(a[k]) != len(b[k]):
+            return ["%s length %d vs %d" % (k, len(a[k]), len(b[k]))]
+        bad = np.nonzero(a[k] != b[k])[0]
+        if len(bad):
+            errs.append("%s differs at %d cells (first loop index %d)" % (k, len(bad), bad[0]))
+    if not np.array_equal(a["nbr_off"], b["nbr_off"]) or not np.array_equal(a["nbr"], b["nbr"]):
+        na = np.diff(a["nbr_off"])
+        nb = np.diff(b["nbr_off"])
+        bad = np.nonzero(na != nb)[0]
+        if len(bad) == 0:
+            for i in range(len(na)):
+                s, e = a["nbr_off"][i], a["nbr_off"][i + 1]
+                if not np.array_equal(a["nbr"][s:e], b["nbr"][s:e]):
+                    bad = [i]
+                    break
+        errs.append("neighbour lists differ (first loop index %s)" % (bad[0] if len(bad) else "?"))
+    va, vb = a["vol"], b["vol"]
+    rel = np.abs(va - vb) / np.maximum(np.abs(vb), 1e-300)
+    rel[(va == 0) & (vb == 0)] = 0
+    if np.any(rel > vol_rtol):
+        errs.append("volume rel err %.3g at loop index %d" % (rel.max(), int(np.argmax(rel))))
+    ca, cb = a["cent"], b["cent"]
+    scale = np.maximum(np.abs(cb).max(axis=1), 1e-300)
+    crel = np.abs(ca - cb).max(axis=1) / scale
+    crel[np.all(ca == cb, axis=1)] = 0
+    if np.any(crel > 1e-9):
+        errs.append("centroid rel err %.3g at loop index %d" % (crel.max(), int(np.argmax(crel))))
+    return errs
