@@ -1,0 +1,125 @@
+"""Benchmark: cells/s of the per-particle cell computation on one MI355X.
+
+Workload: 1,000,000 uniform FP64 particles (seed 42) in the non-periodic cube
+[0,100]^3 with 58^3 blocks (~5 per block), voronoicell_neighbor outputs
+(volume, centroid, face count, neighbour ids in loop-order CSR).  A step is
+one pass of the hot path over the particles already resident in HBM:
+stable block binning + every cell + output compaction.
+
+Prints one JSON line (see the keys below).  The CPU baseline is the
+reference's own container::compute_all_cells() (built from the reference
+sources into oracle/_ref), timed on this host, one thread, on the same
+particles.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Per-cell algorithmic work of the cell kernel ("Measurement"):
+FLOP_PER_CELL = 7.6e3          # FP64 operations per cell (uniform workload)
+BYTES_PER_CELL_FIXED = 36 + 44  # own particle record in + fixed outputs out
+MI355X_HBM_GBS = 8000.0
+MI355X_FP64_TFLOPS = 78.6
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-sample", type=int, default=1_000_000,
+                    help="particles of the workload timed on the CPU")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (--gpus 1); multi-GPU runs are not part of it",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import voro_amd
+    from voro_amd import inputs as I
+
+    ids, xyz, _ = I.make("uniform", 42, n=args.n)
+    dev = torch.device("cuda:0")
+    d_ids = torch.from_numpy(ids).to(dev)
+    d_xyz = torch.from_numpy(xyz).to(dev)
+    torch.cuda.synchronize()
+
+    con = voro_amd.container(0, 100, 0, 100, 0, 100, 58, 58, 58, False, False, False, 8)
+    con.put_device(len(ids), d_ids.data_ptr(), d_xyz.data_ptr())
+    flags = voro_amd.F_NEIGHBORS | voro_amd.F_CSR
+    for _ in range(args.warmup):
+        con.compute_all_cells(flags)
+    torch.cuda.synchronize()
+    fast_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        con.compute_all_cells(flags)
+        fast_ms.append(con.timings()["fast_ms"])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    r = con.results()
+    ncells = len(r["id"])
+    nnbr = len(r["nbr"])
+    cells_s = ncells * args.steps / dt
+    kern_s = float(np.mean(fast_ms)) / 1e3
+    bytes_launch = ncells * BYTES_PER_CELL_FIXED + 4.0 * nnbr
+    hbm_gbs = bytes_launch / kern_s / 1e9
+    fp64_tf = ncells * FLOP_PER_CELL / kern_s / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_k_cells_uniform1m.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        if O.reference() is not None:
+            m = min(args.cpu_sample, args.n)
+            sp = O.make_spec(0, (0, 100, 0, 100, 0, 100), (58, 58, 58))
+            if m < args.n:
+                # the first m particles of the same draw are a uniform subset;
+                # scale the grid to keep ~5 particles per block
+                g = max(1, int(round(58 * (m / args.n) ** (1 / 3))))
+                sp = O.make_spec(0, (0, 100, 0, 100, 0, 100), (g, g, g))
+            secs = O.time_reference(sp, ids[:m], xyz[:m], loop="compute_all_cells")
+            cpu = {"value": m / secs, "unit": "cells/s", "cores": 1, "kind": "reference",
+                   "sample": "container::compute_all_cells() on %d of the uniform particles, %.1f s" % (m, secs)}
+
+    line = {
+        "metric": "cells/s at 1M uniform pts, learned vs plain GPU path (bit-exact nbrs, 1e-12 vol)",
+        "value": cells_s,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "uniform 1M particles in [0,100]^3, 58^3 blocks, seed 42, neighbour outputs",
+                   "n_particles": int(args.n), "grid": [58, 58, 58], "path": "plain"},
+        "roofline": {"bound": "hbm", "achieved": hbm_gbs, "peak": MI355X_HBM_GBS, "unit": "GB/s",
+                     "frac": hbm_gbs / MI355X_HBM_GBS, "traffic": traffic,
+                     "kernel": "k_cells<TierFast>", "kernel_ms": kern_s * 1e3},
+        "roofline_fp64": {"achieved": fp64_tf, "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
+                          "frac": fp64_tf / MI355X_FP64_TFLOPS},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

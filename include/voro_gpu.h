@@ -1,0 +1,123 @@
+/* voro_gpu.h -- C ABI of the MI355X (gfx950) Voronoi cell engine.
+ *
+ * Plain C types only: pointers, sizes, status codes.  One handle is one
+ * particle container; the calls on a handle are not re-entrant (as the
+ * reference's containers are not, src/v_compute.hh:103-119), and a handle is
+ * used from one host thread at a time.
+ *
+ * Each entry point names the reference interface it stands in for
+ * (paths relative to the reference checkout).
+ */
+#ifndef VORO_GPU_H
+#define VORO_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  1..4 are the reference's exit statuses
+ * (src/config.hh:103-119), returned instead of calling exit(). */
+#define VGPU_OK 0
+#define VGPU_FILE_ERROR 1
+#define VGPU_MEMORY_ERROR 2      /* a capacity limit was exceeded */
+#define VGPU_INTERNAL_ERROR 3    /* inconsistent cell (reference: fatal) */
+#define VGPU_CMD_LINE_ERROR 4
+#define VGPU_INVALID_ARGUMENT 5
+#define VGPU_HIP_ERROR 6         /* a HIP runtime call failed */
+#define VGPU_DUPLICATE 7         /* periodic put of a duplicate particle (reference: exit(1)) */
+#define VGPU_NO_DEVICE 8
+
+/* Container kinds. */
+#define VGPU_CONTAINER 0                /* src/container.hh:288  container */
+#define VGPU_CONTAINER_POLY 1           /* src/container.hh:510  container_poly */
+#define VGPU_CONTAINER_PERIODIC 2       /* src/container_prd.hh:213 container_periodic */
+#define VGPU_CONTAINER_PERIODIC_POLY 3  /* src/container_prd.hh:434 container_periodic_poly */
+
+/* vgpu_compute_all flags. */
+#define VGPU_F_NEIGHBORS 1u   /* produce neighbour lists (voronoicell_neighbor) */
+#define VGPU_F_STATS 2u       /* record per-cell candidate statistics */
+#define VGPU_F_LEARNED 4u     /* use the trained candidate predictor if one is loaded */
+#define VGPU_F_CSR 8u         /* compact neighbour lists into loop-order CSR */
+
+typedef struct vgpu_container vgpu_container;
+
+/* Creates a container.
+ *   rectangular kinds: geom = {ax,bx,ay,by,az,bz}, periodic = {xp,yp,zp}
+ *     (src/container.hh:290-291, container(ax_,bx_,...,nx_,ny_,nz_,xp,yp,zp,init_mem))
+ *   periodic kinds:    geom = {bx,bxy,by,bxz,byz,bz}, periodic ignored
+ *     (src/container_prd.hh:215-216, 434-441)
+ * init_mem is accepted for signature parity and ignored: storage is sized
+ * from the particle count. */
+int vgpu_create(int kind, const double *geom, const int *grid, const int *periodic, int init_mem,
+		vgpu_container **out);
+
+/* Frees the container and its device memory. */
+int vgpu_destroy(vgpu_container *h);
+
+/* Adds a sphere wall (src/wall.hh:27-30 wall_sphere(xc,yc,zc,rc,w_id),
+ * src/container.hh:63-69 add_wall).  Rectangular containers only. */
+int vgpu_add_wall_sphere(vgpu_container *h, double xc, double yc, double zc, double rc, int w_id);
+
+/* Appends n particles from host memory, in call order, exactly as n calls
+ * of put(id,x,y,z[,r]) would (src/container.cc:90-115,
+ * src/container_prd.cc:107-123).  xyz is n x 3 doubles; r may be NULL for
+ * the plain kinds and is required for the _poly kinds.  Out-of-box
+ * particles of non-periodic directions are dropped, as the reference does. */
+int vgpu_put(vgpu_container *h, int n, const int *ids, const double *xyz, const double *r);
+
+/* Replaces the container's particles by n particles already in device
+ * memory (same layout as vgpu_put).  Nothing is copied to the host. */
+int vgpu_put_device(vgpu_container *h, int n, const int *d_ids, const double *d_xyz, const double *d_r);
+
+/* Removes all particles (src/container.hh:332-335 clear()). */
+int vgpu_clear(vgpu_container *h);
+
+/* Bins the particles and computes every cell, storing the results
+ * (src/container.cc:438-443 compute_all_cells(); the reference discards the
+ * cells, this engine keeps volume, centroid, face count, existence and, with
+ * VGPU_F_NEIGHBORS, neighbour ids for c_loop_all reads).  `stream` is a
+ * hipStream_t (NULL for the default stream); the call returns when the
+ * results are complete. */
+int vgpu_compute_all(vgpu_container *h, unsigned flags, void *stream);
+
+/* Number of particles stored (the c_loop_all length). */
+int vgpu_num_cells(vgpu_container *h, long long *n);
+
+/* Copies results to host arrays, in c_loop_all order (blocks by ascending
+ * index, then storage order; src/c_loops.hh:165-209).  Any pointer may be
+ * NULL.  nbr_off has n+1 entries; nbr has nbr_cap entries and receives
+ * neighbour ids in the order voronoicell_neighbor::neighbors() gives them
+ * (src/cell.cc:2660-2678).  *nbr_total receives the total id count even
+ * when it exceeds nbr_cap. */
+int vgpu_get_results(vgpu_container *h, int *id, int *ijk, int *q, int *exists, double *pos, double *vol,
+		     double *centroid, int *nfaces, long long *nbr_off, int *nbr, long long nbr_cap,
+		     long long *nbr_total);
+
+/* Per-cell statistics recorded with VGPU_F_STATS: nplane calls, worklist
+ * exit index (64 = reached the block search), and the tier that produced
+ * the cell (0 fast, 1 wide, 2 huge).  Loop order. */
+int vgpu_get_stats(vgpu_container *h, int *n_planes, int *wl_exit, int *tier);
+
+/* Sum of cell volumes in loop order (src/container.cc:451-456
+ * sum_cell_volumes()). */
+int vgpu_sum_volumes(vgpu_container *h, double *out);
+
+/* Device pointers to the loop-order results (valid until the next put or
+ * compute).  Any pointer may be NULL. */
+int vgpu_device_results(vgpu_container *h, const double **vol, const double **centroid, const int **nfaces,
+			const int **status, const int **nbr_start, const int **nbr);
+
+/* Tuning: workgroups of the fast-tier kernel (0 = automatic). */
+int vgpu_set_launch(vgpu_container *h, int fast_waves);
+
+/* Time of the last vgpu_compute_all, split by stage, in milliseconds, from
+ * HIP events: [0] binning, [1] fast tier, [2] wide+huge tiers, [3] outputs. */
+int vgpu_last_timings(vgpu_container *h, double *ms4);
+
+/* Human-readable text of a status code. */
+const char *vgpu_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,150 @@
+// CPU build of the engine's device code, for tests only.
+//
+// Compiles voro_amd/csrc/compute_dev.h with interleave 1 and drives it from a
+// plain host loop (stable counting-sort binning, one cell at a time, tier
+// escalation on overflow), so that the kernel logic can be checked against
+// the oracle on hosts without a GPU.  It is not a product path: the C-ABI
+// library never falls back to it.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../voro_amd/csrc/compute_dev.h"
+#include "../../voro_amd/csrc/grid_host.h"
+
+using namespace vgpu;
+
+extern "C" {
+struct hx_spec {
+	int kind; double geom[6]; int grid[3]; int periodic[3]; int init_mem; int n_spheres; const double *spheres;
+};
+struct hx_out {
+	int *id; int *ijk; int *q; int *exists; double *vol; double *cent; int *nfaces;
+	long long *nbr_off; int *nbr; long long nbr_cap; double *pos;
+};
+}
+
+namespace {
+struct VecSink {
+	std::vector<int> *v;
+	void put(int i, int x) { (void)i; v->push_back(x); }
+};
+
+template<class TR>
+struct Runner {
+	typedef Layout<TR, 1> L;
+	std::vector<char> slab;
+	VCell<TR, 1> c;
+	unsigned stamp = 0;
+	explicit Runner(double mls) : slab(L::bytes + 2 * 65536, 0) { c.setup(slab.data() + 65536, 0, mls); }
+	template<bool POLY>
+	int run(const RectView &v, int ijk, int s, int ci, int cj, int ck, double *vol, double *cent, int *nf,
+		std::vector<int> &nb, CellStats &st) {
+		bool ok = compute_cell_rect<TR, 1, POLY>(c, v, ijk, s, ci, cj, ck, ++stamp, st);
+		if (c.ovf) return c.fatal ? -2 : -1;
+		if (ok) {
+			VecSink sk{&nb};
+			c.outputs(*vol, cent[0], cent[1], cent[2], *nf, sk);
+		}
+		return ok ? 1 : 0;
+	}
+};
+
+static inline int step_int(double a) { return a < 0 ? int(a) - 1 : int(a); }
+static inline int step_mod(int a, int b) { return a >= 0 ? a % b : b - 1 - (b - 1 - a) % b; }
+}  // namespace
+
+extern "C" {
+
+// tier_counts[3]: cells finished by each tier.  Returns cells, or -1 on a
+// cell no tier could finish.
+long long hx_compute_neighbor(const hx_spec *s, int n, const int *ids, const double *xyz, const double *r,
+			      hx_out *o, int *stats_out /* 4 ints per cell or null */, long long *tier_counts) {
+	if (s->kind > 1) return -2;
+	const bool poly = s->kind == 1;
+	RectView v;
+	v.ax = s->geom[0]; v.bx = s->geom[1]; v.ay = s->geom[2]; v.by = s->geom[3]; v.az = s->geom[4]; v.bz = s->geom[5];
+	v.nx = s->grid[0]; v.ny = s->grid[1]; v.nz = s->grid[2]; v.nxy = v.nx * v.ny;
+	v.xper = s->periodic[0]; v.yper = s->periodic[1]; v.zper = s->periodic[2];
+	v.boxx = (v.bx - v.ax) / v.nx; v.boxy = (v.by - v.ay) / v.ny; v.boxz = (v.bz - v.az) / v.nz;
+	v.xsp = 1 / v.boxx; v.ysp = 1 / v.boxy; v.zsp = 1 / v.boxz;
+	v.hx = v.xper ? 2 * v.nx + 1 : v.nx; v.hy = v.yper ? 2 * v.ny + 1 : v.ny; v.hz = v.zper ? 2 * v.nz + 1 : v.nz;
+	v.bxsq = v.boxx * v.boxx + v.boxy * v.boxy + v.boxz * v.boxz;
+	v.max_len_sq = (v.bx - v.ax) * (v.bx - v.ax) * (v.xper ? 0.25 : 1) + (v.by - v.ay) * (v.by - v.ay) * (v.yper ? 0.25 : 1)
+		       + (v.bz - v.az) * (v.bz - v.az) * (v.zper ? 0.25 : 1);
+	const int nxyz = v.nxy * v.nz;
+	// binning: block index as put_remap computes it; stable by input order
+	std::vector<int> blk(n, -1), cnt(nxyz + 1, 0);
+	std::vector<dbl4> rem(n);
+	double maxr = 0;
+	for (int t = 0; t < n; t++) {
+		double x = xyz[3 * t], y = xyz[3 * t + 1], z = xyz[3 * t + 2];
+		int i = step_int((x - v.ax) * v.xsp), j, k, l;
+		if (v.xper) { l = step_mod(i, v.nx); x += v.boxx * (l - i); i = l; } else if (i < 0 || i >= v.nx) continue;
+		j = step_int((y - v.ay) * v.ysp);
+		if (v.yper) { l = step_mod(j, v.ny); y += v.boxy * (l - j); j = l; } else if (j < 0 || j >= v.ny) continue;
+		k = step_int((z - v.az) * v.zsp);
+		if (v.zper) { l = step_mod(k, v.nz); z += v.boxz * (l - k); k = l; } else if (k < 0 || k >= v.nz) continue;
+		blk[t] = i + v.nx * j + v.nxy * k;
+		double rr = r ? r[t] : 0.5;
+		rem[t] = dbl4{x, y, z, rr};
+		if (poly && maxr < rr) maxr = rr;
+		cnt[blk[t] + 1]++;
+	}
+	for (int b = 0; b < nxyz; b++) cnt[b + 1] += cnt[b];
+	std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+	int m = cnt[nxyz];
+	std::vector<dbl4> P(m);
+	std::vector<int> pid(m);
+	for (int t = 0; t < n; t++) if (blk[t] >= 0) { int d = fill[blk[t]]++; P[d] = rem[t]; pid[d] = ids[t]; }
+	std::vector<double> mrad(4096), walls;
+	worklist_radii(v.boxx, v.boxy, v.boxz, mrad.data());
+	for (int w = 0; w < s->n_spheres; w++) for (int t = 0; t < 5; t++) walls.push_back(s->spheres[5 * w + t]);
+	v.P = P.data(); v.pid = pid.data(); v.bstart = cnt.data(); v.mrad = mrad.data(); v.wl = vgpu_worklist_table;
+	v.walls = walls.data(); v.nwalls = s->n_spheres; v.max_radius = maxr;
+
+	Runner<TierFast> r0(v.max_len_sq);
+	Runner<TierWide> r1(v.max_len_sq);
+	Runner<TierHuge> *r2 = 0;
+	std::vector<int> nb;
+	long long nn = 0;
+	int cell = 0;
+	bool bad = false;
+	for (int ijk = 0; ijk < nxyz; ijk++) {
+		int ci = ijk % v.nx, cj = (ijk / v.nx) % v.ny, ck = ijk / v.nxy;
+		for (int q = cnt[ijk]; q < cnt[ijk + 1]; q++, cell++) {
+			double vol = 0, cent[3] = {0, 0, 0};
+			int nf = 0, res;
+			CellStats st;
+			nb.clear();
+			int tier = 0;
+			res = poly ? r0.run<true>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st)
+				   : r0.run<false>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st);
+			if (res < 0) {
+				tier = 1; nb.clear();
+				res = poly ? r1.run<true>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st)
+					   : r1.run<false>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st);
+			}
+			if (res < 0) {
+				tier = 2; nb.clear();
+				if (!r2) r2 = new Runner<TierHuge>(v.max_len_sq);
+				res = poly ? r2->run<true>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st)
+					   : r2->run<false>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st);
+			}
+			if (res < 0) { bad = true; res = 0; }
+			if (tier_counts) tier_counts[tier]++;
+			if (stats_out) { stats_out[4 * cell] = st.n_planes; stats_out[4 * cell + 1] = st.wl_exit; stats_out[4 * cell + 2] = st.phase; stats_out[4 * cell + 3] = tier; }
+			o->id[cell] = pid[q]; o->ijk[cell] = ijk; o->q[cell] = q - cnt[ijk]; o->exists[cell] = res;
+			if (o->pos) { o->pos[3 * cell] = P[q].x; o->pos[3 * cell + 1] = P[q].y; o->pos[3 * cell + 2] = P[q].z; }
+			o->vol[cell] = vol; o->cent[3 * cell] = cent[0]; o->cent[3 * cell + 1] = cent[1]; o->cent[3 * cell + 2] = cent[2];
+			o->nfaces[cell] = res ? nf : 0;
+			o->nbr_off[cell] = nn;
+			if (res) for (size_t t = 0; t < nb.size(); t++) { if (nn < o->nbr_cap) o->nbr[nn] = nb[t]; nn++; }
+		}
+	}
+	o->nbr_off[cell] = nn;
+	delete r2;
+	return bad ? -1 : cell;
+}
+
+}

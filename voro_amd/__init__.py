@@ -1,0 +1,314 @@
+"""voro_amd -- MI355X (gfx950) engine for Voro++-style per-particle Voronoi cells.
+
+Python mirror of the reference's container interface for the hot path
+(src/container.hh:288-728, src/c_loops.hh:165-209, src/cell.hh:432-549),
+over the C ABI of include/voro_gpu.h (libvoro_gpu.so, built in-tree by
+voro_amd/build.py).  There is no CPU fallback: if the library is missing or
+no GPU is visible, construction raises.
+
+    con = container(0, 1, 0, 1, 0, 1, 6, 6, 6, False, False, False, 8)
+    for i, (x, y, z) in enumerate(pts):
+        con.put(i, x, y, z)            # or con.put_array(ids, xyz)
+    con.compute_all_cells()
+    vl = c_loop_all(con)
+    c = voronoicell_neighbor(con)
+    if vl.start():
+        while True:
+            if con.compute_cell(c, vl):
+                c.volume(); c.centroid(); c.number_of_faces(); c.neighbors()
+            if not vl.inc():
+                break
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libvoro_gpu.so")
+
+OK = 0
+MEMORY_ERROR = 2
+INTERNAL_ERROR = 3
+INVALID_ARGUMENT = 5
+HIP_ERROR = 6
+DUPLICATE = 7
+NO_DEVICE = 8
+
+F_NEIGHBORS = 1
+F_STATS = 2
+F_LEARNED = 4
+F_CSR = 8
+
+KIND_CONTAINER = 0
+KIND_POLY = 1
+KIND_PERIODIC = 2
+KIND_PERIODIC_POLY = 3
+
+_lib = None
+
+
+class VoroGPUError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        msg = _lib.vgpu_status_string(status).decode() if _lib is not None else str(status)
+        super().__init__("%s: %s (status %d)" % (what, msg, status))
+
+
+def lib():
+    """The loaded engine library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("voro_amd: %s is missing; build it with voro_amd.build.build()" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    p = C.c_void_p
+    i32p = C.POINTER(C.c_int)
+    f64p = C.POINTER(C.c_double)
+    i64p = C.POINTER(C.c_longlong)
+    sig = {
+        "vgpu_create": (C.c_int, [C.c_int, f64p, i32p, i32p, C.c_int, C.POINTER(p)]),
+        "vgpu_destroy": (C.c_int, [p]),
+        "vgpu_add_wall_sphere": (C.c_int, [p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]),
+        "vgpu_put": (C.c_int, [p, C.c_int, i32p, f64p, f64p]),
+        "vgpu_put_device": (C.c_int, [p, C.c_int, p, p, p]),
+        "vgpu_clear": (C.c_int, [p]),
+        "vgpu_compute_all": (C.c_int, [p, C.c_uint, p]),
+        "vgpu_num_cells": (C.c_int, [p, i64p]),
+        "vgpu_get_results": (C.c_int, [p, i32p, i32p, i32p, i32p, f64p, f64p, f64p, i32p, i64p, i32p,
+                                       C.c_longlong, i64p]),
+        "vgpu_get_stats": (C.c_int, [p, i32p, i32p, i32p]),
+        "vgpu_sum_volumes": (C.c_int, [p, f64p]),
+        "vgpu_set_launch": (C.c_int, [p, C.c_int]),
+        "vgpu_last_timings": (C.c_int, [p, f64p]),
+        "vgpu_status_string": (C.c_char_p, [C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def _check(st, what):
+    if st != OK:
+        raise VoroGPUError(st, what)
+
+
+class _Base:
+    """Common part of the container mirrors: a handle on the C ABI."""
+
+    kind = KIND_CONTAINER
+
+    def __init__(self, geom, grid, periodic, init_mem):
+        L = lib()
+        g = np.asarray(geom, dtype=np.float64)
+        n = np.asarray(grid, dtype=np.int32)
+        pr = np.asarray([1 if b else 0 for b in periodic], dtype=np.int32)
+        h = C.c_void_p()
+        _check(L.vgpu_create(self.kind, _p(g, C.c_double), _p(n, C.c_int), _p(pr, C.c_int), int(init_mem),
+                             C.byref(h)), "vgpu_create")
+        self._h = h
+        self.geom = tuple(float(v) for v in g)
+        self.nx, self.ny, self.nz = (int(v) for v in n)
+        self.periodic = tuple(bool(v) for v in pr)
+        self._res = None
+        self._flags = F_NEIGHBORS
+        self._walls = []
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.vgpu_destroy(h)
+            self._h = None
+
+    # -- particles ------------------------------------------------------
+    def put_array(self, ids, xyz, r=None):
+        """Appends particles in order, as repeated put() calls would."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
+        _check(lib().vgpu_put(self._h, len(ids), _p(ids, C.c_int), _p(xyz, C.c_double), _p(rr, C.c_double)),
+               "vgpu_put")
+        self._res = None
+
+    def put_device(self, n, ids_ptr, xyz_ptr, r_ptr=None):
+        """Replaces the particles by n particles already in device memory
+        (raw device pointers, e.g. torch tensor data_ptr())."""
+        _check(lib().vgpu_put_device(self._h, int(n), C.c_void_p(ids_ptr), C.c_void_p(xyz_ptr),
+                                     C.c_void_p(r_ptr) if r_ptr else None), "vgpu_put_device")
+        self._res = None
+
+    def clear(self):
+        _check(lib().vgpu_clear(self._h), "vgpu_clear")
+        self._res = None
+
+    def add_wall(self, wall):
+        """Adds a wall (wall_sphere only)."""
+        _check(lib().vgpu_add_wall_sphere(self._h, wall.xc, wall.yc, wall.zc, wall.rc, wall.w_id),
+               "vgpu_add_wall_sphere")
+        self._walls.append(wall)
+        self._res = None
+
+    # -- compute ---------------------------------------------------------
+    def compute_all_cells(self, flags=F_NEIGHBORS, stream=None):
+        """Bins the particles and computes every cell (src/container.cc:438-443),
+        keeping the results for c_loop_all reads."""
+        _check(lib().vgpu_compute_all(self._h, int(flags), stream), "vgpu_compute_all")
+        self._flags = flags
+        self._res = None
+
+    def sum_cell_volumes(self):
+        if self._res is None and not self._computed():
+            self.compute_all_cells()
+        out = C.c_double()
+        _check(lib().vgpu_sum_volumes(self._h, C.byref(out)), "vgpu_sum_volumes")
+        return out.value
+
+    def _computed(self):
+        n = C.c_longlong()
+        return lib().vgpu_num_cells(self._h, C.byref(n)) == OK
+
+    def set_launch(self, fast_waves):
+        _check(lib().vgpu_set_launch(self._h, int(fast_waves)), "vgpu_set_launch")
+
+    def timings(self):
+        a = np.zeros(4)
+        _check(lib().vgpu_last_timings(self._h, _p(a, C.c_double)), "vgpu_last_timings")
+        return {"bin_ms": a[0], "fast_ms": a[1], "retry_ms": a[2], "out_ms": a[3]}
+
+    def results(self):
+        """All stored results as numpy arrays in c_loop_all order."""
+        if self._res is not None:
+            return self._res
+        L = lib()
+        n = C.c_longlong()
+        _check(L.vgpu_num_cells(self._h, C.byref(n)), "vgpu_num_cells")
+        n = n.value
+        r = {k: np.zeros(n, np.int32) for k in ("id", "ijk", "q", "exists", "nfaces")}
+        r["pos"] = np.zeros((n, 3))
+        r["vol"] = np.zeros(n)
+        r["cent"] = np.zeros((n, 3))
+        r["nbr_off"] = np.zeros(n + 1, np.int64)
+        tot = C.c_longlong()
+        _check(L.vgpu_get_results(self._h, _p(r["id"], C.c_int), _p(r["ijk"], C.c_int), _p(r["q"], C.c_int),
+                                  _p(r["exists"], C.c_int), _p(r["pos"], C.c_double), _p(r["vol"], C.c_double),
+                                  _p(r["cent"], C.c_double), _p(r["nfaces"], C.c_int),
+                                  _p(r["nbr_off"], C.c_longlong), None, 0, C.byref(tot)), "vgpu_get_results")
+        nb = np.zeros(max(tot.value, 1), np.int32)
+        if self._flags & F_NEIGHBORS:
+            _check(L.vgpu_get_results(self._h, None, None, None, None, None, None, None, None, None,
+                                      _p(nb, C.c_int), tot.value, C.byref(tot)), "vgpu_get_results")
+        r["nbr"] = nb[:tot.value]
+        self._res = r
+        return r
+
+    def stats(self):
+        n = C.c_longlong()
+        _check(lib().vgpu_num_cells(self._h, C.byref(n)), "vgpu_num_cells")
+        a = np.zeros(n.value, np.int32)
+        b = np.zeros(n.value, np.int32)
+        t = np.zeros(n.value, np.int32)
+        _check(lib().vgpu_get_stats(self._h, _p(a, C.c_int), _p(b, C.c_int), _p(t, C.c_int)), "vgpu_get_stats")
+        return {"n_planes": a, "wl_exit": b, "tier": t}
+
+    # -- per-cell access, as voro_compute::compute_cell(c, vl) ----------------
+    def compute_cell(self, c, vl):
+        """Fills c with the stored cell at the loop position; returns the
+        reference's compute_cell() value (False: the cell was removed)."""
+        r = self.results()
+        m = vl._m
+        c._set(r, m)
+        return bool(r["exists"][m])
+
+
+class container(_Base):
+    """container(ax,bx,ay,by,az,bz,nx,ny,nz,xp,yp,zp,init_mem), src/container.hh:290-291."""
+    kind = KIND_CONTAINER
+
+    def __init__(self, ax, bx, ay, by, az, bz, nx, ny, nz, xp, yp, zp, init_mem=8):
+        super().__init__((ax, bx, ay, by, az, bz), (nx, ny, nz), (xp, yp, zp), init_mem)
+        self._pend = []
+
+    def put(self, n, x, y, z):
+        self.put_array(np.array([n]), np.array([[x, y, z]]))
+
+
+class container_poly(_Base):
+    """container_poly(...), src/container.hh:510-515; put(n,x,y,z,r)."""
+    kind = KIND_POLY
+
+    def __init__(self, ax, bx, ay, by, az, bz, nx, ny, nz, xp, yp, zp, init_mem=8):
+        super().__init__((ax, bx, ay, by, az, bz), (nx, ny, nz), (xp, yp, zp), init_mem)
+
+    def put(self, n, x, y, z, r):
+        self.put_array(np.array([n]), np.array([[x, y, z]]), np.array([r]))
+
+
+class wall_sphere:
+    """wall_sphere(xc,yc,zc,rc,w_id=-99), src/wall.hh:27-30."""
+
+    def __init__(self, xc, yc, zc, rc, w_id=-99):
+        self.xc, self.yc, self.zc, self.rc, self.w_id = float(xc), float(yc), float(zc), float(rc), int(w_id)
+
+
+class c_loop_all:
+    """Loop over all stored particles: blocks by ascending index, then
+    storage order (src/c_loops.hh:165-209)."""
+
+    def __init__(self, con):
+        self.con = con
+        self._m = 0
+
+    def start(self):
+        self._m = 0
+        return len(self.con.results()["id"]) > 0
+
+    def inc(self):
+        self._m += 1
+        return self._m < len(self.con.results()["id"])
+
+    def pid(self):
+        return int(self.con.results()["id"][self._m])
+
+    def pos(self):
+        return tuple(self.con.results()["pos"][self._m])
+
+    @property
+    def ijk(self):
+        return int(self.con.results()["ijk"][self._m])
+
+    @property
+    def q(self):
+        return int(self.con.results()["q"][self._m])
+
+
+class voronoicell_neighbor:
+    """Read-only view of one computed cell (src/cell.hh:432-549 outputs)."""
+
+    def __init__(self, con=None):
+        self._r = None
+        self._m = 0
+
+    def _set(self, r, m):
+        self._r = r
+        self._m = m
+
+    def volume(self):
+        return float(self._r["vol"][self._m])
+
+    def centroid(self):
+        return tuple(float(v) for v in self._r["cent"][self._m])
+
+    def number_of_faces(self):
+        return int(self._r["nfaces"][self._m])
+
+    def neighbors(self):
+        s, e = self._r["nbr_off"][self._m], self._r["nbr_off"][self._m + 1]
+        return [int(v) for v in self._r["nbr"][s:e]]

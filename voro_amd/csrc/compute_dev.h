@@ -1,0 +1,434 @@
+// Per-particle cell computation over the binned block grid: start cell,
+// walls, own block, worklist phases 1 and 2 with the security-radius stop,
+// and the phase-3 breadth-first block search.
+//
+// Restates src/v_compute.cc:291-621 (compute_cell), 634-780 (region tests),
+// 797-944 (compute_min_max_radius), 236-264 (queue/mask helpers), together
+// with the container adapters src/container.hh:192-251 (start cell,
+// frac_pos, region_index), src/rad_option.hh:24-158 (radical policy) and
+// src/wall.cc:28-36 (sphere wall).  The candidate order, the count_list
+// schedule on which the maximum vertex radius is refreshed and every
+// floating-point expression are kept as in the reference, so the sequence
+// of real cuts, and hence the cell, is identical.
+//
+// Differences of representation: the container-wide block mask with its
+// visit stamp becomes a per-lane window of (2W+1)^3 bits around the
+// particle's block whose words carry their own stamp (no clearing between
+// cells), and the growable circular block queue becomes a fixed ring; a cell
+// that leaves the window or fills the ring is recomputed by a wider tier.
+#pragma once
+#include "cell_dev.h"
+
+namespace vgpu {
+
+struct dbl4 { double x, y, z, w; };
+
+// Worklist table (generated; identical to the reference's table).
+#include "worklist_table.h"
+
+// Read-only view of a binned rectangular container.
+struct RectView {
+	double ax, bx, ay, by, az, bz;
+	int nx, ny, nz, nxy;
+	double boxx, boxy, boxz, xsp, ysp, zsp;
+	int xper, yper, zper;
+	int hx, hy, hz;
+	double bxsq, max_len_sq, max_radius;
+	const dbl4 *P;          // sorted particles (x,y,z,r), block-major
+	const int *pid;         // sorted particle ids
+	const int *bstart;      // block start offsets, nx*ny*nz+1
+	const double *mrad;     // 64x64 worklist radii for this block size
+	const unsigned *wl;     // 64x64 worklist table
+	const double *walls;    // sphere walls: xc,yc,zc,rc,w_id
+	int nwalls;
+};
+
+// Per-cell statistics (labels for the candidate predictor).
+struct CellStats {
+	int n_planes;   // nplane calls
+	int wl_exit;    // worklist index at which the radius test stopped (64 = phase 3)
+	int phase;      // 1, 2 or 3
+	int max_p;
+};
+
+template<bool POLY>
+struct RadPolicy {
+	double r_rad, r_mul, r_val, max_radius;
+	VG_INL void init(double r, double maxr) {
+		max_radius = maxr;
+		if (POLY) { r_rad = r * r; r_mul = (r - maxr) * (r + maxr); }
+	}
+	VG_INL void prime(double rv) { if (POLY) r_val = 1 + r_mul / rv; }
+	VG_INL bool ctest(double crs, double mrs) const {
+		if (!POLY) return crs > mrs;
+		double cc = crs + r_mul;
+		return cc > 0 && cc * cc > mrs * crs;
+	}
+	VG_INL double cutoff(double lrs) const { return POLY ? lrs * r_val : lrs; }
+	VG_INL double scale(double rs, double rj) const { return POLY ? rs + r_rad - rj * rj : rs; }
+	VG_INL bool scale_check(double &rs, double mrs, double rj) const {
+		if (!POLY) return rs < mrs;
+		double trs = rs;
+		rs += r_rad - rj * rj;
+		return rs <= 0 || rs * rs < mrs * trs;
+	}
+};
+
+template<class T_, int ST>
+struct BlockSearch {
+	typedef Layout<T_, ST> L;
+	enum { W = L::W, WD = L::WD, MAXQ = L::MAXQ };
+	VCell<T_, ST> &c;
+	unsigned stamp;
+	int ci0, cj0, ck0;      // centre block (mask coordinates)
+	int qh, qt;             // ring head / tail
+	VG_INL BlockSearch(VCell<T_, ST> &c_, unsigned st, int i, int j, int k)
+		: c(c_), stamp(st), ci0(i), cj0(j), ck0(k), qh(0), qt(0) {}
+	VG_INL typename VCell<T_, ST>::template Ref<unsigned> BM(int w) { return c.template A<unsigned>(L::o_bm, w); }
+	VG_INL typename VCell<T_, ST>::template Ref<unsigned> BS(int w) { return c.template A<unsigned>(L::o_bms, w); }
+	VG_INL typename VCell<T_, ST>::template Ref<int> QU(int t) { return c.template A<int>(L::o_qu, t); }
+	// Window index of block (ei,ej,ek), or -1 (overflow) if outside.
+	VG_INL int widx(int ei, int ej, int ek) {
+		int a = ei - ci0 + W, b = ej - cj0 + W, d = ek - ck0 + W;
+		if (a < 0 || a >= WD || b < 0 || b >= WD || d < 0 || d >= WD) { c.ovf = true; return -1; }
+		return a + WD * (b + WD * d);
+	}
+	VG_INL bool marked(int ei, int ej, int ek) {
+		int t = widx(ei, ej, ek);
+		if (t < 0) return true;
+		int w = t >> 5;
+		return BS(w) == stamp && ((BM(w) >> (t & 31)) & 1u);
+	}
+	VG_INL void mark(int ei, int ej, int ek) {
+		int t = widx(ei, ej, ek);
+		if (t < 0) return;
+		int w = t >> 5;
+		unsigned b = 1u << (t & 31);
+		if (BS(w) != stamp) { BS(w) = stamp; BM(w) = b; }
+		else BM(w) |= b;
+	}
+	VG_INL void push(int ei, int ej, int ek) {
+		if (qt - qh >= MAXQ) { c.ovf = true; return; }
+		QU(qt % MAXQ) = (ei - ci0 + 512) | ((ej - cj0 + 512) << 10) | ((ek - ck0 + 512) << 20);
+		qt++;
+	}
+	VG_INL bool pop(int &ei, int &ej, int &ek) {
+		if (qh == qt) return false;
+		int v = QU(qh % MAXQ);
+		qh++;
+		ei = (v & 1023) - 512 + ci0; ej = ((v >> 10) & 1023) - 512 + cj0; ek = ((v >> 20) & 1023) - 512 + ck0;
+		return true;
+	}
+	VG_INL void mark_push(int ei, int ej, int ek) { mark(ei, ej, ek); push(ei, ej, ek); }
+};
+
+// compute_min_max_radius, src/v_compute.cc:797-944.  The reference spells
+// out all 26 neighbour directions; they reduce to seven forms (corner, three
+// edges, three faces) once a negative direction is expressed through its
+// sign: (-b)*x == -(b*x) and a+(-y) == a-y exactly in IEEE arithmetic, so
+// every form evaluates the reference's expression for that direction
+// operation for operation.
+template<bool POLY>
+VG_FN bool min_max_radius(const RectView &v, const RadPolicy<POLY> &rp, int di, int dj, int dk, double fx,
+			  double fy, double fz, double gxs, double gys, double gzs, double &crs, double mrs) {
+	const double boxx = v.boxx, boxy = v.boxy, boxz = v.boxz;
+	const double sx = di > 0 ? 1.0 : -1.0, sy = dj > 0 ? 1.0 : -1.0, sz = dk > 0 ? 1.0 : -1.0;
+	const double xlo = (di > 0 ? di : di + 1) * boxx - fx;
+	const double ylo = (dj > 0 ? dj : dj + 1) * boxy - fy;
+	const double zlo = (dk > 0 ? dk : dk + 1) * boxz - fz;
+	const int form = (di != 0 ? 4 : 0) | (dj != 0 ? 2 : 0) | (dk != 0 ? 1 : 0);
+	// squared distance to the near corner/edge/face, axes in x,y,z order
+	bool first = true;
+	crs = 0;
+	if (di != 0) { crs = xlo * xlo; first = false; }
+	if (dj != 0) { if (first) crs = ylo * ylo; else crs += ylo * ylo; first = false; }
+	if (dk != 0) { if (first) crs = zlo * zlo; else crs += zlo * zlo; }
+	if (form != 0 && rp.ctest(crs, mrs)) return true;
+	const double ax = boxx * ((2 * sx) * xlo + boxx);
+	const double ay = boxy * ((2 * sy) * ylo + boxy);
+	const double az = boxz * ((2 * sz) * zlo + boxz);
+	switch (form) {
+	case 7: crs += v.bxsq + 2 * ((sx * boxx) * xlo + (sy * boxy) * ylo + (sz * boxz) * zlo); break;  // corner
+	case 6: crs += ax + ay + gzs; break;                  // x,y: edge along z
+	case 5: crs += az; crs += gys + ax; break;            // x,z: edge along y
+	case 4: crs += gzs; crs += gys + ax; break;           // x: face
+	case 3: crs += az; crs += ay; crs += gxs; break;      // y,z: edge along x
+	case 2: crs += gzs; crs += ay; crs += gxs; break;     // y: face
+	case 1: crs += az; crs += gys; crs += gxs; break;     // z: face
+	default: crs = 0; crs += gys; crs += gxs; break;      // central block: not reached
+	}
+	return false;
+}
+
+// Region tests, src/v_compute.cc:634-780 (corner_test, edge_[xyz]_test,
+// face_[xyz]_test): returns true when no particle of the block can cut the
+// cell.  Each test probes 6 (corner, edge) or 4 (face) points of the block
+// with plane_intersects, the first with the _guess variant.  Per axis the
+// point takes the near (bit 0) or far (bit 1) coordinate; an axis on which
+// the block is level with the particle's block uses low (0) / high (1).  The
+// plane offset is the sum over the non-level axes of near*coordinate, in x,
+// y, z order, and r_prime gets the sum of the near squares -- exactly the
+// arguments the reference passes.  The probe order per form is tabled here
+// (3 bits per probe, x in bit 2).
+template<class T_, int ST, bool POLY>
+VG_FN bool region_skip(VCell<T_, ST> &c, RadPolicy<POLY> &rp, int ei, int ej, int ek, int i, int j, int k,
+		       double xlo, double xhi, double ylo, double yhi, double zlo, double zhi) {
+	const int sx = ei > i ? 1 : (ei < i ? -1 : 0);
+	const int sy = ej > j ? 1 : (ej < j ? -1 : 0);
+	const int sz = ek > k ? 1 : (ek < k ? -1 : 0);
+	// near / far coordinate per axis
+	const double xn = sx < 0 ? xhi : xlo, xf = sx < 0 ? xlo : xhi;
+	const double yn = sy < 0 ? yhi : ylo, yf = sy < 0 ? ylo : yhi;
+	const double zn = sz < 0 ? zhi : zlo, zf = sz < 0 ? zlo : zhi;
+	const int form = (sx != 0 ? 4 : 0) | (sy != 0 ? 2 : 0) | (sz != 0 ? 1 : 0);
+	// probe sequences, 3 bits each (first probe in the low bits)
+	unsigned seq;
+	int np = 6;
+	switch (form) {
+	case 7: seq = 4u | 6u << 3 | 2u << 6 | 3u << 9 | 1u << 12 | 5u << 15; break;  // corner_test
+	case 3: seq = 1u | 5u << 3 | 4u << 6 | 0u << 9 | 2u << 12 | 6u << 15; break;  // edge_x_test
+	case 5: seq = 1u | 3u << 3 | 2u << 6 | 0u << 9 | 4u << 12 | 6u << 15; break;  // edge_y_test
+	case 6: seq = 2u | 3u << 3 | 1u << 6 | 0u << 9 | 4u << 12 | 5u << 15; break;  // edge_z_test
+	case 4: seq = 0u | 1u << 3 | 3u << 6 | 2u << 9; np = 4; break;                  // face_x_test
+	case 2: seq = 0u | 1u << 3 | 5u << 6 | 4u << 9; np = 4; break;                  // face_y_test
+	case 1: seq = 0u | 2u << 3 | 6u << 6 | 4u << 9; np = 4; break;                  // face_z_test
+	default: c.fatal = c.ovf = true; return true;                                    // central block
+	}
+	double pr;
+	if (form == 7) pr = xn * xn + yn * yn + zn * zn;
+	else if (form == 3) pr = yn * yn + zn * zn;
+	else if (form == 5) pr = xn * xn + zn * zn;
+	else if (form == 6) pr = xn * xn + yn * yn;
+	else if (form == 4) pr = xn * xn;
+	else if (form == 2) pr = yn * yn;
+	else pr = zn * zn;
+	rp.prime(pr);
+	for (int t = 0; t < np; t++) {
+		const unsigned b = (seq >> (3 * t)) & 7u;
+		const double X = (b & 4u) ? xf : xn, Y = (b & 2u) ? yf : yn, Z = (b & 1u) ? zf : zn;
+		double rsq;
+		if (form == 7) rsq = xn * X + yn * Y + zn * Z;
+		else if (form == 3) rsq = yn * Y + zn * Z;
+		else if (form == 5) rsq = xn * X + zn * Z;
+		else if (form == 6) rsq = xn * X + yn * Y;
+		else if (form == 4) rsq = xn * X;
+		else if (form == 2) rsq = yn * Y;
+		else rsq = zn * Z;
+		if (c.plane_probe(X, Y, Z, rp.cutoff(rsq), t == 0)) return false;
+	}
+	return true;
+}
+
+// container_base::region_index, src/container.hh:243-251
+VG_INL int rect_region(const RectView &v, int ci, int cj, int ck, int ei, int ej, int ek, double &qx, double &qy,
+		       double &qz, int disp) {
+	if (v.xper) { if (ci + ei < v.nx) { ei += v.nx; qx = -(v.bx - v.ax); } else if (ci + ei >= (v.nx << 1)) { ei -= v.nx; qx = v.bx - v.ax; } else qx = 0; }
+	if (v.yper) { if (cj + ej < v.ny) { ej += v.ny; qy = -(v.by - v.ay); } else if (cj + ej >= (v.ny << 1)) { ej -= v.ny; qy = v.by - v.ay; } else qy = 0; }
+	if (v.zper) { if (ck + ek < v.nz) { ek += v.nz; qz = -(v.bz - v.az); } else if (ck + ek >= (v.nz << 1)) { ek -= v.nz; qz = v.bz - v.az; } else qz = 0; }
+	return disp + ei + v.nx * (ej + v.ny * ek);
+}
+
+// compute_cell for a rectangular container (src/v_compute.cc:291-621).
+// (ijk,s) is the particle's block and index in the block, (ci,cj,ck) the
+// block coordinates.  Returns the reference's compute_cell result; c.ovf /
+// c.fatal report capacity or internal trouble.
+//
+// The reference's nested loops (walls, own block, worklist phases 1-2, block
+// search) are driven as one resumable candidate generator feeding a single
+// nplane call site.  Block-level decisions (radius refresh, radius stop,
+// block skip, region tests) are taken when the previous block's particles
+// have all been cut, exactly when the reference takes them.  The
+// outside-neighbour bookkeeping of a block (mask marks and queue pushes) is
+// done when the block is chosen rather than after its particles are cut;
+// nothing reads it in between, so the order of cuts is unchanged.
+template<class T_, int ST, bool POLY>
+VG_FN bool compute_cell_rect(VCell<T_, ST> &c, const RectView &v, int ijk, int s, int ci, int cj, int ck,
+			     unsigned stamp, CellStats &stt) {
+	double x, y, z, qx = 0, qy = 0, qz = 0;
+	int i, j, k, di, dj, dk, ei, ej, ek, f = 0, g = 0, disp;
+	double fx = 0, fy = 0, fz = 0, gxs = 0, gys = 0, gzs = 0;
+	unsigned q, m1 = 0, m2 = 0;
+	stt.n_planes = 0; stt.wl_exit = 0; stt.phase = 1;
+
+	const int b0 = v.bstart[ijk], co0 = v.bstart[ijk + 1] - b0;
+	const dbl4 me = v.P[b0 + s];
+	x = me.x; y = me.y; z = me.z;
+	{
+		double xa, xb, ya, yb, za, zb;
+		if (v.xper) { xb = 0.5 * (v.bx - v.ax); xa = -xb; i = v.nx; } else { xa = v.ax - x; xb = v.bx - x; i = ci; }
+		if (v.yper) { yb = 0.5 * (v.by - v.ay); ya = -yb; j = v.ny; } else { ya = v.ay - y; yb = v.by - y; j = cj; }
+		if (v.zper) { zb = 0.5 * (v.bz - v.az); za = -zb; k = v.nz; } else { za = v.az - z; zb = v.bz - z; k = ck; }
+		c.init_box(xa, xb, ya, yb, za, zb);
+	}
+	disp = ijk - i - v.nx * (j + v.ny * k);
+	RadPolicy<POLY> rp;
+	rp.init(me.w, v.max_radius);
+	BlockSearch<T_, ST> bs(c, stamp, i, j, k);
+
+	double mrs = 0, crs = 0;
+	int next_count = 3, cpi = 0;
+	const double *radp = v.mrad;
+	const unsigned *e = v.wl;
+	// Generator state: 0 walls, 1 own block, 2 worklist, 3 block search.
+	int stage = 0, w = 0, t = 0;
+	bool marking = false;
+	int bt = 0, be = 0;           // particle range of the block being cut
+	bool bcheck = false;          // per-particle radius check for this block
+	double x2 = 0, y2 = 0, z2 = 0;
+
+	for (;;) {
+		double px_ = 0, py_ = 0, pz_ = 0, rs = 0;
+		int pid = 0;
+		bool have = false;
+		while (!have) {
+			if (bt < be) {
+				// Next particle of the current block (src/v_compute.cc:402-428).
+				const dbl4 o = v.P[bt];
+				pid = v.pid[bt];
+				bt++;
+				px_ = o.x - x2; py_ = o.y - y2; pz_ = o.z - z2;
+				rs = px_ * px_ + py_ * py_ + pz_ * pz_;
+				if (!bcheck) { rs = rp.scale(rs, o.w); have = true; }
+				else have = rp.scale_check(rs, mrs, o.w);
+				continue;
+			}
+			if (stage == 0) {
+				// Sphere walls (src/container.hh:86-89, src/wall.cc:28-36).
+				if (w < v.nwalls) {
+					const double *wp = v.walls + 5 * w;
+					w++;
+					double xd = x - wp[0], yd = y - wp[1], zd = z - wp[2], dq = xd * xd + yd * yd + zd * zd;
+					if (dq > 1e-5) {
+						dq = 2 * (sqrt(dq) * wp[3] - dq);
+						px_ = xd; py_ = yd; pz_ = zd; rs = dq; pid = (int)wp[4];
+						have = true;
+					}
+					continue;
+				}
+				stage = 1;
+				continue;
+			}
+			if (stage == 1) {
+				// Own block in storage order, skipping the particle itself
+				// (src/v_compute.cc:307-323).
+				if (t < co0) {
+					if (t == s) { t++; continue; }
+					const dbl4 o = v.P[b0 + t];
+					pid = v.pid[b0 + t];
+					t++;
+					px_ = o.x - x; py_ = o.y - y; pz_ = o.z - z;
+					rs = rp.scale(px_ * px_ + py_ * py_ + pz_ * pz_, o.w);
+					have = true;
+					continue;
+				}
+				// Worklist selection (src/v_compute.cc:328-362).
+				mrs = c.max_radius_squared();
+				fx = x - v.ax - v.boxx * ci; fy = y - v.ay - v.boxy * cj; fz = z - v.az - v.boxz * ck;
+				di = int(fx * v.xsp * VGPU_WL_FGRID); dj = int(fy * v.ysp * VGPU_WL_FGRID); dk = int(fz * v.zsp * VGPU_WL_FGRID);
+				if (di >= VGPU_WL_HGRID) {
+					gxs = fx;
+					m1 = 127 + (3 << 21); m2 = 1 + (1 << 21); di = VGPU_WL_FGRID - 1 - di; if (di < 0) di = 0;
+				} else { m1 = m2 = 0; gxs = v.boxx - fx; }
+				if (dj >= VGPU_WL_HGRID) {
+					gys = fy;
+					m1 |= (127 << 7) + (3 << 24); m2 |= (1 << 7) + (1 << 24); dj = VGPU_WL_FGRID - 1 - dj; if (dj < 0) dj = 0;
+				} else gys = v.boxy - fy;
+				if (dk >= VGPU_WL_HGRID) {
+					gzs = fz;
+					m1 |= (127 << 14) + (3 << 27); m2 |= (1 << 14) + (1 << 27); dk = VGPU_WL_FGRID - 1 - dk; if (dk < 0) dk = 0;
+				} else gzs = v.boxz - fz;
+				gxs *= gxs; gys *= gys; gzs *= gzs;
+				const int wli = di + VGPU_WL_HGRID * (dj + VGPU_WL_HGRID * dk);
+				radp = v.mrad + wli * 64;
+				e = v.wl + wli * 64;
+				f = e[0]; g = 0;
+				stage = 2;
+				continue;
+			}
+			if (stage == 2) {
+				if (g >= 63) {
+					// Worklist exhausted (src/v_compute.cc:529).
+					if (rp.ctest(radp[g], mrs)) { stt.wl_exit = g; stt.n_planes = c.n_cuts; return true; }
+					stt.wl_exit = 64;
+					stt.phase = 3;
+					stage = 3;
+					continue;
+				}
+				// Phase 1 is a do-while over g < f (src/v_compute.cc:366-431):
+				// its first iteration always runs.  Phase 2 marks the mask
+				// and queues outside neighbours (src/v_compute.cc:443-526).
+				if (!marking && g >= f && g > 0) { marking = true; stt.phase = 2; }
+				if (g == next_count) {
+					mrs = c.max_radius_squared();
+					if (cpi != 8) {
+						const int count_list[8] = {7, 11, 15, 19, 26, 35, 45, 59};
+						next_count = count_list[cpi++];
+					}
+				}
+				if (rp.ctest(radp[g], mrs)) { stt.wl_exit = g; stt.n_planes = c.n_cuts; return true; }
+				g++;
+				q = e[g]; q ^= m1; q += m2;
+				di = q & 127; di -= 64;
+				dj = (q >> 7) & 127; dj -= 64;
+				dk = (q >> 14) & 127; dk -= 64;
+				ei = di + i; if (ei < 0 || ei >= v.hx) continue;
+				ej = dj + j; if (ej < 0 || ej >= v.hy) continue;
+				ek = dk + k; if (ek < 0 || ek >= v.hz) continue;
+				if (marking) bs.mark(ei, ej, ek);
+				if (min_max_radius<POLY>(v, rp, di, dj, dk, fx, fy, fz, gxs, gys, gzs, crs, mrs)) continue;
+				{
+					const int rijk = rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
+					bt = v.bstart[rijk]; be = v.bstart[rijk + 1];
+					x2 = x - qx; y2 = y - qy; z2 = z - qz;
+					bcheck = rp.ctest(crs, mrs);
+				}
+				if (marking) {
+					// scan_bits_mask_add, src/v_compute.cc:249-264
+					const unsigned b1 = 1u << 21, b2 = 1u << 22, b3 = 1u << 24, b4 = 1u << 25, b5 = 1u << 27, b6 = 1u << 28;
+					if ((q & b2) == b2) {
+						if (ei > 0) bs.mark_push(ei - 1, ej, ek);
+						if ((q & b1) == 0 && ei < v.hx - 1) bs.mark_push(ei + 1, ej, ek);
+					} else if ((q & b1) == b1 && ei < v.hx - 1) bs.mark_push(ei + 1, ej, ek);
+					if ((q & b4) == b4) {
+						if (ej > 0) bs.mark_push(ei, ej - 1, ek);
+						if ((q & b3) == 0 && ej < v.hy - 1) bs.mark_push(ei, ej + 1, ek);
+					} else if ((q & b3) == b3 && ej < v.hy - 1) bs.mark_push(ei, ej + 1, ek);
+					if ((q & b6) == b6) {
+						if (ek > 0) bs.mark_push(ei, ej, ek - 1);
+						if ((q & b5) == 0 && ek < v.hz - 1) bs.mark_push(ei, ej, ek + 1);
+					} else if ((q & b5) == b5 && ek < v.hz - 1) bs.mark_push(ei, ej, ek + 1);
+					if (c.ovf) { stt.n_planes = c.n_cuts; return false; }
+				}
+				continue;
+			}
+			// stage 3: breadth-first over queued blocks (src/v_compute.cc:534-618).
+			if (!bs.pop(ei, ej, ek)) { stt.n_planes = c.n_cuts; return true; }
+			{
+				const double xlo = (ei - i) * v.boxx - fx, xhi = xlo + v.boxx;
+				const double ylo = (ej - j) * v.boxy - fy, yhi = ylo + v.boxy;
+				const double zlo = (ek - k) * v.boxz - fz, zhi = zlo + v.boxz;
+				if (region_skip<T_, ST, POLY>(c, rp, ei, ej, ek, i, j, k, xlo, xhi, ylo, yhi, zlo, zhi)) {
+					if (c.ovf) { stt.n_planes = c.n_cuts; return false; }
+					continue;
+				}
+			}
+			{
+				const int rijk = rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
+				bt = v.bstart[rijk]; be = v.bstart[rijk + 1];
+				x2 = x - qx; y2 = y - qy; z2 = z - qz;
+				bcheck = false;
+			}
+			// add_to_mask, src/v_compute.cc:236-243
+			if (ek > 0 && !bs.marked(ei, ej, ek - 1)) bs.mark_push(ei, ej, ek - 1);
+			if (ej > 0 && !bs.marked(ei, ej - 1, ek)) bs.mark_push(ei, ej - 1, ek);
+			if (ei > 0 && !bs.marked(ei - 1, ej, ek)) bs.mark_push(ei - 1, ej, ek);
+			if (ei < v.hx - 1 && !bs.marked(ei + 1, ej, ek)) bs.mark_push(ei + 1, ej, ek);
+			if (ej < v.hy - 1 && !bs.marked(ei, ej + 1, ek)) bs.mark_push(ei, ej + 1, ek);
+			if (ek < v.hz - 1 && !bs.marked(ei, ej, ek + 1)) bs.mark_push(ei, ej, ek + 1);
+			if (c.ovf) { stt.n_planes = c.n_cuts; return false; }
+		}
+		if (!c.nplane(px_, py_, pz_, rs, pid)) { stt.n_planes = c.n_cuts; return false; }
+	}
+}
+
+}  // namespace vgpu

@@ -1,0 +1,577 @@
+// C-ABI of the engine (include/voro_gpu.h): container handles, particle
+// upload, the binning + cell pipeline, and result read-back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/voro_gpu.h"
+#include "grid_host.h"
+#include "vgpu_bin.h"
+
+using namespace vgpu;
+
+namespace {
+
+#define HIPCHK(x)                                                   \
+	do {                                                        \
+		hipError_t e_ = (x);                                \
+		if (e_ != hipSuccess) { return VGPU_HIP_ERROR; }    \
+	} while (0)
+
+// Grow-only device buffer.
+struct DBuf {
+	void *p = nullptr;
+	size_t n = 0;
+	int ensure(size_t bytes) {
+		if (bytes <= n) return VGPU_OK;
+		if (p) (void)hipFree(p);
+		p = nullptr; n = 0;
+		if (bytes == 0) return VGPU_OK;
+		if (hipMalloc(&p, bytes) != hipSuccess) { p = nullptr; return VGPU_MEMORY_ERROR; }
+		n = bytes;
+		return VGPU_OK;
+	}
+	void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+	template<class T> T *as() const { return (T *)p; }
+};
+
+// Lane-interleaved workspace slabs of one tier, with a front and back pad.
+struct Workspace {
+	DBuf buf;
+	size_t slab = 0;
+	int waves = 0;
+	unsigned stamp_base = 0;
+	static const size_t PAD = 1 << 20;
+	int ensure(size_t slab_bytes, int nwaves) {
+		if (slab_bytes == slab && nwaves <= waves) return VGPU_OK;
+		buf.release();
+		int st = buf.ensure(2 * PAD + slab_bytes * (size_t)nwaves);
+		if (st) return st;
+		// Block-mask stamps must start out invalid: zero everything once.
+		if (hipMemset(buf.p, 0, buf.n) != hipSuccess) return VGPU_HIP_ERROR;
+		slab = slab_bytes;
+		waves = nwaves;
+		stamp_base = 0;
+		return VGPU_OK;
+	}
+	char *base() const { return (char *)buf.p + PAD; }
+};
+
+__global__ void k_fill_int(int *p, int n, int v) {
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < n) p[t] = v;
+}
+
+}  // namespace
+
+struct vgpu_container {
+	int kind = 0;
+	double geom[6] = {0, 0, 0, 0, 0, 0};
+	int grid[3] = {1, 1, 1};
+	int per[3] = {0, 0, 0};
+	RectGeom g;
+	double max_len_sq = 0;
+	std::vector<double> walls;
+	std::vector<int> h_ids;
+	std::vector<double> h_xyz, h_r;
+	bool have_dev_input = false;
+	int n_in = 0;
+	int ncells = 0;
+	double max_radius = 0;
+	bool computed = false;
+	unsigned last_flags = 0;
+	int fast_waves = 0;
+	hipEvent_t ev[5];
+	bool events = false;
+	double timings[4] = {0, 0, 0, 0};
+
+	DBuf in_ids, in_xyz, in_r;
+	DBuf blk, slot, rem, cnt, bstart, tmp, part, P, pid, src, cellblk;
+	DBuf d_mrad, d_wl, d_walls, d_maxr, d_counters;
+	DBuf vol, cent, nfaces, status, nbr_start, arena, csr_off, csr, ctr;
+	DBuf retry1, retry2;
+	DBuf st_planes, st_exit, st_tier;
+	Workspace ws_fast, ws_wide, ws_huge;
+	long long nbr_total = 0;
+	bool csr_valid = false;
+};
+
+namespace {
+
+int nblk(long long n, int t) { return (int)((n + t - 1) / t); }
+
+// Exclusive scan of n+1 ints (in[n] must be 0): out[n] = total.
+int scan_excl(vgpu_container *h, int *in, int *out, int n1, hipStream_t s) {
+	int tiles = nblk(n1, 1024);
+	std::vector<int> sizes;
+	// part buffers for up to three levels
+	size_t need = (size_t)tiles + (size_t)nblk(tiles, 1024) + 8;
+	int st = h->part.ensure(2 * need * sizeof(int));
+	if (st) return st;
+	int *p1 = h->part.as<int>();
+	int *p1s = p1 + need;
+	hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(256), 0, s, in, out, n1, p1);
+	if (tiles > 1) {
+		if (tiles > 1024 * 1024) return VGPU_INVALID_ARGUMENT;
+		int t2 = nblk(tiles, 1024);
+		int *p2 = p1 + tiles;
+		int *p2s = p1s + tiles;
+		hipLaunchKernelGGL(k_scan_tiles, dim3(t2), dim3(256), 0, s, p1, p1s, tiles, p2);
+		if (t2 > 1) {
+			int *p3 = p2 + t2;  // t2 <= 1024: one tile
+			hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(256), 0, s, p2, p2s, t2, p3);
+			hipLaunchKernelGGL(k_scan_add, dim3(t2), dim3(256), 0, s, p1s, tiles, p2s);
+		}
+		hipLaunchKernelGGL(k_scan_add, dim3(tiles), dim3(256), 0, s, out, n1, p1s);
+	}
+	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
+}
+
+RectView make_view(vgpu_container *h) {
+	RectView v;
+	const RectGeom &g = h->g;
+	v.ax = g.ax; v.bx = g.bx; v.ay = g.ay; v.by = g.by; v.az = g.az; v.bz = g.bz;
+	v.nx = g.nx; v.ny = g.ny; v.nz = g.nz; v.nxy = g.nx * g.ny;
+	v.boxx = g.boxx; v.boxy = g.boxy; v.boxz = g.boxz; v.xsp = g.xsp; v.ysp = g.ysp; v.zsp = g.zsp;
+	v.xper = g.xper; v.yper = g.yper; v.zper = g.zper;
+	v.hx = g.xper ? 2 * g.nx + 1 : g.nx; v.hy = g.yper ? 2 * g.ny + 1 : g.ny; v.hz = g.zper ? 2 * g.nz + 1 : g.nz;
+	v.bxsq = g.boxx * g.boxx + g.boxy * g.boxy + g.boxz * g.boxz;
+	v.max_len_sq = h->max_len_sq;
+	v.max_radius = h->max_radius;
+	v.P = h->P.as<dbl4>(); v.pid = h->pid.as<int>(); v.bstart = h->bstart.as<int>();
+	v.mrad = h->d_mrad.as<double>(); v.wl = h->d_wl.as<unsigned>();
+	v.walls = h->d_walls.as<double>(); v.nwalls = (int)(h->walls.size() / 5);
+	return v;
+}
+
+template<class T_>
+int launch_tier(vgpu_container *h, Workspace &ws, int waves, const RectView &v, int ncells, const int *list,
+		const int *list_count, CellOut out, hipStream_t s) {
+	typedef Layout<T_, 64> L;
+	int st = ws.ensure(L::bytes, waves);
+	if (st) return st;
+	if ((unsigned long long)ws.stamp_base + (unsigned long long)ncells + 2 >= 0xffffffffull) {
+		HIPCHK(hipMemset(ws.buf.p, 0, ws.buf.n));
+		ws.stamp_base = 0;
+	}
+	int *ctr = h->d_counters.as<int>();
+	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 8, 0);
+	const bool poly = h->kind == VGPU_CONTAINER_POLY;
+	cells_launcher fn = T_::MAXV == TierFast::MAXV ? launch_cells_fast
+			    : (T_::MAXV == TierWide::MAXV ? launch_cells_wide : launch_cells_huge);
+	if (fn(poly, waves, s, v, ncells, list, list_count, h->cellblk.as<int>(), ctr, ws.base(), ws.slab,
+	       ws.stamp_base, out))
+		return VGPU_HIP_ERROR;
+	ws.stamp_base += (unsigned)ncells + 1u;
+	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
+}
+
+int default_fast_waves() {
+	int dev = 0, cus = 256;
+	if (hipGetDevice(&dev) == hipSuccess) {
+		hipDeviceProp_t pr;
+		if (hipGetDeviceProperties(&pr, dev) == hipSuccess && pr.multiProcessorCount > 0) cus = pr.multiProcessorCount;
+	}
+	return cus * 8;
+}
+
+int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
+	const int nxyz = h->g.nx * h->g.ny * h->g.nz;
+	int n = h->n_in;
+	int st;
+	if (!h->events) {
+		for (int t = 0; t < 5; t++) HIPCHK(hipEventCreate(&h->ev[t]));
+		h->events = true;
+	}
+	HIPCHK(hipEventRecord(h->ev[0], s));
+	// upload pending host puts
+	if (!h->have_dev_input) {
+		if ((st = h->in_ids.ensure((size_t)std::max(n, 1) * 4))) return st;
+		if ((st = h->in_xyz.ensure((size_t)std::max(n, 1) * 24))) return st;
+		if (n) {
+			HIPCHK(hipMemcpyAsync(h->in_ids.p, h->h_ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+			HIPCHK(hipMemcpyAsync(h->in_xyz.p, h->h_xyz.data(), (size_t)n * 24, hipMemcpyHostToDevice, s));
+		}
+		if (h->kind == VGPU_CONTAINER_POLY) {
+			if ((st = h->in_r.ensure((size_t)std::max(n, 1) * 8))) return st;
+			if (n) HIPCHK(hipMemcpyAsync(h->in_r.p, h->h_r.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
+		}
+	}
+	const size_t N1 = (size_t)std::max(n, 1);
+	if ((st = h->blk.ensure(N1 * 4)) || (st = h->slot.ensure(N1 * 4)) || (st = h->rem.ensure(N1 * 32)) ||
+	    (st = h->cnt.ensure((size_t)(nxyz + 1) * 4)) || (st = h->bstart.ensure((size_t)(nxyz + 1) * 4)) ||
+	    (st = h->tmp.ensure(N1 * 4)) || (st = h->P.ensure(N1 * 32)) || (st = h->pid.ensure(N1 * 4)) ||
+	    (st = h->src.ensure(N1 * 4)) || (st = h->cellblk.ensure(N1 * 4)) || (st = h->d_maxr.ensure(8)) ||
+	    (st = h->d_counters.ensure(64 * 4)))
+		return st;
+	hipLaunchKernelGGL(k_fill_int, dim3(nblk(nxyz + 1, 256)), dim3(256), 0, s, h->cnt.as<int>(), nxyz + 1, 0);
+	HIPCHK(hipMemsetAsync(h->d_maxr.p, 0, 8, s));
+	const double *rptr = h->kind == VGPU_CONTAINER_POLY ? h->in_r.as<double>() : nullptr;
+	if (n) {
+		hipLaunchKernelGGL(k_locate, dim3(nblk(n, 256)), dim3(256), 0, s, n, h->in_xyz.as<double>(), rptr, h->g,
+				   h->blk.as<int>(), h->slot.as<int>(), h->rem.as<dbl4>(), h->cnt.as<int>(),
+				   h->d_maxr.as<unsigned long long>());
+	}
+	if ((st = scan_excl(h, h->cnt.as<int>(), h->bstart.as<int>(), nxyz + 1, s))) return st;
+	if (n) {
+		hipLaunchKernelGGL(k_place, dim3(nblk(n, 256)), dim3(256), 0, s, n, h->blk.as<int>(), h->slot.as<int>(),
+				   h->bstart.as<int>(), h->tmp.as<int>());
+		hipLaunchKernelGGL(k_rank, dim3(nblk(n, 256)), dim3(256), 0, s, n, h->blk.as<int>(), h->bstart.as<int>(),
+				   h->tmp.as<int>(), h->rem.as<dbl4>(), h->in_ids.as<int>(), h->P.as<dbl4>(), h->pid.as<int>(),
+				   h->src.as<int>(), h->cellblk.as<int>());
+	}
+	HIPCHK(hipGetLastError());
+	int nc = 0;
+	unsigned long long mrb = 0;
+	HIPCHK(hipMemcpyAsync(&nc, h->bstart.as<int>() + nxyz, 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(&mrb, h->d_maxr.p, 8, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
+	h->ncells = nc;
+	double mr;
+	std::memcpy(&mr, &mrb, 8);
+	h->max_radius = h->kind == VGPU_CONTAINER_POLY ? mr : 0;
+	HIPCHK(hipEventRecord(h->ev[1], s));
+
+	// outputs
+	const size_t C1 = (size_t)std::max(nc, 1);
+	const bool nbrs = (flags & VGPU_F_NEIGHBORS) != 0;
+	long long acap = nbrs ? std::min<long long>(2147483000LL, 24LL * C1 + 4096) : 1;
+	if ((st = h->vol.ensure(C1 * 8)) || (st = h->cent.ensure(C1 * 24)) || (st = h->nfaces.ensure(C1 * 4)) ||
+	    (st = h->status.ensure(C1 * 4)) || (st = h->nbr_start.ensure(C1 * 4)) || (st = h->arena.ensure((size_t)acap * 4)) ||
+	    (st = h->ctr.ensure(16 * 4)) || (st = h->retry1.ensure(C1 * 4)) || (st = h->retry2.ensure(C1 * 4)))
+		return st;
+	const bool stats = (flags & VGPU_F_STATS) != 0;
+	if (stats) {
+		if ((st = h->st_planes.ensure(C1 * 2)) || (st = h->st_exit.ensure(C1)) || (st = h->st_tier.ensure(C1))) return st;
+	}
+	int *ctr = h->ctr.as<int>();  // [0] arena top [1] retry1 count [2] retry2 count [3] failures
+	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 16, 0);
+	RectView v = make_view(h);
+	CellOut out;
+	out.vol = h->vol.as<double>(); out.cent = h->cent.as<double>(); out.nfaces = h->nfaces.as<int>();
+	out.status = h->status.as<int>(); out.nbr_start = h->nbr_start.as<int>(); out.nbr = h->arena.as<int>();
+	out.arena_top = ctr; out.arena_cap = (int)acap; out.fail_count = ctr + 3;
+	out.st_planes = stats ? h->st_planes.as<short>() : nullptr;
+	out.st_exit = stats ? h->st_exit.as<signed char>() : nullptr;
+	out.st_tier = stats ? h->st_tier.as<signed char>() : nullptr;
+	out.want_nbrs = nbrs ? 1 : 0;
+
+	int fw = h->fast_waves > 0 ? h->fast_waves : default_fast_waves();
+	fw = std::max(1, std::min(fw, nblk(std::max(nc, 1), 64)));
+	if (nc > 0) {
+		out.tier = 0; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
+		if ((st = launch_tier<TierFast>(h, h->ws_fast, fw, v, nc, nullptr, nullptr, out, s))) return st;
+		HIPCHK(hipEventRecord(h->ev[2], s));
+		out.tier = 1; out.retry = h->retry2.as<int>(); out.retry_count = ctr + 2;
+		if ((st = launch_tier<TierWide>(h, h->ws_wide, 64, v, nc, h->retry1.as<int>(), ctr + 1, out, s))) return st;
+		out.tier = 2; out.retry = nullptr; out.retry_count = nullptr;
+		if ((st = launch_tier<TierHuge>(h, h->ws_huge, 4, v, nc, h->retry2.as<int>(), ctr + 2, out, s))) return st;
+	} else {
+		HIPCHK(hipEventRecord(h->ev[2], s));
+	}
+	HIPCHK(hipEventRecord(h->ev[3], s));
+	h->csr_valid = false;
+	if (nbrs && (flags & VGPU_F_CSR) && nc > 0) {
+		if ((st = h->csr_off.ensure((C1 + 1) * 4))) return st;
+		hipLaunchKernelGGL(k_face_counts, dim3(nblk(nc + 1, 256)), dim3(256), 0, s, nc, h->status.as<int>(),
+				   h->nfaces.as<int>(), h->csr_off.as<int>());
+		// scan in place into a second buffer region
+		if ((st = h->tmp.ensure((C1 + 1) * 4))) return st;
+		if ((st = scan_excl(h, h->csr_off.as<int>(), h->tmp.as<int>(), nc + 1, s))) return st;
+		if ((st = h->csr.ensure((size_t)acap * 4))) return st;
+		hipLaunchKernelGGL(k_gather_nbrs, dim3(nblk(nc, 256)), dim3(256), 0, s, nc, h->status.as<int>(),
+				   h->nfaces.as<int>(), h->nbr_start.as<int>(), h->arena.as<int>(), h->tmp.as<int>(),
+				   h->csr.as<int>());
+		h->csr_valid = true;
+	}
+	HIPCHK(hipEventRecord(h->ev[4], s));
+	int hc[4];
+	HIPCHK(hipMemcpyAsync(hc, ctr, 16, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
+	float ms;
+	for (int t = 0; t < 4; t++) {
+		HIPCHK(hipEventElapsedTime(&ms, h->ev[t], h->ev[t + 1]));
+		h->timings[t] = ms;
+	}
+	h->nbr_total = hc[0];
+	h->computed = true;
+	h->last_flags = flags;
+	if (hc[3] > 0) return VGPU_MEMORY_ERROR;
+	return VGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vgpu_create(int kind, const double *geom, const int *grid, const int *periodic, int init_mem,
+		vgpu_container **out) {
+	(void)init_mem;
+	if (!out || !geom || !grid) return VGPU_INVALID_ARGUMENT;
+	*out = nullptr;
+	if (kind != VGPU_CONTAINER && kind != VGPU_CONTAINER_POLY) return VGPU_INVALID_ARGUMENT;
+	if (grid[0] <= 0 || grid[1] <= 0 || grid[2] <= 0) return VGPU_INVALID_ARGUMENT;
+	if ((long long)grid[0] * grid[1] * grid[2] > 1000000000LL) return VGPU_INVALID_ARGUMENT;
+	if (!(geom[1] > geom[0]) || !(geom[3] > geom[2]) || !(geom[5] > geom[4])) return VGPU_INVALID_ARGUMENT;
+	int ndev = 0;
+	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return VGPU_NO_DEVICE;
+	vgpu_container *h = new vgpu_container();
+	h->kind = kind;
+	for (int t = 0; t < 6; t++) h->geom[t] = geom[t];
+	for (int t = 0; t < 3; t++) { h->grid[t] = grid[t]; h->per[t] = periodic ? (periodic[t] != 0) : 0; }
+	RectGeom &g = h->g;
+	g.ax = geom[0]; g.bx = geom[1]; g.ay = geom[2]; g.by = geom[3]; g.az = geom[4]; g.bz = geom[5];
+	g.nx = grid[0]; g.ny = grid[1]; g.nz = grid[2];
+	// src/container.cc:30-36, src/v_base.cc:23-25
+	g.boxx = (g.bx - g.ax) / g.nx; g.boxy = (g.by - g.ay) / g.ny; g.boxz = (g.bz - g.az) / g.nz;
+	g.xsp = 1 / g.boxx; g.ysp = 1 / g.boxy; g.zsp = 1 / g.boxz;
+	g.xper = h->per[0]; g.yper = h->per[1]; g.zper = h->per[2];
+	h->max_len_sq = (g.bx - g.ax) * (g.bx - g.ax) * (g.xper ? 0.25 : 1) + (g.by - g.ay) * (g.by - g.ay) * (g.yper ? 0.25 : 1)
+			+ (g.bz - g.az) * (g.bz - g.az) * (g.zper ? 0.25 : 1);
+	std::vector<double> mrad(4096);
+	worklist_radii(g.boxx, g.boxy, g.boxz, mrad.data());
+	if (h->d_mrad.ensure(4096 * 8) || h->d_wl.ensure(4096 * 4) || h->d_walls.ensure(8 * 5 * 64)) {
+		delete h;
+		return VGPU_MEMORY_ERROR;
+	}
+	if (hipMemcpy(h->d_mrad.p, mrad.data(), 4096 * 8, hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(h->d_wl.p, vgpu_worklist_table, 4096 * 4, hipMemcpyHostToDevice) != hipSuccess) {
+		delete h;
+		return VGPU_HIP_ERROR;
+	}
+	*out = h;
+	return VGPU_OK;
+}
+
+int vgpu_destroy(vgpu_container *h) {
+	if (!h) return VGPU_INVALID_ARGUMENT;
+	DBuf *bufs[] = {&h->in_ids, &h->in_xyz, &h->in_r, &h->blk, &h->slot, &h->rem, &h->cnt, &h->bstart, &h->tmp,
+			&h->part, &h->P, &h->pid, &h->src, &h->cellblk, &h->d_mrad, &h->d_wl, &h->d_walls, &h->d_maxr,
+			&h->d_counters, &h->vol, &h->cent, &h->nfaces, &h->status, &h->nbr_start, &h->arena, &h->csr_off,
+			&h->csr, &h->ctr, &h->retry1, &h->retry2, &h->st_planes, &h->st_exit, &h->st_tier};
+	for (DBuf *b : bufs) b->release();
+	h->ws_fast.buf.release(); h->ws_wide.buf.release(); h->ws_huge.buf.release();
+	if (h->events) for (int t = 0; t < 5; t++) (void)hipEventDestroy(h->ev[t]);
+	delete h;
+	return VGPU_OK;
+}
+
+int vgpu_add_wall_sphere(vgpu_container *h, double xc, double yc, double zc, double rc, int w_id) {
+	if (!h) return VGPU_INVALID_ARGUMENT;
+	if ((int)h->walls.size() / 5 >= 64) return VGPU_MEMORY_ERROR;
+	double w[5] = {xc, yc, zc, rc, (double)w_id};
+	h->walls.insert(h->walls.end(), w, w + 5);
+	if (hipMemcpy(h->d_walls.p, h->walls.data(), h->walls.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+		return VGPU_HIP_ERROR;
+	h->computed = false;
+	return VGPU_OK;
+}
+
+int vgpu_put(vgpu_container *h, int n, const int *ids, const double *xyz, const double *r) {
+	if (!h || n < 0 || (n > 0 && (!ids || !xyz))) return VGPU_INVALID_ARGUMENT;
+	if (h->kind == VGPU_CONTAINER_POLY && n > 0 && !r) return VGPU_INVALID_ARGUMENT;
+	if (h->have_dev_input) {
+		// switching back to host-side puts: start from the device contents
+		h->h_ids.resize(h->n_in);
+		h->h_xyz.resize((size_t)h->n_in * 3);
+		h->h_r.resize(h->kind == VGPU_CONTAINER_POLY ? h->n_in : 0);
+		if (h->n_in) {
+			if (hipMemcpy(h->h_ids.data(), h->in_ids.p, (size_t)h->n_in * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+			    hipMemcpy(h->h_xyz.data(), h->in_xyz.p, (size_t)h->n_in * 24, hipMemcpyDeviceToHost) != hipSuccess)
+				return VGPU_HIP_ERROR;
+			if (h->kind == VGPU_CONTAINER_POLY &&
+			    hipMemcpy(h->h_r.data(), h->in_r.p, (size_t)h->n_in * 8, hipMemcpyDeviceToHost) != hipSuccess)
+				return VGPU_HIP_ERROR;
+		}
+		h->have_dev_input = false;
+	}
+	if ((long long)h->n_in + n > 2000000000LL / 32) return VGPU_MEMORY_ERROR;
+	h->h_ids.insert(h->h_ids.end(), ids, ids + n);
+	h->h_xyz.insert(h->h_xyz.end(), xyz, xyz + 3 * (size_t)n);
+	if (h->kind == VGPU_CONTAINER_POLY) h->h_r.insert(h->h_r.end(), r, r + n);
+	h->n_in += n;
+	h->computed = false;
+	return VGPU_OK;
+}
+
+int vgpu_put_device(vgpu_container *h, int n, const int *d_ids, const double *d_xyz, const double *d_r) {
+	if (!h || n < 0 || (n > 0 && (!d_ids || !d_xyz))) return VGPU_INVALID_ARGUMENT;
+	if (h->kind == VGPU_CONTAINER_POLY && n > 0 && !d_r) return VGPU_INVALID_ARGUMENT;
+	if (n > 2000000000 / 32) return VGPU_MEMORY_ERROR;
+	int st;
+	const size_t N1 = (size_t)std::max(n, 1);
+	if ((st = h->in_ids.ensure(N1 * 4)) || (st = h->in_xyz.ensure(N1 * 24))) return st;
+	if (n) {
+		if (hipMemcpy(h->in_ids.p, d_ids, (size_t)n * 4, hipMemcpyDeviceToDevice) != hipSuccess ||
+		    hipMemcpy(h->in_xyz.p, d_xyz, (size_t)n * 24, hipMemcpyDeviceToDevice) != hipSuccess)
+			return VGPU_HIP_ERROR;
+	}
+	if (h->kind == VGPU_CONTAINER_POLY) {
+		if ((st = h->in_r.ensure(N1 * 8))) return st;
+		if (n && hipMemcpy(h->in_r.p, d_r, (size_t)n * 8, hipMemcpyDeviceToDevice) != hipSuccess) return VGPU_HIP_ERROR;
+	}
+	h->h_ids.clear(); h->h_xyz.clear(); h->h_r.clear();
+	h->have_dev_input = true;
+	h->n_in = n;
+	h->computed = false;
+	return VGPU_OK;
+}
+
+int vgpu_clear(vgpu_container *h) {
+	if (!h) return VGPU_INVALID_ARGUMENT;
+	h->h_ids.clear(); h->h_xyz.clear(); h->h_r.clear();
+	h->have_dev_input = false;
+	h->n_in = 0;
+	h->computed = false;
+	return VGPU_OK;
+}
+
+int vgpu_compute_all(vgpu_container *h, unsigned flags, void *stream) {
+	if (!h) return VGPU_INVALID_ARGUMENT;
+	return run_pipeline(h, flags, (hipStream_t)stream);
+}
+
+int vgpu_num_cells(vgpu_container *h, long long *n) {
+	if (!h || !n) return VGPU_INVALID_ARGUMENT;
+	if (!h->computed) return VGPU_INVALID_ARGUMENT;
+	*n = h->ncells;
+	return VGPU_OK;
+}
+
+int vgpu_get_results(vgpu_container *h, int *id, int *ijk, int *q, int *exists, double *pos, double *vol,
+		     double *centroid, int *nfaces, long long *nbr_off, int *nbr, long long nbr_cap,
+		     long long *nbr_total) {
+	if (!h || !h->computed) return VGPU_INVALID_ARGUMENT;
+	const int nc = h->ncells;
+	const int nxyz = h->g.nx * h->g.ny * h->g.nz;
+	if (nc == 0) {
+		if (nbr_off) nbr_off[0] = 0;
+		if (nbr_total) *nbr_total = 0;
+		return VGPU_OK;
+	}
+	std::vector<int> st(nc), nf(nc);
+	HIPCHK(hipMemcpy(st.data(), h->status.p, (size_t)nc * 4, hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(nf.data(), h->nfaces.p, (size_t)nc * 4, hipMemcpyDeviceToHost));
+	if (id) HIPCHK(hipMemcpy(id, h->pid.p, (size_t)nc * 4, hipMemcpyDeviceToHost));
+	if (ijk || q) {
+		std::vector<int> cb(nc), bs(nxyz + 1);
+		HIPCHK(hipMemcpy(cb.data(), h->cellblk.p, (size_t)nc * 4, hipMemcpyDeviceToHost));
+		HIPCHK(hipMemcpy(bs.data(), h->bstart.p, (size_t)(nxyz + 1) * 4, hipMemcpyDeviceToHost));
+		for (int c = 0; c < nc; c++) {
+			if (ijk) ijk[c] = cb[c];
+			if (q) q[c] = c - bs[cb[c]];
+		}
+	}
+	if (exists) for (int c = 0; c < nc; c++) exists[c] = (st[c] & ST_EXISTS) ? 1 : 0;
+	if (pos) {
+		std::vector<dbl4> P(nc);
+		HIPCHK(hipMemcpy(P.data(), h->P.p, (size_t)nc * 32, hipMemcpyDeviceToHost));
+		for (int c = 0; c < nc; c++) { pos[3 * c] = P[c].x; pos[3 * c + 1] = P[c].y; pos[3 * c + 2] = P[c].z; }
+	}
+	if (vol) HIPCHK(hipMemcpy(vol, h->vol.p, (size_t)nc * 8, hipMemcpyDeviceToHost));
+	if (centroid) HIPCHK(hipMemcpy(centroid, h->cent.p, (size_t)nc * 24, hipMemcpyDeviceToHost));
+	if (nfaces) for (int c = 0; c < nc; c++) nfaces[c] = (st[c] & ST_EXISTS) ? nf[c] : 0;
+	long long tot = 0;
+	std::vector<long long> off(nc + 1);
+	for (int c = 0; c < nc; c++) { off[c] = tot; if (st[c] & ST_EXISTS) tot += nf[c]; }
+	off[nc] = tot;
+	if (nbr_off) std::memcpy(nbr_off, off.data(), (size_t)(nc + 1) * 8);
+	if (nbr_total) *nbr_total = tot;
+	if (nbr && (h->last_flags & VGPU_F_NEIGHBORS)) {
+		if (h->csr_valid) {
+			long long m = std::min(tot, nbr_cap);
+			if (m > 0) HIPCHK(hipMemcpy(nbr, h->csr.p, (size_t)m * 4, hipMemcpyDeviceToHost));
+		} else {
+			std::vector<int> start(nc), arena((size_t)std::max<long long>(h->nbr_total, 1));
+			HIPCHK(hipMemcpy(start.data(), h->nbr_start.p, (size_t)nc * 4, hipMemcpyDeviceToHost));
+			if (h->nbr_total > 0)
+				HIPCHK(hipMemcpy(arena.data(), h->arena.p, (size_t)h->nbr_total * 4, hipMemcpyDeviceToHost));
+			for (int c = 0; c < nc; c++) {
+				if (!(st[c] & ST_EXISTS)) continue;
+				for (int t = 0; t < nf[c]; t++) {
+					long long d = off[c] + t;
+					if (d < nbr_cap) nbr[d] = arena[(size_t)start[c] + t];
+				}
+			}
+		}
+	}
+	return VGPU_OK;
+}
+
+int vgpu_get_stats(vgpu_container *h, int *n_planes, int *wl_exit, int *tier) {
+	if (!h || !h->computed || !(h->last_flags & VGPU_F_STATS)) return VGPU_INVALID_ARGUMENT;
+	const int nc = h->ncells;
+	if (nc == 0) return VGPU_OK;
+	std::vector<short> a(nc);
+	std::vector<signed char> b(nc), c(nc);
+	HIPCHK(hipMemcpy(a.data(), h->st_planes.p, (size_t)nc * 2, hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(b.data(), h->st_exit.p, (size_t)nc, hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(c.data(), h->st_tier.p, (size_t)nc, hipMemcpyDeviceToHost));
+	for (int t = 0; t < nc; t++) {
+		if (n_planes) n_planes[t] = a[t];
+		if (wl_exit) wl_exit[t] = b[t];
+		if (tier) tier[t] = c[t];
+	}
+	return VGPU_OK;
+}
+
+int vgpu_sum_volumes(vgpu_container *h, double *out) {
+	if (!h || !out || !h->computed) return VGPU_INVALID_ARGUMENT;
+	const int nc = h->ncells;
+	std::vector<double> v(std::max(nc, 1));
+	std::vector<int> st(std::max(nc, 1));
+	if (nc) {
+		HIPCHK(hipMemcpy(v.data(), h->vol.p, (size_t)nc * 8, hipMemcpyDeviceToHost));
+		HIPCHK(hipMemcpy(st.data(), h->status.p, (size_t)nc * 4, hipMemcpyDeviceToHost));
+	}
+	// Sequential sum in loop order, as sum_cell_volumes() accumulates.
+	double s = 0;
+	for (int c = 0; c < nc; c++) if (st[c] & ST_EXISTS) s += v[c];
+	*out = s;
+	return VGPU_OK;
+}
+
+int vgpu_device_results(vgpu_container *h, const double **vol, const double **centroid, const int **nfaces,
+			const int **status, const int **nbr_start, const int **nbr) {
+	if (!h || !h->computed) return VGPU_INVALID_ARGUMENT;
+	if (vol) *vol = h->vol.as<double>();
+	if (centroid) *centroid = h->cent.as<double>();
+	if (nfaces) *nfaces = h->nfaces.as<int>();
+	if (status) *status = h->status.as<int>();
+	if (nbr_start) *nbr_start = h->nbr_start.as<int>();
+	if (nbr) *nbr = h->arena.as<int>();
+	return VGPU_OK;
+}
+
+int vgpu_set_launch(vgpu_container *h, int fast_waves) {
+	if (!h || fast_waves < 0) return VGPU_INVALID_ARGUMENT;
+	h->fast_waves = fast_waves;
+	return VGPU_OK;
+}
+
+int vgpu_last_timings(vgpu_container *h, double *ms4) {
+	if (!h || !ms4) return VGPU_INVALID_ARGUMENT;
+	for (int t = 0; t < 4; t++) ms4[t] = h->timings[t];
+	return VGPU_OK;
+}
+
+const char *vgpu_status_string(int s) {
+	switch (s) {
+	case VGPU_OK: return "ok";
+	case VGPU_FILE_ERROR: return "file error";
+	case VGPU_MEMORY_ERROR: return "capacity exceeded";
+	case VGPU_INTERNAL_ERROR: return "internal error";
+	case VGPU_CMD_LINE_ERROR: return "command line error";
+	case VGPU_INVALID_ARGUMENT: return "invalid argument";
+	case VGPU_HIP_ERROR: return "HIP runtime error";
+	case VGPU_DUPLICATE: return "duplicate particle";
+	case VGPU_NO_DEVICE: return "no GPU device";
+	default: return "unknown status";
+	}
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// Binning (stable counting sort by block), scans and neighbour compaction
+// kernels.  Included by the API translation unit only.
+#pragma once
+#include "vgpu_cells.h"
+
+namespace vgpu {
+
+// Geometry needed to bin particles of a rectangular container.
+struct RectGeom {
+	double ax, bx, ay, by, az, bz;
+	int nx, ny, nz;
+	double boxx, boxy, boxz, xsp, ysp, zsp;
+	int xper, yper, zper;
+};
+
+__device__ __forceinline__ int step_int_d(double a) { return a < 0 ? int(a) - 1 : int(a); }
+__device__ __forceinline__ int step_mod_d(int a, int b) { return a >= 0 ? a % b : b - 1 - (b - 1 - a) % b; }
+
+// Block index of every particle, as container_base::put_remap computes it
+// (src/container.cc:174-191), the position remapped into the primary domain
+// for periodic directions, and an (unordered) slot in its block.  Dropped
+// particles get block -1.  The largest radius of a stored particle is
+// folded into *maxr_bits (positive doubles order like their bit patterns).
+__global__ void k_locate(int n, const double *__restrict__ xyz, const double *__restrict__ r, RectGeom g,
+			 int *__restrict__ blk, int *__restrict__ slot, dbl4 *__restrict__ rem,
+			 int *__restrict__ cnt, unsigned long long *__restrict__ maxr_bits) {
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	double x = xyz[3 * t], y = xyz[3 * t + 1], z = xyz[3 * t + 2];
+	int b = -1;
+	const double lim = 1073741824.0;
+	double ax_ = (x - g.ax) * g.xsp, ay_ = (y - g.ay) * g.ysp, az_ = (z - g.az) * g.zsp;
+	// Non-finite or huge coordinates would make the int conversion undefined:
+	// they are outside any box, so they are dropped.
+	bool finite = fabs(ax_) < lim && fabs(ay_) < lim && fabs(az_) < lim;
+	if (finite) {
+		int i = step_int_d(ax_), j, k, l;
+		bool ok = true;
+		if (g.xper) { l = step_mod_d(i, g.nx); x += g.boxx * (l - i); i = l; }
+		else if (i < 0 || i >= g.nx) ok = false;
+		j = step_int_d(ay_);
+		if (g.yper) { l = step_mod_d(j, g.ny); y += g.boxy * (l - j); j = l; }
+		else if (j < 0 || j >= g.ny) ok = false;
+		k = step_int_d(az_);
+		if (g.zper) { l = step_mod_d(k, g.nz); z += g.boxz * (l - k); k = l; }
+		else if (k < 0 || k >= g.nz) ok = false;
+		if (ok) b = i + g.nx * j + g.nx * g.ny * k;
+	}
+	blk[t] = b;
+	double rr = r ? r[t] : 0.5;
+	rem[t] = dbl4{x, y, z, rr};
+	if (b >= 0) {
+		slot[t] = atomicAdd(&cnt[b], 1);
+		if (r && rr > 0) atomicMax(maxr_bits, (unsigned long long)__double_as_longlong(rr));
+	}
+}
+
+// Exclusive scan, tiles of 1024 ints by 256 threads; tile totals to part.
+__global__ void k_scan_tiles(const int *__restrict__ in, int *__restrict__ out, int n, int *__restrict__ part) {
+	__shared__ int sh[256];
+	int base = blockIdx.x * 1024 + threadIdx.x * 4;
+	int v[4], s = 0;
+	for (int t = 0; t < 4; t++) { v[t] = base + t < n ? in[base + t] : 0; s += v[t]; }
+	sh[threadIdx.x] = s;
+	__syncthreads();
+	for (int off = 1; off < 256; off <<= 1) {
+		int a = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+		__syncthreads();
+		sh[threadIdx.x] += a;
+		__syncthreads();
+	}
+	int run = sh[threadIdx.x] - s;
+	for (int t = 0; t < 4; t++) { if (base + t < n) out[base + t] = run; run += v[t]; }
+	if (threadIdx.x == 255) part[blockIdx.x] = sh[255];
+}
+__global__ void k_scan_add(int *__restrict__ out, int n, const int *__restrict__ part) {
+	int t = blockIdx.x * 1024 + threadIdx.x * 4;
+	int a = part[blockIdx.x];
+	for (int u = 0; u < 4; u++) if (t + u < n) out[t + u] += a;
+}
+
+// Unordered placement into block segments.
+__global__ void k_place(int n, const int *__restrict__ blk, const int *__restrict__ slot,
+			const int *__restrict__ bstart, int *__restrict__ tmp) {
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	int b = blk[t];
+	if (b >= 0) tmp[bstart[b] + slot[t]] = t;
+}
+
+// Stable order within each block: rank = number of particles of the same
+// block that were put earlier (lower input index).  Writes the sorted
+// particle, its id, its input index and its block.
+__global__ void k_rank(int n, const int *__restrict__ blk, const int *__restrict__ bstart,
+		       const int *__restrict__ tmp, const dbl4 *__restrict__ rem, const int *__restrict__ ids,
+		       dbl4 *__restrict__ P, int *__restrict__ pid, int *__restrict__ src, int *__restrict__ cellblk) {
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	int b = blk[t];
+	if (b < 0) return;
+	int s = bstart[b], e = bstart[b + 1], rank = 0;
+	for (int u = s; u < e; u++) rank += tmp[u] < t;
+	int d = s + rank;
+	P[d] = rem[t];
+	pid[d] = ids[t];
+	src[d] = t;
+	cellblk[d] = b;
+}
+
+// Loop-order CSR of the neighbour lists: out[off[c] ..] = arena[start[c] ..].
+__global__ void k_gather_nbrs(int ncells, const int *__restrict__ status, const int *__restrict__ nfaces,
+			      const int *__restrict__ start, const int *__restrict__ arena, const int *__restrict__ off,
+			      int *__restrict__ csr) {
+	int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= ncells) return;
+	if (!(status[c] & ST_EXISTS)) return;
+	int s = start[c], o = off[c], k = nfaces[c];
+	for (int t = 0; t < k; t++) csr[o + t] = arena[s + t];
+}
+__global__ void k_face_counts(int ncells, const int *__restrict__ status, const int *__restrict__ nfaces,
+			      int *__restrict__ cnt) {
+	int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c > ncells) return;
+	cnt[c] = (c < ncells && (status[c] & ST_EXISTS)) ? nfaces[c] : 0;
+}
+
+}  // namespace vgpu

@@ -1,0 +1,139 @@
+// The persistent cell kernel and the per-tier launchers.
+#pragma once
+#include "compute_dev.h"
+
+namespace vgpu {
+
+// Results of the cell kernels, loop order.
+struct CellOut {
+	double *vol;
+	double *cent;        // 3 per cell
+	int *nfaces;
+	int *status;         // ST_* bits
+	int *nbr_start;      // offset into the neighbour arena
+	int *nbr;            // neighbour arena
+	int *arena_top;      // bump allocator over nbr
+	int arena_cap;
+	int *retry;          // cells for the next tier
+	int *retry_count;
+	int *fail_count;     // cells the last tier could not finish
+	short *st_planes;    // optional statistics
+	signed char *st_exit;
+	signed char *st_tier;
+	int tier;
+	int want_nbrs;
+};
+
+// Neighbour ids of one cell go to a scratch row of the lane's slab first,
+// then to the arena once the count is known.
+template<class T_, int ST>
+struct SlabSink {
+	VCell<T_, ST> *c;
+	size_t off;
+	VG_INL void put(int i, int v) { if (i < T_::MAXV) c->template A<int>(off, i) = v; }
+};
+
+// Persistent cell kernel, one wavefront per workgroup.  Wavefronts take
+// 64 consecutive cells at a time; the cell range is split into one
+// contiguous share per XCD (workgroups are dispatched to the 8 XCDs round
+// robin) so that neighbouring cells, which read the same particle blocks,
+// share an L2; a wavefront whose share is exhausted moves to the next one.
+template<class T_, bool POLY>
+__global__ void __launch_bounds__(64, 2)
+k_cells(RectView v, int ncells, const int *__restrict__ list, const int *__restrict__ list_count,
+	const int *__restrict__ cellblk, int *__restrict__ counters, char *__restrict__ ws, size_t slab_bytes,
+	unsigned stamp_base, CellOut out) {
+	typedef Layout<T_, 64> L;
+	const int lane = threadIdx.x;
+	VCell<T_, 64> c;
+	c.setup(ws + (size_t)blockIdx.x * slab_bytes, lane, v.max_len_sq);
+	const int total = list_count ? *list_count : ncells;
+	const int nshare = 8;
+	const int my = blockIdx.x % nshare;
+	const int share = (total + nshare - 1) / nshare;
+	for (int h = 0; h < nshare; h++) {
+		const int sh = (my + h) % nshare;
+		const int lo = sh * share, hi = min(total, lo + share);
+		for (;;) {
+			int base = 0;
+			if (lane == 0) base = atomicAdd(&counters[sh], 64);
+			base = __shfl(base, 0);
+			if (lo + base >= hi) break;
+			const int idx = lo + base + lane;
+			if (idx < hi) {
+				const int cell = list ? list[idx] : idx;
+				const int b = cellblk[cell];
+				const int s = cell - v.bstart[b];
+				const int ci = b % v.nx, cj = (b / v.nx) % v.ny, ck = b / v.nxy;
+				CellStats stt;
+				bool ok = compute_cell_rect<T_, 64, POLY>(c, v, b, s, ci, cj, ck, stamp_base + (unsigned)cell + 1u, stt);
+				if (c.ovf) {
+					if (T_::MAXV >= TierHuge::MAXV) {
+						// Last tier: report, do not retry.
+						atomicAdd(out.fail_count, 1);
+						out.status[cell] = c.fatal ? ST_FATAL : ST_OVERFLOW;
+					} else {
+						int slot_ = atomicAdd(out.retry_count, 1);
+						out.retry[slot_] = cell;
+						out.status[cell] = ST_OVERFLOW;
+					}
+				} else {
+					double vol = 0, cx = 0, cy = 0, cz = 0;
+					int nf = 0;
+					if (ok) {
+						SlabSink<T_, 64> sk{&c, L::o_qu};
+						c.outputs(vol, cx, cy, cz, nf, sk);
+					}
+					out.vol[cell] = vol;
+					out.cent[3 * cell] = cx; out.cent[3 * cell + 1] = cy; out.cent[3 * cell + 2] = cz;
+					out.nfaces[cell] = nf;
+					out.status[cell] = ok ? ST_EXISTS : 0;
+					if (out.want_nbrs) {
+						int nkeep = nf < (int)T_::MAXV ? nf : (int)T_::MAXV;
+						int off = nkeep > 0 ? atomicAdd(out.arena_top, nkeep) : 0;
+						if (off + nkeep > out.arena_cap) { out.nbr_start[cell] = -1; atomicAdd(out.fail_count, 1); }
+						else {
+							out.nbr_start[cell] = off;
+							for (int t = 0; t < nkeep; t++) out.nbr[off + t] = c.template A<int>(L::o_qu, t);
+						}
+					}
+					if (out.st_planes) {
+						out.st_planes[cell] = (short)min(stt.n_planes, 32767);
+						out.st_exit[cell] = (signed char)stt.wl_exit;
+						out.st_tier[cell] = (signed char)out.tier;
+					}
+				}
+			}
+		}
+	}
+}
+
+// Launches k_cells<tier,poly> (one translation unit per tier keeps the
+// build parallel).
+typedef int (*cells_launcher)(bool poly, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
+			      const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
+			      unsigned stamp_base, const CellOut &out);
+int launch_cells_fast(bool poly, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
+		      const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
+		      unsigned stamp_base, const CellOut &out);
+int launch_cells_wide(bool poly, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
+		      const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
+		      unsigned stamp_base, const CellOut &out);
+int launch_cells_huge(bool poly, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
+		      const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
+		      unsigned stamp_base, const CellOut &out);
+
+#define VGPU_DEFINE_CELLS_LAUNCHER(NAME, TIER)                                                              \
+	int NAME(bool poly, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,          \
+		 const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,               \
+		 unsigned stamp_base, const CellOut &out) {                                                    \
+		if (poly)                                                                                      \
+			hipLaunchKernelGGL((k_cells<TIER, true>), dim3(waves), dim3(64), 0, s, v, ncells, list,    \
+					   list_count, cellblk, counters, ws, slab, stamp_base, out);              \
+		else                                                                                           \
+			hipLaunchKernelGGL((k_cells<TIER, false>), dim3(waves), dim3(64), 0, s, v, ncells, list,   \
+					   list_count, cellblk, counters, ws, slab, stamp_base, out);              \
+		return hipGetLastError() == hipSuccess ? 0 : 6;                                                \
+	}
+
+}  // namespace vgpu
