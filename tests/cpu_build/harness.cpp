@@ -30,17 +30,16 @@ struct VecSink {
 	void put(int i, int x) { (void)i; v->push_back(x); }
 };
 
-template<class TR>
+template<class CELL, size_t BYTES>
 struct Runner {
-	typedef Layout<TR, 1> L;
 	std::vector<char> slab;
-	VCell<TR, 1> c;
+	CELL c;
 	unsigned stamp = 0;
-	explicit Runner(double mls) : slab(L::bytes + 2 * 65536, 0) { c.setup(slab.data() + 65536, 0, mls); }
+	explicit Runner(double mls) : slab(BYTES + 2 * 65536, 0) { c.setup(slab.data() + 65536, 0, mls); }
 	template<bool POLY>
 	int run(const RectView &v, int ijk, int s, int ci, int cj, int ck, double *vol, double *cent, int *nf,
 		std::vector<int> &nb, CellStats &st) {
-		bool ok = compute_cell_rect<TR, 1, POLY>(c, v, ijk, s, ci, cj, ck, ++stamp, st);
+		bool ok = compute_cell_rect<CELL, POLY>(c, v, ijk, s, ci, cj, ck, ++stamp, st);
 		if (c.ovf) return c.fatal ? -2 : -1;
 		if (ok) {
 			VecSink sk{&nb};
@@ -103,9 +102,9 @@ long long hx_compute_neighbor(const hx_spec *s, int n, const int *ids, const dou
 	v.P = P.data(); v.pid = pid.data(); v.bstart = cnt.data(); v.mrad = mrad.data(); v.wl = vgpu_worklist_table;
 	v.walls = walls.data(); v.nwalls = s->n_spheres; v.max_radius = maxr;
 
-	Runner<TierFast> r0(v.max_len_sq);
-	Runner<TierWide> r1(v.max_len_sq);
-	Runner<TierHuge> *r2 = 0;
+	Runner<VCell3, Layout3<>::lane_bytes> r0(v.max_len_sq);
+	Runner<VCell<TierWide, 1>, Layout<TierWide, 1>::bytes> r1(v.max_len_sq);
+	Runner<VCell<TierHuge, 1>, Layout<TierHuge, 1>::bytes> *r2 = 0;
 	std::vector<int> nb;
 	long long nn = 0;
 	int cell = 0;
@@ -127,7 +126,7 @@ long long hx_compute_neighbor(const hx_spec *s, int n, const int *ids, const dou
 			}
 			if (res < 0) {
 				tier = 2; nb.clear();
-				if (!r2) r2 = new Runner<TierHuge>(v.max_len_sq);
+				if (!r2) r2 = new Runner<VCell<TierHuge, 1>, Layout<TierHuge, 1>::bytes>(v.max_len_sq);
 				res = poly ? r2->run<true>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st)
 					   : r2->run<false>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st);
 			}

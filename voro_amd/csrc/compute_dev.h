@@ -18,6 +18,7 @@
 // that leaves the window or fills the ring is recomputed by a wider tier.
 #pragma once
 #include "cell_dev.h"
+#include "cell3_dev.h"
 
 namespace vgpu {
 
@@ -74,19 +75,19 @@ struct RadPolicy {
 	}
 };
 
-template<class T_, int ST>
+template<class CELL>
 struct BlockSearch {
-	typedef Layout<T_, ST> L;
+	typedef typename CELL::L L;
 	enum { W = L::W, WD = L::WD, MAXQ = L::MAXQ };
-	VCell<T_, ST> &c;
+	CELL &c;
 	unsigned stamp;
 	int ci0, cj0, ck0;      // centre block (mask coordinates)
 	int qh, qt;             // ring head / tail
-	VG_INL BlockSearch(VCell<T_, ST> &c_, unsigned st, int i, int j, int k)
+	VG_INL BlockSearch(CELL &c_, unsigned st, int i, int j, int k)
 		: c(c_), stamp(st), ci0(i), cj0(j), ck0(k), qh(0), qt(0) {}
-	VG_INL typename VCell<T_, ST>::template Ref<unsigned> BM(int w) { return c.template A<unsigned>(L::o_bm, w); }
-	VG_INL typename VCell<T_, ST>::template Ref<unsigned> BS(int w) { return c.template A<unsigned>(L::o_bms, w); }
-	VG_INL typename VCell<T_, ST>::template Ref<int> QU(int t) { return c.template A<int>(L::o_qu, t); }
+	VG_INL auto BM(int w) { return c.template A<unsigned>(L::o_bm, w); }
+	VG_INL auto BS(int w) { return c.template A<unsigned>(L::o_bms, w); }
+	VG_INL auto QU(int t) { return c.template A<int>(L::o_qu, t); }
 	// Window index of block (ei,ej,ek), or -1 (overflow) if outside.
 	VG_INL int widx(int ei, int ej, int ek) {
 		int a = ei - ci0 + W, b = ej - cj0 + W, d = ek - ck0 + W;
@@ -97,14 +98,14 @@ struct BlockSearch {
 		int t = widx(ei, ej, ek);
 		if (t < 0) return true;
 		int w = t >> 5;
-		return BS(w) == stamp && ((BM(w) >> (t & 31)) & 1u);
+		return (unsigned)BS(w) == stamp && (((unsigned)BM(w) >> (t & 31)) & 1u);
 	}
 	VG_INL void mark(int ei, int ej, int ek) {
 		int t = widx(ei, ej, ek);
 		if (t < 0) return;
 		int w = t >> 5;
 		unsigned b = 1u << (t & 31);
-		if (BS(w) != stamp) { BS(w) = stamp; BM(w) = b; }
+		if ((unsigned)BS(w) != stamp) { BS(w) = stamp; BM(w) = b; }
 		else BM(w) |= b;
 	}
 	VG_INL void push(int ei, int ej, int ek) {
@@ -170,8 +171,8 @@ VG_FN bool min_max_radius(const RectView &v, const RadPolicy<POLY> &rp, int di, 
 // y, z order, and r_prime gets the sum of the near squares -- exactly the
 // arguments the reference passes.  The probe order per form is tabled here
 // (3 bits per probe, x in bit 2).
-template<class T_, int ST, bool POLY>
-VG_FN bool region_skip(VCell<T_, ST> &c, RadPolicy<POLY> &rp, int ei, int ej, int ek, int i, int j, int k,
+template<class CELL, bool POLY>
+VG_FN bool region_skip(CELL &c, RadPolicy<POLY> &rp, int ei, int ej, int ek, int i, int j, int k,
 		       double xlo, double xhi, double ylo, double yhi, double zlo, double zhi) {
 	const int sx = ei > i ? 1 : (ei < i ? -1 : 0);
 	const int sy = ej > j ? 1 : (ej < j ? -1 : 0);
@@ -237,12 +238,13 @@ VG_INL int rect_region(const RectView &v, int ci, int cj, int ck, int ei, int ej
 // search) are driven as one resumable candidate generator feeding a single
 // nplane call site.  Block-level decisions (radius refresh, radius stop,
 // block skip, region tests) are taken when the previous block's particles
-// have all been cut, exactly when the reference takes them.  The
-// outside-neighbour bookkeeping of a block (mask marks and queue pushes) is
-// done when the block is chosen rather than after its particles are cut;
-// nothing reads it in between, so the order of cuts is unchanged.
-template<class T_, int ST, bool POLY>
-VG_FN bool compute_cell_rect(VCell<T_, ST> &c, const RectView &v, int ijk, int s, int ci, int cj, int ck,
+// have all been cut, exactly when the reference takes them.  Phase 2's
+// bookkeeping (mask marks and queue pushes) is only read by the block search,
+// which under 1% of cells reach; it is therefore recorded as one bit per
+// worklist entry and replayed, in the original order, when the block search
+// starts.
+template<class CELL, bool POLY>
+VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci, int cj, int ck,
 			     unsigned stamp, CellStats &stt) {
 	double x, y, z, qx = 0, qy = 0, qz = 0;
 	int i, j, k, di, dj, dk, ei, ej, ek, f = 0, g = 0, disp;
@@ -263,13 +265,18 @@ VG_FN bool compute_cell_rect(VCell<T_, ST> &c, const RectView &v, int ijk, int s
 	disp = ijk - i - v.nx * (j + v.ny * k);
 	RadPolicy<POLY> rp;
 	rp.init(me.w, v.max_radius);
-	BlockSearch<T_, ST> bs(c, stamp, i, j, k);
+	BlockSearch<CELL> bs(c, stamp, i, j, k);
+	// Phase-2 bookkeeping is deferred: bit g of p2_cut says block g was
+	// not skipped by the radius test, so its outside neighbours were due.
+	unsigned long long p2_cut = 0;
+	int g2_first = 0;
 
 	double mrs = 0, crs = 0;
 	int next_count = 3, cpi = 0;
 	const double *radp = v.mrad;
 	const unsigned *e = v.wl;
-	// Generator state: 0 walls, 1 own block, 2 worklist, 3 block search.
+	// Generator state: 0 walls, 1 own block, 2 worklist, 4 replay of the
+	// phase-2 bookkeeping, 3 block search.
 	int stage = 0, w = 0, t = 0;
 	bool marking = false;
 	int bt = 0, be = 0;           // particle range of the block being cut
@@ -351,13 +358,13 @@ VG_FN bool compute_cell_rect(VCell<T_, ST> &c, const RectView &v, int ijk, int s
 					if (rp.ctest(radp[g], mrs)) { stt.wl_exit = g; stt.n_planes = c.n_cuts; return true; }
 					stt.wl_exit = 64;
 					stt.phase = 3;
-					stage = 3;
+					stage = 4;
 					continue;
 				}
 				// Phase 1 is a do-while over g < f (src/v_compute.cc:366-431):
 				// its first iteration always runs.  Phase 2 marks the mask
 				// and queues outside neighbours (src/v_compute.cc:443-526).
-				if (!marking && g >= f && g > 0) { marking = true; stt.phase = 2; }
+				if (!marking && g >= f && g > 0) { marking = true; stt.phase = 2; g2_first = g + 1; }
 				if (g == next_count) {
 					mrs = c.max_radius_squared();
 					if (cpi != 8) {
@@ -374,17 +381,30 @@ VG_FN bool compute_cell_rect(VCell<T_, ST> &c, const RectView &v, int ijk, int s
 				ei = di + i; if (ei < 0 || ei >= v.hx) continue;
 				ej = dj + j; if (ej < 0 || ej >= v.hy) continue;
 				ek = dk + k; if (ek < 0 || ek >= v.hz) continue;
-				if (marking) bs.mark(ei, ej, ek);
 				if (min_max_radius<POLY>(v, rp, di, dj, dk, fx, fy, fz, gxs, gys, gzs, crs, mrs)) continue;
+				if (marking) p2_cut |= 1ull << g;
 				{
 					const int rijk = rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
 					bt = v.bstart[rijk]; be = v.bstart[rijk + 1];
 					x2 = x - qx; y2 = y - qy; z2 = z - qz;
 					bcheck = rp.ctest(crs, mrs);
 				}
-				if (marking) {
-					// scan_bits_mask_add, src/v_compute.cc:249-264
-					const unsigned b1 = 1u << 21, b2 = 1u << 22, b3 = 1u << 24, b4 = 1u << 25, b5 = 1u << 27, b6 = 1u << 28;
+				continue;
+			}
+			if (stage == 4) {
+				// Entering the block search: replay phase 2's mask marks and
+				// queue pushes (src/v_compute.cc:443-526) in their original
+				// order -- each visited worklist block is marked, and a block
+				// not skipped by the radius test queues its outside
+				// neighbours (scan_bits_mask_add, src/v_compute.cc:249-264).
+				const unsigned b1 = 1u << 21, b2 = 1u << 22, b3 = 1u << 24, b4 = 1u << 25, b5 = 1u << 27, b6 = 1u << 28;
+				for (int gg = g2_first; gg <= 63 && g2_first > 0; gg++) {
+					q = e[gg]; q ^= m1; q += m2;
+					ei = (int)(q & 127) - 64 + i; if (ei < 0 || ei >= v.hx) continue;
+					ej = (int)((q >> 7) & 127) - 64 + j; if (ej < 0 || ej >= v.hy) continue;
+					ek = (int)((q >> 14) & 127) - 64 + k; if (ek < 0 || ek >= v.hz) continue;
+					bs.mark(ei, ej, ek);
+					if (!((p2_cut >> gg) & 1ull)) continue;
 					if ((q & b2) == b2) {
 						if (ei > 0) bs.mark_push(ei - 1, ej, ek);
 						if ((q & b1) == 0 && ei < v.hx - 1) bs.mark_push(ei + 1, ej, ek);
@@ -399,6 +419,7 @@ VG_FN bool compute_cell_rect(VCell<T_, ST> &c, const RectView &v, int ijk, int s
 					} else if ((q & b5) == b5 && ek < v.hz - 1) bs.mark_push(ei, ej, ek + 1);
 					if (c.ovf) { stt.n_planes = c.n_cuts; return false; }
 				}
+				stage = 3;
 				continue;
 			}
 			// stage 3: breadth-first over queued blocks (src/v_compute.cc:534-618).
@@ -407,7 +428,7 @@ VG_FN bool compute_cell_rect(VCell<T_, ST> &c, const RectView &v, int ijk, int s
 				const double xlo = (ei - i) * v.boxx - fx, xhi = xlo + v.boxx;
 				const double ylo = (ej - j) * v.boxy - fy, yhi = ylo + v.boxy;
 				const double zlo = (ek - k) * v.boxz - fz, zhi = zlo + v.boxz;
-				if (region_skip<T_, ST, POLY>(c, rp, ei, ej, ek, i, j, k, xlo, xhi, ylo, yhi, zlo, zhi)) {
+				if (region_skip<CELL, POLY>(c, rp, ei, ej, ek, i, j, k, xlo, xhi, ylo, yhi, zlo, zhi)) {
 					if (c.ovf) { stt.n_planes = c.n_cuts; return false; }
 					continue;
 				}

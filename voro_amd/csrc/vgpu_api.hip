@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/voro_gpu.h"
@@ -151,7 +152,7 @@ RectView make_view(vgpu_container *h) {
 template<class T_>
 int launch_tier(vgpu_container *h, Workspace &ws, int waves, const RectView &v, int ncells, const int *list,
 		const int *list_count, CellOut out, hipStream_t s) {
-	typedef Layout<T_, 64> L;
+	typedef typename TierCell<T_>::type::L L;
 	int st = ws.ensure(L::bytes, waves);
 	if (st) return st;
 	if ((unsigned long long)ws.stamp_base + (unsigned long long)ncells + 2 >= 0xffffffffull) {
@@ -161,7 +162,7 @@ int launch_tier(vgpu_container *h, Workspace &ws, int waves, const RectView &v, 
 	int *ctr = h->d_counters.as<int>();
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 8, 0);
 	const bool poly = h->kind == VGPU_CONTAINER_POLY;
-	cells_launcher fn = T_::MAXV == TierFast::MAXV ? launch_cells_fast
+	cells_launcher fn = std::is_same<T_, Tier3>::value ? launch_cells_fast3
 			    : (T_::MAXV == TierWide::MAXV ? launch_cells_wide : launch_cells_huge);
 	if (fn(poly, waves, s, v, ncells, list, list_count, h->cellblk.as<int>(), ctr, ws.base(), ws.slab,
 	       ws.stamp_base, out))
@@ -264,7 +265,7 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 	fw = std::max(1, std::min(fw, nblk(std::max(nc, 1), 64)));
 	if (nc > 0) {
 		out.tier = 0; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
-		if ((st = launch_tier<TierFast>(h, h->ws_fast, fw, v, nc, nullptr, nullptr, out, s))) return st;
+		if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, nullptr, nullptr, out, s))) return st;
 		HIPCHK(hipEventRecord(h->ev[2], s));
 		out.tier = 1; out.retry = h->retry2.as<int>(); out.retry_count = ctr + 2;
 		if ((st = launch_tier<TierWide>(h, h->ws_wide, 64, v, nc, h->retry1.as<int>(), ctr + 1, out, s))) return st;

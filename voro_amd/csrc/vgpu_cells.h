@@ -26,12 +26,17 @@ struct CellOut {
 
 // Neighbour ids of one cell go to a scratch row of the lane's slab first,
 // then to the arena once the count is known.
-template<class T_, int ST>
+template<class CELL>
 struct SlabSink {
-	VCell<T_, ST> *c;
-	size_t off;
-	VG_INL void put(int i, int v) { if (i < T_::MAXV) c->template A<int>(off, i) = v; }
+	CELL *c;
+	VG_INL void put(int i, int v) {
+		if (i < (int)CELL::L::NBCAP) c->template A<int>(CELL::L::o_nbs, i) = v;
+	}
 };
+
+// Cell type of each tier.
+template<class T_> struct TierCell { typedef VCell<T_, 64> type; };
+template<> struct TierCell<Tier3> { typedef VCell3 type; };
 
 // Persistent cell kernel, one wavefront per workgroup.  Wavefronts take
 // 64 consecutive cells at a time; the cell range is split into one
@@ -43,14 +48,15 @@ __global__ void __launch_bounds__(64, 2)
 k_cells(RectView v, int ncells, const int *__restrict__ list, const int *__restrict__ list_count,
 	const int *__restrict__ cellblk, int *__restrict__ counters, char *__restrict__ ws, size_t slab_bytes,
 	unsigned stamp_base, CellOut out) {
-	typedef Layout<T_, 64> L;
+	typedef typename TierCell<T_>::type CELL;
 	const int lane = threadIdx.x;
-	VCell<T_, 64> c;
+	CELL c;
 	c.setup(ws + (size_t)blockIdx.x * slab_bytes, lane, v.max_len_sq);
 	const int total = list_count ? *list_count : ncells;
 	const int nshare = 8;
 	const int my = blockIdx.x % nshare;
 	const int share = (total + nshare - 1) / nshare;
+	const bool last_tier = T_::MAXV >= TierHuge::MAXV;
 	for (int h = 0; h < nshare; h++) {
 		const int sh = (my + h) % nshare;
 		const int lo = sh * share, hi = min(total, lo + share);
@@ -66,10 +72,16 @@ k_cells(RectView v, int ncells, const int *__restrict__ list, const int *__restr
 				const int s = cell - v.bstart[b];
 				const int ci = b % v.nx, cj = (b / v.nx) % v.ny, ck = b / v.nxy;
 				CellStats stt;
-				bool ok = compute_cell_rect<T_, 64, POLY>(c, v, b, s, ci, cj, ck, stamp_base + (unsigned)cell + 1u, stt);
+				bool ok = compute_cell_rect<CELL, POLY>(c, v, b, s, ci, cj, ck, stamp_base + (unsigned)cell + 1u, stt);
+				double vol = 0, cx = 0, cy = 0, cz = 0;
+				int nf = 0;
+				if (!c.ovf && ok) {
+					SlabSink<CELL> sk{&c};
+					c.outputs(vol, cx, cy, cz, nf, sk);
+					if (nf > (int)CELL::L::NBCAP) c.ovf = true;
+				}
 				if (c.ovf) {
-					if (T_::MAXV >= TierHuge::MAXV) {
-						// Last tier: report, do not retry.
+					if (last_tier) {
 						atomicAdd(out.fail_count, 1);
 						out.status[cell] = c.fatal ? ST_FATAL : ST_OVERFLOW;
 					} else {
@@ -78,23 +90,16 @@ k_cells(RectView v, int ncells, const int *__restrict__ list, const int *__restr
 						out.status[cell] = ST_OVERFLOW;
 					}
 				} else {
-					double vol = 0, cx = 0, cy = 0, cz = 0;
-					int nf = 0;
-					if (ok) {
-						SlabSink<T_, 64> sk{&c, L::o_qu};
-						c.outputs(vol, cx, cy, cz, nf, sk);
-					}
 					out.vol[cell] = vol;
 					out.cent[3 * cell] = cx; out.cent[3 * cell + 1] = cy; out.cent[3 * cell + 2] = cz;
 					out.nfaces[cell] = nf;
 					out.status[cell] = ok ? ST_EXISTS : 0;
 					if (out.want_nbrs) {
-						int nkeep = nf < (int)T_::MAXV ? nf : (int)T_::MAXV;
-						int off = nkeep > 0 ? atomicAdd(out.arena_top, nkeep) : 0;
-						if (off + nkeep > out.arena_cap) { out.nbr_start[cell] = -1; atomicAdd(out.fail_count, 1); }
+						int off = nf > 0 ? atomicAdd(out.arena_top, nf) : 0;
+						if (off + nf > out.arena_cap) { out.nbr_start[cell] = -1; atomicAdd(out.fail_count, 1); }
 						else {
 							out.nbr_start[cell] = off;
-							for (int t = 0; t < nkeep; t++) out.nbr[off + t] = c.template A<int>(L::o_qu, t);
+							for (int t = 0; t < nf; t++) out.nbr[off + t] = c.template A<int>(CELL::L::o_nbs, t);
 						}
 					}
 					if (out.st_planes) {
@@ -113,9 +118,9 @@ k_cells(RectView v, int ncells, const int *__restrict__ list, const int *__restr
 typedef int (*cells_launcher)(bool poly, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
 			      const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
 			      unsigned stamp_base, const CellOut &out);
-int launch_cells_fast(bool poly, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
-		      const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
-		      unsigned stamp_base, const CellOut &out);
+int launch_cells_fast3(bool poly, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
+		       const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
+		       unsigned stamp_base, const CellOut &out);
 int launch_cells_wide(bool poly, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
 		      const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
 		      unsigned stamp_base, const CellOut &out);

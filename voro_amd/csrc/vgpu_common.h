@@ -70,6 +70,8 @@ struct Layout {
 	static constexpr size_t o_bms = o_bm + al((size_t)WWORDS * ST * 4);
 	static constexpr size_t o_qu = o_bms + al((size_t)WWORDS * ST * 4);
 	static constexpr size_t bytes = o_qu + al((size_t)MAXQ * ST * 4);
+	static constexpr size_t o_nbs = o_qu;   // output staging (reuses the ring)
+	enum { NBCAP = MAXV };
 };
 
 // Per-cell status bits.
