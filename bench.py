@@ -38,6 +38,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample", type=int, default=1_000_000,
                     help="particles of the workload timed on the CPU")
+    ap.add_argument("--occ", type=int, default=0, help="fast-tier waves per SIMD (2,3,4; 0 = default)")
+    ap.add_argument("--waves", type=int, default=0, help="fast-tier persistent wavefronts (0 = default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (--gpus 1); multi-GPU runs are not part of it",
@@ -57,6 +59,8 @@ def main():
 
     con = voro_amd.container(0, 100, 0, 100, 0, 100, 58, 58, 58, False, False, False, 8)
     con.put_device(len(ids), d_ids.data_ptr(), d_xyz.data_ptr())
+    if args.occ or args.waves:
+        con.set_launch(args.waves, args.occ)
     flags = voro_amd.F_NEIGHBORS | voro_amd.F_CSR
     for _ in range(args.warmup):
         con.compute_all_cells(flags)
@@ -110,7 +114,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "uniform 1M particles in [0,100]^3, 58^3 blocks, seed 42, neighbour outputs",
-                   "n_particles": int(args.n), "grid": [58, 58, 58], "path": "plain"},
+                   "n_particles": int(args.n), "grid": [58, 58, 58], "path": "plain",
+                   "fast_occupancy": args.occ, "fast_waves": args.waves},
         "roofline": {"bound": "hbm", "achieved": hbm_gbs, "peak": MI355X_HBM_GBS, "unit": "GB/s",
                      "frac": hbm_gbs / MI355X_HBM_GBS, "traffic": traffic,
                      "kernel": "k_cells<TierFast>", "kernel_ms": kern_s * 1e3},

@@ -107,8 +107,11 @@ int vgpu_sum_volumes(vgpu_container *h, double *out);
 int vgpu_device_results(vgpu_container *h, const double **vol, const double **centroid, const int **nfaces,
 			const int **status, const int **nbr_start, const int **nbr);
 
-/* Tuning: workgroups of the fast-tier kernel (0 = automatic). */
-int vgpu_set_launch(vgpu_container *h, int fast_waves);
+/* Tuning of the fast-tier cell kernel: the number of persistent
+ * wavefronts (0 = 4 x occupancy x compute units) and the register budget it
+ * was compiled for, in waves per SIMD (2, 3 or 4; 0 = keep).  Results do not
+ * depend on either. */
+int vgpu_set_launch(vgpu_container *h, int fast_waves, int fast_occupancy);
 
 /* Time of the last vgpu_compute_all, split by stage, in milliseconds, from
  * HIP events: [0] binning, [1] fast tier, [2] wide+huge tiers, [3] outputs. */

@@ -80,7 +80,7 @@ def lib():
                                        C.c_longlong, i64p]),
         "vgpu_get_stats": (C.c_int, [p, i32p, i32p, i32p]),
         "vgpu_sum_volumes": (C.c_int, [p, f64p]),
-        "vgpu_set_launch": (C.c_int, [p, C.c_int]),
+        "vgpu_set_launch": (C.c_int, [p, C.c_int, C.c_int]),
         "vgpu_last_timings": (C.c_int, [p, f64p]),
         "vgpu_status_string": (C.c_char_p, [C.c_int]),
     }
@@ -175,8 +175,8 @@ class _Base:
         n = C.c_longlong()
         return lib().vgpu_num_cells(self._h, C.byref(n)) == OK
 
-    def set_launch(self, fast_waves):
-        _check(lib().vgpu_set_launch(self._h, int(fast_waves)), "vgpu_set_launch")
+    def set_launch(self, fast_waves=0, fast_occupancy=0):
+        _check(lib().vgpu_set_launch(self._h, int(fast_waves), int(fast_occupancy)), "vgpu_set_launch")
 
     def timings(self):
         a = np.zeros(4)

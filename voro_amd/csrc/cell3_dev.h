@@ -93,6 +93,21 @@ struct VCell3 {
 #endif
 	}
 
+	// x and y of a record in one 16-byte load (they share a 16-byte slot).
+	VG_INL void ldXY(unsigned off, double &x, double &y) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+		typedef unsigned u4 __attribute__((ext_vector_type(4)));
+		u4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, lo + off, 0, 0);
+		typedef unsigned u2 __attribute__((ext_vector_type(2)));
+		u2 a = {v.x, v.y}, b = {v.z, v.w};
+		x = __builtin_bit_cast(double, a);
+		y = __builtin_bit_cast(double, b);
+#else
+		__builtin_memcpy(&x, wsb + off, 8);
+		__builtin_memcpy(&y, wsb + off + 8, 8);
+#endif
+	}
+
 	// Proxy used by the shared block-search code (BM/BS/QU arrays).
 	template<class T>
 	struct Ref {
@@ -133,6 +148,11 @@ struct VCell3 {
 	VG_INL int DM(int i) const { return ld<unsigned char>(L::o_dm + i); }
 	VG_INL void setDS(int i, int v) const { st<unsigned char>(L::o_ds + i, (unsigned char)v); }
 	VG_INL void setDM(int i, int v) const { st<unsigned char>(L::o_dm + i, (unsigned char)v); }
+
+	// Fields of a topology word held in registers.
+	static VG_INL int twE(unsigned long long w, int i) { int e = (int)((w >> (8 * i)) & 0xff); return e == 0xff ? -1 : e; }
+	static VG_INL int twR(unsigned long long w, int i) { return (int)((w >> (24 + 2 * i)) & 3); }
+	static VG_INL int twBP(unsigned long long w) { return (int)(short)(unsigned short)(w >> 32); }
 
 	static VG_INL int cyc_up(int a) { return a == 2 ? 0 : a + 1; }
 	static VG_INL int cyc_down(int a) { return a == 0 ? 2 : a - 1; }
@@ -195,9 +215,12 @@ struct VCell3 {
 	VG_INL unsigned m_test(int n, double &ans) {
 		if (n < 0 || n >= MAXV) { fatal = ovf = true; ans = 0; return 1u; }
 		const unsigned r = rec(n);
-		ans = ld<double>(r) * px;
-		ans += ld<double>(r + 8) * py;
-		ans += ld<double>(r + 16) * pz - prsq;
+		double x, y;
+		ldXY(r, x, y);
+		const double z = ld<double>(r + 16);
+		ans = x * px;
+		ans += y * py;
+		ans += z * pz - prsq;
 		return ans < -tol ? 0u : (ans > tol ? 2u : 1u);
 	}
 	// src/cell.cc:2072-2083: a kept vertex within big_tol of the plane
@@ -265,9 +288,10 @@ struct VCell3 {
 		int vs = -1, guard = 0;
 		lp = up; l = u;
 		for (;;) {
+			const unsigned long long w = TW(lp);
 			for (ls = 0; ls < 3; ls++) {
 				if (ls == vs) continue;
-				up = E(lp, ls);
+				up = twE(w, ls);
 				uw = m_test(up, u);
 				if (u > l) break;
 			}
@@ -335,9 +359,10 @@ struct VCell3 {
 	VG_FN bool search_downward(unsigned &lw, int &lp, int &ls, int &us, double &l, double &u) {
 		int vs = -1, guard = 0;
 		for (;;) {
+			const unsigned long long w = TW(up);
 			for (us = 0; us < 3; us++) {
 				if (us == vs) continue;
-				lp = E(up, us);
+				lp = twE(w, us);
 				lw = m_test(lp, l);
 				if (u > l) break;
 			}

@@ -1,9 +1,16 @@
 // Cell kernel instantiations for the huge tier (own translation unit so the
-// device code of the three tiers compiles in parallel).
+// device code of the tiers compiles in parallel).
 #include <hip/hip_runtime.h>
 
 #include "vgpu_cells.h"
 
 namespace vgpu {
-VGPU_DEFINE_CELLS_LAUNCHER(launch_cells_huge, TierHuge)
+int launch_cells_huge(bool poly, int occ, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
+		      const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
+		      unsigned stamp_base, const CellOut &out) {
+	(void)occ;
+	if (poly) VGPU_CELLS_LAUNCH(TierHuge, true, 2);
+	else VGPU_CELLS_LAUNCH(TierHuge, false, 2);
+	return hipGetLastError() == hipSuccess ? 0 : 6;
+}
 }

@@ -86,6 +86,7 @@ struct vgpu_container {
 	bool computed = false;
 	unsigned last_flags = 0;
 	int fast_waves = 0;
+	int fast_occ = 2;
 	hipEvent_t ev[5];
 	bool events = false;
 	double timings[4] = {0, 0, 0, 0};
@@ -164,20 +165,20 @@ int launch_tier(vgpu_container *h, Workspace &ws, int waves, const RectView &v, 
 	const bool poly = h->kind == VGPU_CONTAINER_POLY;
 	cells_launcher fn = std::is_same<T_, Tier3>::value ? launch_cells_fast3
 			    : (T_::MAXV == TierWide::MAXV ? launch_cells_wide : launch_cells_huge);
-	if (fn(poly, waves, s, v, ncells, list, list_count, h->cellblk.as<int>(), ctr, ws.base(), ws.slab,
+	if (fn(poly, h->fast_occ, waves, s, v, ncells, list, list_count, h->cellblk.as<int>(), ctr, ws.base(), ws.slab,
 	       ws.stamp_base, out))
 		return VGPU_HIP_ERROR;
 	ws.stamp_base += (unsigned)ncells + 1u;
 	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
 }
 
-int default_fast_waves() {
+int default_fast_waves(int occ) {
 	int dev = 0, cus = 256;
 	if (hipGetDevice(&dev) == hipSuccess) {
 		hipDeviceProp_t pr;
 		if (hipGetDeviceProperties(&pr, dev) == hipSuccess && pr.multiProcessorCount > 0) cus = pr.multiProcessorCount;
 	}
-	return cus * 8;
+	return cus * 4 * occ;
 }
 
 int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
@@ -261,7 +262,7 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 	out.st_tier = stats ? h->st_tier.as<signed char>() : nullptr;
 	out.want_nbrs = nbrs ? 1 : 0;
 
-	int fw = h->fast_waves > 0 ? h->fast_waves : default_fast_waves();
+	int fw = h->fast_waves > 0 ? h->fast_waves : default_fast_waves(h->fast_occ);
 	fw = std::max(1, std::min(fw, nblk(std::max(nc, 1), 64)));
 	if (nc > 0) {
 		out.tier = 0; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
@@ -548,9 +549,10 @@ int vgpu_device_results(vgpu_container *h, const double **vol, const double **ce
 	return VGPU_OK;
 }
 
-int vgpu_set_launch(vgpu_container *h, int fast_waves) {
-	if (!h || fast_waves < 0) return VGPU_INVALID_ARGUMENT;
+int vgpu_set_launch(vgpu_container *h, int fast_waves, int fast_occupancy) {
+	if (!h || fast_waves < 0 || fast_occupancy < 0 || fast_occupancy > 4) return VGPU_INVALID_ARGUMENT;
 	h->fast_waves = fast_waves;
+	if (fast_occupancy > 0) h->fast_occ = fast_occupancy < 2 ? 2 : fast_occupancy;
 	return VGPU_OK;
 }
 

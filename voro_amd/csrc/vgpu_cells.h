@@ -43,8 +43,8 @@ template<> struct TierCell<Tier3> { typedef VCell3 type; };
 // contiguous share per XCD (workgroups are dispatched to the 8 XCDs round
 // robin) so that neighbouring cells, which read the same particle blocks,
 // share an L2; a wavefront whose share is exhausted moves to the next one.
-template<class T_, bool POLY>
-__global__ void __launch_bounds__(64, 2)
+template<class T_, bool POLY, int OCC>
+__global__ void __launch_bounds__(64, OCC)
 k_cells(RectView v, int ncells, const int *__restrict__ list, const int *__restrict__ list_count,
 	const int *__restrict__ cellblk, int *__restrict__ counters, char *__restrict__ ws, size_t slab_bytes,
 	unsigned stamp_base, CellOut out) {
@@ -113,32 +113,24 @@ k_cells(RectView v, int ncells, const int *__restrict__ list, const int *__restr
 	}
 }
 
-// Launches k_cells<tier,poly> (one translation unit per tier keeps the
-// build parallel).
-typedef int (*cells_launcher)(bool poly, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
-			      const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
-			      unsigned stamp_base, const CellOut &out);
-int launch_cells_fast3(bool poly, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
+// Launches k_cells<tier,poly,occupancy> (one translation unit per tier
+// keeps the build parallel).  `occ` is the waves-per-SIMD register budget
+// the kernel was compiled for (2, 3 or 4 for the fast tier).
+typedef int (*cells_launcher)(bool poly, int occ, int waves, hipStream_t s, const RectView &v, int ncells,
+			      const int *list, const int *list_count, const int *cellblk, int *counters, char *ws,
+			      size_t slab, unsigned stamp_base, const CellOut &out);
+int launch_cells_fast3(bool poly, int occ, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
 		       const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
 		       unsigned stamp_base, const CellOut &out);
-int launch_cells_wide(bool poly, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
+int launch_cells_wide(bool poly, int occ, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
 		      const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
 		      unsigned stamp_base, const CellOut &out);
-int launch_cells_huge(bool poly, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
+int launch_cells_huge(bool poly, int occ, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
 		      const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
 		      unsigned stamp_base, const CellOut &out);
 
-#define VGPU_DEFINE_CELLS_LAUNCHER(NAME, TIER)                                                              \
-	int NAME(bool poly, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,          \
-		 const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,               \
-		 unsigned stamp_base, const CellOut &out) {                                                    \
-		if (poly)                                                                                      \
-			hipLaunchKernelGGL((k_cells<TIER, true>), dim3(waves), dim3(64), 0, s, v, ncells, list,    \
-					   list_count, cellblk, counters, ws, slab, stamp_base, out);              \
-		else                                                                                           \
-			hipLaunchKernelGGL((k_cells<TIER, false>), dim3(waves), dim3(64), 0, s, v, ncells, list,   \
-					   list_count, cellblk, counters, ws, slab, stamp_base, out);              \
-		return hipGetLastError() == hipSuccess ? 0 : 6;                                                \
-	}
+#define VGPU_CELLS_LAUNCH(TIER, P, O)                                                                      \
+	hipLaunchKernelGGL((k_cells<TIER, P, O>), dim3(waves), dim3(64), 0, s, v, ncells, list, list_count,    \
+			   cellblk, counters, ws, slab, stamp_base, out)
 
 }  // namespace vgpu
