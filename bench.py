@@ -40,6 +40,9 @@ def main():
                     help="particles of the workload timed on the CPU")
     ap.add_argument("--occ", type=int, default=0, help="fast-tier waves per SIMD (2,3,4; 0 = default)")
     ap.add_argument("--waves", type=int, default=0, help="fast-tier persistent wavefronts (0 = default)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB (scale for the FETCH_SIZE and "
+                         "WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (--gpus 1); multi-GPU runs are not part of it",
@@ -47,6 +50,16 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.calibrate_copy:
+        n = args.calibrate_copy << 20
+        a = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n}))
+        return
 
     import voro_amd
     from voro_amd import inputs as I
