@@ -95,7 +95,9 @@ int vgpu_get_results(vgpu_container *h, int *id, int *ijk, int *q, int *exists, 
 
 /* Per-cell statistics recorded with VGPU_F_STATS: nplane calls, worklist
  * exit index (64 = reached the block search), and the tier that produced
- * the cell (0 fast, 1 wide, 2 huge).  Loop order. */
+ * the cell (0 first tier: order-3 or, with VGPU_F_LEARNED, the list-driven
+ * one; 1 order-3 over the learned tier's leftovers; 2 wide; 3 huge).  Loop
+ * order. */
 int vgpu_get_stats(vgpu_container *h, int *n_planes, int *wl_exit, int *tier);
 
 /* Sum of cell volumes in loop order (src/container.cc:451-456
@@ -116,6 +118,49 @@ int vgpu_set_launch(vgpu_container *h, int fast_waves, int fast_occupancy);
 /* Time of the last vgpu_compute_all, split by stage, in milliseconds, from
  * HIP events: [0] binning, [1] fast tier, [2] wide+huge tiers, [3] outputs. */
 int vgpu_last_timings(vgpu_container *h, double *ms4);
+
+/* ---- Learned candidate-depth predictor (no counterpart in the reference) ----
+ * A small perceptron (28 inputs: the particle counts of the 3x3x3 blocks
+ * around a block divided by the mean count, and in radical mode the radius
+ * divided by the largest one; two hidden layers of 32, ReLU; FP32) predicts
+ * the worklist depth at which the security-radius test stops a cell.  With
+ * VGPU_F_LEARNED, vgpu_compute_all gathers each cell's candidates up to the
+ * predicted depth into a list and cuts from the list; a cell whose list ends
+ * before the radius test stops it is recomputed by the plain path, so the
+ * results are identical with or without the predictor.  Training runs on
+ * the GPU (forward, backward and Adam kernels). */
+typedef struct vgpu_predictor vgpu_predictor;
+
+/* with_radius: predict per particle with the radius as input 28 (for the
+ * _poly kinds); otherwise per block. */
+int vgpu_predictor_create(int with_radius, vgpu_predictor **out);
+int vgpu_predictor_destroy(vgpu_predictor *p);
+/* Appends training samples from the last vgpu_compute_all of h, which must
+ * have been run with VGPU_F_STATS (the plain path's own statistics are the
+ * labels). */
+int vgpu_predictor_add_samples(vgpu_predictor *p, vgpu_container *h);
+int vgpu_predictor_clear_samples(vgpu_predictor *p);
+int vgpu_predictor_num_samples(vgpu_predictor *p, long long *n);
+/* Full-batch Adam for `epochs` steps at learning rate lr (parameters start
+ * from a seeded initialisation unless set before), then sets the depth
+ * margin so that a fraction `fallback` of the samples would be
+ * under-predicted.  *loss receives the final mean squared error. */
+int vgpu_predictor_train(vgpu_predictor *p, int epochs, float lr, unsigned seed, float fallback, float *loss);
+/* Reads (get) and/or replaces (set) the n = vgpu_mlp_num_params() weights. */
+int vgpu_predictor_params(vgpu_predictor *p, float *get, const float *set, int n);
+int vgpu_predictor_margin(vgpu_predictor *p, float *get, const float *set);
+/* Uses p (NULL: none) for VGPU_F_LEARNED computations of h. */
+int vgpu_set_predictor(vgpu_container *h, vgpu_predictor *p);
+/* Cells of the last learned computation that fell back to the plain path,
+ * the list entries gathered (including batch padding), and the time from
+ * the end of binning to the end of list gathering (features, prediction,
+ * gathering; included in vgpu_last_timings' first-tier figure). */
+int vgpu_learned_stats(vgpu_container *h, long long *fallbacks, long long *list_entries, double *gather_ms);
+/* The network's kernels on host data, for checking them: loss
+ * (1/n) sum (y - t)^2 and its gradient, and forward outputs (n x 2). */
+int vgpu_mlp_num_params(void);
+int vgpu_mlp_loss_grad(const float *params, const float *X, const float *T, int n, float *loss, float *grad);
+int vgpu_mlp_forward(const float *params, const float *X, int n, float *Y);
 
 /* Human-readable text of a status code. */
 const char *vgpu_status_string(int status);

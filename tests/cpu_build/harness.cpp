@@ -11,6 +11,7 @@
 
 #include "../../voro_amd/csrc/compute_dev.h"
 #include "../../voro_amd/csrc/grid_host.h"
+#include "../../voro_amd/csrc/list_dev.h"
 
 using namespace vgpu;
 
@@ -49,11 +50,43 @@ struct Runner {
 	}
 };
 
+// The list-driven tier: gather the cell's candidate list at depth G, run
+// compute_cell_list; -1 sends the cell on to the plain tiers.
+template<bool POLY>
+struct ListRunner {
+	std::vector<char> slab;
+	VCell3 c;
+	std::vector<ListEntry> la;
+	std::vector<ListTag> lb;
+	explicit ListRunner(double mls) : slab(Layout3<>::lane_bytes + 2 * 65536, 0) { c.setup(slab.data() + 65536, 0, mls); }
+	int run(const RectView &v, const int *cellblk, int cell, int G, int ijk, int s, double *vol, double *cent, int *nf,
+		std::vector<int> &nb, CellStats &st, long long *fallbacks) {
+		const int n = gather_cell_list<false>(v, cellblk, cell, G, nullptr, nullptr, 1);
+		la.resize(n); lb.resize(n);
+		gather_cell_list<true>(v, cellblk, cell, G, la.data(), lb.data(), 1);
+		int r = compute_cell_list<VCell3, POLY>(c, v, ijk, s, 0, 0, 0, la.data(), lb.data(), 1, n, st);
+		if (r == -1) { (*fallbacks)++; return -1; }
+		if (r < 0 || c.ovf) return -1;
+		if (r == 1) {
+			VecSink sk{&nb};
+			c.outputs(*vol, cent[0], cent[1], cent[2], *nf, sk);
+			if (c.ovf) { nb.clear(); return -1; }
+		}
+		return r;
+	}
+};
+
 static inline int step_int(double a) { return a < 0 ? int(a) - 1 : int(a); }
 static inline int step_mod(int a, int b) { return a >= 0 ? a % b : b - 1 - (b - 1 - a) % b; }
 }  // namespace
 
 extern "C" {
+
+static int list_depth = 0;
+static long long list_fallbacks = 0;
+// G > 0: first try every cell through a candidate list of worklist depth G.
+void hx_use_list(int G) { list_depth = G; list_fallbacks = 0; }
+long long hx_list_fallbacks(void) { return list_fallbacks; }
 
 // tier_counts[3]: cells finished by each tier.  Returns cells, or -1 on a
 // cell no tier could finish.
@@ -102,6 +135,11 @@ long long hx_compute_neighbor(const hx_spec *s, int n, const int *ids, const dou
 	v.P = P.data(); v.pid = pid.data(); v.bstart = cnt.data(); v.mrad = mrad.data(); v.wl = vgpu_worklist_table;
 	v.walls = walls.data(); v.nwalls = s->n_spheres; v.max_radius = maxr;
 
+	std::vector<int> cellblk(m);
+	for (int b = 0; b < nxyz; b++) for (int q = cnt[b]; q < cnt[b + 1]; q++) cellblk[q] = b;
+	ListRunner<false> lr0(v.max_len_sq);
+	ListRunner<true> lr1(v.max_len_sq);
+
 	Runner<VCell3, Layout3<>::lane_bytes> r0(v.max_len_sq);
 	Runner<VCell<TierWide, 1>, Layout<TierWide, 1>::bytes> r1(v.max_len_sq);
 	Runner<VCell<TierHuge, 1>, Layout<TierHuge, 1>::bytes> *r2 = 0;
@@ -117,7 +155,13 @@ long long hx_compute_neighbor(const hx_spec *s, int n, const int *ids, const dou
 			CellStats st;
 			nb.clear();
 			int tier = 0;
-			res = poly ? r0.run<true>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st)
+			res = -1;
+			if (list_depth > 0) {
+				res = poly ? lr1.run(v, cellblk.data(), cell, list_depth, ijk, q - cnt[ijk], &vol, cent, &nf, nb, st, &list_fallbacks)
+					   : lr0.run(v, cellblk.data(), cell, list_depth, ijk, q - cnt[ijk], &vol, cent, &nf, nb, st, &list_fallbacks);
+				if (res < 0) nb.clear();
+			}
+			if (res < 0) res = poly ? r0.run<true>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st)
 				   : r0.run<false>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st);
 			if (res < 0) {
 				tier = 1; nb.clear();

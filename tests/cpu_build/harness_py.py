@@ -16,9 +16,14 @@ def build():
                     "-o", SO, os.path.join(HERE, "harness.cpp")], check=True)
 
 
-def run(O, spec, ids, xyz, r=None):
-    """Same result dict as oracle.run, plus per-cell stats and tier counts."""
+def run(O, spec, ids, xyz, r=None, list_depth=0):
+    """Same result dict as oracle.run, plus per-cell stats and tier counts.
+    list_depth > 0 computes every cell through a candidate list of that
+    worklist depth first (the learned path's kernel), the rest on the plain
+    tiers; res["list_fallbacks"] counts the cells whose list was too short."""
     lib = C.CDLL(SO)
+    lib.hx_use_list(int(list_depth))
+    lib.hx_list_fallbacks.restype = C.c_longlong
     f = lib.hx_compute_neighbor
     f.restype = C.c_longlong
     n = len(ids)
@@ -51,5 +56,6 @@ def run(O, spec, ids, xyz, r=None):
     out["nbr_off"] = res["nbr_off"][:m + 1]
     out["nbr"] = res["nbr"][:int(res["nbr_off"][m])]
     out["stats"] = st[:4 * m].reshape(-1, 4)
+    out["list_fallbacks"] = int(lib.hx_list_fallbacks())
     out["tiers"] = tc
     return out

@@ -1,0 +1,143 @@
+"""GPU checks of the learned path: the predictor's kernels against the numpy
+restatement (oracle/mlp.py), training on statistics the plain path records,
+bit-identical results with the predictor on, forced fallback, and what the
+trained network saves over gathering every cell's lists to full depth."""
+import numpy as np
+import pytest
+
+import voro_amd
+from oracle import mlp as M
+from voro_amd import inputs as I
+
+pytestmark = pytest.mark.gpu
+
+F = voro_amd.F_NEIGHBORS | voro_amd.F_CSR
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def test_gpu_mlp_gradient_matches_numpy():
+    rng = np.random.default_rng(7)
+    p = rng.normal(0, 0.3, M.NP).astype(np.float32)
+    X = rng.uniform(0, 2, (3000, M.IN)).astype(np.float32)
+    T = rng.uniform(0, 1, (3000, M.OUT)).astype(np.float32)
+    _flush("mlp gradient n=3000")
+    lo, g = voro_amd.mlp_loss_grad(p, X, T)
+    lo_ref, g_ref = M.loss_grad(p.astype(np.float64), X.astype(np.float64), T.astype(np.float64))
+    assert abs(lo - lo_ref) <= 1e-5 * abs(lo_ref)
+    scale = np.max(np.abs(g_ref))
+    assert np.max(np.abs(g - g_ref)) <= 2e-5 * scale
+    Y = voro_amd.mlp_forward(p, X)
+    assert np.allclose(Y, M.forward(p.astype(np.float64), X.astype(np.float64))[2], rtol=1e-5, atol=1e-5)
+
+
+def _uniform(seed, n, L, G):
+    xyz = I.uniform_box(seed, n, (0, 0, 0), (L, L, L))
+    con = voro_amd.container(0, L, 0, L, 0, L, G, G, G, False, False, False, 8)
+    con.put_array(np.arange(n, dtype=np.int32), xyz)
+    return con
+
+
+def _train(make, seeds, epochs=300):
+    pred = voro_amd.predictor()
+    for sd in seeds:
+        c = make(sd)
+        c.compute_all_cells(F | voro_amd.F_STATS)
+        pred.add_samples(c)
+    loss0 = pred.train(epochs=1, lr=3e-3, seed=1)
+    loss = pred.train(epochs=epochs, lr=3e-3, seed=1)
+    return pred, loss0, loss
+
+
+def _same(a, b):
+    for k in ("id", "exists", "nfaces", "nbr_off", "nbr"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(a["vol"], b["vol"]) and np.array_equal(a["cent"], b["cent"])
+
+
+def test_gpu_learned_uniform_is_exact_and_saves_work():
+    n, L, G = 100000, 46.4, 27
+    make = lambda sd: _uniform(sd, n, L, G)
+    _flush("train uniform n=%d" % n)
+    pred, loss0, loss = _train(make, [11, 12])
+    assert loss < 0.5 * loss0
+    con = make(13)
+    con.compute_all_cells(F)
+    plain = dict(con.results())
+    con.set_predictor(pred)
+    con.compute_all_cells(F | voro_amd.F_LEARNED)
+    con._res = None
+    _same(plain, con.results())
+    st = con.learned_stats()
+    _flush("fallbacks %d of %d, entries %d" % (st["fallbacks"], n, st["list_entries"]))
+    assert st["fallbacks"] < 0.1 * n
+    # without the network: every list gathered to the full worklist
+    pred.margin = 1000.0
+    con.set_predictor(pred)
+    con.compute_all_cells(F | voro_amd.F_LEARNED)
+    con._res = None
+    _same(plain, con.results())
+    full = con.learned_stats()
+    _flush("entries trained %d, full depth %d" % (st["list_entries"], full["list_entries"]))
+    assert st["list_entries"] < full["list_entries"]
+
+
+def test_gpu_learned_forced_fallback_is_exact():
+    n, L, G = 20000, 27.1, 16
+    make = lambda sd: _uniform(sd, n, L, G)
+    pred, _, _ = _train(make, [21], epochs=50)
+    pred.margin = -1000.0     # depth 1 for every block: lists end at once
+    con = make(22)
+    con.compute_all_cells(F)
+    plain = dict(con.results())
+    con.set_predictor(pred)
+    con.compute_all_cells(F | voro_amd.F_LEARNED)
+    con._res = None
+    _same(plain, con.results())
+    assert con.learned_stats()["fallbacks"] > 0.9 * n
+
+
+def test_gpu_learned_clusters_held_out_seed():
+    n, box, G = 100000, 46.4, 27
+
+    def make(sd):
+        xyz = I.clusters(sd, n, box=box, n_clusters=64)
+        con = voro_amd.container(0, box, 0, box, 0, box, G, G, G, False, False, False, 8)
+        con.put_array(np.arange(len(xyz), dtype=np.int32), xyz)
+        return con
+    _flush("train clusters n=%d" % n)
+    pred, loss0, loss = _train(make, [1, 2, 3])
+    con = make(101)
+    con.compute_all_cells(F)
+    plain = dict(con.results())
+    con.set_predictor(pred)
+    con.compute_all_cells(F | voro_amd.F_LEARNED)
+    con._res = None
+    _same(plain, con.results())
+    _flush("clusters fallback rate %.4f" % (con.learned_stats()["fallbacks"] / n))
+
+
+def test_gpu_learned_radical_per_particle():
+    n, L, G = 30000, 31.0, 18
+
+    def make(sd):
+        xyz = I.uniform_box(sd, n, (0, 0, 0), (L, L, L))
+        r = I.radii_lognormal(sd, n)
+        con = voro_amd.container_poly(0, L, 0, L, 0, L, G, G, G, False, False, False, 8)
+        con.put_array(np.arange(n, dtype=np.int32), xyz, r)
+        return con
+    pred = voro_amd.predictor(with_radius=True)
+    for sd in (31, 32):
+        c = make(sd)
+        c.compute_all_cells(F | voro_amd.F_STATS)
+        pred.add_samples(c)
+    pred.train(epochs=200)
+    con = make(33)
+    con.compute_all_cells(F)
+    plain = dict(con.results())
+    con.set_predictor(pred)
+    con.compute_all_cells(F | voro_amd.F_LEARNED)
+    con._res = None
+    _same(plain, con.results())

@@ -83,6 +83,20 @@ def lib():
         "vgpu_set_launch": (C.c_int, [p, C.c_int, C.c_int]),
         "vgpu_last_timings": (C.c_int, [p, f64p]),
         "vgpu_status_string": (C.c_char_p, [C.c_int]),
+        "vgpu_predictor_create": (C.c_int, [C.c_int, C.POINTER(p)]),
+        "vgpu_predictor_destroy": (C.c_int, [p]),
+        "vgpu_predictor_add_samples": (C.c_int, [p, p]),
+        "vgpu_predictor_clear_samples": (C.c_int, [p]),
+        "vgpu_predictor_num_samples": (C.c_int, [p, i64p]),
+        "vgpu_predictor_train": (C.c_int, [p, C.c_int, C.c_float, C.c_uint, C.c_float, C.POINTER(C.c_float)]),
+        "vgpu_predictor_params": (C.c_int, [p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int]),
+        "vgpu_predictor_margin": (C.c_int, [p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "vgpu_set_predictor": (C.c_int, [p, p]),
+        "vgpu_learned_stats": (C.c_int, [p, i64p, i64p, f64p]),
+        "vgpu_mlp_num_params": (C.c_int, []),
+        "vgpu_mlp_loss_grad": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int,
+                                         C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "vgpu_mlp_forward": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -174,6 +188,16 @@ class _Base:
     def _computed(self):
         n = C.c_longlong()
         return lib().vgpu_num_cells(self._h, C.byref(n)) == OK
+
+    def set_predictor(self, pred):
+        """Uses a trained predictor for compute_all_cells(F_LEARNED | ...)."""
+        _check(lib().vgpu_set_predictor(self._h, pred._h if pred is not None else None), "vgpu_set_predictor")
+        self._pred = pred
+
+    def learned_stats(self):
+        fb, ent, gm = C.c_longlong(), C.c_longlong(), C.c_double()
+        _check(lib().vgpu_learned_stats(self._h, C.byref(fb), C.byref(ent), C.byref(gm)), "vgpu_learned_stats")
+        return {"fallbacks": fb.value, "list_entries": ent.value, "gather_ms": gm.value}
 
     def set_launch(self, fast_waves=0, fast_occupancy=0):
         _check(lib().vgpu_set_launch(self._h, int(fast_waves), int(fast_occupancy)), "vgpu_set_launch")
@@ -312,3 +336,86 @@ class voronoicell_neighbor:
     def neighbors(self):
         s, e = self._r["nbr_off"][self._m], self._r["nbr_off"][self._m + 1]
         return [int(v) for v in self._r["nbr"][s:e]]
+
+
+class predictor:
+    """The learned candidate-depth predictor (no counterpart in the
+    reference): a 28-32-32-2 perceptron trained on the GPU on statistics the
+    plain path records (compute_all_cells with F_STATS)."""
+
+    def __init__(self, with_radius=False):
+        h = C.c_void_p()
+        _check(lib().vgpu_predictor_create(1 if with_radius else 0, C.byref(h)), "vgpu_predictor_create")
+        self._h = h
+        self.with_radius = bool(with_radius)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.vgpu_predictor_destroy(h)
+            self._h = None
+
+    def add_samples(self, con):
+        _check(lib().vgpu_predictor_add_samples(self._h, con._h), "vgpu_predictor_add_samples")
+
+    def clear_samples(self):
+        _check(lib().vgpu_predictor_clear_samples(self._h), "vgpu_predictor_clear_samples")
+
+    def num_samples(self):
+        n = C.c_longlong()
+        _check(lib().vgpu_predictor_num_samples(self._h, C.byref(n)), "vgpu_predictor_num_samples")
+        return n.value
+
+    def train(self, epochs=400, lr=3e-3, seed=1, fallback=0.02):
+        """Full-batch Adam on the GPU; returns the final training loss."""
+        lo = C.c_float()
+        _check(lib().vgpu_predictor_train(self._h, int(epochs), float(lr), int(seed), float(fallback), C.byref(lo)),
+               "vgpu_predictor_train")
+        return lo.value
+
+    def params(self):
+        n = lib().vgpu_mlp_num_params()
+        a = np.zeros(n, np.float32)
+        _check(lib().vgpu_predictor_params(self._h, _p(a, C.c_float), None, n), "vgpu_predictor_params")
+        return a
+
+    def set_params(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        _check(lib().vgpu_predictor_params(self._h, None, _p(a, C.c_float), len(a)), "vgpu_predictor_params")
+
+    @property
+    def margin(self):
+        m = C.c_float()
+        _check(lib().vgpu_predictor_margin(self._h, C.byref(m), None), "vgpu_predictor_margin")
+        return m.value
+
+    @margin.setter
+    def margin(self, v):
+        m = C.c_float(v)
+        _check(lib().vgpu_predictor_margin(self._h, None, C.byref(m)), "vgpu_predictor_margin")
+
+
+def mlp_num_params():
+    return lib().vgpu_mlp_num_params()
+
+
+def mlp_loss_grad(params, X, T):
+    """Loss and gradient of the predictor network on the GPU (its training
+    kernels) for host arrays X (n x 28), T (n x 2)."""
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    T = np.ascontiguousarray(T, dtype=np.float32)
+    g = np.zeros(len(params), np.float32)
+    lo = C.c_float()
+    _check(lib().vgpu_mlp_loss_grad(_p(params, C.c_float), _p(X, C.c_float), _p(T, C.c_float), len(X), C.byref(lo),
+                                    _p(g, C.c_float)), "vgpu_mlp_loss_grad")
+    return lo.value, g
+
+
+def mlp_forward(params, X):
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Y = np.zeros((len(X), 2), np.float32)
+    _check(lib().vgpu_mlp_forward(_p(params, C.c_float), _p(X, C.c_float), len(X), _p(Y, C.c_float)),
+           "vgpu_mlp_forward")
+    return Y

@@ -7,7 +7,7 @@
 namespace vgpu {
 int launch_cells_fast3(bool poly, int occ, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
 		       const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
-		       unsigned stamp_base, const CellOut &out) {
+		       unsigned stamp_base, int per_wave, const CellOut &out) {
 	if (poly) {
 		if (occ >= 4) VGPU_CELLS_LAUNCH(Tier3, true, 4);
 		else if (occ == 3) VGPU_CELLS_LAUNCH(Tier3, true, 3);

@@ -1,0 +1,173 @@
+// The learned path's kernels: candidate-list gathering (count, per-batch
+// capacity, fill) and the list-driven order-3 cell kernel.
+//
+// Lists are stored per batch of 64 consecutive cells, entry k of the cell
+// in lane l at [(batch_offset + k) * 64 + l], so that the cell kernel's 64
+// lanes read entry k of their lists in one coalesced access.
+#include <hip/hip_runtime.h>
+
+#include "list_dev.h"
+#include "vgpu_cells.h"
+
+namespace vgpu {
+
+// Schedule: cells are processed in order of decreasing predicted cost
+// (64 cost classes, any order within a class), so that the 64 cells a
+// wavefront cuts in lockstep need similar work.  Position i of the schedule
+// holds cell order[i]; list counts and batches follow the schedule.
+__global__ void k_order_hist(int nc, const int *__restrict__ cellblk, const unsigned char *__restrict__ cost,
+			     int per_cell, int *__restrict__ hist) {
+	__shared__ int h[64];
+	if (threadIdx.x < 64) h[threadIdx.x] = 0;
+	__syncthreads();
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c < nc) atomicAdd(&h[63 - cost[per_cell ? c : cellblk[c]]], 1);
+	__syncthreads();
+	if (threadIdx.x < 64 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void k_order_scan(int *__restrict__ hist) {
+	if (threadIdx.x == 0) {
+		int a = 0;
+		for (int t = 0; t < 64; t++) { const int v = hist[t]; hist[t] = a; a += v; }
+	}
+}
+
+__global__ void k_order_scatter(int nc, const int *__restrict__ cellblk, const unsigned char *__restrict__ cost,
+				int per_cell, int *__restrict__ fill, int *__restrict__ order) {
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= nc) return;
+	const int k = 63 - cost[per_cell ? c : cellblk[c]];
+	order[atomicAdd(&fill[k], 1)] = c;
+}
+
+__global__ void k_list_count(RectView v, int nc, const int *__restrict__ cellblk, const int *__restrict__ order,
+			     const unsigned char *__restrict__ depth, int per_cell, int *__restrict__ cnt) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nc) return;
+	const int c = order[i];
+	const int G = depth[per_cell ? c : cellblk[c]];
+	cnt[i] = gather_cell_list<false>(v, cellblk, c, G, nullptr, nullptr, 64);
+}
+
+// Capacity of each batch of 64 cells (the longest list in it); entry nb is
+// set to 0 so that an exclusive scan gives the offsets and the total.
+__global__ void k_batch_max(int nc, const int *__restrict__ cnt, int *__restrict__ bmax) {
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	const int nb = (nc + 63) / 64;
+	int v = c < nc ? cnt[c] : 0;
+	for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+	const int b = c >> 6;
+	if ((threadIdx.x & 63) == 0 && b < nb) bmax[b] = v;
+	if (c == 0) bmax[nb] = 0;
+}
+
+__global__ void k_list_fill(RectView v, int nc, const int *__restrict__ cellblk, const int *__restrict__ order,
+			    const unsigned char *__restrict__ depth, int per_cell, const int *__restrict__ boff,
+			    ListEntry *__restrict__ LA, ListTag *__restrict__ LB) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nc) return;
+	const int c = order[i];
+	const int G = depth[per_cell ? c : cellblk[c]];
+	const size_t base = (size_t)boff[i >> 6] * 64 + (i & 63);
+	gather_cell_list<true>(v, cellblk, c, G, LA + base, LB + base, 64);
+}
+
+// One wavefront per workgroup, a batch of 64 cells at a time (batches
+// taken in order through an atomic counter).  Cells whose list ends before
+// the radius test stops them, and cells that overflow the order-3 tier, go to
+// the plain path's retry list; *fallbacks counts the former.
+template<bool POLY, int OCC>
+__global__ void __launch_bounds__(64, OCC)
+k_cells_list(RectView v, int nc, const int *__restrict__ cellblk, const int *__restrict__ order, const int *__restrict__ cnt,
+	     const int *__restrict__ boff, const ListEntry *__restrict__ LA, const ListTag *__restrict__ LB,
+	     int *__restrict__ counters, char *__restrict__ ws, int *__restrict__ fallbacks, CellOut out) {
+	const int lane = threadIdx.x;
+	VCell3 c;
+	c.setup(ws + (size_t)blockIdx.x * VCell3::L::bytes, lane, v.max_len_sq);
+	const int nb = (nc + 63) / 64;
+	for (;;) {
+		int b = 0;
+		if (lane == 0) b = atomicAdd(&counters[0], 1);
+		b = __shfl(b, 0);
+		if (b >= nb) break;
+		const int pos = b * 64 + lane;
+		if (pos >= nc) continue;
+		const int cell = order[pos];
+		const int ijk = cellblk[cell];
+		const int s = cell - v.bstart[ijk];
+		const size_t base = (size_t)boff[b] * 64 + lane;
+		CellStats stt;
+		const int r = compute_cell_list<VCell3, POLY>(c, v, ijk, s, 0, 0, 0, LA + base, LB + base, 64, cnt[pos], stt);
+		double vol = 0, cx = 0, cy = 0, cz = 0;
+		int nf = 0;
+		bool retry = r < 0 || c.ovf;
+		if (!retry && r == 1) {
+			SlabSink<VCell3> sk{&c};
+			c.outputs(vol, cx, cy, cz, nf, sk);
+			if (c.ovf || nf > (int)VCell3::L::NBCAP) retry = true;
+		}
+		if (retry) {
+			if (r == -1) atomicAdd(fallbacks, 1);
+			const int slot_ = atomicAdd(out.retry_count, 1);
+			out.retry[slot_] = cell;
+			out.status[cell] = ST_OVERFLOW;
+			continue;
+		}
+		out.vol[cell] = vol;
+		out.cent[3 * cell] = cx; out.cent[3 * cell + 1] = cy; out.cent[3 * cell + 2] = cz;
+		out.nfaces[cell] = nf;
+		out.status[cell] = r == 1 ? ST_EXISTS : 0;
+		if (out.want_nbrs) {
+			const int off = nf > 0 ? atomicAdd(out.arena_top, nf) : 0;
+			if (off + nf > out.arena_cap) { out.nbr_start[cell] = -1; atomicAdd(out.fail_count, 1); }
+			else {
+				out.nbr_start[cell] = off;
+				for (int t = 0; t < nf; t++) out.nbr[off + t] = c.template A<int>(VCell3::L::o_nbs, t);
+			}
+		}
+		if (out.st_planes) {
+			out.st_planes[cell] = (short)min(stt.n_planes, 32767);
+			out.st_exit[cell] = (signed char)stt.wl_exit;
+			out.st_tier[cell] = (signed char)out.tier;
+		}
+	}
+}
+
+int launch_list_order(hipStream_t s, int nc, const int *cellblk, const unsigned char *cost, int per_cell, int *hist,
+		      int *order) {
+	const int g = (nc + 255) / 256;
+	hipLaunchKernelGGL(k_order_hist, dim3(g), dim3(256), 0, s, nc, cellblk, cost, per_cell, hist);
+	hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(64), 0, s, hist);
+	hipLaunchKernelGGL(k_order_scatter, dim3(g), dim3(256), 0, s, nc, cellblk, cost, per_cell, hist, order);
+	return hipGetLastError() == hipSuccess ? 0 : 6;
+}
+
+int launch_list_gather(hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
+		       const unsigned char *depth, int per_cell, int *cnt, int *bmax) {
+	const int g = (nc + 255) / 256;
+	hipLaunchKernelGGL(k_list_count, dim3(g), dim3(256), 0, s, v, nc, cellblk, order, depth, per_cell, cnt);
+	hipLaunchKernelGGL(k_batch_max, dim3(g), dim3(256), 0, s, nc, cnt, bmax);
+	return hipGetLastError() == hipSuccess ? 0 : 6;
+}
+
+int launch_list_fill(hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
+		     const unsigned char *depth, int per_cell, const int *boff, ListEntry *LA, ListTag *LB) {
+	hipLaunchKernelGGL(k_list_fill, dim3((nc + 255) / 256), dim3(256), 0, s, v, nc, cellblk, order, depth, per_cell,
+			   boff, LA, LB);
+	return hipGetLastError() == hipSuccess ? 0 : 6;
+}
+
+int launch_cells_list(bool poly, int waves, hipStream_t s, const RectView &v, int nc, const int *cellblk,
+		      const int *order, const int *cnt, const int *boff, const ListEntry *LA, const ListTag *LB,
+		      int *counters, char *ws, int *fallbacks, const CellOut &out) {
+	if (poly)
+		hipLaunchKernelGGL((k_cells_list<true, 2>), dim3(waves), dim3(64), 0, s, v, nc, cellblk, order, cnt, boff, LA,
+				   LB, counters, ws, fallbacks, out);
+	else
+		hipLaunchKernelGGL((k_cells_list<false, 2>), dim3(waves), dim3(64), 0, s, v, nc, cellblk, order, cnt, boff, LA,
+				   LB, counters, ws, fallbacks, out);
+	return hipGetLastError() == hipSuccess ? 0 : 6;
+}
+
+}  // namespace vgpu

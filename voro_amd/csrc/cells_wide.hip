@@ -7,7 +7,7 @@
 namespace vgpu {
 int launch_cells_wide(bool poly, int occ, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
 		      const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
-		      unsigned stamp_base, const CellOut &out) {
+		      unsigned stamp_base, int per_wave, const CellOut &out) {
 	(void)occ;
 	if (poly) VGPU_CELLS_LAUNCH(TierWide, true, 2);
 	else VGPU_CELLS_LAUNCH(TierWide, false, 2);

@@ -129,22 +129,25 @@ struct BlockSearch {
 // sign: (-b)*x == -(b*x) and a+(-y) == a-y exactly in IEEE arithmetic, so
 // every form evaluates the reference's expression for that direction
 // operation for operation.
-template<bool POLY>
-VG_FN bool min_max_radius(const RectView &v, const RadPolicy<POLY> &rp, int di, int dj, int dk, double fx,
-			  double fy, double fz, double gxs, double gys, double gzs, double &crs, double mrs) {
+// The two radii of compute_min_max_radius for worklist offset (di,dj,dk):
+// crs_near, the squared distance to the near corner/edge/face, which the
+// reference tests against mrs first, and crs_far, which it goes on to
+// compute from crs_near when the block is not skipped.
+VG_FN void min_max_radius_pair(const RectView &v, int di, int dj, int dk, double fx, double fy, double fz, double gxs,
+			       double gys, double gzs, double &crs_near, double &crs_far, int &form) {
 	const double boxx = v.boxx, boxy = v.boxy, boxz = v.boxz;
 	const double sx = di > 0 ? 1.0 : -1.0, sy = dj > 0 ? 1.0 : -1.0, sz = dk > 0 ? 1.0 : -1.0;
 	const double xlo = (di > 0 ? di : di + 1) * boxx - fx;
 	const double ylo = (dj > 0 ? dj : dj + 1) * boxy - fy;
 	const double zlo = (dk > 0 ? dk : dk + 1) * boxz - fz;
-	const int form = (di != 0 ? 4 : 0) | (dj != 0 ? 2 : 0) | (dk != 0 ? 1 : 0);
+	form = (di != 0 ? 4 : 0) | (dj != 0 ? 2 : 0) | (dk != 0 ? 1 : 0);
 	// squared distance to the near corner/edge/face, axes in x,y,z order
 	bool first = true;
-	crs = 0;
+	double crs = 0;
 	if (di != 0) { crs = xlo * xlo; first = false; }
 	if (dj != 0) { if (first) crs = ylo * ylo; else crs += ylo * ylo; first = false; }
 	if (dk != 0) { if (first) crs = zlo * zlo; else crs += zlo * zlo; }
-	if (form != 0 && rp.ctest(crs, mrs)) return true;
+	crs_near = crs;
 	const double ax = boxx * ((2 * sx) * xlo + boxx);
 	const double ay = boxy * ((2 * sy) * ylo + boxy);
 	const double az = boxz * ((2 * sz) * zlo + boxz);
@@ -158,6 +161,17 @@ VG_FN bool min_max_radius(const RectView &v, const RadPolicy<POLY> &rp, int di, 
 	case 1: crs += az; crs += gys; crs += gxs; break;     // z: face
 	default: crs = 0; crs += gys; crs += gxs; break;      // central block: not reached
 	}
+	crs_far = crs;
+}
+
+template<bool POLY>
+VG_FN bool min_max_radius(const RectView &v, const RadPolicy<POLY> &rp, int di, int dj, int dk, double fx,
+			  double fy, double fz, double gxs, double gys, double gzs, double &crs, double mrs) {
+	double crs_far;
+	int form;
+	min_max_radius_pair(v, di, dj, dk, fx, fy, fz, gxs, gys, gzs, crs, crs_far, form);
+	if (form != 0 && rp.ctest(crs, mrs)) return true;
+	crs = crs_far;
 	return false;
 }
 
@@ -449,6 +463,111 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 			if (c.ovf) { stt.n_planes = c.n_cuts; return false; }
 		}
 		if (!c.nplane(px_, py_, pz_, rs, pid)) { stt.n_planes = c.n_cuts; return false; }
+	}
+}
+
+// ---------------------------------------------------------------------------
+// Candidate lists (the learned path).  A list holds, in the reference's
+// candidate order, every candidate of one cell up to a worklist depth G:
+//   kind 1  a particle of the cell's own block        {dx, dy, dz, r}, pid
+//   kind 2  worklist step g -> g+1, block in the grid  {radp[g], crs_near, crs_far}, particle count
+//   kind 0  a particle of that block                   {dx, dy, dz, r}, pid
+//   kind 3  worklist step g -> g+1, block outside      {radp[g]}
+//   kind 4  end of the list at depth G                 {radp[G]}
+// where (dx, dy, dz) is the particle minus the (image-shifted) cell particle
+// and crs_near / crs_far are the two radii compute_min_max_radius derives
+// for the block (src/v_compute.cc:797-944) before and after its mrs test.
+// Everything in an entry is independent of the cell's shape, so the entries
+// are written by a separate gather pass; the decisions that depend on the
+// shape (count_list refresh, the radius stop, the block skip, the
+// per-particle check) are taken here exactly as compute_cell_rect takes them.
+struct ListEntry { double x, y, z, w; };
+struct ListTag { int pid, kind; };
+enum { LK_PART = 0, LK_OWN = 1, LK_BLOCK = 2, LK_NOBLOCK = 3, LK_END = 4 };
+
+// compute_cell over a candidate list.  Returns 1 / 0 (the cell exists /
+// was deleted), -1 when the list ends before the radius test stops the
+// cell (the plain path must compute it), -2 on overflow (c.ovf).
+template<class CELL, bool POLY>
+VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, int ci, int cj, int ck,
+			    const ListEntry *__restrict__ LA, const ListTag *__restrict__ LB, int stride, int nent,
+			    CellStats &stt) {
+	stt.n_planes = 0; stt.wl_exit = 0; stt.phase = 1;
+	const int b0 = v.bstart[ijk];
+	const dbl4 me = v.P[b0 + s];
+	const double x = me.x, y = me.y, z = me.z;
+	{
+		double xa, xb, ya, yb, za, zb;
+		if (v.xper) { xb = 0.5 * (v.bx - v.ax); xa = -xb; } else { xa = v.ax - x; xb = v.bx - x; }
+		if (v.yper) { yb = 0.5 * (v.by - v.ay); ya = -yb; } else { ya = v.ay - y; yb = v.by - y; }
+		if (v.zper) { zb = 0.5 * (v.bz - v.az); za = -zb; } else { za = v.az - z; zb = v.bz - z; }
+		c.init_box(xa, xb, ya, yb, za, zb);
+	}
+	(void)ci; (void)cj; (void)ck;
+	RadPolicy<POLY> rp;
+	rp.init(me.w, v.max_radius);
+	double mrs = 0;
+	int next_count = 3, cpi = 0, g = 0, w = 0, k = 0, left = 0;
+	bool own_done = false, bcheck = false;
+	for (;;) {
+		double px_ = 0, py_ = 0, pz_ = 0, rs = 0;
+		int pid = 0;
+		bool have = false;
+		while (!have) {
+			if (w < v.nwalls) {
+				// Sphere walls (src/container.hh:86-89, src/wall.cc:28-36).
+				const double *wp = v.walls + 5 * w;
+				w++;
+				double xd = x - wp[0], yd = y - wp[1], zd = z - wp[2], dq = xd * xd + yd * yd + zd * zd;
+				if (dq > 1e-5) {
+					dq = 2 * (sqrt(dq) * wp[3] - dq);
+					px_ = xd; py_ = yd; pz_ = zd; rs = dq; pid = (int)wp[4];
+					have = true;
+				}
+				continue;
+			}
+			if (k >= nent) { stt.n_planes = c.n_cuts; return -1; }
+			const ListEntry a = LA[(size_t)k * stride];
+			const ListTag tg = LB[(size_t)k * stride];
+			k++;
+			if (tg.kind == LK_OWN) {
+				px_ = a.x; py_ = a.y; pz_ = a.z; pid = tg.pid;
+				rs = rp.scale(px_ * px_ + py_ * py_ + pz_ * pz_, a.w);
+				have = true;
+				continue;
+			}
+			if (tg.kind == LK_PART) {
+				px_ = a.x; py_ = a.y; pz_ = a.z; pid = tg.pid;
+				rs = px_ * px_ + py_ * py_ + pz_ * pz_;
+				if (!bcheck) { rs = rp.scale(rs, a.w); have = true; }
+				else have = rp.scale_check(rs, mrs, a.w);
+				if (--left == 0) bcheck = false;
+				continue;
+			}
+			// a worklist step: the own block is done (src/v_compute.cc:328)
+			if (!own_done) { mrs = c.max_radius_squared(); own_done = true; }
+			if (g < 63 && g == next_count) {
+				mrs = c.max_radius_squared();
+				if (cpi != 8) {
+					const int count_list[8] = {7, 11, 15, 19, 26, 35, 45, 59};
+					next_count = count_list[cpi++];
+				}
+			}
+			if (rp.ctest(a.x, mrs)) { stt.wl_exit = g; stt.n_planes = c.n_cuts; return 1; }
+			if (tg.kind == LK_END) {
+				stt.wl_exit = g >= 63 ? 64 : g;
+				stt.n_planes = c.n_cuts;
+				return -1;
+			}
+			g++;
+			if (tg.kind == LK_NOBLOCK) continue;
+			const int cnt = tg.pid;
+			if (rp.ctest(a.y, mrs)) { k += cnt; continue; }   // compute_min_max_radius skip
+			bcheck = rp.ctest(a.z, mrs);
+			left = cnt;
+			if (cnt == 0) bcheck = false;
+		}
+		if (!c.nplane(px_, py_, pz_, rs, pid)) { stt.n_planes = c.n_cuts; return c.ovf ? -2 : 0; }
 	}
 }
 

@@ -1,0 +1,96 @@
+// Gathering of candidate lists for the learned path (see compute_dev.h,
+// "Candidate lists").  One cell's list follows the reference's candidate
+// order (src/v_compute.cc:307-431): its own block in storage order without
+// the particle itself, then worklist steps 0 -> 1 ... G-1 -> G, each with the
+// block's particles in storage order, then the end marker at depth G.
+#pragma once
+#include "compute_dev.h"
+
+namespace vgpu {
+
+// Walks the list of sorted particle `cell` at depth G.  With FILL false it
+// only counts entries; with FILL true it writes entry k at
+// LA[k * stride], LB[k * stride].  Returns the number of entries.
+template<bool FILL>
+VG_FN int gather_cell_list(const RectView &v, const int *__restrict__ cellblk, int cell, int G, ListEntry *LA,
+			   ListTag *LB, int stride) {
+	const int ijk = cellblk[cell];
+	const int b0 = v.bstart[ijk], co0 = v.bstart[ijk + 1] - b0, s = cell - b0;
+	const dbl4 me = v.P[cell];
+	const double x = me.x, y = me.y, z = me.z;
+	const int ci = ijk % v.nx, cj = (ijk / v.nx) % v.ny, ck = ijk / v.nxy;
+	const int i = v.xper ? v.nx : ci, j = v.yper ? v.ny : cj, k = v.zper ? v.nz : ck;
+	const int disp = ijk - i - v.nx * (j + v.ny * k);
+	int n = 0;
+	// own block (src/v_compute.cc:307-323)
+	for (int t = 0; t < co0; t++) {
+		if (t == s) continue;
+		if (FILL) {
+			const dbl4 o = v.P[b0 + t];
+			LA[(size_t)n * stride] = ListEntry{o.x - x, o.y - y, o.z - z, o.w};
+			LB[(size_t)n * stride] = ListTag{v.pid[b0 + t], LK_OWN};
+		}
+		n++;
+	}
+	// worklist selection (src/v_compute.cc:328-362)
+	const double fx = x - v.ax - v.boxx * ci, fy = y - v.ay - v.boxy * cj, fz = z - v.az - v.boxz * ck;
+	double gxs, gys, gzs;
+	unsigned m1, m2;
+	int di = int(fx * v.xsp * VGPU_WL_FGRID), dj = int(fy * v.ysp * VGPU_WL_FGRID), dk = int(fz * v.zsp * VGPU_WL_FGRID);
+	if (di >= VGPU_WL_HGRID) {
+		gxs = fx;
+		m1 = 127 + (3 << 21); m2 = 1 + (1 << 21); di = VGPU_WL_FGRID - 1 - di; if (di < 0) di = 0;
+	} else { m1 = m2 = 0; gxs = v.boxx - fx; }
+	if (dj >= VGPU_WL_HGRID) {
+		gys = fy;
+		m1 |= (127 << 7) + (3 << 24); m2 |= (1 << 7) + (1 << 24); dj = VGPU_WL_FGRID - 1 - dj; if (dj < 0) dj = 0;
+	} else gys = v.boxy - fy;
+	if (dk >= VGPU_WL_HGRID) {
+		gzs = fz;
+		m1 |= (127 << 14) + (3 << 27); m2 |= (1 << 14) + (1 << 27); dk = VGPU_WL_FGRID - 1 - dk; if (dk < 0) dk = 0;
+	} else gzs = v.boxz - fz;
+	gxs *= gxs; gys *= gys; gzs *= gzs;
+	const int wli = di + VGPU_WL_HGRID * (dj + VGPU_WL_HGRID * dk);
+	const double *radp = v.mrad + wli * 64;
+	const unsigned *e = v.wl + wli * 64;
+	for (int g = 0; g < G; g++) {
+		unsigned q = e[g + 1];
+		q ^= m1; q += m2;
+		int ddi = q & 127; ddi -= 64;
+		int ddj = (q >> 7) & 127; ddj -= 64;
+		int ddk = (q >> 14) & 127; ddk -= 64;
+		const int ei = ddi + i, ej = ddj + j, ek = ddk + k;
+		if (ei < 0 || ei >= v.hx || ej < 0 || ej >= v.hy || ek < 0 || ek >= v.hz) {
+			if (FILL) {
+				LA[(size_t)n * stride] = ListEntry{radp[g], 0, 0, 0};
+				LB[(size_t)n * stride] = ListTag{0, LK_NOBLOCK};
+			}
+			n++;
+			continue;
+		}
+		double qx = 0, qy = 0, qz = 0;
+		const int rijk = rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
+		const int r0 = v.bstart[rijk], cnt = v.bstart[rijk + 1] - r0;
+		if (FILL) {
+			double cn, cf;
+			int form;
+			min_max_radius_pair(v, ddi, ddj, ddk, fx, fy, fz, gxs, gys, gzs, cn, cf, form);
+			LA[(size_t)n * stride] = ListEntry{radp[g], cn, cf, 0};
+			LB[(size_t)n * stride] = ListTag{cnt, LK_BLOCK};
+			const double x2 = x - qx, y2 = y - qy, z2 = z - qz;
+			for (int t = 0; t < cnt; t++) {
+				const dbl4 o = v.P[r0 + t];
+				LA[(size_t)(n + 1 + t) * stride] = ListEntry{o.x - x2, o.y - y2, o.z - z2, o.w};
+				LB[(size_t)(n + 1 + t) * stride] = ListTag{v.pid[r0 + t], LK_PART};
+			}
+		}
+		n += 1 + cnt;
+	}
+	if (FILL) {
+		LA[(size_t)n * stride] = ListEntry{radp[G], 0, 0, 0};
+		LB[(size_t)n * stride] = ListTag{0, LK_END};
+	}
+	return n + 1;
+}
+
+}  // namespace vgpu

@@ -11,6 +11,7 @@
 
 #include "../../include/voro_gpu.h"
 #include "grid_host.h"
+#include "learn.h"
 #include "vgpu_bin.h"
 
 using namespace vgpu;
@@ -88,6 +89,8 @@ struct vgpu_container {
 	int fast_waves = 0;
 	int fast_occ = 2;
 	hipEvent_t ev[5];
+	hipEvent_t ev_list = nullptr;   // learned tier: lists gathered, cell kernel starts
+	double gather_ms = 0;
 	bool events = false;
 	double timings[4] = {0, 0, 0, 0};
 
@@ -95,8 +98,14 @@ struct vgpu_container {
 	DBuf blk, slot, rem, cnt, bstart, tmp, part, P, pid, src, cellblk;
 	DBuf d_mrad, d_wl, d_walls, d_maxr, d_counters;
 	DBuf vol, cent, nfaces, status, nbr_start, arena, csr_off, csr, ctr;
-	DBuf retry1, retry2;
+	DBuf retry0, retry1, retry2;
 	DBuf st_planes, st_exit, st_tier;
+	// learned path
+	bool have_pred = false;
+	int pred_with_r = 0;
+	float pred_margin = 0;
+	DBuf d_pred, d_feat, d_cfeat, d_depth, d_cost, d_order, d_hist, lcnt, bmax, boff, list_a, list_b;
+	long long last_fallbacks = 0, last_entries = 0;
 	Workspace ws_fast, ws_wide, ws_huge;
 	long long nbr_total = 0;
 	bool csr_valid = false;
@@ -161,12 +170,22 @@ int launch_tier(vgpu_container *h, Workspace &ws, int waves, const RectView &v, 
 		ws.stamp_base = 0;
 	}
 	int *ctr = h->d_counters.as<int>();
+	int per_wave = 64;
+	if (list_count) {
+		// a retry list: spread its (few, expensive) cells over all wavefronts
+		int cnt = 0;
+		HIPCHK(hipMemcpyAsync(&cnt, list_count, 4, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipStreamSynchronize(s));
+		if (cnt == 0) return VGPU_OK;
+		per_wave = std::max(1, std::min(64, (cnt + waves - 1) / waves));
+		waves = std::max(1, std::min(waves, (cnt + per_wave - 1) / per_wave));
+	}
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 8, 0);
 	const bool poly = h->kind == VGPU_CONTAINER_POLY;
 	cells_launcher fn = std::is_same<T_, Tier3>::value ? launch_cells_fast3
 			    : (T_::MAXV == TierWide::MAXV ? launch_cells_wide : launch_cells_huge);
 	if (fn(poly, h->fast_occ, waves, s, v, ncells, list, list_count, h->cellblk.as<int>(), ctr, ws.base(), ws.slab,
-	       ws.stamp_base, out))
+	       ws.stamp_base, per_wave, out))
 		return VGPU_HIP_ERROR;
 	ws.stamp_base += (unsigned)ncells + 1u;
 	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
@@ -179,6 +198,67 @@ int default_fast_waves(int occ) {
 		if (hipGetDeviceProperties(&pr, dev) == hipSuccess && pr.multiProcessorCount > 0) cus = pr.multiProcessorCount;
 	}
 	return cus * 4 * occ;
+}
+
+// The learned tier: features, depth prediction, list gathering, the
+// list-driven cell kernel.  Cells it does not finish go to retry0.
+int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut out, hipStream_t s) {
+	using namespace vgpu::learn;
+	const int nxyz = h->g.nx * h->g.ny * h->g.nz;
+	int st;
+	int *ctr = h->ctr.as<int>();
+	if ((st = h->d_feat.ensure((size_t)nxyz * IN * 4))) return st;
+	if ((st = block_features_dev(h->g.nx, h->g.ny, h->g.nz, h->bstart.as<int>(), (float)nxyz / (float)nc,
+				     h->d_feat.as<float>(), s)))
+		return st;
+	const float *X = h->d_feat.as<float>();
+	int ns = nxyz;
+	if (h->pred_with_r) {
+		if ((st = h->d_cfeat.ensure((size_t)nc * IN * 4))) return st;
+		const float inv = h->max_radius > 0 ? (float)(1.0 / h->max_radius) : 0.f;
+		if ((st = cell_features_dev(nc, h->cellblk.as<int>(), h->P.as<double>() + 3, inv, h->d_feat.as<float>(),
+					    h->d_cfeat.as<float>(), s)))
+			return st;
+		X = h->d_cfeat.as<float>();
+		ns = nc;
+	}
+	if ((st = h->d_depth.ensure((size_t)ns)) || (st = h->d_cost.ensure((size_t)ns))) return st;
+	if ((st = mlp_depth_dev(h->d_pred.as<float>(), X, ns, h->pred_margin, h->d_depth.as<unsigned char>(),
+				h->d_cost.as<unsigned char>(), s)))
+		return st;
+	const int nb = nblk(nc, 64);
+	if ((st = h->lcnt.ensure((size_t)nc * 4)) || (st = h->bmax.ensure((size_t)(nb + 1) * 4)) ||
+	    (st = h->boff.ensure((size_t)(nb + 1) * 4)) || (st = h->d_order.ensure((size_t)nc * 4)) ||
+	    (st = h->d_hist.ensure(64 * 4)))
+		return st;
+	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, h->d_hist.as<int>(), 64, 0);
+	if (launch_list_order(s, nc, h->cellblk.as<int>(), h->d_cost.as<unsigned char>(), h->pred_with_r,
+			      h->d_hist.as<int>(), h->d_order.as<int>()))
+		return VGPU_HIP_ERROR;
+	if (launch_list_gather(s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(), h->d_depth.as<unsigned char>(),
+			       h->pred_with_r, h->lcnt.as<int>(), h->bmax.as<int>()))
+		return VGPU_HIP_ERROR;
+	if ((st = scan_excl(h, h->bmax.as<int>(), h->boff.as<int>(), nb + 1, s))) return st;
+	int rows = 0;
+	HIPCHK(hipMemcpyAsync(&rows, h->boff.as<int>() + nb, 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
+	const size_t ent = (size_t)std::max(rows, 1) * 64;
+	h->last_entries = (long long)rows * 64;
+	if ((st = h->list_a.ensure(ent * sizeof(ListEntry))) || (st = h->list_b.ensure(ent * sizeof(ListTag)))) return st;
+	if (launch_list_fill(s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(), h->d_depth.as<unsigned char>(),
+			     h->pred_with_r, h->boff.as<int>(), h->list_a.as<ListEntry>(), h->list_b.as<ListTag>()))
+		return VGPU_HIP_ERROR;
+	if (!h->ev_list) HIPCHK(hipEventCreate(&h->ev_list));
+	HIPCHK(hipEventRecord(h->ev_list, s));
+	if ((st = h->ws_fast.ensure(VCell3::L::bytes, waves))) return st;
+	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, h->d_counters.as<int>(), 8, 0);
+	out.tier = 0; out.retry = h->retry0.as<int>(); out.retry_count = ctr + 4;
+	if (launch_cells_list(h->kind == VGPU_CONTAINER_POLY, waves, s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(),
+			      h->lcnt.as<int>(),
+			      h->boff.as<int>(), h->list_a.as<ListEntry>(), h->list_b.as<ListTag>(), h->d_counters.as<int>(),
+			      h->ws_fast.base(), ctr + 5, out))
+		return VGPU_HIP_ERROR;
+	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
 }
 
 int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
@@ -244,13 +324,15 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 	long long acap = nbrs ? std::min<long long>(2147483000LL, 24LL * C1 + 4096) : 1;
 	if ((st = h->vol.ensure(C1 * 8)) || (st = h->cent.ensure(C1 * 24)) || (st = h->nfaces.ensure(C1 * 4)) ||
 	    (st = h->status.ensure(C1 * 4)) || (st = h->nbr_start.ensure(C1 * 4)) || (st = h->arena.ensure((size_t)acap * 4)) ||
-	    (st = h->ctr.ensure(16 * 4)) || (st = h->retry1.ensure(C1 * 4)) || (st = h->retry2.ensure(C1 * 4)))
+	    (st = h->ctr.ensure(16 * 4)) || (st = h->retry0.ensure(C1 * 4)) || (st = h->retry1.ensure(C1 * 4)) ||
+	    (st = h->retry2.ensure(C1 * 4)))
 		return st;
 	const bool stats = (flags & VGPU_F_STATS) != 0;
 	if (stats) {
 		if ((st = h->st_planes.ensure(C1 * 2)) || (st = h->st_exit.ensure(C1)) || (st = h->st_tier.ensure(C1))) return st;
 	}
-	int *ctr = h->ctr.as<int>();  // [0] arena top [1] retry1 count [2] retry2 count [3] failures
+	// [0] arena top [1..2] retry counts of tiers 1, 2 [3] failures [4] learned-tier retries [5] fallbacks
+	int *ctr = h->ctr.as<int>();
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 16, 0);
 	RectView v = make_view(h);
 	CellOut out;
@@ -264,16 +346,27 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 
 	int fw = h->fast_waves > 0 ? h->fast_waves : default_fast_waves(h->fast_occ);
 	fw = std::max(1, std::min(fw, nblk(std::max(nc, 1), 64)));
-	if (nc > 0) {
+	const bool learned = (flags & VGPU_F_LEARNED) != 0;
+	h->last_fallbacks = 0;
+	h->last_entries = 0;
+	if (nc > 0 && learned) {
+		// tier 0: candidate lists cut at the predicted worklist depth
+		if ((st = run_learned(h, v, nc, fw, out, s))) return st;
+		HIPCHK(hipEventRecord(h->ev[2], s));
+		out.tier = 1; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
+		if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, h->retry0.as<int>(), ctr + 4, out, s))) return st;
+	} else if (nc > 0) {
 		out.tier = 0; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
 		if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, nullptr, nullptr, out, s))) return st;
 		HIPCHK(hipEventRecord(h->ev[2], s));
-		out.tier = 1; out.retry = h->retry2.as<int>(); out.retry_count = ctr + 2;
-		if ((st = launch_tier<TierWide>(h, h->ws_wide, 64, v, nc, h->retry1.as<int>(), ctr + 1, out, s))) return st;
-		out.tier = 2; out.retry = nullptr; out.retry_count = nullptr;
-		if ((st = launch_tier<TierHuge>(h, h->ws_huge, 4, v, nc, h->retry2.as<int>(), ctr + 2, out, s))) return st;
 	} else {
 		HIPCHK(hipEventRecord(h->ev[2], s));
+	}
+	if (nc > 0) {
+		out.tier = 2; out.retry = h->retry2.as<int>(); out.retry_count = ctr + 2;
+		if ((st = launch_tier<TierWide>(h, h->ws_wide, 64, v, nc, h->retry1.as<int>(), ctr + 1, out, s))) return st;
+		out.tier = 3; out.retry = nullptr; out.retry_count = nullptr;
+		if ((st = launch_tier<TierHuge>(h, h->ws_huge, 4, v, nc, h->retry2.as<int>(), ctr + 2, out, s))) return st;
 	}
 	HIPCHK(hipEventRecord(h->ev[3], s));
 	h->csr_valid = false;
@@ -291,8 +384,8 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 		h->csr_valid = true;
 	}
 	HIPCHK(hipEventRecord(h->ev[4], s));
-	int hc[4];
-	HIPCHK(hipMemcpyAsync(hc, ctr, 16, hipMemcpyDeviceToHost, s));
+	int hc[8];
+	HIPCHK(hipMemcpyAsync(hc, ctr, 32, hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
 	float ms;
 	for (int t = 0; t < 4; t++) {
@@ -300,6 +393,12 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 		h->timings[t] = ms;
 	}
 	h->nbr_total = hc[0];
+	h->gather_ms = 0;
+	if (learned && nc > 0) {
+		h->last_fallbacks = hc[5];
+		HIPCHK(hipEventElapsedTime(&ms, h->ev[1], h->ev_list));
+		h->gather_ms = ms;
+	}
 	h->computed = true;
 	h->last_flags = flags;
 	if (hc[3] > 0) return VGPU_MEMORY_ERROR;
@@ -358,6 +457,7 @@ int vgpu_destroy(vgpu_container *h) {
 	for (DBuf *b : bufs) b->release();
 	h->ws_fast.buf.release(); h->ws_wide.buf.release(); h->ws_huge.buf.release();
 	if (h->events) for (int t = 0; t < 5; t++) (void)hipEventDestroy(h->ev[t]);
+	if (h->ev_list) (void)hipEventDestroy(h->ev_list);
 	delete h;
 	return VGPU_OK;
 }
@@ -433,6 +533,7 @@ int vgpu_clear(vgpu_container *h) {
 }
 
 int vgpu_compute_all(vgpu_container *h, unsigned flags, void *stream) {
+	if (h && (flags & VGPU_F_LEARNED) && !h->have_pred) return VGPU_INVALID_ARGUMENT;
 	if (!h) return VGPU_INVALID_ARGUMENT;
 	return run_pipeline(h, flags, (hipStream_t)stream);
 }
@@ -559,6 +660,103 @@ int vgpu_set_launch(vgpu_container *h, int fast_waves, int fast_occupancy) {
 int vgpu_last_timings(vgpu_container *h, double *ms4) {
 	if (!h || !ms4) return VGPU_INVALID_ARGUMENT;
 	for (int t = 0; t < 4; t++) ms4[t] = h->timings[t];
+	return VGPU_OK;
+}
+
+int vgpu_set_predictor(vgpu_container *h, vgpu_predictor *p) {
+	if (!h) return VGPU_INVALID_ARGUMENT;
+	if (!p) { h->have_pred = false; return VGPU_OK; }
+	if (!p->trained || p->params.size() != (size_t)vgpu::learn::NP) return VGPU_INVALID_ARGUMENT;
+	if (p->with_radius && h->kind != VGPU_CONTAINER_POLY) return VGPU_INVALID_ARGUMENT;
+	int st = h->d_pred.ensure(sizeof(float) * vgpu::learn::NP);
+	if (st) return st;
+	HIPCHK(hipMemcpy(h->d_pred.p, p->params.data(), sizeof(float) * vgpu::learn::NP, hipMemcpyHostToDevice));
+	h->pred_margin = p->margin;
+	h->pred_with_r = p->with_radius;
+	h->have_pred = true;
+	return VGPU_OK;
+}
+
+int vgpu_learned_stats(vgpu_container *h, long long *fallbacks, long long *list_entries, double *gather_ms) {
+	if (!h) return VGPU_INVALID_ARGUMENT;
+	if (gather_ms) *gather_ms = h->gather_ms;
+	if (fallbacks) *fallbacks = h->last_fallbacks;
+	if (list_entries) *list_entries = h->last_entries;
+	return VGPU_OK;
+}
+
+// Training samples from the last vgpu_compute_all (run with VGPU_F_STATS):
+// per block (or per cell, for a predictor that takes the radius) the
+// features and the largest worklist exit and nplane count.
+int vgpu_predictor_add_samples(vgpu_predictor *p, vgpu_container *h) {
+	using namespace vgpu::learn;
+	if (!p || !h || !h->computed || !(h->last_flags & VGPU_F_STATS)) return VGPU_INVALID_ARGUMENT;
+	if (p->with_radius && h->kind != VGPU_CONTAINER_POLY) return VGPU_INVALID_ARGUMENT;
+	const int nxyz = h->g.nx * h->g.ny * h->g.nz, nc = h->ncells;
+	if (nc <= 0) return VGPU_OK;
+	hipStream_t s = nullptr;
+	int st;
+	if ((st = h->d_feat.ensure((size_t)nxyz * IN * 4))) return st;
+	if ((st = block_features_dev(h->g.nx, h->g.ny, h->g.nz, h->bstart.as<int>(), (float)nxyz / (float)nc,
+				     h->d_feat.as<float>(), s)))
+		return st;
+	std::vector<float> X;
+	std::vector<int> lab;
+	int ns;
+	if (p->with_radius) {
+		if ((st = h->d_cfeat.ensure((size_t)nc * IN * 4))) return st;
+		const float inv = h->max_radius > 0 ? (float)(1.0 / h->max_radius) : 0.f;
+		if ((st = cell_features_dev(nc, h->cellblk.as<int>(), h->P.as<double>() + 3, inv, h->d_feat.as<float>(),
+					    h->d_cfeat.as<float>(), s)))
+			return st;
+		ns = nc;
+		X.resize((size_t)nc * IN);
+		HIPCHK(hipMemcpy(X.data(), h->d_cfeat.p, X.size() * 4, hipMemcpyDeviceToHost));
+		std::vector<signed char> ex(nc);
+		std::vector<short> pl(nc);
+		HIPCHK(hipMemcpy(ex.data(), h->st_exit.p, (size_t)nc, hipMemcpyDeviceToHost));
+		HIPCHK(hipMemcpy(pl.data(), h->st_planes.p, (size_t)nc * 2, hipMemcpyDeviceToHost));
+		lab.resize((size_t)2 * nc);
+		for (int c = 0; c < nc; c++) { lab[2 * c] = ex[c]; lab[2 * c + 1] = pl[c]; }
+	} else {
+		DBuf d_lab;
+		if ((st = d_lab.ensure((size_t)nxyz * 8))) return st;
+		if ((st = block_labels_dev(nc, h->cellblk.as<int>(), h->st_exit.as<signed char>(), h->st_planes.as<short>(),
+					   nxyz, d_lab.as<int>(), s))) {
+			d_lab.release();
+			return st;
+		}
+		X.resize((size_t)nxyz * IN);
+		lab.resize((size_t)2 * nxyz);
+		std::vector<int> bs(nxyz + 1);
+		bool ok = hipMemcpy(X.data(), h->d_feat.p, X.size() * 4, hipMemcpyDeviceToHost) == hipSuccess &&
+			  hipMemcpy(lab.data(), d_lab.p, lab.size() * 4, hipMemcpyDeviceToHost) == hipSuccess &&
+			  hipMemcpy(bs.data(), h->bstart.p, bs.size() * 4, hipMemcpyDeviceToHost) == hipSuccess;
+		d_lab.release();
+		if (!ok) return VGPU_HIP_ERROR;
+		// only blocks that hold cells carry a label
+		int m = 0;
+		for (int b = 0; b < nxyz; b++) {
+			if (bs[b + 1] == bs[b]) continue;
+			std::memmove(&X[(size_t)m * IN], &X[(size_t)b * IN], IN * 4);
+			lab[2 * m] = lab[2 * b];
+			lab[2 * m + 1] = lab[2 * b + 1];
+			m++;
+		}
+		ns = m;
+	}
+	for (int t = 0; t < ns; t++) {
+		p->X.insert(p->X.end(), &X[(size_t)t * IN], &X[(size_t)t * IN] + IN);
+		p->T.push_back((float)lab[2 * t] / 64.f);
+		p->T.push_back(std::log1p((float)lab[2 * t + 1]) / 8.f);
+	}
+	return VGPU_OK;
+}
+
+int vgpu_predictor_clear_samples(vgpu_predictor *p) {
+	if (!p) return VGPU_INVALID_ARGUMENT;
+	p->X.clear();
+	p->T.clear();
 	return VGPU_OK;
 }
 

@@ -43,11 +43,14 @@ template<> struct TierCell<Tier3> { typedef VCell3 type; };
 // contiguous share per XCD (workgroups are dispatched to the 8 XCDs round
 // robin) so that neighbouring cells, which read the same particle blocks,
 // share an L2; a wavefront whose share is exhausted moves to the next one.
+// per_wave (1..64) is the number of cells a wavefront takes at a time: 64
+// for a full tier, fewer for a short retry list of expensive cells, which
+// then spreads over all wavefronts instead of a few full ones.
 template<class T_, bool POLY, int OCC>
 __global__ void __launch_bounds__(64, OCC)
 k_cells(RectView v, int ncells, const int *__restrict__ list, const int *__restrict__ list_count,
 	const int *__restrict__ cellblk, int *__restrict__ counters, char *__restrict__ ws, size_t slab_bytes,
-	unsigned stamp_base, CellOut out) {
+	unsigned stamp_base, int per_wave, CellOut out) {
 	typedef typename TierCell<T_>::type CELL;
 	const int lane = threadIdx.x;
 	CELL c;
@@ -62,11 +65,11 @@ k_cells(RectView v, int ncells, const int *__restrict__ list, const int *__restr
 		const int lo = sh * share, hi = min(total, lo + share);
 		for (;;) {
 			int base = 0;
-			if (lane == 0) base = atomicAdd(&counters[sh], 64);
+			if (lane == 0) base = atomicAdd(&counters[sh], per_wave);
 			base = __shfl(base, 0);
 			if (lo + base >= hi) break;
 			const int idx = lo + base + lane;
-			if (idx < hi) {
+			if (idx < hi && lane < per_wave) {
 				const int cell = list ? list[idx] : idx;
 				const int b = cellblk[cell];
 				const int s = cell - v.bstart[b];
@@ -118,19 +121,30 @@ k_cells(RectView v, int ncells, const int *__restrict__ list, const int *__restr
 // the kernel was compiled for (2, 3 or 4 for the fast tier).
 typedef int (*cells_launcher)(bool poly, int occ, int waves, hipStream_t s, const RectView &v, int ncells,
 			      const int *list, const int *list_count, const int *cellblk, int *counters, char *ws,
-			      size_t slab, unsigned stamp_base, const CellOut &out);
+			      size_t slab, unsigned stamp_base, int per_wave, const CellOut &out);
 int launch_cells_fast3(bool poly, int occ, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
 		       const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
-		       unsigned stamp_base, const CellOut &out);
+		       unsigned stamp_base, int per_wave, const CellOut &out);
 int launch_cells_wide(bool poly, int occ, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
 		      const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
-		      unsigned stamp_base, const CellOut &out);
+		      unsigned stamp_base, int per_wave, const CellOut &out);
 int launch_cells_huge(bool poly, int occ, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
 		      const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
-		      unsigned stamp_base, const CellOut &out);
+		      unsigned stamp_base, int per_wave, const CellOut &out);
+
+// Learned path (cells_list.hip): list gathering and the list-driven tier.
+int launch_list_order(hipStream_t s, int nc, const int *cellblk, const unsigned char *cost, int per_cell, int *hist,
+		      int *order);
+int launch_list_gather(hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
+		       const unsigned char *depth, int per_cell, int *cnt, int *bmax);
+int launch_list_fill(hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
+		     const unsigned char *depth, int per_cell, const int *boff, ListEntry *LA, ListTag *LB);
+int launch_cells_list(bool poly, int waves, hipStream_t s, const RectView &v, int nc, const int *cellblk,
+		      const int *order, const int *cnt, const int *boff, const ListEntry *LA, const ListTag *LB,
+		      int *counters, char *ws, int *fallbacks, const CellOut &out);
 
 #define VGPU_CELLS_LAUNCH(TIER, P, O)                                                                      \
 	hipLaunchKernelGGL((k_cells<TIER, P, O>), dim3(waves), dim3(64), 0, s, v, ncells, list, list_count,    \
-			   cellblk, counters, ws, slab, stamp_base, out)
+			   cellblk, counters, ws, slab, stamp_base, per_wave, out)
 
 }  // namespace vgpu
