@@ -3,8 +3,9 @@
 //
 // Mirrors the reference classes a program uses to compute cells:
 //   container / container_poly        src/container.hh:288-728
+//   container_periodic / _poly        src/container_prd.hh:213-651
 //   wall_sphere                       src/wall.hh:27-30
-//   c_loop_all                        src/c_loops.hh:165-209
+//   c_loop_all / c_loop_all_periodic  src/c_loops.hh:165-209, 325-393
 //   voronoicell_neighbor outputs      src/cell.hh:432-549
 // A reference program that does
 //     container con(...); con.put(i,x,y,z); ...
@@ -135,6 +136,46 @@ class container_poly : public container_base_gpu {
 	struct g3 { int v[3]; g3(int a, int b, int c) : v{a, b, c} {} };
 };
 
+// container_periodic(bx,bxy,by,bxz,byz,bz,nx,ny,nz,init_mem),
+// src/container_prd.hh:213-217
+class container_periodic : public container_base_gpu {
+    public:
+	container_periodic(double bx, double bxy, double by, double bxz, double byz, double bz, int nx, int ny, int nz,
+			   int init_mem = 8)
+		: container_base_gpu(VGPU_CONTAINER_PERIODIC, g6(bx, bxy, by, bxz, byz, bz).v, g3(nx, ny, nz).v, nullptr) {
+		(void)init_mem;
+	}
+	void put(int n, double x, double y, double z) {
+		double p[3] = {x, y, z};
+		check(vgpu_put(h, 1, &n, p, nullptr), "put");
+		dirty = true;
+	}
+
+    private:
+	struct g6 { double v[6]; g6(double a, double b, double c, double d, double e, double f) : v{a, b, c, d, e, f} {} };
+	struct g3 { int v[3]; g3(int a, int b, int c) : v{a, b, c} {} };
+};
+
+// container_periodic_poly, src/container_prd.hh:434-441
+class container_periodic_poly : public container_base_gpu {
+    public:
+	container_periodic_poly(double bx, double bxy, double by, double bxz, double byz, double bz, int nx, int ny,
+				int nz, int init_mem = 8)
+		: container_base_gpu(VGPU_CONTAINER_PERIODIC_POLY, g6(bx, bxy, by, bxz, byz, bz).v, g3(nx, ny, nz).v,
+				     nullptr) {
+		(void)init_mem;
+	}
+	void put(int n, double x, double y, double z, double r) {
+		double p[3] = {x, y, z};
+		check(vgpu_put(h, 1, &n, p, &r), "put");
+		dirty = true;
+	}
+
+    private:
+	struct g6 { double v[6]; g6(double a, double b, double c, double d, double e, double f) : v{a, b, c, d, e, f} {} };
+	struct g3 { int v[3]; g3(int a, int b, int c) : v{a, b, c} {} };
+};
+
 // c_loop_all, src/c_loops.hh:165-209: blocks by ascending index, then
 // storage order.
 class c_loop_all {
@@ -152,6 +193,11 @@ class c_loop_all {
 	long long m = 0;
 	friend class container_base_gpu;
 };
+
+// c_loop_all_periodic, src/c_loops.hh:325-393: the primary domain's
+// blocks by ascending index, then storage order -- the order the engine
+// stores a periodic container's cells in.
+typedef c_loop_all c_loop_all_periodic;
 
 // The outputs of voronoicell_neighbor that the engine produces.
 class voronoicell_neighbor {

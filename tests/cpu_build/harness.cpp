@@ -12,6 +12,7 @@
 #include "../../voro_amd/csrc/compute_dev.h"
 #include "../../voro_amd/csrc/grid_host.h"
 #include "../../voro_amd/csrc/list_dev.h"
+#include "../../voro_amd/csrc/prd_host.h"
 
 using namespace vgpu;
 
@@ -37,10 +38,10 @@ struct Runner {
 	CELL c;
 	unsigned stamp = 0;
 	explicit Runner(double mls) : slab(BYTES + 2 * 65536, 0) { c.setup(slab.data() + 65536, 0, mls); }
-	template<bool POLY>
+	template<bool POLY, bool PRD = false>
 	int run(const RectView &v, int ijk, int s, int ci, int cj, int ck, double *vol, double *cent, int *nf,
-		std::vector<int> &nb, CellStats &st) {
-		bool ok = compute_cell_rect<CELL, POLY>(c, v, ijk, s, ci, cj, ck, ++stamp, st);
+		std::vector<int> &nb, CellStats &st, unsigned long long key0 = 0) {
+		bool ok = compute_cell_rect<CELL, POLY, PRD>(c, v, ijk, s, ci, cj, ck, ++stamp, st, key0);
 		if (c.ovf) return c.fatal ? -2 : -1;
 		if (ok) {
 			VecSink sk{&nb};
@@ -88,13 +89,25 @@ static long long list_fallbacks = 0;
 void hx_use_list(int G) { list_depth = G; list_fallbacks = 0; }
 long long hx_list_fallbacks(void) { return list_fallbacks; }
 
+static long long prd_passes = 0;
+static int prd_replay = 1;
+// Passes over the cells of the last periodic computation (1 + replays).
+long long hx_image_passes(void) { return prd_passes; }
+// 0: keep the first pass (image blocks in create_all_images order), to show
+// what the replay of the first-visit order changes.
+void hx_set_replay(int on) { prd_replay = on; }
+
+static long long compute_prd(const hx_spec *s, int n, const int *ids, const double *xyz, const double *r,
+			     hx_out *o, int *stats_out, long long *tier_counts);
+
 // tier_counts[3]: cells finished by each tier.  Returns cells, or -1 on a
 // cell no tier could finish.
+
 long long hx_compute_neighbor(const hx_spec *s, int n, const int *ids, const double *xyz, const double *r,
 			      hx_out *o, int *stats_out /* 4 ints per cell or null */, long long *tier_counts) {
-	if (s->kind > 1) return -2;
+	if (s->kind > 1) return compute_prd(s, n, ids, xyz, r, o, stats_out, tier_counts);
 	const bool poly = s->kind == 1;
-	RectView v;
+	RectView v = RectView();
 	v.ax = s->geom[0]; v.bx = s->geom[1]; v.ay = s->geom[2]; v.by = s->geom[3]; v.az = s->geom[4]; v.bz = s->geom[5];
 	v.nx = s->grid[0]; v.ny = s->grid[1]; v.nz = s->grid[2]; v.nxy = v.nx * v.ny;
 	v.xper = s->periodic[0]; v.yper = s->periodic[1]; v.zper = s->periodic[2];
@@ -132,7 +145,8 @@ long long hx_compute_neighbor(const hx_spec *s, int n, const int *ids, const dou
 	std::vector<double> mrad(4096), walls;
 	worklist_radii(v.boxx, v.boxy, v.boxz, mrad.data());
 	for (int w = 0; w < s->n_spheres; w++) for (int t = 0; t < 5; t++) walls.push_back(s->spheres[5 * w + t]);
-	v.P = P.data(); v.pid = pid.data(); v.bstart = cnt.data(); v.mrad = mrad.data(); v.wl = vgpu_worklist_table;
+	v.gy = v.ny;
+	v.P = P.data(); v.pid = pid.data(); v.bstart = cnt.data(); v.cstart = cnt.data(); v.mrad = mrad.data(); v.wl = vgpu_worklist_table;
 	v.walls = walls.data(); v.nwalls = s->n_spheres; v.max_radius = maxr;
 
 	std::vector<int> cellblk(m);
@@ -191,3 +205,140 @@ long long hx_compute_neighbor(const hx_spec *s, int n, const int *ids, const dou
 }
 
 }
+
+// Periodic containers: the engine's algorithm (binning as put_locate_block,
+// image blocks first in create_all_images order, then replayed in the
+// recorded first-visit order until that order reproduces itself), one cell
+// at a time.  Returns cells, -1 on a cell no tier could finish, -3 on a
+// duplicate particle, -4 when the unit cell fails, -5 when the first-visit
+// order does not settle.
+static long long compute_prd(const hx_spec *s, int n, const int *ids, const double *xyz, const double *r,
+			     hx_out *o, int *stats_out, long long *tier_counts) {
+	const bool poly = s->kind == 3;
+	PrdGeom g;
+	if (!g.build(s->geom[0], s->geom[1], s->geom[2], s->geom[3], s->geom[4], s->geom[5], s->grid[0], s->grid[1],
+		     s->grid[2]))
+		return -4;
+	const int nb = (int)g.nblocks();
+	// binning (src/container_prd.cc:189-217), stable by input order
+	std::vector<int> blk(n, -1), cnt(nb + 1, 0);
+	std::vector<dbl4> rem(n);
+	double maxr = 0;
+	for (int t = 0; t < n; t++) {
+		double x = xyz[3 * t], y = xyz[3 * t + 1], z = xyz[3 * t + 2];
+		int k = step_int(z * g.zsp);
+		if (k < 0 || k >= g.nz) { int ak = prd_step_div(k, g.nz); z -= ak * g.bz; y -= ak * g.byz; x -= ak * g.bxz; k -= ak * g.nz; }
+		int j = step_int(y * g.ysp);
+		if (j < 0 || j >= g.ny) { int aj = prd_step_div(j, g.ny); y -= aj * g.by; x -= aj * g.bxy; j -= aj * g.ny; }
+		int i = step_int(x * g.xsp);
+		if (i < 0 || i >= g.nx) { int ai = prd_step_div(i, g.nx); x -= ai * g.bx; i -= ai * g.nx; }
+		j += g.ey; k += g.ez;
+		blk[t] = i + g.nx * (j + g.oy * k);
+		double rr = r ? r[t] : 0.5;
+		rem[t] = dbl4{x, y, z, rr};
+		if (poly && maxr < rr) maxr = rr;
+		cnt[blk[t] + 1]++;
+	}
+	for (int b = 0; b < nb; b++) cnt[b + 1] += cnt[b];
+	std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+	const int m = cnt[nb];
+	std::vector<dbl4> P(std::max(m, 1));
+	std::vector<int> pid(std::max(m, 1)), cellblk(std::max(m, 1));
+	for (int t = 0; t < n; t++) {
+		int d = fill[blk[t]]++;
+		P[d] = rem[t]; pid[d] = ids[t]; cellblk[d] = blk[t];
+	}
+	for (int d = 0; d < m; d++)
+		for (int u = cnt[cellblk[d]]; u < d; u++) {
+			double dx = P[u].x - P[d].x, dy = P[u].y - P[d].y, dz = P[u].z - P[d].z;
+			if (dx * dx + dy * dy + dz * dz < 1e-10) return -3;
+		}
+	std::vector<double> mrad(4096);
+	worklist_radii(g.boxx, g.boxy, g.boxz, mrad.data());
+	RectView v = RectView();
+	v.ax = v.ay = v.az = 0; v.bx = g.bx; v.by = g.by; v.bz = g.bz;
+	v.nx = g.nx; v.ny = g.ny; v.nz = g.nz; v.gy = g.oy; v.nxy = g.nx * g.oy;
+	v.boxx = g.boxx; v.boxy = g.boxy; v.boxz = g.boxz; v.xsp = g.xsp; v.ysp = g.ysp; v.zsp = g.zsp;
+	v.hx = 2 * g.nx + 1; v.hy = 2 * g.ey + 1; v.hz = 2 * g.ez + 1;
+	v.bxsq = g.boxx * g.boxx + g.boxy * g.boxy + g.boxz * g.boxz;
+	v.max_len_sq = g.max_len_sq; v.max_radius = maxr;
+	v.cstart = cnt.data(); v.mrad = mrad.data(); v.wl = vgpu_worklist_table; v.nwalls = 0;
+	v.bxy = g.bxy; v.bxz = g.bxz; v.byz = g.byz;
+	v.ey = g.ey; v.ez = g.ez; v.wy = g.wy; v.wz = g.wz; v.oy = g.oy; v.oz = g.oz;
+	v.unit = g.unit.data(); v.unit_p = (int)g.unit.size();
+	std::vector<unsigned long long> fv(nb, ~0ull), prev;
+	v.fv = fv.data();
+
+	Runner<VCell3, Layout3<>::lane_bytes> r0(v.max_len_sq);
+	Runner<VCell<TierWide, 1>, Layout<TierWide, 1>::bytes> r1(v.max_len_sq);
+	Runner<VCell<TierHuge, 1>, Layout<TierHuge, 1>::bytes> *r2 = 0;
+	struct Res { double vol, cent[3]; int nf, exists, tier; std::vector<int> nb; CellStats st; };
+	std::vector<Res> res(m);
+	std::vector<unsigned char> touched(m, 0);
+	bool bad = false;
+	auto one = [&](int cell) {
+		const int ijk = cellblk[cell], q = cell - cnt[ijk];
+		const int ci = ijk % g.nx, cj = (ijk / g.nx) % g.oy, ck = ijk / (g.nx * g.oy);
+		Res &R = res[cell];
+		R.vol = 0; R.cent[0] = R.cent[1] = R.cent[2] = 0; R.nf = 0; R.nb.clear(); R.tier = 0;
+		const unsigned long long key = (unsigned long long)cell << 24;
+		int rr = poly ? r0.run<true, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key)
+			      : r0.run<false, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key);
+		if (rr < 0) {
+			R.tier = 1; R.nb.clear();
+			rr = poly ? r1.run<true, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key)
+				  : r1.run<false, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key);
+		}
+		if (rr < 0) {
+			R.tier = 2; R.nb.clear();
+			if (!r2) r2 = new Runner<VCell<TierHuge, 1>, Layout<TierHuge, 1>::bytes>(v.max_len_sq);
+			rr = poly ? r2->run<true, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key)
+				  : r2->run<false, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key);
+		}
+		if (rr < 0) { bad = true; rr = 0; }
+		R.exists = rr;
+		if (R.st.img) touched[cell] = 1;
+	};
+	std::vector<dbl4> Pa;
+	std::vector<int> ia, ba;
+	{
+		ImageSet im(g, P.data(), pid.data(), cnt.data());
+		for (int b : prd_all_images(g)) im.create_block(b);
+		im.assemble(Pa, ia, ba);
+		v.P = Pa.data(); v.pid = ia.data(); v.bstart = ba.data();
+	}
+	for (int c = 0; c < m; c++) one(c);
+	prd_passes = 1;
+	bool settled = false;
+	if (!prd_replay) settled = true;
+	for (int pass = 0; pass < 8 && !settled; pass++) {
+		prev = fv;
+		ImageSet im(g, P.data(), pid.data(), cnt.data());
+		for (int b : prd_visit_order(g, prev.data())) im.create_block(b);
+		if (im.late) { delete r2; return -5; }
+		im.assemble(Pa, ia, ba);
+		v.P = Pa.data(); v.pid = ia.data(); v.bstart = ba.data();
+		std::fill(fv.begin(), fv.end(), ~0ull);
+		for (int c = 0; c < m; c++) if (touched[c]) one(c);
+		prd_passes = pass + 2;
+		settled = fv == prev;
+	}
+	delete r2;
+	if (!settled) return -5;
+	long long nn = 0;
+	for (int c = 0; c < m; c++) {
+		const Res &R = res[c];
+		const int ijk = cellblk[c];
+		if (tier_counts) tier_counts[R.tier]++;
+		if (stats_out) { stats_out[4 * c] = R.st.n_planes; stats_out[4 * c + 1] = R.st.wl_exit; stats_out[4 * c + 2] = R.st.phase; stats_out[4 * c + 3] = R.tier; }
+		o->id[c] = pid[c]; o->ijk[c] = ijk; o->q[c] = c - cnt[ijk]; o->exists[c] = R.exists;
+		if (o->pos) { o->pos[3 * c] = P[c].x; o->pos[3 * c + 1] = P[c].y; o->pos[3 * c + 2] = P[c].z; }
+		o->vol[c] = R.vol; o->cent[3 * c] = R.cent[0]; o->cent[3 * c + 1] = R.cent[1]; o->cent[3 * c + 2] = R.cent[2];
+		o->nfaces[c] = R.exists ? R.nf : 0;
+		o->nbr_off[c] = nn;
+		if (R.exists) for (size_t t = 0; t < R.nb.size(); t++) { if (nn < o->nbr_cap) o->nbr[nn] = R.nb[t]; nn++; }
+	}
+	o->nbr_off[m] = nn;
+	return bad ? -1 : m;
+}
+

@@ -10,6 +10,10 @@ Fixtures (numpy .npz, no pickles):
       (id, ijk, q, exists, vol, cent, nfaces, nbr_off, nbr).
   poly_small.npz           -- 2,000 particles, container_poly [0,10]^3, 8^3 blocks.
   wall_small.npz           -- perturbed FCC in a sphere wall, ~2,000 particles.
+  prd_poly_small.npz       -- container_periodic_poly(60,5,60,3,4,60), 8^3 blocks,
+      3,000 particles drawn in [0,60)^3 (remapped by put), lognormal radii.
+  prd_tri_small.npz        -- container_periodic(10,1.5,10,0.7,-1.2,10), 5^3 blocks,
+      1,000 particles drawn in [-2,12)^3.
   worklist_sha256.txt      -- SHA-256 of the reference worklist table values.
 """
 import hashlib
@@ -55,6 +59,14 @@ def main():
     save("wall_small.npz", (0, (0, L, 0, L, 0, L), (g, g, g),
                             {"spheres": [[L / 2, L / 2, L / 2, 0.45 * L, -99]]}),
          np.arange(len(pts), dtype=np.int32), pts)
+    n = 3000
+    ids = np.arange(n, dtype=np.int32)
+    save("prd_poly_small.npz", (3, (60, 5, 60, 3, 4, 60), (8, 8, 8), {}), ids,
+         I.uniform_box(21, n, (0, 0, 0), (60, 60, 60)), I.radii_lognormal(21, n))
+    n = 1000
+    ids = np.arange(n, dtype=np.int32)
+    save("prd_tri_small.npz", (2, (10, 1.5, 10, 0.7, -1.2, 10), (5, 5, 5), {}), ids,
+         I.uniform_box(22, n, (-2, -2, -2), (12, 12, 12)))
     src = open(os.path.join(O.REF_ROOT, "src", "v_base_wl.cc")).read()
     body = src[src.index("{") + 1:src.rindex("}")]
     vals = [int(x, 0) for x in re.findall(r"0x[0-9a-f]+|\b\d+\b", body)]

@@ -5,8 +5,12 @@ import voro_amd
 
 
 def engine_run(kind, geom, grid, ids, xyz, r=None, periodic=(0, 0, 0), spheres=None, flags=None, stats=False):
-    cls = voro_amd.container_poly if kind == 1 else voro_amd.container
-    con = cls(*geom, *grid, *[bool(p) for p in periodic], 8)
+    if kind >= 2:
+        cls = voro_amd.container_periodic_poly if kind == 3 else voro_amd.container_periodic
+        con = cls(*geom, *grid, 8)
+    else:
+        cls = voro_amd.container_poly if kind == 1 else voro_amd.container
+        con = cls(*geom, *grid, *[bool(p) for p in periodic], 8)
     for s in spheres or []:
         con.add_wall(voro_amd.wall_sphere(*s))
     con.put_array(ids, xyz, r)

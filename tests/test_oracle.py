@@ -67,6 +67,20 @@ def _ref_available(O):
     return O.reference() is not None
 
 
+@pytest.mark.parametrize("name", ["prd_poly_small.npz", "prd_tri_small.npz"])
+def test_periodic_golden_fixtures_match_reference_library(oracle_mod, name):
+    """The periodic fixtures are the reference's own outputs (regenerated
+    live where the reference library is built)."""
+    O = oracle_mod
+    if not _ref_available(O):
+        pytest.skip("reference library not built here")
+    d = load(name)
+    out = O.run("reference", O.make_spec(int(d["spec_kind"]), d["spec_geom"], d["spec_grid"]), d["ids"], d["xyz"],
+                d.get("r"))
+    assert O.compare(out, d, vol_rtol=0.0) == []
+    assert np.array_equal(out["vol"], d["vol"]) and np.array_equal(out["cent"], d["cent"])
+
+
 @pytest.mark.parametrize("case", ["uniform", "clusters", "poly", "periodic_flags", "poly_periodic", "sc_lattice",
                                   "fcc_exact", "dropped"])
 def test_restatement_matches_reference_library(oracle_mod, case):

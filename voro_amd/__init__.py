@@ -1,7 +1,8 @@
 """voro_amd -- MI355X (gfx950) engine for Voro++-style per-particle Voronoi cells.
 
 Python mirror of the reference's container interface for the hot path
-(src/container.hh:288-728, src/c_loops.hh:165-209, src/cell.hh:432-549),
+(src/container.hh:288-728, src/container_prd.hh:213-651,
+src/c_loops.hh:165-209, 325-393, src/cell.hh:432-549),
 over the C ABI of include/voro_gpu.h (libvoro_gpu.so, built in-tree by
 voro_amd/build.py).  There is no CPU fallback: if the library is missing or
 no GPU is visible, construction raises.
@@ -275,6 +276,32 @@ class container_poly(_Base):
         self.put_array(np.array([n]), np.array([[x, y, z]]), np.array([r]))
 
 
+class container_periodic(_Base):
+    """container_periodic(bx,bxy,by,bxz,byz,bz,nx,ny,nz,init_mem),
+    src/container_prd.hh:213-217: a triclinic lattice periodic in all three
+    directions; put() remaps a particle into the primary domain
+    (src/container_prd.cc:189-217) and rejects one within 1e-5 of another in
+    its block, which surfaces as VoroGPUError(DUPLICATE) at compute time."""
+    kind = KIND_PERIODIC
+
+    def __init__(self, bx, bxy, by, bxz, byz, bz, nx, ny, nz, init_mem=8):
+        super().__init__((bx, bxy, by, bxz, byz, bz), (nx, ny, nz), (True, True, True), init_mem)
+
+    def put(self, n, x, y, z):
+        self.put_array(np.array([n]), np.array([[x, y, z]]))
+
+
+class container_periodic_poly(_Base):
+    """container_periodic_poly(...), src/container_prd.hh:434-441; put(n,x,y,z,r)."""
+    kind = KIND_PERIODIC_POLY
+
+    def __init__(self, bx, bxy, by, bxz, byz, bz, nx, ny, nz, init_mem=8):
+        super().__init__((bx, bxy, by, bxz, byz, bz), (nx, ny, nz), (True, True, True), init_mem)
+
+    def put(self, n, x, y, z, r):
+        self.put_array(np.array([n]), np.array([[x, y, z]]), np.array([r]))
+
+
 class wall_sphere:
     """wall_sphere(xc,yc,zc,rc,w_id=-99), src/wall.hh:27-30."""
 
@@ -311,6 +338,11 @@ class c_loop_all:
     @property
     def q(self):
         return int(self.con.results()["q"][self._m])
+
+
+class c_loop_all_periodic(c_loop_all):
+    """Loop over the particles of a periodic container: primary-domain
+    blocks by ascending index, then storage order (src/c_loops.hh:325-393)."""
 
 
 class voronoicell_neighbor:

@@ -210,6 +210,28 @@ struct VCell3 {
 		}
 	}
 
+	// The unit Voronoi cell of a periodic container as the start cell
+	// (src/container_prd.hh:124-130: c=unit_voro, which copies vertices and
+	// edges, zeroes the neighbour ids and sets up=0, src/cell.cc:71-95).  A
+	// vertex of another order than 3 leaves the cell to the general tier.
+	VG_FN void init_unit(const UnitVert *__restrict__ u, int np) {
+		ovf = false; fatal = false; n_cuts = 0;
+		up = 0;
+		if (np > MAXV) { ovf = true; return; }
+		p = np;
+		for (int v = 0; v < np; v++) {
+			const UnitVert a = u[v];
+			if (a.nu != 3) { ovf = true; return; }
+			setXYZ(v, a.x, a.y, a.z);
+			unsigned long long w = (unsigned long long)(a.ed[0] & 0xff) | (unsigned long long)(a.ed[1] & 0xff) << 8 |
+					       (unsigned long long)(a.ed[2] & 0xff) << 16;
+			w |= (unsigned long long)((a.rel[0] & 3) | (a.rel[1] & 3) << 2 | (a.rel[2] & 3) << 4) << 24;
+			w |= (unsigned long long)v << 32;
+			setTW(v, w);
+			for (int i = 0; i < 3; i++) setN(v, i, 0);
+		}
+	}
+
 	// src/cell.cc:2043-2059 (the reference caches the result per vertex and
 	// plane; recomputing gives the same value).
 	VG_INL unsigned m_test(int n, double &ans) {

@@ -182,6 +182,25 @@ struct VCell {
 		}
 	}
 
+	// The unit Voronoi cell of a periodic container as the start cell
+	// (src/container_prd.hh:124-130: c=unit_voro copies vertices and edges,
+	// zeroes the neighbour ids and sets up=0, src/cell.cc:71-95).  The unit
+	// cell holds no order-1 or order-2 vertex, so both pools start empty.
+	VG_FN void init_unit(const UnitVert *__restrict__ u, int np) {
+		np0 = np1 = np2 = 0;
+		ovf = false; fatal = false; n_cuts = 0;
+		up = 0;
+		if (np > MAXV) { ovf = true; return; }
+		p = np;
+		for (int v = 0; v < np; v++) {
+			const UnitVert a = u[v];
+			if (a.nu > MAXO) { ovf = true; return; }
+			X(v) = a.x; Y(v) = a.y; Z(v) = a.z;
+			NU(v) = (signed char)a.nu; BP(v) = (short)v;
+			for (int i = 0; i < a.nu; i++) { E(v, i) = a.ed[i]; R(v, i) = a.rel[i]; N(v, i) = 0; }
+		}
+	}
+
 	// src/cell.cc:2043-2059
 	VG_INL unsigned m_calc(int n, double &ans) {
 		ans = X(n) * px;

@@ -27,21 +27,35 @@ struct dbl4 { double x, y, z, w; };
 // Worklist table (generated; identical to the reference's table).
 #include "worklist_table.h"
 
-// Read-only view of a binned rectangular container.
+// Read-only view of a binned container.
+//
+// Rectangular containers: blocks nx*ny*nz.  Periodic containers
+// (container_periodic_base, src/container_prd.hh:83-211): blocks nx*oy*oz,
+// the primary domain at j in [ey,wy), k in [ez,wz), image blocks around it
+// in y and z; ax=ay=az=0, (bx,by,bz) the diagonal of the lattice matrix and
+// bxy, bxz, byz its off-diagonal terms.
 struct RectView {
 	double ax, bx, ay, by, az, bz;
-	int nx, ny, nz, nxy;
+	int nx, ny, nz, nxy;    // nxy = nx * gy
+	int gy;                 // block rows in y: ny, or oy for a periodic container
 	double boxx, boxy, boxz, xsp, ysp, zsp;
 	int xper, yper, zper;
 	int hx, hy, hz;
 	double bxsq, max_len_sq, max_radius;
-	const dbl4 *P;          // sorted particles (x,y,z,r), block-major
-	const int *pid;         // sorted particle ids
-	const int *bstart;      // block start offsets, nx*ny*nz+1
+	const dbl4 *P;          // particles (x,y,z,r), block-major, every block
+	const int *pid;         // their ids
+	const int *bstart;      // block start offsets into P, nblocks+1
+	const int *cstart;      // offsets of the cells (stored particles) per block, nblocks+1
 	const double *mrad;     // 64x64 worklist radii for this block size
 	const unsigned *wl;     // 64x64 worklist table
 	const double *walls;    // sphere walls: xc,yc,zc,rc,w_id
 	int nwalls;
+	// periodic containers
+	double bxy, bxz, byz;
+	int ey, ez, wy, wz, oy, oz;
+	const UnitVert *unit;   // the unit Voronoi cell
+	int unit_p;
+	unsigned long long *fv; // first visit of each block (null: not recorded)
 };
 
 // Per-cell statistics (labels for the candidate predictor).
@@ -50,6 +64,7 @@ struct CellStats {
 	int wl_exit;    // worklist index at which the radius test stopped (64 = phase 3)
 	int phase;      // 1, 2 or 3
 	int max_p;
+	int img;        // periodic containers: an image block was read
 };
 
 template<bool POLY>
@@ -243,10 +258,42 @@ VG_INL int rect_region(const RectView &v, int ci, int cj, int ck, int ei, int ej
 	return disp + ei + v.nx * (ej + v.ny * ek);
 }
 
-// compute_cell for a rectangular container (src/v_compute.cc:291-621).
-// (ijk,s) is the particle's block and index in the block, (ci,cj,ck) the
-// block coordinates.  Returns the reference's compute_cell result; c.ovf /
-// c.fatal report capacity or internal trouble.
+VG_INL int step_div_i(int a, int b) { return a >= 0 ? a / b : -1 + (a + 1) / b; }
+
+// Records the first visit of block b: the smallest (cell << 24 | visit)
+// key over all cells.
+VG_INL void note_first_visit(unsigned long long *p, unsigned long long key) {
+#if defined(__HIP_DEVICE_COMPILE__)
+	if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > key) atomicMin(p, key);
+#else
+	if (*p > key) *p = key;
+#endif
+}
+
+// container_periodic_base::region_index, src/container_prd.hh:163-168.
+// The reference creates the image block here when it is first reached;
+// the engine's image blocks are prepared beforehand (prd_host.h), and the
+// visit is recorded so that their creation order can be replayed.
+VG_INL int prd_region(const RectView &v, int ci, int cj, int ck, int ei, int ej, int ek, double &qx,
+		      unsigned long long key, int &img) {
+	int qi = ci + (ei - v.nx);
+	const int qj = cj + (ej - v.ey), qk = ck + (ek - v.ez);
+	const int iv = step_div_i(qi, v.nx);
+	if (iv != 0) { qx = iv * v.bx; qi -= v.nx * iv; } else qx = 0;
+	const int b = qi + v.nx * (qj + v.oy * qk);
+	if (qj < v.ey || qj >= v.wy || qk < v.ez || qk >= v.wz) {
+		img = 1;
+		if (v.fv) note_first_visit(v.fv + b, key);
+	}
+	return b;
+}
+
+// compute_cell (src/v_compute.cc:291-621) for a rectangular container or,
+// with PRD, a periodic one (start cell, frac_pos and region_index of
+// src/container_prd.hh:124-168).  (ijk,s) is the particle's block and
+// index among the block's cells, (ci,cj,ck) the block coordinates, key0 the
+// cell's loop position << 24 (first-visit keys).  Returns the reference's
+// compute_cell result; c.ovf / c.fatal report capacity or internal trouble.
 //
 // The reference's nested loops (walls, own block, worklist phases 1-2, block
 // search) are driven as one resumable candidate generator feeding a single
@@ -257,26 +304,31 @@ VG_INL int rect_region(const RectView &v, int ci, int cj, int ck, int ei, int ej
 // which under 1% of cells reach; it is therefore recorded as one bit per
 // worklist entry and replayed, in the original order, when the block search
 // starts.
-template<class CELL, bool POLY>
+template<class CELL, bool POLY, bool PRD = false>
 VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci, int cj, int ck,
-			     unsigned stamp, CellStats &stt) {
+			     unsigned stamp, CellStats &stt, unsigned long long key0 = 0) {
 	double x, y, z, qx = 0, qy = 0, qz = 0;
-	int i, j, k, di, dj, dk, ei, ej, ek, f = 0, g = 0, disp;
+	int i, j, k, di, dj, dk, ei, ej, ek, f = 0, g = 0, disp = 0;
 	double fx = 0, fy = 0, fz = 0, gxs = 0, gys = 0, gzs = 0;
 	unsigned q, m1 = 0, m2 = 0;
-	stt.n_planes = 0; stt.wl_exit = 0; stt.phase = 1;
+	unsigned long long key = key0;
+	stt.n_planes = 0; stt.wl_exit = 0; stt.phase = 1; stt.img = 0;
 
 	const int b0 = v.bstart[ijk], co0 = v.bstart[ijk + 1] - b0;
 	const dbl4 me = v.P[b0 + s];
 	x = me.x; y = me.y; z = me.z;
-	{
+	if (PRD) {
+		c.init_unit(v.unit, v.unit_p);
+		if (c.ovf) return false;
+		i = v.nx; j = v.ey; k = v.ez;
+	} else {
 		double xa, xb, ya, yb, za, zb;
 		if (v.xper) { xb = 0.5 * (v.bx - v.ax); xa = -xb; i = v.nx; } else { xa = v.ax - x; xb = v.bx - x; i = ci; }
 		if (v.yper) { yb = 0.5 * (v.by - v.ay); ya = -yb; j = v.ny; } else { ya = v.ay - y; yb = v.by - y; j = cj; }
 		if (v.zper) { zb = 0.5 * (v.bz - v.az); za = -zb; k = v.nz; } else { za = v.az - z; zb = v.bz - z; k = ck; }
 		c.init_box(xa, xb, ya, yb, za, zb);
+		disp = ijk - i - v.nx * (j + v.ny * k);
 	}
-	disp = ijk - i - v.nx * (j + v.ny * k);
 	RadPolicy<POLY> rp;
 	rp.init(me.w, v.max_radius);
 	BlockSearch<CELL> bs(c, stamp, i, j, k);
@@ -344,7 +396,8 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				}
 				// Worklist selection (src/v_compute.cc:328-362).
 				mrs = c.max_radius_squared();
-				fx = x - v.ax - v.boxx * ci; fy = y - v.ay - v.boxy * cj; fz = z - v.az - v.boxz * ck;
+				if (PRD) { fx = x - v.boxx * ci; fy = y - v.boxy * (cj - v.ey); fz = z - v.boxz * (ck - v.ez); }
+				else { fx = x - v.ax - v.boxx * ci; fy = y - v.ay - v.boxy * cj; fz = z - v.az - v.boxz * ck; }
 				di = int(fx * v.xsp * VGPU_WL_FGRID); dj = int(fy * v.ysp * VGPU_WL_FGRID); dk = int(fz * v.zsp * VGPU_WL_FGRID);
 				if (di >= VGPU_WL_HGRID) {
 					gxs = fx;
@@ -398,7 +451,8 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				if (min_max_radius<POLY>(v, rp, di, dj, dk, fx, fy, fz, gxs, gys, gzs, crs, mrs)) continue;
 				if (marking) p2_cut |= 1ull << g;
 				{
-					const int rijk = rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
+					const int rijk = PRD ? prd_region(v, ci, cj, ck, ei, ej, ek, qx, key++, stt.img)
+							     : rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
 					bt = v.bstart[rijk]; be = v.bstart[rijk + 1];
 					x2 = x - qx; y2 = y - qy; z2 = z - qz;
 					bcheck = rp.ctest(crs, mrs);
@@ -448,7 +502,8 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				}
 			}
 			{
-				const int rijk = rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
+				const int rijk = PRD ? prd_region(v, ci, cj, ck, ei, ej, ek, qx, key++, stt.img)
+						     : rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
 				bt = v.bstart[rijk]; be = v.bstart[rijk + 1];
 				x2 = x - qx; y2 = y - qy; z2 = z - qz;
 				bcheck = false;

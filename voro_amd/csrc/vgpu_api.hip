@@ -12,6 +12,7 @@
 #include "../../include/voro_gpu.h"
 #include "grid_host.h"
 #include "learn.h"
+#include "prd_host.h"
 #include "vgpu_bin.h"
 
 using namespace vgpu;
@@ -72,6 +73,10 @@ __global__ void k_fill_int(int *p, int n, int v) {
 
 struct vgpu_container {
 	int kind = 0;
+	bool poly = false;   // radical (power) variant: container_poly, container_periodic_poly
+	bool prd = false;    // periodic lattice: container_periodic(_poly)
+	PrdGeom pg;          // periodic geometry and unit cell
+	int nblocks = 0;     // nx*ny*nz, or nx*oy*oz for a periodic container
 	double geom[6] = {0, 0, 0, 0, 0, 0};
 	int grid[3] = {1, 1, 1};
 	int per[3] = {0, 0, 0};
@@ -109,6 +114,12 @@ struct vgpu_container {
 	Workspace ws_fast, ws_wide, ws_huge;
 	long long nbr_total = 0;
 	bool csr_valid = false;
+	// periodic containers: unit cell, block arrays with image blocks, first
+	// visits, cells that read an image block
+	DBuf d_unit, Pall, pidall, ball, fv, touch, d_flag;
+	std::vector<dbl4> hP;
+	std::vector<int> hpid, hcs;
+	int image_passes = 0;
 };
 
 namespace {
@@ -146,16 +157,24 @@ RectView make_view(vgpu_container *h) {
 	RectView v;
 	const RectGeom &g = h->g;
 	v.ax = g.ax; v.bx = g.bx; v.ay = g.ay; v.by = g.by; v.az = g.az; v.bz = g.bz;
-	v.nx = g.nx; v.ny = g.ny; v.nz = g.nz; v.nxy = g.nx * g.ny;
+	v.nx = g.nx; v.ny = g.ny; v.nz = g.nz;
+	v.gy = h->prd ? h->pg.oy : g.ny;
+	v.nxy = g.nx * v.gy;
 	v.boxx = g.boxx; v.boxy = g.boxy; v.boxz = g.boxz; v.xsp = g.xsp; v.ysp = g.ysp; v.zsp = g.zsp;
 	v.xper = g.xper; v.yper = g.yper; v.zper = g.zper;
 	v.hx = g.xper ? 2 * g.nx + 1 : g.nx; v.hy = g.yper ? 2 * g.ny + 1 : g.ny; v.hz = g.zper ? 2 * g.nz + 1 : g.nz;
+	if (h->prd) { v.hx = 2 * g.nx + 1; v.hy = 2 * h->pg.ey + 1; v.hz = 2 * h->pg.ez + 1; }
 	v.bxsq = g.boxx * g.boxx + g.boxy * g.boxy + g.boxz * g.boxz;
 	v.max_len_sq = h->max_len_sq;
 	v.max_radius = h->max_radius;
-	v.P = h->P.as<dbl4>(); v.pid = h->pid.as<int>(); v.bstart = h->bstart.as<int>();
+	v.P = h->P.as<dbl4>(); v.pid = h->pid.as<int>(); v.bstart = h->bstart.as<int>(); v.cstart = h->bstart.as<int>();
 	v.mrad = h->d_mrad.as<double>(); v.wl = h->d_wl.as<unsigned>();
 	v.walls = h->d_walls.as<double>(); v.nwalls = (int)(h->walls.size() / 5);
+	const PrdGeom &pg = h->pg;
+	v.bxy = h->prd ? pg.bxy : 0; v.bxz = h->prd ? pg.bxz : 0; v.byz = h->prd ? pg.byz : 0;
+	v.ey = pg.ey; v.ez = pg.ez; v.wy = pg.wy; v.wz = pg.wz; v.oy = pg.oy; v.oz = pg.oz;
+	v.unit = h->d_unit.as<UnitVert>(); v.unit_p = h->prd ? (int)pg.unit.size() : 0;
+	v.fv = nullptr;
 	return v;
 }
 
@@ -181,12 +200,18 @@ int launch_tier(vgpu_container *h, Workspace &ws, int waves, const RectView &v, 
 		waves = std::max(1, std::min(waves, (cnt + per_wave - 1) / per_wave));
 	}
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 8, 0);
-	const bool poly = h->kind == VGPU_CONTAINER_POLY;
-	cells_launcher fn = std::is_same<T_, Tier3>::value ? launch_cells_fast3
-			    : (T_::MAXV == TierWide::MAXV ? launch_cells_wide : launch_cells_huge);
-	if (fn(poly, h->fast_occ, waves, s, v, ncells, list, list_count, h->cellblk.as<int>(), ctr, ws.base(), ws.slab,
-	       ws.stamp_base, per_wave, out))
-		return VGPU_HIP_ERROR;
+	const bool poly = h->poly;
+	const int ti = std::is_same<T_, Tier3>::value ? 0 : (T_::MAXV == TierWide::MAXV ? 1 : 2);
+	if (h->prd) {
+		if (launch_cells_prd(ti, poly, waves, s, v, ncells, list, list_count, h->cellblk.as<int>(), ctr, ws.base(),
+				     ws.slab, ws.stamp_base, per_wave, out))
+			return VGPU_HIP_ERROR;
+	} else {
+		cells_launcher fn = ti == 0 ? launch_cells_fast3 : (ti == 1 ? launch_cells_wide : launch_cells_huge);
+		if (fn(poly, h->fast_occ, waves, s, v, ncells, list, list_count, h->cellblk.as<int>(), ctr, ws.base(), ws.slab,
+		       ws.stamp_base, per_wave, out))
+			return VGPU_HIP_ERROR;
+	}
 	ws.stamp_base += (unsigned)ncells + 1u;
 	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
 }
@@ -253,7 +278,7 @@ int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut
 	if ((st = h->ws_fast.ensure(VCell3::L::bytes, waves))) return st;
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, h->d_counters.as<int>(), 8, 0);
 	out.tier = 0; out.retry = h->retry0.as<int>(); out.retry_count = ctr + 4;
-	if (launch_cells_list(h->kind == VGPU_CONTAINER_POLY, waves, s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(),
+	if (launch_cells_list(h->poly, waves, s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(),
 			      h->lcnt.as<int>(),
 			      h->boff.as<int>(), h->list_a.as<ListEntry>(), h->list_b.as<ListTag>(), h->d_counters.as<int>(),
 			      h->ws_fast.base(), ctr + 5, out))
@@ -261,8 +286,105 @@ int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut
 	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
 }
 
+// The plain tiers over every cell (list == nullptr) or over a device list
+// of cells whose count is at *list_count: order-3, then wide, then huge
+// over each tier's overflow.
+int run_tiers(vgpu_container *h, const RectView &v, int nc, int fw, const int *list, const int *list_count,
+	      CellOut out, hipStream_t s, bool mark_ev2) {
+	int st;
+	int *ctr = h->ctr.as<int>();
+	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr + 1, 2, 0);
+	out.tier = 0; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
+	if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, list, list_count, out, s))) return st;
+	if (mark_ev2) HIPCHK(hipEventRecord(h->ev[2], s));
+	out.tier = 2; out.retry = h->retry2.as<int>(); out.retry_count = ctr + 2;
+	if ((st = launch_tier<TierWide>(h, h->ws_wide, 64, v, nc, h->retry1.as<int>(), ctr + 1, out, s))) return st;
+	out.tier = 3; out.retry = nullptr; out.retry_count = nullptr;
+	return launch_tier<TierHuge>(h, h->ws_huge, 4, v, nc, h->retry2.as<int>(), ctr + 2, out, s);
+}
+
+// Uploads the block arrays of every block (primary and image) and points
+// the view at them.
+int upload_blocks(vgpu_container *h, const ImageSet &im, RectView &v, hipStream_t s) {
+	std::vector<dbl4> Pa;
+	std::vector<int> ia, ba;
+	im.assemble(Pa, ia, ba);
+	int st;
+	if ((st = h->Pall.ensure(Pa.size() * sizeof(dbl4))) || (st = h->pidall.ensure(ia.size() * 4)) ||
+	    (st = h->ball.ensure(ba.size() * 4)))
+		return st;
+	HIPCHK(hipMemcpyAsync(h->Pall.p, Pa.data(), Pa.size() * sizeof(dbl4), hipMemcpyHostToDevice, s));
+	HIPCHK(hipMemcpyAsync(h->pidall.p, ia.data(), ia.size() * 4, hipMemcpyHostToDevice, s));
+	HIPCHK(hipMemcpyAsync(h->ball.p, ba.data(), ba.size() * 4, hipMemcpyHostToDevice, s));
+	HIPCHK(hipStreamSynchronize(s));
+	v.P = h->Pall.as<dbl4>(); v.pid = h->pidall.as<int>(); v.bstart = h->ball.as<int>();
+	return VGPU_OK;
+}
+
+// The cells of a periodic container.  Pass 1 computes every cell against
+// image blocks built in create_all_images order and records, per block, the
+// first visit over the loop.  The image blocks are then rebuilt by creating
+// them in that order -- the reference's lazy creation order -- and the cells
+// that read an image block are recomputed, recording first visits again.
+// When the recorded order reproduces itself, every cell has been computed
+// against the reference's block arrays.
+int run_prd_cells(vgpu_container *h, RectView &v, CellOut out, int nc, int fw, hipStream_t s) {
+	const PrdGeom &g = h->pg;
+	const int nb = h->nblocks;
+	int st;
+	h->hP.resize(nc); h->hpid.resize(nc); h->hcs.resize(nb + 1);
+	HIPCHK(hipMemcpyAsync(h->hP.data(), h->P.p, (size_t)nc * sizeof(dbl4), hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(h->hpid.data(), h->pid.p, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(h->hcs.data(), h->bstart.p, (size_t)(nb + 1) * 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
+	{
+		ImageSet im(g, h->hP.data(), h->hpid.data(), h->hcs.data());
+		for (int b : prd_all_images(g)) im.create_block(b);
+		if ((st = upload_blocks(h, im, v, s))) return st;
+	}
+	if ((st = h->fv.ensure((size_t)nb * 8)) || (st = h->touch.ensure((size_t)nc))) return st;
+	unsigned long long *fv = h->fv.as<unsigned long long>();
+	hipLaunchKernelGGL(k_fill_u64, dim3(nblk(nb, 256)), dim3(256), 0, s, fv, nb, ~0ull);
+	hipLaunchKernelGGL(k_fill_u8, dim3(nblk(nc, 256)), dim3(256), 0, s, h->touch.as<unsigned char>(), nc, 0);
+	v.fv = fv;
+	out.touch = h->touch.as<unsigned char>();
+	if ((st = run_tiers(h, v, nc, fw, nullptr, nullptr, out, s, true))) return st;
+	std::vector<unsigned long long> prev(nb), cur(nb);
+	std::vector<unsigned char> t(nc), touched(nc, 0);
+	HIPCHK(hipMemcpyAsync(prev.data(), fv, (size_t)nb * 8, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(t.data(), h->touch.p, (size_t)nc, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
+	h->image_passes = 1;
+	for (int pass = 0; pass < 8; pass++) {
+		std::vector<int> list;
+		for (int c = 0; c < nc; c++) {
+			touched[c] |= t[c];
+			if (touched[c]) list.push_back(c);
+		}
+		if (list.empty()) return VGPU_OK;   // no cell read an image block
+		ImageSet im(g, h->hP.data(), h->hpid.data(), h->hcs.data());
+		for (int b : prd_visit_order(g, prev.data())) im.create_block(b);
+		if (im.late) return VGPU_INTERNAL_ERROR;
+		if ((st = upload_blocks(h, im, v, s))) return st;
+		int *ctr = h->ctr.as<int>();
+		const int m = (int)list.size();
+		HIPCHK(hipMemcpyAsync(h->retry0.p, list.data(), (size_t)m * 4, hipMemcpyHostToDevice, s));
+		HIPCHK(hipMemcpyAsync(ctr + 4, &m, 4, hipMemcpyHostToDevice, s));
+		hipLaunchKernelGGL(k_fill_u64, dim3(nblk(nb, 256)), dim3(256), 0, s, fv, nb, ~0ull);
+		hipLaunchKernelGGL(k_fill_u8, dim3(nblk(nc, 256)), dim3(256), 0, s, h->touch.as<unsigned char>(), nc, 0);
+		if ((st = run_tiers(h, v, nc, fw, h->retry0.as<int>(), ctr + 4, out, s, false))) return st;
+		HIPCHK(hipMemcpyAsync(cur.data(), fv, (size_t)nb * 8, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipMemcpyAsync(t.data(), h->touch.p, (size_t)nc, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipStreamSynchronize(s));
+		h->image_passes = pass + 2;
+		if (cur == prev) return VGPU_OK;
+		prev.swap(cur);
+	}
+	return VGPU_INTERNAL_ERROR;   // the first-visit order did not settle
+}
+
 int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
-	const int nxyz = h->g.nx * h->g.ny * h->g.nz;
+	const int nxyz = h->nblocks;
 	int n = h->n_in;
 	int st;
 	if (!h->events) {
@@ -278,7 +400,7 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 			HIPCHK(hipMemcpyAsync(h->in_ids.p, h->h_ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
 			HIPCHK(hipMemcpyAsync(h->in_xyz.p, h->h_xyz.data(), (size_t)n * 24, hipMemcpyHostToDevice, s));
 		}
-		if (h->kind == VGPU_CONTAINER_POLY) {
+		if (h->poly) {
 			if ((st = h->in_r.ensure((size_t)std::max(n, 1) * 8))) return st;
 			if (n) HIPCHK(hipMemcpyAsync(h->in_r.p, h->h_r.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
 		}
@@ -292,8 +414,15 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 		return st;
 	hipLaunchKernelGGL(k_fill_int, dim3(nblk(nxyz + 1, 256)), dim3(256), 0, s, h->cnt.as<int>(), nxyz + 1, 0);
 	HIPCHK(hipMemsetAsync(h->d_maxr.p, 0, 8, s));
-	const double *rptr = h->kind == VGPU_CONTAINER_POLY ? h->in_r.as<double>() : nullptr;
-	if (n) {
+	const double *rptr = h->poly ? h->in_r.as<double>() : nullptr;
+	if (n && h->prd) {
+		const PrdGeom &pg = h->pg;
+		PrdBin pb{pg.bx, pg.bxy, pg.by, pg.bxz, pg.byz, pg.bz, pg.xsp, pg.ysp, pg.zsp,
+			  pg.nx, pg.ny, pg.nz, pg.ey, pg.ez, pg.oy};
+		hipLaunchKernelGGL(k_locate_prd, dim3(nblk(n, 256)), dim3(256), 0, s, n, h->in_xyz.as<double>(), rptr, pb,
+				   h->blk.as<int>(), h->slot.as<int>(), h->rem.as<dbl4>(), h->cnt.as<int>(),
+				   h->d_maxr.as<unsigned long long>());
+	} else if (n) {
 		hipLaunchKernelGGL(k_locate, dim3(nblk(n, 256)), dim3(256), 0, s, n, h->in_xyz.as<double>(), rptr, h->g,
 				   h->blk.as<int>(), h->slot.as<int>(), h->rem.as<dbl4>(), h->cnt.as<int>(),
 				   h->d_maxr.as<unsigned long long>());
@@ -307,21 +436,31 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 				   h->src.as<int>(), h->cellblk.as<int>());
 	}
 	HIPCHK(hipGetLastError());
-	int nc = 0;
+	int nc = 0, dup = 0;
 	unsigned long long mrb = 0;
+	if (h->prd && n) {
+		// periodic put() rejects a particle within 1e-5 of one in its block
+		if ((st = h->d_flag.ensure(4))) return st;
+		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, h->d_flag.as<int>(), 1, 0);
+		hipLaunchKernelGGL(k_duplicates, dim3(nblk(n, 256)), dim3(256), 0, s, nxyz, h->P.as<dbl4>(), h->cellblk.as<int>(),
+				   h->bstart.as<int>(), h->d_flag.as<int>());
+	}
 	HIPCHK(hipMemcpyAsync(&nc, h->bstart.as<int>() + nxyz, 4, hipMemcpyDeviceToHost, s));
 	HIPCHK(hipMemcpyAsync(&mrb, h->d_maxr.p, 8, hipMemcpyDeviceToHost, s));
+	if (h->prd && n) HIPCHK(hipMemcpyAsync(&dup, h->d_flag.p, 4, hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
 	h->ncells = nc;
 	double mr;
 	std::memcpy(&mr, &mrb, 8);
-	h->max_radius = h->kind == VGPU_CONTAINER_POLY ? mr : 0;
+	h->max_radius = h->poly ? mr : 0;
+	if (dup) { h->computed = false; return VGPU_DUPLICATE; }
 	HIPCHK(hipEventRecord(h->ev[1], s));
 
 	// outputs
 	const size_t C1 = (size_t)std::max(nc, 1);
 	const bool nbrs = (flags & VGPU_F_NEIGHBORS) != 0;
-	long long acap = nbrs ? std::min<long long>(2147483000LL, 24LL * C1 + 4096) : 1;
+	// periodic containers recompute the cells near image blocks: room for both
+	long long acap = nbrs ? std::min<long long>(2147483000LL, (h->prd ? 48LL : 24LL) * C1 + 4096) : 1;
 	if ((st = h->vol.ensure(C1 * 8)) || (st = h->cent.ensure(C1 * 24)) || (st = h->nfaces.ensure(C1 * 4)) ||
 	    (st = h->status.ensure(C1 * 4)) || (st = h->nbr_start.ensure(C1 * 4)) || (st = h->arena.ensure((size_t)acap * 4)) ||
 	    (st = h->ctr.ensure(16 * 4)) || (st = h->retry0.ensure(C1 * 4)) || (st = h->retry1.ensure(C1 * 4)) ||
@@ -343,13 +482,17 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 	out.st_exit = stats ? h->st_exit.as<signed char>() : nullptr;
 	out.st_tier = stats ? h->st_tier.as<signed char>() : nullptr;
 	out.want_nbrs = nbrs ? 1 : 0;
+	out.touch = nullptr;
 
 	int fw = h->fast_waves > 0 ? h->fast_waves : default_fast_waves(h->fast_occ);
 	fw = std::max(1, std::min(fw, nblk(std::max(nc, 1), 64)));
 	const bool learned = (flags & VGPU_F_LEARNED) != 0;
 	h->last_fallbacks = 0;
 	h->last_entries = 0;
-	if (nc > 0 && learned) {
+	h->image_passes = 0;
+	if (nc > 0 && h->prd) {
+		if ((st = run_prd_cells(h, v, out, nc, fw, s))) return st;
+	} else if (nc > 0 && learned) {
 		// tier 0: candidate lists cut at the predicted worklist depth
 		if ((st = run_learned(h, v, nc, fw, out, s))) return st;
 		HIPCHK(hipEventRecord(h->ev[2], s));
@@ -362,7 +505,7 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 	} else {
 		HIPCHK(hipEventRecord(h->ev[2], s));
 	}
-	if (nc > 0) {
+	if (nc > 0 && !h->prd) {
 		out.tier = 2; out.retry = h->retry2.as<int>(); out.retry_count = ctr + 2;
 		if ((st = launch_tier<TierWide>(h, h->ws_wide, 64, v, nc, h->retry1.as<int>(), ctr + 1, out, s))) return st;
 		out.tier = 3; out.retry = nullptr; out.retry_count = nullptr;
@@ -414,25 +557,55 @@ int vgpu_create(int kind, const double *geom, const int *grid, const int *period
 	(void)init_mem;
 	if (!out || !geom || !grid) return VGPU_INVALID_ARGUMENT;
 	*out = nullptr;
-	if (kind != VGPU_CONTAINER && kind != VGPU_CONTAINER_POLY) return VGPU_INVALID_ARGUMENT;
+	if (kind < VGPU_CONTAINER || kind > VGPU_CONTAINER_PERIODIC_POLY) return VGPU_INVALID_ARGUMENT;
 	if (grid[0] <= 0 || grid[1] <= 0 || grid[2] <= 0) return VGPU_INVALID_ARGUMENT;
 	if ((long long)grid[0] * grid[1] * grid[2] > 1000000000LL) return VGPU_INVALID_ARGUMENT;
-	if (!(geom[1] > geom[0]) || !(geom[3] > geom[2]) || !(geom[5] > geom[4])) return VGPU_INVALID_ARGUMENT;
+	const bool prd = kind >= VGPU_CONTAINER_PERIODIC;
+	for (int t = 0; t < 6; t++) if (!std::isfinite(geom[t])) return VGPU_INVALID_ARGUMENT;
+	if (prd) {
+		if (!(geom[0] > 0) || !(geom[2] > 0) || !(geom[5] > 0)) return VGPU_INVALID_ARGUMENT;
+	} else if (!(geom[1] > geom[0]) || !(geom[3] > geom[2]) || !(geom[5] > geom[4])) return VGPU_INVALID_ARGUMENT;
 	int ndev = 0;
 	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return VGPU_NO_DEVICE;
 	vgpu_container *h = new vgpu_container();
 	h->kind = kind;
+	h->prd = prd;
+	h->poly = kind == VGPU_CONTAINER_POLY || kind == VGPU_CONTAINER_PERIODIC_POLY;
 	for (int t = 0; t < 6; t++) h->geom[t] = geom[t];
-	for (int t = 0; t < 3; t++) { h->grid[t] = grid[t]; h->per[t] = periodic ? (periodic[t] != 0) : 0; }
+	for (int t = 0; t < 3; t++) { h->grid[t] = grid[t]; h->per[t] = (!prd && periodic) ? (periodic[t] != 0) : 0; }
 	RectGeom &g = h->g;
-	g.ax = geom[0]; g.bx = geom[1]; g.ay = geom[2]; g.by = geom[3]; g.az = geom[4]; g.bz = geom[5];
-	g.nx = grid[0]; g.ny = grid[1]; g.nz = grid[2];
-	// src/container.cc:30-36, src/v_base.cc:23-25
-	g.boxx = (g.bx - g.ax) / g.nx; g.boxy = (g.by - g.ay) / g.ny; g.boxz = (g.bz - g.az) / g.nz;
-	g.xsp = 1 / g.boxx; g.ysp = 1 / g.boxy; g.zsp = 1 / g.boxz;
-	g.xper = h->per[0]; g.yper = h->per[1]; g.zper = h->per[2];
-	h->max_len_sq = (g.bx - g.ax) * (g.bx - g.ax) * (g.xper ? 0.25 : 1) + (g.by - g.ay) * (g.by - g.ay) * (g.yper ? 0.25 : 1)
-			+ (g.bz - g.az) * (g.bz - g.az) * (g.zper ? 0.25 : 1);
+	if (prd) {
+		// geom = {bx,bxy,by,bxz,byz,bz}: unit cell, then the block grid
+		// (src/container_prd.cc:30-37, src/unitcell.cc:26-85)
+		if (!h->pg.build(geom[0], geom[1], geom[2], geom[3], geom[4], geom[5], grid[0], grid[1], grid[2])) {
+			delete h;
+			return VGPU_MEMORY_ERROR;
+		}
+		if ((long long)h->pg.nblocks() > 1000000000LL) { delete h; return VGPU_INVALID_ARGUMENT; }
+		g.ax = 0; g.bx = geom[0]; g.ay = 0; g.by = geom[2]; g.az = 0; g.bz = geom[5];
+		g.nx = grid[0]; g.ny = grid[1]; g.nz = grid[2];
+		g.boxx = h->pg.boxx; g.boxy = h->pg.boxy; g.boxz = h->pg.boxz;
+		g.xsp = h->pg.xsp; g.ysp = h->pg.ysp; g.zsp = h->pg.zsp;
+		g.xper = g.yper = g.zper = 0;
+		h->max_len_sq = h->pg.max_len_sq;
+		h->nblocks = (int)h->pg.nblocks();
+		const size_t ub = h->pg.unit.size() * sizeof(UnitVert);
+		if (h->d_unit.ensure(ub) ||
+		    hipMemcpy(h->d_unit.p, h->pg.unit.data(), ub, hipMemcpyHostToDevice) != hipSuccess) {
+			delete h;
+			return VGPU_HIP_ERROR;
+		}
+	} else {
+		g.ax = geom[0]; g.bx = geom[1]; g.ay = geom[2]; g.by = geom[3]; g.az = geom[4]; g.bz = geom[5];
+		g.nx = grid[0]; g.ny = grid[1]; g.nz = grid[2];
+		// src/container.cc:30-36, src/v_base.cc:23-25
+		g.boxx = (g.bx - g.ax) / g.nx; g.boxy = (g.by - g.ay) / g.ny; g.boxz = (g.bz - g.az) / g.nz;
+		g.xsp = 1 / g.boxx; g.ysp = 1 / g.boxy; g.zsp = 1 / g.boxz;
+		g.xper = h->per[0]; g.yper = h->per[1]; g.zper = h->per[2];
+		h->max_len_sq = (g.bx - g.ax) * (g.bx - g.ax) * (g.xper ? 0.25 : 1) + (g.by - g.ay) * (g.by - g.ay) * (g.yper ? 0.25 : 1)
+				+ (g.bz - g.az) * (g.bz - g.az) * (g.zper ? 0.25 : 1);
+		h->nblocks = g.nx * g.ny * g.nz;
+	}
 	std::vector<double> mrad(4096);
 	worklist_radii(g.boxx, g.boxy, g.boxz, mrad.data());
 	if (h->d_mrad.ensure(4096 * 8) || h->d_wl.ensure(4096 * 4) || h->d_walls.ensure(8 * 5 * 64)) {
@@ -453,7 +626,10 @@ int vgpu_destroy(vgpu_container *h) {
 	DBuf *bufs[] = {&h->in_ids, &h->in_xyz, &h->in_r, &h->blk, &h->slot, &h->rem, &h->cnt, &h->bstart, &h->tmp,
 			&h->part, &h->P, &h->pid, &h->src, &h->cellblk, &h->d_mrad, &h->d_wl, &h->d_walls, &h->d_maxr,
 			&h->d_counters, &h->vol, &h->cent, &h->nfaces, &h->status, &h->nbr_start, &h->arena, &h->csr_off,
-			&h->csr, &h->ctr, &h->retry1, &h->retry2, &h->st_planes, &h->st_exit, &h->st_tier};
+			&h->csr, &h->ctr, &h->retry0, &h->retry1, &h->retry2, &h->st_planes, &h->st_exit, &h->st_tier,
+			&h->d_pred, &h->d_feat, &h->d_cfeat, &h->d_depth, &h->d_cost, &h->d_order, &h->d_hist, &h->lcnt,
+			&h->bmax, &h->boff, &h->list_a, &h->list_b, &h->d_unit, &h->Pall, &h->pidall, &h->ball, &h->fv,
+			&h->touch, &h->d_flag};
 	for (DBuf *b : bufs) b->release();
 	h->ws_fast.buf.release(); h->ws_wide.buf.release(); h->ws_huge.buf.release();
 	if (h->events) for (int t = 0; t < 5; t++) (void)hipEventDestroy(h->ev[t]);
@@ -463,7 +639,7 @@ int vgpu_destroy(vgpu_container *h) {
 }
 
 int vgpu_add_wall_sphere(vgpu_container *h, double xc, double yc, double zc, double rc, int w_id) {
-	if (!h) return VGPU_INVALID_ARGUMENT;
+	if (!h || h->prd) return VGPU_INVALID_ARGUMENT;
 	if ((int)h->walls.size() / 5 >= 64) return VGPU_MEMORY_ERROR;
 	double w[5] = {xc, yc, zc, rc, (double)w_id};
 	h->walls.insert(h->walls.end(), w, w + 5);
@@ -475,17 +651,17 @@ int vgpu_add_wall_sphere(vgpu_container *h, double xc, double yc, double zc, dou
 
 int vgpu_put(vgpu_container *h, int n, const int *ids, const double *xyz, const double *r) {
 	if (!h || n < 0 || (n > 0 && (!ids || !xyz))) return VGPU_INVALID_ARGUMENT;
-	if (h->kind == VGPU_CONTAINER_POLY && n > 0 && !r) return VGPU_INVALID_ARGUMENT;
+	if (h->poly && n > 0 && !r) return VGPU_INVALID_ARGUMENT;
 	if (h->have_dev_input) {
 		// switching back to host-side puts: start from the device contents
 		h->h_ids.resize(h->n_in);
 		h->h_xyz.resize((size_t)h->n_in * 3);
-		h->h_r.resize(h->kind == VGPU_CONTAINER_POLY ? h->n_in : 0);
+		h->h_r.resize(h->poly ? h->n_in : 0);
 		if (h->n_in) {
 			if (hipMemcpy(h->h_ids.data(), h->in_ids.p, (size_t)h->n_in * 4, hipMemcpyDeviceToHost) != hipSuccess ||
 			    hipMemcpy(h->h_xyz.data(), h->in_xyz.p, (size_t)h->n_in * 24, hipMemcpyDeviceToHost) != hipSuccess)
 				return VGPU_HIP_ERROR;
-			if (h->kind == VGPU_CONTAINER_POLY &&
+			if (h->poly &&
 			    hipMemcpy(h->h_r.data(), h->in_r.p, (size_t)h->n_in * 8, hipMemcpyDeviceToHost) != hipSuccess)
 				return VGPU_HIP_ERROR;
 		}
@@ -494,7 +670,7 @@ int vgpu_put(vgpu_container *h, int n, const int *ids, const double *xyz, const 
 	if ((long long)h->n_in + n > 2000000000LL / 32) return VGPU_MEMORY_ERROR;
 	h->h_ids.insert(h->h_ids.end(), ids, ids + n);
 	h->h_xyz.insert(h->h_xyz.end(), xyz, xyz + 3 * (size_t)n);
-	if (h->kind == VGPU_CONTAINER_POLY) h->h_r.insert(h->h_r.end(), r, r + n);
+	if (h->poly) h->h_r.insert(h->h_r.end(), r, r + n);
 	h->n_in += n;
 	h->computed = false;
 	return VGPU_OK;
@@ -502,7 +678,7 @@ int vgpu_put(vgpu_container *h, int n, const int *ids, const double *xyz, const 
 
 int vgpu_put_device(vgpu_container *h, int n, const int *d_ids, const double *d_xyz, const double *d_r) {
 	if (!h || n < 0 || (n > 0 && (!d_ids || !d_xyz))) return VGPU_INVALID_ARGUMENT;
-	if (h->kind == VGPU_CONTAINER_POLY && n > 0 && !d_r) return VGPU_INVALID_ARGUMENT;
+	if (h->poly && n > 0 && !d_r) return VGPU_INVALID_ARGUMENT;
 	if (n > 2000000000 / 32) return VGPU_MEMORY_ERROR;
 	int st;
 	const size_t N1 = (size_t)std::max(n, 1);
@@ -512,7 +688,7 @@ int vgpu_put_device(vgpu_container *h, int n, const int *d_ids, const double *d_
 		    hipMemcpy(h->in_xyz.p, d_xyz, (size_t)n * 24, hipMemcpyDeviceToDevice) != hipSuccess)
 			return VGPU_HIP_ERROR;
 	}
-	if (h->kind == VGPU_CONTAINER_POLY) {
+	if (h->poly) {
 		if ((st = h->in_r.ensure(N1 * 8))) return st;
 		if (n && hipMemcpy(h->in_r.p, d_r, (size_t)n * 8, hipMemcpyDeviceToDevice) != hipSuccess) return VGPU_HIP_ERROR;
 	}
@@ -533,7 +709,7 @@ int vgpu_clear(vgpu_container *h) {
 }
 
 int vgpu_compute_all(vgpu_container *h, unsigned flags, void *stream) {
-	if (h && (flags & VGPU_F_LEARNED) && !h->have_pred) return VGPU_INVALID_ARGUMENT;
+	if (h && (flags & VGPU_F_LEARNED) && (!h->have_pred || h->prd)) return VGPU_INVALID_ARGUMENT;
 	if (!h) return VGPU_INVALID_ARGUMENT;
 	return run_pipeline(h, flags, (hipStream_t)stream);
 }
@@ -550,7 +726,7 @@ int vgpu_get_results(vgpu_container *h, int *id, int *ijk, int *q, int *exists, 
 		     long long *nbr_total) {
 	if (!h || !h->computed) return VGPU_INVALID_ARGUMENT;
 	const int nc = h->ncells;
-	const int nxyz = h->g.nx * h->g.ny * h->g.nz;
+	const int nxyz = h->nblocks;
 	if (nc == 0) {
 		if (nbr_off) nbr_off[0] = 0;
 		if (nbr_total) *nbr_total = 0;
@@ -667,7 +843,7 @@ int vgpu_set_predictor(vgpu_container *h, vgpu_predictor *p) {
 	if (!h) return VGPU_INVALID_ARGUMENT;
 	if (!p) { h->have_pred = false; return VGPU_OK; }
 	if (!p->trained || p->params.size() != (size_t)vgpu::learn::NP) return VGPU_INVALID_ARGUMENT;
-	if (p->with_radius && h->kind != VGPU_CONTAINER_POLY) return VGPU_INVALID_ARGUMENT;
+	if (p->with_radius && !h->poly) return VGPU_INVALID_ARGUMENT;
 	int st = h->d_pred.ensure(sizeof(float) * vgpu::learn::NP);
 	if (st) return st;
 	HIPCHK(hipMemcpy(h->d_pred.p, p->params.data(), sizeof(float) * vgpu::learn::NP, hipMemcpyHostToDevice));
@@ -691,7 +867,7 @@ int vgpu_learned_stats(vgpu_container *h, long long *fallbacks, long long *list_
 int vgpu_predictor_add_samples(vgpu_predictor *p, vgpu_container *h) {
 	using namespace vgpu::learn;
 	if (!p || !h || !h->computed || !(h->last_flags & VGPU_F_STATS)) return VGPU_INVALID_ARGUMENT;
-	if (p->with_radius && h->kind != VGPU_CONTAINER_POLY) return VGPU_INVALID_ARGUMENT;
+	if ((p->with_radius && !h->poly) || h->prd) return VGPU_INVALID_ARGUMENT;
 	const int nxyz = h->g.nx * h->g.ny * h->g.nz, nc = h->ncells;
 	if (nc <= 0) return VGPU_OK;
 	hipStream_t s = nullptr;

@@ -55,6 +55,87 @@ __global__ void k_locate(int n, const double *__restrict__ xyz, const double *__
 	}
 }
 
+// Geometry needed to bin particles of a periodic container.
+struct PrdBin {
+	double bx, bxy, by, bxz, byz, bz;
+	double xsp, ysp, zsp;
+	int nx, ny, nz, ey, ez, oy;
+};
+
+__device__ __forceinline__ int step_div_d(int a, int b) { return a >= 0 ? a / b : -1 + (a + 1) / b; }
+
+// Block index of every particle of a periodic container and its position
+// remapped into the primary domain, as put_locate_block computes them
+// (src/container_prd.cc:189-217): z first (shifting y and x by the lattice's
+// off-diagonal terms), then y (shifting x), then x.  A coordinate whose
+// block index would not fit an int is dropped (block -1).
+__global__ void k_locate_prd(int n, const double *__restrict__ xyz, const double *__restrict__ r, PrdBin g,
+			     int *__restrict__ blk, int *__restrict__ slot, dbl4 *__restrict__ rem,
+			     int *__restrict__ cnt, unsigned long long *__restrict__ maxr_bits) {
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	double x = xyz[3 * t], y = xyz[3 * t + 1], z = xyz[3 * t + 2];
+	int b = -1;
+	const double lim = 1073741824.0;
+	const double zs = z * g.zsp;
+	if (fabs(zs) < lim) {
+		int k = step_int_d(zs);
+		if (k < 0 || k >= g.nz) {
+			const int ak = step_div_d(k, g.nz);
+			z -= ak * g.bz; y -= ak * g.byz; x -= ak * g.bxz; k -= ak * g.nz;
+		}
+		const double ys = y * g.ysp;
+		if (fabs(ys) < lim) {
+			int j = step_int_d(ys);
+			if (j < 0 || j >= g.ny) {
+				const int aj = step_div_d(j, g.ny);
+				y -= aj * g.by; x -= aj * g.bxy; j -= aj * g.ny;
+			}
+			const double xs = x * g.xsp;
+			if (fabs(xs) < lim) {
+				int i = step_int_d(xs);
+				if (i < 0 || i >= g.nx) {
+					const int ai = step_div_d(i, g.nx);
+					x -= ai * g.bx; i -= ai * g.nx;
+				}
+				j += g.ey; k += g.ez;
+				b = i + g.nx * (j + g.oy * k);
+			}
+		}
+	}
+	blk[t] = b;
+	double rr = r ? r[t] : 0.5;
+	rem[t] = dbl4{x, y, z, rr};
+	if (b >= 0) {
+		slot[t] = atomicAdd(&cnt[b], 1);
+		if (r && rr > 0) atomicMax(maxr_bits, (unsigned long long)__double_as_longlong(rr));
+	}
+}
+
+// check_duplicate (src/common.cc:14-20), which the periodic containers'
+// put() applies against every particle already in the block
+// (src/container_prd.cc:94-112): flags a pair closer than 1e-5.
+__global__ void k_duplicates(int nblocks, const dbl4 *__restrict__ P, const int *__restrict__ cellblk,
+			     const int *__restrict__ bstart, int *__restrict__ flag) {
+	int d = blockIdx.x * blockDim.x + threadIdx.x;
+	if (d >= bstart[nblocks]) return;
+	const dbl4 a = P[d];
+	for (int u = bstart[cellblk[d]]; u < d; u++) {
+		const dbl4 q = P[u];
+		const double dx = q.x - a.x, dy = q.y - a.y, dz = q.z - a.z;
+		if (dx * dx + dy * dy + dz * dz < 1e-10) { atomicOr(flag, 1); return; }
+	}
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v) {
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < n) p[t] = v;
+}
+__global__ void k_fill_u8(unsigned char *p, int n, unsigned char v) {
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < n) p[t] = v;
+}
+
 // Exclusive scan, tiles of 1024 ints by 256 threads; tile totals to part.
 __global__ void k_scan_tiles(const int *__restrict__ in, int *__restrict__ out, int n, int *__restrict__ part) {
 	__shared__ int sh[256];

@@ -20,6 +20,7 @@ struct CellOut {
 	short *st_planes;    // optional statistics
 	signed char *st_exit;
 	signed char *st_tier;
+	unsigned char *touch; // periodic containers: 1 if the cell read an image block
 	int tier;
 	int want_nbrs;
 };
@@ -45,8 +46,9 @@ template<> struct TierCell<Tier3> { typedef VCell3 type; };
 // share an L2; a wavefront whose share is exhausted moves to the next one.
 // per_wave (1..64) is the number of cells a wavefront takes at a time: 64
 // for a full tier, fewer for a short retry list of expensive cells, which
-// then spreads over all wavefronts instead of a few full ones.
-template<class T_, bool POLY, int OCC>
+// then spreads over all wavefronts instead of a few full ones.  PRD selects
+// the periodic container's start cell and block indexing.
+template<class T_, bool POLY, int OCC, bool PRD = false>
 __global__ void __launch_bounds__(64, OCC)
 k_cells(RectView v, int ncells, const int *__restrict__ list, const int *__restrict__ list_count,
 	const int *__restrict__ cellblk, int *__restrict__ counters, char *__restrict__ ws, size_t slab_bytes,
@@ -72,10 +74,11 @@ k_cells(RectView v, int ncells, const int *__restrict__ list, const int *__restr
 			if (idx < hi && lane < per_wave) {
 				const int cell = list ? list[idx] : idx;
 				const int b = cellblk[cell];
-				const int s = cell - v.bstart[b];
-				const int ci = b % v.nx, cj = (b / v.nx) % v.ny, ck = b / v.nxy;
+				const int s = cell - v.cstart[b];
+				const int ci = b % v.nx, cj = (b / v.nx) % v.gy, ck = b / v.nxy;
 				CellStats stt;
-				bool ok = compute_cell_rect<CELL, POLY>(c, v, b, s, ci, cj, ck, stamp_base + (unsigned)cell + 1u, stt);
+				bool ok = compute_cell_rect<CELL, POLY, PRD>(c, v, b, s, ci, cj, ck, stamp_base + (unsigned)cell + 1u,
+									     stt, (unsigned long long)cell << 24);
 				double vol = 0, cx = 0, cy = 0, cz = 0;
 				int nf = 0;
 				if (!c.ovf && ok) {
@@ -110,6 +113,7 @@ k_cells(RectView v, int ncells, const int *__restrict__ list, const int *__restr
 						out.st_exit[cell] = (signed char)stt.wl_exit;
 						out.st_tier[cell] = (signed char)out.tier;
 					}
+					if (PRD && out.touch) out.touch[cell] = (unsigned char)stt.img;
 				}
 			}
 		}
@@ -132,6 +136,11 @@ int launch_cells_huge(bool poly, int occ, int waves, hipStream_t s, const RectVi
 		      const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
 		      unsigned stamp_base, int per_wave, const CellOut &out);
 
+// Periodic containers (cells_prd.hip): tier 0 order-3, 1 wide, 2 huge.
+int launch_cells_prd(int tier, bool poly, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
+		     const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
+		     unsigned stamp_base, int per_wave, const CellOut &out);
+
 // Learned path (cells_list.hip): list gathering and the list-driven tier.
 int launch_list_order(hipStream_t s, int nc, const int *cellblk, const unsigned char *cost, int per_cell, int *hist,
 		      int *order);
@@ -146,5 +155,8 @@ int launch_cells_list(bool poly, int waves, hipStream_t s, const RectView &v, in
 #define VGPU_CELLS_LAUNCH(TIER, P, O)                                                                      \
 	hipLaunchKernelGGL((k_cells<TIER, P, O>), dim3(waves), dim3(64), 0, s, v, ncells, list, list_count,    \
 			   cellblk, counters, ws, slab, stamp_base, per_wave, out)
+#define VGPU_CELLS_LAUNCH_PRD(TIER, P, O)                                                                  \
+	hipLaunchKernelGGL((k_cells<TIER, P, O, true>), dim3(waves), dim3(64), 0, s, v, ncells, list,          \
+			   list_count, cellblk, counters, ws, slab, stamp_base, per_wave, out)
 
 }  // namespace vgpu

@@ -74,6 +74,17 @@ struct Layout {
 	enum { NBCAP = MAXV };
 };
 
+// One vertex of the unit Voronoi cell of a periodic container, the start
+// cell of every particle's cell there: doubled coordinates (as the cells
+// store them), order, edges and edge relations.  64 bytes.
+struct UnitVert {
+	double x, y, z;
+	int nu;
+	short ed[8];
+	signed char rel[8];
+	int pad[3];
+};
+
 // Per-cell status bits.
 enum : int {
 	ST_EXISTS = 1,      // compute_cell returned true
