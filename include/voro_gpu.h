@@ -38,6 +38,7 @@ extern "C" {
 #define VGPU_F_STATS 2u       /* record per-cell candidate statistics */
 #define VGPU_F_LEARNED 4u     /* use the trained candidate predictor if one is loaded */
 #define VGPU_F_CSR 8u         /* compact neighbour lists into loop-order CSR */
+#define VGPU_F_GRAPH 16u      /* batches: capture the compute as a hipGraph once, replay it afterwards */
 
 typedef struct vgpu_container vgpu_container;
 
@@ -161,6 +162,47 @@ int vgpu_learned_stats(vgpu_container *h, long long *fallbacks, long long *list_
 int vgpu_mlp_num_params(void);
 int vgpu_mlp_loss_grad(const float *params, const float *X, const float *T, int n, float *loss, float *grad);
 int vgpu_mlp_forward(const float *params, const float *X, int n, float *Y);
+
+/* ---- Batches of containers (no counterpart in the reference) ----
+ * Many independent rectangular containers -- each created, walled and
+ * filled through the calls above, exactly as the reference's program would
+ * set up one container object per system -- computed together: one binning
+ * pass and one persistent work-queue cell kernel over all of them, in place
+ * of a compute_all_cells() (src/container.cc:438-443) per container.  Every
+ * container's results are those vgpu_compute_all would give it.  All
+ * containers of a batch must be of one kind (container or container_poly). */
+typedef struct vgpu_batch vgpu_batch;
+
+/* Copies the containers' geometry, walls and particles (as put so far) into
+ * a new batch; the handles are not kept. */
+int vgpu_batch_create(vgpu_container *const *hs, int count, vgpu_batch **out);
+int vgpu_batch_destroy(vgpu_batch *b);
+/* Computes every container (flags as vgpu_compute_all; VGPU_F_GRAPH
+ * captures the whole device sequence as a hipGraph on the first call and
+ * replays it on later calls with the same flags). */
+int vgpu_batch_compute(vgpu_batch *b, unsigned flags, void *stream);
+/* Replaces all particle positions (same count and order as at creation:
+ * n x 3 doubles, host or device memory). */
+int vgpu_batch_set_positions(vgpu_batch *b, const double *xyz, int on_device);
+/* Container `index`'s cell count and results, as vgpu_num_cells and
+ * vgpu_get_results give them for a single container. */
+int vgpu_batch_num_cells(vgpu_batch *b, int index, long long *n);
+int vgpu_batch_get_results(vgpu_batch *b, int index, int *id, int *ijk, int *q, int *exists, double *pos, double *vol,
+			   double *centroid, int *nfaces, long long *nbr_off, int *nbr, long long nbr_cap,
+			   long long *nbr_total);
+/* Per-container sum_cell_volumes() (out has one entry per container). */
+int vgpu_batch_sum_volumes(vgpu_batch *b, double *out);
+/* Launch configuration {persistent wavefronts (0 = as many as fit),
+ * waves per SIMD of the first tier (2..4), cells per queue entry (1..64)};
+ * get and/or set.  Results do not depend on it. */
+int vgpu_batch_config(vgpu_batch *b, int *get3, const int *set3);
+/* Times every configuration on this batch (reps computes each) and keeps
+ * the fastest; best3 / best_ms receive it and its time. */
+int vgpu_batch_autotune(vgpu_batch *b, unsigned flags, int reps, int *best3, double *best_ms);
+/* Milliseconds of the last compute: [0] binning and queue set-up, [1] first
+ * tier, [2] retry tiers and neighbour compaction (graph replays: the total
+ * in [1]). */
+int vgpu_batch_last_timings(vgpu_batch *b, double *ms3);
 
 /* Human-readable text of a status code. */
 const char *vgpu_status_string(int status);

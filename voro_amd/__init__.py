@@ -40,6 +40,7 @@ F_NEIGHBORS = 1
 F_STATS = 2
 F_LEARNED = 4
 F_CSR = 8
+F_GRAPH = 16
 
 KIND_CONTAINER = 0
 KIND_POLY = 1
@@ -94,6 +95,17 @@ def lib():
         "vgpu_predictor_margin": (C.c_int, [p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "vgpu_set_predictor": (C.c_int, [p, p]),
         "vgpu_learned_stats": (C.c_int, [p, i64p, i64p, f64p]),
+        "vgpu_batch_create": (C.c_int, [C.POINTER(p), C.c_int, C.POINTER(p)]),
+        "vgpu_batch_destroy": (C.c_int, [p]),
+        "vgpu_batch_compute": (C.c_int, [p, C.c_uint, p]),
+        "vgpu_batch_set_positions": (C.c_int, [p, p, C.c_int]),
+        "vgpu_batch_num_cells": (C.c_int, [p, C.c_int, i64p]),
+        "vgpu_batch_get_results": (C.c_int, [p, C.c_int, i32p, i32p, i32p, i32p, f64p, f64p, f64p, i32p, i64p, i32p,
+                                             C.c_longlong, i64p]),
+        "vgpu_batch_sum_volumes": (C.c_int, [p, f64p]),
+        "vgpu_batch_config": (C.c_int, [p, i32p, i32p]),
+        "vgpu_batch_autotune": (C.c_int, [p, C.c_uint, C.c_int, i32p, f64p]),
+        "vgpu_batch_last_timings": (C.c_int, [p, f64p]),
         "vgpu_mlp_num_params": (C.c_int, []),
         "vgpu_mlp_loss_grad": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int,
                                          C.POINTER(C.c_float), C.POINTER(C.c_float)]),
@@ -368,6 +380,93 @@ class voronoicell_neighbor:
     def neighbors(self):
         s, e = self._r["nbr_off"][self._m], self._r["nbr_off"][self._m + 1]
         return [int(v) for v in self._r["nbr"][s:e]]
+
+
+class container_batch:
+    """Many independent rectangular containers computed together (no
+    counterpart in the reference, which computes one container at a time):
+    one binning pass and one persistent work-queue cell kernel over all of
+    them.  Built from containers set up as usual (geometry, walls, put());
+    results(i) gives container i's results exactly as its own
+    compute_all_cells() would."""
+
+    def __init__(self, containers):
+        arr = (C.c_void_p * len(containers))(*[c._h for c in containers])
+        h = C.c_void_p()
+        _check(lib().vgpu_batch_create(arr, len(containers), C.byref(h)), "vgpu_batch_create")
+        self._h = h
+        self.count = len(containers)
+        self._flags = F_NEIGHBORS
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.vgpu_batch_destroy(h)
+            self._h = None
+
+    def compute(self, flags=F_NEIGHBORS | F_CSR, graph=False, stream=None):
+        """Computes every container; graph=True captures the device sequence
+        as a hipGraph on the first call and replays it afterwards."""
+        f = int(flags) | (F_GRAPH if graph else 0)
+        _check(lib().vgpu_batch_compute(self._h, f, stream), "vgpu_batch_compute")
+        self._flags = f
+
+    def set_positions(self, xyz):
+        """New positions for every particle (same count and order)."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        _check(lib().vgpu_batch_set_positions(self._h, xyz.ctypes.data_as(C.c_void_p), 0), "vgpu_batch_set_positions")
+
+    def results(self, i):
+        L = lib()
+        n = C.c_longlong()
+        _check(L.vgpu_batch_num_cells(self._h, int(i), C.byref(n)), "vgpu_batch_num_cells")
+        n = n.value
+        r = {k: np.zeros(n, np.int32) for k in ("id", "ijk", "q", "exists", "nfaces")}
+        r["pos"] = np.zeros((n, 3))
+        r["vol"] = np.zeros(n)
+        r["cent"] = np.zeros((n, 3))
+        r["nbr_off"] = np.zeros(n + 1, np.int64)
+        tot = C.c_longlong()
+        _check(L.vgpu_batch_get_results(self._h, int(i), _p(r["id"], C.c_int), _p(r["ijk"], C.c_int),
+                                        _p(r["q"], C.c_int), _p(r["exists"], C.c_int), _p(r["pos"], C.c_double),
+                                        _p(r["vol"], C.c_double), _p(r["cent"], C.c_double),
+                                        _p(r["nfaces"], C.c_int), _p(r["nbr_off"], C.c_longlong), None, 0,
+                                        C.byref(tot)), "vgpu_batch_get_results")
+        nb = np.zeros(max(tot.value, 1), np.int32)
+        if self._flags & F_NEIGHBORS:
+            _check(L.vgpu_batch_get_results(self._h, int(i), None, None, None, None, None, None, None, None, None,
+                                            _p(nb, C.c_int), tot.value, C.byref(tot)), "vgpu_batch_get_results")
+        r["nbr"] = nb[:tot.value]
+        return r
+
+    def sum_cell_volumes(self):
+        out = np.zeros(self.count)
+        _check(lib().vgpu_batch_sum_volumes(self._h, _p(out, C.c_double)), "vgpu_batch_sum_volumes")
+        return out
+
+    @property
+    def config(self):
+        a = np.zeros(3, np.int32)
+        _check(lib().vgpu_batch_config(self._h, _p(a, C.c_int), None), "vgpu_batch_config")
+        return {"waves": int(a[0]), "occupancy": int(a[1]), "chunk": int(a[2])}
+
+    def set_config(self, waves=0, occupancy=2, chunk=64):
+        a = np.array([waves, occupancy, chunk], np.int32)
+        _check(lib().vgpu_batch_config(self._h, None, _p(a, C.c_int)), "vgpu_batch_config")
+
+    def autotune(self, flags=F_NEIGHBORS | F_CSR, reps=1):
+        """The project's autotuner: times each launch configuration on this
+        batch and keeps the fastest; returns it with its time (ms)."""
+        a = np.zeros(3, np.int32)
+        ms = C.c_double()
+        _check(lib().vgpu_batch_autotune(self._h, int(flags), int(reps), _p(a, C.c_int), C.byref(ms)),
+               "vgpu_batch_autotune")
+        return {"waves": int(a[0]), "occupancy": int(a[1]), "chunk": int(a[2]), "ms": ms.value}
+
+    def timings(self):
+        a = np.zeros(3)
+        _check(lib().vgpu_batch_last_timings(self._h, _p(a, C.c_double)), "vgpu_batch_last_timings")
+        return {"setup_ms": a[0], "first_ms": a[1], "retry_ms": a[2]}
 
 
 class predictor:

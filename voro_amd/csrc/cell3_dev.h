@@ -167,6 +167,10 @@ struct VCell3 {
 		wsb = slab + lo;
 		lo = 0;
 #endif
+		set_tol(max_len_sq);
+	}
+	// src/cell.cc:20-27: tol = 10 eps max_len_sq, tol_cu = tol^1.5, big_tol = 20 tol.
+	VG_INL void set_tol(double max_len_sq) {
 		const double eps = 2.220446049250313e-16;
 		tol = 10.0 * eps * max_len_sq;
 		tol_cu = tol * sqrt(tol);

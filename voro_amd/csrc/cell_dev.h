@@ -123,12 +123,15 @@ struct VCell {
 #if defined(__HIP_DEVICE_COMPILE__)
 		rs = __builtin_amdgcn_make_buffer_rsrc(slab, 0, (int)L::bytes, 0x00020000);
 #endif
+		set_tol(max_len_sq);
+		maskc = 0;
+		for (int v = 0; v < MAXV; v++) M(v) = 0;
+	}
+	VG_INL void set_tol(double max_len_sq) {
 		const double eps = 2.220446049250313e-16;
 		tol = 10.0 * eps * max_len_sq;
 		tol_cu = tol * sqrt(tol);
 		big_tol = 20.0 * tol;
-		maskc = 0;
-		for (int v = 0; v < MAXV; v++) M(v) = 0;
 	}
 
 	// -- capacity-checked containers -------------------------------------

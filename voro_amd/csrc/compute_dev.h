@@ -56,6 +56,9 @@ struct RectView {
 	const UnitVert *unit;   // the unit Voronoi cell
 	int unit_p;
 	unsigned long long *fv; // first visit of each block (null: not recorded)
+	// batches of containers: this container's first block in the batch's
+	// block arrays (bstart and cstart above already point at it)
+	int boff;
 };
 
 // Per-cell statistics (labels for the candidate predictor).

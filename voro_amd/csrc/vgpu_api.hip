@@ -127,14 +127,23 @@ namespace {
 int nblk(long long n, int t) { return (int)((n + t - 1) / t); }
 
 // Exclusive scan of n+1 ints (in[n] must be 0): out[n] = total.
-int scan_excl(vgpu_container *h, int *in, int *out, int n1, hipStream_t s) {
+// The partial-sum buffer is grown here unless `no_alloc` (graph capture:
+// the caller has sized it with scan_part_bytes).
+size_t scan_part_bytes(int n1) {
+	const int tiles = nblk(n1, 1024);
+	return 2 * ((size_t)tiles + (size_t)nblk(tiles, 1024) + 8) * sizeof(int);
+}
+int scan_excl(DBuf &part, int *in, int *out, int n1, hipStream_t s, bool no_alloc = false) {
 	int tiles = nblk(n1, 1024);
-	std::vector<int> sizes;
 	// part buffers for up to three levels
 	size_t need = (size_t)tiles + (size_t)nblk(tiles, 1024) + 8;
-	int st = h->part.ensure(2 * need * sizeof(int));
-	if (st) return st;
-	int *p1 = h->part.as<int>();
+	if (no_alloc) {
+		if (part.n < 2 * need * sizeof(int)) return VGPU_INTERNAL_ERROR;
+	} else {
+		int st = part.ensure(2 * need * sizeof(int));
+		if (st) return st;
+	}
+	int *p1 = part.as<int>();
 	int *p1s = p1 + need;
 	hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(256), 0, s, in, out, n1, p1);
 	if (tiles > 1) {
@@ -175,6 +184,7 @@ RectView make_view(vgpu_container *h) {
 	v.ey = pg.ey; v.ez = pg.ez; v.wy = pg.wy; v.wz = pg.wz; v.oy = pg.oy; v.oz = pg.oz;
 	v.unit = h->d_unit.as<UnitVert>(); v.unit_p = h->prd ? (int)pg.unit.size() : 0;
 	v.fv = nullptr;
+	v.boff = 0;
 	return v;
 }
 
@@ -263,7 +273,7 @@ int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut
 	if (launch_list_gather(s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(), h->d_depth.as<unsigned char>(),
 			       h->pred_with_r, h->lcnt.as<int>(), h->bmax.as<int>()))
 		return VGPU_HIP_ERROR;
-	if ((st = scan_excl(h, h->bmax.as<int>(), h->boff.as<int>(), nb + 1, s))) return st;
+	if ((st = scan_excl(h->part, h->bmax.as<int>(), h->boff.as<int>(), nb + 1, s))) return st;
 	int rows = 0;
 	HIPCHK(hipMemcpyAsync(&rows, h->boff.as<int>() + nb, 4, hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
@@ -427,7 +437,7 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 				   h->blk.as<int>(), h->slot.as<int>(), h->rem.as<dbl4>(), h->cnt.as<int>(),
 				   h->d_maxr.as<unsigned long long>());
 	}
-	if ((st = scan_excl(h, h->cnt.as<int>(), h->bstart.as<int>(), nxyz + 1, s))) return st;
+	if ((st = scan_excl(h->part, h->cnt.as<int>(), h->bstart.as<int>(), nxyz + 1, s))) return st;
 	if (n) {
 		hipLaunchKernelGGL(k_place, dim3(nblk(n, 256)), dim3(256), 0, s, n, h->blk.as<int>(), h->slot.as<int>(),
 				   h->bstart.as<int>(), h->tmp.as<int>());
@@ -519,7 +529,7 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 				   h->nfaces.as<int>(), h->csr_off.as<int>());
 		// scan in place into a second buffer region
 		if ((st = h->tmp.ensure((C1 + 1) * 4))) return st;
-		if ((st = scan_excl(h, h->csr_off.as<int>(), h->tmp.as<int>(), nc + 1, s))) return st;
+		if ((st = scan_excl(h->part, h->csr_off.as<int>(), h->tmp.as<int>(), nc + 1, s))) return st;
 		if ((st = h->csr.ensure((size_t)acap * 4))) return st;
 		hipLaunchKernelGGL(k_gather_nbrs, dim3(nblk(nc, 256)), dim3(256), 0, s, nc, h->status.as<int>(),
 				   h->nfaces.as<int>(), h->nbr_start.as<int>(), h->arena.as<int>(), h->tmp.as<int>(),
@@ -546,6 +556,244 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 	h->last_flags = flags;
 	if (hc[3] > 0) return VGPU_MEMORY_ERROR;
 	return VGPU_OK;
+}
+
+}  // namespace
+
+
+// ---------------------------------------------------------------------------
+// Batches of containers (vgpu_batch_*): many independent rectangular
+// containers computed in one pass over one set of device arrays.
+
+struct vgpu_batch {
+	int ncont = 0;
+	bool poly = false;
+	int n = 0;                       // particles put, over all containers
+	int nblocks = 0;
+	std::vector<int> h_boff, h_pfirst, h_nwalls;
+	std::vector<int> h_cfirst;       // first cell of each container (after a compute)
+	DBuf views, geoms, boff, pcont, pfirst, mrad, walls, d_wl, maxr, cfirst, nchunk, coff, chunks, nchunks, stamp;
+	DBuf in_ids, in_xyz, in_r;
+	DBuf blk, slot, rem, cnt, bstart, tmp, part, P, pid, src, cellblk;
+	DBuf vol, cent, nfaces, status, nbr_start, arena, csr_off, csr, ctr, counters, retry1, retry2;
+	DBuf st_planes, st_exit, st_tier;
+	Workspace ws_fast, ws_wide, ws_huge;
+	long long acap = 0;
+	int waves = 0, occ = 2, chunk = 64;  // launch configuration (autotuned)
+	unsigned long long stamp_used = 0;   // stamps handed out since the workspaces were cleared
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	unsigned graph_flags = 0;
+	hipStream_t gstream = nullptr;   // capture stream when the caller passes the null stream
+	hipEvent_t ev[4];
+	bool events = false;
+	double timings[3] = {0, 0, 0};
+	bool computed = false;
+	unsigned last_flags = 0;
+	long long nbr_total = 0;
+	int fails = 0;
+	void drop_graph() {
+		if (exec) (void)hipGraphExecDestroy(exec);
+		if (graph) (void)hipGraphDestroy(graph);
+		exec = nullptr; graph = nullptr;
+	}
+};
+
+namespace {
+
+__global__ void k_batch_pcont(int ncont, const int *__restrict__ pfirst, int *__restrict__ pcont) {
+	const int c = blockIdx.x;
+	if (c >= ncont) return;
+	for (int t = pfirst[c] + threadIdx.x; t < pfirst[c + 1]; t += blockDim.x) pcont[t] = c;
+}
+
+// Adds `inc` to the device stamp base (after each pass, so that the next
+// pass -- a graph replay included -- uses fresh block-mask stamps).
+__global__ void k_stamp_advance(unsigned *stamp, unsigned inc) {
+	if (threadIdx.x == 0 && blockIdx.x == 0) *stamp += inc;
+}
+
+// Stamp bases per tier are read on the device, so a captured graph keeps
+// working after replays.
+__global__ void k_batch_stamp_args(const unsigned *stamp, unsigned *args, int n) {
+	if (threadIdx.x == 0 && blockIdx.x == 0) for (int t = 0; t < 3; t++) args[t] = *stamp + (unsigned)t * (unsigned)(n + 1);
+}
+
+// Sizes every buffer a compute of this batch uses, so that the kernel
+// sequence itself allocates nothing (and can be captured).
+int batch_prepare(vgpu_batch *b, unsigned flags) {
+	const size_t N1 = (size_t)std::max(b->n, 1), NB1 = (size_t)b->nblocks + 1, C1 = (size_t)b->ncont + 1;
+	int st;
+	const bool stats = (flags & VGPU_F_STATS) != 0;
+	b->acap = std::min<long long>(2147483000LL, 24LL * (long long)N1 + 4096);
+	if ((st = b->blk.ensure(N1 * 4)) || (st = b->slot.ensure(N1 * 4)) || (st = b->rem.ensure(N1 * 32)) ||
+	    (st = b->cnt.ensure(NB1 * 4)) || (st = b->bstart.ensure(NB1 * 4)) || (st = b->tmp.ensure((N1 + 1) * 4)) ||
+	    (st = b->P.ensure(N1 * 32)) || (st = b->pid.ensure(N1 * 4)) || (st = b->src.ensure(N1 * 4)) ||
+	    (st = b->cellblk.ensure(N1 * 4)) || (st = b->maxr.ensure(C1 * 8)) || (st = b->cfirst.ensure(C1 * 4)) ||
+	    (st = b->nchunk.ensure(C1 * 4)) || (st = b->coff.ensure(C1 * 4)) ||
+	    (st = b->chunks.ensure((N1 + C1) * sizeof(BatchChunk))) || (st = b->nchunks.ensure(4)) ||
+	    (st = b->stamp.ensure(16 * 4)) || (st = b->vol.ensure(N1 * 8)) || (st = b->cent.ensure(N1 * 24)) ||
+	    (st = b->nfaces.ensure(N1 * 4)) || (st = b->status.ensure(N1 * 4)) || (st = b->nbr_start.ensure(N1 * 4)) ||
+	    (st = b->arena.ensure((size_t)b->acap * 4)) || (st = b->csr_off.ensure((N1 + 1) * 4)) ||
+	    (st = b->csr.ensure((size_t)b->acap * 4)) || (st = b->ctr.ensure(16 * 4)) || (st = b->counters.ensure(64 * 4)) ||
+	    (st = b->retry1.ensure(N1 * 4)) || (st = b->retry2.ensure(N1 * 4)))
+		return st;
+	const size_t pb = std::max(std::max(scan_part_bytes((int)NB1), scan_part_bytes((int)C1)), scan_part_bytes((int)N1 + 1));
+	if ((st = b->part.ensure(pb))) return st;
+	if (stats && ((st = b->st_planes.ensure(N1 * 2)) || (st = b->st_exit.ensure(N1)) || (st = b->st_tier.ensure(N1))))
+		return st;
+	int w = b->waves > 0 ? b->waves : default_fast_waves(b->occ);
+	w = std::max(1, std::min(w, nblk((long long)N1, 64) + b->ncont));
+	b->waves = w;
+	typedef Layout<TierWide, 64> LW;
+	typedef Layout<TierHuge, 64> LH;
+	if ((st = b->ws_fast.ensure(VCell3::L::bytes, w)) || (st = b->ws_wide.ensure(LW::bytes, 64)) ||
+	    (st = b->ws_huge.ensure(LH::bytes, 4)))
+		return st;
+	// stamps: three passes of n+1 per compute; clear the workspaces before
+	// the 32-bit stamp space runs out
+	const unsigned long long per = 3ull * ((unsigned long long)N1 + 1) + 3;
+	if (b->stamp_used + per >= 0xffffffffull || b->stamp_used == 0) {
+		Workspace *wss[3] = {&b->ws_fast, &b->ws_wide, &b->ws_huge};
+		for (Workspace *x : wss) HIPCHK(hipMemset(x->buf.p, 0, x->buf.n));
+		HIPCHK(hipMemset(b->stamp.p, 0, 16 * 4));
+		b->stamp_used = 1;
+		b->drop_graph();   // the graph's stamps restart from zero with the workspace
+	}
+	if (!b->events) {
+		for (int t = 0; t < 4; t++) HIPCHK(hipEventCreate(&b->ev[t]));
+		b->events = true;
+	}
+	return VGPU_OK;
+}
+
+// The kernel sequence of one batch compute (no allocation, no host sync).
+int batch_enqueue(vgpu_batch *b, unsigned flags, hipStream_t s, bool timing) {
+	int st;
+	const int n = b->n, nb = b->nblocks, nc = b->ncont;
+	const bool stats = (flags & VGPU_F_STATS) != 0, nbrs = (flags & VGPU_F_NEIGHBORS) != 0;
+	if (timing) HIPCHK(hipEventRecord(b->ev[0], s));
+	hipLaunchKernelGGL(k_fill_int, dim3(nblk(nb + 1, 256)), dim3(256), 0, s, b->cnt.as<int>(), nb + 1, 0);
+	hipLaunchKernelGGL(k_fill_u64, dim3(nblk(nc, 256)), dim3(256), 0, s, b->maxr.as<unsigned long long>(), nc, 0ull);
+	if (launch_batch_locate(s, n, b->in_xyz.as<double>(), b->poly ? b->in_r.as<double>() : nullptr,
+				b->geoms.as<RectGeom>(), b->pcont.as<int>(), b->boff.as<int>(), b->blk.as<int>(),
+				b->slot.as<int>(), b->rem.as<dbl4>(), b->cnt.as<int>(), b->maxr.as<unsigned long long>()))
+		return VGPU_HIP_ERROR;
+	if ((st = scan_excl(b->part, b->cnt.as<int>(), b->bstart.as<int>(), nb + 1, s, true))) return st;
+	if (n) {
+		hipLaunchKernelGGL(k_place, dim3(nblk(n, 256)), dim3(256), 0, s, n, b->blk.as<int>(), b->slot.as<int>(),
+				   b->bstart.as<int>(), b->tmp.as<int>());
+		hipLaunchKernelGGL(k_rank, dim3(nblk(n, 256)), dim3(256), 0, s, n, b->blk.as<int>(), b->bstart.as<int>(),
+				   b->tmp.as<int>(), b->rem.as<dbl4>(), b->in_ids.as<int>(), b->P.as<dbl4>(), b->pid.as<int>(),
+				   b->src.as<int>(), b->cellblk.as<int>());
+	}
+	if (launch_batch_counts(s, nc, b->chunk, b->boff.as<int>(), b->bstart.as<int>(), b->maxr.as<unsigned long long>(),
+				b->poly ? 1 : 0, b->views.as<RectView>(), b->cfirst.as<int>(), b->nchunk.as<int>()))
+		return VGPU_HIP_ERROR;
+	if ((st = scan_excl(b->part, b->nchunk.as<int>(), b->coff.as<int>(), nc + 1, s, true))) return st;
+	if (launch_batch_chunks(s, nc, b->chunk, b->cfirst.as<int>(), b->coff.as<int>(), b->chunks.as<BatchChunk>(),
+				b->nchunks.as<int>()))
+		return VGPU_HIP_ERROR;
+	int *ctr = b->ctr.as<int>();
+	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 16, 0);
+	if (timing) HIPCHK(hipEventRecord(b->ev[1], s));
+	CellOut out;
+	out.vol = b->vol.as<double>(); out.cent = b->cent.as<double>(); out.nfaces = b->nfaces.as<int>();
+	out.status = b->status.as<int>(); out.nbr_start = b->nbr_start.as<int>(); out.nbr = b->arena.as<int>();
+	out.arena_top = ctr; out.arena_cap = (int)b->acap; out.fail_count = ctr + 3;
+	out.st_planes = stats ? b->st_planes.as<short>() : nullptr;
+	out.st_exit = stats ? b->st_exit.as<signed char>() : nullptr;
+	out.st_tier = stats ? b->st_tier.as<signed char>() : nullptr;
+	out.touch = nullptr;
+	out.want_nbrs = nbrs ? 1 : 0;
+	// per-tier stamp bases from the device counter (ctr[8..10])
+	hipLaunchKernelGGL(k_batch_stamp_args, dim3(1), dim3(64), 0, s, b->stamp.as<unsigned>(), (unsigned *)(ctr + 8), n);
+	int *cnt8 = b->counters.as<int>();
+	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, cnt8, 8, 0);
+	out.tier = 0; out.retry = b->retry1.as<int>(); out.retry_count = ctr + 1;
+	if (launch_batch_first(b->poly, b->occ, b->waves, s, b->views.as<RectView>(), b->chunks.as<BatchChunk>(),
+			       b->nchunks.as<int>(), b->cellblk.as<int>(), cnt8, b->ws_fast.base(), b->ws_fast.slab,
+			       (const unsigned *)(ctr + 8), out))
+		return VGPU_HIP_ERROR;
+	if (timing) HIPCHK(hipEventRecord(b->ev[2], s));
+	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, cnt8, 8, 0);
+	out.tier = 2; out.retry = b->retry2.as<int>(); out.retry_count = ctr + 2;
+	if (launch_batch_retry(1, b->poly, 64, s, b->views.as<RectView>(), b->boff.as<int>(), nc, b->retry1.as<int>(),
+			       ctr + 1, b->cellblk.as<int>(), cnt8, b->ws_wide.base(), b->ws_wide.slab,
+			       (const unsigned *)(ctr + 9), out))
+		return VGPU_HIP_ERROR;
+	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, cnt8, 8, 0);
+	out.tier = 3; out.retry = nullptr; out.retry_count = nullptr;
+	if (launch_batch_retry(2, b->poly, 4, s, b->views.as<RectView>(), b->boff.as<int>(), nc, b->retry2.as<int>(),
+			       ctr + 2, b->cellblk.as<int>(), cnt8, b->ws_huge.base(), b->ws_huge.slab,
+			       (const unsigned *)(ctr + 10), out))
+		return VGPU_HIP_ERROR;
+	hipLaunchKernelGGL(k_stamp_advance, dim3(1), dim3(64), 0, s, b->stamp.as<unsigned>(), 3u * (unsigned)(n + 1) + 3u);
+	if (nbrs && n > 0) {
+		// cells: bstart[nblocks] (particles outside their box are not cells)
+		const int *ncells = b->bstart.as<int>() + nb;
+		hipLaunchKernelGGL(k_face_counts_dev, dim3(nblk(n + 1, 256)), dim3(256), 0, s, n, ncells, b->status.as<int>(),
+				   b->nfaces.as<int>(), b->csr_off.as<int>());
+		if ((st = scan_excl(b->part, b->csr_off.as<int>(), b->tmp.as<int>(), n + 1, s, true))) return st;
+		hipLaunchKernelGGL(k_gather_nbrs_dev, dim3(nblk(n, 256)), dim3(256), 0, s, n, ncells, b->status.as<int>(),
+				   b->nfaces.as<int>(), b->nbr_start.as<int>(), b->arena.as<int>(), b->tmp.as<int>(),
+				   b->csr.as<int>());
+	}
+	if (timing) HIPCHK(hipEventRecord(b->ev[3], s));
+	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
+}
+
+int batch_run(vgpu_batch *b, unsigned flags, hipStream_t s) {
+	int st;
+	if ((st = batch_prepare(b, flags))) return st;
+	const bool graph = (flags & VGPU_F_GRAPH) != 0;
+	if (graph && s == nullptr) {
+		// the null stream cannot be captured: use the batch's own stream
+		if (!b->gstream) HIPCHK(hipStreamCreateWithFlags(&b->gstream, hipStreamNonBlocking));
+		HIPCHK(hipDeviceSynchronize());
+		s = b->gstream;
+	}
+	if (graph) {
+		const unsigned key = flags & ~VGPU_F_GRAPH;
+		if (b->exec && b->graph_flags != key) b->drop_graph();
+		if (!b->exec) {
+			HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+			int est = batch_enqueue(b, key, s, false);
+			hipGraph_t g = nullptr;
+			hipError_t ce = hipStreamEndCapture(s, &g);
+			if (est) { if (g) (void)hipGraphDestroy(g); return est; }
+			if (ce != hipSuccess) return VGPU_HIP_ERROR;
+			b->graph = g;
+			HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+			b->graph_flags = key;
+		}
+		HIPCHK(hipEventRecord(b->ev[0], s));
+		HIPCHK(hipGraphLaunch(b->exec, s));
+		HIPCHK(hipEventRecord(b->ev[3], s));
+	} else {
+		if ((st = batch_enqueue(b, flags, s, true))) return st;
+	}
+	b->stamp_used += 3ull * ((unsigned long long)std::max(b->n, 1) + 1) + 3;
+	int hc[8];
+	HIPCHK(hipMemcpyAsync(hc, b->ctr.p, 32, hipMemcpyDeviceToHost, s));
+	b->h_cfirst.resize(b->ncont + 1);
+	HIPCHK(hipMemcpyAsync(b->h_cfirst.data(), b->cfirst.p, (size_t)(b->ncont + 1) * 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
+	float ms = 0;
+	if (graph) {
+		HIPCHK(hipEventElapsedTime(&ms, b->ev[0], b->ev[3]));
+		b->timings[0] = 0; b->timings[1] = ms; b->timings[2] = 0;
+	} else {
+		for (int t = 0; t < 3; t++) {
+			HIPCHK(hipEventElapsedTime(&ms, b->ev[t], b->ev[t + 1]));
+			b->timings[t] = ms;
+		}
+	}
+	b->nbr_total = hc[0];
+	b->fails = hc[3];
+	b->computed = true;
+	b->last_flags = flags;
+	return hc[3] > 0 ? VGPU_MEMORY_ERROR : VGPU_OK;
 }
 
 }  // namespace
@@ -933,6 +1181,247 @@ int vgpu_predictor_clear_samples(vgpu_predictor *p) {
 	if (!p) return VGPU_INVALID_ARGUMENT;
 	p->X.clear();
 	p->T.clear();
+	return VGPU_OK;
+}
+
+
+// ---- batches of containers ----
+
+int vgpu_batch_create(vgpu_container *const *hs, int count, vgpu_batch **out) {
+	if (!out || count <= 0 || !hs) return VGPU_INVALID_ARGUMENT;
+	*out = nullptr;
+	const int kind = hs[0] ? hs[0]->kind : -1;
+	if (kind != VGPU_CONTAINER && kind != VGPU_CONTAINER_POLY) return VGPU_INVALID_ARGUMENT;
+	long long n = 0, nb = 0, nw = 0;
+	for (int c = 0; c < count; c++) {
+		if (!hs[c] || hs[c]->kind != kind) return VGPU_INVALID_ARGUMENT;
+		n += hs[c]->n_in;
+		nb += hs[c]->nblocks;
+		nw += (long long)hs[c]->walls.size() / 5;
+	}
+	if (n > 2000000000LL / 32 || nb + 1 > 2000000000LL) return VGPU_MEMORY_ERROR;
+	vgpu_batch *b = new vgpu_batch();
+	b->ncont = count;
+	b->poly = kind == VGPU_CONTAINER_POLY;
+	b->n = (int)n;
+	b->nblocks = (int)nb;
+	b->h_boff.assign(count + 1, 0);
+	b->h_pfirst.assign(count + 1, 0);
+	std::vector<int> woff(count + 1, 0);
+	for (int c = 0; c < count; c++) {
+		b->h_boff[c + 1] = b->h_boff[c] + hs[c]->nblocks;
+		b->h_pfirst[c + 1] = b->h_pfirst[c] + hs[c]->n_in;
+		woff[c + 1] = woff[c] + (int)hs[c]->walls.size() / 5;
+	}
+	int st = VGPU_OK;
+	auto fail = [&](int e) { vgpu_batch_destroy(b); return e; };
+	const size_t N1 = (size_t)std::max(b->n, 1), C1 = (size_t)count + 1;
+	if ((st = b->in_ids.ensure(N1 * 4)) || (st = b->in_xyz.ensure(N1 * 24)) || (st = b->in_r.ensure(N1 * 8)) ||
+	    (st = b->geoms.ensure(count * sizeof(RectGeom))) || (st = b->boff.ensure(C1 * 4)) ||
+	    (st = b->pfirst.ensure(C1 * 4)) || (st = b->pcont.ensure(N1 * 4)) ||
+	    (st = b->mrad.ensure((size_t)count * 4096 * 8)) || (st = b->d_wl.ensure(4096 * 4)) ||
+	    (st = b->walls.ensure((size_t)std::max<long long>(nw, 1) * 5 * 8)) ||
+	    (st = b->views.ensure(count * sizeof(RectView))))
+		return fail(st);
+	// particles, in container order
+	std::vector<int> ids((size_t)b->n);
+	std::vector<double> xyz((size_t)b->n * 3), r(b->poly ? (size_t)b->n : 0), wall((size_t)nw * 5);
+	std::vector<RectGeom> geo(count);
+	for (int c = 0; c < count; c++) {
+		vgpu_container *h = hs[c];
+		const size_t o = (size_t)b->h_pfirst[c], m = (size_t)h->n_in;
+		if (h->have_dev_input) {
+			if (m && (hipMemcpy(ids.data() + o, h->in_ids.p, m * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+				  hipMemcpy(xyz.data() + 3 * o, h->in_xyz.p, m * 24, hipMemcpyDeviceToHost) != hipSuccess ||
+				  (b->poly && hipMemcpy(r.data() + o, h->in_r.p, m * 8, hipMemcpyDeviceToHost) != hipSuccess)))
+				return fail(VGPU_HIP_ERROR);
+		} else if (m) {
+			std::memcpy(ids.data() + o, h->h_ids.data(), m * 4);
+			std::memcpy(xyz.data() + 3 * o, h->h_xyz.data(), m * 24);
+			if (b->poly) std::memcpy(r.data() + o, h->h_r.data(), m * 8);
+		}
+		if (!h->walls.empty()) std::memcpy(wall.data() + (size_t)woff[c] * 5, h->walls.data(), h->walls.size() * 8);
+		geo[c] = h->g;
+	}
+	if ((b->n && (hipMemcpy(b->in_ids.p, ids.data(), (size_t)b->n * 4, hipMemcpyHostToDevice) != hipSuccess ||
+		      hipMemcpy(b->in_xyz.p, xyz.data(), (size_t)b->n * 24, hipMemcpyHostToDevice) != hipSuccess ||
+		      (b->poly && hipMemcpy(b->in_r.p, r.data(), (size_t)b->n * 8, hipMemcpyHostToDevice) != hipSuccess))) ||
+	    (nw && hipMemcpy(b->walls.p, wall.data(), wall.size() * 8, hipMemcpyHostToDevice) != hipSuccess) ||
+	    hipMemcpy(b->geoms.p, geo.data(), count * sizeof(RectGeom), hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(b->boff.p, b->h_boff.data(), C1 * 4, hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(b->pfirst.p, b->h_pfirst.data(), C1 * 4, hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(b->d_wl.p, vgpu_worklist_table, 4096 * 4, hipMemcpyHostToDevice) != hipSuccess)
+		return fail(VGPU_HIP_ERROR);
+	hipLaunchKernelGGL(k_batch_pcont, dim3(count), dim3(256), 0, nullptr, count, b->pfirst.as<int>(), b->pcont.as<int>());
+	if (launch_batch_mrad(nullptr, count, b->geoms.as<RectGeom>(), b->d_wl.as<unsigned>(), b->mrad.as<double>()))
+		return fail(VGPU_HIP_ERROR);
+	// buffers (their addresses go into the views)
+	if ((st = batch_prepare(b, VGPU_F_NEIGHBORS))) return fail(st);
+	std::vector<RectView> vw(count);
+	for (int c = 0; c < count; c++) {
+		vgpu_container *h = hs[c];
+		RectView v = make_view(h);
+		v.P = b->P.as<dbl4>(); v.pid = b->pid.as<int>();
+		v.bstart = b->bstart.as<int>() + b->h_boff[c]; v.cstart = v.bstart;
+		v.mrad = b->mrad.as<double>() + (size_t)c * 4096; v.wl = b->d_wl.as<unsigned>();
+		v.walls = b->walls.as<double>() + (size_t)woff[c] * 5; v.nwalls = woff[c + 1] - woff[c];
+		v.max_radius = 0;
+		v.unit = nullptr; v.unit_p = 0; v.fv = nullptr;
+		v.boff = b->h_boff[c];
+		vw[c] = v;
+	}
+	if (hipMemcpy(b->views.p, vw.data(), count * sizeof(RectView), hipMemcpyHostToDevice) != hipSuccess ||
+	    hipDeviceSynchronize() != hipSuccess)
+		return fail(VGPU_HIP_ERROR);
+	*out = b;
+	return VGPU_OK;
+}
+
+int vgpu_batch_destroy(vgpu_batch *b) {
+	if (!b) return VGPU_INVALID_ARGUMENT;
+	b->drop_graph();
+	if (b->gstream) (void)hipStreamDestroy(b->gstream);
+	DBuf *bufs[] = {&b->views, &b->geoms, &b->boff, &b->pcont, &b->pfirst, &b->mrad, &b->walls, &b->d_wl, &b->maxr,
+			&b->cfirst, &b->nchunk, &b->coff, &b->chunks, &b->nchunks, &b->stamp, &b->in_ids, &b->in_xyz,
+			&b->in_r, &b->blk, &b->slot, &b->rem, &b->cnt, &b->bstart, &b->tmp, &b->part, &b->P, &b->pid,
+			&b->src, &b->cellblk, &b->vol, &b->cent, &b->nfaces, &b->status, &b->nbr_start, &b->arena,
+			&b->csr_off, &b->csr, &b->ctr, &b->counters, &b->retry1, &b->retry2, &b->st_planes, &b->st_exit,
+			&b->st_tier};
+	for (DBuf *x : bufs) x->release();
+	b->ws_fast.buf.release(); b->ws_wide.buf.release(); b->ws_huge.buf.release();
+	if (b->events) for (int t = 0; t < 4; t++) (void)hipEventDestroy(b->ev[t]);
+	delete b;
+	return VGPU_OK;
+}
+
+int vgpu_batch_compute(vgpu_batch *b, unsigned flags, void *stream) {
+	if (!b || (flags & VGPU_F_LEARNED)) return VGPU_INVALID_ARGUMENT;
+	return batch_run(b, flags, (hipStream_t)stream);
+}
+
+int vgpu_batch_set_positions(vgpu_batch *b, const double *xyz, int on_device) {
+	if (!b || (!xyz && b->n > 0)) return VGPU_INVALID_ARGUMENT;
+	if (b->n == 0) return VGPU_OK;
+	HIPCHK(hipMemcpy(b->in_xyz.p, xyz, (size_t)b->n * 24, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+	b->computed = false;
+	return VGPU_OK;
+}
+
+int vgpu_batch_num_cells(vgpu_batch *b, int index, long long *n) {
+	if (!b || !n || !b->computed || index < 0 || index >= b->ncont) return VGPU_INVALID_ARGUMENT;
+	*n = b->h_cfirst[index + 1] - b->h_cfirst[index];
+	return VGPU_OK;
+}
+
+int vgpu_batch_get_results(vgpu_batch *b, int index, int *id, int *ijk, int *q, int *exists, double *pos, double *vol,
+			   double *centroid, int *nfaces, long long *nbr_off, int *nbr, long long nbr_cap,
+			   long long *nbr_total) {
+	if (!b || !b->computed || index < 0 || index >= b->ncont) return VGPU_INVALID_ARGUMENT;
+	const int cf = b->h_cfirst[index], nc = b->h_cfirst[index + 1] - cf;
+	if (nc == 0) {
+		if (nbr_off) nbr_off[0] = 0;
+		if (nbr_total) *nbr_total = 0;
+		return VGPU_OK;
+	}
+	std::vector<int> st(nc), nf(nc);
+	HIPCHK(hipMemcpy(st.data(), b->status.as<int>() + cf, (size_t)nc * 4, hipMemcpyDeviceToHost));
+	HIPCHK(hipMemcpy(nf.data(), b->nfaces.as<int>() + cf, (size_t)nc * 4, hipMemcpyDeviceToHost));
+	if (id) HIPCHK(hipMemcpy(id, b->pid.as<int>() + cf, (size_t)nc * 4, hipMemcpyDeviceToHost));
+	if (ijk || q) {
+		const int b0 = b->h_boff[index], nbk = b->h_boff[index + 1] - b0;
+		std::vector<int> cb(nc), bs(nbk + 1);
+		HIPCHK(hipMemcpy(cb.data(), b->cellblk.as<int>() + cf, (size_t)nc * 4, hipMemcpyDeviceToHost));
+		HIPCHK(hipMemcpy(bs.data(), b->bstart.as<int>() + b0, (size_t)(nbk + 1) * 4, hipMemcpyDeviceToHost));
+		for (int c = 0; c < nc; c++) {
+			if (ijk) ijk[c] = cb[c] - b0;
+			if (q) q[c] = cf + c - bs[cb[c] - b0];
+		}
+	}
+	if (exists) for (int c = 0; c < nc; c++) exists[c] = (st[c] & ST_EXISTS) ? 1 : 0;
+	if (pos) {
+		std::vector<dbl4> P(nc);
+		HIPCHK(hipMemcpy(P.data(), b->P.as<dbl4>() + cf, (size_t)nc * 32, hipMemcpyDeviceToHost));
+		for (int c = 0; c < nc; c++) { pos[3 * c] = P[c].x; pos[3 * c + 1] = P[c].y; pos[3 * c + 2] = P[c].z; }
+	}
+	if (vol) HIPCHK(hipMemcpy(vol, b->vol.as<double>() + cf, (size_t)nc * 8, hipMemcpyDeviceToHost));
+	if (centroid) HIPCHK(hipMemcpy(centroid, b->cent.as<double>() + (size_t)3 * cf, (size_t)nc * 24, hipMemcpyDeviceToHost));
+	if (nfaces) for (int c = 0; c < nc; c++) nfaces[c] = (st[c] & ST_EXISTS) ? nf[c] : 0;
+	long long tot = 0;
+	std::vector<long long> off(nc + 1);
+	for (int c = 0; c < nc; c++) { off[c] = tot; if (st[c] & ST_EXISTS) tot += nf[c]; }
+	off[nc] = tot;
+	if (nbr_off) std::memcpy(nbr_off, off.data(), (size_t)(nc + 1) * 8);
+	if (nbr_total) *nbr_total = tot;
+	if (nbr && (b->last_flags & VGPU_F_NEIGHBORS) && tot > 0) {
+		int g0 = 0;
+		HIPCHK(hipMemcpy(&g0, b->tmp.as<int>() + cf, 4, hipMemcpyDeviceToHost));
+		const long long m = std::min(tot, nbr_cap);
+		if (m > 0) HIPCHK(hipMemcpy(nbr, b->csr.as<int>() + g0, (size_t)m * 4, hipMemcpyDeviceToHost));
+	}
+	return VGPU_OK;
+}
+
+int vgpu_batch_sum_volumes(vgpu_batch *b, double *out) {
+	if (!b || !out || !b->computed) return VGPU_INVALID_ARGUMENT;
+	const int n = b->h_cfirst[b->ncont];
+	std::vector<double> v(std::max(n, 1));
+	std::vector<int> st(std::max(n, 1));
+	if (n) {
+		HIPCHK(hipMemcpy(v.data(), b->vol.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+		HIPCHK(hipMemcpy(st.data(), b->status.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+	}
+	for (int c = 0; c < b->ncont; c++) {
+		double s = 0;
+		for (int t = b->h_cfirst[c]; t < b->h_cfirst[c + 1]; t++) if (st[t] & ST_EXISTS) s += v[t];
+		out[c] = s;
+	}
+	return VGPU_OK;
+}
+
+int vgpu_batch_config(vgpu_batch *b, int *get3, const int *set3) {
+	if (!b) return VGPU_INVALID_ARGUMENT;
+	if (get3) { get3[0] = b->waves; get3[1] = b->occ; get3[2] = b->chunk; }
+	if (set3) {
+		if (set3[0] < 0 || set3[1] < 2 || set3[1] > 4 || set3[2] < 1 || set3[2] > 64) return VGPU_INVALID_ARGUMENT;
+		b->waves = set3[0]; b->occ = set3[1]; b->chunk = set3[2];
+		b->drop_graph();
+	}
+	return VGPU_OK;
+}
+
+// The project's autotuner for batches: times the compute (eager, `reps`
+// runs each, best kept) for every register budget of the first tier (2, 3,
+// 4 waves per SIMD, with as many persistent wavefronts as fit) and chunk
+// length (64 or 32 cells per queue entry), and keeps the fastest.  Results
+// do not depend on the choice.
+int vgpu_batch_autotune(vgpu_batch *b, unsigned flags, int reps, int *best3, double *best_ms) {
+	if (!b || reps < 1 || (flags & (VGPU_F_LEARNED | VGPU_F_GRAPH))) return VGPU_INVALID_ARGUMENT;
+	const int occs[3] = {2, 3, 4}, chunks[2] = {64, 32};
+	double best = 1e300;
+	int bo = 2, bc = 64;
+	for (int oi = 0; oi < 3; oi++)
+		for (int ci = 0; ci < 2; ci++) {
+			const int cfg[3] = {0, occs[oi], chunks[ci]};
+			int st = vgpu_batch_config(b, nullptr, cfg);
+			if (st) return st;
+			double t = 1e300;
+			for (int r = 0; r < reps; r++) {
+				if ((st = batch_run(b, flags, nullptr))) return st;
+				t = std::min(t, b->timings[0] + b->timings[1] + b->timings[2]);
+			}
+			if (t < best) { best = t; bo = occs[oi]; bc = chunks[ci]; }
+		}
+	const int cfg[3] = {0, bo, bc};
+	int st = vgpu_batch_config(b, nullptr, cfg);
+	if (st) return st;
+	if (best3) { best3[0] = default_fast_waves(bo); best3[1] = bo; best3[2] = bc; }
+	if (best_ms) *best_ms = best;
+	return VGPU_OK;
+}
+
+int vgpu_batch_last_timings(vgpu_batch *b, double *ms3) {
+	if (!b || !ms3) return VGPU_INVALID_ARGUMENT;
+	for (int t = 0; t < 3; t++) ms3[t] = b->timings[t];
 	return VGPU_OK;
 }
 

@@ -1,20 +1,9 @@
 // Binning (stable counting sort by block), scans and neighbour compaction
 // kernels.  Included by the API translation unit only.
 #pragma once
-#include "vgpu_cells.h"
+#include "batch_dev.h"
 
 namespace vgpu {
-
-// Geometry needed to bin particles of a rectangular container.
-struct RectGeom {
-	double ax, bx, ay, by, az, bz;
-	int nx, ny, nz;
-	double boxx, boxy, boxz, xsp, ysp, zsp;
-	int xper, yper, zper;
-};
-
-__device__ __forceinline__ int step_int_d(double a) { return a < 0 ? int(a) - 1 : int(a); }
-__device__ __forceinline__ int step_mod_d(int a, int b) { return a >= 0 ? a % b : b - 1 - (b - 1 - a) % b; }
 
 // Block index of every particle, as container_base::put_remap computes it
 // (src/container.cc:174-191), the position remapped into the primary domain
@@ -27,25 +16,7 @@ __global__ void k_locate(int n, const double *__restrict__ xyz, const double *__
 	int t = blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= n) return;
 	double x = xyz[3 * t], y = xyz[3 * t + 1], z = xyz[3 * t + 2];
-	int b = -1;
-	const double lim = 1073741824.0;
-	double ax_ = (x - g.ax) * g.xsp, ay_ = (y - g.ay) * g.ysp, az_ = (z - g.az) * g.zsp;
-	// Non-finite or huge coordinates would make the int conversion undefined:
-	// they are outside any box, so they are dropped.
-	bool finite = fabs(ax_) < lim && fabs(ay_) < lim && fabs(az_) < lim;
-	if (finite) {
-		int i = step_int_d(ax_), j, k, l;
-		bool ok = true;
-		if (g.xper) { l = step_mod_d(i, g.nx); x += g.boxx * (l - i); i = l; }
-		else if (i < 0 || i >= g.nx) ok = false;
-		j = step_int_d(ay_);
-		if (g.yper) { l = step_mod_d(j, g.ny); y += g.boxy * (l - j); j = l; }
-		else if (j < 0 || j >= g.ny) ok = false;
-		k = step_int_d(az_);
-		if (g.zper) { l = step_mod_d(k, g.nz); z += g.boxz * (l - k); k = l; }
-		else if (k < 0 || k >= g.nz) ok = false;
-		if (ok) b = i + g.nx * j + g.nx * g.ny * k;
-	}
+	const int b = locate_rect(g, x, y, z);
 	blk[t] = b;
 	double rr = r ? r[t] : 0.5;
 	rem[t] = dbl4{x, y, z, rr};
@@ -186,6 +157,24 @@ __global__ void k_rank(int n, const int *__restrict__ blk, const int *__restrict
 	pid[d] = ids[t];
 	src[d] = t;
 	cellblk[d] = b;
+}
+
+// Batches: the same two steps with the cell count read on the device
+// (*ncells <= n; entries from *ncells on count zero faces).
+__global__ void k_face_counts_dev(int n, const int *__restrict__ ncells, const int *__restrict__ status,
+				  const int *__restrict__ nfaces, int *__restrict__ cnt) {
+	int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c > n) return;
+	cnt[c] = (c < *ncells && (status[c] & ST_EXISTS)) ? nfaces[c] : 0;
+}
+__global__ void k_gather_nbrs_dev(int n, const int *__restrict__ ncells, const int *__restrict__ status,
+				  const int *__restrict__ nfaces, const int *__restrict__ start,
+				  const int *__restrict__ arena, const int *__restrict__ off, int *__restrict__ csr) {
+	int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= n || c >= *ncells) return;
+	if (!(status[c] & ST_EXISTS)) return;
+	int s = start[c], o = off[c], k = nfaces[c];
+	for (int t = 0; t < k; t++) csr[o + t] = arena[s + t];
 }
 
 // Loop-order CSR of the neighbour lists: out[off[c] ..] = arena[start[c] ..].

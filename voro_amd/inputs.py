@@ -112,6 +112,37 @@ def fcc_sphere(seed, n_target, spacing=1.0, jitter=0.05):
     return pts[keep], L
 
 
+def fcc_batch(seed, count, n_lo=100, n_hi=20000, spacing=1.0):
+    """The batched workload: `count` independent FCC-jitter containers.
+    Container c targets N_c = round(exp(U[ln n_lo, ln n_hi])) particles: an
+    FCC lattice of spacing a with m = round(cbrt(N_c / (4 * 0.3817))) cells a
+    side (L = m a; the wall sphere keeps ~0.3817 of the cube), Gaussian
+    jitter of sigma ~ U[0.02, 0.2] a, kept inside the wall sphere of radius
+    0.45 L at the box centre; grid round(cbrt(N/5))^3 for the N kept.
+    Returns a list of (L, grid, xyz)."""
+    lnN = np.log(n_lo) + (np.log(n_hi) - np.log(n_lo)) * uniform(seed, 200, count)
+    sig = 0.02 + 0.18 * uniform(seed, 201, count)
+    out = []
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]]) + 0.25
+    for c in range(count):
+        nt = int(round(np.exp(lnN[c])))
+        m = max(2, int(round((nt / (4 * 0.3817)) ** (1.0 / 3.0))))
+        L = m * spacing
+        ii, jj, kk = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+        cells = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], axis=1).astype(np.float64)
+        pts = (cells[:, None, :] + base[None, :, :]).reshape(-1, 3) * spacing
+        k = len(pts)
+        for a in range(3):
+            pts[:, a] += sig[c] * spacing * normal(seed * 100003 + c, 300 + 2 * a, k)
+        cen = 0.5 * L
+        rr = np.sqrt(((pts - cen) ** 2).sum(axis=1))
+        keep = (rr < 0.45 * L) & np.all((pts >= 0) & (pts < L), axis=1)
+        pts = pts[keep]
+        g = max(1, int(round((len(pts) / 5.0) ** (1.0 / 3.0))))
+        out.append((L, g, pts))
+    return out
+
+
 WORKLOADS = {
     "unit_cube": dict(kind=0, box=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), grid=(6, 6, 6), n=1000),
     "uniform": dict(kind=0, box=(0.0, 100.0, 0.0, 100.0, 0.0, 100.0), grid=(58, 58, 58), n=1_000_000),
