@@ -29,9 +29,14 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Per-cell algorithmic work of the cell kernel ("Measurement"):
-FLOP_PER_CELL = 7.6e3          # FP64 operations per cell (uniform workload)
-BYTES_PER_CELL_FIXED = 36 + 44  # own particle record in + fixed outputs out
+# Per-cell algorithmic work of the cell kernel on the uniform workload:
+# 7.6 kFLOP FP64 per cell (operation counts of the reference algorithm: plane
+# tests, new vertices, radius refreshes, region tests, output walks), and
+# compulsory HBM bytes: the own particle record and id in (36 B), the fixed
+# outputs out (volume 8, centroid 24, face count 4, status 4, neighbour
+# start 4 = 44 B), plus 4 B per neighbour id.
+FLOP_PER_CELL = 7.6e3
+BYTES_PER_CELL_FIXED = 36 + 44
 MI355X_HBM_GBS = 8000.0
 MI355X_FP64_TFLOPS = 78.6
 
@@ -52,6 +57,11 @@ def main():
     ap.add_argument("--epochs", type=int, default=400, help="predictor training steps (full-batch Adam)")
     ap.add_argument("--fallback", type=float, default=0.02,
                     help="fraction of training blocks the depth margin allows to be under-predicted")
+    ap.add_argument("--workload", choices=["uniform", "periodic_poly", "batch"], default="uniform",
+                    help="uniform: the headline measurement (default); periodic_poly / batch: only that "
+                         "secondary workload, for profiling")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="skip the secondary workloads (periodic radical 2M, batch of 4096 containers)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (scale for the FETCH_SIZE and "
                          "WRITE_SIZE counters) and exit")
@@ -75,6 +85,11 @@ def main():
 
     import voro_amd
     from voro_amd import inputs as I
+
+    if args.workload != "uniform":
+        args.no_cpu = True
+        print(json.dumps(secondary_workloads(args, torch, only=args.workload)))
+        return
 
     ids, xyz, _ = I.make("uniform", 42, n=args.n)
     dev = torch.device("cuda:0")
@@ -159,6 +174,8 @@ def main():
             cpu = {"value": m / secs, "unit": "cells/s", "cores": 1, "kind": "reference",
                    "sample": "container::compute_all_cells() on %d of the uniform particles, %.1f s" % (m, secs)}
 
+    others = None if args.headline_only else secondary_workloads(args, torch)
+
     line = {
         "metric": "cells/s at 1M uniform pts, learned vs plain GPU path (bit-exact nbrs, 1e-12 vol)",
         "value": cells_s,
@@ -186,8 +203,103 @@ def main():
         "roofline_fp64": {"achieved": fp64_tf, "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
                           "frac": fp64_tf / MI355X_FP64_TFLOPS},
         "cpu_baseline": cpu,
+        "other_workloads": others,
     }
     print(json.dumps(line))
+
+
+def secondary_workloads(args, torch, only=None):
+    """The radical periodic workload (container_periodic_poly(60,5,60,3,4,60),
+    2,000,000 particles, 74^3 blocks) and the batched one (4,096 FCC-jitter
+    containers with sphere walls, ~15.5M particles, one graph launch after
+    autotuning): GPU cells/s (plain path; inputs resident on the device),
+    and the reference timed on a bounded sample on this host."""
+    import voro_amd
+    out = {}
+    flags = voro_amd.F_NEIGHBORS | voro_amd.F_CSR
+    specs = None
+    if only != "batch":
+        periodic_poly(out, flags, torch, args.steps if only else 1)
+    if only != "periodic_poly":
+        specs = batch(out, flags, args.steps if only else 1)
+    if only is None and not args.no_cpu:
+        cpu_samples(out, specs)
+    return out
+
+
+def periodic_poly(out, flags, torch, steps):
+    import voro_amd
+    from voro_amd import inputs as I
+    ids, xyz, r = I.make("periodic_poly", 42)
+    con = voro_amd.container_periodic_poly(60, 5, 60, 3, 4, 60, 74, 74, 74)
+    d_ids = torch.from_numpy(ids).to("cuda:0")
+    d_xyz = torch.from_numpy(xyz).to("cuda:0")
+    d_r = torch.from_numpy(r).to("cuda:0")
+    torch.cuda.synchronize()
+    con.put_device(len(ids), d_ids.data_ptr(), d_xyz.data_ptr(), d_r.data_ptr())
+    con.compute_all_cells(flags)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        con.compute_all_cells(flags)
+    dt = (time.perf_counter() - t0) / steps
+    res = con.results()
+    out["periodic_poly_2m"] = {"cells_per_s": len(res["id"]) / dt, "ms": dt * 1e3,
+                               "volume_sum_rel_err": abs(float(res["vol"].sum()) - 216000.0) / 216000.0,
+                               "timings_ms": {k: float(v) for k, v in con.timings().items()}}
+
+
+def batch(out, flags, steps):
+    import voro_amd
+    from voro_amd import inputs as I
+    t0 = time.perf_counter()
+    specs = I.fcc_batch(11, 4096)
+    cons = []
+    for L, g, pts in specs:
+        c = voro_amd.container(0, L, 0, L, 0, L, g, g, g, False, False, False, 8)
+        c.add_wall(voro_amd.wall_sphere(L / 2, L / 2, L / 2, 0.45 * L, -99))
+        c.put_array(np.arange(len(pts), dtype=np.int32), pts)
+        cons.append(c)
+    b = voro_amd.container_batch(cons)
+    setup = time.perf_counter() - t0
+    tune = voro_amd.container_batch(cons[:256]).autotune(flags)
+    b.set_config(0, tune["occupancy"], tune["chunk"])
+    b.compute(flags, graph=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.compute(flags, graph=True)
+    dt = (time.perf_counter() - t0) / steps
+    n = sum(len(s[2]) for s in specs)
+    out["batch_4096"] = {"cells_per_s": n / dt, "ms": dt * 1e3, "containers": len(specs), "particles": n,
+                         "graph": True, "autotuned": tune, "setup_s": setup}
+    return specs
+
+
+def cpu_samples(out, specs):
+    """The reference (oracle/_ref, one thread) on bounded samples of the two
+    secondary workloads."""
+    from oracle import oracle as O
+    from voro_amd import inputs as I
+    if O.reference() is None:
+        return
+    # periodic: the same lattice shape at 1/8 the volume and particle count
+    # (same density), 250,000 particles, 37^3 blocks
+    ids2, xyz2, r2 = I.make("periodic_poly", 42, 250000)
+    xyz2 = xyz2 * 0.5
+    sp = O.make_spec(3, (30, 2.5, 30, 1.5, 2.0, 30), (37, 37, 37))
+    secs = O.time_reference(sp, ids2, xyz2, r2, loop="compute_all_cells")
+    out["periodic_poly_2m"]["cpu_baseline"] = {
+        "value": 250000 / secs, "unit": "cells/s", "cores": 1, "kind": "reference",
+        "sample": "container_periodic_poly(30,2.5,30,1.5,2,30) with 250,000 particles at the same "
+                  "density, compute_all_cells(), %.1f s" % secs}
+    secs = 0.0
+    m = 0
+    for L, g, pts in specs[:64]:
+        sp = O.make_spec(0, (0, L, 0, L, 0, L), (g, g, g), spheres=[[L / 2, L / 2, L / 2, 0.45 * L, -99]])
+        secs += O.time_reference(sp, np.arange(len(pts), dtype=np.int32), pts, loop="compute_all_cells")
+        m += len(pts)
+    out["batch_4096"]["cpu_baseline"] = {
+        "value": m / secs, "unit": "cells/s", "cores": 1, "kind": "reference",
+        "sample": "the first 64 containers (%d particles), compute_all_cells() each, %.1f s" % (m, secs)}
 
 
 if __name__ == "__main__":

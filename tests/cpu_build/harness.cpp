@@ -303,7 +303,7 @@ static long long compute_prd(const hx_spec *s, int n, const int *ids, const doub
 	std::vector<int> ia, ba;
 	{
 		ImageSet im(g, P.data(), pid.data(), cnt.data());
-		for (int b : prd_all_images(g)) im.create_block(b);
+		for (int b : prd_margin_images(g, PRD_FIRST_MARGIN)) im.create_block(b);
 		im.assemble(Pa, ia, ba);
 		v.P = Pa.data(); v.pid = ia.data(); v.bstart = ba.data();
 	}

@@ -134,30 +134,33 @@ struct PrdGeom {
 // The image blocks of a periodic container, built by the reference's
 // creation routines (src/container_prd.cc:583-775, src/container_prd.hh:
 // 188-194) from the binned primary blocks: block b's particles are
-// P[cstart[b] .. cstart[b+1]) with ids pid[...] for a primary block.
+// P[cstart[b] .. cstart[b+1]) with ids pid[...] for a primary block.  Image
+// particles are appended to one flat list with their block; assemble() sorts
+// them by block, stably, so each block keeps its order of creation.
 struct ImageSet {
 	const PrdGeom &g;
 	const dbl4 *P;
 	const int *pid;
 	const int *cstart;
-	std::vector<std::vector<dbl4>> ip;     // image block particles
-	std::vector<std::vector<int>> iid;
+	std::vector<int> reg;                   // block of each image particle
+	std::vector<dbl4> ip;                   // image particles, in creation order
+	std::vector<int> iid;
 	std::vector<unsigned char> img, created;
 	bool late = false;   // a particle reached a block that was already complete
 	ImageSet(const PrdGeom &g_, const dbl4 *P_, const int *pid_, const int *cstart_)
-		: g(g_), P(P_), pid(pid_), cstart(cstart_), ip(g_.nblocks()), iid(g_.nblocks()),
-		  img(g_.nblocks(), 0), created(g_.nblocks(), 0) {}
+		: g(g_), P(P_), pid(pid_), cstart(cstart_), img(g_.nblocks(), 0), created(g_.nblocks(), 0) {}
 
 	int co(int f) const { return cstart[f + 1] - cstart[f]; }
 	double px(int f, int l) const { return P[cstart[f] + l].x; }
 	double py(int f, int l) const { return P[cstart[f] + l].y; }
 
 	// put_image, src/container_prd.cc:766-774 (the source is a primary block)
-	void put_image(int reg, int fijk, int l, double dx, double dy, double dz) {
+	void put_image(int r, int fijk, int l, double dx, double dy, double dz) {
 		const dbl4 s = P[cstart[fijk] + l];
-		if (created[reg]) late = true;
-		ip[reg].push_back(dbl4{s.x + dx, s.y + dy, s.z + dz, s.w});
-		iid[reg].push_back(pid[cstart[fijk] + l]);
+		if (created[r]) late = true;
+		reg.push_back(r);
+		ip.push_back(dbl4{s.x + dx, s.y + dy, s.z + dz, s.w});
+		iid.push_back(pid[cstart[fijk] + l]);
 	}
 	// create_periodic_image, src/container_prd.hh:188-194
 	void create(int di, int dj, int dk) {
@@ -308,31 +311,46 @@ struct ImageSet {
 	// image blocks as built.  bstart has nblocks+1 entries.
 	void assemble(std::vector<dbl4> &Pall, std::vector<int> &pidall, std::vector<int> &bstart) const {
 		const int nb = (int)g.nblocks();
+		std::vector<int> icnt(nb, 0);
+		for (int r : reg) icnt[r]++;
 		bstart.assign(nb + 1, 0);
 		long long tot = 0;
 		for (int b = 0; b < nb; b++) {
 			bstart[b] = (int)tot;
-			tot += g.primary(b) ? co(b) : (long long)ip[b].size();
+			tot += g.primary(b) ? co(b) : icnt[b];
 		}
 		bstart[nb] = (int)tot;
 		Pall.resize(std::max<long long>(tot, 1));
 		pidall.resize(std::max<long long>(tot, 1));
 		for (int b = 0; b < nb; b++) {
+			if (!g.primary(b)) { icnt[b] = bstart[b]; continue; }
 			int d = bstart[b];
-			if (g.primary(b)) {
-				for (int l = 0; l < co(b); l++, d++) { Pall[d] = P[cstart[b] + l]; pidall[d] = pid[cstart[b] + l]; }
-			} else {
-				for (size_t l = 0; l < ip[b].size(); l++, d++) { Pall[d] = ip[b][l]; pidall[d] = iid[b][l]; }
-			}
+			for (int l = 0; l < co(b); l++, d++) { Pall[d] = P[cstart[b] + l]; pidall[d] = pid[cstart[b] + l]; }
+		}
+		for (size_t t = 0; t < reg.size(); t++) {
+			const int d = icnt[reg[t]]++;
+			Pall[d] = ip[t];
+			pidall[d] = iid[t];
 		}
 	}
 };
 
-// Image blocks in the order of create_all_images (src/container_prd.cc:
-// 551-554): every block, k then j then i.
-inline std::vector<int> prd_all_images(const PrdGeom &g) {
+// Image blocks within `margin` block layers of the primary domain, in the
+// order of create_all_images (src/container_prd.cc:551-554: k, then j, then
+// i).  The margins the reference allocates (ey, ez layers) can be far wider
+// than what the cells reach; blocks beyond `margin` that a cell does reach
+// are created once their first visit has been recorded.
+enum { PRD_FIRST_MARGIN = 6 };   // block layers created before any visit is known
+inline std::vector<int> prd_margin_images(const PrdGeom &g, int margin) {
 	std::vector<int> o;
-	for (int b = 0; b < (int)g.nblocks(); b++) if (!g.primary(b)) o.push_back(b);
+	const int j0 = std::max(0, g.ey - margin), j1 = std::min(g.oy, g.wy + margin);
+	const int k0 = std::max(0, g.ez - margin), k1 = std::min(g.oz, g.wz + margin);
+	for (int k = k0; k < k1; k++)
+		for (int j = j0; j < j1; j++)
+			for (int i = 0; i < g.nx; i++) {
+				const int b = i + g.nx * (j + g.oy * k);
+				if (!g.primary(b)) o.push_back(b);
+			}
 	return o;
 }
 

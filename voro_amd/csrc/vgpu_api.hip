@@ -126,6 +126,11 @@ namespace {
 
 int nblk(long long n, int t) { return (int)((n + t - 1) / t); }
 
+// Most wavefronts a retry tier spreads its list over (a wide-tier slab is
+// ~16 MB, a huge-tier slab ~64 MB; the slabs are allocated as lists need
+// them).  Retried cells are few and expensive, so the list is spread thin.
+enum { RETRY_WAVES_WIDE = 512, RETRY_WAVES_HUGE = 32 };
+
 // Exclusive scan of n+1 ints (in[n] must be 0): out[n] = total.
 // The partial-sum buffer is grown here unless `no_alloc` (graph capture:
 // the caller has sized it with scan_part_bytes).
@@ -308,9 +313,9 @@ int run_tiers(vgpu_container *h, const RectView &v, int nc, int fw, const int *l
 	if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, list, list_count, out, s))) return st;
 	if (mark_ev2) HIPCHK(hipEventRecord(h->ev[2], s));
 	out.tier = 2; out.retry = h->retry2.as<int>(); out.retry_count = ctr + 2;
-	if ((st = launch_tier<TierWide>(h, h->ws_wide, 64, v, nc, h->retry1.as<int>(), ctr + 1, out, s))) return st;
+	if ((st = launch_tier<TierWide>(h, h->ws_wide, RETRY_WAVES_WIDE, v, nc, h->retry1.as<int>(), ctr + 1, out, s))) return st;
 	out.tier = 3; out.retry = nullptr; out.retry_count = nullptr;
-	return launch_tier<TierHuge>(h, h->ws_huge, 4, v, nc, h->retry2.as<int>(), ctr + 2, out, s);
+	return launch_tier<TierHuge>(h, h->ws_huge, RETRY_WAVES_HUGE, v, nc, h->retry2.as<int>(), ctr + 2, out, s);
 }
 
 // Uploads the block arrays of every block (primary and image) and points
@@ -332,8 +337,8 @@ int upload_blocks(vgpu_container *h, const ImageSet &im, RectView &v, hipStream_
 }
 
 // The cells of a periodic container.  Pass 1 computes every cell against
-// image blocks built in create_all_images order and records, per block, the
-// first visit over the loop.  The image blocks are then rebuilt by creating
+// the image blocks near the primary domain, built in create_all_images
+// order, and records, per block, the first visit over the loop.  The image blocks are then rebuilt by creating
 // them in that order -- the reference's lazy creation order -- and the cells
 // that read an image block are recomputed, recording first visits again.
 // When the recorded order reproduces itself, every cell has been computed
@@ -349,7 +354,7 @@ int run_prd_cells(vgpu_container *h, RectView &v, CellOut out, int nc, int fw, h
 	HIPCHK(hipStreamSynchronize(s));
 	{
 		ImageSet im(g, h->hP.data(), h->hpid.data(), h->hcs.data());
-		for (int b : prd_all_images(g)) im.create_block(b);
+		for (int b : prd_margin_images(g, PRD_FIRST_MARGIN)) im.create_block(b);
 		if ((st = upload_blocks(h, im, v, s))) return st;
 	}
 	if ((st = h->fv.ensure((size_t)nb * 8)) || (st = h->touch.ensure((size_t)nc))) return st;
@@ -517,9 +522,9 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 	}
 	if (nc > 0 && !h->prd) {
 		out.tier = 2; out.retry = h->retry2.as<int>(); out.retry_count = ctr + 2;
-		if ((st = launch_tier<TierWide>(h, h->ws_wide, 64, v, nc, h->retry1.as<int>(), ctr + 1, out, s))) return st;
+		if ((st = launch_tier<TierWide>(h, h->ws_wide, RETRY_WAVES_WIDE, v, nc, h->retry1.as<int>(), ctr + 1, out, s))) return st;
 		out.tier = 3; out.retry = nullptr; out.retry_count = nullptr;
-		if ((st = launch_tier<TierHuge>(h, h->ws_huge, 4, v, nc, h->retry2.as<int>(), ctr + 2, out, s))) return st;
+		if ((st = launch_tier<TierHuge>(h, h->ws_huge, RETRY_WAVES_HUGE, v, nc, h->retry2.as<int>(), ctr + 2, out, s))) return st;
 	}
 	HIPCHK(hipEventRecord(h->ev[3], s));
 	h->csr_valid = false;
