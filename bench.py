@@ -149,6 +149,7 @@ def main():
     r = con.results()
     identical = bool(np.array_equal(r["vol"], plain_ref[0]) and np.array_equal(r["nbr"], plain_ref[1]))
     cells_s = ncells * args.steps / dt
+
     # roofline of the list-driven cell kernel alone (its HIP events, gathering excluded)
     kern_s = (tier_ms - gather_ms) / 1e3
     bytes_launch = ncells * BYTES_PER_CELL_FIXED + 4.0 * nnbr

@@ -33,12 +33,23 @@ __global__ void k_order_scan(int *__restrict__ hist) {
 	}
 }
 
+// Slots are claimed per workgroup: a class's cells of one workgroup are
+// counted in LDS and take one global atomic together.
 __global__ void k_order_scatter(int nc, const int *__restrict__ cellblk, const unsigned char *__restrict__ cost,
 				int per_cell, int *__restrict__ fill, int *__restrict__ order) {
+	__shared__ int h[64], base[64];
+	if (threadIdx.x < 64) h[threadIdx.x] = 0;
+	__syncthreads();
 	const int c = blockIdx.x * blockDim.x + threadIdx.x;
-	if (c >= nc) return;
-	const int k = 63 - cost[per_cell ? c : cellblk[c]];
-	order[atomicAdd(&fill[k], 1)] = c;
+	int k = 0, r = 0;
+	if (c < nc) {
+		k = 63 - cost[per_cell ? c : cellblk[c]];
+		r = atomicAdd(&h[k], 1);
+	}
+	__syncthreads();
+	if (threadIdx.x < 64 && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&fill[threadIdx.x], h[threadIdx.x]);
+	__syncthreads();
+	if (c < nc) order[base[k] + r] = c;
 }
 
 __global__ void k_list_count(RectView v, int nc, const int *__restrict__ cellblk, const int *__restrict__ order,
