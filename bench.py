@@ -156,7 +156,7 @@ def main():
     hbm_gbs = bytes_launch / kern_s / 1e9
     fp64_tf = ncells * FLOP_PER_CELL / kern_s / 1e12
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_k_cells_uniform1m.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_k_cells_list_uniform1m.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("bytes_per_launch")
 
