@@ -57,7 +57,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=400, help="predictor training steps (full-batch Adam)")
     ap.add_argument("--fallback", type=float, default=0.02,
                     help="fraction of training blocks the depth margin allows to be under-predicted")
-    ap.add_argument("--workload", choices=["uniform", "periodic_poly", "batch"], default="uniform",
+    ap.add_argument("--workload", choices=["uniform", "periodic_poly", "batch", "clusters"], default="uniform",
                     help="uniform: the headline measurement (default); periodic_poly / batch: only that "
                          "secondary workload, for profiling")
     ap.add_argument("--headline-only", action="store_true",
@@ -211,19 +211,22 @@ def main():
 
 def secondary_workloads(args, torch, only=None):
     """The radical periodic workload (container_periodic_poly(60,5,60,3,4,60),
-    2,000,000 particles, 74^3 blocks) and the batched one (4,096 FCC-jitter
+    2,000,000 particles, 74^3 blocks), the batched one (4,096 FCC-jitter
     containers with sphere walls, ~15.5M particles, one graph launch after
-    autotuning): GPU cells/s (plain path; inputs resident on the device),
-    and the reference timed on a bounded sample on this host."""
+    autotuning) and the clustered one with the predictor: GPU cells/s
+    (inputs resident on the device), and the reference timed on bounded
+    samples on this host."""
     import voro_amd
     out = {}
     flags = voro_amd.F_NEIGHBORS | voro_amd.F_CSR
     specs = None
-    if only != "batch":
+    if only is None or only == "periodic_poly":
         periodic_poly(out, flags, torch, args.steps if only else 1)
-    if only != "periodic_poly":
+    if only != "periodic_poly" and only != "clusters":
         specs = batch(out, flags, args.steps if only else 1)
-    if only is None and not args.no_cpu:
+    if only is None or only == "clusters":
+        clusters_learned(out, flags, torch)
+    if only is None and not args.no_cpu and specs is not None:
         cpu_samples(out, specs)
     return out
 
@@ -273,6 +276,55 @@ def batch(out, flags, steps):
     out["batch_4096"] = {"cells_per_s": n / dt, "ms": dt * 1e3, "containers": len(specs), "particles": n,
                          "graph": True, "autotuned": tune, "setup_s": setup}
     return specs
+
+
+def clusters_learned(out, flags, torch):
+    """The clustered workload (1M particles, 64 Gaussian clusters + 30%
+    uniform background, 58^3 blocks): the predictor trained on the plain
+    path's statistics for seeds 1..10, evaluated on seed 101 against the
+    plain path (results checked identical)."""
+    import voro_amd
+    from voro_amd import inputs as I
+    dev = "cuda:0"
+
+    def make(seed):
+        ids, xyz, _ = I.make("clusters", seed)
+        d_ids = torch.from_numpy(ids).to(dev)
+        d_xyz = torch.from_numpy(xyz).to(dev)
+        c = voro_amd.container(0, 100, 0, 100, 0, 100, 58, 58, 58, False, False, False, 8)
+        c.put_device(len(ids), d_ids.data_ptr(), d_xyz.data_ptr())
+        c._keep = (d_ids, d_xyz)
+        return c
+
+    t0 = time.perf_counter()
+    pred = voro_amd.predictor()
+    for seed in range(1, 11):
+        c = make(seed)
+        c.compute_all_cells(flags | voro_amd.F_STATS)
+        pred.add_samples(c)
+        del c
+    loss = pred.train(epochs=400, lr=3e-3, seed=1, fallback=0.02)
+    t_train = time.perf_counter() - t0
+    con = make(101)
+    con.compute_all_cells(flags)
+    t0 = time.perf_counter()
+    con.compute_all_cells(flags)
+    dt_plain = time.perf_counter() - t0
+    ref = con.results()
+    ref = (ref["vol"].copy(), ref["nbr"].copy())
+    con.set_predictor(pred)
+    con.compute_all_cells(flags | voro_amd.F_LEARNED)
+    t0 = time.perf_counter()
+    con.compute_all_cells(flags | voro_amd.F_LEARNED)
+    dt = time.perf_counter() - t0
+    r = con.results()
+    n = len(r["id"])
+    ls = con.learned_stats()
+    out["clusters_learned"] = {
+        "plain_cells_per_s": n / dt_plain, "learned_cells_per_s": n / dt, "speedup_vs_plain": dt_plain / dt,
+        "fallbacks": ls["fallbacks"], "fallback_rate": ls["fallbacks"] / n, "train_seeds": list(range(1, 11)),
+        "eval_seed": 101, "train_seconds": t_train, "train_loss": loss,
+        "identical_to_plain": bool(np.array_equal(r["vol"], ref[0]) and np.array_equal(r["nbr"], ref[1]))}
 
 
 def cpu_samples(out, specs):

@@ -260,13 +260,14 @@ struct VCell3 {
 
 	// src/cell.cc:522-595 (the reference pushes the member `up` in the
 	// second loop, src/cell.cc:546-549; kept).
-	VG_FN bool definite_max(int &lp, int &ls, double &l, double &u, unsigned &uw) {
+	// t0: the plane values of lp's three neighbours, already computed.
+	VG_FN bool definite_max(int &lp, int &ls, double &l, double &u, unsigned &uw, const double *t0) {
 		int tp = lp, ts, qp = 0;
 		unsigned qw;
 		double q;
 		for (ts = 0; ts < 3; ts++) {
 			qp = E(tp, ts);
-			m_test(qp, q);
+			q = t0[ts];
 			if (q > l - big_tol) break;
 		}
 		if (ts == 3) return true;
@@ -310,18 +311,28 @@ struct VCell3 {
 	}
 
 	// src/cell.cc:480-515, folded so definite_max has one call site.
+	// The three neighbours' plane values are loaded together (one memory
+	// round trip instead of three dependent ones); they are then taken in
+	// the reference's order with its early exit, so up/u/uw end as there.
 	VG_FN bool search_upward(unsigned &uw, int &lp, int &ls, int &us, double &l, double &u) {
 		int vs = -1, guard = 0;
 		lp = up; l = u;
 		for (;;) {
 			const unsigned long long w = TW(lp);
+			const int e0 = twE(w, 0), e1 = twE(w, 1), e2 = twE(w, 2);
+			double t[3];
+			unsigned tw[3];
+			tw[0] = m_test(e0, t[0]);
+			tw[1] = m_test(e1, t[1]);
+			tw[2] = m_test(e2, t[2]);
 			for (ls = 0; ls < 3; ls++) {
 				if (ls == vs) continue;
-				up = twE(w, ls);
-				uw = m_test(up, u);
+				up = ls == 0 ? e0 : (ls == 1 ? e1 : e2);
+				uw = tw[ls];
+				u = t[ls];
 				if (u > l) break;
 			}
-			if (ls == 3 && definite_max(lp, ls, l, u, uw)) { up = lp; return false; }
+			if (ls == 3 && definite_max(lp, ls, l, u, uw, t)) { up = lp; return false; }
 			if (uw != 0) break;
 			if (ovf || ++guard > 4 * MAXV) { fatal = ovf = true; return false; }
 			vs = R(lp, ls); lp = up; l = u;
@@ -332,13 +343,14 @@ struct VCell3 {
 
 	// src/cell.cc:631-708 (the reference pushes the argument `lp` in the
 	// second loop, src/cell.cc:654-657; kept).
-	VG_FN bool definite_min(int &lp, int &us, double &l, double &u, unsigned &lw) {
+	// t0: the plane values of up's three neighbours, already computed.
+	VG_FN bool definite_min(int &lp, int &us, double &l, double &u, unsigned &lw, const double *t0) {
 		int tp = up, ts, qp = 0;
 		unsigned qw;
 		double q;
 		for (ts = 0; ts < 3; ts++) {
 			qp = E(tp, ts);
-			m_test(qp, q);
+			q = t0[ts];
 			if (q < u + big_tol) break;
 		}
 		if (ts == 3) return true;
@@ -386,13 +398,20 @@ struct VCell3 {
 		int vs = -1, guard = 0;
 		for (;;) {
 			const unsigned long long w = TW(up);
+			const int e0 = twE(w, 0), e1 = twE(w, 1), e2 = twE(w, 2);
+			double t[3];
+			unsigned tw[3];
+			tw[0] = m_test(e0, t[0]);
+			tw[1] = m_test(e1, t[1]);
+			tw[2] = m_test(e2, t[2]);
 			for (us = 0; us < 3; us++) {
 				if (us == vs) continue;
-				lp = twE(w, us);
-				lw = m_test(lp, l);
+				lp = us == 0 ? e0 : (us == 1 ? e1 : e2);
+				lw = tw[us];
+				l = t[us];
 				if (u > l) break;
 			}
-			if (us == 3 && definite_min(lp, us, l, u, lw)) return false;
+			if (us == 3 && definite_min(lp, us, l, u, lw, t)) return false;
 			if (lw != 2) break;
 			if (ovf || ++guard > 4 * MAXV) { fatal = ovf = true; return false; }
 			vs = R(up, us); up = lp; u = l;
