@@ -257,6 +257,19 @@ struct VCell3 {
 		if (r == 0 && ans > -big_tol) ovf = true;
 		return r;
 	}
+	// m_testx that also hands back the vertex's coordinates.
+	VG_INL unsigned m_testx3(int n, double &ans, double &x, double &y, double &z) {
+		if (n < 0 || n >= MAXV) { fatal = ovf = true; ans = 0; x = y = z = 0; return 1u; }
+		const unsigned rr = rec(n);
+		ldXY(rr, x, y);
+		z = ld<double>(rr + 16);
+		ans = x * px;
+		ans += y * py;
+		ans += z * pz - prsq;
+		const unsigned r = ans < -tol ? 0u : (ans > tol ? 2u : 1u);
+		if (r == 0 && ans > -big_tol) ovf = true;
+		return r;
+	}
 
 	// src/cell.cc:522-595 (the reference pushes the member `up` in the
 	// second loop, src/cell.cc:546-549; kept).
@@ -442,22 +455,26 @@ struct VCell3 {
 		cs = 2;
 		qs = cyc_up(us);
 		qp = up; q = u;
+		// coordinates of qp, carried along the walk
+		double qx = X(up), qy = Y(up), qz = Z(up);
 		cp = p; rp = p; p++;
 		int guard = 0;
 		while (qp != up || qs != us) {
 			if (++guard > 16 * MAXV) { fatal = ovf = true; return true; }
 			lp = E(qp, qs);
-			lw = m_testx(lp, l);
+			double lx, ly, lz;
+			lw = m_testx3(lp, l, lx, ly, lz);
 			if (ovf) return true;
 			if (lw == 2) {
 				qs = cyc_up(R(qp, qs));
 				qp = lp;
 				q = l;
+				qx = lx; qy = ly; qz = lz;
 				if (!push_ds(qp)) return true;
 			} else if (lw == 0) {
 				if (!room_v(p)) return true;
 				r = q / (q - l); l = 1 - r;
-				new_vertex(p, X(lp) * r + X(qp) * l, Y(lp) * r + Y(qp) * l, Z(lp) * r + Z(qp) * l);
+				new_vertex(p, lx * r + qx * l, ly * r + qy * l, lz * r + qz * l);
 				ls = R(qp, qs);
 				setN(p, 0, pid);
 				setN(p, 1, N(qp, qs));
