@@ -310,6 +310,7 @@ def clusters_learned(out, flags, torch):
     t0 = time.perf_counter()
     con.compute_all_cells(flags)
     dt_plain = time.perf_counter() - t0
+    plain_ms = {k: float(v) for k, v in con.timings().items()}
     ref = con.results()
     ref = (ref["vol"].copy(), ref["nbr"].copy())
     con.set_predictor(pred)
@@ -323,7 +324,8 @@ def clusters_learned(out, flags, torch):
     out["clusters_learned"] = {
         "plain_cells_per_s": n / dt_plain, "learned_cells_per_s": n / dt, "speedup_vs_plain": dt_plain / dt,
         "fallbacks": ls["fallbacks"], "fallback_rate": ls["fallbacks"] / n, "train_seeds": list(range(1, 11)),
-        "eval_seed": 101, "train_seconds": t_train, "train_loss": loss,
+        "eval_seed": 101, "train_seconds": t_train, "train_loss": loss, "plain_timings_ms": plain_ms,
+        "learned_timings_ms": {k: float(v) for k, v in con.timings().items()},
         "identical_to_plain": bool(np.array_equal(r["vol"], ref[0]) and np.array_equal(r["nbr"], ref[1]))}
 
 

@@ -352,8 +352,15 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 	bool bcheck = false;          // per-particle radius check for this block
 	double x2 = 0, y2 = 0, z2 = 0;
 
+	// Upper bound on the cell's current max_radius_squared() (the cell only
+	// shrinks), refreshed with the reference's mrs and after every 16
+	// candidates; see the skip before the nplane call.
+	double mrs_b = 1.7976931348623157e308;
+	int since = 0;
+
 	for (;;) {
 		double px_ = 0, py_ = 0, pz_ = 0, rs = 0;
+		double rs_raw = -1;           // squared distance of a particle candidate (walls: -1)
 		int pid = 0;
 		bool have = false;
 		while (!have) {
@@ -364,6 +371,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				bt++;
 				px_ = o.x - x2; py_ = o.y - y2; pz_ = o.z - z2;
 				rs = px_ * px_ + py_ * py_ + pz_ * pz_;
+				rs_raw = rs;
 				if (!bcheck) { rs = rp.scale(rs, o.w); have = true; }
 				else have = rp.scale_check(rs, mrs, o.w);
 				continue;
@@ -393,12 +401,14 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 					pid = v.pid[b0 + t];
 					t++;
 					px_ = o.x - x; py_ = o.y - y; pz_ = o.z - z;
-					rs = rp.scale(px_ * px_ + py_ * py_ + pz_ * pz_, o.w);
+					rs_raw = px_ * px_ + py_ * py_ + pz_ * pz_;
+					rs = rp.scale(rs_raw, o.w);
 					have = true;
 					continue;
 				}
 				// Worklist selection (src/v_compute.cc:328-362).
 				mrs = c.max_radius_squared();
+				mrs_b = mrs; since = 0;
 				if (PRD) { fx = x - v.boxx * ci; fy = y - v.boxy * (cj - v.ey); fz = z - v.boxz * (ck - v.ez); }
 				else { fx = x - v.ax - v.boxx * ci; fy = y - v.ay - v.boxy * cj; fz = z - v.az - v.boxz * ck; }
 				di = int(fx * v.xsp * VGPU_WL_FGRID); dj = int(fy * v.ysp * VGPU_WL_FGRID); dk = int(fz * v.zsp * VGPU_WL_FGRID);
@@ -437,6 +447,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				if (!marking && g >= f && g > 0) { marking = true; stt.phase = 2; g2_first = g + 1; }
 				if (g == next_count) {
 					mrs = c.max_radius_squared();
+					mrs_b = mrs; since = 0;
 					if (cpi != 8) {
 						const int count_list[8] = {7, 11, 15, 19, 26, 35, 45, 59};
 						next_count = count_list[cpi++];
@@ -519,6 +530,16 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 			if (ej < v.hy - 1 && !bs.marked(ei, ej + 1, ek)) bs.mark_push(ei, ej + 1, ek);
 			if (ek < v.hz - 1 && !bs.marked(ei, ej, ek + 1)) bs.mark_push(ei, ej, ek + 1);
 			if (c.ovf) { stt.n_planes = c.n_cuts; return false; }
+		}
+		if (rs_raw >= 0) {
+			// Every vertex V (doubled coordinates, |V|^2 <= mrs_b) has
+			// x.V - rs <= sqrt(rs_raw mrs_b) - rs: when that is below
+			// -2 tol the plane misses the cell with every vertex strictly
+			// inside (rounding of both sides is far below tol), and the
+			// reference's nplane returns without changing the cell
+			// (src/cell.cc:719-757).  The call is skipped; it still counts.
+			if (++since >= 16) { mrs_b = c.max_radius_squared(); since = 0; }
+			if (rs - sqrt(rs_raw * mrs_b) > 2.0 * c.tol) { c.n_cuts++; continue; }
 		}
 		if (!c.nplane(px_, py_, pz_, rs, pid)) { stt.n_planes = c.n_cuts; return false; }
 	}
