@@ -96,9 +96,9 @@ int vgpu_get_results(vgpu_container *h, int *id, int *ijk, int *q, int *exists, 
 
 /* Per-cell statistics recorded with VGPU_F_STATS: nplane calls, worklist
  * exit index (64 = reached the block search), and the tier that produced
- * the cell (0 first tier: order-3 or, with VGPU_F_LEARNED, the list-driven
- * one; 1 order-3 over the learned tier's leftovers; 2 wide; 3 huge).  Loop
- * order. */
+ * the cell (0 first tier: one cell per wavefront, order-3 per lane, or with
+ * VGPU_F_LEARNED the list-driven one; 1 order-3 over the first tier's
+ * leftovers; 2 wide; 3 huge).  Loop order. */
 int vgpu_get_stats(vgpu_container *h, int *n_planes, int *wl_exit, int *tier);
 
 /* Sum of cell volumes in loop order (src/container.cc:451-456
@@ -115,6 +115,14 @@ int vgpu_device_results(vgpu_container *h, const double **vol, const double **ce
  * was compiled for, in waves per SIMD (2, 3 or 4; 0 = keep).  Results do not
  * depend on either. */
 int vgpu_set_launch(vgpu_container *h, int fast_waves, int fast_occupancy);
+
+/* The first tier of the plain path of rectangular containers: 0 (default)
+ * computes one cell per wavefront, vertex v in lane v; 1 one order-3 cell per
+ * lane.  Cells the first tier cannot finish go on to the per-lane tiers, so
+ * results do not depend on the choice.  waves: persistent wavefronts of the
+ * wavefront tier (0 = 4 x occupancy x compute units); occupancy: its
+ * register budget in waves per SIMD (2, 3, 4 or 8; 0 = keep). */
+int vgpu_set_first_tier(vgpu_container *h, int tier, int waves, int occupancy);
 
 /* Time of the last vgpu_compute_all, split by stage, in milliseconds, from
  * HIP events: [0] binning, [1] fast tier, [2] wide+huge tiers, [3] outputs. */

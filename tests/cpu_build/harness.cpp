@@ -13,6 +13,7 @@
 #include "../../voro_amd/csrc/grid_host.h"
 #include "../../voro_amd/csrc/list_dev.h"
 #include "../../voro_amd/csrc/prd_host.h"
+#include "../../voro_amd/csrc/wcell_dev.h"
 
 using namespace vgpu;
 
@@ -89,6 +90,13 @@ static long long list_fallbacks = 0;
 void hx_use_list(int G) { list_depth = G; list_fallbacks = 0; }
 long long hx_list_fallbacks(void) { return list_fallbacks; }
 
+static int use_wave = 0;
+static long long wave_done = 0;
+// on: every cell is first tried by the wavefront-cell tier (wcell_dev.h,
+// 64 lanes emulated); cells it cannot finish go on to the per-lane tiers.
+void hx_use_wave(int on) { use_wave = on; wave_done = 0; }
+long long hx_wave_done(void) { return wave_done; }
+
 static long long prd_passes = 0;
 static int prd_replay = 1;
 // Passes over the cells of the last periodic computation (1 + replays).
@@ -154,6 +162,7 @@ long long hx_compute_neighbor(const hx_spec *s, int n, const int *ids, const dou
 	ListRunner<false> lr0(v.max_len_sq);
 	ListRunner<true> lr1(v.max_len_sq);
 
+	Runner<WCell, WLayout::bytes> rw(v.max_len_sq);
 	Runner<VCell3, Layout3<>::lane_bytes> r0(v.max_len_sq);
 	Runner<VCell<TierWide, 1>, Layout<TierWide, 1>::bytes> r1(v.max_len_sq);
 	Runner<VCell<TierHuge, 1>, Layout<TierHuge, 1>::bytes> *r2 = 0;
@@ -174,6 +183,12 @@ long long hx_compute_neighbor(const hx_spec *s, int n, const int *ids, const dou
 				res = poly ? lr1.run(v, cellblk.data(), cell, list_depth, ijk, q - cnt[ijk], &vol, cent, &nf, nb, st, &list_fallbacks)
 					   : lr0.run(v, cellblk.data(), cell, list_depth, ijk, q - cnt[ijk], &vol, cent, &nf, nb, st, &list_fallbacks);
 				if (res < 0) nb.clear();
+			}
+			if (res < 0 && use_wave) {
+				res = poly ? rw.run<true>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st)
+					   : rw.run<false>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st);
+				if (res >= 0 && nf > (int)WLayout::NBCAP) res = -1;
+				if (res < 0) nb.clear(); else wave_done++;
 			}
 			if (res < 0) res = poly ? r0.run<true>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st)
 				   : r0.run<false>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st);
@@ -269,6 +284,7 @@ static long long compute_prd(const hx_spec *s, int n, const int *ids, const doub
 	std::vector<unsigned long long> fv(nb, ~0ull), prev;
 	v.fv = fv.data();
 
+	Runner<WCell, WLayout::bytes> rw(v.max_len_sq);
 	Runner<VCell3, Layout3<>::lane_bytes> r0(v.max_len_sq);
 	Runner<VCell<TierWide, 1>, Layout<TierWide, 1>::bytes> r1(v.max_len_sq);
 	Runner<VCell<TierHuge, 1>, Layout<TierHuge, 1>::bytes> *r2 = 0;
@@ -282,8 +298,16 @@ static long long compute_prd(const hx_spec *s, int n, const int *ids, const doub
 		Res &R = res[cell];
 		R.vol = 0; R.cent[0] = R.cent[1] = R.cent[2] = 0; R.nf = 0; R.nb.clear(); R.tier = 0;
 		const unsigned long long key = (unsigned long long)cell << 24;
-		int rr = poly ? r0.run<true, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key)
-			      : r0.run<false, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key);
+		int rr = -1;
+		if (use_wave) {
+			rr = poly ? rw.run<true, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key)
+				  : rw.run<false, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key);
+			if (rr >= 0 && R.nf > (int)WLayout::NBCAP) rr = -1;
+			if (rr < 0) R.nb.clear(); else wave_done++;
+		}
+		if (rr < 0)
+			rr = poly ? r0.run<true, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key)
+				  : r0.run<false, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key);
 		if (rr < 0) {
 			R.tier = 1; R.nb.clear();
 			rr = poly ? r1.run<true, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key)

@@ -131,3 +131,52 @@ def test_periodic_image_order_matters(oracle_mod, harness):
     b = harness.run(O, sp, d["ids"], d["xyz"], d.get("r"))
     assert O.compare(b, d, vol_rtol=0.0) == []
     assert lib.hx_image_passes() >= 2
+
+
+# ---- the wavefront-cell tier (wcell_dev.h), 64 lanes emulated on the CPU ----
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_wave_cell_on_cpu_matches_oracle(oracle_mod, harness, case):
+    """Every cell first goes through the wavefront cell (vertex v in lane v);
+    the results, bit for bit, and the share it finishes."""
+    O = oracle_mod
+    kind, geom, grid, sph, xyz, r, per = CASES[case]()
+    ids = np.arange(len(xyz), dtype=np.int32)
+    sp = O.make_spec(kind, geom, grid, periodic=per, spheres=sph)
+    a = O.run("restatement", sp, ids, xyz, r)
+    b = harness.run(O, sp, ids, xyz, r, wave=True)
+    assert O.compare(b, a, vol_rtol=0.0) == []
+    assert np.array_equal(a["vol"], b["vol"]) and np.array_equal(a["cent"], b["cent"])
+    assert b["wave_done"] >= 0.99 * len(ids)
+
+
+def test_wave_cell_walls_and_escalation(oracle_mod, harness):
+    """Sphere walls, and exact FCC (vertices of order > 3): the wavefront cell
+    hands every such cell on, with identical results."""
+    O = oracle_mod
+    xyz, L = I.fcc_sphere(1, 3000, 1.0, 0.0)
+    ids = np.arange(len(xyz), dtype=np.int32)
+    sp = O.make_spec(0, (0, L, 0, L, 0, L), (7, 7, 7), spheres=[[L / 2, L / 2, L / 2, 0.45 * L, -99]])
+    a = O.run("restatement", sp, ids, xyz)
+    b = harness.run(O, sp, ids, xyz, wave=True)
+    assert b["tiers"][1] > 0
+    assert O.compare(b, a, vol_rtol=0.0) == []
+    xyz, L = I.fcc_sphere(2, 3000, 1.0, 0.08)
+    ids = np.arange(len(xyz), dtype=np.int32)
+    sp = O.make_spec(0, (0, L, 0, L, 0, L), (7, 7, 7), spheres=[[L / 2, L / 2, L / 2, 0.45 * L, -99]])
+    a = O.run("restatement", sp, ids, xyz)
+    b = harness.run(O, sp, ids, xyz, wave=True)
+    assert b["wave_done"] > 0.9 * len(ids)
+    assert O.compare(b, a, vol_rtol=0.0) == []
+    assert np.array_equal(a["vol"], b["vol"])
+
+
+@pytest.mark.parametrize("name", ["prd_poly_small.npz", "prd_tri_small.npz"])
+def test_wave_cell_periodic_matches_golden(oracle_mod, harness, name):
+    O = oracle_mod
+    d = dict(np.load(os.path.join(GOLD, name), allow_pickle=False))
+    sp = O.make_spec(int(d["spec_kind"]), d["spec_geom"], d["spec_grid"])
+    b = harness.run(O, sp, d["ids"], d["xyz"], d.get("r"), wave=True)
+    assert b["wave_done"] > 0
+    assert O.compare(b, d, vol_rtol=0.0) == []
+    assert np.array_equal(b["vol"], d["vol"]) and np.array_equal(b["cent"], d["cent"])

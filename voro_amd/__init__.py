@@ -83,6 +83,7 @@ def lib():
         "vgpu_get_stats": (C.c_int, [p, i32p, i32p, i32p]),
         "vgpu_sum_volumes": (C.c_int, [p, f64p]),
         "vgpu_set_launch": (C.c_int, [p, C.c_int, C.c_int]),
+        "vgpu_set_first_tier": (C.c_int, [p, C.c_int, C.c_int, C.c_int]),
         "vgpu_last_timings": (C.c_int, [p, f64p]),
         "vgpu_status_string": (C.c_char_p, [C.c_int]),
         "vgpu_predictor_create": (C.c_int, [C.c_int, C.POINTER(p)]),
@@ -214,6 +215,11 @@ class _Base:
 
     def set_launch(self, fast_waves=0, fast_occupancy=0):
         _check(lib().vgpu_set_launch(self._h, int(fast_waves), int(fast_occupancy)), "vgpu_set_launch")
+
+    def set_first_tier(self, tier=0, waves=0, occupancy=0):
+        """0: one cell per wavefront (default); 1: one order-3 cell per lane.
+        Results do not depend on the choice."""
+        _check(lib().vgpu_set_first_tier(self._h, int(tier), int(waves), int(occupancy)), "vgpu_set_first_tier")
 
     def timings(self):
         a = np.zeros(4)

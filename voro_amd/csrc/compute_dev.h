@@ -19,6 +19,7 @@
 #pragma once
 #include "cell_dev.h"
 #include "cell3_dev.h"
+#include "wcell_dev.h"
 
 namespace vgpu {
 
@@ -69,6 +70,10 @@ struct CellStats {
 	int max_p;
 	int img;        // periodic containers: an image block was read
 };
+
+// Worklist indices at which the reference refreshes mrs, src/v_compute.cc
+// (count_list {7,11,15,19,26,35,45,59}), one byte each.
+VG_INL int count_list(int i) { return (int)((0x3b2d231a130f0b07ull >> (8 * i)) & 0xffull); }
 
 template<bool POLY>
 struct RadPolicy {
@@ -291,6 +296,78 @@ VG_INL int prd_region(const RectView &v, int ci, int cj, int ck, int ei, int ej,
 	return b;
 }
 
+// Candidate data of compute_cell_rect: the worklist entries and radii, the
+// particle range of a worklist block, and the particles themselves.  The
+// per-lane tiers load each value where it is used.
+template<class CELL, bool PRD>
+struct CandFetch {
+	VG_INL void wl_begin(const RectView &, const double *, const unsigned *, int, int, int, unsigned, unsigned, int,
+			     int, int, int) {}
+	VG_INL double rad(const double *radp, int g) const { return radp[g]; }
+	VG_INL unsigned ent(const unsigned *e, int g) const { return e[g]; }
+	VG_INL void range(const RectView &v, int, int rijk, int &bt, int &be) const {
+		bt = v.bstart[rijk]; be = v.bstart[rijk + 1];
+	}
+	VG_INL void block(int, int) {}
+	VG_INL void part(const RectView &v, int idx, dbl4 &o, int &pid) { o = v.P[idx]; pid = v.pid[idx]; }
+};
+
+#if VG_WDEV
+// The wavefront cell stages them in lanes, so that one vector load replaces a
+// chain of dependent scalar ones: lane g holds worklist entry g, its radius
+// and (rectangular containers) its block's particle range, computed as the
+// step computes it; a block's particles are loaded 64 at a time, particle
+// cb + l in lane l.  Values are then read with readlane.
+template<bool PRD>
+struct CandFetch<WCell, PRD> {
+	int lane;
+	double wr;            // radius of worklist entry `lane`
+	unsigned we;          // entry `lane`
+	int wbt, wbe;         // particle range of its block
+	dbl4 cp;              // particle cb + lane
+	int cpid;
+	int cb, ce;           // staged range start; end of the current block
+	bool have;
+	VG_INL void wl_begin(const RectView &v, const double *radp, const unsigned *e, int i, int j, int k, unsigned m1,
+			     unsigned m2, int ci, int cj, int ck, int disp) {
+		lane = (int)__lane_id();
+		wr = radp[lane];
+		we = e[lane];
+		if (!PRD) {
+			unsigned q = we ^ m1;
+			q += m2;
+			const int ei = (int)(q & 127) - 64 + i, ej = (int)((q >> 7) & 127) - 64 + j,
+				  ek = (int)((q >> 14) & 127) - 64 + k;
+			wbt = wbe = 0;
+			if (lane >= 1 && ei >= 0 && ei < v.hx && ej >= 0 && ej < v.hy && ek >= 0 && ek < v.hz) {
+				double qx, qy, qz;
+				const int rijk = rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
+				wbt = v.bstart[rijk];
+				wbe = v.bstart[rijk + 1];
+			}
+		}
+	}
+	VG_INL double rad(const double *, int g) const { return WCell::rld(wr, g); }
+	VG_INL unsigned ent(const unsigned *, int g) const { return WCell::rlu(we, g); }
+	VG_INL void range(const RectView &v, int g, int rijk, int &bt, int &be) const {
+		if (PRD) { bt = v.bstart[rijk]; be = v.bstart[rijk + 1]; }
+		else { bt = WCell::rli(wbt, g); be = WCell::rli(wbe, g); }
+	}
+	VG_INL void block(int bt, int be) { (void)bt; ce = be; have = false; }
+	VG_INL void part(const RectView &v, int idx, dbl4 &o, int &pid) {
+		if (!have || idx >= cb + 64) {
+			have = true;
+			cb = idx;
+			const int t = idx + (int)__lane_id();
+			if (t < ce) { cp = v.P[t]; cpid = v.pid[t]; }
+		}
+		const int l = idx - cb;
+		o.x = WCell::rld(cp.x, l); o.y = WCell::rld(cp.y, l); o.z = WCell::rld(cp.z, l); o.w = WCell::rld(cp.w, l);
+		pid = WCell::rli(cpid, l);
+	}
+};
+#endif
+
 // compute_cell (src/v_compute.cc:291-621) for a rectangular container or,
 // with PRD, a periodic one (start cell, frac_pos and region_index of
 // src/container_prd.hh:124-168).  (ijk,s) is the particle's block and
@@ -335,6 +412,8 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 	RadPolicy<POLY> rp;
 	rp.init(me.w, v.max_radius);
 	BlockSearch<CELL> bs(c, stamp, i, j, k);
+	CandFetch<CELL, PRD> cf;
+	cf.block(b0, b0 + co0);
 	// Phase-2 bookkeeping is deferred: bit g of p2_cut says block g was
 	// not skipped by the radius test, so its outside neighbours were due.
 	unsigned long long p2_cut = 0;
@@ -366,8 +445,8 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 		while (!have) {
 			if (bt < be) {
 				// Next particle of the current block (src/v_compute.cc:402-428).
-				const dbl4 o = v.P[bt];
-				pid = v.pid[bt];
+				dbl4 o;
+				cf.part(v, bt, o, pid);
 				bt++;
 				px_ = o.x - x2; py_ = o.y - y2; pz_ = o.z - z2;
 				rs = px_ * px_ + py_ * py_ + pz_ * pz_;
@@ -397,8 +476,8 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				// (src/v_compute.cc:307-323).
 				if (t < co0) {
 					if (t == s) { t++; continue; }
-					const dbl4 o = v.P[b0 + t];
-					pid = v.pid[b0 + t];
+					dbl4 o;
+					cf.part(v, b0 + t, o, pid);
 					t++;
 					px_ = o.x - x; py_ = o.y - y; pz_ = o.z - z;
 					rs_raw = px_ * px_ + py_ * py_ + pz_ * pz_;
@@ -428,14 +507,15 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				const int wli = di + VGPU_WL_HGRID * (dj + VGPU_WL_HGRID * dk);
 				radp = v.mrad + wli * 64;
 				e = v.wl + wli * 64;
-				f = e[0]; g = 0;
+				cf.wl_begin(v, radp, e, i, j, k, m1, m2, ci, cj, ck, disp);
+				f = cf.ent(e, 0); g = 0;
 				stage = 2;
 				continue;
 			}
 			if (stage == 2) {
 				if (g >= 63) {
 					// Worklist exhausted (src/v_compute.cc:529).
-					if (rp.ctest(radp[g], mrs)) { stt.wl_exit = g; stt.n_planes = c.n_cuts; return true; }
+					if (rp.ctest(cf.rad(radp, g), mrs)) { stt.wl_exit = g; stt.n_planes = c.n_cuts; return true; }
 					stt.wl_exit = 64;
 					stt.phase = 3;
 					stage = 4;
@@ -449,13 +529,12 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 					mrs = c.max_radius_squared();
 					mrs_b = mrs; since = 0;
 					if (cpi != 8) {
-						const int count_list[8] = {7, 11, 15, 19, 26, 35, 45, 59};
-						next_count = count_list[cpi++];
+						next_count = count_list(cpi++);
 					}
 				}
-				if (rp.ctest(radp[g], mrs)) { stt.wl_exit = g; stt.n_planes = c.n_cuts; return true; }
+				if (rp.ctest(cf.rad(radp, g), mrs)) { stt.wl_exit = g; stt.n_planes = c.n_cuts; return true; }
 				g++;
-				q = e[g]; q ^= m1; q += m2;
+				q = cf.ent(e, g); q ^= m1; q += m2;
 				di = q & 127; di -= 64;
 				dj = (q >> 7) & 127; dj -= 64;
 				dk = (q >> 14) & 127; dk -= 64;
@@ -467,7 +546,8 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				{
 					const int rijk = PRD ? prd_region(v, ci, cj, ck, ei, ej, ek, qx, key++, stt.img)
 							     : rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
-					bt = v.bstart[rijk]; be = v.bstart[rijk + 1];
+					cf.range(v, g, rijk, bt, be);
+					cf.block(bt, be);
 					x2 = x - qx; y2 = y - qy; z2 = z - qz;
 					bcheck = rp.ctest(crs, mrs);
 				}
@@ -481,7 +561,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				// neighbours (scan_bits_mask_add, src/v_compute.cc:249-264).
 				const unsigned b1 = 1u << 21, b2 = 1u << 22, b3 = 1u << 24, b4 = 1u << 25, b5 = 1u << 27, b6 = 1u << 28;
 				for (int gg = g2_first; gg <= 63 && g2_first > 0; gg++) {
-					q = e[gg]; q ^= m1; q += m2;
+					q = cf.ent(e, gg); q ^= m1; q += m2;
 					ei = (int)(q & 127) - 64 + i; if (ei < 0 || ei >= v.hx) continue;
 					ej = (int)((q >> 7) & 127) - 64 + j; if (ej < 0 || ej >= v.hy) continue;
 					ek = (int)((q >> 14) & 127) - 64 + k; if (ek < 0 || ek >= v.hz) continue;
@@ -519,6 +599,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				const int rijk = PRD ? prd_region(v, ci, cj, ck, ei, ej, ek, qx, key++, stt.img)
 						     : rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
 				bt = v.bstart[rijk]; be = v.bstart[rijk + 1];
+				cf.block(bt, be);
 				x2 = x - qx; y2 = y - qy; z2 = z - qz;
 				bcheck = false;
 			}
@@ -628,8 +709,7 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, int ci, 
 			if (g < 63 && g == next_count) {
 				mrs = c.max_radius_squared();
 				if (cpi != 8) {
-					const int count_list[8] = {7, 11, 15, 19, 26, 35, 45, 59};
-					next_count = count_list[cpi++];
+					next_count = count_list(cpi++);
 				}
 			}
 			if (rp.ctest(a.x, mrs)) { stt.wl_exit = g; stt.n_planes = c.n_cuts; return 1; }

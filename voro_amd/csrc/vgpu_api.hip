@@ -14,6 +14,7 @@
 #include "learn.h"
 #include "prd_host.h"
 #include "vgpu_bin.h"
+#include "wcell_dev.h"
 
 using namespace vgpu;
 
@@ -93,6 +94,9 @@ struct vgpu_container {
 	unsigned last_flags = 0;
 	int fast_waves = 0;
 	int fast_occ = 2;
+	int first_tier = 0;      // 0: wavefront cells, 1: order-3 cells, one per lane
+	int wave_waves = 0;
+	int wave_occ = 4;
 	hipEvent_t ev[5];
 	hipEvent_t ev_list = nullptr;   // learned tier: lists gathered, cell kernel starts
 	double gather_ms = 0;
@@ -103,7 +107,7 @@ struct vgpu_container {
 	DBuf blk, slot, rem, cnt, bstart, tmp, part, P, pid, src, cellblk;
 	DBuf d_mrad, d_wl, d_walls, d_maxr, d_counters;
 	DBuf vol, cent, nfaces, status, nbr_start, arena, csr_off, csr, ctr;
-	DBuf retry0, retry1, retry2;
+	DBuf retry0, retry1, retry2, retry3;
 	DBuf st_planes, st_exit, st_tier;
 	// learned path
 	bool have_pred = false;
@@ -111,7 +115,7 @@ struct vgpu_container {
 	float pred_margin = 0;
 	DBuf d_pred, d_feat, d_cfeat, d_depth, d_cost, d_order, d_hist, lcnt, bmax, boff, list_a, list_b;
 	long long last_fallbacks = 0, last_entries = 0;
-	Workspace ws_fast, ws_wide, ws_huge;
+	Workspace ws_fast, ws_wide, ws_huge, ws_wave;
 	long long nbr_total = 0;
 	bool csr_valid = false;
 	// periodic containers: unit cell, block arrays with image blocks, first
@@ -240,6 +244,35 @@ int default_fast_waves(int occ) {
 	return cus * 4 * occ;
 }
 
+// The wavefront-cell tier over every cell (list == nullptr) or over a device
+// list of cells whose count is at *list_count; leftovers go to out.retry.
+int launch_wave(vgpu_container *h, const RectView &v, int nc, const int *list, const int *list_count, CellOut out,
+		hipStream_t s) {
+	const int occ = h->wave_occ;
+	int waves = h->wave_waves > 0 ? h->wave_waves : default_fast_waves(occ);
+	int cnt = nc;
+	if (list_count) {
+		HIPCHK(hipMemcpyAsync(&cnt, list_count, 4, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipStreamSynchronize(s));
+		if (cnt == 0) return VGPU_OK;
+	}
+	waves = std::max(1, std::min(waves, (cnt + 3) / 4));
+	Workspace &ws = h->ws_wave;
+	int st = ws.ensure(WLayout::bytes, waves);
+	if (st) return st;
+	if ((unsigned long long)ws.stamp_base + (unsigned long long)nc + 2 >= 0xffffffffull) {
+		HIPCHK(hipMemset(ws.buf.p, 0, ws.buf.n));
+		ws.stamp_base = 0;
+	}
+	int *ctr = h->d_counters.as<int>();
+	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 8, 0);
+	if (launch_cells_wave(h->poly, h->prd, occ, waves, s, v, nc, list, list_count, h->cellblk.as<int>(), ctr,
+			      ws.base(), ws.stamp_base, out))
+		return VGPU_HIP_ERROR;
+	ws.stamp_base += (unsigned)nc + 1u;
+	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
+}
+
 // The learned tier: features, depth prediction, list gathering, the
 // list-driven cell kernel.  Cells it does not finish go to retry0.
 int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut out, hipStream_t s) {
@@ -309,9 +342,19 @@ int run_tiers(vgpu_container *h, const RectView &v, int nc, int fw, const int *l
 	int st;
 	int *ctr = h->ctr.as<int>();
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr + 1, 2, 0);
-	out.tier = 0; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
-	if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, list, list_count, out, s))) return st;
-	if (mark_ev2) HIPCHK(hipEventRecord(h->ev[2], s));
+	if (h->first_tier == 0) {
+		// one cell per wavefront; its leftovers (count at ctr[6]) to order-3
+		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr + 6, 1, 0);
+		out.tier = 0; out.retry = h->retry3.as<int>(); out.retry_count = ctr + 6;
+		if ((st = launch_wave(h, v, nc, list, list_count, out, s))) return st;
+		if (mark_ev2) HIPCHK(hipEventRecord(h->ev[2], s));
+		out.tier = 1; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
+		if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, h->retry3.as<int>(), ctr + 6, out, s))) return st;
+	} else {
+		out.tier = 0; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
+		if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, list, list_count, out, s))) return st;
+		if (mark_ev2) HIPCHK(hipEventRecord(h->ev[2], s));
+	}
 	out.tier = 2; out.retry = h->retry2.as<int>(); out.retry_count = ctr + 2;
 	if ((st = launch_tier<TierWide>(h, h->ws_wide, RETRY_WAVES_WIDE, v, nc, h->retry1.as<int>(), ctr + 1, out, s))) return st;
 	out.tier = 3; out.retry = nullptr; out.retry_count = nullptr;
@@ -479,7 +522,7 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 	if ((st = h->vol.ensure(C1 * 8)) || (st = h->cent.ensure(C1 * 24)) || (st = h->nfaces.ensure(C1 * 4)) ||
 	    (st = h->status.ensure(C1 * 4)) || (st = h->nbr_start.ensure(C1 * 4)) || (st = h->arena.ensure((size_t)acap * 4)) ||
 	    (st = h->ctr.ensure(16 * 4)) || (st = h->retry0.ensure(C1 * 4)) || (st = h->retry1.ensure(C1 * 4)) ||
-	    (st = h->retry2.ensure(C1 * 4)))
+	    (st = h->retry2.ensure(C1 * 4)) || (st = h->retry3.ensure(C1 * 4)))
 		return st;
 	const bool stats = (flags & VGPU_F_STATS) != 0;
 	if (stats) {
@@ -514,13 +557,12 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 		out.tier = 1; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
 		if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, h->retry0.as<int>(), ctr + 4, out, s))) return st;
 	} else if (nc > 0) {
-		out.tier = 0; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
-		if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, nullptr, nullptr, out, s))) return st;
-		HIPCHK(hipEventRecord(h->ev[2], s));
+		// the plain tiers: wavefront cells (or order-3 per lane), wide, huge
+		if ((st = run_tiers(h, v, nc, fw, nullptr, nullptr, out, s, true))) return st;
 	} else {
 		HIPCHK(hipEventRecord(h->ev[2], s));
 	}
-	if (nc > 0 && !h->prd) {
+	if (nc > 0 && !h->prd && learned) {
 		out.tier = 2; out.retry = h->retry2.as<int>(); out.retry_count = ctr + 2;
 		if ((st = launch_tier<TierWide>(h, h->ws_wide, RETRY_WAVES_WIDE, v, nc, h->retry1.as<int>(), ctr + 1, out, s))) return st;
 		out.tier = 3; out.retry = nullptr; out.retry_count = nullptr;
@@ -884,7 +926,7 @@ int vgpu_destroy(vgpu_container *h) {
 			&h->bmax, &h->boff, &h->list_a, &h->list_b, &h->d_unit, &h->Pall, &h->pidall, &h->ball, &h->fv,
 			&h->touch, &h->d_flag};
 	for (DBuf *b : bufs) b->release();
-	h->ws_fast.buf.release(); h->ws_wide.buf.release(); h->ws_huge.buf.release();
+	h->ws_fast.buf.release(); h->ws_wide.buf.release(); h->ws_huge.buf.release(); h->ws_wave.buf.release();
 	if (h->events) for (int t = 0; t < 5; t++) (void)hipEventDestroy(h->ev[t]);
 	if (h->ev_list) (void)hipEventDestroy(h->ev_list);
 	delete h;
@@ -1083,6 +1125,16 @@ int vgpu_set_launch(vgpu_container *h, int fast_waves, int fast_occupancy) {
 	if (!h || fast_waves < 0 || fast_occupancy < 0 || fast_occupancy > 4) return VGPU_INVALID_ARGUMENT;
 	h->fast_waves = fast_waves;
 	if (fast_occupancy > 0) h->fast_occ = fast_occupancy < 2 ? 2 : fast_occupancy;
+	return VGPU_OK;
+}
+
+int vgpu_set_first_tier(vgpu_container *h, int tier, int waves, int occupancy) {
+	if (!h || tier < 0 || tier > 1 || waves < 0 || occupancy < 0 || occupancy > 8 || occupancy == 1 ||
+	    (occupancy > 4 && occupancy < 8))
+		return VGPU_INVALID_ARGUMENT;
+	h->first_tier = tier;
+	h->wave_waves = waves;
+	if (occupancy) h->wave_occ = occupancy;
 	return VGPU_OK;
 }
 

@@ -1,0 +1,762 @@
+// Wavefront cell: one Voronoi cell per wavefront, vertex v held in the
+// registers of lane v.
+//
+// The algorithm, the order of every operation and every floating-point
+// expression are those of the order-3 tier (cell3_dev.h, which restates
+// src/cell.cc: nplane 718-870 with search_upward/downward 480-515 and
+// 597-629, definite_max/min 522-595 and 631-708, create_facet 875-1364,
+// max_radius_squared 1907-1917, plane_intersects(_guess) 2366-2410 and the
+// output walks 1584-1616, 1861-1901, 2168-2187, 2660-2678).  What differs is
+// where the cell lives:
+//
+//   per lane v:  x, y, z (doubles, twice the coordinates, as the reference
+//                stores them), the topology word (three neighbouring vertex
+//                indices, 0xff = the reference's -1, and the three edge
+//                relations), the three neighbour ids, and the vertex's
+//                value for the current cutting plane;
+//   uniform:     the vertex count, the delete / visited / search marks as
+//                64-bit masks (bit v = vertex v), the stacks as byte arrays
+//                spread four entries per lane.
+//
+// The cell's control flow is the same in every lane, so it runs on the
+// scalar unit; reading vertex v is a readlane, writing it a lane-masked
+// select, and the plane value of every vertex is one vector expression.
+// A plane that leaves every vertex below -tol cannot cut the cell: the
+// reference's upward search then ends at a definite maximum without a
+// change to the cell (all vertices classify as below, and its ascent and
+// tie search are bounded well inside the capacities here), so that test
+// replaces the search.
+//
+// Cells this tier does not finish set ovf and are recomputed from scratch
+// by the per-lane tiers: more than 64 vertices, more than 64 faces, and the
+// marginal configurations the order-3 tier leaves to the general one (a
+// vertex within tol of a cutting plane, a kept vertex within big_tol of it).
+//
+// CPU builds hold the 64 lanes in arrays, so the same code can be checked
+// against the oracle on a host.
+#pragma once
+#include "vgpu_common.h"
+
+namespace vgpu {
+
+struct WLayout {
+	enum { MAXV = 64, MAXS = 256, W = 6, MAXQ = 256, MAXF = 64 };
+	enum { WD = 2 * W + 1, WBITS = WD * WD * WD, WWORDS = (WBITS + 31) / 32 };
+	// per-wavefront global slab: block-search window and ring only
+	static constexpr unsigned o_bm = 0;                          // WWORDS
+	static constexpr unsigned o_bms = o_bm + WWORDS * 4;         // WWORDS stamps
+	static constexpr unsigned o_qu = o_bms + WWORDS * 4;         // MAXQ ints
+	static constexpr size_t bytes = (o_qu + MAXQ * 4 + 255) & ~255u;
+	enum { NBCAP = MAXF };
+};
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#define VG_WDEV 1
+#else
+#define VG_WDEV 0
+#endif
+
+struct WCell {
+	typedef WLayout L;
+	enum { MAXV = L::MAXV, MAXS = L::MAXS };
+
+#if VG_WDEV
+	double vx, vy, vz;       // vertex `lane`
+	unsigned vt;             // e0 | e1 << 8 | e2 << 16 | (r0 | r1 << 2 | r2 << 4) << 24
+	int vn0, vn1, vn2;       // neighbour ids
+	double vtv;              // value for the current plane
+	unsigned vds, vdm;       // stack entries 4 * lane .. 4 * lane + 3
+	int nbv;                 // output staging: neighbour id number `lane`
+	int lane;
+	__amdgpu_buffer_rsrc_t rs;
+#else
+	double vx[64], vy[64], vz[64];
+	unsigned vt[64];
+	int vn[64][3];
+	double vtv[64];
+	unsigned char vds[MAXS], vdm[MAXS];
+	int nbv[64];
+	char *wsb;
+#endif
+
+	int p, up;
+	double px, py, pz, prsq;
+	double tol, tol_cu, big_tol;
+	int nds;
+	bool ovf, fatal;
+	int n_cuts;
+	unsigned long long delm;     // deletion marks
+	unsigned long long flipm;    // tie-search marks (the reference negates back pointers)
+	unsigned long long vis0, vis1, vis2;   // face-walk marks per edge slot
+
+	// Value (never lvalue) selection: a conditional over members would make
+	// the compiler address the cell in memory.
+	template<class T> static VG_INL T sel3(int i, T a, T b, T c) { return i == 0 ? a : (i == 1 ? b : c); }
+
+	// ---- lane primitives ----
+#if VG_WDEV
+	static VG_INL unsigned rlu(unsigned a, int i) { return (unsigned)__builtin_amdgcn_readlane((int)a, i); }
+	static VG_INL int rli(int a, int i) { return __builtin_amdgcn_readlane(a, i); }
+	static VG_INL double rld(double a, int i) {
+		const unsigned long long b = __builtin_bit_cast(unsigned long long, a);
+		const unsigned lo = rlu((unsigned)b, i), hi = rlu((unsigned)(b >> 32), i);
+		return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+	}
+	static VG_INL double rfd(double a) {
+		const unsigned long long b = __builtin_bit_cast(unsigned long long, a);
+		const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+		const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+		return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+	}
+	VG_INL double X(int v) const { return rld(vx, v); }
+	VG_INL double Y(int v) const { return rld(vy, v); }
+	VG_INL double Z(int v) const { return rld(vz, v); }
+	VG_INL double T(int v) const { return rld(vtv, v); }
+	VG_INL unsigned TW(int v) const { return rlu(vt, v); }
+	VG_INL void setXYZ(int v, double a, double b, double c) {
+		const bool m = lane == v;
+		vx = m ? a : vx; vy = m ? b : vy; vz = m ? c : vz;
+	}
+	VG_INL void setTW(int v, unsigned w) { vt = lane == v ? w : vt; }
+	VG_INL int N(int v, int i) const { return sel3(i, rli(vn0, v), rli(vn1, v), rli(vn2, v)); }
+	VG_INL void setN3(int v, int a, int b, int c) {
+		const bool m = lane == v;
+		vn0 = m ? a : vn0; vn1 = m ? b : vn1; vn2 = m ? c : vn2;
+	}
+	VG_INL int DS(int i) const { return (int)((rlu(vds, i >> 2) >> (8 * (i & 3))) & 0xffu); }
+	VG_INL void setDS(int i, int v) {
+		const unsigned sh = 8u * (unsigned)(i & 3);
+		const unsigned w = (rlu(vds, i >> 2) & ~(0xffu << sh)) | ((unsigned)v & 0xffu) << sh;
+		vds = lane == (i >> 2) ? w : vds;
+	}
+	VG_INL int DM(int i) const { return (int)((rlu(vdm, i >> 2) >> (8 * (i & 3))) & 0xffu); }
+	VG_INL void setDM(int i, int v) {
+		const unsigned sh = 8u * (unsigned)(i & 3);
+		const unsigned w = (rlu(vdm, i >> 2) & ~(0xffu << sh)) | ((unsigned)v & 0xffu) << sh;
+		vdm = lane == (i >> 2) ? w : vdm;
+	}
+	VG_INL void put_nb(int i, int v) { nbv = lane == i ? v : nbv; }
+	VG_INL int nb_lane() const { return nbv; }
+	// vertex-parallel: bit v set for live vertices v whose predicate holds
+	VG_INL unsigned long long live_ballot(bool b) const {
+		const unsigned long long m = __ballot(b);
+		return p >= 64 ? m : (m & ((1ull << p) - 1));
+	}
+#else
+	VG_INL double X(int v) const { return vx[v]; }
+	VG_INL double Y(int v) const { return vy[v]; }
+	VG_INL double Z(int v) const { return vz[v]; }
+	VG_INL double T(int v) const { return vtv[v]; }
+	VG_INL unsigned TW(int v) const { return vt[v]; }
+	VG_INL void setXYZ(int v, double a, double b, double c) { vx[v] = a; vy[v] = b; vz[v] = c; }
+	VG_INL void setTW(int v, unsigned w) { vt[v] = w; }
+	VG_INL int N(int v, int i) const { return vn[v][i]; }
+	VG_INL void setN3(int v, int a, int b, int c) { vn[v][0] = a; vn[v][1] = b; vn[v][2] = c; }
+	VG_INL int DS(int i) const { return vds[i]; }
+	VG_INL void setDS(int i, int v) { vds[i] = (unsigned char)v; }
+	VG_INL int DM(int i) const { return vdm[i]; }
+	VG_INL void setDM(int i, int v) { vdm[i] = (unsigned char)v; }
+	VG_INL void put_nb(int i, int v) { nbv[i] = v; }
+	VG_INL int nb_lane() const { return nbv[0]; }
+#endif
+
+	// ---- per-wavefront slab (block-search arrays) ----
+	template<class T> VG_INL T ld(unsigned off) const {
+#if VG_WDEV
+		static_assert(sizeof(T) == 4, "slab words");
+		return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+#else
+		T v;
+		__builtin_memcpy(&v, wsb + off, sizeof(T));
+		return v;
+#endif
+	}
+	template<class T> VG_INL void st(unsigned off, T v) const {
+#if VG_WDEV
+		static_assert(sizeof(T) == 4, "slab words");
+		__builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, off, 0, 0);
+#else
+		__builtin_memcpy(wsb + off, &v, sizeof(T));
+#endif
+	}
+	template<class T>
+	struct Ref {
+		const WCell *c;
+		unsigned off;
+		VG_INL operator T() const { return c->template ld<T>(off); }
+		VG_INL const Ref &operator=(T v) const { c->template st<T>(off, v); return *this; }
+		VG_INL const Ref &operator|=(T b) const { c->template st<T>(off, (T)(c->template ld<T>(off) | b)); return *this; }
+	};
+	template<class T> VG_INL Ref<T> A(unsigned off, int i) const { return Ref<T>{this, off + (unsigned)i * (unsigned)sizeof(T)}; }
+
+	// ---- topology fields ----
+	static VG_INL int twE(unsigned w, int i) { int e = (int)((w >> (8 * i)) & 0xff); return e == 0xff ? -1 : e; }
+	static VG_INL int twR(unsigned w, int i) { return (int)((w >> (24 + 2 * i)) & 3); }
+	static VG_INL unsigned twSetE(unsigned w, int i, int e) {
+		const unsigned sh = 8u * (unsigned)i;
+		return (w & ~(0xffu << sh)) | (((unsigned)e & 0xffu) << sh);
+	}
+	static VG_INL unsigned twSetR(unsigned w, int i, int r) {
+		const unsigned sh = 24u + 2u * (unsigned)i;
+		return (w & ~(3u << sh)) | ((unsigned)r << sh);
+	}
+	VG_INL int E(int v, int i) const { return twE(TW(v), i); }
+	VG_INL int R(int v, int i) const { return twR(TW(v), i); }
+	VG_INL void setE(int v, int i, int e) { setTW(v, twSetE(TW(v), i, e)); }
+	VG_INL void setER(int v, int i, int e, int r) { setTW(v, twSetR(twSetE(TW(v), i, e), i, r)); }
+	VG_INL bool DEL(int v) const { return (delm >> v) & 1ull; }
+	VG_INL void setDEL(int v) { delm |= 1ull << v; }
+	VG_INL bool FL(int v) const { return (flipm >> v) & 1ull; }
+	VG_INL void flip(int v) { flipm ^= 1ull << v; }
+
+	static VG_INL int cyc_up(int a) { return a == 2 ? 0 : a + 1; }
+
+	VG_FN void setup(char *slab, int lane_, double max_len_sq) {
+#if VG_WDEV
+		lane = lane_;
+		rs = __builtin_amdgcn_make_buffer_rsrc(slab, 0, (int)L::bytes, 0x00020000);
+#else
+		(void)lane_;
+		wsb = slab;
+#endif
+		set_tol(max_len_sq);
+	}
+	// src/cell.cc:20-27: tol = 10 eps max_len_sq, tol_cu = tol^1.5, big_tol = 20 tol.
+	VG_INL void set_tol(double max_len_sq) {
+		const double eps = 2.220446049250313e-16;
+		tol = 10.0 * eps * max_len_sq;
+		tol_cu = tol * sqrt(tol);
+		big_tol = 20.0 * tol;
+	}
+
+	VG_INL bool room_v(int v) { if (v >= MAXV) { ovf = true; return false; } return true; }
+	VG_INL bool push_ds(int v) { if (nds >= MAXS) { ovf = true; return false; } setDS(nds++, v); return true; }
+
+	// A fresh vertex: coordinates, no edges yet, no marks.
+	VG_INL void new_vertex(int v, double x, double y, double z) {
+		setXYZ(v, x, y, z);
+		setTW(v, 0x00ffffffu);
+		delm &= ~(1ull << v);
+		flipm &= ~(1ull << v);
+	}
+
+	// Vertex `from` takes index `to` (src/cell.cc:849-861).
+	VG_FN void move_vertex(int from, int to) {
+		setXYZ(to, X(from), Y(from), Z(from));
+		const unsigned w = TW(from);
+		setTW(to, w);
+		setN3(to, N(from, 0), N(from, 1), N(from, 2));
+		const unsigned long long bt = 1ull << to;
+		delm = DEL(from) ? (delm | bt) : (delm & ~bt);
+		flipm &= ~bt;
+		for (int i = 0; i < 3; i++) setE(twE(w, i), twR(w, i), to);
+	}
+
+	// Box start cell, faces -1..-6 (src/cell.cc:294-318, 2567-2580); corner v
+	// has bits x,y,z = v&1, v>>1&1, v>>2&1 and relations 2,1,0.
+	VG_FN void init_box(double xmin, double xmax, double ymin, double ymax, double zmin, double zmax) {
+		ovf = false; fatal = false; n_cuts = 0;
+		up = 0;
+		p = 8;
+		delm = 0; flipm = 0;
+		xmin *= 2; xmax *= 2; ymin *= 2; ymax *= 2; zmin *= 2; zmax *= 2;
+#if VG_WDEV
+		if (lane < 8) box_corner(lane, xmin, xmax, ymin, ymax, zmin, zmax);
+#else
+		for (int v = 0; v < 8; v++) box_corner(v, xmin, xmax, ymin, ymax, zmin, zmax);
+#endif
+	}
+	// Corner v of the box: edges to v^(1+par), v^4, v^(2-par) with relations
+	// 2, 1, 0, faces z, then x and y (x first for odd parity), where par is
+	// the parity of v -- the reference's corner table in closed form.
+	VG_INL void box_corner(int v, double xmin, double xmax, double ymin, double ymax, double zmin, double zmax) {
+		const int par = __builtin_popcount((unsigned)v) & 1;
+		const unsigned e = (unsigned)(v ^ (1 + par)) | (unsigned)(v ^ 4) << 8 | (unsigned)(v ^ (2 - par)) << 16;
+		const int xf = -1 - (v & 1), yf = -3 - ((v >> 1) & 1);
+#if VG_WDEV
+		vx = (v & 1) ? xmax : xmin; vy = (v & 2) ? ymax : ymin; vz = (v & 4) ? zmax : zmin;
+		vt = e | (6u << 24);
+		vn0 = -5 - (v >> 2); vn1 = par ? xf : yf; vn2 = par ? yf : xf;
+#else
+		setXYZ(v, (v & 1) ? xmax : xmin, (v & 2) ? ymax : ymin, (v & 4) ? zmax : zmin);
+		setTW(v, e | (6u << 24));
+		setN3(v, -5 - (v >> 2), par ? xf : yf, par ? yf : xf);
+#endif
+	}
+
+	// The unit Voronoi cell of a periodic container as the start cell
+	// (src/container_prd.hh:124-130, src/cell.cc:71-95).  A vertex of
+	// another order than 3 leaves the cell to the per-lane tiers.
+	VG_FN void init_unit(const UnitVert *__restrict__ u, int np) {
+		ovf = false; fatal = false; n_cuts = 0;
+		up = 0;
+		delm = 0; flipm = 0;
+		if (np > MAXV) { ovf = true; return; }
+		p = np;
+#if VG_WDEV
+		bool bad = false;
+		if (lane < np) {
+			const UnitVert a = u[lane];
+			bad = a.nu != 3;
+			vx = a.x; vy = a.y; vz = a.z;
+			vt = (unsigned)(a.ed[0] & 0xff) | (unsigned)(a.ed[1] & 0xff) << 8 | (unsigned)(a.ed[2] & 0xff) << 16 |
+			     (unsigned)((a.rel[0] & 3) | (a.rel[1] & 3) << 2 | (a.rel[2] & 3) << 4) << 24;
+			vn0 = vn1 = vn2 = 0;
+		}
+		if (__ballot(bad)) ovf = true;
+#else
+		for (int v = 0; v < np; v++) {
+			const UnitVert a = u[v];
+			if (a.nu != 3) { ovf = true; return; }
+			setXYZ(v, a.x, a.y, a.z);
+			setTW(v, (unsigned)(a.ed[0] & 0xff) | (unsigned)(a.ed[1] & 0xff) << 8 | (unsigned)(a.ed[2] & 0xff) << 16 |
+				 (unsigned)((a.rel[0] & 3) | (a.rel[1] & 3) << 2 | (a.rel[2] & 3) << 4) << 24);
+			setN3(v, 0, 0, 0);
+		}
+#endif
+	}
+
+	// Plane values of every vertex, the expression of src/cell.cc:2043-2059.
+	// Returns true if some live vertex is not below -tol.
+	VG_FN bool plane_values() {
+#if VG_WDEV
+		double a = vx * px;
+		a += vy * py;
+		a += vz * pz - prsq;
+		vtv = a;
+		return live_ballot(!(a < -tol)) != 0;
+#else
+		bool any = false;
+		for (int v = 0; v < 64; v++) {
+			double a = vx[v] * px;
+			a += vy[v] * py;
+			a += vz[v] * pz - prsq;
+			vtv[v] = a;
+			if (v < p && !(a < -tol)) any = true;
+		}
+		return any;
+#endif
+	}
+
+	VG_INL unsigned m_test(int n, double &ans) {
+		if (n < 0 || n >= MAXV) { fatal = ovf = true; ans = 0; return 1u; }
+		ans = T(n);
+		return ans < -tol ? 0u : (ans > tol ? 2u : 1u);
+	}
+	// m_test that also hands back the vertex's coordinates, and flags a kept
+	// vertex within big_tol of the plane (src/cell.cc:2072-2083).
+	VG_INL unsigned m_testx3(int n, double &ans, double &x, double &y, double &z) {
+		if (n < 0 || n >= MAXV) { fatal = ovf = true; ans = 0; x = y = z = 0; return 1u; }
+		ans = T(n);
+		x = X(n); y = Y(n); z = Z(n);
+		const unsigned r = ans < -tol ? 0u : (ans > tol ? 2u : 1u);
+		if (r == 0 && ans > -big_tol) ovf = true;
+		return r;
+	}
+
+	// src/cell.cc:522-595 (the member `up` is pushed in the second loop,
+	// src/cell.cc:546-549, as there).  ta, tb, tc: the values of lp's
+	// neighbours.
+	VG_FN bool definite_max(int &lp, int &ls, double &l, double &u, unsigned &uw, double ta, double tb, double tc) {
+		int tp = lp, ts, qp = 0;
+		unsigned qw;
+		double q;
+		for (ts = 0; ts < 3; ts++) {
+			qp = E(tp, ts);
+			q = sel3(ts, ta, tb, tc);
+			if (q > l - big_tol) break;
+		}
+		if (ts == 3) return true;
+		int nst = 0;
+		flip(lp); flip(qp);
+		setDM(nst++, qp);
+		ts++;
+		while (ts < 3) {
+			qp = E(tp, ts);
+			m_test(qp, q);
+			if (q > l - big_tol) {
+				if (nst >= MAXS) { ovf = true; return true; }
+				setDM(nst++, up); flip(up);
+			}
+			ts++;
+		}
+		int spp = 0;
+		while (spp < nst) {
+			tp = DM(spp++);
+			for (ts = 0; ts < 3; ts++) {
+				qp = E(tp, ts);
+				if (FL(qp)) continue;
+				qw = m_test(qp, q);
+				if (q > l) {
+					flip(lp);
+					lp = tp; ls = ts;
+					m_test(lp, l);
+					up = qp; uw = qw; u = q;
+					while (nst > 0) flip(DM(--nst));
+					return false;
+				}
+				if (q > l - big_tol) {
+					if (nst >= MAXS) { ovf = true; return true; }
+					setDM(nst++, qp); flip(qp);
+				}
+			}
+		}
+		flip(lp);
+		while (nst > 0) flip(DM(--nst));
+		return true;
+	}
+
+	// src/cell.cc:480-515 with definite_max folded in.
+	VG_FN bool search_upward(unsigned &uw, int &lp, int &ls, int &us, double &l, double &u) {
+		int vs = -1, guard = 0;
+		lp = up; l = u;
+		for (;;) {
+			const unsigned w = TW(lp);
+			const int e0 = twE(w, 0), e1 = twE(w, 1), e2 = twE(w, 2);
+			double ta, tb, tc;
+			const unsigned wa = m_test(e0, ta), wb = m_test(e1, tb), wc = m_test(e2, tc);
+			for (ls = 0; ls < 3; ls++) {
+				if (ls == vs) continue;
+				up = sel3(ls, e0, e1, e2);
+				uw = sel3(ls, wa, wb, wc);
+				u = sel3(ls, ta, tb, tc);
+				if (u > l) break;
+			}
+			if (ls == 3 && definite_max(lp, ls, l, u, uw, ta, tb, tc)) { up = lp; return false; }
+			if (uw != 0) break;
+			if (ovf || ++guard > 4 * MAXV) { fatal = ovf = true; return false; }
+			vs = R(lp, ls); lp = up; l = u;
+		}
+		us = R(lp, ls);
+		return true;
+	}
+
+	// src/cell.cc:631-708 (the argument `lp` is pushed in the second loop,
+	// src/cell.cc:654-657, as there).  ta, tb, tc: the values of up's
+	// neighbours.
+	VG_FN bool definite_min(int &lp, int &us, double &l, double &u, unsigned &lw, double ta, double tb, double tc) {
+		int tp = up, ts, qp = 0;
+		unsigned qw;
+		double q;
+		for (ts = 0; ts < 3; ts++) {
+			qp = E(tp, ts);
+			q = sel3(ts, ta, tb, tc);
+			if (q < u + big_tol) break;
+		}
+		if (ts == 3) return true;
+		int nst = 0;
+		flip(up); flip(qp);
+		setDM(nst++, qp);
+		ts++;
+		while (ts < 3) {
+			qp = E(tp, ts);
+			m_test(qp, q);
+			if (q < u + big_tol) {
+				if (nst >= MAXS) { ovf = true; return true; }
+				setDM(nst++, lp); flip(lp);
+			}
+			ts++;
+		}
+		int spp = 0;
+		while (spp < nst) {
+			tp = DM(spp++);
+			for (ts = 0; ts < 3; ts++) {
+				qp = E(tp, ts);
+				if (FL(qp)) continue;
+				qw = m_test(qp, q);
+				if (q < u) {
+					flip(up);
+					up = tp; us = ts;
+					m_test(up, u);
+					lp = qp; lw = qw; l = q;
+					while (nst > 0) flip(DM(--nst));
+					return false;
+				}
+				if (q < u + big_tol) {
+					if (nst >= MAXS) { ovf = true; return true; }
+					setDM(nst++, qp); flip(qp);
+				}
+			}
+		}
+		flip(up);
+		while (nst > 0) flip(DM(--nst));
+		return true;
+	}
+
+	// src/cell.cc:597-629 with definite_min folded in.
+	VG_FN bool search_downward(unsigned &lw, int &lp, int &ls, int &us, double &l, double &u) {
+		int vs = -1, guard = 0;
+		for (;;) {
+			const unsigned w = TW(up);
+			const int e0 = twE(w, 0), e1 = twE(w, 1), e2 = twE(w, 2);
+			double ta, tb, tc;
+			const unsigned wa = m_test(e0, ta), wb = m_test(e1, tb), wc = m_test(e2, tc);
+			for (us = 0; us < 3; us++) {
+				if (us == vs) continue;
+				lp = sel3(us, e0, e1, e2);
+				lw = sel3(us, wa, wb, wc);
+				l = sel3(us, ta, tb, tc);
+				if (u > l) break;
+			}
+			if (us == 3 && definite_min(lp, us, l, u, lw, ta, tb, tc)) return false;
+			if (lw != 2) break;
+			if (ovf || ++guard > 4 * MAXV) { fatal = ovf = true; return false; }
+			vs = R(up, us); up = lp; u = l;
+		}
+		ls = R(up, us);
+		return true;
+	}
+
+	// src/cell.cc:875-1364 for a plane that cuts edges cleanly.  Returns
+	// true if the cell was deleted (or on overflow).
+	VG_FN bool create_facet(int lp, int ls, double l, int us, double u, int pid) {
+		int qp, qs, cp, cs, rp;
+		unsigned lw;
+		double q, r;
+		if (!room_v(p)) return true;
+		if (!push_ds(up)) return true;
+		r = u / (u - l); l = 1 - r;
+		new_vertex(p, X(lp) * r + X(up) * l, Y(lp) * r + Y(up) * l, Z(lp) * r + Z(up) * l);
+		setN3(p, pid, N(up, us), N(lp, ls));
+		setE(up, us, -1);
+		setER(lp, ls, p, 1);
+		setTW(p, twSetR(twSetE(0x00ffffffu, 1, lp), 1, ls));
+		cs = 2;
+		qs = cyc_up(us);
+		qp = up; q = u;
+		double qx = X(up), qy = Y(up), qz = Z(up);
+		cp = p; rp = p; p++;
+		int guard = 0;
+		while (qp != up || qs != us) {
+			if (++guard > 16 * MAXV) { fatal = ovf = true; return true; }
+			lp = E(qp, qs);
+			double lx, ly, lz;
+			lw = m_testx3(lp, l, lx, ly, lz);
+			if (ovf) return true;
+			if (lw == 2) {
+				qs = cyc_up(R(qp, qs));
+				qp = lp;
+				q = l;
+				qx = lx; qy = ly; qz = lz;
+				if (!push_ds(qp)) return true;
+			} else if (lw == 0) {
+				if (!room_v(p)) return true;
+				r = q / (q - l); l = 1 - r;
+				new_vertex(p, lx * r + qx * l, ly * r + qy * l, lz * r + qz * l);
+				ls = R(qp, qs);
+				setN3(p, pid, N(qp, qs), N(lp, ls));
+				setTW(p, twSetR(twSetR(twSetE(twSetE(0x00ffffffu, 0, cp), 1, lp), 0, cs), 1, ls));
+				setER(lp, ls, p, 1);
+				setER(cp, cs, p, 0);
+				setE(qp, qs, -1);
+				qs = cyc_up(qs);
+				cp = p++;
+				cs = 2;
+			} else {
+				ovf = true;      // a vertex on the plane: marginal case
+				return true;
+			}
+		}
+		setER(cp, cs, rp, 0);
+		setER(rp, 0, cp, cs);
+		return false;
+	}
+
+	// src/cell.cc:718-870.  Returns false if the cell was deleted or a
+	// capacity was exceeded (ovf set).
+	VG_FN bool nplane(double x, double y, double z, double rsq, int pid) {
+		bool r = nplane_impl(x, y, z, rsq, pid);
+		return ovf ? false : r;
+	}
+
+	VG_FN bool nplane_impl(double x, double y, double z, double rsq, int pid) {
+		n_cuts++;
+		int j, lp = up, cp, qp;
+		int us = 0, ls = 0;
+		unsigned uw, lw;
+		double u, l = 0;
+		up = 0;
+		px = x; py = y; pz = z; prsq = rsq;
+		if (!plane_values()) return true;   // every vertex below -tol: no cut
+		uw = m_test(up, u);
+		if (uw == 2) {
+			if (!search_downward(lw, lp, ls, us, l, u)) return false;
+			if (lw == 1) { ovf = true; return false; }     // vertex on the plane
+		} else if (uw == 0) {
+			if (!search_upward(uw, lp, ls, us, l, u)) return true;
+			if (uw == 1) { ovf = true; return false; }     // vertex on the plane
+		} else {
+			ovf = true;                                      // vertex 0 on the plane
+			return false;
+		}
+		if (ovf) return false;
+		nds = 0;
+		if (create_facet(lp, ls, l, us, u, pid)) return false;
+		// Delete the cut vertices and everything reachable from them
+		// (src/cell.cc:786-863).
+		int d = 0;
+		while (d < nds) {
+			j = DS(d);
+			if (!DEL(j)) { setDEL(j); d++; }
+			else setDS(d, DS(--nds));
+		}
+		for (d = 0; d < nds; d++) {
+			cp = DS(d);
+			const unsigned w = TW(cp);
+			for (int e = 0; e < 3; e++) {
+				qp = twE(w, e);
+				if (qp != -1 && !DEL(qp)) {
+					if (!push_ds(qp)) return false;
+					setDEL(qp);
+				}
+			}
+		}
+		up = 0;
+		while (nds > 0) {
+			--p;
+			while (DEL(p)) {
+				if (p == 0) { fatal = ovf = true; return false; }
+				--p;
+			}
+			up = DS(--nds);
+			if (up < p) move_vertex(p, up);
+			else up = p++;
+		}
+		return true;
+	}
+
+	// src/cell.cc:1907-1917: the largest squared vertex distance (maxima are
+	// exact, so the order of the comparisons does not matter).
+	VG_FN double max_radius_squared() {
+#if VG_WDEV
+		double s = vx * vx;
+		s += vy * vy;
+		s += vz * vz;
+		if (lane >= p) s = 0;
+		for (int o = 32; o > 0; o >>= 1) {
+			const double t = __shfl_xor(s, o);
+			s = t > s ? t : s;
+		}
+		return rfd(s);
+#else
+		double r = X(0) * X(0) + Y(0) * Y(0) + Z(0) * Z(0), s;
+		for (int v = 1; v < p; v++) {
+			double a = X(v), b = Y(v), c = Z(v);
+			s = a * a;
+			s += b * b;
+			s += c * c;
+			if (s > r) r = s;
+		}
+		return r;
+#endif
+	}
+
+	// plane_intersects(_guess), src/cell.cc:2366-2410: true if the value of
+	// the start vertex is >= rsq or that of any vertex exceeds rsq (the
+	// guess samples are vertices too); `up` follows the guess walk.
+	VG_FN bool plane_probe(double x, double y, double z, double rsq, bool guess) {
+		if (guess) up = 0;
+#if VG_WDEV
+		const double gl = x * vx + y * vy + z * vz;
+		double g = rld(gl, up);
+		if (!(g < rsq)) return true;
+		if (guess) {
+			int ca = 1, cc = p >> 3, mp = 1;
+			while (ca < cc) {
+				const double m = rld(gl, mp);
+				if (m > g) {
+					if (m > rsq) return true;
+					g = m; up = mp;
+				}
+				ca += mp++;
+			}
+		}
+		return live_ballot(gl > rsq) != 0;
+#else
+		double g = x * X(up) + y * Y(up) + z * Z(up);
+		if (!(g < rsq)) return true;
+		if (guess) {
+			int ca = 1, cc = p >> 3, mp = 1;
+			while (ca < cc) {
+				double m = x * X(mp) + y * Y(mp) + z * Z(mp);
+				if (m > g) {
+					if (m > rsq) return true;
+					g = m; up = mp;
+				}
+				ca += mp++;
+			}
+		}
+		for (int tp = 0; tp < p; tp++)
+			if (x * X(tp) + y * Y(tp) + z * Z(tp) > rsq) return true;
+		return false;
+#endif
+	}
+
+	// Volume, centroid, face count and neighbour ids in one face walk
+	// (src/cell.cc:1584-1616, 1861-1901, 2168-2187, 2660-2678).  Walked
+	// edges are marked in the visited masks (the reference negates them).
+	VG_INL bool vis(int v, int i) const {
+		const unsigned long long a = vis0, b = vis1, c = vis2;
+		return (sel3(i, a, b, c) >> v) & 1ull;
+	}
+	VG_INL void setvis(int v, int i) {
+		const unsigned long long b = 1ull << v;
+		if (i == 0) vis0 |= b; else if (i == 1) vis1 |= b; else vis2 |= b;
+	}
+	template<class NbSink>
+	VG_FN void outputs(double &vol, double &cx, double &cy, double &cz, int &nf, NbSink &nb) {
+		const double fe = 1 / 48.0;
+		double v = 0, tvol;
+		double ux, uy, uz, vx_, vy_, vz_, wx, wy, wz;
+		const double x0 = X(0), y0 = Y(0), z0 = Z(0);
+		vis0 = vis1 = vis2 = 0;
+		cx = cy = cz = 0;
+		nf = 0;
+		for (int i = 1; i < p; i++) {
+			const double xi = X(i), yi = Y(i), zi = Z(i);
+			ux = x0 - xi;
+			uy = y0 - yi;
+			uz = z0 - zi;
+			for (int j = 0; j < 3; j++) {
+				if (vis(i, j)) continue;
+				const unsigned wi = TW(i);
+				int k = twE(wi, j);
+				nb.put(nf++, N(i, j));
+				setvis(i, j);
+				int l = cyc_up(twR(wi, j));
+				vx_ = X(k) - x0;
+				vy_ = Y(k) - y0;
+				vz_ = Z(k) - z0;
+				unsigned wk = TW(k);
+				int m = twE(wk, l);
+				setvis(k, l);
+				int guard = 0;
+				while (m != i) {
+					if (++guard > 4 * MAXV) { fatal = ovf = true; return; }
+					int n = cyc_up(twR(wk, l));
+					wx = X(m) - x0;
+					wy = Y(m) - y0;
+					wz = Z(m) - z0;
+					tvol = ux * vy_ * wz + uy * vz_ * wx + uz * vx_ * wy - uz * vy_ * wx - uy * vx_ * wz - ux * vz_ * wy;
+					v += tvol;
+					cx += (wx + vx_ - ux) * tvol;
+					cy += (wy + vy_ - uy) * tvol;
+					cz += (wz + vz_ - uz) * tvol;
+					k = m; l = n; vx_ = wx; vy_ = wy; vz_ = wz;
+					wk = TW(k);
+					m = twE(wk, l);
+					setvis(k, l);
+				}
+			}
+		}
+		vol = v * fe;
+		if (v > tol_cu) {
+			v = 0.125 / v;
+			cx = cx * v + 0.5 * x0;
+			cy = cy * v + 0.5 * y0;
+			cz = cz * v + 0.5 * z0;
+		} else cx = cy = cz = 0;
+	}
+};
+
+}  // namespace vgpu
