@@ -297,8 +297,10 @@ VG_INL int prd_region(const RectView &v, int ci, int cj, int ck, int ei, int ej,
 }
 
 // Candidate data of compute_cell_rect: the worklist entries and radii, the
-// particle range of a worklist block, and the particles themselves.  The
-// per-lane tiers load each value where it is used.
+// particle range of a worklist block, and the particles themselves, each
+// loaded where it is used.  (Staging them in the lanes of a wavefront cell,
+// 64 at a time, measured slower: the staging registers cost more than the
+// loads they save.)
 template<class CELL, bool PRD>
 struct CandFetch {
 	VG_INL void wl_begin(const RectView &, const double *, const unsigned *, int, int, int, unsigned, unsigned, int,
@@ -312,61 +314,6 @@ struct CandFetch {
 	VG_INL void part(const RectView &v, int idx, dbl4 &o, int &pid) { o = v.P[idx]; pid = v.pid[idx]; }
 };
 
-#if VG_WDEV
-// The wavefront cell stages them in lanes, so that one vector load replaces a
-// chain of dependent scalar ones: lane g holds worklist entry g, its radius
-// and (rectangular containers) its block's particle range, computed as the
-// step computes it; a block's particles are loaded 64 at a time, particle
-// cb + l in lane l.  Values are then read with readlane.
-template<bool PRD>
-struct CandFetch<WCell, PRD> {
-	int lane;
-	double wr;            // radius of worklist entry `lane`
-	unsigned we;          // entry `lane`
-	int wbt, wbe;         // particle range of its block
-	dbl4 cp;              // particle cb + lane
-	int cpid;
-	int cb, ce;           // staged range start; end of the current block
-	bool have;
-	VG_INL void wl_begin(const RectView &v, const double *radp, const unsigned *e, int i, int j, int k, unsigned m1,
-			     unsigned m2, int ci, int cj, int ck, int disp) {
-		lane = (int)__lane_id();
-		wr = radp[lane];
-		we = e[lane];
-		if (!PRD) {
-			unsigned q = we ^ m1;
-			q += m2;
-			const int ei = (int)(q & 127) - 64 + i, ej = (int)((q >> 7) & 127) - 64 + j,
-				  ek = (int)((q >> 14) & 127) - 64 + k;
-			wbt = wbe = 0;
-			if (lane >= 1 && ei >= 0 && ei < v.hx && ej >= 0 && ej < v.hy && ek >= 0 && ek < v.hz) {
-				double qx, qy, qz;
-				const int rijk = rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
-				wbt = v.bstart[rijk];
-				wbe = v.bstart[rijk + 1];
-			}
-		}
-	}
-	VG_INL double rad(const double *, int g) const { return WCell::rld(wr, g); }
-	VG_INL unsigned ent(const unsigned *, int g) const { return WCell::rlu(we, g); }
-	VG_INL void range(const RectView &v, int g, int rijk, int &bt, int &be) const {
-		if (PRD) { bt = v.bstart[rijk]; be = v.bstart[rijk + 1]; }
-		else { bt = WCell::rli(wbt, g); be = WCell::rli(wbe, g); }
-	}
-	VG_INL void block(int bt, int be) { (void)bt; ce = be; have = false; }
-	VG_INL void part(const RectView &v, int idx, dbl4 &o, int &pid) {
-		if (!have || idx >= cb + 64) {
-			have = true;
-			cb = idx;
-			const int t = idx + (int)__lane_id();
-			if (t < ce) { cp = v.P[t]; cpid = v.pid[t]; }
-		}
-		const int l = idx - cb;
-		o.x = WCell::rld(cp.x, l); o.y = WCell::rld(cp.y, l); o.z = WCell::rld(cp.z, l); o.w = WCell::rld(cp.w, l);
-		pid = WCell::rli(cpid, l);
-	}
-};
-#endif
 
 // compute_cell (src/v_compute.cc:291-621) for a rectangular container or,
 // with PRD, a periodic one (start cell, frac_pos and region_index of

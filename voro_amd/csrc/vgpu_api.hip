@@ -573,7 +573,7 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 	if (nbrs && (flags & VGPU_F_CSR) && nc > 0) {
 		if ((st = h->csr_off.ensure((C1 + 1) * 4))) return st;
 		hipLaunchKernelGGL(k_face_counts, dim3(nblk(nc + 1, 256)), dim3(256), 0, s, nc, h->status.as<int>(),
-				   h->nfaces.as<int>(), h->csr_off.as<int>());
+				   h->nfaces.as<int>(), h->nbr_start.as<int>(), h->csr_off.as<int>());
 		// scan in place into a second buffer region
 		if ((st = h->tmp.ensure((C1 + 1) * 4))) return st;
 		if ((st = scan_excl(h->part, h->csr_off.as<int>(), h->tmp.as<int>(), nc + 1, s))) return st;
@@ -599,7 +599,8 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 		HIPCHK(hipEventElapsedTime(&ms, h->ev[1], h->ev_list));
 		h->gather_ms = ms;
 	}
-	h->computed = true;
+	// a cell no tier finished, or ids beyond the arena: no results to read
+	h->computed = hc[3] == 0;
 	h->last_flags = flags;
 	if (hc[3] > 0) return VGPU_MEMORY_ERROR;
 	return VGPU_OK;
@@ -780,7 +781,7 @@ int batch_enqueue(vgpu_batch *b, unsigned flags, hipStream_t s, bool timing) {
 		// cells: bstart[nblocks] (particles outside their box are not cells)
 		const int *ncells = b->bstart.as<int>() + nb;
 		hipLaunchKernelGGL(k_face_counts_dev, dim3(nblk(n + 1, 256)), dim3(256), 0, s, n, ncells, b->status.as<int>(),
-				   b->nfaces.as<int>(), b->csr_off.as<int>());
+				   b->nfaces.as<int>(), b->nbr_start.as<int>(), b->csr_off.as<int>());
 		if ((st = scan_excl(b->part, b->csr_off.as<int>(), b->tmp.as<int>(), n + 1, s, true))) return st;
 		hipLaunchKernelGGL(k_gather_nbrs_dev, dim3(nblk(n, 256)), dim3(256), 0, s, n, ncells, b->status.as<int>(),
 				   b->nfaces.as<int>(), b->nbr_start.as<int>(), b->arena.as<int>(), b->tmp.as<int>(),
@@ -838,7 +839,7 @@ int batch_run(vgpu_batch *b, unsigned flags, hipStream_t s) {
 	}
 	b->nbr_total = hc[0];
 	b->fails = hc[3];
-	b->computed = true;
+	b->computed = hc[3] == 0;   // no results to read after a failure
 	b->last_flags = flags;
 	return hc[3] > 0 ? VGPU_MEMORY_ERROR : VGPU_OK;
 }

@@ -161,18 +161,20 @@ __global__ void k_rank(int n, const int *__restrict__ blk, const int *__restrict
 
 // Batches: the same two steps with the cell count read on the device
 // (*ncells <= n; entries from *ncells on count zero faces).
+// A cell whose ids did not fit the arena (start -1; the compute reports a
+// memory error) contributes no entries, so the CSR never outgrows the arena.
 __global__ void k_face_counts_dev(int n, const int *__restrict__ ncells, const int *__restrict__ status,
-				  const int *__restrict__ nfaces, int *__restrict__ cnt) {
+				  const int *__restrict__ nfaces, const int *__restrict__ start, int *__restrict__ cnt) {
 	int c = blockIdx.x * blockDim.x + threadIdx.x;
 	if (c > n) return;
-	cnt[c] = (c < *ncells && (status[c] & ST_EXISTS)) ? nfaces[c] : 0;
+	cnt[c] = (c < *ncells && (status[c] & ST_EXISTS) && start[c] >= 0) ? nfaces[c] : 0;
 }
 __global__ void k_gather_nbrs_dev(int n, const int *__restrict__ ncells, const int *__restrict__ status,
 				  const int *__restrict__ nfaces, const int *__restrict__ start,
 				  const int *__restrict__ arena, const int *__restrict__ off, int *__restrict__ csr) {
 	int c = blockIdx.x * blockDim.x + threadIdx.x;
 	if (c >= n || c >= *ncells) return;
-	if (!(status[c] & ST_EXISTS)) return;
+	if (!(status[c] & ST_EXISTS) || start[c] < 0) return;
 	int s = start[c], o = off[c], k = nfaces[c];
 	for (int t = 0; t < k; t++) csr[o + t] = arena[s + t];
 }
@@ -183,15 +185,15 @@ __global__ void k_gather_nbrs(int ncells, const int *__restrict__ status, const 
 			      int *__restrict__ csr) {
 	int c = blockIdx.x * blockDim.x + threadIdx.x;
 	if (c >= ncells) return;
-	if (!(status[c] & ST_EXISTS)) return;
+	if (!(status[c] & ST_EXISTS) || start[c] < 0) return;
 	int s = start[c], o = off[c], k = nfaces[c];
 	for (int t = 0; t < k; t++) csr[o + t] = arena[s + t];
 }
 __global__ void k_face_counts(int ncells, const int *__restrict__ status, const int *__restrict__ nfaces,
-			      int *__restrict__ cnt) {
+			      const int *__restrict__ start, int *__restrict__ cnt) {
 	int c = blockIdx.x * blockDim.x + threadIdx.x;
 	if (c > ncells) return;
-	cnt[c] = (c < ncells && (status[c] & ST_EXISTS)) ? nfaces[c] : 0;
+	cnt[c] = (c < ncells && (status[c] & ST_EXISTS) && start[c] >= 0) ? nfaces[c] : 0;
 }
 
 }  // namespace vgpu
