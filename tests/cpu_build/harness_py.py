@@ -21,13 +21,15 @@ def run(O, spec, ids, xyz, r=None, list_depth=0, wave=False):
     list_depth > 0 computes every cell through a candidate list of that
     worklist depth first (the learned path's kernel), the rest on the plain
     tiers; res["list_fallbacks"] counts the cells whose list was too short.
-    wave: every cell is first tried by the wavefront-cell tier;
-    res["wave_done"] counts the cells it finished."""
+    wave: every cell is first tried by the wavefront-cell tiers;
+    res["wave_done"] / res["wave2_done"] count the cells the 64- and the
+    128-vertex one finished."""
     lib = C.CDLL(SO)
     lib.hx_use_list(int(list_depth))
     lib.hx_use_wave(1 if wave else 0)
     lib.hx_list_fallbacks.restype = C.c_longlong
     lib.hx_wave_done.restype = C.c_longlong
+    lib.hx_wave2_done.restype = C.c_longlong
     f = lib.hx_compute_neighbor
     f.restype = C.c_longlong
     n = len(ids)
@@ -62,5 +64,6 @@ def run(O, spec, ids, xyz, r=None, list_depth=0, wave=False):
     out["stats"] = st[:4 * m].reshape(-1, 4)
     out["list_fallbacks"] = int(lib.hx_list_fallbacks())
     out["wave_done"] = int(lib.hx_wave_done())
+    out["wave2_done"] = int(lib.hx_wave2_done())
     out["tiers"] = tc
     return out

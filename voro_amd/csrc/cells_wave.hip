@@ -10,28 +10,28 @@ namespace vgpu {
 
 namespace {
 
-// Neighbour ids in face-walk order: id i in lane i.
+// Neighbour ids in face-walk order: id i in lane i & 63 of set i >> 6.
+template<int K>
 struct WSink {
-	WCell *c;
-	VG_INL void put(int i, int v) { if (i < (int)WLayout::NBCAP) c->put_nb(i, v); }
+	WCellT<K> *c;
+	VG_INL void put(int i, int v) { if (i < (int)WLayoutT<K>::NBCAP) c->put_nb(i, v); }
 };
 
-// Cells taken per counter increment.
-enum { WCHUNK = 4 };
-
 // Persistent kernel, one wavefront per workgroup, over every cell or over the
-// device list of cells whose count is at *list_count.  The cells are split into
+// device list of cells whose count is at *list_count, `chunk` cells taken per
+// counter increment.  The cells are split into
 // one contiguous share per XCD (workgroups go to the 8 XCDs round robin) so
 // that the cells in flight on an XCD read neighbouring particle blocks
 // through its L2; a wavefront whose share is exhausted moves on to the next.
-template<bool POLY, int OCC, bool PRD>
+template<int K, bool POLY, int OCC, bool PRD>
 __global__ void __launch_bounds__(64, OCC)
 k_wcells(RectView v, int ncells, const int *__restrict__ list, const int *__restrict__ list_count,
 	 const int *__restrict__ cellblk, int *__restrict__ counters, char *__restrict__ ws, unsigned stamp_base,
-	 CellOut out) {
+	 int chunk, CellOut out) {
 	const int lane = threadIdx.x;
-	WCell c;
-	c.setup(ws + (size_t)blockIdx.x * WLayout::bytes, lane, v.max_len_sq);
+	typedef WLayoutT<K> L;
+	WCellT<K> c;
+	c.setup(ws + (size_t)blockIdx.x * L::bytes, lane, v.max_len_sq);
 	const int total = list_count ? *list_count : ncells;
 	const int nshare = 8;
 	const int my = blockIdx.x % nshare;
@@ -41,25 +41,25 @@ k_wcells(RectView v, int ncells, const int *__restrict__ list, const int *__rest
 		const int lo = sh * share, hi = min(total, lo + share);
 		for (;;) {
 			int base = 0;
-			if (lane == 0) base = atomicAdd(&counters[sh], (int)WCHUNK);
+			if (lane == 0) base = atomicAdd(&counters[sh], chunk);
 			base = __builtin_amdgcn_readfirstlane(base);
 			if (lo + base >= hi) break;
-			const int end = min(hi, lo + base + (int)WCHUNK);
+			const int end = min(hi, lo + base + chunk);
 			for (int idx = lo + base; idx < end; idx++) {
 				const int cell = list ? list[idx] : idx;
 				const int b = cellblk[cell];
 				const int s = cell - v.cstart[b];
 				const int ci = b % v.nx, cj = (b / v.nx) % v.gy, ck = b / v.nxy;
 				CellStats stt;
-				const bool ok = compute_cell_rect<WCell, POLY, PRD>(c, v, b, s, ci, cj, ck,
+				const bool ok = compute_cell_rect<WCellT<K>, POLY, PRD>(c, v, b, s, ci, cj, ck,
 										  stamp_base + (unsigned)cell + 1u, stt,
 										  (unsigned long long)cell << 24);
 				double vol = 0, cx = 0, cy = 0, cz = 0;
 				int nf = 0;
 				if (!c.ovf && ok) {
-					WSink sk{&c};
+					WSink<K> sk{&c};
 					c.outputs(vol, cx, cy, cz, nf, sk);
-					if (nf > (int)WLayout::NBCAP) c.ovf = true;
+					if (nf > (int)L::NBCAP) c.ovf = true;
 				}
 				if (c.ovf) {
 					if (lane == 0) {
@@ -89,7 +89,8 @@ k_wcells(RectView v, int ncells, const int *__restrict__ list, const int *__rest
 						if (lane == 0) { out.nbr_start[cell] = -1; atomicAdd(out.fail_count, 1); }
 					} else {
 						if (lane == 0) out.nbr_start[cell] = off;
-						if (lane < nf) out.nbr[off + lane] = c.nb_lane();
+						if (lane < nf) out.nbr[off + lane] = c.nb_lane(0);
+						if (K == 2 && lane + 64 < nf) out.nbr[off + 64 + lane] = c.nb_lane(1);
 					}
 				}
 			}
@@ -99,28 +100,37 @@ k_wcells(RectView v, int ncells, const int *__restrict__ list, const int *__rest
 
 }  // namespace
 
-#define VGPU_WAVE_LAUNCH(P, O, R)                                                                               \
-	hipLaunchKernelGGL((k_wcells<P, O, R>), dim3(waves), dim3(64), 0, s, v, ncells, list, list_count, cellblk,   \
-			   counters, ws, stamp_base, out)
+#define VGPU_WAVE_LAUNCH(K, P, O, R)                                                                            \
+	hipLaunchKernelGGL((k_wcells<K, P, O, R>), dim3(waves), dim3(64), 0, s, v, ncells, list, list_count, cellblk, \
+			   counters, ws, stamp_base, chunk, out)
 
-int launch_cells_wave(bool poly, bool prd, int occ, int waves, hipStream_t s, const RectView &v, int ncells,
+int launch_cells_wave(int k, bool poly, bool prd, int occ, int waves, hipStream_t s, const RectView &v, int ncells,
 		      const int *list, const int *list_count, const int *cellblk, int *counters, char *ws,
-		      unsigned stamp_base, const CellOut &out) {
-	if (prd) {
-		if (poly) VGPU_WAVE_LAUNCH(true, 4, true);
-		else VGPU_WAVE_LAUNCH(false, 4, true);
+		      unsigned stamp_base, int chunk, const CellOut &out) {
+	if (k == 2) {
+		// the overflow tier: few cells, registers for two vertex sets
+		if (prd) {
+			if (poly) VGPU_WAVE_LAUNCH(2, true, 2, true);
+			else VGPU_WAVE_LAUNCH(2, false, 2, true);
+		} else {
+			if (poly) VGPU_WAVE_LAUNCH(2, true, 2, false);
+			else VGPU_WAVE_LAUNCH(2, false, 2, false);
+		}
+	} else if (prd) {
+		if (poly) VGPU_WAVE_LAUNCH(1, true, 4, true);
+		else VGPU_WAVE_LAUNCH(1, false, 4, true);
 	} else if (occ >= 8) {
-		if (poly) VGPU_WAVE_LAUNCH(true, 8, false);
-		else VGPU_WAVE_LAUNCH(false, 8, false);
+		if (poly) VGPU_WAVE_LAUNCH(1, true, 8, false);
+		else VGPU_WAVE_LAUNCH(1, false, 8, false);
 	} else if (occ <= 2) {
-		if (poly) VGPU_WAVE_LAUNCH(true, 2, false);
-		else VGPU_WAVE_LAUNCH(false, 2, false);
+		if (poly) VGPU_WAVE_LAUNCH(1, true, 2, false);
+		else VGPU_WAVE_LAUNCH(1, false, 2, false);
 	} else if (occ == 3) {
-		if (poly) VGPU_WAVE_LAUNCH(true, 3, false);
-		else VGPU_WAVE_LAUNCH(false, 3, false);
+		if (poly) VGPU_WAVE_LAUNCH(1, true, 3, false);
+		else VGPU_WAVE_LAUNCH(1, false, 3, false);
 	} else {
-		if (poly) VGPU_WAVE_LAUNCH(true, 4, false);
-		else VGPU_WAVE_LAUNCH(false, 4, false);
+		if (poly) VGPU_WAVE_LAUNCH(1, true, 4, false);
+		else VGPU_WAVE_LAUNCH(1, false, 4, false);
 	}
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }

@@ -107,7 +107,7 @@ struct vgpu_container {
 	DBuf blk, slot, rem, cnt, bstart, tmp, part, P, pid, src, cellblk;
 	DBuf d_mrad, d_wl, d_walls, d_maxr, d_counters;
 	DBuf vol, cent, nfaces, status, nbr_start, arena, csr_off, csr, ctr;
-	DBuf retry0, retry1, retry2, retry3;
+	DBuf retry0, retry1, retry2, retry3, retry4;
 	DBuf st_planes, st_exit, st_tier;
 	// learned path
 	bool have_pred = false;
@@ -244,19 +244,23 @@ int default_fast_waves(int occ) {
 	return cus * 4 * occ;
 }
 
-// The wavefront-cell tier over every cell (list == nullptr) or over a device
-// list of cells whose count is at *list_count; leftovers go to out.retry.
-int launch_wave(vgpu_container *h, const RectView &v, int nc, const int *list, const int *list_count, CellOut out,
-		hipStream_t s) {
-	const int occ = h->wave_occ;
-	int waves = h->wave_waves > 0 ? h->wave_waves : default_fast_waves(occ);
+// A wavefront-cell tier (k vertices per lane) over every cell (list ==
+// nullptr) or over a device list of cells whose count is at *list_count;
+// leftovers go to out.retry.
+int launch_wave(vgpu_container *h, int k, const RectView &v, int nc, const int *list, const int *list_count,
+		CellOut out, hipStream_t s) {
+	const int occ = k == 2 ? 2 : h->wave_occ;
+	int waves = (h->wave_waves > 0 && k == 1) ? h->wave_waves : default_fast_waves(occ);
 	int cnt = nc;
 	if (list_count) {
 		HIPCHK(hipMemcpyAsync(&cnt, list_count, 4, hipMemcpyDeviceToHost, s));
 		HIPCHK(hipStreamSynchronize(s));
 		if (cnt == 0) return VGPU_OK;
 	}
-	waves = std::max(1, std::min(waves, (cnt + 3) / 4));
+	// four cells per counter increment, one for a list too short to fill
+	// every wavefront that way
+	const int chunk = cnt >= 4 * waves ? 4 : 1;
+	waves = std::max(1, std::min(waves, (cnt + chunk - 1) / chunk));
 	Workspace &ws = h->ws_wave;
 	int st = ws.ensure(WLayout::bytes, waves);
 	if (st) return st;
@@ -266,8 +270,8 @@ int launch_wave(vgpu_container *h, const RectView &v, int nc, const int *list, c
 	}
 	int *ctr = h->d_counters.as<int>();
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 8, 0);
-	if (launch_cells_wave(h->poly, h->prd, occ, waves, s, v, nc, list, list_count, h->cellblk.as<int>(), ctr,
-			      ws.base(), ws.stamp_base, out))
+	if (launch_cells_wave(k, h->poly, h->prd, occ, waves, s, v, nc, list, list_count, h->cellblk.as<int>(), ctr,
+			      ws.base(), ws.stamp_base, chunk, out))
 		return VGPU_HIP_ERROR;
 	ws.stamp_base += (unsigned)nc + 1u;
 	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
@@ -343,13 +347,17 @@ int run_tiers(vgpu_container *h, const RectView &v, int nc, int fw, const int *l
 	int *ctr = h->ctr.as<int>();
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr + 1, 2, 0);
 	if (h->first_tier == 0) {
-		// one cell per wavefront; its leftovers (count at ctr[6]) to order-3
-		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr + 6, 1, 0);
+		// one cell per wavefront (up to 64 vertices); its leftovers (count at
+		// ctr[6]) to the same with up to 128 vertices, whose leftovers (count
+		// at ctr[7]) go to order-3 per lane
+		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr + 6, 2, 0);
 		out.tier = 0; out.retry = h->retry3.as<int>(); out.retry_count = ctr + 6;
-		if ((st = launch_wave(h, v, nc, list, list_count, out, s))) return st;
+		if ((st = launch_wave(h, 1, v, nc, list, list_count, out, s))) return st;
 		if (mark_ev2) HIPCHK(hipEventRecord(h->ev[2], s));
+		out.tier = 1; out.retry = h->retry4.as<int>(); out.retry_count = ctr + 7;
+		if ((st = launch_wave(h, 2, v, nc, h->retry3.as<int>(), ctr + 6, out, s))) return st;
 		out.tier = 1; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
-		if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, h->retry3.as<int>(), ctr + 6, out, s))) return st;
+		if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, h->retry4.as<int>(), ctr + 7, out, s))) return st;
 	} else {
 		out.tier = 0; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
 		if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, list, list_count, out, s))) return st;
@@ -522,7 +530,8 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 	if ((st = h->vol.ensure(C1 * 8)) || (st = h->cent.ensure(C1 * 24)) || (st = h->nfaces.ensure(C1 * 4)) ||
 	    (st = h->status.ensure(C1 * 4)) || (st = h->nbr_start.ensure(C1 * 4)) || (st = h->arena.ensure((size_t)acap * 4)) ||
 	    (st = h->ctr.ensure(16 * 4)) || (st = h->retry0.ensure(C1 * 4)) || (st = h->retry1.ensure(C1 * 4)) ||
-	    (st = h->retry2.ensure(C1 * 4)) || (st = h->retry3.ensure(C1 * 4)))
+	    (st = h->retry2.ensure(C1 * 4)) || (st = h->retry3.ensure(C1 * 4)) ||
+	    (st = h->retry4.ensure(C1 * 4)))
 		return st;
 	const bool stats = (flags & VGPU_F_STATS) != 0;
 	if (stats) {

@@ -136,12 +136,13 @@ int launch_cells_huge(bool poly, int occ, int waves, hipStream_t s, const RectVi
 		      const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
 		      unsigned stamp_base, int per_wave, const CellOut &out);
 
-// The wavefront-cell tier (cells_wave.hip): one cell per wavefront, over
-// every cell or a device list (count at *list_count), at 2, 3, 4 or 8 waves
-// per SIMD; cells it cannot finish go to out.retry.
-int launch_cells_wave(bool poly, bool prd, int occ, int waves, hipStream_t s, const RectView &v, int ncells,
+// The wavefront-cell tiers (cells_wave.hip): one cell per wavefront with k
+// vertices per lane (up to 64 k vertices), over every cell or a device list
+// (count at *list_count), at 2, 3, 4 or 8 waves per SIMD (k = 1; k = 2 runs
+// at 2); cells it cannot finish go to out.retry.
+int launch_cells_wave(int k, bool poly, bool prd, int occ, int waves, hipStream_t s, const RectView &v, int ncells,
 		      const int *list, const int *list_count, const int *cellblk, int *counters, char *ws,
-		      unsigned stamp_base, const CellOut &out);
+		      unsigned stamp_base, int chunk, const CellOut &out);
 
 // Periodic containers (cells_prd.hip): tier 0 order-3, 1 wide, 2 huge.
 int launch_cells_prd(int tier, bool poly, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,

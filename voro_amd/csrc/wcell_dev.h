@@ -39,8 +39,9 @@
 
 namespace vgpu {
 
-struct WLayout {
-	enum { MAXV = 64, MAXS = 256, W = 6, MAXQ = 256, MAXF = 64 };
+template<int K>
+struct WLayoutT {
+	enum { MAXV = 64 * K, MAXS = 256, W = 6, MAXQ = 256, MAXF = 64 * K };
 	enum { WD = 2 * W + 1, WBITS = WD * WD * WD, WWORDS = (WBITS + 31) / 32 };
 	// per-wavefront global slab: block-search window and ring only
 	static constexpr unsigned o_bm = 0;                          // WWORDS
@@ -49,6 +50,7 @@ struct WLayout {
 	static constexpr size_t bytes = (o_qu + MAXQ * 4 + 255) & ~255u;
 	enum { NBCAP = MAXF };
 };
+typedef WLayoutT<1> WLayout;
 
 #if defined(__HIP_DEVICE_COMPILE__)
 #define VG_WDEV 1
@@ -56,26 +58,57 @@ struct WLayout {
 #define VG_WDEV 0
 #endif
 
-struct WCell {
-	typedef WLayout L;
+// Vertex marks, bit v = vertex v (K words of 64 bits).
+template<int K>
+struct WMask {
+	unsigned long long lo, hi;
+	VG_INL void clear() { lo = 0; hi = 0; }
+	VG_INL bool get(int v) const {
+		if (K == 1) return (lo >> v) & 1ull;
+		return v < 64 ? ((lo >> v) & 1ull) : ((hi >> (v - 64)) & 1ull);
+	}
+	VG_INL void set(int v) {
+		if (K == 1 || v < 64) lo |= 1ull << (v & 63); else hi |= 1ull << (v - 64);
+	}
+	VG_INL void unset(int v) {
+		if (K == 1 || v < 64) lo &= ~(1ull << (v & 63)); else hi &= ~(1ull << (v - 64));
+	}
+	VG_INL void toggle(int v) {
+		if (K == 1 || v < 64) lo ^= 1ull << (v & 63); else hi ^= 1ull << (v - 64);
+	}
+	// set(v) when c, as value arithmetic (no branch the compiler could turn
+	// into a choice between mask objects)
+	VG_INL void set_if(bool c, int v) {
+		const unsigned long long b = c ? 1ull << (v & 63) : 0ull;
+		if (K == 1) lo |= b;
+		else { lo |= v < 64 ? b : 0ull; hi |= v < 64 ? 0ull : b; }
+	}
+};
+
+// K vertices per lane: vertex v in lane v & 63 of register set v >> 6.
+template<int K>
+struct WCellT {
+	static_assert(K == 1 || K == 2, "one or two vertices per lane");
+	typedef WLayoutT<K> L;
 	enum { MAXV = L::MAXV, MAXS = L::MAXS };
 
 #if VG_WDEV
-	double vx, vy, vz;       // vertex `lane`
-	unsigned vt;             // e0 | e1 << 8 | e2 << 16 | (r0 | r1 << 2 | r2 << 4) << 24
-	int vn0, vn1, vn2;       // neighbour ids
-	double vtv;              // value for the current plane
+	// register set 0 (vertex `lane`) and 1 (vertex 64 + `lane`)
+	double vx0, vy0, vz0, vx1, vy1, vz1;
+	unsigned vt0, vt1;       // e0 | e1 << 8 | e2 << 16 | (r0 | r1 << 2 | r2 << 4) << 24
+	int vna0, vnb0, vnc0, vna1, vnb1, vnc1;   // neighbour ids
+	double vtv0, vtv1;       // value for the current plane
 	unsigned vds, vdm;       // stack entries 4 * lane .. 4 * lane + 3
-	int nbv;                 // output staging: neighbour id number `lane`
+	int nbv0, nbv1;          // output staging: neighbour ids number `lane`, 64 + `lane`
 	int lane;
 	__amdgpu_buffer_rsrc_t rs;
 #else
-	double vx[64], vy[64], vz[64];
-	unsigned vt[64];
-	int vn[64][3];
-	double vtv[64];
+	double vx[MAXV], vy[MAXV], vz[MAXV];
+	unsigned vt[MAXV];
+	int vn[MAXV][3];
+	double vtv[MAXV];
 	unsigned char vds[MAXS], vdm[MAXS];
-	int nbv[64];
+	int nbv[MAXV];
 	char *wsb;
 #endif
 
@@ -85,13 +118,14 @@ struct WCell {
 	int nds;
 	bool ovf, fatal;
 	int n_cuts;
-	unsigned long long delm;     // deletion marks
-	unsigned long long flipm;    // tie-search marks (the reference negates back pointers)
-	unsigned long long vis0, vis1, vis2;   // face-walk marks per edge slot
+	WMask<K> delm;               // deletion marks
+	WMask<K> flipm;              // tie-search marks (the reference negates back pointers)
+	WMask<K> vis0, vis1, vis2;   // face-walk marks per edge slot
 
 	// Value (never lvalue) selection: a conditional over members would make
 	// the compiler address the cell in memory.
 	template<class T> static VG_INL T sel3(int i, T a, T b, T c) { return i == 0 ? a : (i == 1 ? b : c); }
+	template<class T> static VG_INL T selh(int v, T a, T b) { return (K == 1 || v < 64) ? a : b; }
 
 	// ---- lane primitives ----
 #if VG_WDEV
@@ -108,20 +142,28 @@ struct WCell {
 		const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
 		return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 	}
-	VG_INL double X(int v) const { return rld(vx, v); }
-	VG_INL double Y(int v) const { return rld(vy, v); }
-	VG_INL double Z(int v) const { return rld(vz, v); }
-	VG_INL double T(int v) const { return rld(vtv, v); }
-	VG_INL unsigned TW(int v) const { return rlu(vt, v); }
+	VG_INL double X(int v) const { return rld(selh(v, vx0, vx1), v & 63); }
+	VG_INL double Y(int v) const { return rld(selh(v, vy0, vy1), v & 63); }
+	VG_INL double Z(int v) const { return rld(selh(v, vz0, vz1), v & 63); }
+	VG_INL double T(int v) const { return rld(selh(v, vtv0, vtv1), v & 63); }
+	VG_INL unsigned TW(int v) const { return rlu(selh(v, vt0, vt1), v & 63); }
 	VG_INL void setXYZ(int v, double a, double b, double c) {
-		const bool m = lane == v;
-		vx = m ? a : vx; vy = m ? b : vy; vz = m ? c : vz;
+		const bool m = lane == (v & 63);
+		if (K == 1 || v < 64) { vx0 = m ? a : vx0; vy0 = m ? b : vy0; vz0 = m ? c : vz0; }
+		else { vx1 = m ? a : vx1; vy1 = m ? b : vy1; vz1 = m ? c : vz1; }
 	}
-	VG_INL void setTW(int v, unsigned w) { vt = lane == v ? w : vt; }
-	VG_INL int N(int v, int i) const { return sel3(i, rli(vn0, v), rli(vn1, v), rli(vn2, v)); }
+	VG_INL void setTW(int v, unsigned w) {
+		const bool m = lane == (v & 63);
+		if (K == 1 || v < 64) vt0 = m ? w : vt0; else vt1 = m ? w : vt1;
+	}
+	VG_INL int N(int v, int i) const {
+		const int l = v & 63;
+		return sel3(i, rli(selh(v, vna0, vna1), l), rli(selh(v, vnb0, vnb1), l), rli(selh(v, vnc0, vnc1), l));
+	}
 	VG_INL void setN3(int v, int a, int b, int c) {
-		const bool m = lane == v;
-		vn0 = m ? a : vn0; vn1 = m ? b : vn1; vn2 = m ? c : vn2;
+		const bool m = lane == (v & 63);
+		if (K == 1 || v < 64) { vna0 = m ? a : vna0; vnb0 = m ? b : vnb0; vnc0 = m ? c : vnc0; }
+		else { vna1 = m ? a : vna1; vnb1 = m ? b : vnb1; vnc1 = m ? c : vnc1; }
 	}
 	VG_INL int DS(int i) const { return (int)((rlu(vds, i >> 2) >> (8 * (i & 3))) & 0xffu); }
 	VG_INL void setDS(int i, int v) {
@@ -135,12 +177,17 @@ struct WCell {
 		const unsigned w = (rlu(vdm, i >> 2) & ~(0xffu << sh)) | ((unsigned)v & 0xffu) << sh;
 		vdm = lane == (i >> 2) ? w : vdm;
 	}
-	VG_INL void put_nb(int i, int v) { nbv = lane == i ? v : nbv; }
-	VG_INL int nb_lane() const { return nbv; }
-	// vertex-parallel: bit v set for live vertices v whose predicate holds
-	VG_INL unsigned long long live_ballot(bool b) const {
+	VG_INL void put_nb(int i, int v) {
+		const bool m = lane == (i & 63);
+		if (K == 1 || i < 64) nbv0 = m ? v : nbv0; else nbv1 = m ? v : nbv1;
+	}
+	// neighbour id number h * 64 + `lane`
+	VG_INL int nb_lane(int h = 0) const { const int a = nbv0, b = nbv1; return h == 0 ? a : b; }
+	// vertex-parallel: bit l set for live vertices h * 64 + l whose predicate holds
+	VG_INL unsigned long long live_ballot(bool b, int h = 0) const {
 		const unsigned long long m = __ballot(b);
-		return p >= 64 ? m : (m & ((1ull << p) - 1));
+		const int q = p - 64 * h;
+		return q >= 64 ? m : (q <= 0 ? 0ull : (m & ((1ull << q) - 1)));
 	}
 #else
 	VG_INL double X(int v) const { return vx[v]; }
@@ -157,7 +204,7 @@ struct WCell {
 	VG_INL int DM(int i) const { return vdm[i]; }
 	VG_INL void setDM(int i, int v) { vdm[i] = (unsigned char)v; }
 	VG_INL void put_nb(int i, int v) { nbv[i] = v; }
-	VG_INL int nb_lane() const { return nbv[0]; }
+	VG_INL int nb_lane(int h = 0) const { return nbv[64 * h]; }
 #endif
 
 	// ---- per-wavefront slab (block-search arrays) ----
@@ -181,7 +228,7 @@ struct WCell {
 	}
 	template<class T>
 	struct Ref {
-		const WCell *c;
+		const WCellT *c;
 		unsigned off;
 		VG_INL operator T() const { return c->template ld<T>(off); }
 		VG_INL const Ref &operator=(T v) const { c->template st<T>(off, v); return *this; }
@@ -204,10 +251,10 @@ struct WCell {
 	VG_INL int R(int v, int i) const { return twR(TW(v), i); }
 	VG_INL void setE(int v, int i, int e) { setTW(v, twSetE(TW(v), i, e)); }
 	VG_INL void setER(int v, int i, int e, int r) { setTW(v, twSetR(twSetE(TW(v), i, e), i, r)); }
-	VG_INL bool DEL(int v) const { return (delm >> v) & 1ull; }
-	VG_INL void setDEL(int v) { delm |= 1ull << v; }
-	VG_INL bool FL(int v) const { return (flipm >> v) & 1ull; }
-	VG_INL void flip(int v) { flipm ^= 1ull << v; }
+	VG_INL bool DEL(int v) const { return delm.get(v); }
+	VG_INL void setDEL(int v) { delm.set(v); }
+	VG_INL bool FL(int v) const { return flipm.get(v); }
+	VG_INL void flip(int v) { flipm.toggle(v); }
 
 	static VG_INL int cyc_up(int a) { return a == 2 ? 0 : a + 1; }
 
@@ -236,8 +283,8 @@ struct WCell {
 	VG_INL void new_vertex(int v, double x, double y, double z) {
 		setXYZ(v, x, y, z);
 		setTW(v, 0x00ffffffu);
-		delm &= ~(1ull << v);
-		flipm &= ~(1ull << v);
+		delm.unset(v);
+		flipm.unset(v);
 	}
 
 	// Vertex `from` takes index `to` (src/cell.cc:849-861).
@@ -246,9 +293,8 @@ struct WCell {
 		const unsigned w = TW(from);
 		setTW(to, w);
 		setN3(to, N(from, 0), N(from, 1), N(from, 2));
-		const unsigned long long bt = 1ull << to;
-		delm = DEL(from) ? (delm | bt) : (delm & ~bt);
-		flipm &= ~bt;
+		if (DEL(from)) delm.set(to); else delm.unset(to);
+		flipm.unset(to);
 		for (int i = 0; i < 3; i++) setE(twE(w, i), twR(w, i), to);
 	}
 
@@ -258,7 +304,7 @@ struct WCell {
 		ovf = false; fatal = false; n_cuts = 0;
 		up = 0;
 		p = 8;
-		delm = 0; flipm = 0;
+		delm.clear(); flipm.clear();
 		xmin *= 2; xmax *= 2; ymin *= 2; ymax *= 2; zmin *= 2; zmax *= 2;
 #if VG_WDEV
 		if (lane < 8) box_corner(lane, xmin, xmax, ymin, ymax, zmin, zmax);
@@ -274,9 +320,9 @@ struct WCell {
 		const unsigned e = (unsigned)(v ^ (1 + par)) | (unsigned)(v ^ 4) << 8 | (unsigned)(v ^ (2 - par)) << 16;
 		const int xf = -1 - (v & 1), yf = -3 - ((v >> 1) & 1);
 #if VG_WDEV
-		vx = (v & 1) ? xmax : xmin; vy = (v & 2) ? ymax : ymin; vz = (v & 4) ? zmax : zmin;
-		vt = e | (6u << 24);
-		vn0 = -5 - (v >> 2); vn1 = par ? xf : yf; vn2 = par ? yf : xf;
+		vx0 = (v & 1) ? xmax : xmin; vy0 = (v & 2) ? ymax : ymin; vz0 = (v & 4) ? zmax : zmin;
+		vt0 = e | (6u << 24);
+		vna0 = -5 - (v >> 2); vnb0 = par ? xf : yf; vnc0 = par ? yf : xf;
 #else
 		setXYZ(v, (v & 1) ? xmax : xmin, (v & 2) ? ymax : ymin, (v & 4) ? zmax : zmin);
 		setTW(v, e | (6u << 24));
@@ -290,7 +336,7 @@ struct WCell {
 	VG_FN void init_unit(const UnitVert *__restrict__ u, int np) {
 		ovf = false; fatal = false; n_cuts = 0;
 		up = 0;
-		delm = 0; flipm = 0;
+		delm.clear(); flipm.clear();
 		if (np > MAXV) { ovf = true; return; }
 		p = np;
 #if VG_WDEV
@@ -298,10 +344,16 @@ struct WCell {
 		if (lane < np) {
 			const UnitVert a = u[lane];
 			bad = a.nu != 3;
-			vx = a.x; vy = a.y; vz = a.z;
-			vt = (unsigned)(a.ed[0] & 0xff) | (unsigned)(a.ed[1] & 0xff) << 8 | (unsigned)(a.ed[2] & 0xff) << 16 |
-			     (unsigned)((a.rel[0] & 3) | (a.rel[1] & 3) << 2 | (a.rel[2] & 3) << 4) << 24;
-			vn0 = vn1 = vn2 = 0;
+			vx0 = a.x; vy0 = a.y; vz0 = a.z;
+			vt0 = unit_word(a);
+			vna0 = vnb0 = vnc0 = 0;
+		}
+		if (K == 2 && lane + 64 < np) {
+			const UnitVert a = u[lane + 64];
+			bad = bad || a.nu != 3;
+			vx1 = a.x; vy1 = a.y; vz1 = a.z;
+			vt1 = unit_word(a);
+			vna1 = vnb1 = vnc1 = 0;
 		}
 		if (__ballot(bad)) ovf = true;
 #else
@@ -309,25 +361,36 @@ struct WCell {
 			const UnitVert a = u[v];
 			if (a.nu != 3) { ovf = true; return; }
 			setXYZ(v, a.x, a.y, a.z);
-			setTW(v, (unsigned)(a.ed[0] & 0xff) | (unsigned)(a.ed[1] & 0xff) << 8 | (unsigned)(a.ed[2] & 0xff) << 16 |
-				 (unsigned)((a.rel[0] & 3) | (a.rel[1] & 3) << 2 | (a.rel[2] & 3) << 4) << 24);
+			setTW(v, unit_word(a));
 			setN3(v, 0, 0, 0);
 		}
 #endif
+	}
+	static VG_INL unsigned unit_word(const UnitVert &a) {
+		return (unsigned)(a.ed[0] & 0xff) | (unsigned)(a.ed[1] & 0xff) << 8 | (unsigned)(a.ed[2] & 0xff) << 16 |
+		       (unsigned)((a.rel[0] & 3) | (a.rel[1] & 3) << 2 | (a.rel[2] & 3) << 4) << 24;
 	}
 
 	// Plane values of every vertex, the expression of src/cell.cc:2043-2059.
 	// Returns true if some live vertex is not below -tol.
 	VG_FN bool plane_values() {
 #if VG_WDEV
-		double a = vx * px;
-		a += vy * py;
-		a += vz * pz - prsq;
-		vtv = a;
-		return live_ballot(!(a < -tol)) != 0;
+		double a = vx0 * px;
+		a += vy0 * py;
+		a += vz0 * pz - prsq;
+		vtv0 = a;
+		bool any = live_ballot(!(a < -tol)) != 0;
+		if (K == 2 && p > 64) {
+			double b = vx1 * px;
+			b += vy1 * py;
+			b += vz1 * pz - prsq;
+			vtv1 = b;
+			any = any || live_ballot(!(b < -tol), 1) != 0;
+		}
+		return any;
 #else
 		bool any = false;
-		for (int v = 0; v < 64; v++) {
+		for (int v = 0; v < MAXV; v++) {
 			double a = vx[v] * px;
 			a += vy[v] * py;
 			a += vz[v] * pz - prsq;
@@ -629,10 +692,16 @@ struct WCell {
 	// exact, so the order of the comparisons does not matter).
 	VG_FN double max_radius_squared() {
 #if VG_WDEV
-		double s = vx * vx;
-		s += vy * vy;
-		s += vz * vz;
+		double s = vx0 * vx0;
+		s += vy0 * vy0;
+		s += vz0 * vz0;
 		if (lane >= p) s = 0;
+		if (K == 2 && p > 64) {
+			double t = vx1 * vx1;
+			t += vy1 * vy1;
+			t += vz1 * vz1;
+			if (lane + 64 < p && t > s) s = t;
+		}
 		for (int o = 32; o > 0; o >>= 1) {
 			const double t = __shfl_xor(s, o);
 			s = t > s ? t : s;
@@ -657,13 +726,15 @@ struct WCell {
 	VG_FN bool plane_probe(double x, double y, double z, double rsq, bool guess) {
 		if (guess) up = 0;
 #if VG_WDEV
-		const double gl = x * vx + y * vy + z * vz;
-		double g = rld(gl, up);
+		const double gl = x * vx0 + y * vy0 + z * vz0;
+		double gh = 0;
+		if (K == 2 && p > 64) gh = x * vx1 + y * vy1 + z * vz1;
+		double g = rld(selh(up, gl, gh), up & 63);
 		if (!(g < rsq)) return true;
 		if (guess) {
 			int ca = 1, cc = p >> 3, mp = 1;
 			while (ca < cc) {
-				const double m = rld(gl, mp);
+				const double m = rld(selh(mp, gl, gh), mp & 63);
 				if (m > g) {
 					if (m > rsq) return true;
 					g = m; up = mp;
@@ -671,7 +742,8 @@ struct WCell {
 				ca += mp++;
 			}
 		}
-		return live_ballot(gl > rsq) != 0;
+		if (live_ballot(gl > rsq) != 0) return true;
+		return K == 2 && p > 64 && live_ballot(gh > rsq, 1) != 0;
 #else
 		double g = x * X(up) + y * Y(up) + z * Z(up);
 		if (!(g < rsq)) return true;
@@ -696,12 +768,11 @@ struct WCell {
 	// (src/cell.cc:1584-1616, 1861-1901, 2168-2187, 2660-2678).  Walked
 	// edges are marked in the visited masks (the reference negates them).
 	VG_INL bool vis(int v, int i) const {
-		const unsigned long long a = vis0, b = vis1, c = vis2;
-		return (sel3(i, a, b, c) >> v) & 1ull;
+		const bool a = vis0.get(v), b = vis1.get(v), c = vis2.get(v);
+		return sel3(i, a, b, c);
 	}
 	VG_INL void setvis(int v, int i) {
-		const unsigned long long b = 1ull << v;
-		if (i == 0) vis0 |= b; else if (i == 1) vis1 |= b; else vis2 |= b;
+		vis0.set_if(i == 0, v); vis1.set_if(i == 1, v); vis2.set_if(i == 2, v);
 	}
 	template<class NbSink>
 	VG_FN void outputs(double &vol, double &cx, double &cy, double &cz, int &nf, NbSink &nb) {
@@ -709,7 +780,7 @@ struct WCell {
 		double v = 0, tvol;
 		double ux, uy, uz, vx_, vy_, vz_, wx, wy, wz;
 		const double x0 = X(0), y0 = Y(0), z0 = Z(0);
-		vis0 = vis1 = vis2 = 0;
+		vis0.clear(); vis1.clear(); vis2.clear();
 		cx = cy = cz = 0;
 		nf = 0;
 		for (int i = 1; i < p; i++) {
@@ -758,5 +829,8 @@ struct WCell {
 		} else cx = cy = cz = 0;
 	}
 };
+
+typedef WCellT<1> WCell;     // up to 64 vertices
+typedef WCellT<2> WCell2;    // up to 128 vertices
 
 }  // namespace vgpu
