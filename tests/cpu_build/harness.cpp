@@ -91,14 +91,16 @@ void hx_use_list(int G) { list_depth = G; list_fallbacks = 0; }
 long long hx_list_fallbacks(void) { return list_fallbacks; }
 
 static int use_wave = 0;
-static long long wave_done = 0, wave2_done = 0;
+static long long wave_done = 0, wave2_done = 0, wave4_done = 0;
 // on: every cell is first tried by the wavefront-cell tiers (wcell_dev.h,
 // 64 lanes emulated, up to 64 then 128 vertices); cells they cannot finish
 // go on to the per-lane tiers.
-void hx_use_wave(int on) { use_wave = on; wave_done = 0; wave2_done = 0; }
+void hx_use_wave(int on) { use_wave = on; wave_done = 0; wave2_done = 0; wave4_done = 0; }
 long long hx_wave_done(void) { return wave_done; }
 // cells the 128-vertex wavefront cell finished after the 64-vertex one overflowed
 long long hx_wave2_done(void) { return wave2_done; }
+// ... and the 254-vertex one after both overflowed
+long long hx_wave4_done(void) { return wave4_done; }
 
 static long long prd_passes = 0;
 static int prd_replay = 1;
@@ -167,6 +169,7 @@ long long hx_compute_neighbor(const hx_spec *s, int n, const int *ids, const dou
 
 	Runner<WCell, WLayout::bytes> rw(v.max_len_sq);
 	Runner<WCell2, WLayoutT<2>::bytes> rw2(v.max_len_sq);
+	Runner<WCell4, WLayoutT<4>::bytes> rw4(v.max_len_sq);
 	Runner<VCell3, Layout3<>::lane_bytes> r0(v.max_len_sq);
 	Runner<VCell<TierWide, 1>, Layout<TierWide, 1>::bytes> r1(v.max_len_sq);
 	Runner<VCell<TierHuge, 1>, Layout<TierHuge, 1>::bytes> *r2 = 0;
@@ -197,7 +200,13 @@ long long hx_compute_neighbor(const hx_spec *s, int n, const int *ids, const dou
 					res = poly ? rw2.run<true>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st)
 						   : rw2.run<false>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st);
 					if (res >= 0 && nf > (int)WLayoutT<2>::NBCAP) res = -1;
-					if (res < 0) nb.clear(); else wave2_done++;
+					if (res < 0) {
+						nb.clear();
+						res = poly ? rw4.run<true>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st)
+							   : rw4.run<false>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st);
+						if (res >= 0 && nf > (int)WLayoutT<4>::NBCAP) res = -1;
+						if (res < 0) nb.clear(); else wave4_done++;
+					} else wave2_done++;
 				} else wave_done++;
 			}
 			if (res < 0) res = poly ? r0.run<true>(v, ijk, q - cnt[ijk], ci, cj, ck, &vol, cent, &nf, nb, st)
@@ -296,6 +305,7 @@ static long long compute_prd(const hx_spec *s, int n, const int *ids, const doub
 
 	Runner<WCell, WLayout::bytes> rw(v.max_len_sq);
 	Runner<WCell2, WLayoutT<2>::bytes> rw2(v.max_len_sq);
+	Runner<WCell4, WLayoutT<4>::bytes> rw4(v.max_len_sq);
 	Runner<VCell3, Layout3<>::lane_bytes> r0(v.max_len_sq);
 	Runner<VCell<TierWide, 1>, Layout<TierWide, 1>::bytes> r1(v.max_len_sq);
 	Runner<VCell<TierHuge, 1>, Layout<TierHuge, 1>::bytes> *r2 = 0;
@@ -319,7 +329,13 @@ static long long compute_prd(const hx_spec *s, int n, const int *ids, const doub
 				rr = poly ? rw2.run<true, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key)
 					  : rw2.run<false, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key);
 				if (rr >= 0 && R.nf > (int)WLayoutT<2>::NBCAP) rr = -1;
-				if (rr < 0) R.nb.clear(); else wave2_done++;
+				if (rr < 0) {
+					R.nb.clear();
+					rr = poly ? rw4.run<true, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key)
+						  : rw4.run<false, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key);
+					if (rr >= 0 && R.nf > (int)WLayoutT<4>::NBCAP) rr = -1;
+					if (rr < 0) R.nb.clear(); else wave4_done++;
+				} else wave2_done++;
 			} else wave_done++;
 		}
 		if (rr < 0)

@@ -22,14 +22,15 @@ def run(O, spec, ids, xyz, r=None, list_depth=0, wave=False):
     worklist depth first (the learned path's kernel), the rest on the plain
     tiers; res["list_fallbacks"] counts the cells whose list was too short.
     wave: every cell is first tried by the wavefront-cell tiers;
-    res["wave_done"] / res["wave2_done"] count the cells the 64- and the
-    128-vertex one finished."""
+    res["wave_done"] / res["wave2_done"] / res["wave4_done"] count the cells
+    the 64-, 128- and 254-vertex ones finished."""
     lib = C.CDLL(SO)
     lib.hx_use_list(int(list_depth))
     lib.hx_use_wave(1 if wave else 0)
     lib.hx_list_fallbacks.restype = C.c_longlong
     lib.hx_wave_done.restype = C.c_longlong
     lib.hx_wave2_done.restype = C.c_longlong
+    lib.hx_wave4_done.restype = C.c_longlong
     f = lib.hx_compute_neighbor
     f.restype = C.c_longlong
     n = len(ids)
@@ -65,5 +66,6 @@ def run(O, spec, ids, xyz, r=None, list_depth=0, wave=False):
     out["list_fallbacks"] = int(lib.hx_list_fallbacks())
     out["wave_done"] = int(lib.hx_wave_done())
     out["wave2_done"] = int(lib.hx_wave2_done())
+    out["wave4_done"] = int(lib.hx_wave4_done())
     out["tiers"] = tc
     return out

@@ -184,8 +184,8 @@ def test_wave_cell_periodic_matches_golden(oracle_mod, harness, name):
 
 def test_wave_cell_128_vertices(oracle_mod, harness):
     """Radical cells of a few large particles among small ones grow beyond 64
-    vertices: the 128-vertex wavefront cell (two vertices per lane) takes
-    them, the rest go on to the per-lane tiers, all bit for bit."""
+    vertices: the 128- and 254-vertex wavefront cells (two and four vertices
+    per lane) take them, all bit for bit."""
     O = oracle_mod
     n = 6000
     xyz = I.uniform_box(11, n, (0, 0, 0), (20, 20, 20))
@@ -195,6 +195,6 @@ def test_wave_cell_128_vertices(oracle_mod, harness):
     sp = O.make_spec(1, (0, 20, 0, 20, 0, 20), (10, 10, 10))
     a = O.run("restatement", sp, ids, xyz, r)
     b = harness.run(O, sp, ids, xyz, r, wave=True)
-    assert b["wave2_done"] > 0
+    assert b["wave2_done"] > 0 and b["wave4_done"] > 0
     assert O.compare(b, a, vol_rtol=0.0) == []
     assert np.array_equal(a["vol"], b["vol"]) and np.array_equal(a["cent"], b["cent"])

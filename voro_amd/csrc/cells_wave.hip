@@ -89,8 +89,10 @@ k_wcells(RectView v, int ncells, const int *__restrict__ list, const int *__rest
 						if (lane == 0) { out.nbr_start[cell] = -1; atomicAdd(out.fail_count, 1); }
 					} else {
 						if (lane == 0) out.nbr_start[cell] = off;
-						if (lane < nf) out.nbr[off + lane] = c.nb_lane(0);
-						if (K == 2 && lane + 64 < nf) out.nbr[off + 64 + lane] = c.nb_lane(1);
+						eachk<K>([&](auto hc) __attribute__((always_inline)) {
+							constexpr int h = decltype(hc)::value;
+							if (64 * h + lane < nf) out.nbr[off + 64 * h + lane] = c.nb_lane(h);
+						});
 					}
 				}
 			}
@@ -107,7 +109,16 @@ k_wcells(RectView v, int ncells, const int *__restrict__ list, const int *__rest
 int launch_cells_wave(int k, bool poly, bool prd, int occ, int waves, hipStream_t s, const RectView &v, int ncells,
 		      const int *list, const int *list_count, const int *cellblk, int *counters, char *ws,
 		      unsigned stamp_base, int chunk, const CellOut &out) {
-	if (k == 2) {
+	if (k == 4) {
+		// the second overflow tier: few cells, four vertex sets
+		if (prd) {
+			if (poly) VGPU_WAVE_LAUNCH(4, true, 2, true);
+			else VGPU_WAVE_LAUNCH(4, false, 2, true);
+		} else {
+			if (poly) VGPU_WAVE_LAUNCH(4, true, 2, false);
+			else VGPU_WAVE_LAUNCH(4, false, 2, false);
+		}
+	} else if (k == 2) {
 		// the overflow tier: few cells, registers for two vertex sets
 		if (prd) {
 			if (poly) VGPU_WAVE_LAUNCH(2, true, 2, true);

@@ -107,7 +107,7 @@ struct vgpu_container {
 	DBuf blk, slot, rem, cnt, bstart, tmp, part, P, pid, src, cellblk;
 	DBuf d_mrad, d_wl, d_walls, d_maxr, d_counters;
 	DBuf vol, cent, nfaces, status, nbr_start, arena, csr_off, csr, ctr;
-	DBuf retry0, retry1, retry2, retry3, retry4;
+	DBuf retry0, retry1, retry2, retry3, retry4, retry5;
 	DBuf st_planes, st_exit, st_tier;
 	// learned path
 	bool have_pred = false;
@@ -249,7 +249,7 @@ int default_fast_waves(int occ) {
 // leftovers go to out.retry.
 int launch_wave(vgpu_container *h, int k, const RectView &v, int nc, const int *list, const int *list_count,
 		CellOut out, hipStream_t s) {
-	const int occ = k == 2 ? 2 : h->wave_occ;
+	const int occ = k > 1 ? 2 : h->wave_occ;
 	int waves = (h->wave_waves > 0 && k == 1) ? h->wave_waves : default_fast_waves(occ);
 	int cnt = nc;
 	if (list_count) {
@@ -347,17 +347,19 @@ int run_tiers(vgpu_container *h, const RectView &v, int nc, int fw, const int *l
 	int *ctr = h->ctr.as<int>();
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr + 1, 2, 0);
 	if (h->first_tier == 0) {
-		// one cell per wavefront (up to 64 vertices); its leftovers (count at
-		// ctr[6]) to the same with up to 128 vertices, whose leftovers (count
-		// at ctr[7]) go to order-3 per lane
-		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr + 6, 2, 0);
+		// one cell per wavefront, up to 64 vertices; leftovers (count at
+		// ctr[6]) to the same with up to 128, theirs (ctr[7]) up to 254, and
+		// theirs (ctr[8]) to order-3 per lane
+		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr + 6, 3, 0);
 		out.tier = 0; out.retry = h->retry3.as<int>(); out.retry_count = ctr + 6;
 		if ((st = launch_wave(h, 1, v, nc, list, list_count, out, s))) return st;
 		if (mark_ev2) HIPCHK(hipEventRecord(h->ev[2], s));
 		out.tier = 1; out.retry = h->retry4.as<int>(); out.retry_count = ctr + 7;
 		if ((st = launch_wave(h, 2, v, nc, h->retry3.as<int>(), ctr + 6, out, s))) return st;
-		out.tier = 1; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
-		if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, h->retry4.as<int>(), ctr + 7, out, s))) return st;
+		out.retry = h->retry5.as<int>(); out.retry_count = ctr + 8;
+		if ((st = launch_wave(h, 4, v, nc, h->retry4.as<int>(), ctr + 7, out, s))) return st;
+		out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
+		if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, h->retry5.as<int>(), ctr + 8, out, s))) return st;
 	} else {
 		out.tier = 0; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
 		if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, list, list_count, out, s))) return st;
@@ -531,7 +533,7 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 	    (st = h->status.ensure(C1 * 4)) || (st = h->nbr_start.ensure(C1 * 4)) || (st = h->arena.ensure((size_t)acap * 4)) ||
 	    (st = h->ctr.ensure(16 * 4)) || (st = h->retry0.ensure(C1 * 4)) || (st = h->retry1.ensure(C1 * 4)) ||
 	    (st = h->retry2.ensure(C1 * 4)) || (st = h->retry3.ensure(C1 * 4)) ||
-	    (st = h->retry4.ensure(C1 * 4)))
+	    (st = h->retry4.ensure(C1 * 4)) || (st = h->retry5.ensure(C1 * 4)))
 		return st;
 	const bool stats = (flags & VGPU_F_STATS) != 0;
 	if (stats) {

@@ -137,9 +137,10 @@ int launch_cells_huge(bool poly, int occ, int waves, hipStream_t s, const RectVi
 		      unsigned stamp_base, int per_wave, const CellOut &out);
 
 // The wavefront-cell tiers (cells_wave.hip): one cell per wavefront with k
-// vertices per lane (up to 64 k vertices), over every cell or a device list
-// (count at *list_count), at 2, 3, 4 or 8 waves per SIMD (k = 1; k = 2 runs
-// at 2); cells it cannot finish go to out.retry.
+// = 1, 2 or 4 vertices per lane (up to 64, 128, 254 vertices), over every
+// cell or a device list (count at *list_count), at 2, 3, 4 or 8 waves per
+// SIMD (k = 1; k = 2 and 4 run at 2); cells it cannot finish go to
+// out.retry.
 int launch_cells_wave(int k, bool poly, bool prd, int occ, int waves, hipStream_t s, const RectView &v, int ncells,
 		      const int *list, const int *list_count, const int *cellblk, int *counters, char *ws,
 		      unsigned stamp_base, int chunk, const CellOut &out);

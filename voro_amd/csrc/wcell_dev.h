@@ -37,11 +37,15 @@
 #pragma once
 #include "vgpu_common.h"
 
+#include <type_traits>
+
 namespace vgpu {
 
+// Vertex indices are bytes (0xff stands for the reference's -1), so four
+// vertices per lane hold up to 254.
 template<int K>
 struct WLayoutT {
-	enum { MAXV = 64 * K, MAXS = 256, W = 6, MAXQ = 256, MAXF = 64 * K };
+	enum { MAXV = K < 4 ? 64 * K : 254, MAXS = 256, W = 6, MAXQ = 256, MAXF = 64 * K };
 	enum { WD = 2 * W + 1, WBITS = WD * WD * WD, WWORDS = (WBITS + 31) / 32 };
 	// per-wavefront global slab: block-search window and ring only
 	static constexpr unsigned o_bm = 0;                          // WWORDS
@@ -58,48 +62,92 @@ typedef WLayoutT<1> WLayout;
 #define VG_WDEV 0
 #endif
 
+// Value selection among the K register sets of a wavefront cell (never an
+// lvalue choice: a conditional or branch choosing between members makes the
+// compiler address the cell in memory).
+template<int K, class T>
+VG_INL T pickk(const T (&a)[K], int h) {
+	if constexpr (K == 1) {
+		return a[0];
+	} else if constexpr (K == 2) {
+		const T a0 = a[0], a1 = a[1];
+		return h == 0 ? a0 : a1;
+	} else {
+		// an OR of masked words: a chain of selects over consecutive
+		// elements would be turned into an indexed load
+		typedef typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type U;
+		const U r = (h == 0 ? __builtin_bit_cast(U, a[0]) : U(0)) | (h == 1 ? __builtin_bit_cast(U, a[1]) : U(0)) |
+			    (h == 2 ? __builtin_bit_cast(U, a[2]) : U(0)) | (h == 3 ? __builtin_bit_cast(U, a[3]) : U(0));
+		return __builtin_bit_cast(T, r);
+	}
+}
+// a[h] = x where c, every set updated by value arithmetic
+template<int K, class T>
+VG_INL void putk(T (&a)[K], int h, bool c, T x) {
+	a[0] = (c && h == 0) ? x : a[0];
+	if (K > 1) a[1] = (c && h == 1) ? x : a[1];
+	if (K > 2) { a[2] = (c && h == 2) ? x : a[2]; a[3] = (c && h == 3) ? x : a[3]; }
+}
+
+// f(integral_constant<h>) for h = 0 .. K-1: register sets indexed by constants.
+template<int K, int H = 0, class F>
+VG_INL void eachk(F &&f) {
+	if constexpr (H < K) {
+		f(std::integral_constant<int, H>());
+		eachk<K, H + 1>(f);
+	}
+}
+
 // Vertex marks, bit v = vertex v (K words of 64 bits).
 template<int K>
 struct WMask {
-	unsigned long long lo, hi;
-	VG_INL void clear() { lo = 0; hi = 0; }
-	VG_INL bool get(int v) const {
-		if (K == 1) return (lo >> v) & 1ull;
-		return v < 64 ? ((lo >> v) & 1ull) : ((hi >> (v - 64)) & 1ull);
+	unsigned long long w[K];
+	VG_INL void clear() {
+		w[0] = 0;
+		if (K > 1) w[1] = 0;
+		if (K > 2) { w[2] = 0; w[3] = 0; }
 	}
-	VG_INL void set(int v) {
-		if (K == 1 || v < 64) lo |= 1ull << (v & 63); else hi |= 1ull << (v - 64);
-	}
+	VG_INL bool get(int v) const { return (pickk<K>(w, v >> 6) >> (v & 63)) & 1ull; }
+	VG_INL void set(int v) { set_if(true, v); }
 	VG_INL void unset(int v) {
-		if (K == 1 || v < 64) lo &= ~(1ull << (v & 63)); else hi &= ~(1ull << (v - 64));
+		const unsigned long long b = 1ull << (v & 63);
+		const int h = v >> 6;
+		w[0] &= h == 0 ? ~b : ~0ull;
+		if (K > 1) w[1] &= h == 1 ? ~b : ~0ull;
+		if (K > 2) { w[2] &= h == 2 ? ~b : ~0ull; w[3] &= h == 3 ? ~b : ~0ull; }
 	}
 	VG_INL void toggle(int v) {
-		if (K == 1 || v < 64) lo ^= 1ull << (v & 63); else hi ^= 1ull << (v - 64);
+		const unsigned long long b = 1ull << (v & 63);
+		const int h = v >> 6;
+		w[0] ^= h == 0 ? b : 0ull;
+		if (K > 1) w[1] ^= h == 1 ? b : 0ull;
+		if (K > 2) { w[2] ^= h == 2 ? b : 0ull; w[3] ^= h == 3 ? b : 0ull; }
 	}
-	// set(v) when c, as value arithmetic (no branch the compiler could turn
-	// into a choice between mask objects)
+	// set(v) when c
 	VG_INL void set_if(bool c, int v) {
 		const unsigned long long b = c ? 1ull << (v & 63) : 0ull;
-		if (K == 1) lo |= b;
-		else { lo |= v < 64 ? b : 0ull; hi |= v < 64 ? 0ull : b; }
+		const int h = v >> 6;
+		w[0] |= h == 0 ? b : 0ull;
+		if (K > 1) w[1] |= h == 1 ? b : 0ull;
+		if (K > 2) { w[2] |= h == 2 ? b : 0ull; w[3] |= h == 3 ? b : 0ull; }
 	}
 };
 
 // K vertices per lane: vertex v in lane v & 63 of register set v >> 6.
 template<int K>
 struct WCellT {
-	static_assert(K == 1 || K == 2, "one or two vertices per lane");
+	static_assert(K == 1 || K == 2 || K == 4, "one, two or four vertices per lane");
 	typedef WLayoutT<K> L;
 	enum { MAXV = L::MAXV, MAXS = L::MAXS };
 
 #if VG_WDEV
-	// register set 0 (vertex `lane`) and 1 (vertex 64 + `lane`)
-	double vx0, vy0, vz0, vx1, vy1, vz1;
-	unsigned vt0, vt1;       // e0 | e1 << 8 | e2 << 16 | (r0 | r1 << 2 | r2 << 4) << 24
-	int vna0, vnb0, vnc0, vna1, vnb1, vnc1;   // neighbour ids
-	double vtv0, vtv1;       // value for the current plane
+	// register set h holds vertex 64 h + `lane`
+	double vx[K], vy[K], vz[K];
+	unsigned vt[K];          // e0 | e1 << 8 | e2 << 16 | (r0 | r1 << 2 | r2 << 4) << 24
+	int vna[K], vnb[K], vnc[K];   // neighbour ids
+	double vtv[K];           // value for the current plane
 	unsigned vds, vdm;       // stack entries 4 * lane .. 4 * lane + 3
-	int nbv0, nbv1;          // output staging: neighbour ids number `lane`, 64 + `lane`
+	int nbv[K];              // output staging: neighbour id number 64 h + `lane`
 	int lane;
 	__amdgpu_buffer_rsrc_t rs;
 #else
@@ -125,7 +173,6 @@ struct WCellT {
 	// Value (never lvalue) selection: a conditional over members would make
 	// the compiler address the cell in memory.
 	template<class T> static VG_INL T sel3(int i, T a, T b, T c) { return i == 0 ? a : (i == 1 ? b : c); }
-	template<class T> static VG_INL T selh(int v, T a, T b) { return (K == 1 || v < 64) ? a : b; }
 
 	// ---- lane primitives ----
 #if VG_WDEV
@@ -142,28 +189,23 @@ struct WCellT {
 		const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
 		return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 	}
-	VG_INL double X(int v) const { return rld(selh(v, vx0, vx1), v & 63); }
-	VG_INL double Y(int v) const { return rld(selh(v, vy0, vy1), v & 63); }
-	VG_INL double Z(int v) const { return rld(selh(v, vz0, vz1), v & 63); }
-	VG_INL double T(int v) const { return rld(selh(v, vtv0, vtv1), v & 63); }
-	VG_INL unsigned TW(int v) const { return rlu(selh(v, vt0, vt1), v & 63); }
+	VG_INL double X(int v) const { return rld(pickk<K>(vx, v >> 6), v & 63); }
+	VG_INL double Y(int v) const { return rld(pickk<K>(vy, v >> 6), v & 63); }
+	VG_INL double Z(int v) const { return rld(pickk<K>(vz, v >> 6), v & 63); }
+	VG_INL double T(int v) const { return rld(pickk<K>(vtv, v >> 6), v & 63); }
+	VG_INL unsigned TW(int v) const { return rlu(pickk<K>(vt, v >> 6), v & 63); }
 	VG_INL void setXYZ(int v, double a, double b, double c) {
 		const bool m = lane == (v & 63);
-		if (K == 1 || v < 64) { vx0 = m ? a : vx0; vy0 = m ? b : vy0; vz0 = m ? c : vz0; }
-		else { vx1 = m ? a : vx1; vy1 = m ? b : vy1; vz1 = m ? c : vz1; }
+		putk<K>(vx, v >> 6, m, a); putk<K>(vy, v >> 6, m, b); putk<K>(vz, v >> 6, m, c);
 	}
-	VG_INL void setTW(int v, unsigned w) {
-		const bool m = lane == (v & 63);
-		if (K == 1 || v < 64) vt0 = m ? w : vt0; else vt1 = m ? w : vt1;
-	}
+	VG_INL void setTW(int v, unsigned w) { putk<K>(vt, v >> 6, lane == (v & 63), w); }
 	VG_INL int N(int v, int i) const {
-		const int l = v & 63;
-		return sel3(i, rli(selh(v, vna0, vna1), l), rli(selh(v, vnb0, vnb1), l), rli(selh(v, vnc0, vnc1), l));
+		const int l = v & 63, h = v >> 6;
+		return sel3(i, rli(pickk<K>(vna, h), l), rli(pickk<K>(vnb, h), l), rli(pickk<K>(vnc, h), l));
 	}
 	VG_INL void setN3(int v, int a, int b, int c) {
 		const bool m = lane == (v & 63);
-		if (K == 1 || v < 64) { vna0 = m ? a : vna0; vnb0 = m ? b : vnb0; vnc0 = m ? c : vnc0; }
-		else { vna1 = m ? a : vna1; vnb1 = m ? b : vnb1; vnc1 = m ? c : vnc1; }
+		putk<K>(vna, v >> 6, m, a); putk<K>(vnb, v >> 6, m, b); putk<K>(vnc, v >> 6, m, c);
 	}
 	VG_INL int DS(int i) const { return (int)((rlu(vds, i >> 2) >> (8 * (i & 3))) & 0xffu); }
 	VG_INL void setDS(int i, int v) {
@@ -177,12 +219,9 @@ struct WCellT {
 		const unsigned w = (rlu(vdm, i >> 2) & ~(0xffu << sh)) | ((unsigned)v & 0xffu) << sh;
 		vdm = lane == (i >> 2) ? w : vdm;
 	}
-	VG_INL void put_nb(int i, int v) {
-		const bool m = lane == (i & 63);
-		if (K == 1 || i < 64) nbv0 = m ? v : nbv0; else nbv1 = m ? v : nbv1;
-	}
+	VG_INL void put_nb(int i, int v) { putk<K>(nbv, i >> 6, lane == (i & 63), v); }
 	// neighbour id number h * 64 + `lane`
-	VG_INL int nb_lane(int h = 0) const { const int a = nbv0, b = nbv1; return h == 0 ? a : b; }
+	VG_INL int nb_lane(int h = 0) const { return pickk<K>(nbv, h); }
 	// vertex-parallel: bit l set for live vertices h * 64 + l whose predicate holds
 	VG_INL unsigned long long live_ballot(bool b, int h = 0) const {
 		const unsigned long long m = __ballot(b);
@@ -320,9 +359,9 @@ struct WCellT {
 		const unsigned e = (unsigned)(v ^ (1 + par)) | (unsigned)(v ^ 4) << 8 | (unsigned)(v ^ (2 - par)) << 16;
 		const int xf = -1 - (v & 1), yf = -3 - ((v >> 1) & 1);
 #if VG_WDEV
-		vx0 = (v & 1) ? xmax : xmin; vy0 = (v & 2) ? ymax : ymin; vz0 = (v & 4) ? zmax : zmin;
-		vt0 = e | (6u << 24);
-		vna0 = -5 - (v >> 2); vnb0 = par ? xf : yf; vnc0 = par ? yf : xf;
+		vx[0] = (v & 1) ? xmax : xmin; vy[0] = (v & 2) ? ymax : ymin; vz[0] = (v & 4) ? zmax : zmin;
+		vt[0] = e | (6u << 24);
+		vna[0] = -5 - (v >> 2); vnb[0] = par ? xf : yf; vnc[0] = par ? yf : xf;
 #else
 		setXYZ(v, (v & 1) ? xmax : xmin, (v & 2) ? ymax : ymin, (v & 4) ? zmax : zmin);
 		setTW(v, e | (6u << 24));
@@ -341,20 +380,16 @@ struct WCellT {
 		p = np;
 #if VG_WDEV
 		bool bad = false;
-		if (lane < np) {
-			const UnitVert a = u[lane];
-			bad = a.nu != 3;
-			vx0 = a.x; vy0 = a.y; vz0 = a.z;
-			vt0 = unit_word(a);
-			vna0 = vnb0 = vnc0 = 0;
-		}
-		if (K == 2 && lane + 64 < np) {
-			const UnitVert a = u[lane + 64];
-			bad = bad || a.nu != 3;
-			vx1 = a.x; vy1 = a.y; vz1 = a.z;
-			vt1 = unit_word(a);
-			vna1 = vnb1 = vnc1 = 0;
-		}
+		eachk<K>([&](auto hc) __attribute__((always_inline)) {
+			constexpr int h = decltype(hc)::value;
+			if (64 * h + lane < np) {
+				const UnitVert a = u[64 * h + lane];
+				bad = bad || a.nu != 3;
+				vx[h] = a.x; vy[h] = a.y; vz[h] = a.z;
+				vt[h] = unit_word(a);
+				vna[h] = vnb[h] = vnc[h] = 0;
+			}
+		});
 		if (__ballot(bad)) ovf = true;
 #else
 		for (int v = 0; v < np; v++) {
@@ -375,18 +410,17 @@ struct WCellT {
 	// Returns true if some live vertex is not below -tol.
 	VG_FN bool plane_values() {
 #if VG_WDEV
-		double a = vx0 * px;
-		a += vy0 * py;
-		a += vz0 * pz - prsq;
-		vtv0 = a;
-		bool any = live_ballot(!(a < -tol)) != 0;
-		if (K == 2 && p > 64) {
-			double b = vx1 * px;
-			b += vy1 * py;
-			b += vz1 * pz - prsq;
-			vtv1 = b;
-			any = any || live_ballot(!(b < -tol), 1) != 0;
-		}
+		bool any = false;
+		eachk<K>([&](auto hc) __attribute__((always_inline)) {
+			constexpr int h = decltype(hc)::value;
+			if (h == 0 || p > 64 * h) {
+				double a = vx[h] * px;
+				a += vy[h] * py;
+				a += vz[h] * pz - prsq;
+				vtv[h] = a;
+				any = any || live_ballot(!(a < -tol), h) != 0;
+			}
+		});
 		return any;
 #else
 		bool any = false;
@@ -692,16 +726,16 @@ struct WCellT {
 	// exact, so the order of the comparisons does not matter).
 	VG_FN double max_radius_squared() {
 #if VG_WDEV
-		double s = vx0 * vx0;
-		s += vy0 * vy0;
-		s += vz0 * vz0;
-		if (lane >= p) s = 0;
-		if (K == 2 && p > 64) {
-			double t = vx1 * vx1;
-			t += vy1 * vy1;
-			t += vz1 * vz1;
-			if (lane + 64 < p && t > s) s = t;
-		}
+		double s = 0;
+		eachk<K>([&](auto hc) __attribute__((always_inline)) {
+			constexpr int h = decltype(hc)::value;
+			if (h == 0 || p > 64 * h) {
+				double t = vx[h] * vx[h];
+				t += vy[h] * vy[h];
+				t += vz[h] * vz[h];
+				if (64 * h + lane < p && t > s) s = t;
+			}
+		});
 		for (int o = 32; o > 0; o >>= 1) {
 			const double t = __shfl_xor(s, o);
 			s = t > s ? t : s;
@@ -726,15 +760,17 @@ struct WCellT {
 	VG_FN bool plane_probe(double x, double y, double z, double rsq, bool guess) {
 		if (guess) up = 0;
 #if VG_WDEV
-		const double gl = x * vx0 + y * vy0 + z * vz0;
-		double gh = 0;
-		if (K == 2 && p > 64) gh = x * vx1 + y * vy1 + z * vz1;
-		double g = rld(selh(up, gl, gh), up & 63);
+		double gl[K];
+		eachk<K>([&](auto hc) __attribute__((always_inline)) {
+			constexpr int h = decltype(hc)::value;
+			gl[h] = (h == 0 || p > 64 * h) ? x * vx[h] + y * vy[h] + z * vz[h] : 0.0;
+		});
+		double g = rld(pickk<K>(gl, up >> 6), up & 63);
 		if (!(g < rsq)) return true;
 		if (guess) {
 			int ca = 1, cc = p >> 3, mp = 1;
 			while (ca < cc) {
-				const double m = rld(selh(mp, gl, gh), mp & 63);
+				const double m = rld(pickk<K>(gl, mp >> 6), mp & 63);
 				if (m > g) {
 					if (m > rsq) return true;
 					g = m; up = mp;
@@ -742,8 +778,12 @@ struct WCellT {
 				ca += mp++;
 			}
 		}
-		if (live_ballot(gl > rsq) != 0) return true;
-		return K == 2 && p > 64 && live_ballot(gh > rsq, 1) != 0;
+		bool any = false;
+		eachk<K>([&](auto hc) __attribute__((always_inline)) {
+			constexpr int h = decltype(hc)::value;
+			any = any || ((h == 0 || p > 64 * h) && live_ballot(gl[h] > rsq, h) != 0);
+		});
+		return any;
 #else
 		double g = x * X(up) + y * Y(up) + z * Z(up);
 		if (!(g < rsq)) return true;
@@ -832,5 +872,6 @@ struct WCellT {
 
 typedef WCellT<1> WCell;     // up to 64 vertices
 typedef WCellT<2> WCell2;    // up to 128 vertices
+typedef WCellT<4> WCell4;    // up to 254 vertices
 
 }  // namespace vgpu
