@@ -7,11 +7,13 @@ one pass of the hot path over the particles already resident in HBM:
 stable block binning + every cell + output compaction.
 
 Two paths are timed on the same particles: the plain one (expanding
-worklist, as the reference) and the learned one (candidate lists cut at the
-depth the predictor gives, cells whose list is too short recomputed by the
-plain path; results are checked identical).  The predictor is first trained
-on the GPU on the plain path's statistics for other draws of the workload.
-`value` is the learned path's throughput, the plain path's is under "plain".
+worklist, as the reference; first tier one cell per wavefront) and the
+learned one (candidate lists cut at the depth the predictor gives, cells
+whose list is too short recomputed by the plain path; results are checked
+identical).  The predictor is first trained on the GPU on the plain path's
+statistics for other draws of the workload.  `value` is the learned path's
+throughput, the plain path's is under "plain", with the plain path's
+one-cell-per-lane first tier timed beside it.
 
 Prints one JSON line (see the keys below).  The CPU baseline is the
 reference's own container::compute_all_cells() (built from the reference
@@ -137,6 +139,9 @@ def main():
         dt = time.perf_counter() - t0
         return dt, float(np.mean(tier_ms)), float(np.mean(gather_ms)) if gather_ms else 0.0
 
+    con.set_first_tier(1)
+    dt_lane, lane_tier_ms, _ = timed(flags)
+    con.set_first_tier(0)
     dt_plain, plain_tier_ms, _ = timed(flags)
     r = con.results()
     ncells = len(r["id"])
@@ -155,6 +160,9 @@ def main():
     bytes_launch = ncells * BYTES_PER_CELL_FIXED + 4.0 * nnbr
     hbm_gbs = bytes_launch / kern_s / 1e9
     fp64_tf = ncells * FLOP_PER_CELL / kern_s / 1e12
+    # and of the plain path's wavefront-cell kernel
+    wave_s = plain_tier_ms / 1e3
+    wave_fp64 = ncells * FLOP_PER_CELL / wave_s / 1e12
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_k_cells_list_uniform1m.json")
     if os.path.exists(tpath):
@@ -193,7 +201,15 @@ def main():
         "config": {"workload": "uniform 1M particles in [0,100]^3, 58^3 blocks, seed 42, neighbour outputs",
                    "n_particles": int(args.n), "grid": [G, G, G], "path": "learned",
                    "fast_occupancy": args.occ, "fast_waves": args.waves},
-        "plain": {"value": plain_cells_s, "ms_per_step": dt_plain / args.steps * 1e3, "tier_ms": plain_tier_ms},
+        "plain": {"value": plain_cells_s, "ms_per_step": dt_plain / args.steps * 1e3, "tier_ms": plain_tier_ms,
+                  "first_tier": "one cell per wavefront (k_wcells<1>)",
+                  "per_lane_first_tier": {"value": ncells * args.steps / dt_lane,
+                                          "ms_per_step": dt_lane / args.steps * 1e3, "tier_ms": lane_tier_ms},
+                  "roofline_fp64": {"kernel": "k_wcells<1, mono, 4>", "kernel_ms": plain_tier_ms,
+                                    "achieved": wave_fp64, "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
+                                    "frac": wave_fp64 / MI355X_FP64_TFLOPS},
+                  "roofline_hbm": {"achieved": bytes_launch / wave_s / 1e9, "peak": MI355X_HBM_GBS, "unit": "GB/s",
+                                   "frac": bytes_launch / wave_s / 1e9 / MI355X_HBM_GBS}},
         "learned": {"speedup_vs_plain": cells_s / plain_cells_s, "fallbacks": ls["fallbacks"],
                     "fallback_rate": ls["fallbacks"] / max(ncells, 1), "list_entries": ls["list_entries"],
                     "gather_ms": gather_ms, "tier_ms": tier_ms, "identical_to_plain": identical,

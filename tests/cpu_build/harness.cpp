@@ -54,19 +54,19 @@ struct Runner {
 
 // The list-driven tier: gather the cell's candidate list at depth G, run
 // compute_cell_list; -1 sends the cell on to the plain tiers.
-template<bool POLY>
+template<bool POLY, class CELL = VCell3, size_t BYTES = Layout3<>::lane_bytes>
 struct ListRunner {
 	std::vector<char> slab;
-	VCell3 c;
+	CELL c;
 	std::vector<ListEntry> la;
 	std::vector<ListTag> lb;
-	explicit ListRunner(double mls) : slab(Layout3<>::lane_bytes + 2 * 65536, 0) { c.setup(slab.data() + 65536, 0, mls); }
+	explicit ListRunner(double mls) : slab(BYTES + 2 * 65536, 0) { c.setup(slab.data() + 65536, 0, mls); }
 	int run(const RectView &v, const int *cellblk, int cell, int G, int ijk, int s, double *vol, double *cent, int *nf,
 		std::vector<int> &nb, CellStats &st, long long *fallbacks) {
 		const int n = gather_cell_list<false>(v, cellblk, cell, G, nullptr, nullptr, 1);
 		la.resize(n); lb.resize(n);
 		gather_cell_list<true>(v, cellblk, cell, G, la.data(), lb.data(), 1);
-		int r = compute_cell_list<VCell3, POLY>(c, v, ijk, s, 0, 0, 0, la.data(), lb.data(), 1, n, st);
+		int r = compute_cell_list<CELL, POLY>(c, v, ijk, s, 0, 0, 0, la.data(), lb.data(), 1, n, st);
 		if (r == -1) { (*fallbacks)++; return -1; }
 		if (r < 0 || c.ovf) return -1;
 		if (r == 1) {
@@ -166,6 +166,8 @@ long long hx_compute_neighbor(const hx_spec *s, int n, const int *ids, const dou
 	for (int b = 0; b < nxyz; b++) for (int q = cnt[b]; q < cnt[b + 1]; q++) cellblk[q] = b;
 	ListRunner<false> lr0(v.max_len_sq);
 	ListRunner<true> lr1(v.max_len_sq);
+	ListRunner<false, WCell, WLayout::bytes> lw0(v.max_len_sq);
+	ListRunner<true, WCell, WLayout::bytes> lw1(v.max_len_sq);
 
 	Runner<WCell, WLayout::bytes> rw(v.max_len_sq);
 	Runner<WCell2, WLayoutT<2>::bytes> rw2(v.max_len_sq);
@@ -187,8 +189,12 @@ long long hx_compute_neighbor(const hx_spec *s, int n, const int *ids, const dou
 			int tier = 0;
 			res = -1;
 			if (list_depth > 0) {
-				res = poly ? lr1.run(v, cellblk.data(), cell, list_depth, ijk, q - cnt[ijk], &vol, cent, &nf, nb, st, &list_fallbacks)
-					   : lr0.run(v, cellblk.data(), cell, list_depth, ijk, q - cnt[ijk], &vol, cent, &nf, nb, st, &list_fallbacks);
+				if (use_wave)
+					res = poly ? lw1.run(v, cellblk.data(), cell, list_depth, ijk, q - cnt[ijk], &vol, cent, &nf, nb, st, &list_fallbacks)
+						   : lw0.run(v, cellblk.data(), cell, list_depth, ijk, q - cnt[ijk], &vol, cent, &nf, nb, st, &list_fallbacks);
+				else
+					res = poly ? lr1.run(v, cellblk.data(), cell, list_depth, ijk, q - cnt[ijk], &vol, cent, &nf, nb, st, &list_fallbacks)
+						   : lr0.run(v, cellblk.data(), cell, list_depth, ijk, q - cnt[ijk], &vol, cent, &nf, nb, st, &list_fallbacks);
 				if (res < 0) nb.clear();
 			}
 			if (res < 0 && use_wave) {

@@ -64,8 +64,11 @@ CASES = {
 }
 
 
+@pytest.mark.parametrize("wave", [False, True], ids=["lane", "wave"])
 @pytest.mark.parametrize("case", sorted(CASES))
-def test_list_cutting_on_cpu_matches_oracle(oracle_mod, harness, case):
+def test_list_cutting_on_cpu_matches_oracle(oracle_mod, harness, case, wave):
+    """The list-driven cell loop, on the order-3 cell (one per lane) and on
+    the wavefront cell (64 lanes emulated)."""
     O = oracle_mod
     kind, geom, grid, sph, xyz, r, per = CASES[case]()
     ids = np.arange(len(xyz), dtype=np.int32)
@@ -73,7 +76,7 @@ def test_list_cutting_on_cpu_matches_oracle(oracle_mod, harness, case):
     a = O.run("restatement", sp, ids, xyz, r)
     last_fb = None
     for G in (3, 30, 63):
-        b = harness.run(O, sp, ids, xyz, r, list_depth=G)
+        b = harness.run(O, sp, ids, xyz, r, list_depth=G, wave=wave)
         assert O.compare(b, a, vol_rtol=0.0) == []
         assert np.array_equal(a["vol"], b["vol"]) and np.array_equal(a["cent"], b["cent"])
         if last_fb is not None:

@@ -8,8 +8,15 @@
 
 #include "list_dev.h"
 #include "vgpu_cells.h"
+#include "wcell_dev.h"
 
 namespace vgpu {
+
+// Neighbour ids of a wavefront cell in face-walk order: id i in lane i.
+struct WListSink {
+	WCell *c;
+	VG_INL void put(int i, int v) { if (i < (int)WLayout::NBCAP) c->put_nb(i, v); }
+};
 
 // Schedule: cells are processed in order of decreasing predicted cost
 // (64 cost classes, any order within a class), so that the 64 cells a
@@ -73,15 +80,22 @@ __global__ void k_batch_max(int nc, const int *__restrict__ cnt, int *__restrict
 	if (c == 0) bmax[nb] = 0;
 }
 
+// csr: position i's entries contiguous from boff[i] (wavefront cells);
+// otherwise interleaved per batch of 64 positions from row boff[i / 64].
 __global__ void k_list_fill(RectView v, int nc, const int *__restrict__ cellblk, const int *__restrict__ order,
 			    const unsigned char *__restrict__ depth, int per_cell, const int *__restrict__ boff,
-			    ListEntry *__restrict__ LA, ListTag *__restrict__ LB) {
+			    int csr, ListEntry *__restrict__ LA, ListTag *__restrict__ LB) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= nc) return;
 	const int c = order[i];
 	const int G = depth[per_cell ? c : cellblk[c]];
-	const size_t base = (size_t)boff[i >> 6] * 64 + (i & 63);
-	gather_cell_list<true>(v, cellblk, c, G, LA + base, LB + base, 64);
+	if (csr) {
+		const size_t base = (size_t)boff[i];
+		gather_cell_list<true>(v, cellblk, c, G, LA + base, LB + base, 1);
+	} else {
+		const size_t base = (size_t)boff[i >> 6] * 64 + (i & 63);
+		gather_cell_list<true>(v, cellblk, c, G, LA + base, LB + base, 64);
+	}
 }
 
 // One wavefront per workgroup, a batch of 64 cells at a time (batches
@@ -145,6 +159,95 @@ k_cells_list(RectView v, int nc, const int *__restrict__ cellblk, const int *__r
 	}
 }
 
+// The same over the lists, one cell per wavefront (wcell_dev.h): the list
+// of schedule position pos is contiguous from boff[pos], its entries read
+// with uniform loads.  Positions are taken `chunk` at a time from one
+// counter per XCD share.
+template<bool POLY, int OCC>
+__global__ void __launch_bounds__(64, OCC)
+k_wcells_list(RectView v, int nc, const int *__restrict__ cellblk, const int *__restrict__ order,
+	      const int *__restrict__ cnt, const int *__restrict__ boff, const ListEntry *__restrict__ LA,
+	      const ListTag *__restrict__ LB, int *__restrict__ counters, char *__restrict__ ws, int chunk,
+	      int *__restrict__ fallbacks, CellOut out) {
+	const int lane = threadIdx.x;
+	WCell c;
+	c.setup(ws + (size_t)blockIdx.x * WLayout::bytes, lane, v.max_len_sq);
+	const int nshare = 8;
+	const int my = blockIdx.x % nshare;
+	const int share = (nc + nshare - 1) / nshare;
+	for (int h = 0; h < nshare; h++) {
+		const int sh = (my + h) % nshare;
+		const int lo = sh * share, hi = min(nc, lo + share);
+		for (;;) {
+			int base = 0;
+			if (lane == 0) base = atomicAdd(&counters[sh], chunk);
+			base = __builtin_amdgcn_readfirstlane(base);
+			if (lo + base >= hi) break;
+			const int end = min(hi, lo + base + chunk);
+			for (int pos = lo + base; pos < end; pos++) {
+				const int cell = order[pos];
+				const int ijk = cellblk[cell];
+				const int s = cell - v.bstart[ijk];
+				const size_t bs = (size_t)boff[pos];
+				CellStats stt;
+				const int r = compute_cell_list<WCell, POLY>(c, v, ijk, s, 0, 0, 0, LA + bs, LB + bs, 1, cnt[pos],
+									     stt);
+				double vol = 0, cx = 0, cy = 0, cz = 0;
+				int nf = 0;
+				bool retry = r < 0 || c.ovf;
+				if (!retry && r == 1) {
+					WListSink sk{&c};
+					c.outputs(vol, cx, cy, cz, nf, sk);
+					if (c.ovf || nf > (int)WLayout::NBCAP) retry = true;
+				}
+				if (retry) {
+					if (lane == 0) {
+						if (r == -1) atomicAdd(fallbacks, 1);
+						const int slot_ = atomicAdd(out.retry_count, 1);
+						out.retry[slot_] = cell;
+						out.status[cell] = ST_OVERFLOW;
+					}
+					continue;
+				}
+				if (lane == 0) {
+					out.vol[cell] = vol;
+					out.cent[3 * cell] = cx; out.cent[3 * cell + 1] = cy; out.cent[3 * cell + 2] = cz;
+					out.nfaces[cell] = nf;
+					out.status[cell] = r == 1 ? ST_EXISTS : 0;
+					if (out.st_planes) {
+						out.st_planes[cell] = (short)min(stt.n_planes, 32767);
+						out.st_exit[cell] = (signed char)stt.wl_exit;
+						out.st_tier[cell] = (signed char)out.tier;
+					}
+				}
+				if (out.want_nbrs) {
+					int off = 0;
+					if (lane == 0 && nf > 0) off = atomicAdd(out.arena_top, nf);
+					off = __builtin_amdgcn_readfirstlane(off);
+					if (off + nf > out.arena_cap) {
+						if (lane == 0) { out.nbr_start[cell] = -1; atomicAdd(out.fail_count, 1); }
+					} else {
+						if (lane == 0) out.nbr_start[cell] = off;
+						if (lane < nf) out.nbr[off + lane] = c.nb_lane(0);
+					}
+				}
+			}
+		}
+	}
+}
+
+int launch_cells_wlist(bool poly, int waves, int chunk, hipStream_t s, const RectView &v, int nc, const int *cellblk,
+		       const int *order, const int *cnt, const int *boff, const ListEntry *LA, const ListTag *LB,
+		       int *counters, char *ws, int *fallbacks, const CellOut &out) {
+	if (poly)
+		hipLaunchKernelGGL((k_wcells_list<true, 4>), dim3(waves), dim3(64), 0, s, v, nc, cellblk, order, cnt, boff, LA,
+				   LB, counters, ws, chunk, fallbacks, out);
+	else
+		hipLaunchKernelGGL((k_wcells_list<false, 4>), dim3(waves), dim3(64), 0, s, v, nc, cellblk, order, cnt, boff,
+				   LA, LB, counters, ws, chunk, fallbacks, out);
+	return hipGetLastError() == hipSuccess ? 0 : 6;
+}
+
 int launch_list_order(hipStream_t s, int nc, const int *cellblk, const unsigned char *cost, int per_cell, int *hist,
 		      int *order) {
 	const int g = (nc + 255) / 256;
@@ -163,9 +266,9 @@ int launch_list_gather(hipStream_t s, const RectView &v, int nc, const int *cell
 }
 
 int launch_list_fill(hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
-		     const unsigned char *depth, int per_cell, const int *boff, ListEntry *LA, ListTag *LB) {
+		     const unsigned char *depth, int per_cell, const int *boff, bool csr, ListEntry *LA, ListTag *LB) {
 	hipLaunchKernelGGL(k_list_fill, dim3((nc + 255) / 256), dim3(256), 0, s, v, nc, cellblk, order, depth, per_cell,
-			   boff, LA, LB);
+			   boff, csr ? 1 : 0, LA, LB);
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 

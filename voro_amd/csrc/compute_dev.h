@@ -616,8 +616,12 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, int ci, 
 	double mrs = 0;
 	int next_count = 3, cpi = 0, g = 0, w = 0, k = 0, left = 0;
 	bool own_done = false, bcheck = false;
+	// upper bound on max_radius_squared(), as in compute_cell_rect
+	double mrs_b = 1.7976931348623157e308;
+	int since = 0;
 	for (;;) {
 		double px_ = 0, py_ = 0, pz_ = 0, rs = 0;
+		double rs_raw = -1;
 		int pid = 0;
 		bool have = false;
 		while (!have) {
@@ -639,22 +643,25 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, int ci, 
 			k++;
 			if (tg.kind == LK_OWN) {
 				px_ = a.x; py_ = a.y; pz_ = a.z; pid = tg.pid;
-				rs = rp.scale(px_ * px_ + py_ * py_ + pz_ * pz_, a.w);
+				rs_raw = px_ * px_ + py_ * py_ + pz_ * pz_;
+				rs = rp.scale(rs_raw, a.w);
 				have = true;
 				continue;
 			}
 			if (tg.kind == LK_PART) {
 				px_ = a.x; py_ = a.y; pz_ = a.z; pid = tg.pid;
 				rs = px_ * px_ + py_ * py_ + pz_ * pz_;
+				rs_raw = rs;
 				if (!bcheck) { rs = rp.scale(rs, a.w); have = true; }
 				else have = rp.scale_check(rs, mrs, a.w);
 				if (--left == 0) bcheck = false;
 				continue;
 			}
 			// a worklist step: the own block is done (src/v_compute.cc:328)
-			if (!own_done) { mrs = c.max_radius_squared(); own_done = true; }
+			if (!own_done) { mrs = c.max_radius_squared(); mrs_b = mrs; since = 0; own_done = true; }
 			if (g < 63 && g == next_count) {
 				mrs = c.max_radius_squared();
+				mrs_b = mrs; since = 0;
 				if (cpi != 8) {
 					next_count = count_list(cpi++);
 				}
@@ -672,6 +679,12 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, int ci, 
 			bcheck = rp.ctest(a.z, mrs);
 			left = cnt;
 			if (cnt == 0) bcheck = false;
+		}
+		if (rs_raw >= 0) {
+			// the exact skip of a plane that provably misses the cell (see
+			// compute_cell_rect); the call still counts
+			if (++since >= 16) { mrs_b = c.max_radius_squared(); since = 0; }
+			if (rs - sqrt(rs_raw * mrs_b) > 2.0 * c.tol) { c.n_cuts++; continue; }
 		}
 		if (!c.nplane(px_, py_, pz_, rs, pid)) { stt.n_planes = c.n_cuts; return c.ovf ? -2 : 0; }
 	}

@@ -304,8 +304,9 @@ int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut
 				h->d_cost.as<unsigned char>(), s)))
 		return st;
 	const int nb = nblk(nc, 64);
-	if ((st = h->lcnt.ensure((size_t)nc * 4)) || (st = h->bmax.ensure((size_t)(nb + 1) * 4)) ||
-	    (st = h->boff.ensure((size_t)(nb + 1) * 4)) || (st = h->d_order.ensure((size_t)nc * 4)) ||
+	const bool csr = h->first_tier == 0;   // wavefront cells: each cell's list contiguous
+	if ((st = h->lcnt.ensure((size_t)(nc + 1) * 4)) || (st = h->bmax.ensure((size_t)(nb + 1) * 4)) ||
+	    (st = h->boff.ensure((size_t)(std::max(nb, nc) + 1) * 4)) || (st = h->d_order.ensure((size_t)nc * 4)) ||
 	    (st = h->d_hist.ensure(64 * 4)))
 		return st;
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, h->d_hist.as<int>(), 64, 0);
@@ -315,26 +316,51 @@ int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut
 	if (launch_list_gather(s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(), h->d_depth.as<unsigned char>(),
 			       h->pred_with_r, h->lcnt.as<int>(), h->bmax.as<int>()))
 		return VGPU_HIP_ERROR;
-	if ((st = scan_excl(h->part, h->bmax.as<int>(), h->boff.as<int>(), nb + 1, s))) return st;
-	int rows = 0;
-	HIPCHK(hipMemcpyAsync(&rows, h->boff.as<int>() + nb, 4, hipMemcpyDeviceToHost, s));
-	HIPCHK(hipStreamSynchronize(s));
-	const size_t ent = (size_t)std::max(rows, 1) * 64;
-	h->last_entries = (long long)rows * 64;
+	size_t ent;
+	if (csr) {
+		// boff[pos] = entries before schedule position pos
+		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, h->lcnt.as<int>() + nc, 1, 0);
+		if ((st = scan_excl(h->part, h->lcnt.as<int>(), h->boff.as<int>(), nc + 1, s))) return st;
+		int tot = 0;
+		HIPCHK(hipMemcpyAsync(&tot, h->boff.as<int>() + nc, 4, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipStreamSynchronize(s));
+		if (tot < 0) return VGPU_MEMORY_ERROR;
+		ent = (size_t)std::max(tot, 1);
+		h->last_entries = tot;
+	} else {
+		// boff[b] = rows before batch b of 64 positions (the longest list sets a batch's rows)
+		if ((st = scan_excl(h->part, h->bmax.as<int>(), h->boff.as<int>(), nb + 1, s))) return st;
+		int rows = 0;
+		HIPCHK(hipMemcpyAsync(&rows, h->boff.as<int>() + nb, 4, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipStreamSynchronize(s));
+		ent = (size_t)std::max(rows, 1) * 64;
+		h->last_entries = (long long)rows * 64;
+	}
 	if ((st = h->list_a.ensure(ent * sizeof(ListEntry))) || (st = h->list_b.ensure(ent * sizeof(ListTag)))) return st;
 	if (launch_list_fill(s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(), h->d_depth.as<unsigned char>(),
-			     h->pred_with_r, h->boff.as<int>(), h->list_a.as<ListEntry>(), h->list_b.as<ListTag>()))
+			     h->pred_with_r, h->boff.as<int>(), csr, h->list_a.as<ListEntry>(), h->list_b.as<ListTag>()))
 		return VGPU_HIP_ERROR;
 	if (!h->ev_list) HIPCHK(hipEventCreate(&h->ev_list));
 	HIPCHK(hipEventRecord(h->ev_list, s));
-	if ((st = h->ws_fast.ensure(VCell3::L::bytes, waves))) return st;
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, h->d_counters.as<int>(), 8, 0);
 	out.tier = 0; out.retry = h->retry0.as<int>(); out.retry_count = ctr + 4;
-	if (launch_cells_list(h->poly, waves, s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(),
-			      h->lcnt.as<int>(),
-			      h->boff.as<int>(), h->list_a.as<ListEntry>(), h->list_b.as<ListTag>(), h->d_counters.as<int>(),
-			      h->ws_fast.base(), ctr + 5, out))
-		return VGPU_HIP_ERROR;
+	if (h->first_tier == 0) {
+		// one cell per wavefront
+		int w = h->wave_waves > 0 ? h->wave_waves : default_fast_waves(4);
+		const int chunk = nc >= 4 * w ? 4 : 1;
+		w = std::max(1, std::min(w, (nc + chunk - 1) / chunk));
+		if ((st = h->ws_wave.ensure(WLayout::bytes, w))) return st;
+		if (launch_cells_wlist(h->poly, w, chunk, s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(),
+				       h->lcnt.as<int>(), h->boff.as<int>(), h->list_a.as<ListEntry>(),
+				       h->list_b.as<ListTag>(), h->d_counters.as<int>(), h->ws_wave.base(), ctr + 5, out))
+			return VGPU_HIP_ERROR;
+	} else {
+		if ((st = h->ws_fast.ensure(VCell3::L::bytes, waves))) return st;
+		if (launch_cells_list(h->poly, waves, s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(),
+				      h->lcnt.as<int>(), h->boff.as<int>(), h->list_a.as<ListEntry>(),
+				      h->list_b.as<ListTag>(), h->d_counters.as<int>(), h->ws_fast.base(), ctr + 5, out))
+			return VGPU_HIP_ERROR;
+	}
 	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
 }
 
@@ -342,7 +368,7 @@ int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut
 // of cells whose count is at *list_count: order-3, then wide, then huge
 // over each tier's overflow.
 int run_tiers(vgpu_container *h, const RectView &v, int nc, int fw, const int *list, const int *list_count,
-	      CellOut out, hipStream_t s, bool mark_ev2) {
+	      CellOut out, hipStream_t s, bool mark_ev2, int tier0 = 0) {
 	int st;
 	int *ctr = h->ctr.as<int>();
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr + 1, 2, 0);
@@ -351,7 +377,7 @@ int run_tiers(vgpu_container *h, const RectView &v, int nc, int fw, const int *l
 		// ctr[6]) to the same with up to 128, theirs (ctr[7]) up to 254, and
 		// theirs (ctr[8]) to order-3 per lane
 		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr + 6, 3, 0);
-		out.tier = 0; out.retry = h->retry3.as<int>(); out.retry_count = ctr + 6;
+		out.tier = tier0; out.retry = h->retry3.as<int>(); out.retry_count = ctr + 6;
 		if ((st = launch_wave(h, 1, v, nc, list, list_count, out, s))) return st;
 		if (mark_ev2) HIPCHK(hipEventRecord(h->ev[2], s));
 		out.tier = 1; out.retry = h->retry4.as<int>(); out.retry_count = ctr + 7;
@@ -361,7 +387,7 @@ int run_tiers(vgpu_container *h, const RectView &v, int nc, int fw, const int *l
 		out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
 		if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, h->retry5.as<int>(), ctr + 8, out, s))) return st;
 	} else {
-		out.tier = 0; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
+		out.tier = tier0; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
 		if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, list, list_count, out, s))) return st;
 		if (mark_ev2) HIPCHK(hipEventRecord(h->ev[2], s));
 	}
@@ -562,22 +588,16 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 	if (nc > 0 && h->prd) {
 		if ((st = run_prd_cells(h, v, out, nc, fw, s))) return st;
 	} else if (nc > 0 && learned) {
-		// tier 0: candidate lists cut at the predicted worklist depth
+		// tier 0: candidate lists cut at the predicted worklist depth; the
+		// cells it does not finish (retry0, count ctr[4]) take the plain tiers
 		if ((st = run_learned(h, v, nc, fw, out, s))) return st;
 		HIPCHK(hipEventRecord(h->ev[2], s));
-		out.tier = 1; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
-		if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, h->retry0.as<int>(), ctr + 4, out, s))) return st;
+		if ((st = run_tiers(h, v, nc, fw, h->retry0.as<int>(), ctr + 4, out, s, false, 1))) return st;
 	} else if (nc > 0) {
 		// the plain tiers: wavefront cells (or order-3 per lane), wide, huge
 		if ((st = run_tiers(h, v, nc, fw, nullptr, nullptr, out, s, true))) return st;
 	} else {
 		HIPCHK(hipEventRecord(h->ev[2], s));
-	}
-	if (nc > 0 && !h->prd && learned) {
-		out.tier = 2; out.retry = h->retry2.as<int>(); out.retry_count = ctr + 2;
-		if ((st = launch_tier<TierWide>(h, h->ws_wide, RETRY_WAVES_WIDE, v, nc, h->retry1.as<int>(), ctr + 1, out, s))) return st;
-		out.tier = 3; out.retry = nullptr; out.retry_count = nullptr;
-		if ((st = launch_tier<TierHuge>(h, h->ws_huge, RETRY_WAVES_HUGE, v, nc, h->retry2.as<int>(), ctr + 2, out, s))) return st;
 	}
 	HIPCHK(hipEventRecord(h->ev[3], s));
 	h->csr_valid = false;

@@ -156,10 +156,16 @@ int launch_list_order(hipStream_t s, int nc, const int *cellblk, const unsigned 
 int launch_list_gather(hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
 		       const unsigned char *depth, int per_cell, int *cnt, int *bmax);
 int launch_list_fill(hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
-		     const unsigned char *depth, int per_cell, const int *boff, ListEntry *LA, ListTag *LB);
+		     const unsigned char *depth, int per_cell, const int *boff, bool csr, ListEntry *LA, ListTag *LB);
 int launch_cells_list(bool poly, int waves, hipStream_t s, const RectView &v, int nc, const int *cellblk,
 		      const int *order, const int *cnt, const int *boff, const ListEntry *LA, const ListTag *LB,
 		      int *counters, char *ws, int *fallbacks, const CellOut &out);
+// The list-driven tier with one cell per wavefront (4 waves per SIMD) over
+// lists filled with csr, cells taken `chunk` at a time; ws holds one
+// wavefront-cell slab per wavefront.
+int launch_cells_wlist(bool poly, int waves, int chunk, hipStream_t s, const RectView &v, int nc, const int *cellblk,
+		       const int *order, const int *cnt, const int *boff, const ListEntry *LA, const ListTag *LB,
+		       int *counters, char *ws, int *fallbacks, const CellOut &out);
 
 #define VGPU_CELLS_LAUNCH(TIER, P, O)                                                                      \
 	hipLaunchKernelGGL((k_cells<TIER, P, O>), dim3(waves), dim3(64), 0, s, v, ncells, list, list_count,    \
