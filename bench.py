@@ -163,10 +163,11 @@ def main():
     # and of the plain path's wavefront-cell kernel
     wave_s = plain_tier_ms / 1e3
     wave_fp64 = ncells * FLOP_PER_CELL / wave_s / 1e12
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_k_cells_list_uniform1m.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
+    def traffic_of(name):
+        tpath = os.path.join(ROOT, "profiles", name)
+        return json.load(open(tpath)).get("bytes_per_launch") if os.path.exists(tpath) else None
+    traffic = traffic_of("traffic_k_wcells_list_uniform1m.json")
+    wave_traffic = traffic_of("traffic_k_wcells_uniform1m.json")
 
     cpu = None
     if not args.no_cpu:
@@ -209,14 +210,14 @@ def main():
                                     "achieved": wave_fp64, "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
                                     "frac": wave_fp64 / MI355X_FP64_TFLOPS},
                   "roofline_hbm": {"achieved": bytes_launch / wave_s / 1e9, "peak": MI355X_HBM_GBS, "unit": "GB/s",
-                                   "frac": bytes_launch / wave_s / 1e9 / MI355X_HBM_GBS}},
+                                   "frac": bytes_launch / wave_s / 1e9 / MI355X_HBM_GBS, "traffic": wave_traffic}},
         "learned": {"speedup_vs_plain": cells_s / plain_cells_s, "fallbacks": ls["fallbacks"],
                     "fallback_rate": ls["fallbacks"] / max(ncells, 1), "list_entries": ls["list_entries"],
                     "gather_ms": gather_ms, "tier_ms": tier_ms, "identical_to_plain": identical,
                     "train_seconds": t_train, "train_loss": train_loss, "train_seeds": list(args.train_seeds)},
         "roofline": {"bound": "hbm", "achieved": hbm_gbs, "peak": MI355X_HBM_GBS, "unit": "GB/s",
                      "frac": hbm_gbs / MI355X_HBM_GBS, "traffic": traffic,
-                     "kernel": "k_cells_list<mono>", "kernel_ms": kern_s * 1e3},
+                     "kernel": "k_wcells_list<mono, 4>", "kernel_ms": kern_s * 1e3},
         "roofline_fp64": {"achieved": fp64_tf, "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
                           "frac": fp64_tf / MI355X_FP64_TFLOPS},
         "cpu_baseline": cpu,
