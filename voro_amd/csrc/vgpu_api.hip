@@ -121,6 +121,8 @@ struct vgpu_container {
 	// periodic containers: unit cell, block arrays with image blocks, first
 	// visits, cells that read an image block
 	DBuf d_unit, Pall, pidall, ball, fv, touch, d_flag;
+	DBuf img_p, img_id, img_reg, img_rank, img_bc, img_cnt;   // image particles for the device assembly
+	std::vector<int> h_rank;                                   // per-block creation counters (host, kept zero)
 	std::vector<dbl4> hP;
 	std::vector<int> hpid, hcs;
 	int image_passes = 0;
@@ -399,20 +401,49 @@ int run_tiers(vgpu_container *h, const RectView &v, int nc, int fw, const int *l
 
 // Uploads the block arrays of every block (primary and image) and points
 // the view at them.
-int upload_blocks(vgpu_container *h, const ImageSet &im, RectView &v, hipStream_t s) {
-	std::vector<dbl4> Pa;
-	std::vector<int> ia, ba;
-	im.assemble(Pa, ia, ba);
+int upload_blocks(vgpu_container *h, const ImageSet &im, int nc, RectView &v, hipStream_t s) {
+	const int nb = h->nblocks;
+	const int ni = (int)im.reg.size();
+	// position of every image particle within its block (creation order),
+	// and the particle count of every image block
+	if ((int)h->h_rank.size() != nb) h->h_rank.assign(nb, 0);
+	std::vector<int> rank(std::max(ni, 1));
+	std::vector<int2> bc;
+	for (int t = 0; t < ni; t++) {
+		const int r = im.reg[t];
+		if (h->h_rank[r] == 0) bc.push_back(int2{r, 0});
+		rank[t] = h->h_rank[r]++;
+	}
+	for (int2 &q : bc) { q.y = h->h_rank[q.x]; h->h_rank[q.x] = 0; }
+	const int nbc = (int)bc.size();
 	int st;
-	if ((st = h->Pall.ensure(Pa.size() * sizeof(dbl4))) || (st = h->pidall.ensure(ia.size() * 4)) ||
-	    (st = h->ball.ensure(ba.size() * 4)))
+	if ((st = h->img_p.ensure((size_t)std::max(ni, 1) * sizeof(dbl4))) || (st = h->img_id.ensure((size_t)std::max(ni, 1) * 4)) ||
+	    (st = h->img_reg.ensure((size_t)std::max(ni, 1) * 4)) || (st = h->img_rank.ensure((size_t)std::max(ni, 1) * 4)) ||
+	    (st = h->img_bc.ensure((size_t)std::max(nbc, 1) * sizeof(int2))) ||
+	    (st = h->img_cnt.ensure((size_t)(nb + 1) * 4)) || (st = h->Pall.ensure((size_t)std::max(nc + ni, 1) * sizeof(dbl4))) ||
+	    (st = h->pidall.ensure((size_t)std::max(nc + ni, 1) * 4)) || (st = h->ball.ensure((size_t)(nb + 1) * 4)))
 		return st;
-	HIPCHK(hipMemcpyAsync(h->Pall.p, Pa.data(), Pa.size() * sizeof(dbl4), hipMemcpyHostToDevice, s));
-	HIPCHK(hipMemcpyAsync(h->pidall.p, ia.data(), ia.size() * 4, hipMemcpyHostToDevice, s));
-	HIPCHK(hipMemcpyAsync(h->ball.p, ba.data(), ba.size() * 4, hipMemcpyHostToDevice, s));
+	if (ni) {
+		HIPCHK(hipMemcpyAsync(h->img_p.p, im.ip.data(), (size_t)ni * sizeof(dbl4), hipMemcpyHostToDevice, s));
+		HIPCHK(hipMemcpyAsync(h->img_id.p, im.iid.data(), (size_t)ni * 4, hipMemcpyHostToDevice, s));
+		HIPCHK(hipMemcpyAsync(h->img_reg.p, im.reg.data(), (size_t)ni * 4, hipMemcpyHostToDevice, s));
+		HIPCHK(hipMemcpyAsync(h->img_rank.p, rank.data(), (size_t)ni * 4, hipMemcpyHostToDevice, s));
+	}
+	if (nbc) HIPCHK(hipMemcpyAsync(h->img_bc.p, bc.data(), (size_t)nbc * sizeof(int2), hipMemcpyHostToDevice, s));
+	int *cnt = h->img_cnt.as<int>();
+	hipLaunchKernelGGL(k_prd_counts, dim3(nblk(nb + 1, 256)), dim3(256), 0, s, nb, h->bstart.as<int>(), cnt);
+	if (nbc) hipLaunchKernelGGL(k_prd_img_counts, dim3(nblk(nbc, 256)), dim3(256), 0, s, nbc, h->img_bc.as<int2>(), cnt);
+	if ((st = scan_excl(h->part, cnt, h->ball.as<int>(), nb + 1, s))) return st;
+	if (nc) hipLaunchKernelGGL(k_prd_copy_primary, dim3(nblk(nc, 256)), dim3(256), 0, s, nc, h->cellblk.as<int>(),
+				   h->bstart.as<int>(), h->P.as<dbl4>(), h->pid.as<int>(), h->ball.as<int>(),
+				   h->Pall.as<dbl4>(), h->pidall.as<int>());
+	if (ni) hipLaunchKernelGGL(k_prd_copy_images, dim3(nblk(ni, 256)), dim3(256), 0, s, ni, h->img_reg.as<int>(),
+				   h->img_rank.as<int>(), h->img_p.as<dbl4>(), h->img_id.as<int>(), h->ball.as<int>(),
+				   h->Pall.as<dbl4>(), h->pidall.as<int>());
+	// the host vectors are read by the copies above
 	HIPCHK(hipStreamSynchronize(s));
 	v.P = h->Pall.as<dbl4>(); v.pid = h->pidall.as<int>(); v.bstart = h->ball.as<int>();
-	return VGPU_OK;
+	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
 }
 
 // The cells of a periodic container.  Pass 1 computes every cell against
@@ -434,7 +465,7 @@ int run_prd_cells(vgpu_container *h, RectView &v, CellOut out, int nc, int fw, h
 	{
 		ImageSet im(g, h->hP.data(), h->hpid.data(), h->hcs.data());
 		for (int b : prd_margin_images(g, PRD_FIRST_MARGIN)) im.create_block(b);
-		if ((st = upload_blocks(h, im, v, s))) return st;
+		if ((st = upload_blocks(h, im, nc, v, s))) return st;
 	}
 	if ((st = h->fv.ensure((size_t)nb * 8)) || (st = h->touch.ensure((size_t)nc))) return st;
 	unsigned long long *fv = h->fv.as<unsigned long long>();
@@ -459,7 +490,7 @@ int run_prd_cells(vgpu_container *h, RectView &v, CellOut out, int nc, int fw, h
 		ImageSet im(g, h->hP.data(), h->hpid.data(), h->hcs.data());
 		for (int b : prd_visit_order(g, prev.data())) im.create_block(b);
 		if (im.late) return VGPU_INTERNAL_ERROR;
-		if ((st = upload_blocks(h, im, v, s))) return st;
+		if ((st = upload_blocks(h, im, nc, v, s))) return st;
 		int *ctr = h->ctr.as<int>();
 		const int m = (int)list.size();
 		HIPCHK(hipMemcpyAsync(h->retry0.p, list.data(), (size_t)m * 4, hipMemcpyHostToDevice, s));
@@ -956,7 +987,8 @@ int vgpu_destroy(vgpu_container *h) {
 			&h->csr, &h->ctr, &h->retry0, &h->retry1, &h->retry2, &h->st_planes, &h->st_exit, &h->st_tier,
 			&h->d_pred, &h->d_feat, &h->d_cfeat, &h->d_depth, &h->d_cost, &h->d_order, &h->d_hist, &h->lcnt,
 			&h->bmax, &h->boff, &h->list_a, &h->list_b, &h->d_unit, &h->Pall, &h->pidall, &h->ball, &h->fv,
-			&h->touch, &h->d_flag};
+			&h->touch, &h->d_flag, &h->retry3, &h->retry4, &h->retry5, &h->img_p, &h->img_id,
+			&h->img_reg, &h->img_rank, &h->img_bc, &h->img_cnt};
 	for (DBuf *b : bufs) b->release();
 	h->ws_fast.buf.release(); h->ws_wide.buf.release(); h->ws_huge.buf.release(); h->ws_wave.buf.release();
 	if (h->events) for (int t = 0; t < 5; t++) (void)hipEventDestroy(h->ev[t]);

@@ -83,6 +83,43 @@ __global__ void k_locate_prd(int n, const double *__restrict__ xyz, const double
 	}
 }
 
+// Block arrays of a periodic container with its image blocks, assembled on
+// the device: the primary blocks' particles as binned, the image particles
+// (built on the host in creation order, `rank` = position within its block)
+// after them in block order.  cnt[b] = particles of block b (primary counts
+// from the binning, image counts scattered), then an exclusive scan gives
+// the block starts.
+__global__ void k_prd_counts(int nblocks, const int *__restrict__ bstart, int *__restrict__ cnt) {
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b > nblocks) return;
+	cnt[b] = b < nblocks ? bstart[b + 1] - bstart[b] : 0;
+}
+__global__ void k_prd_img_counts(int n, const int2 *__restrict__ bc, int *__restrict__ cnt) {
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	const int2 q = bc[t];
+	cnt[q.x] = q.y;
+}
+__global__ void k_prd_copy_primary(int nc, const int *__restrict__ cellblk, const int *__restrict__ bstart,
+				   const dbl4 *__restrict__ P, const int *__restrict__ pid, const int *__restrict__ ball,
+				   dbl4 *__restrict__ Pall, int *__restrict__ pidall) {
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= nc) return;
+	const int b = cellblk[c];
+	const int d = ball[b] + (c - bstart[b]);
+	Pall[d] = P[c];
+	pidall[d] = pid[c];
+}
+__global__ void k_prd_copy_images(int n, const int *__restrict__ reg, const int *__restrict__ rank,
+				  const dbl4 *__restrict__ ip, const int *__restrict__ iid, const int *__restrict__ ball,
+				  dbl4 *__restrict__ Pall, int *__restrict__ pidall) {
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	const int d = ball[reg[t]] + rank[t];
+	Pall[d] = ip[t];
+	pidall[d] = iid[t];
+}
+
 // check_duplicate (src/common.cc:14-20), which the periodic containers'
 // put() applies against every particle already in the block
 // (src/container_prd.cc:94-112): flags a pair closer than 1e-5.
