@@ -84,6 +84,22 @@ def test_gpu_learned_uniform_is_exact_and_saves_work():
     assert st["list_entries"] < full["list_entries"]
 
 
+def test_gpu_learned_per_lane_lists_are_exact():
+    """With the per-lane first tier the lists are interleaved per batch of 64
+    cells and cut by the per-lane list kernel: the same results."""
+    n, L, G = 50000, 36.8, 21
+    make = lambda sd: _uniform(sd, n, L, G)
+    pred, _, _ = _train(make, [31], epochs=100)
+    con = make(32)
+    con.compute_all_cells(F)
+    plain = dict(con.results())
+    con.set_first_tier(1)
+    con.set_predictor(pred)
+    con.compute_all_cells(F | voro_amd.F_LEARNED)
+    con._res = None
+    _same(plain, con.results())
+
+
 def test_gpu_learned_forced_fallback_is_exact():
     n, L, G = 20000, 27.1, 16
     make = lambda sd: _uniform(sd, n, L, G)

@@ -128,3 +128,36 @@ def test_gpu_full_size_uniform(oracle_mod):
     a = O.run("restatement", O.make_spec(0, (0, 100, 0, 100, 0, 100), (58, 58, 58)), ids, xyz)
     assert O.compare(b, a, vol_rtol=VOL_RTOL) == []
     assert np.array_equal(a["vol"], b["vol"])
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_gpu_first_tiers_agree(oracle_mod, case):
+    """One cell per wavefront (default) and one order-3 cell per lane as the
+    first tier: both bit-exact against the restatement."""
+    O = oracle_mod
+    kind, geom, grid, xyz, r, per, sph = CASES[case]()
+    ids = np.arange(len(xyz), dtype=np.int32)
+    _flush("first tiers %s n=%d" % (case, len(ids)))
+    a = O.run("restatement", O.make_spec(kind, geom, grid, periodic=per, spheres=sph), ids, xyz, r)
+    for tier in (0, 1):
+        b = engine_run(kind, geom, grid, ids, xyz, r, periodic=per, spheres=sph, stats=True, first_tier=tier)
+        assert O.compare(b, a, vol_rtol=VOL_RTOL) == []
+        assert np.array_equal(a["vol"], b["vol"]) and np.array_equal(a["cent"], b["cent"])
+
+
+def test_gpu_large_cells_take_the_wide_wavefront_tiers(oracle_mod):
+    """Radical cells of a few large particles among small ones have more
+    than 64 (and more than 128) vertices: the 128- and 254-vertex wavefront
+    tiers (stats tier 1) finish them, bit-exact."""
+    O = oracle_mod
+    n = 6000
+    xyz = I.uniform_box(11, n, (0, 0, 0), (20, 20, 20))
+    r = np.full(n, 0.3)
+    r[::100] = 2.0
+    ids = np.arange(n, dtype=np.int32)
+    _flush("large radical cells n=%d" % n)
+    a = O.run("restatement", O.make_spec(1, (0, 20, 0, 20, 0, 20), (10, 10, 10)), ids, xyz, r, stats=True)
+    b = engine_run(1, (0, 20, 0, 20, 0, 20), (10, 10, 10), ids, xyz, r, stats=True)
+    assert (a["st_max_p"] > 128).any() and (b["st_tier"] == 1).sum() >= (a["st_max_p"] > 64).sum()
+    assert O.compare(b, a, vol_rtol=VOL_RTOL) == []
+    assert np.array_equal(a["vol"], b["vol"])
