@@ -3,7 +3,8 @@ results from one batched compute equal the oracle's for that container
 alone, bit for bit (neighbours, face counts, existence, volumes), for the
 plain and radical kinds, with sphere walls, empty containers and dropped
 particles; graph replays (also after new positions) equal eager computes;
-the autotuner's choice does not change results."""
+both first tiers (one cell per wavefront, one cell per lane) and the
+autotuner's choice do not change results."""
 import numpy as np
 import pytest
 
@@ -79,6 +80,40 @@ def test_gpu_batch_poly_uneven_matches_oracle(oracle_mod):
         assert O.compare(got, want, vol_rtol=VOL_RTOL) == [], i
 
 
+def test_gpu_batch_first_tiers_and_leftovers_match_oracle(oracle_mod):
+    """Radical containers with a few large particles among small ones: their
+    cells exceed the 64 vertices of the wavefront-cell first tier (more than
+    34 faces), so they pass to the order-3 per-lane tier and, beyond 128
+    vertices, to the wide tier.  Both first tiers give the oracle's results
+    bit for bit."""
+    O = oracle_mod
+    cases = []
+    for c in range(3):
+        n = 3000 + 1000 * c
+        xyz = I.uniform_box(21 + c, n, (0, 0, 0), (16, 16, 16))
+        r = np.full(n, 0.3)
+        r[c::90] = 2.0
+        cases.append((xyz, r))
+    cons = []
+    want = []
+    for xyz, r in cases:
+        con = voro_amd.container_poly(0, 16, 0, 16, 0, 16, 8, 8, 8, False, False, False, 8)
+        con.put_array(np.arange(len(xyz), dtype=np.int32), xyz, r)
+        cons.append(con)
+        want.append(O.run("restatement", O.make_spec(1, (0, 16, 0, 16, 0, 16), (8, 8, 8)),
+                          np.arange(len(xyz), dtype=np.int32), xyz, r))
+    assert max(int(w["nfaces"].max()) for w in want) > 66     # cells over 128 vertices
+    b = voro_amd.container_batch(cons)
+    for cfg in ((0, 0, 16), (0, 0, 1), (0, 2, 64)):
+        _flush("batch first tier config %s" % (cfg,))
+        b.set_config(*cfg)
+        b.compute()
+        for i in range(len(cases)):
+            got = b.results(i)
+            assert O.compare(got, want[i], vol_rtol=0.0) == [], (cfg, i)
+            assert np.array_equal(got["vol"], want[i]["vol"]), (cfg, i)
+
+
 def test_gpu_batch_equals_single_compute():
     specs, cons = _fcc_containers(4, 12)
     _flush("batch vs single")
@@ -134,7 +169,7 @@ def test_gpu_batch_autotune_keeps_results():
     b.compute()
     before = [b.results(i) for i in range(len(specs))]
     best = b.autotune(reps=1)
-    assert best["occupancy"] in (2, 3, 4) and best["chunk"] in (32, 64) and best["ms"] > 0
+    assert best["occupancy"] in (0, 2, 3, 4) and best["chunk"] in (16, 32, 64) and best["ms"] > 0
     b.compute()
     for i in range(len(specs)):
         got = b.results(i)

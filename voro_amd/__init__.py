@@ -457,6 +457,10 @@ class container_batch:
         return {"waves": int(a[0]), "occupancy": int(a[1]), "chunk": int(a[2])}
 
     def set_config(self, waves=0, occupancy=2, chunk=64):
+        """occupancy 2..4: one cell per lane at that many waves per SIMD
+        (the default first tier); 0: one cell per wavefront.  waves: the
+        first tier's persistent wavefronts (0 = as many as fit); chunk:
+        cells per queue entry."""
         a = np.array([waves, occupancy, chunk], np.int32)
         _check(lib().vgpu_batch_config(self._h, None, _p(a, C.c_int)), "vgpu_batch_config")
 

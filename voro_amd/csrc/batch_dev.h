@@ -52,11 +52,16 @@ int launch_batch_chunks(hipStream_t s, int ncont, int ch, const int *cfirst, con
 			int *nchunks);
 
 // The cell kernels.  First tier over the chunk queue (order-3 cells at
-// 2/3/4 waves per SIMD); retry tiers (tier 1 wide, 2 huge) over a list.
+// 2/3/4 waves per SIMD); retry tiers (tier 0 order-3, 1 wide, 2 huge) over a
+// list.
 // Block-mask stamps start at *stamp_dev, read on the device.
 int launch_batch_first(bool poly, int occ, int waves, hipStream_t s, const RectView *views, const BatchChunk *chunks,
 		       const int *nchunks, const int *cellblk, int *counters, char *ws, size_t slab,
 		       const unsigned *stamp_dev, const CellOut &out);
+// Wavefront-cell first tier over the chunk queue (cells_wave.hip).
+int launch_batch_wfirst(bool poly, int waves, hipStream_t s, const RectView *views, const BatchChunk *chunks,
+			const int *nchunks, const int *cellblk, int *counters, char *ws, size_t slab,
+			const unsigned *stamp_dev, const CellOut &out);
 int launch_batch_retry(int tier, bool poly, int waves, hipStream_t s, const RectView *views, const int *boff,
 		       int ncont, const int *list, const int *list_count, const int *cellblk, int *counters, char *ws,
 		       size_t slab, const unsigned *stamp_dev, const CellOut &out);

@@ -258,7 +258,10 @@ int launch_batch_retry(int tier, bool poly, int waves, hipStream_t s, const Rect
 #define VGPU_BATCH_RETRY(T, P, L)                                                                                  \
 	hipLaunchKernelGGL((k_batch_retry<T, P, L>), dim3(waves), dim3(64), 0, s, views, boff, ncont, list, list_count, \
 			   cellblk, counters, ws, slab, stamp_dev, out)
-	if (tier == 1) {
+	if (tier == 0) {
+		if (poly) VGPU_BATCH_RETRY(Tier3, true, false);
+		else VGPU_BATCH_RETRY(Tier3, false, false);
+	} else if (tier == 1) {
 		if (poly) VGPU_BATCH_RETRY(TierWide, true, false);
 		else VGPU_BATCH_RETRY(TierWide, false, false);
 	} else {

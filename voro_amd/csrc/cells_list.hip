@@ -165,13 +165,14 @@ k_cells_list(RectView v, int nc, const int *__restrict__ cellblk, const int *__r
 // counter per XCD share.
 template<bool POLY, int OCC>
 __global__ void __launch_bounds__(64, OCC)
-k_wcells_list(RectView v, int nc, const int *__restrict__ cellblk, const int *__restrict__ order,
+k_wcells_list(RectView v_, int nc, const int *__restrict__ cellblk, const int *__restrict__ order,
 	      const int *__restrict__ cnt, const int *__restrict__ boff, const ListEntry *__restrict__ LA,
 	      const ListTag *__restrict__ LB, int *__restrict__ counters, char *__restrict__ ws, int chunk,
 	      int *__restrict__ fallbacks, CellOut out) {
+	typedef const __attribute__((address_space(4))) RectView CView;
 	const int lane = threadIdx.x;
 	WCell c;
-	c.setup(ws + (size_t)blockIdx.x * WLayout::bytes, lane, v.max_len_sq);
+	c.setup(ws + (size_t)blockIdx.x * WLayout::bytes, lane, v_.max_len_sq);
 	const int nshare = 8;
 	const int my = blockIdx.x % nshare;
 	const int share = (nc + nshare - 1) / nshare;
@@ -184,6 +185,10 @@ k_wcells_list(RectView v, int nc, const int *__restrict__ cellblk, const int *__
 			base = __builtin_amdgcn_readfirstlane(base);
 			if (lo + base >= hi) break;
 			const int end = min(hi, lo + base + chunk);
+			// the view read through the argument segment, as in k_wcells
+			int z;
+			asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+			const RectView &v = *(const RectView *)((CView *)__builtin_amdgcn_kernarg_segment_ptr() + z);
 			for (int pos = lo + base; pos < end; pos++) {
 				const int cell = order[pos];
 				const int ijk = cellblk[cell];

@@ -3,6 +3,7 @@
 // the retry list in `out`.
 #include <hip/hip_runtime.h>
 
+#include "batch_dev.h"
 #include "vgpu_cells.h"
 #include "wcell_dev.h"
 
@@ -17,6 +18,62 @@ struct WSink {
 	VG_INL void put(int i, int v) { if (i < (int)WLayoutT<K>::NBCAP) c->put_nb(i, v); }
 };
 
+// One cell of the container with view v, block b of the view's numbering:
+// compute_cell, then the outputs (lane 0 writes the scalars, every lane its
+// neighbour ids) or, if the cell overflowed, an entry in out's retry list.
+template<int K, bool POLY, bool PRD>
+__device__ __forceinline__ void wave_cell(WCellT<K> &c, const RectView &v, int cell, int b, unsigned stamp,
+					  const CellOut &out) {
+	typedef WLayoutT<K> L;
+	const int lane = threadIdx.x;
+	const int s = cell - v.cstart[b];
+	const int ci = b % v.nx, cj = (b / v.nx) % v.gy, ck = b / v.nxy;
+	CellStats stt;
+	const bool ok = compute_cell_rect<WCellT<K>, POLY, PRD>(c, v, b, s, ci, cj, ck, stamp, stt,
+							  (unsigned long long)cell << 24);
+	double vol = 0, cx = 0, cy = 0, cz = 0;
+	int nf = 0;
+	if (!c.ovf && ok) {
+		WSink<K> sk{&c};
+		c.outputs(vol, cx, cy, cz, nf, sk);
+		if (nf > (int)L::NBCAP) c.ovf = true;
+	}
+	if (c.ovf) {
+		if (lane == 0) {
+			const int slot_ = atomicAdd(out.retry_count, 1);
+			out.retry[slot_] = cell;
+			out.status[cell] = ST_OVERFLOW;
+		}
+		return;
+	}
+	if (lane == 0) {
+		out.vol[cell] = vol;
+		out.cent[3 * cell] = cx; out.cent[3 * cell + 1] = cy; out.cent[3 * cell + 2] = cz;
+		out.nfaces[cell] = nf;
+		out.status[cell] = ok ? ST_EXISTS : 0;
+		if (out.st_planes) {
+			out.st_planes[cell] = (short)min(stt.n_planes, 32767);
+			out.st_exit[cell] = (signed char)stt.wl_exit;
+			out.st_tier[cell] = (signed char)out.tier;
+		}
+		if (PRD && out.touch) out.touch[cell] = (unsigned char)stt.img;
+	}
+	if (out.want_nbrs) {
+		int off = 0;
+		if (lane == 0 && nf > 0) off = atomicAdd(out.arena_top, nf);
+		off = __builtin_amdgcn_readfirstlane(off);
+		if (off + nf > out.arena_cap) {
+			if (lane == 0) { out.nbr_start[cell] = -1; atomicAdd(out.fail_count, 1); }
+		} else {
+			if (lane == 0) out.nbr_start[cell] = off;
+			eachk<K>([&](auto hc) __attribute__((always_inline)) {
+				constexpr int h = decltype(hc)::value;
+				if (64 * h + lane < nf) out.nbr[off + 64 * h + lane] = c.nb_lane(h);
+			});
+		}
+	}
+}
+
 // Persistent kernel, one wavefront per workgroup, over every cell or over the
 // device list of cells whose count is at *list_count, `chunk` cells taken per
 // counter increment.  The cells are split into
@@ -25,13 +82,14 @@ struct WSink {
 // through its L2; a wavefront whose share is exhausted moves on to the next.
 template<int K, bool POLY, int OCC, bool PRD>
 __global__ void __launch_bounds__(64, OCC)
-k_wcells(RectView v, int ncells, const int *__restrict__ list, const int *__restrict__ list_count,
+k_wcells(RectView v_, int ncells, const int *__restrict__ list, const int *__restrict__ list_count,
 	 const int *__restrict__ cellblk, int *__restrict__ counters, char *__restrict__ ws, unsigned stamp_base,
 	 int chunk, CellOut out) {
+	typedef const __attribute__((address_space(4))) RectView CView;
 	const int lane = threadIdx.x;
 	typedef WLayoutT<K> L;
 	WCellT<K> c;
-	c.setup(ws + (size_t)blockIdx.x * L::bytes, lane, v.max_len_sq);
+	c.setup(ws + (size_t)blockIdx.x * L::bytes, lane, v_.max_len_sq);
 	const int total = list_count ? *list_count : ncells;
 	const int nshare = 8;
 	const int my = blockIdx.x % nshare;
@@ -45,56 +103,62 @@ k_wcells(RectView v, int ncells, const int *__restrict__ list, const int *__rest
 			base = __builtin_amdgcn_readfirstlane(base);
 			if (lo + base >= hi) break;
 			const int end = min(hi, lo + base + chunk);
+			// The view (the first kernel argument, at offset 0 of the
+			// argument segment) is read through a pointer the compiler
+			// cannot hoist out of this loop: its fields are then reloaded
+			// where needed instead of being held, and spilled, across the
+			// candidate loop (scratch 124 -> 16 bytes per lane).
+			int z;
+			asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+			const RectView &v = *(const RectView *)((CView *)__builtin_amdgcn_kernarg_segment_ptr() + z);
 			for (int idx = lo + base; idx < end; idx++) {
 				const int cell = list ? list[idx] : idx;
-				const int b = cellblk[cell];
-				const int s = cell - v.cstart[b];
-				const int ci = b % v.nx, cj = (b / v.nx) % v.gy, ck = b / v.nxy;
-				CellStats stt;
-				const bool ok = compute_cell_rect<WCellT<K>, POLY, PRD>(c, v, b, s, ci, cj, ck,
-										  stamp_base + (unsigned)cell + 1u, stt,
-										  (unsigned long long)cell << 24);
-				double vol = 0, cx = 0, cy = 0, cz = 0;
-				int nf = 0;
-				if (!c.ovf && ok) {
-					WSink<K> sk{&c};
-					c.outputs(vol, cx, cy, cz, nf, sk);
-					if (nf > (int)L::NBCAP) c.ovf = true;
-				}
-				if (c.ovf) {
-					if (lane == 0) {
-						const int slot_ = atomicAdd(out.retry_count, 1);
-						out.retry[slot_] = cell;
-						out.status[cell] = ST_OVERFLOW;
-					}
-					continue;
-				}
-				if (lane == 0) {
-					out.vol[cell] = vol;
-					out.cent[3 * cell] = cx; out.cent[3 * cell + 1] = cy; out.cent[3 * cell + 2] = cz;
-					out.nfaces[cell] = nf;
-					out.status[cell] = ok ? ST_EXISTS : 0;
-					if (out.st_planes) {
-						out.st_planes[cell] = (short)min(stt.n_planes, 32767);
-						out.st_exit[cell] = (signed char)stt.wl_exit;
-						out.st_tier[cell] = (signed char)out.tier;
-					}
-					if (PRD && out.touch) out.touch[cell] = (unsigned char)stt.img;
-				}
-				if (out.want_nbrs) {
-					int off = 0;
-					if (lane == 0 && nf > 0) off = atomicAdd(out.arena_top, nf);
-					off = __builtin_amdgcn_readfirstlane(off);
-					if (off + nf > out.arena_cap) {
-						if (lane == 0) { out.nbr_start[cell] = -1; atomicAdd(out.fail_count, 1); }
-					} else {
-						if (lane == 0) out.nbr_start[cell] = off;
-						eachk<K>([&](auto hc) __attribute__((always_inline)) {
-							constexpr int h = decltype(hc)::value;
-							if (64 * h + lane < nf) out.nbr[off + 64 * h + lane] = c.nb_lane(h);
-						});
-					}
-				}
+				wave_cell<K, POLY, PRD>(c, v, cell, cellblk[cell], stamp_base + (unsigned)cell + 1u, out);
+			}
+		}
+	}
+}
+
+// First tier of a batch of containers over its chunk queue (each entry up to
+// 64 consecutive cells of one container): a wavefront takes one entry at a
+// time and computes its cells one after another as wavefront cells (up to 64
+// vertices).  The queue is split into one contiguous share per XCD, as in
+// k_wcells; cells it does not finish go to out's retry list.
+template<bool POLY>
+__global__ void __launch_bounds__(64, 4)
+k_batch_wfirst(const RectView *__restrict__ views, const BatchChunk *__restrict__ chunks,
+	       const int *__restrict__ nchunks, const int *__restrict__ cellblk, int *__restrict__ counters,
+	       char *__restrict__ ws, size_t slab_bytes, const unsigned *__restrict__ stamp_dev, CellOut out) {
+	const int lane = threadIdx.x;
+	WCell c;
+	c.setup(ws + (size_t)blockIdx.x * slab_bytes, lane, 0.0);
+	const unsigned stamp_base = *stamp_dev;
+	const int total = *nchunks;
+	const int nshare = 8;
+	const int my = blockIdx.x % nshare;
+	const int share = (total + nshare - 1) / nshare;
+	for (int h = 0; h < nshare; h++) {
+		const int sh = (my + h) % nshare;
+		const int lo = sh * share, hi = min(total, lo + share);
+		for (;;) {
+			int k = 0;
+			if (lane == 0) k = atomicAdd(&counters[sh], 1);
+			k = __builtin_amdgcn_readfirstlane(k);
+			if (lo + k >= hi) break;
+			const int cont = __builtin_amdgcn_readfirstlane(chunks[lo + k].cont);
+			const int first = __builtin_amdgcn_readfirstlane(chunks[lo + k].first);
+			const int count = __builtin_amdgcn_readfirstlane(chunks[lo + k].count);
+			// The view is read through the constant address space: its fields
+			// are then reloaded where needed instead of being held (and
+			// spilled) across the cell loop (measured: 16 instead of 472 bytes
+			// of scratch per lane).
+			typedef const __attribute__((address_space(4))) RectView CView;
+			const RectView &v = *(const RectView *)((CView *)views + cont);
+			c.set_tol(v.max_len_sq);
+			for (int t = 0; t < count; t++) {
+				const int cell = first + t;
+				wave_cell<1, POLY, false>(c, v, cell, cellblk[cell] - v.boff, stamp_base + (unsigned)cell + 1u,
+							  out);
 			}
 		}
 	}
@@ -143,6 +207,18 @@ int launch_cells_wave(int k, bool poly, bool prd, int occ, int waves, hipStream_
 		if (poly) VGPU_WAVE_LAUNCH(1, true, 4, false);
 		else VGPU_WAVE_LAUNCH(1, false, 4, false);
 	}
+	return hipGetLastError() == hipSuccess ? 0 : 6;
+}
+
+int launch_batch_wfirst(bool poly, int waves, hipStream_t s, const RectView *views, const BatchChunk *chunks,
+			const int *nchunks, const int *cellblk, int *counters, char *ws, size_t slab,
+			const unsigned *stamp_dev, const CellOut &out) {
+	if (poly)
+		hipLaunchKernelGGL((k_batch_wfirst<true>), dim3(waves), dim3(64), 0, s, views, chunks, nchunks, cellblk,
+				   counters, ws, slab, stamp_dev, out);
+	else
+		hipLaunchKernelGGL((k_batch_wfirst<false>), dim3(waves), dim3(64), 0, s, views, chunks, nchunks, cellblk,
+				   counters, ws, slab, stamp_dev, out);
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 

@@ -237,6 +237,10 @@ int launch_tier(vgpu_container *h, Workspace &ws, int waves, const RectView &v, 
 	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
 }
 
+// Persistent wavefronts of a batch's order-3 per-lane tier when it takes
+// the wavefront-cell tier's leftovers (few cells).
+const int BATCH_LEFTOVER_WAVES = 512;
+
 int default_fast_waves(int occ) {
 	int dev = 0, cus = 256;
 	if (hipGetDevice(&dev) == hipSuccess) {
@@ -685,11 +689,14 @@ struct vgpu_batch {
 	DBuf views, geoms, boff, pcont, pfirst, mrad, walls, d_wl, maxr, cfirst, nchunk, coff, chunks, nchunks, stamp;
 	DBuf in_ids, in_xyz, in_r;
 	DBuf blk, slot, rem, cnt, bstart, tmp, part, P, pid, src, cellblk;
-	DBuf vol, cent, nfaces, status, nbr_start, arena, csr_off, csr, ctr, counters, retry1, retry2;
+	DBuf vol, cent, nfaces, status, nbr_start, arena, csr_off, csr, ctr, counters, retry1, retry2, retry3;
 	DBuf st_planes, st_exit, st_tier;
-	Workspace ws_fast, ws_wide, ws_huge;
+	Workspace ws_fast, ws_wide, ws_huge, ws_wave;
 	long long acap = 0;
-	int waves = 0, occ = 2, chunk = 64;  // launch configuration (autotuned)
+	// launch configuration (autotuned): persistent wavefronts of the first
+	// tier, its kind (0 wavefront cells, 2..4 one cell per lane at that many
+	// waves per SIMD) and the cells per queue entry
+	int waves = 0, occ = 2, chunk = 64;
 	unsigned long long stamp_used = 0;   // stamps handed out since the workspaces were cleared
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
@@ -746,26 +753,29 @@ int batch_prepare(vgpu_batch *b, unsigned flags) {
 	    (st = b->nfaces.ensure(N1 * 4)) || (st = b->status.ensure(N1 * 4)) || (st = b->nbr_start.ensure(N1 * 4)) ||
 	    (st = b->arena.ensure((size_t)b->acap * 4)) || (st = b->csr_off.ensure((N1 + 1) * 4)) ||
 	    (st = b->csr.ensure((size_t)b->acap * 4)) || (st = b->ctr.ensure(16 * 4)) || (st = b->counters.ensure(64 * 4)) ||
-	    (st = b->retry1.ensure(N1 * 4)) || (st = b->retry2.ensure(N1 * 4)))
+	    (st = b->retry1.ensure(N1 * 4)) || (st = b->retry2.ensure(N1 * 4)) || (st = b->retry3.ensure(N1 * 4)))
 		return st;
 	const size_t pb = std::max(std::max(scan_part_bytes((int)NB1), scan_part_bytes((int)C1)), scan_part_bytes((int)N1 + 1));
 	if ((st = b->part.ensure(pb))) return st;
 	if (stats && ((st = b->st_planes.ensure(N1 * 2)) || (st = b->st_exit.ensure(N1)) || (st = b->st_tier.ensure(N1))))
 		return st;
-	int w = b->waves > 0 ? b->waves : default_fast_waves(b->occ);
-	w = std::max(1, std::min(w, nblk((long long)N1, 64) + b->ncont));
+	const bool wave = b->occ == 0;
+	int w = b->waves > 0 ? b->waves : default_fast_waves(wave ? 4 : b->occ);
+	w = std::max(1, std::min(w, (wave ? nblk((long long)N1, b->chunk) : nblk((long long)N1, 64)) + b->ncont));
 	b->waves = w;
 	typedef Layout<TierWide, 64> LW;
 	typedef Layout<TierHuge, 64> LH;
-	if ((st = b->ws_fast.ensure(VCell3::L::bytes, w)) || (st = b->ws_wide.ensure(LW::bytes, 64)) ||
-	    (st = b->ws_huge.ensure(LH::bytes, 4)))
+	if ((wave && (st = b->ws_wave.ensure(WLayout::bytes, w))) ||
+	    (st = b->ws_fast.ensure(VCell3::L::bytes, wave ? BATCH_LEFTOVER_WAVES : w)) ||
+	    (st = b->ws_wide.ensure(LW::bytes, 64)) || (st = b->ws_huge.ensure(LH::bytes, 4)))
 		return st;
 	// stamps: three passes of n+1 per compute; clear the workspaces before
 	// the 32-bit stamp space runs out
 	const unsigned long long per = 3ull * ((unsigned long long)N1 + 1) + 3;
 	if (b->stamp_used + per >= 0xffffffffull || b->stamp_used == 0) {
-		Workspace *wss[3] = {&b->ws_fast, &b->ws_wide, &b->ws_huge};
-		for (Workspace *x : wss) HIPCHK(hipMemset(x->buf.p, 0, x->buf.n));
+		Workspace *wss[4] = {&b->ws_fast, &b->ws_wide, &b->ws_huge, &b->ws_wave};
+		for (Workspace *x : wss)
+			if (x->buf.p) HIPCHK(hipMemset(x->buf.p, 0, x->buf.n));
 		HIPCHK(hipMemset(b->stamp.p, 0, 16 * 4));
 		b->stamp_used = 1;
 		b->drop_graph();   // the graph's stamps restart from zero with the workspace
@@ -821,16 +831,31 @@ int batch_enqueue(vgpu_batch *b, unsigned flags, hipStream_t s, bool timing) {
 	int *cnt8 = b->counters.as<int>();
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, cnt8, 8, 0);
 	out.tier = 0; out.retry = b->retry1.as<int>(); out.retry_count = ctr + 1;
-	if (launch_batch_first(b->poly, b->occ, b->waves, s, b->views.as<RectView>(), b->chunks.as<BatchChunk>(),
-			       b->nchunks.as<int>(), b->cellblk.as<int>(), cnt8, b->ws_fast.base(), b->ws_fast.slab,
-			       (const unsigned *)(ctr + 8), out))
+	// the wide tier's list: the first tier's leftovers, or (wavefront cells
+	// first) those the order-3 per-lane tier leaves (ctr[5])
+	const int *wide_list = b->retry1.as<int>(), *wide_count = ctr + 1;
+	if (b->occ == 0) {
+		if (launch_batch_wfirst(b->poly, b->waves, s, b->views.as<RectView>(), b->chunks.as<BatchChunk>(),
+					b->nchunks.as<int>(), b->cellblk.as<int>(), cnt8, b->ws_wave.base(), b->ws_wave.slab,
+					(const unsigned *)(ctr + 8), out))
+			return VGPU_HIP_ERROR;
+		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, cnt8, 8, 0);
+		out.tier = 1; out.retry = b->retry3.as<int>(); out.retry_count = ctr + 5;
+		if (launch_batch_retry(0, b->poly, BATCH_LEFTOVER_WAVES, s, b->views.as<RectView>(), b->boff.as<int>(), nc,
+				       b->retry1.as<int>(), ctr + 1, b->cellblk.as<int>(), cnt8, b->ws_fast.base(),
+				       b->ws_fast.slab, (const unsigned *)(ctr + 8), out))
+			return VGPU_HIP_ERROR;
+		wide_list = b->retry3.as<int>(); wide_count = ctr + 5;
+	} else if (launch_batch_first(b->poly, b->occ, b->waves, s, b->views.as<RectView>(), b->chunks.as<BatchChunk>(),
+				      b->nchunks.as<int>(), b->cellblk.as<int>(), cnt8, b->ws_fast.base(), b->ws_fast.slab,
+				      (const unsigned *)(ctr + 8), out))
 		return VGPU_HIP_ERROR;
 	if (timing) HIPCHK(hipEventRecord(b->ev[2], s));
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, cnt8, 8, 0);
 	out.tier = 2; out.retry = b->retry2.as<int>(); out.retry_count = ctr + 2;
-	if (launch_batch_retry(1, b->poly, 64, s, b->views.as<RectView>(), b->boff.as<int>(), nc, b->retry1.as<int>(),
-			       ctr + 1, b->cellblk.as<int>(), cnt8, b->ws_wide.base(), b->ws_wide.slab,
-			       (const unsigned *)(ctr + 9), out))
+	if (launch_batch_retry(1, b->poly, 64, s, b->views.as<RectView>(), b->boff.as<int>(), nc, wide_list, wide_count,
+			       b->cellblk.as<int>(), cnt8, b->ws_wide.base(), b->ws_wide.slab, (const unsigned *)(ctr + 9),
+			       out))
 		return VGPU_HIP_ERROR;
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, cnt8, 8, 0);
 	out.tier = 3; out.retry = nullptr; out.retry_count = nullptr;
@@ -1406,10 +1431,10 @@ int vgpu_batch_destroy(vgpu_batch *b) {
 			&b->cfirst, &b->nchunk, &b->coff, &b->chunks, &b->nchunks, &b->stamp, &b->in_ids, &b->in_xyz,
 			&b->in_r, &b->blk, &b->slot, &b->rem, &b->cnt, &b->bstart, &b->tmp, &b->part, &b->P, &b->pid,
 			&b->src, &b->cellblk, &b->vol, &b->cent, &b->nfaces, &b->status, &b->nbr_start, &b->arena,
-			&b->csr_off, &b->csr, &b->ctr, &b->counters, &b->retry1, &b->retry2, &b->st_planes, &b->st_exit,
-			&b->st_tier};
+			&b->csr_off, &b->csr, &b->ctr, &b->counters, &b->retry1, &b->retry2, &b->retry3, &b->st_planes,
+			&b->st_exit, &b->st_tier};
 	for (DBuf *x : bufs) x->release();
-	b->ws_fast.buf.release(); b->ws_wide.buf.release(); b->ws_huge.buf.release();
+	b->ws_fast.buf.release(); b->ws_wide.buf.release(); b->ws_huge.buf.release(); b->ws_wave.buf.release();
 	if (b->events) for (int t = 0; t < 4; t++) (void)hipEventDestroy(b->ev[t]);
 	delete b;
 	return VGPU_OK;
@@ -1503,7 +1528,8 @@ int vgpu_batch_config(vgpu_batch *b, int *get3, const int *set3) {
 	if (!b) return VGPU_INVALID_ARGUMENT;
 	if (get3) { get3[0] = b->waves; get3[1] = b->occ; get3[2] = b->chunk; }
 	if (set3) {
-		if (set3[0] < 0 || set3[1] < 2 || set3[1] > 4 || set3[2] < 1 || set3[2] > 64) return VGPU_INVALID_ARGUMENT;
+		if (set3[0] < 0 || set3[1] == 1 || set3[1] < 0 || set3[1] > 4 || set3[2] < 1 || set3[2] > 64)
+			return VGPU_INVALID_ARGUMENT;
 		b->waves = set3[0]; b->occ = set3[1]; b->chunk = set3[2];
 		b->drop_graph();
 	}
@@ -1511,31 +1537,32 @@ int vgpu_batch_config(vgpu_batch *b, int *get3, const int *set3) {
 }
 
 // The project's autotuner for batches: times the compute (eager, `reps`
-// runs each, best kept) for every register budget of the first tier (2, 3,
-// 4 waves per SIMD, with as many persistent wavefronts as fit) and chunk
-// length (64 or 32 cells per queue entry), and keeps the fastest.  Results
-// do not depend on the choice.
+// runs each, best kept) for the per-lane first tier at every register budget
+// (2, 3, 4 waves per SIMD) with 64 or 32 cells per queue entry and for the
+// wavefront-cell first tier (16 cells per entry), each with as many
+// persistent wavefronts as fit, and keeps the fastest.  Results do not depend
+// on the choice.  (On FCC-like containers, whose cells are alike, the
+// per-lane tier wins; on uneven cells the wavefront cells.)
 int vgpu_batch_autotune(vgpu_batch *b, unsigned flags, int reps, int *best3, double *best_ms) {
 	if (!b || reps < 1 || (flags & (VGPU_F_LEARNED | VGPU_F_GRAPH))) return VGPU_INVALID_ARGUMENT;
-	const int occs[3] = {2, 3, 4}, chunks[2] = {64, 32};
+	const int cands[7][2] = {{2, 64}, {2, 32}, {3, 64}, {3, 32}, {4, 64}, {4, 32}, {0, 16}};
 	double best = 1e300;
 	int bo = 2, bc = 64;
-	for (int oi = 0; oi < 3; oi++)
-		for (int ci = 0; ci < 2; ci++) {
-			const int cfg[3] = {0, occs[oi], chunks[ci]};
-			int st = vgpu_batch_config(b, nullptr, cfg);
-			if (st) return st;
-			double t = 1e300;
-			for (int r = 0; r < reps; r++) {
-				if ((st = batch_run(b, flags, nullptr))) return st;
-				t = std::min(t, b->timings[0] + b->timings[1] + b->timings[2]);
-			}
-			if (t < best) { best = t; bo = occs[oi]; bc = chunks[ci]; }
+	for (int oi = 0; oi < 7; oi++) {
+		const int cfg[3] = {0, cands[oi][0], cands[oi][1]};
+		int st = vgpu_batch_config(b, nullptr, cfg);
+		if (st) return st;
+		double t = 1e300;
+		for (int r = 0; r < reps; r++) {
+			if ((st = batch_run(b, flags, nullptr))) return st;
+			t = std::min(t, b->timings[0] + b->timings[1] + b->timings[2]);
 		}
+		if (t < best) { best = t; bo = cands[oi][0]; bc = cands[oi][1]; }
+	}
 	const int cfg[3] = {0, bo, bc};
 	int st = vgpu_batch_config(b, nullptr, cfg);
 	if (st) return st;
-	if (best3) { best3[0] = default_fast_waves(bo); best3[1] = bo; best3[2] = bc; }
+	if (best3) { best3[0] = default_fast_waves(bo == 0 ? 4 : bo); best3[1] = bo; best3[2] = bc; }
 	if (best_ms) *best_ms = best;
 	return VGPU_OK;
 }
