@@ -315,17 +315,6 @@ struct WCellT {
 		big_tol = 20.0 * tol;
 	}
 
-	VG_INL bool room_v(int v) { if (v >= MAXV) { ovf = true; return false; } return true; }
-	VG_INL bool push_ds(int v) { if (nds >= MAXS) { ovf = true; return false; } setDS(nds++, v); return true; }
-
-	// A fresh vertex: coordinates, no edges yet, no marks.
-	VG_INL void new_vertex(int v, double x, double y, double z) {
-		setXYZ(v, x, y, z);
-		setTW(v, 0x00ffffffu);
-		delm.unset(v);
-		flipm.unset(v);
-	}
-
 	// Vertex `from` takes index `to` (src/cell.cc:849-861).
 	VG_FN void move_vertex(int from, int to) {
 		setXYZ(to, X(from), Y(from), Z(from));
@@ -335,6 +324,17 @@ struct WCellT {
 		if (DEL(from)) delm.set(to); else delm.unset(to);
 		flipm.unset(to);
 		for (int i = 0; i < 3; i++) setE(twE(w, i), twR(w, i), to);
+	}
+
+	VG_INL bool room_v(int v) { if (v >= MAXV) { ovf = true; return false; } return true; }
+	VG_INL bool push_ds(int v) { if (nds >= MAXS) { ovf = true; return false; } setDS(nds++, v); return true; }
+
+	// A fresh vertex: coordinates, no edges yet, no marks.
+	VG_INL void new_vertex(int v, double x, double y, double z) {
+		setXYZ(v, x, y, z);
+		setTW(v, 0x00ffffffu);
+		delm.unset(v);
+		flipm.unset(v);
 	}
 
 	// Box start cell, faces -1..-6 (src/cell.cc:294-318, 2567-2580); corner v
@@ -708,6 +708,46 @@ struct WCellT {
 				}
 			}
 		}
+		return compact();
+	}
+
+	// Compaction, src/cell.cc:843-862: the deleted slots are refilled from
+	// the end of the vertex list, delete-stack entries taken last first.  The
+	// loop runs on the marks alone and records, per slot, the original vertex
+	// it ends up holding (org) and, per moved vertex, the slot it ends in
+	// (inv); one permutation then gathers the vertex records and renames
+	// every edge through inv -- the state the reference's vertex-by-vertex
+	// moves (each rewriting the back pointers of the moved vertex's
+	// neighbours) leave behind.
+	// (With four register sets the permutation's temporaries do not fit
+	// the register budget: the moves are made one by one.)
+	VG_FN bool compact() {
+		if constexpr (K == 4) {
+			up = 0;
+			while (nds > 0) {
+				--p;
+				while (DEL(p)) {
+					if (p == 0) { fatal = ovf = true; return false; }
+					--p;
+				}
+				up = DS(--nds);
+				if (up < p) move_vertex(p, up);
+				else up = p++;
+			}
+			return true;
+		}
+#if VG_WDEV
+		int org[K], inv[K];
+		eachk<K>([&](auto hc) __attribute__((always_inline)) {
+			constexpr int h = decltype(hc)::value;
+			org[h] = 64 * h + lane;
+			inv[h] = 64 * h + lane;
+		});
+#else
+		int org[MAXV], inv[MAXV];
+		for (int v = 0; v < MAXV; v++) org[v] = inv[v] = v;
+#endif
+		bool moved = false;
 		up = 0;
 		while (nds > 0) {
 			--p;
@@ -716,11 +756,107 @@ struct WCellT {
 				--p;
 			}
 			up = DS(--nds);
-			if (up < p) move_vertex(p, up);
-			else up = p++;
+			if (up < p) {
+				// slot p's vertex moves to slot up
+#if VG_WDEV
+				const int o = rli(pickk<K>(org, p >> 6), p & 63);
+				putk<K>(org, up >> 6, lane == (up & 63), o);
+				putk<K>(inv, o >> 6, lane == (o & 63), up);
+#else
+				const int o = org[p];
+				org[up] = o;
+				inv[o] = up;
+#endif
+				delm.unset(up);
+				flipm.unset(up);
+				moved = true;
+			} else up = p++;
 		}
+		if (moved) permute(org, inv);
 		return true;
 	}
+
+#if VG_WDEV
+	static VG_INL unsigned bperm(unsigned v, int addr) {
+		return (unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)v);
+	}
+	// value of a[] at vertex idx (per lane), 0 <= idx < 256
+	template<class T>
+	static VG_INL unsigned lookup(const T (&a)[K], unsigned idx) {
+		const int addr = (int)((idx & 63u) << 2);
+		const unsigned hs = idx >> 6;
+		unsigned r = 0;
+		eachk<K>([&](auto hc) __attribute__((always_inline)) {
+			constexpr int q = decltype(hc)::value;
+			const unsigned g = bperm((unsigned)a[q], addr);
+			r = hs == (unsigned)q ? g : r;
+		});
+		return r;
+	}
+	template<class T>
+	static VG_INL void gather32(T (&a)[K], const int (&src)[K]) {
+		T o[K];
+		eachk<K>([&](auto hc) __attribute__((always_inline)) {
+			constexpr int h = decltype(hc)::value;
+			o[h] = (T)lookup<T>(a, (unsigned)src[h]);
+		});
+		eachk<K>([&](auto hc) __attribute__((always_inline)) {
+			constexpr int h = decltype(hc)::value;
+			a[h] = o[h];
+		});
+	}
+	static VG_INL void gather64(double (&a)[K], const int (&src)[K]) {
+		unsigned lo[K], hi[K];
+		eachk<K>([&](auto hc) __attribute__((always_inline)) {
+			constexpr int h = decltype(hc)::value;
+			const unsigned long long b = __builtin_bit_cast(unsigned long long, a[h]);
+			lo[h] = (unsigned)b;
+			hi[h] = (unsigned)(b >> 32);
+		});
+		gather32(lo, src);
+		gather32(hi, src);
+		eachk<K>([&](auto hc) __attribute__((always_inline)) {
+			constexpr int h = decltype(hc)::value;
+			a[h] = __builtin_bit_cast(double, ((unsigned long long)hi[h] << 32) | lo[h]);
+		});
+	}
+	VG_FN void permute(const int (&org)[K], const int (&inv)[K]) {
+		gather64(vx, org); gather64(vy, org); gather64(vz, org);
+		gather32(vt, org); gather32(vna, org); gather32(vnb, org); gather32(vnc, org);
+		eachk<K>([&](auto hc) __attribute__((always_inline)) {
+			constexpr int h = decltype(hc)::value;
+			// every lane takes part in every lookup (a lane outside the
+			// exec mask would read as zero to the others)
+			const unsigned w = vt[h];
+			const unsigned e0 = w & 0xffu, e1 = (w >> 8) & 0xffu, e2 = (w >> 16) & 0xffu;
+			const unsigned l0 = lookup<int>(inv, e0), l1 = lookup<int>(inv, e1), l2 = lookup<int>(inv, e2);
+			const unsigned n0 = e0 == 0xffu ? 0xffu : l0 & 0xffu;
+			const unsigned n1 = e1 == 0xffu ? 0xffu : l1 & 0xffu;
+			const unsigned n2 = e2 == 0xffu ? 0xffu : l2 & 0xffu;
+			vt[h] = (w & 0xff000000u) | n0 | n1 << 8 | n2 << 16;
+		});
+	}
+#else
+	VG_FN void permute(const int *org, const int *inv) {
+		double ax[MAXV], ay[MAXV], az[MAXV];
+		unsigned at[MAXV];
+		int an[MAXV][3];
+		for (int v = 0; v < MAXV; v++) {
+			const int o = org[v];
+			ax[v] = vx[o]; ay[v] = vy[o]; az[v] = vz[o]; at[v] = vt[o];
+			an[v][0] = vn[o][0]; an[v][1] = vn[o][1]; an[v][2] = vn[o][2];
+		}
+		for (int v = 0; v < MAXV; v++) {
+			unsigned w = at[v];
+			for (int i = 0; i < 3; i++) {
+				const int e = (int)((w >> (8 * i)) & 0xffu);
+				if (e != 0xff && e < MAXV) w = twSetE(w, i, inv[e]);
+			}
+			vx[v] = ax[v]; vy[v] = ay[v]; vz[v] = az[v]; vt[v] = w;
+			vn[v][0] = an[v][0]; vn[v][1] = an[v][1]; vn[v][2] = an[v][2];
+		}
+	}
+#endif
 
 	// src/cell.cc:1907-1917: the largest squared vertex distance (maxima are
 	// exact, so the order of the comparisons does not matter).
