@@ -90,6 +90,8 @@ struct RadPolicy {
 	}
 	VG_INL double cutoff(double lrs) const { return POLY ? lrs * r_val : lrs; }
 	VG_INL double scale(double rs, double rj) const { return POLY ? rs + r_rad - rj * rj : rs; }
+	// the value scale_check leaves in rs
+	VG_INL double scale_checked(double rs, double rj) const { return POLY ? rs + (r_rad - rj * rj) : rs; }
 	VG_INL bool scale_check(double &rs, double mrs, double rj) const {
 		if (!POLY) return rs < mrs;
 		double trs = rs;
@@ -296,6 +298,20 @@ VG_INL int prd_region(const RectView &v, int ci, int cj, int ck, int ei, int ej,
 	return b;
 }
 
+// Cell types that evaluate worklist steps one per lane (wavefront cells on
+// the device).
+template<class C, class = void> struct LaneSteps { static constexpr bool value = false; };
+template<class C> struct LaneSteps<C, std::void_t<decltype(C::lane_parallel)>> {
+	static constexpr bool value = C::lane_parallel;
+};
+#if VG_WDEV
+VG_INL unsigned long long steps_ballot(bool b) { return __ballot(b); }
+VG_INL int steps_lane() { return (int)threadIdx.x; }
+#else
+VG_INL unsigned long long steps_ballot(bool) { return 0; }
+VG_INL int steps_lane() { return 0; }
+#endif
+
 // Candidate data of compute_cell_rect: the worklist entries and radii, the
 // particle range of a worklist block, and the particles themselves, each
 // loaded where it is used.  (Staging them in the lanes of a wavefront cell,
@@ -383,6 +399,16 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 	// candidates; see the skip before the nplane call.
 	double mrs_b = 1.7976931348623157e308;
 	int since = 0;
+	// Lane-parallel worklist steps (wavefront cells): the steps of one radius
+	// interval, taken from step gbase, whose blocks are still to be cut
+	// (pend) and need the per-particle check (bmask), bit l = step gbase + l;
+	// gstop >= 0: the radius test stops the cell at that step once they are.
+	unsigned long long pend = 0, bmask = 0;
+	int gbase = 0, gstop = -1;
+	// Lane-parallel particle checks: particles cbase + l of the current block
+	// that are candidates (bit l of cm), up to 64 evaluated at a time.
+	unsigned long long cm = 0;
+	int cbase = 0;
 
 	for (;;) {
 		double px_ = 0, py_ = 0, pz_ = 0, rs = 0;
@@ -390,6 +416,40 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 		int pid = 0;
 		bool have = false;
 		while (!have) {
+			if constexpr (LaneSteps<CELL>::value) {
+				if (cm) {
+					// the next candidate of the block, in storage order
+					const int idx = cbase + __builtin_ctzll(cm);
+					cm &= cm - 1;
+					dbl4 o;
+					cf.part(v, idx, o, pid);
+					px_ = o.x - x2; py_ = o.y - y2; pz_ = o.z - z2;
+					rs_raw = px_ * px_ + py_ * py_ + pz_ * pz_;
+					rs = bcheck ? rp.scale_checked(rs_raw, o.w) : rp.scale(rs_raw, o.w);
+					have = true;
+					continue;
+				}
+				if (bt < be) {
+					// Up to 64 particles of the block at once, one per lane,
+					// each with the per-particle test of src/v_compute.cc:402-428
+					// (mrs is constant within a block).
+					const int cnt = be - bt < 64 ? be - bt : 64;
+					const int lane = steps_lane();
+					bool pass = false;
+					if (lane < cnt) {
+						dbl4 o;
+						int pl;
+						cf.part(v, bt + lane, o, pl);
+						const double ax = o.x - x2, ay = o.y - y2, az = o.z - z2;
+						double r2 = ax * ax + ay * ay + az * az;
+						pass = !bcheck || rp.scale_check(r2, mrs, o.w);
+					}
+					cm = steps_ballot(pass);
+					cbase = bt;
+					bt += cnt;
+					continue;
+				}
+			}
 			if (bt < be) {
 				// Next particle of the current block (src/v_compute.cc:402-428).
 				dbl4 o;
@@ -457,6 +517,82 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				cf.wl_begin(v, radp, e, i, j, k, m1, m2, ci, cj, ck, disp);
 				f = cf.ent(e, 0); g = 0;
 				stage = 2;
+				continue;
+			}
+			if constexpr (LaneSteps<CELL>::value) if (stage == 2) {
+				if (pend) {
+					// the next block of the interval, in worklist order
+					const int l = __builtin_ctzll(pend);
+					pend &= pend - 1;
+					const int g1 = gbase + l + 1;
+					q = cf.ent(e, g1); q ^= m1; q += m2;
+					ei = (int)(q & 127) - 64 + i;
+					ej = (int)((q >> 7) & 127) - 64 + j;
+					ek = (int)((q >> 14) & 127) - 64 + k;
+					const int rijk = PRD ? prd_region(v, ci, cj, ck, ei, ej, ek, qx, key++, stt.img)
+							     : rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
+					cf.range(v, g1, rijk, bt, be);
+					cf.block(bt, be);
+					x2 = x - qx; y2 = y - qy; z2 = z - qz;
+					bcheck = ((bmask >> l) & 1ull) != 0;
+					continue;
+				}
+				if (gstop >= 0) { stt.wl_exit = gstop; stt.n_planes = c.n_cuts; return true; }
+				if (g >= 63) {
+					if (rp.ctest(cf.rad(radp, g), mrs)) { stt.wl_exit = g; stt.n_planes = c.n_cuts; return true; }
+					stt.wl_exit = 64;
+					stt.phase = 3;
+					stage = 4;
+					continue;
+				}
+				// The top of step g (phase-2 start, radius refresh), then every
+				// step up to the next refresh at once: mrs is constant there, so
+				// each step's radius stop, block bounds and block skip
+				// (src/v_compute.cc:371-431,443-526) are decided in its own lane
+				// with the expressions of the sequential loop; the blocks are
+				// then cut in worklist order.
+				if (!marking && g >= f && g > 0) { marking = true; stt.phase = 2; g2_first = g + 1; }
+				if (g == next_count) {
+					mrs = c.max_radius_squared();
+					mrs_b = mrs; since = 0;
+					if (cpi != 8) next_count = count_list(cpi++);
+				}
+				const int gE = (next_count > g && next_count < 63) ? next_count : 63;
+				const int n = gE - g;
+				const int lane = steps_lane();
+				const int gg = g + lane;
+				bool stop = false, acc = false, bc = false, mk = false;
+				if (lane < n) {
+					stop = rp.ctest(radp[gg], mrs);
+					unsigned qq = e[gg + 1];
+					qq ^= m1; qq += m2;
+					const int ddi = (int)(qq & 127) - 64, ddj = (int)((qq >> 7) & 127) - 64,
+						  ddk = (int)((qq >> 14) & 127) - 64;
+					const int eei = ddi + i, eej = ddj + j, eek = ddk + k;
+					if (eei >= 0 && eei < v.hx && eej >= 0 && eej < v.hy && eek >= 0 && eek < v.hz) {
+						double crs_l = 0;
+						if (!min_max_radius<POLY>(v, rp, ddi, ddj, ddk, fx, fy, fz, gxs, gys, gzs, crs_l, mrs)) {
+							acc = true;
+							bc = rp.ctest(crs_l, mrs);
+						}
+					}
+					mk = marking || (gg >= f && gg > 0);
+				}
+				const unsigned long long sm = steps_ballot(stop);
+				const int s0 = sm ? __builtin_ctzll(sm) : n;
+				const unsigned long long below = (1ull << s0) - 1;   // s0 <= 62
+				pend = steps_ballot(acc) & below;
+				bmask = steps_ballot(bc) & below;
+				p2_cut |= (steps_ballot(acc && mk) & below) << (g + 1);
+				// phase 2 starts at the first later step whose top is reached
+				const int last = s0 < n ? g + s0 : g + n - 1;
+				if (!marking) {
+					const int gm = f > g + 1 ? f : g + 1;
+					if (gm <= last) { marking = true; stt.phase = 2; g2_first = gm + 1; }
+				}
+				gbase = g;
+				if (s0 < n) gstop = g + s0;
+				g += s0 < n ? s0 : n;
 				continue;
 			}
 			if (stage == 2) {

@@ -139,6 +139,8 @@ struct WCellT {
 	static_assert(K == 1 || K == 2 || K == 4, "one, two or four vertices per lane");
 	typedef WLayoutT<K> L;
 	enum { MAXV = L::MAXV, MAXS = L::MAXS };
+	// the candidate loop may evaluate worklist steps one per lane
+	static constexpr bool lane_parallel = VG_WDEV != 0;
 
 #if VG_WDEV
 	// register set h holds vertex 64 h + `lane`
