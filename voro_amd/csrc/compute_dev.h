@@ -755,6 +755,10 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, int ci, 
 	// upper bound on max_radius_squared(), as in compute_cell_rect
 	double mrs_b = 1.7976931348623157e308;
 	int since = 0;
+	// lane-parallel particle checks (as in compute_cell_rect): entries
+	// cbase + l of a checked block that are candidates, up to cend
+	unsigned long long cm = 0;
+	int cbase = 0, cend = 0;
 	for (;;) {
 		double px_ = 0, py_ = 0, pz_ = 0, rs = 0;
 		double rs_raw = -1;
@@ -772,6 +776,32 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, int ci, 
 					have = true;
 				}
 				continue;
+			}
+			if constexpr (LaneSteps<CELL>::value) {
+				if (cm) {
+					const int idx = cbase + __builtin_ctzll(cm);
+					cm &= cm - 1;
+					const ListEntry a = LA[(size_t)idx * stride];
+					px_ = a.x; py_ = a.y; pz_ = a.z; pid = LB[(size_t)idx * stride].pid;
+					rs_raw = px_ * px_ + py_ * py_ + pz_ * pz_;
+					rs = rp.scale_checked(rs_raw, a.w);
+					have = true;
+					continue;
+				}
+				if (k < cend) {
+					const int cnt = cend - k < 64 ? cend - k : 64;
+					const int lane = steps_lane();
+					bool pass = false;
+					if (lane < cnt) {
+						const ListEntry a = LA[(size_t)(k + lane) * stride];
+						double r2 = a.x * a.x + a.y * a.y + a.z * a.z;
+						pass = rp.scale_check(r2, mrs, a.w);
+					}
+					cm = steps_ballot(pass);
+					cbase = k;
+					k += cnt;
+					continue;
+				}
 			}
 			if (k >= nent) { stt.n_planes = c.n_cuts; return -1; }
 			const ListEntry a = LA[(size_t)k * stride];
@@ -815,6 +845,10 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, int ci, 
 			bcheck = rp.ctest(a.z, mrs);
 			left = cnt;
 			if (cnt == 0) bcheck = false;
+			if constexpr (LaneSteps<CELL>::value) {
+				// a checked block's particles go through the lane checks
+				if (bcheck) { cend = k + cnt < nent ? k + cnt : nent; left = 0; bcheck = false; }
+			}
 		}
 		if (rs_raw >= 0) {
 			// the exact skip of a plane that provably misses the cell (see
