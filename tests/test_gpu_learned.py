@@ -100,6 +100,22 @@ def test_gpu_learned_per_lane_lists_are_exact():
     _same(plain, con.results())
 
 
+def test_gpu_learned_dense_blocks_are_exact():
+    """About 250 particles per block: a checked block's list entries are
+    radius-checked 64 at a time in several rounds; the results equal the
+    plain path's."""
+    n, L, G = 32000, 20.0, 5
+    make = lambda sd: _uniform(sd, n, L, G)
+    pred, _, _ = _train(make, [41], epochs=100)
+    con = make(42)
+    con.compute_all_cells(F)
+    plain = dict(con.results())
+    con.set_predictor(pred)
+    con.compute_all_cells(F | voro_amd.F_LEARNED)
+    con._res = None
+    _same(plain, con.results())
+
+
 def test_gpu_learned_forced_fallback_is_exact():
     n, L, G = 20000, 27.1, 16
     make = lambda sd: _uniform(sd, n, L, G)

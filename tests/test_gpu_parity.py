@@ -48,6 +48,12 @@ CASES = {
                                I.uniform_box(6, 6000, (-3, -3, -3), (23, 23, 23)), None, (1, 0, 1), None),
     "dropped": lambda: (0, (0, 20, 0, 20, 0, 20), (10, 10, 10), I.uniform_box(7, 6000, (-2, -2, -2), (22, 22, 22)),
                         None, (0, 0, 0), None),
+    # ~1,000 particles per block: the wavefront cells check a block's
+    # particles 64 at a time, so these take many rounds per block
+    "dense_blocks": lambda: (0, (0, 10, 0, 10, 0, 10), (2, 2, 2), I.uniform_box(8, 8000, (0, 0, 0), (10, 10, 10)),
+                             None, (0, 0, 0), None),
+    "dense_poly": lambda: (1, (0, 10, 0, 10, 0, 10), (3, 3, 3), I.uniform_box(9, 8000, (0, 0, 0), (10, 10, 10)),
+                           I.radii_lognormal(9, 8000) * 0.5, (0, 0, 0), None),
 }
 
 
