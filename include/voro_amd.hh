@@ -111,6 +111,16 @@ class container : public container_base_gpu {
 		check(vgpu_put(h, 1, &n, p, nullptr), "put");
 		dirty = true;
 	}
+	// find_voronoi_cell, src/container.cc:237-297 (one query; batches of
+	// queries: vgpu_find_voronoi_cells)
+	bool find_voronoi_cell(double x, double y, double z, double &rx, double &ry, double &rz, int &pid) {
+		double xyz[3] = {x, y, z}, r[3];
+		int found = 0, id_ = 0;
+		check(vgpu_find_voronoi_cells(h, 1, xyz, &found, &id_, r, 0), "find_voronoi_cell");
+		if (!found) return false;
+		rx = r[0]; ry = r[1]; rz = r[2]; pid = id_;
+		return true;
+	}
 
     private:
 	struct geom6 { double v[6]; geom6(double a, double b, double c, double d, double e, double f) : v{a, b, c, d, e, f} {} };
@@ -129,6 +139,16 @@ class container_poly : public container_base_gpu {
 		double p[3] = {x, y, z};
 		check(vgpu_put(h, 1, &n, p, &r), "put");
 		dirty = true;
+	}
+	// find_voronoi_cell, src/container.cc:237-297 (one query; batches of
+	// queries: vgpu_find_voronoi_cells)
+	bool find_voronoi_cell(double x, double y, double z, double &rx, double &ry, double &rz, int &pid) {
+		double xyz[3] = {x, y, z}, r[3];
+		int found = 0, id_ = 0;
+		check(vgpu_find_voronoi_cells(h, 1, xyz, &found, &id_, r, 0), "find_voronoi_cell");
+		if (!found) return false;
+		rx = r[0]; ry = r[1]; rz = r[2]; pid = id_;
+		return true;
 	}
 
     private:

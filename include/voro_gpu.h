@@ -81,6 +81,16 @@ int vgpu_clear(vgpu_container *h);
  * results are complete. */
 int vgpu_compute_all(vgpu_container *h, unsigned flags, void *stream);
 
+/* container::find_voronoi_cell / container_poly::find_voronoi_cell
+ * (src/container.cc:237-297) for n query points at once: found[i] = 1 with
+ * pid[i] and pos[3i..3i+2] (the particle's position, shifted by whole
+ * periods along periodic directions) for the particle whose Voronoi cell
+ * contains point i, or 0 (point outside a non-periodic direction, or no
+ * particle).  Rectangular containers only (VGPU_INVALID_ARGUMENT for the
+ * periodic classes).  Bins the particles if they changed since the last
+ * compute.  on_device: the four arrays are device pointers. */
+int vgpu_find_voronoi_cells(vgpu_container *h, int n, const double *xyz, int *found, int *pid, double *pos,
+			    int on_device);
 /* Number of particles stored (the c_loop_all length). */
 int vgpu_num_cells(vgpu_container *h, long long *n);
 

@@ -174,6 +174,32 @@ def time_reference(spec, ids, xyz, r=None, loop="compute_all_cells"):
               None if rr is None else _ptr(rr, C.c_double))
 
 
+def find_reference(spec, ids, xyz, r, q, timed=False):
+    """find_voronoi_cell of the reference library for every row of q:
+    (found, pid, pos[, seconds of the query loop]); None where the library is
+    unavailable."""
+    lib = _load(REF_SO, "ref_compute_neighbor") and _LIBS[REF_SO][0]
+    if not lib:
+        return None
+    fn = lib.ref_find_voronoi_cells
+    fn.restype = C.c_longlong
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+    found = np.zeros(len(q), np.int32)
+    pid = np.zeros(len(q), np.int32)
+    pos = np.zeros((len(q), 3))
+    secs = C.c_double(0)
+    m = fn(C.byref(spec), C.c_int(len(ids)), _ptr(ids, C.c_int), _ptr(xyz, C.c_double),
+           None if rr is None else _ptr(rr, C.c_double), C.c_int(len(q)), _ptr(q, C.c_double),
+           _ptr(found, C.c_int), _ptr(pid, C.c_int), _ptr(pos, C.c_double), C.byref(secs))
+    if m < 0:
+        raise RuntimeError("reference find_voronoi_cell: rectangular containers only")
+    out = (found.astype(bool), pid, pos)
+    return out + (secs.value,) if timed else out
+
+
 def compare(a, b, vol_rtol=1e-12):
     """Returns a list of human-readable mismatches between two result dicts
     (empty when identical to the stated tolerance)."""

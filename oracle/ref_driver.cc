@@ -206,6 +206,42 @@ double ref_time_compute_all_cells(const ref_spec *s, int n, const int *ids, cons
 	return secs;
 }
 
+// container(_poly)::find_voronoi_cell (src/container.cc:237-297) for nq
+// query points; found[i] is its return value.  With `secs`, the time of the
+// query loop alone.  Rectangular kinds only (returns -1 otherwise).
+long long ref_find_voronoi_cells(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r,
+				 int nq, const double *q, int *found, int *pid, double *pos, double *secs) {
+	const double *g = s->geom;
+	if (s->kind > 1) return -1;
+	long long nf = 0;
+	std::chrono::steady_clock::time_point t0, t1;
+	if (s->kind == 0) {
+		container con(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
+			      s->periodic[0] != 0, s->periodic[1] != 0, s->periodic[2] != 0, s->init_mem);
+		put_all(con, n, ids, xyz, r);
+		t0 = std::chrono::steady_clock::now();
+		for (int i = 0; i < nq; i++) {
+			found[i] = con.find_voronoi_cell(q[3 * i], q[3 * i + 1], q[3 * i + 2], pos[3 * i], pos[3 * i + 1],
+							 pos[3 * i + 2], pid[i]) ? 1 : 0;
+			nf += found[i];
+		}
+		t1 = std::chrono::steady_clock::now();
+	} else {
+		container_poly con(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
+				   s->periodic[0] != 0, s->periodic[1] != 0, s->periodic[2] != 0, s->init_mem);
+		put_all_r(con, n, ids, xyz, r);
+		t0 = std::chrono::steady_clock::now();
+		for (int i = 0; i < nq; i++) {
+			found[i] = con.find_voronoi_cell(q[3 * i], q[3 * i + 1], q[3 * i + 2], pos[3 * i], pos[3 * i + 1],
+							 pos[3 * i + 2], pid[i]) ? 1 : 0;
+			nf += found[i];
+		}
+		t1 = std::chrono::steady_clock::now();
+	}
+	if (secs) *secs = std::chrono::duration<double>(t1 - t0).count();
+	return nf;
+}
+
 // Times the voronoicell_neighbor loop with volume/centroid/faces/neighbours.
 double ref_time_neighbor_loop(const ref_spec *s, int n, const int *ids, const double *xyz,
 			      const double *r) {

@@ -1,0 +1,90 @@
+"""find_voronoi_cell (src/container.cc:237-297): the reference library's
+answers pinned by brute force (nearest particle; least power distance for
+radical containers), and the HIP engine's batched queries against the
+reference, bit for bit (found flag, particle id, returned position)."""
+import numpy as np
+import pytest
+
+import voro_amd
+from voro_amd import inputs as I
+
+
+def _ref_or_skip(O):
+    if O.reference() is None:
+        pytest.skip("reference library not built")
+
+
+def test_reference_find_matches_brute_force(oracle_mod):
+    O = oracle_mod
+    _ref_or_skip(O)
+    n = 3000
+    xyz = I.uniform_box(13, n, (0, 0, 0), (12, 12, 12))
+    r = I.radii_lognormal(13, n) * 0.6
+    q = I.uniform_box(14, 400, (-1, -1, -1), (13, 13, 13))
+    inside = np.all((q >= 0) & (q < 12), axis=1)
+    d = ((q[:, None, :] - xyz[None, :, :]) ** 2).sum(-1)
+    for kind, rr in ((0, None), (1, r)):
+        sp = O.make_spec(kind, (0, 12, 0, 12, 0, 12), (6, 6, 6))
+        found, pid, pos = O.find_reference(sp, np.arange(n, dtype=np.int32), xyz, rr, q)
+        assert np.array_equal(found, inside)
+        best = (d - (rr[None, :] ** 2 if rr is not None else 0)).argmin(axis=1)
+        assert np.array_equal(pid[found], best[found])
+        assert np.array_equal(pos[found], xyz[best[found]])
+
+
+CASES = {
+    # kind, geometry, grid, periodic flags, particles, radii, queries
+    "plain": lambda: (0, (0, 20, 0, 20, 0, 20), (10, 10, 10), (0, 0, 0),
+                      I.uniform_box(21, 6000, (0, 0, 0), (20, 20, 20)), None,
+                      I.uniform_box(22, 20000, (-2, -2, -2), (22, 22, 22))),
+    "poly": lambda: (1, (0, 20, 0, 20, 0, 20), (10, 10, 10), (0, 0, 0),
+                     I.uniform_box(23, 6000, (0, 0, 0), (20, 20, 20)), I.radii_lognormal(23, 6000),
+                     I.uniform_box(24, 20000, (0, 0, 0), (20, 20, 20))),
+    "periodic": lambda: (0, (0, 20, 0, 20, 0, 20), (10, 10, 10), (1, 0, 1),
+                         I.uniform_box(25, 6000, (0, 0, 0), (20, 20, 20)), None,
+                         I.uniform_box(26, 20000, (-25, -2, -25), (45, 22, 45))),
+    "clusters": lambda: (0, (0, 20, 0, 20, 0, 20), (10, 10, 10), (0, 0, 0),
+                         I.clusters(27, 6000, box=20.0, n_clusters=6), None,
+                         I.uniform_box(28, 20000, (0, 0, 0), (20, 20, 20))),
+    # a few particles in many blocks: long block searches, some leaving the
+    # first search window
+    "sparse": lambda: (0, (0, 40, 0, 40, 0, 40), (40, 40, 40), (0, 0, 0),
+                       I.uniform_box(29, 12, (0, 0, 0), (40, 40, 40)), None,
+                       I.uniform_box(30, 5000, (0, 0, 0), (40, 40, 40))),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_gpu_find_matches_reference(oracle_mod, case):
+    O = oracle_mod
+    _ref_or_skip(O)
+    kind, geom, grid, per, xyz, r, q = CASES[case]()
+    n = len(xyz)
+    ids = np.arange(n, dtype=np.int32) * 3 + 7
+    print("find case %s n=%d queries=%d" % (case, n, len(q)), flush=True)
+    want = O.find_reference(O.make_spec(kind, geom, grid, periodic=per), ids, xyz, r, q)
+    cls = voro_amd.container_poly if kind == 1 else voro_amd.container
+    con = cls(*geom, *grid, *[bool(p) for p in per], 8)
+    con.put_array(ids, xyz, r)
+    got = con.find_voronoi_cells(q)
+    assert np.array_equal(got[0], want[0])
+    f = want[0]
+    assert np.array_equal(got[1][f], want[1][f])
+    assert np.array_equal(got[2][f], want[2][f])
+    # after a compute the same answers (the binned arrays are reused)
+    con.compute_all_cells()
+    again = con.find_voronoi_cells(q)
+    assert np.array_equal(again[1][f], want[1][f]) and np.array_equal(again[2][f], want[2][f])
+
+
+@pytest.mark.gpu
+def test_gpu_find_single_point_and_periodic_class_rejected():
+    con = voro_amd.container(0, 1, 0, 1, 0, 1, 2, 2, 2, False, False, False, 8)
+    con.put_array(np.array([5, 6], np.int32), np.array([[0.25, 0.25, 0.25], [0.75, 0.75, 0.75]]))
+    assert con.find_voronoi_cell(0.3, 0.2, 0.1) == (True, (0.25, 0.25, 0.25), 5)
+    assert con.find_voronoi_cell(1.5, 0.2, 0.1)[0] is False
+    pc = voro_amd.container_periodic(1, 0, 1, 0, 0, 1, 2, 2, 2, 8)
+    pc.put_array(np.array([1], np.int32), np.array([[0.5, 0.5, 0.5]]))
+    with pytest.raises(voro_amd.VoroGPUError):
+        pc.find_voronoi_cells(np.zeros((1, 3)))

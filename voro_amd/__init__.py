@@ -78,6 +78,7 @@ def lib():
         "vgpu_clear": (C.c_int, [p]),
         "vgpu_compute_all": (C.c_int, [p, C.c_uint, p]),
         "vgpu_num_cells": (C.c_int, [p, i64p]),
+        "vgpu_find_voronoi_cells": (C.c_int, [p, C.c_int, f64p, i32p, i32p, f64p, C.c_int]),
         "vgpu_get_results": (C.c_int, [p, i32p, i32p, i32p, i32p, f64p, f64p, f64p, i32p, i64p, i32p,
                                        C.c_longlong, i64p]),
         "vgpu_get_stats": (C.c_int, [p, i32p, i32p, i32p]),
@@ -260,6 +261,25 @@ class _Base:
         t = np.zeros(n.value, np.int32)
         _check(lib().vgpu_get_stats(self._h, _p(a, C.c_int), _p(b, C.c_int), _p(t, C.c_int)), "vgpu_get_stats")
         return {"n_planes": a, "wl_exit": b, "tier": t}
+
+    # -- point location, as container::find_voronoi_cell ----------------------
+    def find_voronoi_cells(self, xyz):
+        """find_voronoi_cell (src/container.cc:237-297) for every row of xyz
+        (rectangular containers): returns (found, pid, pos) arrays; found[i]
+        is False where the point lies outside a non-periodic direction."""
+        q = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        n = len(q)
+        found = np.zeros(n, np.int32)
+        pid = np.zeros(n, np.int32)
+        pos = np.zeros((n, 3))
+        _check(lib().vgpu_find_voronoi_cells(self._h, n, _p(q, C.c_double), _p(found, C.c_int), _p(pid, C.c_int),
+                                             _p(pos, C.c_double), 0), "vgpu_find_voronoi_cells")
+        return found.astype(bool), pid, pos
+
+    def find_voronoi_cell(self, x, y, z):
+        """The reference's signature in Python form: (found, (rx, ry, rz), pid)."""
+        f, pid, pos = self.find_voronoi_cells(np.array([[x, y, z]], np.float64))
+        return bool(f[0]), tuple(float(t) for t in pos[0]), int(pid[0])
 
     # -- per-cell access, as voro_compute::compute_cell(c, vl) ----------------
     def compute_cell(self, c, vl):

@@ -20,7 +20,7 @@ LIB = os.path.join(LIBDIR, "libvoro_gpu.so")
 ARCH = os.environ.get("VGPU_ARCH", "gfx950")
 
 UNITS = ["vgpu_api.hip", "cells_fast3.hip", "cells_wide.hip", "cells_huge.hip", "cells_prd.hip", "cells_list.hip",
-         "cells_batch.hip", "cells_wave.hip", "learn.hip"]
+         "cells_batch.hip", "cells_wave.hip", "find_cell.hip", "learn.hip"]
 HEADERS = ["vgpu_common.h", "cell_dev.h", "compute_dev.h", "vgpu_cells.h", "vgpu_bin.h", "grid_host.h",
            "worklist_table.h", "cell3_dev.h", "list_dev.h", "learn.h", "prd_host.h", "batch_dev.h", "wcell_dev.h"]
 

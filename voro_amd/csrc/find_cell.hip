@@ -1,0 +1,256 @@
+// find_voronoi_cell for many query points at once: for each point, the
+// particle whose Voronoi cell contains it (the nearest particle; in power
+// distance for radical containers), for rectangular containers (with or
+// without periodic directions).  One query per lane; the block search of a
+// query keeps its mask window and queue in the lane's slab of a per-lane
+// cell type (the order-3 tier's; then the wide and the huge tier's for the
+// few searches that leave the smaller windows).
+#include <hip/hip_runtime.h>
+
+#include "vgpu_cells.h"
+
+namespace vgpu {
+
+namespace {
+
+// step_int / step_div of src/common.hh (rounding toward minus infinity).
+VG_INL int q_step_int(double a) { return a < 0 ? int(a) - 1 : int(a); }
+
+struct Found {
+	int b, l;           // storage block and index of the particle, b = -1: none yet
+	int di, dj, dk;     // worklist displacement of that block
+};
+
+// scan_all, src/v_compute.cc:46-57: the particles of storage block b against
+// (x,y,z), keeping the first strict minimum.
+template<bool POLY>
+VG_INL void scan_block(const RectView &v, int b, double x, double y, double z, int di, int dj, int dk, Found &w,
+		       double &mrs) {
+	const int b0 = v.bstart[b], n = v.bstart[b + 1] - b0;
+	bool in_block = false;
+	for (int l = 0; l < n; l++) {
+		const dbl4 o = v.P[b0 + l];
+		const double x1 = o.x - x, y1 = o.y - y, z1 = o.z - z;
+		double rs = x1 * x1 + y1 * y1 + z1 * z1;
+		if (POLY) rs = rs - o.w * o.w;
+		if (rs < mrs) { mrs = rs; w.l = l; in_block = true; }
+	}
+	if (in_block) { w.b = b; w.di = di; w.dj = dj; w.dk = dk; }
+}
+
+// compute_min_radius, src/v_compute.cc:947-961.
+template<bool POLY>
+VG_INL bool min_radius_skip(const RectView &v, int di, int dj, int dk, double fx, double fy, double fz, double mrs) {
+	double t, crs;
+	if (di > 0) { t = di * v.boxx - fx; crs = t * t; }
+	else if (di < 0) { t = (di + 1) * v.boxx - fx; crs = t * t; }
+	else crs = 0;
+	if (dj > 0) { t = dj * v.boxy - fy; crs += t * t; }
+	else if (dj < 0) { t = (dj + 1) * v.boxy - fy; crs += t * t; }
+	if (dk > 0) { t = dk * v.boxz - fz; crs += t * t; }
+	else if (dk < 0) { t = (dk + 1) * v.boxz - fz; crs += t * t; }
+	return crs > (POLY ? mrs + v.max_radius * v.max_radius : mrs);
+}
+
+// container::find_voronoi_cell (src/container.cc:237-262, container_poly's
+// src/container.cc:274-297) around voro_compute::find_voronoi_cell
+// (src/v_compute.cc:71-228).  Returns 1 and the particle (its storage index,
+// its position shifted by whole periods) if one was found, 0 if none (a point
+// outside a non-periodic direction, or no particle), -1 if the block search
+// left the mask window (retry with a wider one).
+template<class CELL, bool POLY>
+VG_FN int find_cell(CELL &c, const RectView &v, double x, double y, double z, unsigned stamp, int &idx, double &rx,
+		    double &ry, double &rz) {
+	// remap, src/container.cc:203-224
+	int ai, aj, ak;
+	int ci = q_step_int((x - v.ax) * v.xsp);
+	if (ci < 0 || ci >= v.nx) {
+		if (!v.xper) return 0;
+		ai = step_div_i(ci, v.nx); x -= ai * (v.bx - v.ax); ci -= ai * v.nx;
+	} else ai = 0;
+	int cj = q_step_int((y - v.ay) * v.ysp);
+	if (cj < 0 || cj >= v.ny) {
+		if (!v.yper) return 0;
+		aj = step_div_i(cj, v.ny); y -= aj * (v.by - v.ay); cj -= aj * v.ny;
+	} else aj = 0;
+	int ck = q_step_int((z - v.az) * v.zsp);
+	if (ck < 0 || ck >= v.nz) {
+		if (!v.zper) return 0;
+		ak = step_div_i(ck, v.nz); z -= ak * (v.bz - v.az); ck -= ak * v.nz;
+	} else ak = 0;
+	const int ijk0 = ci + v.nx * cj + v.nxy * ck;
+
+	Found w{-1, 0, 0, 0, 0};
+	double mrs = 1.7976931348623157e308;     // large_number, src/config.hh:87
+	// initialize_search, src/container.hh:214-219
+	const int i = v.xper ? v.nx : ci, j = v.yper ? v.ny : cj, k = v.zper ? v.nz : ck;
+	const int disp = ijk0 - i - v.nx * (j + v.ny * k);
+	scan_block<POLY>(v, ijk0, x, y, z, 0, 0, 0, w, mrs);
+
+	const double fx = x - v.ax - v.boxx * ci, fy = y - v.ay - v.boxy * cj, fz = z - v.az - v.boxz * ck;
+	int di = int(fx * v.xsp * VGPU_WL_FGRID), dj = int(fy * v.ysp * VGPU_WL_FGRID), dk = int(fz * v.zsp * VGPU_WL_FGRID);
+	unsigned m1, m2;
+	double mxs, mys, mzs;
+	if (di >= VGPU_WL_HGRID) {
+		mxs = v.boxx - fx;
+		m1 = 127 + (3 << 21); m2 = 1 + (1 << 21); di = VGPU_WL_FGRID - 1 - di; if (di < 0) di = 0;
+	} else { m1 = m2 = 0; mxs = fx; }
+	if (dj >= VGPU_WL_HGRID) {
+		mys = v.boxy - fy;
+		m1 |= (127 << 7) + (3 << 24); m2 |= (1 << 7) + (1 << 24); dj = VGPU_WL_FGRID - 1 - dj; if (dj < 0) dj = 0;
+	} else mys = fy;
+	if (dk >= VGPU_WL_HGRID) {
+		mzs = v.boxz - fz;
+		m1 |= (127 << 14) + (3 << 27); m2 |= (1 << 14) + (1 << 27); dk = VGPU_WL_FGRID - 1 - dk; if (dk < 0) dk = 0;
+	} else mzs = fz;
+	const double r2 = POLY ? v.max_radius * v.max_radius : 0.0;
+	bool done = false;
+	{
+		const double rs = POLY ? mrs + r2 : mrs;
+		if (mxs * mxs > rs && mys * mys > rs && mzs * mzs > rs) done = true;
+	}
+	if (!done) {
+		const int wli = di + VGPU_WL_HGRID * (dj + VGPU_WL_HGRID * dk);
+		const double *radp = v.mrad + wli * 64;
+		const unsigned *e = v.wl + wli * 64;
+		double qx = 0, qy = 0, qz = 0;
+		const int f = (int)e[0];
+		int g = 0;
+		// worklist blocks that need no mask
+		do {
+			if ((POLY ? mrs + r2 : mrs) < radp[g]) { done = true; break; }
+			g++;
+			unsigned q = e[g];
+			q ^= m1; q += m2;
+			const int ddi = (int)(q & 127) - 64, ddj = (int)((q >> 7) & 127) - 64, ddk = (int)((q >> 14) & 127) - 64;
+			const int ei = ddi + i, ej = ddj + j, ek = ddk + k;
+			if (ei < 0 || ei >= v.hx || ej < 0 || ej >= v.hy || ek < 0 || ek >= v.hz) continue;
+			if (min_radius_skip<POLY>(v, ddi, ddj, ddk, fx, fy, fz, mrs)) continue;
+			const int b = rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
+			scan_block<POLY>(v, b, x - qx, y - qy, z - qz, ddi, ddj, ddk, w, mrs);
+		} while (g < f);
+		BlockSearch<CELL> bs(c, stamp, i, j, k);
+		const unsigned b1 = 1u << 21, b2 = 1u << 22, b3 = 1u << 24, b4 = 1u << 25, b5 = 1u << 27, b6 = 1u << 28;
+		// the rest of the worklist, marking blocks and queueing their outside
+		// neighbours (scan_bits_mask_add, src/v_compute.cc:249-264)
+		while (!done && g < 63) {
+			if ((POLY ? mrs + r2 : mrs) < radp[g]) { done = true; break; }
+			g++;
+			unsigned q = e[g];
+			q ^= m1; q += m2;
+			const int ddi = (int)(q & 127) - 64, ddj = (int)((q >> 7) & 127) - 64, ddk = (int)((q >> 14) & 127) - 64;
+			const int ei = ddi + i, ej = ddj + j, ek = ddk + k;
+			if (ei < 0 || ei >= v.hx || ej < 0 || ej >= v.hy || ek < 0 || ek >= v.hz) continue;
+			bs.mark(ei, ej, ek);
+			if (min_radius_skip<POLY>(v, ddi, ddj, ddk, fx, fy, fz, mrs)) continue;
+			const int b = rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
+			scan_block<POLY>(v, b, x - qx, y - qy, z - qz, ddi, ddj, ddk, w, mrs);
+			if ((q & b2) == b2) {
+				if (ei > 0) bs.mark_push(ei - 1, ej, ek);
+				if ((q & b1) == 0 && ei < v.hx - 1) bs.mark_push(ei + 1, ej, ek);
+			} else if ((q & b1) == b1 && ei < v.hx - 1) bs.mark_push(ei + 1, ej, ek);
+			if ((q & b4) == b4) {
+				if (ej > 0) bs.mark_push(ei, ej - 1, ek);
+				if ((q & b3) == 0 && ej < v.hy - 1) bs.mark_push(ei, ej + 1, ek);
+			} else if ((q & b3) == b3 && ej < v.hy - 1) bs.mark_push(ei, ej + 1, ek);
+			if ((q & b6) == b6) {
+				if (ek > 0) bs.mark_push(ei, ej, ek - 1);
+				if ((q & b5) == 0 && ek < v.hz - 1) bs.mark_push(ei, ej, ek + 1);
+			} else if ((q & b5) == b5 && ek < v.hz - 1) bs.mark_push(ei, ej, ek + 1);
+			if (c.ovf) return -1;
+		}
+		if (!done && (POLY ? mrs + r2 : mrs) < radp[g]) done = true;
+		// breadth-first over the queued blocks (src/v_compute.cc:211-227)
+		int ei, ej, ek;
+		while (!done && bs.pop(ei, ej, ek)) {
+			const int ddi = ei - i, ddj = ej - j, ddk = ek - k;
+			if (min_radius_skip<POLY>(v, ddi, ddj, ddk, fx, fy, fz, mrs)) continue;
+			const int b = rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
+			scan_block<POLY>(v, b, x - qx, y - qy, z - qz, ddi, ddj, ddk, w, mrs);
+			// add_to_mask, src/v_compute.cc:236-243
+			if (ek > 0 && !bs.marked(ei, ej, ek - 1)) bs.mark_push(ei, ej, ek - 1);
+			if (ej > 0 && !bs.marked(ei, ej - 1, ek)) bs.mark_push(ei, ej - 1, ek);
+			if (ei > 0 && !bs.marked(ei - 1, ej, ek)) bs.mark_push(ei - 1, ej, ek);
+			if (ei < v.hx - 1 && !bs.marked(ei + 1, ej, ek)) bs.mark_push(ei + 1, ej, ek);
+			if (ej < v.hy - 1 && !bs.marked(ei, ej + 1, ek)) bs.mark_push(ei, ej + 1, ek);
+			if (ek < v.hz - 1 && !bs.marked(ei, ej, ek + 1)) bs.mark_push(ei, ej, ek + 1);
+			if (c.ovf) return -1;
+		}
+		if (c.ovf) return -1;
+	}
+	if (w.b < 0) return 0;
+	// the particle's position, shifted by whole periods
+	if (v.xper) { ci += w.di; if (ci < 0 || ci >= v.nx) ai += step_div_i(ci, v.nx); }
+	if (v.yper) { cj += w.dj; if (cj < 0 || cj >= v.ny) aj += step_div_i(cj, v.ny); }
+	if (v.zper) { ck += w.dk; if (ck < 0 || ck >= v.nz) ak += step_div_i(ck, v.nz); }
+	idx = v.bstart[w.b] + w.l;
+	const dbl4 o = v.P[idx];
+	rx = o.x + ai * (v.bx - v.ax);
+	ry = o.y + aj * (v.by - v.ay);
+	rz = o.z + ak * (v.bz - v.az);
+	return 1;
+}
+
+// Persistent kernel, one query per lane, 64 queries per counter increment,
+// over every query or over a device list of them (its count at *list_count).
+// found[q]: 1 / 0 as find_cell returns; a query whose search left the window
+// goes to the retry list (or, without one, gets -1).
+template<class T_, bool POLY>
+__global__ void __launch_bounds__(64, 2)
+k_find_cells(RectView v, int n, const double *__restrict__ xyz, const int *__restrict__ list,
+	     const int *__restrict__ list_count, int *__restrict__ found, int *__restrict__ pid,
+	     double *__restrict__ pos, int *__restrict__ retry, int *__restrict__ retry_count, int *__restrict__ counter,
+	     char *__restrict__ ws, size_t slab, unsigned stamp_base) {
+	typedef typename TierCell<T_>::type CELL;
+	const int lane = threadIdx.x;
+	CELL c;
+	c.setup(ws + (size_t)blockIdx.x * slab, lane, v.max_len_sq);
+	const int total = list_count ? *list_count : n;
+	for (;;) {
+		int base = 0;
+		if (lane == 0) base = atomicAdd(counter, 64);
+		base = __builtin_amdgcn_readfirstlane(base);
+		if (base >= total) break;
+		const int t = base + lane;
+		if (t >= total) continue;
+		const int qi = list ? list[t] : t;
+		c.ovf = false;
+		int idx = 0;
+		double rx = 0, ry = 0, rz = 0;
+		const int r = find_cell<CELL, POLY>(c, v, xyz[3 * qi], xyz[3 * qi + 1], xyz[3 * qi + 2],
+						    stamp_base + (unsigned)t + 1u, idx, rx, ry, rz);
+		if (r < 0 && retry) {
+			retry[atomicAdd(retry_count, 1)] = qi;
+			continue;
+		}
+		found[qi] = r;
+		if (r > 0) {
+			pid[qi] = v.pid[idx];
+			pos[3 * qi] = rx; pos[3 * qi + 1] = ry; pos[3 * qi + 2] = rz;
+		}
+	}
+}
+
+}  // namespace
+
+int launch_find_cells(int tier, bool poly, int waves, hipStream_t s, const RectView &v, int n, const double *xyz,
+		      const int *list, const int *list_count, int *found, int *pid, double *pos, int *retry,
+		      int *retry_count, int *counter, char *ws, size_t slab, unsigned stamp_base) {
+#define VGPU_FIND(T, P)                                                                                          \
+	hipLaunchKernelGGL((k_find_cells<T, P>), dim3(waves), dim3(64), 0, s, v, n, xyz, list, list_count, found, pid, \
+			   pos, retry, retry_count, counter, ws, slab, stamp_base)
+	if (tier == 0) {
+		if (poly) VGPU_FIND(Tier3, true);
+		else VGPU_FIND(Tier3, false);
+	} else if (tier == 1) {
+		if (poly) VGPU_FIND(TierWide, true);
+		else VGPU_FIND(TierWide, false);
+	} else {
+		if (poly) VGPU_FIND(TierHuge, true);
+		else VGPU_FIND(TierHuge, false);
+	}
+#undef VGPU_FIND
+	return hipGetLastError() == hipSuccess ? 0 : 6;
+}
+
+}  // namespace vgpu

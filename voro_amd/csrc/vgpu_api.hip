@@ -91,6 +91,7 @@ struct vgpu_container {
 	int ncells = 0;
 	double max_radius = 0;
 	bool computed = false;
+	bool binned = false;     // block arrays current (binning done since the last put)
 	unsigned last_flags = 0;
 	int fast_waves = 0;
 	int fast_occ = 2;
@@ -108,6 +109,7 @@ struct vgpu_container {
 	DBuf d_mrad, d_wl, d_walls, d_maxr, d_counters;
 	DBuf vol, cent, nfaces, status, nbr_start, arena, csr_off, csr, ctr;
 	DBuf retry0, retry1, retry2, retry3, retry4, retry5;
+	DBuf fq_xyz, fq_found, fq_pid, fq_pos, fq_retry, fq_retry2;   // find_voronoi_cell queries
 	DBuf st_planes, st_exit, st_tier;
 	// learned path
 	bool have_pred = false;
@@ -512,7 +514,7 @@ int run_prd_cells(vgpu_container *h, RectView &v, CellOut out, int nc, int fw, h
 	return VGPU_INTERNAL_ERROR;   // the first-visit order did not settle
 }
 
-int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
+int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s, bool bin_only = false) {
 	const int nxyz = h->nblocks;
 	int n = h->n_in;
 	int st;
@@ -584,6 +586,8 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s) {
 	h->max_radius = h->poly ? mr : 0;
 	if (dup) { h->computed = false; return VGPU_DUPLICATE; }
 	HIPCHK(hipEventRecord(h->ev[1], s));
+	h->binned = true;
+	if (bin_only) return VGPU_OK;
 
 	// outputs
 	const size_t C1 = (size_t)std::max(nc, 1);
@@ -1013,7 +1017,8 @@ int vgpu_destroy(vgpu_container *h) {
 			&h->d_pred, &h->d_feat, &h->d_cfeat, &h->d_depth, &h->d_cost, &h->d_order, &h->d_hist, &h->lcnt,
 			&h->bmax, &h->boff, &h->list_a, &h->list_b, &h->d_unit, &h->Pall, &h->pidall, &h->ball, &h->fv,
 			&h->touch, &h->d_flag, &h->retry3, &h->retry4, &h->retry5, &h->img_p, &h->img_id,
-			&h->img_reg, &h->img_rank, &h->img_bc, &h->img_cnt};
+			&h->img_reg, &h->img_rank, &h->img_bc, &h->img_cnt, &h->fq_xyz, &h->fq_found, &h->fq_pid,
+			&h->fq_pos, &h->fq_retry, &h->fq_retry2};
 	for (DBuf *b : bufs) b->release();
 	h->ws_fast.buf.release(); h->ws_wide.buf.release(); h->ws_huge.buf.release(); h->ws_wave.buf.release();
 	if (h->events) for (int t = 0; t < 5; t++) (void)hipEventDestroy(h->ev[t]);
@@ -1030,6 +1035,7 @@ int vgpu_add_wall_sphere(vgpu_container *h, double xc, double yc, double zc, dou
 	if (hipMemcpy(h->d_walls.p, h->walls.data(), h->walls.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
 		return VGPU_HIP_ERROR;
 	h->computed = false;
+	h->binned = false;
 	return VGPU_OK;
 }
 
@@ -1057,6 +1063,7 @@ int vgpu_put(vgpu_container *h, int n, const int *ids, const double *xyz, const 
 	if (h->poly) h->h_r.insert(h->h_r.end(), r, r + n);
 	h->n_in += n;
 	h->computed = false;
+	h->binned = false;
 	return VGPU_OK;
 }
 
@@ -1080,6 +1087,7 @@ int vgpu_put_device(vgpu_container *h, int n, const int *d_ids, const double *d_
 	h->have_dev_input = true;
 	h->n_in = n;
 	h->computed = false;
+	h->binned = false;
 	return VGPU_OK;
 }
 
@@ -1089,6 +1097,7 @@ int vgpu_clear(vgpu_container *h) {
 	h->have_dev_input = false;
 	h->n_in = 0;
 	h->computed = false;
+	h->binned = false;
 	return VGPU_OK;
 }
 
@@ -1096,6 +1105,74 @@ int vgpu_compute_all(vgpu_container *h, unsigned flags, void *stream) {
 	if (h && (flags & VGPU_F_LEARNED) && (!h->have_pred || h->prd)) return VGPU_INVALID_ARGUMENT;
 	if (!h) return VGPU_INVALID_ARGUMENT;
 	return run_pipeline(h, flags, (hipStream_t)stream);
+}
+
+int vgpu_find_voronoi_cells(vgpu_container *h, int n, const double *xyz, int *found, int *pid, double *pos,
+			    int on_device) {
+	if (!h || n < 0 || (n > 0 && (!xyz || !found || !pid || !pos))) return VGPU_INVALID_ARGUMENT;
+	if (h->prd) return VGPU_INVALID_ARGUMENT;       // rectangular containers only
+	if (n == 0) return VGPU_OK;
+	hipStream_t s = nullptr;
+	int st;
+	if (!h->binned && (st = run_pipeline(h, 0, s, true))) return st;
+	const size_t N = (size_t)n;
+	const double *dq = xyz;
+	int *dfound = found, *dpid = pid;
+	double *dpos = pos;
+	if (!on_device) {
+		if ((st = h->fq_xyz.ensure(N * 24)) || (st = h->fq_found.ensure(N * 4)) || (st = h->fq_pid.ensure(N * 4)) ||
+		    (st = h->fq_pos.ensure(N * 24)))
+			return st;
+		HIPCHK(hipMemcpyAsync(h->fq_xyz.p, xyz, N * 24, hipMemcpyHostToDevice, s));
+		dq = h->fq_xyz.as<double>(); dfound = h->fq_found.as<int>(); dpid = h->fq_pid.as<int>();
+		dpos = h->fq_pos.as<double>();
+	}
+	if ((st = h->fq_retry.ensure(N * 4)) || (st = h->fq_retry2.ensure(N * 4)) || (st = h->d_counters.ensure(64 * 4)))
+		return st;
+	typedef Layout<TierWide, 64> LW;
+	typedef Layout<TierHuge, 64> LH;
+	const int w0 = std::max(1, std::min(default_fast_waves(2), nblk((long long)n, 64)));
+	if ((st = h->ws_fast.ensure(VCell3::L::bytes, std::max(w0, h->ws_fast.waves))) ||
+	    (st = h->ws_wide.ensure(LW::bytes, std::max(64, h->ws_wide.waves))) ||
+	    (st = h->ws_huge.ensure(LH::bytes, std::max(4, h->ws_huge.waves))))
+		return st;
+	Workspace *wss[3] = {&h->ws_fast, &h->ws_wide, &h->ws_huge};
+	for (Workspace *x : wss)
+		if ((unsigned long long)x->stamp_base + N + 2 >= 0xffffffffull) {
+			HIPCHK(hipMemset(x->buf.p, 0, x->buf.n));
+			x->stamp_base = 0;
+		}
+	// [0] / [2] / [4] query counters of the three tiers, [1] / [3] their retry counts
+	int *ctr = h->d_counters.as<int>();
+	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 8, 0);
+	const RectView v = make_view(h);
+	if (launch_find_cells(0, h->poly, w0, s, v, n, dq, nullptr, nullptr, dfound, dpid, dpos, h->fq_retry.as<int>(),
+			      ctr + 1, ctr, h->ws_fast.base(), h->ws_fast.slab, h->ws_fast.stamp_base))
+		return VGPU_HIP_ERROR;
+	h->ws_fast.stamp_base += (unsigned)n + 1u;
+	// the few searches that left the order-3 window, with the wide one, then
+	// the huge one
+	if (launch_find_cells(1, h->poly, 64, s, v, n, dq, h->fq_retry.as<int>(), ctr + 1, dfound, dpid, dpos,
+			      h->fq_retry2.as<int>(), ctr + 3, ctr + 2, h->ws_wide.base(), h->ws_wide.slab,
+			      h->ws_wide.stamp_base))
+		return VGPU_HIP_ERROR;
+	h->ws_wide.stamp_base += (unsigned)n + 1u;
+	if (launch_find_cells(2, h->poly, 4, s, v, n, dq, h->fq_retry2.as<int>(), ctr + 3, dfound, dpid, dpos, nullptr,
+			      nullptr, ctr + 4, h->ws_huge.base(), h->ws_huge.slab, h->ws_huge.stamp_base))
+		return VGPU_HIP_ERROR;
+	h->ws_huge.stamp_base += (unsigned)n + 1u;
+	int retried = 0;
+	HIPCHK(hipMemcpyAsync(&retried, ctr + 3, 4, hipMemcpyDeviceToHost, s));
+	if (!on_device) {
+		HIPCHK(hipMemcpyAsync(found, dfound, N * 4, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipMemcpyAsync(pid, dpid, N * 4, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipMemcpyAsync(pos, dpos, N * 24, hipMemcpyDeviceToHost, s));
+	}
+	HIPCHK(hipStreamSynchronize(s));
+	if (retried > 0 && !on_device) {
+		for (int q = 0; q < n; q++) if (found[q] < 0) return VGPU_MEMORY_ERROR;
+	}
+	return VGPU_OK;
 }
 
 int vgpu_num_cells(vgpu_container *h, long long *n) {

@@ -141,6 +141,12 @@ int launch_cells_huge(bool poly, int occ, int waves, hipStream_t s, const RectVi
 // cell or a device list (count at *list_count), at 2, 3, 4 or 8 waves per
 // SIMD (k = 1; k = 2 and 4 run at 2); cells it cannot finish go to
 // out.retry.
+// find_voronoi_cell for n query points (find_cell.hip): tiers 0, 1, 2 search
+// with the order-3, wide and huge cells' mask windows and send queries that
+// leave the window to `retry` (none: found[q] = -1).
+int launch_find_cells(int tier, bool poly, int waves, hipStream_t s, const RectView &v, int n, const double *xyz,
+		      const int *list, const int *list_count, int *found, int *pid, double *pos, int *retry,
+		      int *retry_count, int *counter, char *ws, size_t slab, unsigned stamp_base);
 int launch_cells_wave(int k, bool poly, bool prd, int occ, int waves, hipStream_t s, const RectView &v, int ncells,
 		      const int *list, const int *list_count, const int *cellblk, int *counters, char *ws,
 		      unsigned stamp_base, int chunk, const CellOut &out);
