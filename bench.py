@@ -21,6 +21,7 @@ sources into oracle/_ref), timed on this host, one thread, on the same
 particles.
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -59,7 +60,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=400, help="predictor training steps (full-batch Adam)")
     ap.add_argument("--fallback", type=float, default=0.02,
                     help="fraction of training blocks the depth margin allows to be under-predicted")
-    ap.add_argument("--workload", choices=["uniform", "periodic_poly", "batch", "clusters"], default="uniform",
+    ap.add_argument("--workload", choices=["uniform", "periodic_poly", "batch", "clusters", "find"], default="uniform",
                     help="uniform: the headline measurement (default); periodic_poly / batch: only that "
                          "secondary workload, for profiling")
     ap.add_argument("--headline-only", action="store_true",
@@ -230,22 +231,70 @@ def secondary_workloads(args, torch, only=None):
     """The radical periodic workload (container_periodic_poly(60,5,60,3,4,60),
     2,000,000 particles, 74^3 blocks), the batched one (4,096 FCC-jitter
     containers with sphere walls, ~15.5M particles, one graph launch after
-    autotuning) and the clustered one with the predictor: GPU cells/s
-    (inputs resident on the device), and the reference timed on bounded
-    samples on this host."""
+    autotuning), the clustered one with the predictor and 1M find_voronoi_cell
+    queries: GPU rates (inputs resident on the device), and the reference
+    timed on bounded samples on this host."""
     import voro_amd
     out = {}
     flags = voro_amd.F_NEIGHBORS | voro_amd.F_CSR
     specs = None
     if only is None or only == "periodic_poly":
         periodic_poly(out, flags, torch, args.steps if only else 1)
-    if only != "periodic_poly" and only != "clusters":
+    if only is None or only == "batch":
         specs = batch(out, flags, args.steps if only else 1)
     if only is None or only == "clusters":
         clusters_learned(out, flags, torch)
+    if only is None or only == "find":
+        find_cells(out, args.steps if only else 3, not args.no_cpu)
     if only is None and not args.no_cpu and specs is not None:
         cpu_samples(out, specs)
     return out
+
+
+def find_cells(out, steps, cpu):
+    """find_voronoi_cell: 1,000,000 uniform query points in the 1M uniform
+    container ([0,100]^3, 58^3 blocks), queries and answers resident on the
+    device; beside it the reference library's container::find_voronoi_cell
+    on this host, one thread, over the first 200,000 queries."""
+    import torch
+    import voro_amd
+    from voro_amd import inputs as I
+    ids, xyz, _ = I.make("uniform", 42)
+    q = I.uniform_box(7, 1000000, (0, 0, 0), (100, 100, 100))
+    con = voro_amd.container(0, 100, 0, 100, 0, 100, 58, 58, 58, False, False, False, 8)
+    con.put_array(ids, xyz)
+    dq = torch.from_numpy(q).to("cuda:0")
+    found = torch.zeros(len(q), dtype=torch.int32, device="cuda:0")
+    pid = torch.zeros(len(q), dtype=torch.int32, device="cuda:0")
+    pos = torch.zeros((len(q), 3), dtype=torch.float64, device="cuda:0")
+    L = voro_amd.lib()
+
+    def run():
+        voro_amd._check(L.vgpu_find_voronoi_cells(con._h, len(q), C.cast(dq.data_ptr(), C.POINTER(C.c_double)),
+                                                  C.cast(found.data_ptr(), C.POINTER(C.c_int)),
+                                                  C.cast(pid.data_ptr(), C.POINTER(C.c_int)),
+                                                  C.cast(pos.data_ptr(), C.POINTER(C.c_double)), 1),
+                        "vgpu_find_voronoi_cells")
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    out["find_voronoi_cell_1m"] = {"queries_per_s": len(q) / dt, "ms": dt * 1e3, "queries": len(q),
+                                   "found": int(found.sum().item())}
+    if cpu:
+        from oracle import oracle as O
+        if O.reference() is not None:
+            m = 200000
+            sp = O.make_spec(0, (0, 100, 0, 100, 0, 100), (58, 58, 58))
+            f, p, _, secs = O.find_reference(sp, ids, xyz, None, q[:m], timed=True)
+            same = bool(np.array_equal(p[f], pid[:m].cpu().numpy()[f]))
+            out["find_voronoi_cell_1m"]["cpu_baseline"] = {
+                "value": m / secs, "unit": "queries/s", "cores": 1, "kind": "reference",
+                "sample": "container::find_voronoi_cell on the first %d queries, %.2f s" % (m, secs),
+                "same_particles": same}
 
 
 def periodic_poly(out, flags, torch, steps):
