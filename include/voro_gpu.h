@@ -88,7 +88,9 @@ int vgpu_compute_all(vgpu_container *h, unsigned flags, void *stream);
  * contains point i, or 0 (point outside a non-periodic direction, or no
  * particle).  Rectangular containers only (VGPU_INVALID_ARGUMENT for the
  * periodic classes).  Bins the particles if they changed since the last
- * compute.  on_device: the four arrays are device pointers. */
+ * compute.  on_device: the four arrays are device pointers.  Returns
+ * VGPU_MEMORY_ERROR (found[i] = -1) if a search outgrew the largest block
+ * window (±63 blocks). */
 int vgpu_find_voronoi_cells(vgpu_container *h, int n, const double *xyz, int *found, int *pid, double *pos,
 			    int on_device);
 /* Number of particles stored (the c_loop_all length). */

@@ -194,7 +194,8 @@ VG_FN int find_cell(CELL &c, const RectView &v, double x, double y, double z, un
 // Persistent kernel, one query per lane, 64 queries per counter increment,
 // over every query or over a device list of them (its count at *list_count).
 // found[q]: 1 / 0 as find_cell returns; a query whose search left the window
-// goes to the retry list (or, without one, gets -1).
+// goes to the retry list (or, without one, gets -1 and is counted at
+// *retry_count).
 template<class T_, bool POLY>
 __global__ void __launch_bounds__(64, 2)
 k_find_cells(RectView v, int n, const double *__restrict__ xyz, const int *__restrict__ list,
@@ -219,9 +220,12 @@ k_find_cells(RectView v, int n, const double *__restrict__ xyz, const int *__res
 		double rx = 0, ry = 0, rz = 0;
 		const int r = find_cell<CELL, POLY>(c, v, xyz[3 * qi], xyz[3 * qi + 1], xyz[3 * qi + 2],
 						    stamp_base + (unsigned)t + 1u, idx, rx, ry, rz);
-		if (r < 0 && retry) {
-			retry[atomicAdd(retry_count, 1)] = qi;
-			continue;
+		if (r < 0) {
+			if (retry) {
+				retry[atomicAdd(retry_count, 1)] = qi;
+				continue;
+			}
+			if (retry_count) atomicAdd(retry_count, 1);   // the last tier counts failures
 		}
 		found[qi] = r;
 		if (r > 0) {

@@ -1142,7 +1142,8 @@ int vgpu_find_voronoi_cells(vgpu_container *h, int n, const double *xyz, int *fo
 			HIPCHK(hipMemset(x->buf.p, 0, x->buf.n));
 			x->stamp_base = 0;
 		}
-	// [0] / [2] / [4] query counters of the three tiers, [1] / [3] their retry counts
+	// [0] / [2] / [4] query counters of the three tiers, [1] / [3] their retry
+	// counts, [5] queries no window held
 	int *ctr = h->d_counters.as<int>();
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 8, 0);
 	const RectView v = make_view(h);
@@ -1151,28 +1152,37 @@ int vgpu_find_voronoi_cells(vgpu_container *h, int n, const double *xyz, int *fo
 		return VGPU_HIP_ERROR;
 	h->ws_fast.stamp_base += (unsigned)n + 1u;
 	// the few searches that left the order-3 window, with the wide one, then
-	// the huge one
-	if (launch_find_cells(1, h->poly, 64, s, v, n, dq, h->fq_retry.as<int>(), ctr + 1, dfound, dpid, dpos,
-			      h->fq_retry2.as<int>(), ctr + 3, ctr + 2, h->ws_wide.base(), h->ws_wide.slab,
-			      h->ws_wide.stamp_base))
-		return VGPU_HIP_ERROR;
-	h->ws_wide.stamp_base += (unsigned)n + 1u;
-	if (launch_find_cells(2, h->poly, 4, s, v, n, dq, h->fq_retry2.as<int>(), ctr + 3, dfound, dpid, dpos, nullptr,
-			      nullptr, ctr + 4, h->ws_huge.base(), h->ws_huge.slab, h->ws_huge.stamp_base))
-		return VGPU_HIP_ERROR;
-	h->ws_huge.stamp_base += (unsigned)n + 1u;
+	// the huge one (each launched only if it has queries)
 	int retried = 0;
-	HIPCHK(hipMemcpyAsync(&retried, ctr + 3, 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(&retried, ctr + 1, 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
+	if (retried > 0) {
+		if (launch_find_cells(1, h->poly, 64, s, v, n, dq, h->fq_retry.as<int>(), ctr + 1, dfound, dpid, dpos,
+				      h->fq_retry2.as<int>(), ctr + 3, ctr + 2, h->ws_wide.base(), h->ws_wide.slab,
+				      h->ws_wide.stamp_base))
+			return VGPU_HIP_ERROR;
+		h->ws_wide.stamp_base += (unsigned)n + 1u;
+		HIPCHK(hipMemcpyAsync(&retried, ctr + 3, 4, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipStreamSynchronize(s));
+	}
+	if (retried > 0) {
+		if (launch_find_cells(2, h->poly, 4, s, v, n, dq, h->fq_retry2.as<int>(), ctr + 3, dfound, dpid, dpos,
+				      nullptr, ctr + 5, ctr + 4, h->ws_huge.base(), h->ws_huge.slab, h->ws_huge.stamp_base))
+			return VGPU_HIP_ERROR;
+		h->ws_huge.stamp_base += (unsigned)n + 1u;
+	}
 	if (!on_device) {
 		HIPCHK(hipMemcpyAsync(found, dfound, N * 4, hipMemcpyDeviceToHost, s));
 		HIPCHK(hipMemcpyAsync(pid, dpid, N * 4, hipMemcpyDeviceToHost, s));
 		HIPCHK(hipMemcpyAsync(pos, dpos, N * 24, hipMemcpyDeviceToHost, s));
 	}
 	HIPCHK(hipStreamSynchronize(s));
-	if (retried > 0 && !on_device) {
-		for (int q = 0; q < n; q++) if (found[q] < 0) return VGPU_MEMORY_ERROR;
+	int failed = 0;
+	if (retried > 0) {
+		HIPCHK(hipMemcpyAsync(&failed, ctr + 5, 4, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipStreamSynchronize(s));
 	}
-	return VGPU_OK;
+	return failed > 0 ? VGPU_MEMORY_ERROR : VGPU_OK;
 }
 
 int vgpu_num_cells(vgpu_container *h, long long *n) {
