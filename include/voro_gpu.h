@@ -93,6 +93,18 @@ int vgpu_compute_all(vgpu_container *h, unsigned flags, void *stream);
  * window (±63 blocks). */
 int vgpu_find_voronoi_cells(vgpu_container *h, int n, const double *xyz, int *found, int *pid, double *pos,
 			    int on_device);
+/* container::compute_ghost_cell / container_poly::compute_ghost_cell
+ * (src/container.hh:488-498,710-721) for n points (host arrays; r: radii,
+ * radical containers only): the cell a particle at each point would have if
+ * it were put last in its block.  exists[i]: the reference's return value
+ * (0: the point lies outside a non-periodic direction, or its cell vanished);
+ * vol, cent (relative to the point), nfaces, and up to max_nbr neighbour ids
+ * per point at nbr[i * max_nbr].  Rectangular containers only.  Returns
+ * VGPU_MEMORY_ERROR (exists[i] = -1) for a cell over 1,024 vertices or
+ * max_nbr faces, or whose search comes back to the point's own block through
+ * a periodic image. */
+int vgpu_compute_ghost_cells(vgpu_container *h, int n, const double *xyz, const double *r, int *exists, double *vol,
+			     double *cent, int *nfaces, int *nbr, int max_nbr);
 /* Number of particles stored (the c_loop_all length). */
 int vgpu_num_cells(vgpu_container *h, long long *n);
 

@@ -200,6 +200,32 @@ def find_reference(spec, ids, xyz, r, q, timed=False):
     return out + (secs.value,) if timed else out
 
 
+def ghost_reference(spec, ids, xyz, r, q, qr=None, max_nbr=128):
+    """compute_ghost_cell of the reference library for every row of q:
+    dict of exists, vol, cent, nfaces, nbr (max_nbr per point); None where
+    the library is unavailable."""
+    lib = _load(REF_SO, "ref_compute_neighbor") and _LIBS[REF_SO][0]
+    if not lib:
+        return None
+    fn = lib.ref_compute_ghost_cells
+    fn.restype = C.c_longlong
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+    m = len(q)
+    qq = None if qr is None else np.ascontiguousarray(qr, dtype=np.float64)
+    o = {"exists": np.zeros(m, np.int32), "vol": np.zeros(m), "cent": np.zeros((m, 3)),
+         "nfaces": np.zeros(m, np.int32), "nbr": np.zeros((m, max_nbr), np.int32)}
+    k = fn(C.byref(spec), C.c_int(len(ids)), _ptr(ids, C.c_int), _ptr(xyz, C.c_double),
+           None if rr is None else _ptr(rr, C.c_double), C.c_int(m), _ptr(q, C.c_double),
+           None if qq is None else _ptr(qq, C.c_double), _ptr(o["exists"], C.c_int), _ptr(o["vol"], C.c_double),
+           _ptr(o["cent"], C.c_double), _ptr(o["nfaces"], C.c_int), _ptr(o["nbr"], C.c_int), C.c_int(max_nbr))
+    if k < 0:
+        raise RuntimeError("reference compute_ghost_cell: rectangular containers only")
+    return o
+
+
 def compare(a, b, vol_rtol=1e-12):
     """Returns a list of human-readable mismatches between two result dicts
     (empty when identical to the stated tolerance)."""

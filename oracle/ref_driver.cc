@@ -242,6 +242,52 @@ long long ref_find_voronoi_cells(const ref_spec *s, int n, const int *ids, const
 	return nf;
 }
 
+// container(_poly)::compute_ghost_cell (src/container.hh:488-498,710-721)
+// for nq points: return value, volume, centroid, face count and up to
+// max_nbr neighbour ids per point.  Rectangular kinds only (returns -1).
+long long ref_compute_ghost_cells(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r,
+				  int nq, const double *q, const double *qr, int *exists, double *vol, double *cent,
+				  int *nfaces, int *nbr, int max_nbr) {
+	const double *g = s->geom;
+	if (s->kind > 1) return -1;
+	std::vector<int> nb;
+	long long m = 0;
+	if (s->kind == 0) {
+		container con(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
+			      s->periodic[0] != 0, s->periodic[1] != 0, s->periodic[2] != 0, s->init_mem);
+		put_all(con, n, ids, xyz, r);
+		voronoicell_neighbor c(con);
+		for (int i = 0; i < nq; i++) {
+			exists[i] = con.compute_ghost_cell(c, q[3 * i], q[3 * i + 1], q[3 * i + 2]) ? 1 : 0;
+			vol[i] = 0; cent[3 * i] = cent[3 * i + 1] = cent[3 * i + 2] = 0; nfaces[i] = 0;
+			if (!exists[i]) continue;
+			m++;
+			vol[i] = c.volume();
+			c.centroid(cent[3 * i], cent[3 * i + 1], cent[3 * i + 2]);
+			nfaces[i] = c.number_of_faces();
+			c.neighbors(nb);
+			for (int f = 0; f < (int)nb.size() && f < max_nbr; f++) nbr[(size_t)i * max_nbr + f] = nb[f];
+		}
+	} else {
+		container_poly con(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
+				   s->periodic[0] != 0, s->periodic[1] != 0, s->periodic[2] != 0, s->init_mem);
+		put_all_r(con, n, ids, xyz, r);
+		voronoicell_neighbor c(con);
+		for (int i = 0; i < nq; i++) {
+			exists[i] = con.compute_ghost_cell(c, q[3 * i], q[3 * i + 1], q[3 * i + 2], qr[i]) ? 1 : 0;
+			vol[i] = 0; cent[3 * i] = cent[3 * i + 1] = cent[3 * i + 2] = 0; nfaces[i] = 0;
+			if (!exists[i]) continue;
+			m++;
+			vol[i] = c.volume();
+			c.centroid(cent[3 * i], cent[3 * i + 1], cent[3 * i + 2]);
+			nfaces[i] = c.number_of_faces();
+			c.neighbors(nb);
+			for (int f = 0; f < (int)nb.size() && f < max_nbr; f++) nbr[(size_t)i * max_nbr + f] = nb[f];
+		}
+	}
+	return m;
+}
+
 // Times the voronoicell_neighbor loop with volume/centroid/faces/neighbours.
 double ref_time_neighbor_loop(const ref_spec *s, int n, const int *ids, const double *xyz,
 			      const double *r) {

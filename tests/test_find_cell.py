@@ -88,3 +88,57 @@ def test_gpu_find_single_point_and_periodic_class_rejected():
     pc.put_array(np.array([1], np.int32), np.array([[0.5, 0.5, 0.5]]))
     with pytest.raises(voro_amd.VoroGPUError):
         pc.find_voronoi_cells(np.zeros((1, 3)))
+
+
+# ---- compute_ghost_cell ----------------------------------------------------
+
+def test_reference_ghost_equals_appended_particle(oracle_mod):
+    """The reference's ghost cell is the cell the point gets when put as the
+    last particle (what src/container.hh:488-498 does in place)."""
+    O = oracle_mod
+    _ref_or_skip(O)
+    n = 3000
+    xyz = I.uniform_box(31, n, (0, 0, 0), (12, 12, 12))
+    q = I.uniform_box(32, 8, (0, 0, 0), (12, 12, 12))
+    sp = O.make_spec(0, (0, 12, 0, 12, 0, 12), (6, 6, 6))
+    g = O.ghost_reference(sp, np.arange(n, dtype=np.int32), xyz, None, q)
+    for i in range(len(q)):
+        a = O.run("reference", sp, np.arange(n + 1, dtype=np.int32), np.vstack([xyz, q[i:i + 1]]))
+        m = int(np.nonzero(a["id"] == n)[0][0])
+        assert a["vol"][m] == g["vol"][i] and a["nfaces"][m] == g["nfaces"][i]
+        assert np.array_equal(a["nbr"][a["nbr_off"][m]:a["nbr_off"][m + 1]], g["nbr"][i, :g["nfaces"][i]])
+
+
+GHOSTS = {
+    "plain": lambda: (0, (0, 20, 0, 20, 0, 20), (10, 10, 10), (0, 0, 0),
+                      I.uniform_box(41, 6000, (0, 0, 0), (20, 20, 20)), None,
+                      I.uniform_box(42, 3000, (-1, -1, -1), (21, 21, 21)), None),
+    "poly": lambda: (1, (0, 20, 0, 20, 0, 20), (10, 10, 10), (0, 0, 0),
+                     I.uniform_box(43, 6000, (0, 0, 0), (20, 20, 20)), I.radii_lognormal(43, 6000),
+                     I.uniform_box(44, 3000, (0, 0, 0), (20, 20, 20)), I.radii_lognormal(44, 3000) * 1.5),
+    "periodic": lambda: (0, (0, 20, 0, 20, 0, 20), (10, 10, 10), (1, 1, 0),
+                         I.uniform_box(45, 6000, (0, 0, 0), (20, 20, 20)), None,
+                         I.uniform_box(46, 3000, (-30, -30, 0), (50, 50, 20)), None),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", sorted(GHOSTS))
+def test_gpu_ghost_cells_match_reference(oracle_mod, case):
+    O = oracle_mod
+    _ref_or_skip(O)
+    kind, geom, grid, per, xyz, r, q, qr = GHOSTS[case]()
+    ids = np.arange(len(xyz), dtype=np.int32) * 5 + 2
+    print("ghost case %s n=%d points=%d" % (case, len(xyz), len(q)), flush=True)
+    want = O.ghost_reference(O.make_spec(kind, geom, grid, periodic=per), ids, xyz, r, q, qr)
+    cls = voro_amd.container_poly if kind == 1 else voro_amd.container
+    con = cls(*geom, *grid, *[bool(p) for p in per], 8)
+    con.put_array(ids, xyz, r)
+    got = con.compute_ghost_cells(q, qr)
+    assert np.array_equal(got["exists"], want["exists"])
+    e = want["exists"] == 1
+    for k in ("vol", "cent", "nfaces"):
+        assert np.array_equal(got[k][e], want[k][e]), k
+    for i in np.nonzero(e)[0]:
+        nf = want["nfaces"][i]
+        assert np.array_equal(got["nbr"][i, :nf], want["nbr"][i, :nf]), i

@@ -79,6 +79,7 @@ def lib():
         "vgpu_compute_all": (C.c_int, [p, C.c_uint, p]),
         "vgpu_num_cells": (C.c_int, [p, i64p]),
         "vgpu_find_voronoi_cells": (C.c_int, [p, C.c_int, f64p, i32p, i32p, f64p, C.c_int]),
+        "vgpu_compute_ghost_cells": (C.c_int, [p, C.c_int, f64p, f64p, i32p, f64p, f64p, i32p, i32p, C.c_int]),
         "vgpu_get_results": (C.c_int, [p, i32p, i32p, i32p, i32p, f64p, f64p, f64p, i32p, i64p, i32p,
                                        C.c_longlong, i64p]),
         "vgpu_get_stats": (C.c_int, [p, i32p, i32p, i32p]),
@@ -280,6 +281,22 @@ class _Base:
         """The reference's signature in Python form: (found, (rx, ry, rz), pid)."""
         f, pid, pos = self.find_voronoi_cells(np.array([[x, y, z]], np.float64))
         return bool(f[0]), tuple(float(t) for t in pos[0]), int(pid[0])
+
+    def compute_ghost_cells(self, xyz, r=None, max_nbr=128):
+        """compute_ghost_cell (src/container.hh:488-498,710-721) for every row
+        of xyz (rectangular containers; r: radii for radical containers):
+        dict of exists, vol, cent (relative to the point), nfaces and nbr
+        (max_nbr ids per point)."""
+        q = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        m = len(q)
+        rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
+        o = {"exists": np.zeros(m, np.int32), "vol": np.zeros(m), "cent": np.zeros((m, 3)),
+             "nfaces": np.zeros(m, np.int32), "nbr": np.zeros((m, max_nbr), np.int32)}
+        _check(lib().vgpu_compute_ghost_cells(self._h, m, _p(q, C.c_double), None if rr is None else _p(rr, C.c_double),
+                                              _p(o["exists"], C.c_int), _p(o["vol"], C.c_double),
+                                              _p(o["cent"], C.c_double), _p(o["nfaces"], C.c_int),
+                                              _p(o["nbr"], C.c_int), int(max_nbr)), "vgpu_compute_ghost_cells")
+        return o
 
     # -- per-cell access, as voro_compute::compute_cell(c, vl) ----------------
     def compute_cell(self, c, vl):

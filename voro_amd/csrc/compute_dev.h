@@ -347,9 +347,16 @@ struct CandFetch {
 // which under 1% of cells reach; it is therefore recorded as one bit per
 // worklist entry and replayed, in the original order, when the block search
 // starts.
+//
+// ghost (compute_ghost_cell, src/container.hh:488-498,710-721): the cell of a
+// point that is not stored, as if put last in block ijk (s = the block's
+// count) with the container's maximum radius ghost_maxr; a block search that
+// comes back to block ijk (where the reference would also meet the point's
+// own periodic image) ends as an overflow.
 template<class CELL, bool POLY, bool PRD = false>
 VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci, int cj, int ck,
-			     unsigned stamp, CellStats &stt, unsigned long long key0 = 0) {
+			     unsigned stamp, CellStats &stt, unsigned long long key0 = 0, const dbl4 *ghost = nullptr,
+			     double ghost_maxr = 0) {
 	double x, y, z, qx = 0, qy = 0, qz = 0;
 	int i, j, k, di, dj, dk, ei, ej, ek, f = 0, g = 0, disp = 0;
 	double fx = 0, fy = 0, fz = 0, gxs = 0, gys = 0, gzs = 0;
@@ -358,7 +365,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 	stt.n_planes = 0; stt.wl_exit = 0; stt.phase = 1; stt.img = 0;
 
 	const int b0 = v.bstart[ijk], co0 = v.bstart[ijk + 1] - b0;
-	const dbl4 me = v.P[b0 + s];
+	const dbl4 me = ghost ? *ghost : v.P[b0 + s];
 	x = me.x; y = me.y; z = me.z;
 	if (PRD) {
 		c.init_unit(v.unit, v.unit_p);
@@ -373,7 +380,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 		disp = ijk - i - v.nx * (j + v.ny * k);
 	}
 	RadPolicy<POLY> rp;
-	rp.init(me.w, v.max_radius);
+	rp.init(me.w, ghost ? ghost_maxr : v.max_radius);
 	BlockSearch<CELL> bs(c, stamp, i, j, k);
 	CandFetch<CELL, PRD> cf;
 	cf.block(b0, b0 + co0);
@@ -531,6 +538,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 					ek = (int)((q >> 14) & 127) - 64 + k;
 					const int rijk = PRD ? prd_region(v, ci, cj, ck, ei, ej, ek, qx, key++, stt.img)
 							     : rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
+					if (ghost && rijk == ijk) { c.ovf = true; stt.n_planes = c.n_cuts; return false; }
 					cf.range(v, g1, rijk, bt, be);
 					cf.block(bt, be);
 					x2 = x - qx; y2 = y - qy; z2 = z - qz;
@@ -629,6 +637,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				{
 					const int rijk = PRD ? prd_region(v, ci, cj, ck, ei, ej, ek, qx, key++, stt.img)
 							     : rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
+					if (ghost && rijk == ijk) { c.ovf = true; stt.n_planes = c.n_cuts; return false; }
 					cf.range(v, g, rijk, bt, be);
 					cf.block(bt, be);
 					x2 = x - qx; y2 = y - qy; z2 = z - qz;
@@ -681,6 +690,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 			{
 				const int rijk = PRD ? prd_region(v, ci, cj, ck, ei, ej, ek, qx, key++, stt.img)
 						     : rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
+				if (ghost && rijk == ijk) { c.ovf = true; stt.n_planes = c.n_cuts; return false; }
 				bt = v.bstart[rijk]; be = v.bstart[rijk + 1];
 				cf.block(bt, be);
 				x2 = x - qx; y2 = y - qy; z2 = z - qz;

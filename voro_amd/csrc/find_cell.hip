@@ -7,6 +7,7 @@
 // few searches that leave the smaller windows).
 #include <hip/hip_runtime.h>
 
+#include "batch_dev.h"
 #include "vgpu_cells.h"
 
 namespace vgpu {
@@ -235,6 +236,61 @@ k_find_cells(RectView v, int n, const double *__restrict__ xyz, const int *__res
 	}
 }
 
+
+// compute_ghost_cell (src/container.hh:488-498, container_poly's
+// src/container.hh:710-721) for n points, one per lane, with the general
+// per-lane cell (up to 1,024 vertices): the point is located as put() would
+// (put_remap, dropped outside a non-periodic direction: exists 0), then its
+// cell is computed as that of a particle stored last in its block, with the
+// container's maximum radius raised to the point's own (radical).  exists:
+// 1 / 0 as the reference's return value, -1 if the cell could not be
+// computed here (capacity, or a search that comes back to the point's own
+// block); nbr holds up to max_nbr neighbour ids per point.
+template<bool POLY>
+__global__ void __launch_bounds__(64, 2)
+k_ghost_cells(RectView v, RectGeom geo, int n, const double *__restrict__ xyz, const double *__restrict__ rad,
+	      int *__restrict__ exists, double *__restrict__ vol, double *__restrict__ cent, int *__restrict__ nfaces,
+	      int *__restrict__ nbr, int max_nbr, int *__restrict__ counter, int *__restrict__ failed,
+	      char *__restrict__ ws, size_t slab, unsigned stamp_base) {
+	typedef typename TierCell<TierWide>::type CELL;
+	const int lane = threadIdx.x;
+	CELL c;
+	c.setup(ws + (size_t)blockIdx.x * slab, lane, v.max_len_sq);
+	for (;;) {
+		int base = 0;
+		if (lane == 0) base = atomicAdd(counter, 64);
+		base = __builtin_amdgcn_readfirstlane(base);
+		if (base >= n) break;
+		const int t = base + lane;
+		if (t >= n) continue;
+		double x = xyz[3 * t], y = xyz[3 * t + 1], z = xyz[3 * t + 2];
+		const int ijk = locate_rect(geo, x, y, z);
+		if (ijk < 0) { exists[t] = 0; nfaces[t] = 0; continue; }
+		const double r = POLY ? rad[t] : 0.5;
+		const dbl4 gp{x, y, z, r};
+		const double maxr = POLY && r > v.max_radius ? r : v.max_radius;
+		const int ci = ijk % v.nx, cj = (ijk / v.nx) % v.ny, ck = ijk / v.nxy;
+		const int s = v.bstart[ijk + 1] - v.bstart[ijk];
+		CellStats stt;
+		c.ovf = false; c.fatal = false;
+		const bool ok = compute_cell_rect<CELL, POLY>(c, v, ijk, s, ci, cj, ck, stamp_base + (unsigned)t + 1u, stt, 0,
+							      &gp, maxr);
+		double cv = 0, cx = 0, cy = 0, cz = 0;
+		int nf = 0;
+		if (!c.ovf && ok) {
+			SlabSink<CELL> sk{&c};
+			c.outputs(cv, cx, cy, cz, nf, sk);
+			if (nf > (int)CELL::L::NBCAP || nf > max_nbr) c.ovf = true;
+		}
+		if (c.ovf) { exists[t] = -1; nfaces[t] = 0; atomicAdd(failed, 1); continue; }
+		exists[t] = ok ? 1 : 0;
+		vol[t] = cv;
+		cent[3 * t] = cx; cent[3 * t + 1] = cy; cent[3 * t + 2] = cz;
+		nfaces[t] = nf;
+		for (int f = 0; f < nf; f++) nbr[(size_t)t * max_nbr + f] = c.template A<int>(CELL::L::o_nbs, f);
+	}
+}
+
 }  // namespace
 
 int launch_find_cells(int tier, bool poly, int waves, hipStream_t s, const RectView &v, int n, const double *xyz,
@@ -254,6 +310,18 @@ int launch_find_cells(int tier, bool poly, int waves, hipStream_t s, const RectV
 		else VGPU_FIND(TierHuge, false);
 	}
 #undef VGPU_FIND
+	return hipGetLastError() == hipSuccess ? 0 : 6;
+}
+
+int launch_ghost_cells(bool poly, int waves, hipStream_t s, const RectView &v, const RectGeom &geo, int n,
+		       const double *xyz, const double *rad, int *exists, double *vol, double *cent, int *nfaces, int *nbr,
+		       int max_nbr, int *counter, int *failed, char *ws, size_t slab, unsigned stamp_base) {
+	if (poly)
+		hipLaunchKernelGGL((k_ghost_cells<true>), dim3(waves), dim3(64), 0, s, v, geo, n, xyz, rad, exists, vol, cent,
+				   nfaces, nbr, max_nbr, counter, failed, ws, slab, stamp_base);
+	else
+		hipLaunchKernelGGL((k_ghost_cells<false>), dim3(waves), dim3(64), 0, s, v, geo, n, xyz, rad, exists, vol, cent,
+				   nfaces, nbr, max_nbr, counter, failed, ws, slab, stamp_base);
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 

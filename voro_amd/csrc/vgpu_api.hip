@@ -110,6 +110,7 @@ struct vgpu_container {
 	DBuf vol, cent, nfaces, status, nbr_start, arena, csr_off, csr, ctr;
 	DBuf retry0, retry1, retry2, retry3, retry4, retry5;
 	DBuf fq_xyz, fq_found, fq_pid, fq_pos, fq_retry, fq_retry2;   // find_voronoi_cell queries
+	DBuf fq_r, fq_vol, fq_nbr;                                   // ghost cells
 	DBuf st_planes, st_exit, st_tier;
 	// learned path
 	bool have_pred = false;
@@ -1018,7 +1019,7 @@ int vgpu_destroy(vgpu_container *h) {
 			&h->bmax, &h->boff, &h->list_a, &h->list_b, &h->d_unit, &h->Pall, &h->pidall, &h->ball, &h->fv,
 			&h->touch, &h->d_flag, &h->retry3, &h->retry4, &h->retry5, &h->img_p, &h->img_id,
 			&h->img_reg, &h->img_rank, &h->img_bc, &h->img_cnt, &h->fq_xyz, &h->fq_found, &h->fq_pid,
-			&h->fq_pos, &h->fq_retry, &h->fq_retry2};
+			&h->fq_pos, &h->fq_retry, &h->fq_retry2, &h->fq_r, &h->fq_vol, &h->fq_nbr};
 	for (DBuf *b : bufs) b->release();
 	h->ws_fast.buf.release(); h->ws_wide.buf.release(); h->ws_huge.buf.release(); h->ws_wave.buf.release();
 	if (h->events) for (int t = 0; t < 5; t++) (void)hipEventDestroy(h->ev[t]);
@@ -1182,6 +1183,50 @@ int vgpu_find_voronoi_cells(vgpu_container *h, int n, const double *xyz, int *fo
 		HIPCHK(hipMemcpyAsync(&failed, ctr + 5, 4, hipMemcpyDeviceToHost, s));
 		HIPCHK(hipStreamSynchronize(s));
 	}
+	return failed > 0 ? VGPU_MEMORY_ERROR : VGPU_OK;
+}
+
+int vgpu_compute_ghost_cells(vgpu_container *h, int n, const double *xyz, const double *r, int *exists, double *vol,
+			     double *cent, int *nfaces, int *nbr, int max_nbr) {
+	if (!h || n < 0 || max_nbr < 1 || (n > 0 && (!xyz || !exists || !vol || !cent || !nfaces || !nbr)) ||
+	    (h->poly && n > 0 && !r))
+		return VGPU_INVALID_ARGUMENT;
+	if (h->prd) return VGPU_INVALID_ARGUMENT;       // rectangular containers only
+	if (n == 0) return VGPU_OK;
+	hipStream_t s = nullptr;
+	int st;
+	if (!h->binned && (st = run_pipeline(h, 0, s, true))) return st;
+	const size_t N = (size_t)n;
+	if ((st = h->fq_xyz.ensure(N * 24)) || (st = h->fq_r.ensure(N * 8)) || (st = h->fq_found.ensure(N * 4)) ||
+	    (st = h->fq_vol.ensure(N * 8)) || (st = h->fq_pos.ensure(N * 24)) || (st = h->fq_pid.ensure(N * 4)) ||
+	    (st = h->fq_nbr.ensure(N * (size_t)max_nbr * 4)) || (st = h->d_counters.ensure(64 * 4)))
+		return st;
+	HIPCHK(hipMemcpyAsync(h->fq_xyz.p, xyz, N * 24, hipMemcpyHostToDevice, s));
+	if (h->poly) HIPCHK(hipMemcpyAsync(h->fq_r.p, r, N * 8, hipMemcpyHostToDevice, s));
+	typedef Layout<TierWide, 64> LW;
+	const int waves = std::max(1, std::min(64, nblk((long long)n, 64)));
+	if ((st = h->ws_wide.ensure(LW::bytes, std::max(waves, h->ws_wide.waves)))) return st;
+	if ((unsigned long long)h->ws_wide.stamp_base + N + 2 >= 0xffffffffull) {
+		HIPCHK(hipMemset(h->ws_wide.buf.p, 0, h->ws_wide.buf.n));
+		h->ws_wide.stamp_base = 0;
+	}
+	int *ctr = h->d_counters.as<int>();
+	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 8, 0);
+	const RectView v = make_view(h);
+	if (launch_ghost_cells(h->poly, waves, s, v, h->g, n, h->fq_xyz.as<double>(), h->fq_r.as<double>(),
+			       h->fq_found.as<int>(), h->fq_vol.as<double>(), h->fq_pos.as<double>(), h->fq_pid.as<int>(),
+			       h->fq_nbr.as<int>(), max_nbr, ctr, ctr + 1, h->ws_wide.base(), h->ws_wide.slab,
+			       h->ws_wide.stamp_base))
+		return VGPU_HIP_ERROR;
+	h->ws_wide.stamp_base += (unsigned)n + 1u;
+	int failed = 0;
+	HIPCHK(hipMemcpyAsync(exists, h->fq_found.p, N * 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(vol, h->fq_vol.p, N * 8, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(cent, h->fq_pos.p, N * 24, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(nfaces, h->fq_pid.p, N * 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(nbr, h->fq_nbr.p, N * (size_t)max_nbr * 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(&failed, ctr + 1, 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
 	return failed > 0 ? VGPU_MEMORY_ERROR : VGPU_OK;
 }
 
