@@ -226,6 +226,22 @@ def ghost_reference(spec, ids, xyz, r, q, qr=None, max_nbr=128):
     return o
 
 
+def print_custom_reference(spec, ids, xyz, r, fmt, filename):
+    """The reference library's print_custom(format, filename) for this
+    container; False where the library is unavailable."""
+    lib = _load(REF_SO, "ref_compute_neighbor") and _LIBS[REF_SO][0]
+    if not lib:
+        return False
+    fn = lib.ref_print_custom
+    fn.restype = C.c_longlong
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
+    m = fn(C.byref(spec), C.c_int(len(ids)), _ptr(ids, C.c_int), _ptr(xyz, C.c_double),
+           None if rr is None else _ptr(rr, C.c_double), fmt.encode(), filename.encode())
+    return m == 0
+
+
 def compare(a, b, vol_rtol=1e-12):
     """Returns a list of human-readable mismatches between two result dicts
     (empty when identical to the stated tolerance)."""

@@ -288,6 +288,26 @@ long long ref_compute_ghost_cells(const ref_spec *s, int n, const int *ids, cons
 	return m;
 }
 
+// container(_poly)::print_custom(format, filename) (src/container.cc:401-432).
+// Rectangular kinds only (returns -1).
+long long ref_print_custom(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r,
+			   const char *format, const char *filename) {
+	const double *g = s->geom;
+	if (s->kind > 1) return -1;
+	if (s->kind == 0) {
+		container con(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
+			      s->periodic[0] != 0, s->periodic[1] != 0, s->periodic[2] != 0, s->init_mem);
+		put_all(con, n, ids, xyz, r);
+		con.print_custom(format, filename);
+	} else {
+		container_poly con(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
+				   s->periodic[0] != 0, s->periodic[1] != 0, s->periodic[2] != 0, s->init_mem);
+		put_all_r(con, n, ids, xyz, r);
+		con.print_custom(format, filename);
+	}
+	return 0;
+}
+
 // Times the voronoicell_neighbor loop with volume/centroid/faces/neighbours.
 double ref_time_neighbor_loop(const ref_spec *s, int n, const int *ids, const double *xyz,
 			      const double *r) {

@@ -142,3 +142,39 @@ def test_gpu_ghost_cells_match_reference(oracle_mod, case):
     for i in np.nonzero(e)[0]:
         nf = want["nfaces"][i]
         assert np.array_equal(got["nbr"][i, :nf], want["nbr"][i, :nf]), i
+
+
+# ---- print_custom and import ---------------------------------------------
+
+FORMAT = "%i %q %v %s %n %c %C 100%% %z"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", [0, 1])
+def test_gpu_print_custom_matches_reference_text(oracle_mod, tmp_path, kind):
+    """print_custom's text, byte for byte, for the codes the engine's stored
+    outputs carry; the particles come in through import_()."""
+    O = oracle_mod
+    _ref_or_skip(O)
+    n = 2000
+    xyz = I.uniform_box(51 + kind, n, (0, 0, 0), (10, 10, 10))
+    r = I.radii_lognormal(51, n) * 0.5 if kind else None
+    ids = np.arange(n, dtype=np.int32) + 11
+    src = tmp_path / "pack.txt"
+    with open(src, "w") as f:
+        for t in range(n):
+            f.write("%d %.17g %.17g %.17g%s\n" % (ids[t], *xyz[t], (" %.17g" % r[t]) if kind else ""))
+    want = tmp_path / "ref.txt"
+    fmt = FORMAT if kind else FORMAT + " %r"
+    assert O.print_custom_reference(O.make_spec(kind, (0, 10, 0, 10, 0, 10), (5, 5, 5)), ids, xyz, r, fmt, str(want))
+    cls = voro_amd.container_poly if kind else voro_amd.container
+    con = cls(0, 10, 0, 10, 0, 10, 5, 5, 5, False, False, False, 8)
+    con.import_(str(src))
+    assert con.print_custom(fmt) == want.read_text()
+
+
+def test_print_custom_rejects_geometry_codes():
+    con = voro_amd.container.__new__(voro_amd.container)
+    con.kind = voro_amd.KIND_CONTAINER
+    with pytest.raises(ValueError):
+        voro_amd.container.print_custom(con, "%i %f")

@@ -29,6 +29,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libvoro_gpu.so")
 
 OK = 0
+FILE_ERROR = 1
 MEMORY_ERROR = 2
 INTERNAL_ERROR = 3
 INVALID_ARGUMENT = 5
@@ -262,6 +263,86 @@ class _Base:
         t = np.zeros(n.value, np.int32)
         _check(lib().vgpu_get_stats(self._h, _p(a, C.c_int), _p(b, C.c_int), _p(t, C.c_int)), "vgpu_get_stats")
         return {"n_planes": a, "wl_exit": b, "tier": t}
+
+    # -- text formats: print_custom and import ---------------------------------
+    _CUSTOM_UNSUPPORTED = set("wpPomgEeFAaftl")
+
+    def print_custom(self, format, filename=None, flags=F_NEIGHBORS):
+        """container::print_custom (src/container.cc:401-432, the per-cell line
+        of src/cell.cc:2485-2558) from the stored results: one line per
+        computed cell in loop order.  Codes %i %x %y %z %q %r %s %n %v %c %C;
+        the vertex, edge and face-geometry codes (%w %p %P %o %m %g %E %e %F
+        %A %a %f %t %l) need the cell's vertex tables, which this engine does
+        not keep, and raise ValueError.  Returns the text, or writes it to
+        filename."""
+        codes = [format[k + 1] for k in range(len(format) - 1) if format[k] == "%"]
+        bad = sorted(set(codes) & self._CUSTOM_UNSUPPORTED)
+        if bad:
+            raise ValueError("print_custom: unsupported codes %s" % " ".join("%" + c for c in bad))
+        if "r" in codes and self.kind in (KIND_POLY, KIND_PERIODIC_POLY):
+            raise ValueError("print_custom: %r needs the stored radii, not kept for radical containers")
+        if self._res is None or ("n" in codes and not (self._flags & F_NEIGHBORS)):
+            self.compute_all_cells(flags | (F_NEIGHBORS if "n" in codes else 0))
+        r = self.results()
+        out = []
+        for m in np.nonzero(r["exists"])[0]:
+            x, y, z = (float(t) for t in r["pos"][m])
+            line = []
+            k = 0
+            while k < len(format):
+                ch = format[k]
+                if ch != "%":
+                    line.append(ch)
+                    k += 1
+                    continue
+                k += 1
+                if k >= len(format):
+                    break                   # a trailing '%' prints nothing
+                c = format[k]
+                if c == "i": line.append("%d" % r["id"][m])
+                elif c == "x": line.append("%g" % x)
+                elif c == "y": line.append("%g" % y)
+                elif c == "z": line.append("%g" % z)
+                elif c == "q": line.append("%g %g %g" % (x, y, z))
+                elif c == "r": line.append("%g" % 0.5)
+                elif c == "s": line.append("%d" % r["nfaces"][m])
+                elif c == "n":
+                    line.append(" ".join("%d" % t for t in r["nbr"][r["nbr_off"][m]:r["nbr_off"][m + 1]]))
+                elif c == "v": line.append("%g" % r["vol"][m])
+                elif c == "c": line.append("%g %g %g" % tuple(float(t) for t in r["cent"][m]))
+                elif c == "C":
+                    cx, cy, cz = (float(t) for t in r["cent"][m])
+                    line.append("%g %g %g" % (x + cx, y + cy, z + cz))
+                else: line.append("%" + c)
+                k += 1
+            out.append("".join(line) + "\n")
+        text = "".join(out)
+        if filename is None:
+            return text
+        with open(filename, "w") as f:
+            f.write(text)
+
+    def import_(self, filename):
+        """container::import (src/container.cc:333-376): reads "id x y z" lines
+        ("id x y z r" for radical containers) and puts each particle; a line
+        that does not parse ends the file with VoroGPUError(FILE_ERROR), as
+        the reference's "File import error"."""
+        poly = self.kind in (KIND_POLY, KIND_PERIODIC_POLY)
+        width = 5 if poly else 4
+        ids, xyz, rr = [], [], []
+        with open(filename) as f:
+            tok = f.read().split()
+        for k in range(0, len(tok) - len(tok) % width, width):
+            try:
+                i = int(tok[k]); v = [float(t) for t in tok[k + 1:k + width]]
+            except ValueError:
+                raise VoroGPUError(FILE_ERROR, "import")
+            ids.append(i); xyz.append(v[:3])
+            if poly: rr.append(v[3])
+        if len(tok) % width:
+            raise VoroGPUError(FILE_ERROR, "import")
+        if ids:
+            self.put_array(np.array(ids, np.int32), np.array(xyz, np.float64), np.array(rr) if poly else None)
 
     # -- point location, as container::find_voronoi_cell ----------------------
     def find_voronoi_cells(self, xyz):
