@@ -105,6 +105,12 @@ int vgpu_find_voronoi_cells(vgpu_container *h, int n, const double *xyz, int *fo
  * a periodic image. */
 int vgpu_compute_ghost_cells(vgpu_container *h, int n, const double *xyz, const double *r, int *exists, double *vol,
 			     double *cent, int *nfaces, int *nbr, int max_nbr);
+/* For each stored particle in loop order, its position in the sequence of
+ * put calls (0-based, counting every particle given to vgpu_put /
+ * vgpu_put_device since the last vgpu_clear, dropped ones included): the
+ * record a particle_order keeps (src/c_loops.hh:35-75) turned around, for
+ * c_loop_order.  Valid after a compute. */
+int vgpu_get_put_index(vgpu_container *h, int *put_index);
 /* Number of particles stored (the c_loop_all length). */
 int vgpu_num_cells(vgpu_container *h, long long *n);
 

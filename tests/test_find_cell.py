@@ -178,3 +178,52 @@ def test_print_custom_rejects_geometry_codes():
     con.kind = voro_amd.KIND_CONTAINER
     with pytest.raises(ValueError):
         voro_amd.container.print_custom(con, "%i %f")
+
+
+# ---- particle_order / c_loop_order, pre_container --------------------------
+
+def test_pre_container_guess_optimal():
+    pc = voro_amd.pre_container(0, 10, 0, 20, 0, 5, False, False, True)
+    for t, p in enumerate(I.uniform_box(61, 1000, (0, 0, -3), (10, 20, 8))):
+        pc.put(t, *p)
+    # z periodic: every particle kept; the grid of src/pre_container.cc:57-63
+    assert pc.total_particles() == 1000
+    s = (1000 / (5.6 * 10 * 20 * 5)) ** (1 / 3.0)
+    assert pc.guess_optimal() == (int(10 * s + 1), int(20 * s + 1), int(5 * s + 1))
+    pc2 = voro_amd.pre_container(0, 1, 0, 1, 0, 1, False, False, False)
+    pc2.put(0, 0.5, 0.5, 1.5)
+    assert pc2.total_particles() == 0
+
+
+@pytest.mark.gpu
+def test_gpu_loop_order_and_pre_container():
+    xyz = I.uniform_box(62, 4000, (-1, 0, 0), (11, 10, 10))     # some outside in x: dropped
+    ids = np.arange(4000, dtype=np.int32) + 100
+    con = voro_amd.container(0, 10, 0, 10, 0, 10, 5, 5, 5, False, False, False, 8)
+    po = voro_amd.particle_order()
+    con.put_array(ids[:1000], xyz[:1000])
+    con.put_array(ids[1000:], xyz[1000:], order=po)
+    con.compute_all_cells()
+    vl = voro_amd.c_loop_order(con, po)
+    seen = []
+    if vl.start():
+        while True:
+            seen.append(vl.pid())
+            if not vl.inc():
+                break
+    inside = (xyz[:, 0] >= 0) & (xyz[:, 0] < 10)
+    assert seen == [int(i) for i, k in zip(ids[1000:], inside[1000:]) if k]
+    # pre_container -> container: the same cells as putting directly
+    pc = voro_amd.pre_container(0, 10, 0, 10, 0, 10, False, False, False)
+    for i, p in zip(ids, xyz):
+        pc.put(int(i), *p)
+    nx, ny, nz = pc.guess_optimal()
+    a = voro_amd.container(0, 10, 0, 10, 0, 10, nx, ny, nz, False, False, False, 8)
+    pc.setup(a)
+    a.compute_all_cells()
+    b = voro_amd.container(0, 10, 0, 10, 0, 10, nx, ny, nz, False, False, False, 8)
+    b.put_array(ids[inside], xyz[inside])
+    b.compute_all_cells()
+    ra, rb = a.results(), b.results()
+    for k in ("id", "vol", "nbr"):
+        assert np.array_equal(ra[k], rb[k]), k

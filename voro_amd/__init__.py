@@ -80,6 +80,7 @@ def lib():
         "vgpu_compute_all": (C.c_int, [p, C.c_uint, p]),
         "vgpu_num_cells": (C.c_int, [p, i64p]),
         "vgpu_find_voronoi_cells": (C.c_int, [p, C.c_int, f64p, i32p, i32p, f64p, C.c_int]),
+        "vgpu_get_put_index": (C.c_int, [p, i32p]),
         "vgpu_compute_ghost_cells": (C.c_int, [p, C.c_int, f64p, f64p, i32p, f64p, f64p, i32p, i32p, C.c_int]),
         "vgpu_get_results": (C.c_int, [p, i32p, i32p, i32p, i32p, f64p, f64p, f64p, i32p, i64p, i32p,
                                        C.c_longlong, i64p]),
@@ -160,13 +161,18 @@ class _Base:
             self._h = None
 
     # -- particles ------------------------------------------------------
-    def put_array(self, ids, xyz, r=None):
-        """Appends particles in order, as repeated put() calls would."""
+    def put_array(self, ids, xyz, r=None, order=None):
+        """Appends particles in order, as repeated put() calls would; with a
+        particle_order, as put(po, ...) calls (src/container.hh:294-296)."""
         ids = np.ascontiguousarray(ids, dtype=np.int32)
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
         _check(lib().vgpu_put(self._h, len(ids), _p(ids, C.c_int), _p(xyz, C.c_double), _p(rr, C.c_double)),
                "vgpu_put")
+        first = getattr(self, "_nput", 0)
+        self._nput = first + len(ids)
+        if order is not None:
+            order.add(self, first, len(ids))
         self._res = None
 
     def put_device(self, n, ids_ptr, xyz_ptr, r_ptr=None):
@@ -174,10 +180,12 @@ class _Base:
         (raw device pointers, e.g. torch tensor data_ptr())."""
         _check(lib().vgpu_put_device(self._h, int(n), C.c_void_p(ids_ptr), C.c_void_p(xyz_ptr),
                                      C.c_void_p(r_ptr) if r_ptr else None), "vgpu_put_device")
+        self._nput = int(n)
         self._res = None
 
     def clear(self):
         _check(lib().vgpu_clear(self._h), "vgpu_clear")
+        self._nput = 0
         self._res = None
 
     def add_wall(self, wall):
@@ -443,6 +451,100 @@ class wall_sphere:
 
     def __init__(self, xc, yc, zc, rc, w_id=-99):
         self.xc, self.yc, self.zc, self.rc, self.w_id = float(xc), float(yc), float(zc), float(rc), int(w_id)
+
+
+class particle_order:
+    """particle_order (src/c_loops.hh:35-75): the particles put through
+    container.put_array(..., order=po), in put order."""
+
+    def __init__(self):
+        self._con = None
+        self._seq = []          # put-sequence numbers
+
+    def add(self, con, first, count):
+        if self._con is not None and self._con is not con:
+            raise ValueError("a particle_order records the puts of one container")
+        self._con = con
+        self._seq.extend(range(first, first + count))
+
+
+class c_loop_order:
+    """c_loop_order (src/c_loops.hh:271-317): the stored particles a
+    particle_order recorded, in record order (particles the container
+    dropped are not stored, so not visited)."""
+
+    def __init__(self, con, vo):
+        self.con = con
+        res = con.results()
+        m = len(res["id"])
+        put = np.zeros(max(m, 1), np.int32)
+        _check(lib().vgpu_get_put_index(con._h, _p(put, C.c_int)), "vgpu_get_put_index")
+        where = {int(put[k]): k for k in range(m)}
+        self._idx = [where[s] for s in vo._seq if s in where]
+        self._k = 0
+        self._m = 0
+
+    def start(self):
+        self._k = 0
+        if not self._idx:
+            return False
+        self._m = self._idx[0]
+        return True
+
+    def inc(self):
+        self._k += 1
+        if self._k >= len(self._idx):
+            return False
+        self._m = self._idx[self._k]
+        return True
+
+    def pid(self):
+        return int(self.con.results()["id"][self._m])
+
+    def pos(self):
+        return tuple(self.con.results()["pos"][self._m])
+
+    @property
+    def ijk(self):
+        return int(self.con.results()["ijk"][self._m])
+
+    @property
+    def q(self):
+        return int(self.con.results()["q"][self._m])
+
+
+class pre_container:
+    """pre_container (src/pre_container.hh, src/pre_container.cc): keeps
+    particles before the container exists, guesses its block grid and puts
+    them into it.  radii=True for pre_container_poly."""
+    optimal_particles = 5.6     # src/config.hh:97
+
+    def __init__(self, ax, bx, ay, by, az, bz, xp, yp, zp, radii=False):
+        self.ax, self.bx, self.ay, self.by, self.az, self.bz = ax, bx, ay, by, az, bz
+        self.xp, self.yp, self.zp = bool(xp), bool(yp), bool(zp)
+        self.radii = radii
+        self.ids, self.xyz, self.r = [], [], []
+
+    def put(self, n, x, y, z, r=None):
+        # src/pre_container.cc:71-79 (out-of-bounds particles are not kept)
+        if ((self.xp or (self.ax <= x <= self.bx)) and (self.yp or (self.ay <= y <= self.by)) and
+                (self.zp or (self.az <= z <= self.bz))):
+            self.ids.append(int(n)); self.xyz.append((x, y, z))
+            if self.radii: self.r.append(r)
+
+    def total_particles(self):
+        return len(self.ids)
+
+    def guess_optimal(self):
+        """(nx, ny, nz), src/pre_container.cc:57-63."""
+        dx, dy, dz = self.bx - self.ax, self.by - self.ay, self.bz - self.az
+        ilscale = (self.total_particles() / (self.optimal_particles * dx * dy * dz)) ** (1 / 3.0)
+        return int(dx * ilscale + 1), int(dy * ilscale + 1), int(dz * ilscale + 1)
+
+    def setup(self, con, order=None):
+        if self.ids:
+            con.put_array(np.array(self.ids, np.int32), np.array(self.xyz, np.float64),
+                          np.array(self.r, np.float64) if self.radii else None, order=order)
 
 
 class c_loop_all:

@@ -1230,6 +1230,16 @@ int vgpu_compute_ghost_cells(vgpu_container *h, int n, const double *xyz, const 
 	return failed > 0 ? VGPU_MEMORY_ERROR : VGPU_OK;
 }
 
+int vgpu_get_put_index(vgpu_container *h, int *put_index) {
+	if (!h || !put_index) return VGPU_INVALID_ARGUMENT;
+	if (!h->binned) return VGPU_INTERNAL_ERROR;
+	long long n = 0;
+	int st = vgpu_num_cells(h, &n);
+	if (st) return st;
+	if (n > 0) HIPCHK(hipMemcpy(put_index, h->src.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+	return VGPU_OK;
+}
+
 int vgpu_num_cells(vgpu_container *h, long long *n) {
 	if (!h || !n) return VGPU_INVALID_ARGUMENT;
 	if (!h->computed) return VGPU_INVALID_ARGUMENT;
