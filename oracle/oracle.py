@@ -103,11 +103,27 @@ def make_spec(kind, geom, grid, periodic=(0, 0, 0), spheres=None, init_mem=8):
     return s
 
 
-def run(which, spec, ids, xyz, r=None, stats=False, nbr_per_cell=40):
-    """Runs one oracle; returns a dict of numpy arrays in c_loop_all order."""
+def default_threads():
+    """Host threads for the threaded reference runs (at most 16)."""
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return max(1, min(16, os.cpu_count() or 1))
+
+
+def run(which, spec, ids, xyz, r=None, stats=False, nbr_per_cell=40, threads=None):
+    """Runs one oracle; returns a dict of numpy arrays in c_loop_all order.
+    which = "reference" with threads > 1 runs the reference library on that
+    many threads (z-slabs of blocks, one container copy each; rectangular
+    kinds), with outputs identical to the single-threaded loop."""
     f = restatement() if which == "restatement" else reference()
     if f is None:
         raise RuntimeError("reference oracle not available")
+    mt = which != "restatement" and threads is not None and threads > 1 and int(spec.kind) <= 1
+    if mt:
+        lib = _LIBS[REF_SO][0]
+        fmt = lib.ref_compute_neighbor_mt
+        fmt.restype = C.c_longlong
     n = len(ids)
     ids = np.ascontiguousarray(ids, dtype=np.int32)
     xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
@@ -139,10 +155,12 @@ def run(which, spec, ids, xyz, r=None, stats=False, nbr_per_cell=40):
                 None if rr is None else _ptr(rr, C.c_double), C.byref(o)]
         if which == "restatement":
             args.append(C.byref(st) if st is not None else None)
+        elif mt:
+            args += [C.c_int(threads), None]
         else:
             secs = C.c_double(0)
             args.append(C.byref(secs))
-        m = f(*args)
+        m = (fmt if mt else f)(*args)
         if m < 0:
             raise RuntimeError("oracle reported an internal error (%d)" % m)
         if res["nbr_off"][m] > cap:
@@ -172,6 +190,23 @@ def time_reference(spec, ids, xyz, r=None, loop="compute_all_cells"):
     rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
     return fn(C.byref(spec), C.c_int(len(ids)), _ptr(ids, C.c_int), _ptr(xyz, C.c_double),
               None if rr is None else _ptr(rr, C.c_double))
+
+
+def time_reference_mt(spec, ids, xyz, r=None, threads=None, loop="compute_all_cells"):
+    """Wall seconds of the reference on `threads` host threads (z-slabs of
+    blocks, one container copy each, copies built before the clock starts):
+    the compute_all_cells() body, or the voronoicell_neighbor output loop."""
+    lib = _load(REF_SO, "ref_compute_neighbor") and _LIBS[REF_SO][0]
+    if not lib:
+        return None
+    fn = lib.ref_time_mt
+    fn.restype = C.c_double
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
+    t = threads or default_threads()
+    return fn(C.byref(spec), C.c_int(len(ids)), _ptr(ids, C.c_int), _ptr(xyz, C.c_double),
+              None if rr is None else _ptr(rr, C.c_double), C.c_int(t), C.c_int(1 if loop == "compute_all_cells" else 2))
 
 
 def find_reference(spec, ids, xyz, r, q, timed=False):

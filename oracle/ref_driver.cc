@@ -17,8 +17,10 @@
 //   compute_cell(c,vl)                    src/container.hh:463-465
 //   voronoicell_neighbor outputs          src/cell.cc:1584,1861,2168,2660
 //   compute_all_cells()                   src/container.cc:438-443
+#include <atomic>
 #include <chrono>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "voro++.hh"
@@ -313,6 +315,260 @@ double ref_time_neighbor_loop(const ref_spec *s, int n, const int *ids, const do
 			      const double *r) {
 	double secs = 0;
 	dispatch(s, n, ids, xyz, r, 0, &secs, 2);
+	return secs;
+}
+
+
+}  // extern "C"
+
+namespace {
+
+// ---- Threaded runs: one container copy per thread, each thread computing
+// the cells of its own z-slab of blocks through c_loop_subset::setup_intbox
+// (src/c_loops.cc:44-50) -- the all-core harness of the CPU baseline.  The
+// reference has no threading of its own; separate container objects share
+// nothing (its scratch lives in each container, src/v_compute.hh:103-119).
+// A cell of a rectangular container depends only on the particles, so the
+// threaded outputs equal c_loop_all's, in the same order.
+
+// z-slab boundaries [kb[t], kb[t+1]) balanced by particle count
+std::vector<int> slab_bounds(const int *co, int nx, int ny, int nz, int T) {
+	std::vector<long long> layer(nz, 0);
+	long long tot = 0;
+	for (int k = 0; k < nz; k++) {
+		for (int b = 0; b < nx * ny; b++) layer[k] += co[k * nx * ny + b];
+		tot += layer[k];
+	}
+	std::vector<int> kb(T + 1, nz);
+	kb[0] = 0;
+	long long acc = 0;
+	int t = 1;
+	for (int k = 0; k < nz && t < T; k++) {
+		acc += layer[k];
+		while (t < T && acc * T >= tot * t) kb[t++] = k + 1;
+	}
+	for (int u = 1; u <= T; u++) if (kb[u] < kb[u - 1]) kb[u] = kb[u - 1];
+	return kb;
+}
+
+// Releases every thread at once after all have built their containers.
+struct Gate {
+	std::atomic<int> ready{0};
+	std::atomic<int> go{0};
+	void arrive_and_wait() {
+		ready.fetch_add(1);
+		while (!go.load()) std::this_thread::yield();
+	}
+};
+
+template<class c_class>
+void slab_outputs(c_class &con, int k0, int k1, long long m0, ref_out *o, std::vector<int> &nb,
+		  std::vector<long long> &cnt, double &sink) {
+	voronoicell_neighbor c(con);
+	c_loop_subset vl(con);
+	std::vector<int> v;
+	if (k1 <= k0) return;
+	vl.setup_intbox(0, con.nx - 1, 0, con.ny - 1, k0, k1 - 1);
+	long long m = m0;
+	if (vl.start()) do {
+		const bool ok = con.compute_cell(c, vl);
+		if (!o) {
+			if (ok) {
+				double cx, cy, cz;
+				c.centroid(cx, cy, cz);
+				c.neighbors(v);
+				sink += c.volume() + cx + c.number_of_faces() + (double)v.size();
+			}
+			continue;
+		}
+		o->id[m] = vl.pid();
+		o->ijk[m] = vl.ijk;
+		o->q[m] = vl.q;
+		o->exists[m] = ok ? 1 : 0;
+		if (o->pos) {
+			double x, y, z;
+			vl.pos(x, y, z);
+			o->pos[3 * m] = x; o->pos[3 * m + 1] = y; o->pos[3 * m + 2] = z;
+		}
+		if (ok) {
+			o->vol[m] = c.volume();
+			double cx, cy, cz;
+			c.centroid(cx, cy, cz);
+			o->cent[3 * m] = cx; o->cent[3 * m + 1] = cy; o->cent[3 * m + 2] = cz;
+			o->nfaces[m] = c.number_of_faces();
+			c.neighbors(v);
+			nb.insert(nb.end(), v.begin(), v.end());
+			cnt.push_back((long long)v.size());
+		} else {
+			o->vol[m] = 0; o->cent[3 * m] = o->cent[3 * m + 1] = o->cent[3 * m + 2] = 0;
+			o->nfaces[m] = 0;
+			cnt.push_back(0);
+		}
+		m++;
+	} while (vl.inc());
+}
+
+// compute_all_cells() restricted to a slab (the same loop body,
+// src/container.cc:438-443).
+template<class c_class>
+void slab_compute(c_class &con, int k0, int k1) {
+	if (k1 <= k0) return;
+	voronoicell c(con);
+	c_loop_subset vl(con);
+	vl.setup_intbox(0, con.nx - 1, 0, con.ny - 1, k0, k1 - 1);
+	if (vl.start()) do con.compute_cell(c, vl);
+	while (vl.inc());
+}
+
+template<class c_class>
+c_class *make_rect(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r,
+		   std::vector<wall_sphere*> &keep);
+template<>
+container *make_rect<container>(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r,
+				std::vector<wall_sphere*> &keep) {
+	const double *g = s->geom;
+	container *con = new container(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
+				       s->periodic[0] != 0, s->periodic[1] != 0, s->periodic[2] != 0, s->init_mem);
+	add_spheres(*con, s, keep);
+	put_all(*con, n, ids, xyz, r);
+	return con;
+}
+template<>
+container_poly *make_rect<container_poly>(const ref_spec *s, int n, const int *ids, const double *xyz,
+					  const double *r, std::vector<wall_sphere*> &keep) {
+	const double *g = s->geom;
+	container_poly *con = new container_poly(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1],
+						 s->grid[2], s->periodic[0] != 0, s->periodic[1] != 0,
+						 s->periodic[2] != 0, s->init_mem);
+	add_spheres(*con, s, keep);
+	put_all_r(*con, n, ids, xyz, r);
+	return con;
+}
+
+// mode 0: outputs into o; mode 1: compute_all_cells-equivalent timing;
+// mode 2: neighbour-loop timing.  Returns the loop length.
+template<class c_class>
+long long threaded_rect(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r, ref_out *o,
+			int T, double *secs, int mode) {
+	std::vector<wall_sphere*> keep;
+	// the block occupancy decides the slabs and each slab's first loop index
+	c_class *probe = make_rect<c_class>(s, n, ids, xyz, r, keep);
+	const int nx = probe->nx, ny = probe->ny, nz = probe->nz;
+	std::vector<int> kb = slab_bounds(probe->co, nx, ny, nz, T);
+	std::vector<long long> first(T + 1, 0);
+	for (int t = 0; t < T; t++) {
+		long long a = 0;
+		for (int b = kb[t] * nx * ny; b < kb[t + 1] * nx * ny; b++) a += probe->co[b];
+		first[t + 1] = first[t] + a;
+	}
+	std::vector<std::vector<int>> nbs(T);
+	std::vector<std::vector<long long>> cnts(T);
+	std::vector<c_class*> cons(T, nullptr);
+	cons[0] = probe;
+	std::vector<std::vector<wall_sphere*>> keeps(T);
+	Gate gate;
+	std::vector<double> sinks(T, 0);
+	std::vector<std::thread> th;
+	for (int t = 0; t < T; t++) {
+		th.emplace_back([&, t]() {
+			if (t > 0) cons[t] = make_rect<c_class>(s, n, ids, xyz, r, keeps[t]);
+			gate.arrive_and_wait();
+			if (mode == 1) slab_compute(*cons[t], kb[t], kb[t + 1]);
+			else slab_outputs(*cons[t], kb[t], kb[t + 1], first[t], mode == 0 ? o : 0, nbs[t], cnts[t], sinks[t]);
+		});
+	}
+	while (gate.ready.load() < T) std::this_thread::yield();
+	auto t0 = std::chrono::steady_clock::now();
+	gate.go.store(1);
+	for (auto &x : th) x.join();
+	auto t1 = std::chrono::steady_clock::now();
+	if (secs) *secs = std::chrono::duration<double>(t1 - t0).count();
+	const long long m = first[T];
+	if (o) {
+		long long nn = 0, i = 0;
+		for (int t = 0; t < T; t++) {
+			for (size_t c = 0; c < cnts[t].size(); c++) { o->nbr_off[i++] = nn; nn += cnts[t][c]; }
+		}
+		o->nbr_off[m] = nn;
+		long long w = 0;
+		for (int t = 0; t < T; t++)
+			for (size_t c = 0; c < nbs[t].size(); c++, w++) if (w < o->nbr_cap) o->nbr[w] = nbs[t][c];
+	}
+	for (int t = 0; t < T; t++) delete cons[t];
+	for (auto *w : keep) delete w;
+	for (auto &kv : keeps) for (auto *w : kv) delete w;
+	double sk = 0;
+	for (double x : sinks) sk += x;
+	return sk == -1.0 ? -m : m;
+}
+
+void put_prd(container_periodic &con, int n, const int *ids, const double *xyz, const double *r) {
+	put_all(con, n, ids, xyz, r);
+}
+void put_prd(container_periodic_poly &con, int n, const int *ids, const double *xyz, const double *r) {
+	put_all_r(con, n, ids, xyz, r);
+}
+
+// Periodic kinds: each thread loops c_loop_all_periodic over its own
+// container and computes the cells of its z-slab of primary blocks.  Image
+// blocks are created lazily per container, so only the timing is used.
+template<class c_class>
+double threaded_prd_time(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r, int T) {
+	const double *g = s->geom;
+	std::vector<c_class*> cons(T, nullptr);
+	Gate gate;
+	std::vector<std::thread> th;
+	for (int t = 0; t < T; t++) {
+		th.emplace_back([&, t]() {
+			cons[t] = new c_class(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
+					      s->init_mem);
+			put_prd(*cons[t], n, ids, xyz, r);
+			gate.arrive_and_wait();
+			c_class &con = *cons[t];
+			const int nz = con.nz, k0 = nz * t / T, k1 = nz * (t + 1) / T;
+			voronoicell c(con);
+			c_loop_all_periodic vl(con);
+			if (vl.start()) do {
+				const int kk = vl.k - con.ez;   // primary layers are ez .. wz-1
+				if (kk >= k0 && kk < k1) con.compute_cell(c, vl);
+			} while (vl.inc());
+		});
+	}
+	while (gate.ready.load() < T) std::this_thread::yield();
+	auto t0 = std::chrono::steady_clock::now();
+	gate.go.store(1);
+	for (auto &x : th) x.join();
+	auto t1 = std::chrono::steady_clock::now();
+	for (int t = 0; t < T; t++) delete cons[t];
+	return std::chrono::duration<double>(t1 - t0).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+// The voronoicell_neighbor loop of a rectangular container on T threads
+// (z-slabs), filling `o` in c_loop_all order exactly as
+// ref_compute_neighbor does.  Returns the loop length, -1 for periodic kinds.
+long long ref_compute_neighbor_mt(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r,
+				  ref_out *o, int threads, double *secs) {
+	if (s->kind > 1) return -1;
+	if (threads < 1) threads = 1;
+	return s->kind == 0 ? threaded_rect<container>(s, n, ids, xyz, r, o, threads, secs, 0)
+			    : threaded_rect<container_poly>(s, n, ids, xyz, r, o, threads, secs, 0);
+}
+
+// Wall time of compute_all_cells() spread over T threads (z-slabs, one
+// container copy each; building the copies is not timed).  loop 1: the
+// compute_all_cells body; loop 2: the voronoicell_neighbor output loop.
+double ref_time_mt(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r, int threads,
+		   int loop) {
+	double secs = 0;
+	if (threads < 1) threads = 1;
+	if (s->kind == 0) threaded_rect<container>(s, n, ids, xyz, r, 0, threads, &secs, loop);
+	else if (s->kind == 1) threaded_rect<container_poly>(s, n, ids, xyz, r, 0, threads, &secs, loop);
+	else if (s->kind == 2) secs = threaded_prd_time<container_periodic>(s, n, ids, xyz, nullptr, threads);
+	else secs = threaded_prd_time<container_periodic_poly>(s, n, ids, xyz, r, threads);
 	return secs;
 }
 

@@ -1,7 +1,8 @@
 """GPU checks of the learned path: the predictor's kernels against the numpy
 restatement (oracle/mlp.py), training on statistics the plain path records,
-bit-identical results with the predictor on, forced fallback, and what the
-trained network saves over gathering every cell's lists to full depth."""
+results with the predictor on equal to the oracle's (the restatement, or the
+reference library itself) bit for bit, forced fallback, and what the trained
+network saves over gathering every cell's lists to full depth."""
 import numpy as np
 import pytest
 
@@ -37,6 +38,7 @@ def _uniform(seed, n, L, G):
     xyz = I.uniform_box(seed, n, (0, 0, 0), (L, L, L))
     con = voro_amd.container(0, L, 0, L, 0, L, G, G, G, False, False, False, 8)
     con.put_array(np.arange(n, dtype=np.int32), xyz)
+    con.oracle_inputs = (0, (0, L, 0, L, 0, L), (G, G, G), xyz, None)
     return con
 
 
@@ -57,7 +59,18 @@ def _same(a, b):
     assert np.array_equal(a["vol"], b["vol"]) and np.array_equal(a["cent"], b["cent"])
 
 
-def test_gpu_learned_uniform_is_exact_and_saves_work():
+def _vs_oracle(O, got, con, which="restatement"):
+    """The learned path's results against the oracle run on the container's
+    own inputs: neighbours, face counts and existence bit-exact, volumes
+    within 1e-12 relative (they come out bit-identical)."""
+    kind, geom, grid, xyz, r = con.oracle_inputs
+    want = O.run(which, O.make_spec(kind, geom, grid), np.arange(len(xyz), dtype=np.int32), xyz, r,
+                 threads=O.default_threads() if which == "reference" else None)
+    assert O.compare(got, want, vol_rtol=1e-12) == []
+    assert np.array_equal(got["vol"], want["vol"])
+
+
+def test_gpu_learned_uniform_is_exact_and_saves_work(oracle_mod):
     n, L, G = 100000, 46.4, 27
     make = lambda sd: _uniform(sd, n, L, G)
     _flush("train uniform n=%d" % n)
@@ -69,7 +82,9 @@ def test_gpu_learned_uniform_is_exact_and_saves_work():
     con.set_predictor(pred)
     con.compute_all_cells(F | voro_amd.F_LEARNED)
     con._res = None
-    _same(plain, con.results())
+    got = con.results()
+    _vs_oracle(oracle_mod, got, con)
+    _same(plain, got)
     st = con.learned_stats()
     _flush("fallbacks %d of %d, entries %d" % (st["fallbacks"], n, st["list_entries"]))
     assert st["fallbacks"] < 0.1 * n
@@ -78,13 +93,15 @@ def test_gpu_learned_uniform_is_exact_and_saves_work():
     con.set_predictor(pred)
     con.compute_all_cells(F | voro_amd.F_LEARNED)
     con._res = None
-    _same(plain, con.results())
+    got = con.results()
+    _vs_oracle(oracle_mod, got, con)
+    _same(plain, got)
     full = con.learned_stats()
     _flush("entries trained %d, full depth %d" % (st["list_entries"], full["list_entries"]))
     assert st["list_entries"] < full["list_entries"]
 
 
-def test_gpu_learned_per_lane_lists_are_exact():
+def test_gpu_learned_per_lane_lists_are_exact(oracle_mod):
     """With the per-lane first tier the lists are interleaved per batch of 64
     cells and cut by the per-lane list kernel: the same results."""
     n, L, G = 50000, 36.8, 21
@@ -97,10 +114,12 @@ def test_gpu_learned_per_lane_lists_are_exact():
     con.set_predictor(pred)
     con.compute_all_cells(F | voro_amd.F_LEARNED)
     con._res = None
-    _same(plain, con.results())
+    got = con.results()
+    _vs_oracle(oracle_mod, got, con)
+    _same(plain, got)
 
 
-def test_gpu_learned_dense_blocks_are_exact():
+def test_gpu_learned_dense_blocks_are_exact(oracle_mod):
     """About 250 particles per block: a checked block's list entries are
     radius-checked 64 at a time in several rounds; the results equal the
     plain path's."""
@@ -113,10 +132,12 @@ def test_gpu_learned_dense_blocks_are_exact():
     con.set_predictor(pred)
     con.compute_all_cells(F | voro_amd.F_LEARNED)
     con._res = None
-    _same(plain, con.results())
+    got = con.results()
+    _vs_oracle(oracle_mod, got, con)
+    _same(plain, got)
 
 
-def test_gpu_learned_forced_fallback_is_exact():
+def test_gpu_learned_forced_fallback_is_exact(oracle_mod):
     n, L, G = 20000, 27.1, 16
     make = lambda sd: _uniform(sd, n, L, G)
     pred, _, _ = _train(make, [21], epochs=50)
@@ -127,17 +148,20 @@ def test_gpu_learned_forced_fallback_is_exact():
     con.set_predictor(pred)
     con.compute_all_cells(F | voro_amd.F_LEARNED)
     con._res = None
-    _same(plain, con.results())
+    got = con.results()
+    _vs_oracle(oracle_mod, got, con)
+    _same(plain, got)
     assert con.learned_stats()["fallbacks"] > 0.9 * n
 
 
-def test_gpu_learned_clusters_held_out_seed():
+def test_gpu_learned_clusters_held_out_seed(oracle_mod):
     n, box, G = 100000, 46.4, 27
 
     def make(sd):
         xyz = I.clusters(sd, n, box=box, n_clusters=64)
         con = voro_amd.container(0, box, 0, box, 0, box, G, G, G, False, False, False, 8)
         con.put_array(np.arange(len(xyz), dtype=np.int32), xyz)
+        con.oracle_inputs = (0, (0, box, 0, box, 0, box), (G, G, G), xyz, None)
         return con
     _flush("train clusters n=%d" % n)
     pred, loss0, loss = _train(make, [1, 2, 3])
@@ -147,11 +171,13 @@ def test_gpu_learned_clusters_held_out_seed():
     con.set_predictor(pred)
     con.compute_all_cells(F | voro_amd.F_LEARNED)
     con._res = None
-    _same(plain, con.results())
+    got = con.results()
+    _vs_oracle(oracle_mod, got, con, which="reference")
+    _same(plain, got)
     _flush("clusters fallback rate %.4f" % (con.learned_stats()["fallbacks"] / n))
 
 
-def test_gpu_learned_radical_per_particle():
+def test_gpu_learned_radical_per_particle(oracle_mod):
     n, L, G = 30000, 31.0, 18
 
     def make(sd):
@@ -159,6 +185,7 @@ def test_gpu_learned_radical_per_particle():
         r = I.radii_lognormal(sd, n)
         con = voro_amd.container_poly(0, L, 0, L, 0, L, G, G, G, False, False, False, 8)
         con.put_array(np.arange(n, dtype=np.int32), xyz, r)
+        con.oracle_inputs = (1, (0, L, 0, L, 0, L), (G, G, G), xyz, r)
         return con
     pred = voro_amd.predictor(with_radius=True)
     for sd in (31, 32):
@@ -172,4 +199,6 @@ def test_gpu_learned_radical_per_particle():
     con.set_predictor(pred)
     con.compute_all_cells(F | voro_amd.F_LEARNED)
     con._res = None
-    _same(plain, con.results())
+    got = con.results()
+    _vs_oracle(oracle_mod, got, con)
+    _same(plain, got)

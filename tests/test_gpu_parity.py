@@ -125,13 +125,17 @@ def test_gpu_repeated_compute_is_identical():
 
 def test_gpu_full_size_uniform(oracle_mod):
     """The benchmark workload (1,000,000 uniform particles, [0,100]^3, 58^3
-    blocks), bit for bit against the restatement, plus volume closure."""
+    blocks), bit for bit against the reference library itself (run on the
+    host's cores over z-slabs, outputs identical to its single-threaded
+    loop), plus volume closure."""
     O = oracle_mod
     ids, xyz, _ = I.make("uniform", 42)
     _flush("full size uniform 1M")
     b = engine_run(0, (0, 100, 0, 100, 0, 100), (58, 58, 58), ids, xyz)
     assert abs(b["vol"].sum() - 1e6) / 1e6 < 1e-12
-    a = O.run("restatement", O.make_spec(0, (0, 100, 0, 100, 0, 100), (58, 58, 58)), ids, xyz)
+    _flush("full size uniform 1M: reference on %d threads" % O.default_threads())
+    a = O.run("reference", O.make_spec(0, (0, 100, 0, 100, 0, 100), (58, 58, 58)), ids, xyz,
+              threads=O.default_threads())
     assert O.compare(b, a, vol_rtol=VOL_RTOL) == []
     assert np.array_equal(a["vol"], b["vol"])
 

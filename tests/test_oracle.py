@@ -236,3 +236,30 @@ def test_planted_mistake_is_caught(oracle_mod):
     bad["nbr"] = d["nbr"].copy()
     bad["nbr"][[3, 4]] = bad["nbr"][[4, 3]]
     assert O.compare(bad, d) != []
+
+
+@pytest.mark.parametrize("case", ["uniform", "clusters_wall", "poly_periodic_flags"])
+def test_threaded_reference_equals_single_thread(oracle_mod, case):
+    """The threaded reference run (one container copy per thread, z-slabs of
+    blocks through c_loop_subset::setup_intbox) gives the single-threaded
+    c_loop_all outputs exactly, in the same order."""
+    O = oracle_mod
+    if not _ref_available(O):
+        pytest.skip("reference library not built")
+    r = None
+    if case == "uniform":
+        xyz = I.uniform_box(5, 30000, (0, 0, 0), (30, 30, 30))
+        spec = O.make_spec(0, (0, 30, 0, 30, 0, 30), (18, 18, 18))
+    elif case == "clusters_wall":
+        xyz = I.clusters(3, 20000, box=30, n_clusters=8)
+        spec = O.make_spec(0, (0, 30, 0, 30, 0, 30), (16, 16, 16), spheres=[(15, 15, 15, 13, -99)])
+    else:
+        xyz = I.uniform_box(6, 20000, (0, 0, 0), (25, 25, 25))
+        r = I.radii_lognormal(6, len(xyz))
+        spec = O.make_spec(1, (0, 25, 0, 25, 0, 25), (15, 15, 15), periodic=(1, 0, 1))
+    ids = np.arange(len(xyz), dtype=np.int32)
+    a = O.run("reference", spec, ids, xyz, r)
+    for t in (3, 8):
+        b = O.run("reference", spec, ids, xyz, r, threads=t)
+        for k in a:
+            assert np.array_equal(a[k], b[k]), (t, k)
