@@ -200,6 +200,21 @@ int vgpu_learned_stats(vgpu_container *h, long long *fallbacks, long long *list_
 int vgpu_mlp_num_params(void);
 int vgpu_mlp_loss_grad(const float *params, const float *X, const float *T, int n, float *loss, float *grad);
 int vgpu_mlp_forward(const float *params, const float *X, int n, float *Y);
+/* vgpu_mlp_loss_grad without any allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight training loops.  params (vgpu_mlp_num_params()
+ * floats), X (n x 28), T (n x 2), grad (vgpu_mlp_num_params() floats) and
+ * loss (1 float, may be NULL) are DEVICE pointers; stream is a hipStream_t
+ * (NULL: the default stream); the call returns when the results are
+ * written.  The same kernels and summation order as vgpu_mlp_loss_grad, so
+ * the results are bit-identical to it.  Its scratch is a fixed device array
+ * of the library (17.6 MB per GPU, held until the process ends) sized for
+ * at most VGPU_MLP_NOALLOC_MAX_SAMPLES samples: larger n returns
+ * VGPU_INVALID_ARGUMENT.  All calls on one GPU share that array, so calls
+ * that overlap on one GPU (other host threads, other streams) race. */
+#define VGPU_MLP_NOALLOC_MAX_SAMPLES 32768
+int vgpu_mlp_loss_grad_noalloc(const float *params, const float *X, const float *T, int n, float *loss, float *grad,
+			       void *stream);
 
 /* ---- Batches of containers (no counterpart in the reference) ----
  * Many independent rectangular containers -- each created, walled and

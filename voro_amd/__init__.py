@@ -115,6 +115,8 @@ def lib():
         "vgpu_mlp_loss_grad": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int,
                                          C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "vgpu_mlp_forward": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)]),
+        "vgpu_mlp_loss_grad_noalloc": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                                 C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

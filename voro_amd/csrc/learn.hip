@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -317,7 +318,64 @@ void init_params(std::vector<float> &P, unsigned seed) {
 
 }  // namespace
 
+// ---- Loss and gradient without allocating ----
+// The scratch of mlp_loss_grad_dev (activations and deltas, n x ACT floats
+// each; per-sample losses, n; gradient partial sums, LG_CHUNKS x NP) for up
+// to NOALLOC_MAX_SAMPLES samples lives in one file-scope device array: every
+// GPU the library runs on holds its own copy (17.6 MB) until the process
+// ends.  Host code never passes `scratch` itself to a kernel (on the host the
+// name is a stand-in, not a device address): each device's address comes
+// from hipGetSymbolAddress, taken once per device with that device current.
+enum { LG_CHUNKS = 16 };   // the chunk count of vgpu_mlp_loss_grad (same summation order)
+enum : size_t {
+	NOALLOC_MAX_SAMPLES = VGPU_MLP_NOALLOC_MAX_SAMPLES,
+	SCR_A = 0,
+	SCR_D = SCR_A + NOALLOC_MAX_SAMPLES * ACT,
+	SCR_L = SCR_D + NOALLOC_MAX_SAMPLES * ACT,
+	SCR_PART = SCR_L + NOALLOC_MAX_SAMPLES,
+	SCR_FLOATS = SCR_PART + (size_t)LG_CHUNKS * NP
+};
+static __device__ __align__(256) float scratch[SCR_FLOATS];
+
+namespace {
+
+enum { MAX_DEVICES = 64 };
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[MAX_DEVICES];
+
+// The current device's scratch address (looked up on its first use).
+int scratch_of_current_device(float **out) {
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess) return VGPU_HIP_ERROR;
+	if (dev < 0 || dev >= MAX_DEVICES) return VGPU_INVALID_ARGUMENT;
+	std::lock_guard<std::mutex> lk(g_scratch_mu);
+	if (!g_scratch_ptr[dev]) {
+		float *ptr = nullptr;
+		if (hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return VGPU_HIP_ERROR;
+		g_scratch_ptr[dev] = ptr;
+	}
+	*out = g_scratch_ptr[dev];
+	return VGPU_OK;
+}
+
+}  // namespace
+
 extern "C" {
+
+int vgpu_mlp_loss_grad_noalloc(const float *params, const float *X, const float *T, int n, float *loss, float *grad,
+			       void *stream) {
+	if (!params || !X || !T || !grad || n <= 0) return VGPU_INVALID_ARGUMENT;
+	if ((size_t)n > NOALLOC_MAX_SAMPLES) return VGPU_INVALID_ARGUMENT;
+	float *scr = nullptr;
+	int st = scratch_of_current_device(&scr);
+	if (st) return st;
+	hipStream_t s = (hipStream_t)stream;
+	// every scratch word read is written first in the same call: nothing to clear
+	st = mlp_loss_grad_dev(params, X, T, n, scr + SCR_A, scr + SCR_D, scr + SCR_L, scr + SCR_PART, LG_CHUNKS, grad,
+			       loss, s);
+	if (st) return st;
+	return hipStreamSynchronize(s) == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
+}
 
 int vgpu_predictor_create(int with_radius, vgpu_predictor **out) {
 	if (!out) return VGPU_INVALID_ARGUMENT;
@@ -405,7 +463,7 @@ int vgpu_predictor_train(vgpu_predictor *p, int epochs, float lr, unsigned seed,
 int vgpu_mlp_loss_grad(const float *params, const float *X, const float *T, int n, float *loss, float *grad) {
 	if (!params || !X || !T || n <= 0) return VGPU_INVALID_ARGUMENT;
 	DevMem mem;
-	const int chunks = 16;
+	const int chunks = LG_CHUNKS;
 	float *P = mem.get<float>(NP), *G = mem.get<float>(NP), *dX = mem.get<float>((size_t)n * IN),
 	      *dT = mem.get<float>((size_t)n * OUT), *A = mem.get<float>((size_t)n * ACT), *D = mem.get<float>((size_t)n * ACT),
 	      *L = mem.get<float>(n), *part = mem.get<float>((size_t)chunks * NP), *lo = mem.get<float>(1);
