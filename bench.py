@@ -17,8 +17,11 @@ one-cell-per-lane first tier timed beside it.
 
 Prints one JSON line (see the keys below).  The CPU baseline is the
 reference's own container::compute_all_cells() (built from the reference
-sources into oracle/_ref), timed on this host, one thread, on the same
-particles.
+sources into oracle/_ref), timed on this host on the same particles, on all
+the cores this process may use (z-slabs of blocks through
+c_loop_subset::setup_intbox, one container copy per thread; median of three
+runs), with the one-thread time and the voronoicell_neighbor output loop
+beside it.
 """
 import argparse
 import ctypes as C
@@ -181,9 +184,7 @@ def main():
                 # scale the grid to keep ~5 particles per block
                 g = max(1, int(round(G * (m / args.n) ** (1 / 3))))
                 sp = O.make_spec(0, (0, 100, 0, 100, 0, 100), (g, g, g))
-            secs = O.time_reference(sp, ids[:m], xyz[:m], loop="compute_all_cells")
-            cpu = {"value": m / secs, "unit": "cells/s", "cores": 1, "kind": "reference",
-                   "sample": "container::compute_all_cells() on %d of the uniform particles, %.1f s" % (m, secs)}
+            cpu = cpu_reference(O, sp, ids[:m], xyz[:m], None, m, "uniform particles")
 
     others = None if args.headline_only else secondary_workloads(args, torch)
 
@@ -216,15 +217,53 @@ def main():
                     "fallback_rate": ls["fallbacks"] / max(ncells, 1), "list_entries": ls["list_entries"],
                     "gather_ms": gather_ms, "tier_ms": tier_ms, "identical_to_plain": identical,
                     "train_seconds": t_train, "train_loss": train_loss, "train_seeds": list(args.train_seeds)},
-        "roofline": {"bound": "hbm", "achieved": hbm_gbs, "peak": MI355X_HBM_GBS, "unit": "GB/s",
-                     "frac": hbm_gbs / MI355X_HBM_GBS, "traffic": traffic,
-                     "kernel": "k_wcells_list<mono, 4>", "kernel_ms": kern_s * 1e3},
-        "roofline_fp64": {"achieved": fp64_tf, "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
-                          "frac": fp64_tf / MI355X_FP64_TFLOPS},
+        "roofline": {"bound": "mfma", "achieved": fp64_tf, "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": fp64_tf / MI355X_FP64_TFLOPS, "traffic": traffic,
+                     "kernel": "k_wcells_list<mono, 4>", "kernel_ms": kern_s * 1e3,
+                     "compute": "FP64 vector arithmetic, no matrix instructions; the peak is MI355X's FP64 rate",
+                     "limiter": "scalar-instruction issue (see the instruction-mix profile)",
+                     "flop_per_cell": FLOP_PER_CELL},
+        "roofline_hbm": {"achieved": hbm_gbs, "peak": MI355X_HBM_GBS, "unit": "GB/s",
+                         "frac": hbm_gbs / MI355X_HBM_GBS, "traffic": traffic, "bytes_per_launch": bytes_launch},
         "cpu_baseline": cpu,
         "other_workloads": others,
     }
     print(json.dumps(line))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    """The host cores this process may use (its affinity set, at most 16)."""
+    from oracle import oracle as O
+    return O.default_threads()
+
+
+def cpu_reference(O, sp, ids, xyz, r, m, what, runs=3):
+    """The reference on this host: container::compute_all_cells() on all the
+    process's cores (one container copy per thread, each computing a z-slab
+    of blocks via c_loop_subset::setup_intbox; median of `runs`), the
+    voronoicell_neighbor output loop the same way, and compute_all_cells()
+    on one thread once."""
+    T = host_threads()
+    mt = sorted(O.time_reference_mt(sp, ids, xyz, r, threads=T, loop="compute_all_cells") for _ in range(runs))
+    nb = sorted(O.time_reference_mt(sp, ids, xyz, r, threads=T, loop="neighbor") for _ in range(runs))
+    one = O.time_reference(sp, ids, xyz, r, loop="compute_all_cells")
+    med, nmed = mt[len(mt) // 2], nb[len(nb) // 2]
+    return {"value": m / med, "unit": "cells/s", "cores": T, "kind": "reference", "cpu": cpu_model(),
+            "sample": "container::compute_all_cells() over %d %s on %d threads (z-slabs of blocks, one container "
+                      "copy each), median of %d runs: %.3f s" % (m, what, T, runs, med),
+            "neighbor_loop": {"value": m / nmed, "seconds_median": nmed,
+                              "what": "voronoicell_neighbor loop with volume, centroid, face count, neighbours"},
+            "one_thread": {"value": m / one, "seconds": one}}
 
 
 def secondary_workloads(args, torch, only=None):
@@ -396,31 +435,60 @@ def clusters_learned(out, flags, torch):
 
 
 def cpu_samples(out, specs):
-    """The reference (oracle/_ref, one thread) on bounded samples of the two
-    secondary workloads."""
+    """The reference (oracle/_ref) on bounded samples of the secondary
+    workloads, on all of this process's cores."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
     from oracle import oracle as O
     from voro_amd import inputs as I
     if O.reference() is None:
         return
+    T = host_threads()
     # periodic: the same lattice shape at 1/8 the volume and particle count
-    # (same density), 250,000 particles, 37^3 blocks
+    # (same density), 250,000 particles, 37^3 blocks; each thread its own
+    # container, computing the cells of its z-slab of primary blocks
     ids2, xyz2, r2 = I.make("periodic_poly", 42, 250000)
     xyz2 = xyz2 * 0.5
     sp = O.make_spec(3, (30, 2.5, 30, 1.5, 2.0, 30), (37, 37, 37))
-    secs = O.time_reference(sp, ids2, xyz2, r2, loop="compute_all_cells")
+    mt = sorted(O.time_reference_mt(sp, ids2, xyz2, r2, threads=T) for _ in range(3))[1]
+    one = O.time_reference(sp, ids2, xyz2, r2, loop="compute_all_cells")
     out["periodic_poly_2m"]["cpu_baseline"] = {
-        "value": 250000 / secs, "unit": "cells/s", "cores": 1, "kind": "reference",
-        "sample": "container_periodic_poly(30,2.5,30,1.5,2,30) with 250,000 particles at the same "
-                  "density, compute_all_cells(), %.1f s" % secs}
-    secs = 0.0
-    m = 0
-    for L, g, pts in specs[:64]:
+        "value": 250000 / mt, "unit": "cells/s", "cores": T, "kind": "reference", "cpu": cpu_model(),
+        "sample": "container_periodic_poly(30,2.5,30,1.5,2,30) with 250,000 particles at the same density, "
+                  "compute_all_cells() split over %d threads (z-slabs, one container copy each), median of 3: "
+                  "%.3f s" % (T, mt),
+        "one_thread": {"value": 250000 / one, "seconds": one}}
+    # batch: containers shared out over the threads, each thread computing
+    # whole containers (compute_all_cells() each); the rate is the cells over
+    # the busiest thread's compute time
+    sample = specs[:512]
+    busy = {}
+    lock = threading.Lock()
+
+    def one_container(spec):
+        L, g, pts = spec
         sp = O.make_spec(0, (0, L, 0, L, 0, L), (g, g, g), spheres=[[L / 2, L / 2, L / 2, 0.45 * L, -99]])
-        secs += O.time_reference(sp, np.arange(len(pts), dtype=np.int32), pts, loop="compute_all_cells")
-        m += len(pts)
+        t = O.time_reference(sp, np.arange(len(pts), dtype=np.int32), pts, loop="compute_all_cells")
+        with lock:
+            busy[threading.get_ident()] = busy.get(threading.get_ident(), 0.0) + t
+    with ThreadPoolExecutor(max_workers=T) as ex:
+        list(ex.map(one_container, sample))
+    m = sum(len(s[2]) for s in sample)
+    wall = max(busy.values())
     out["batch_4096"]["cpu_baseline"] = {
-        "value": m / secs, "unit": "cells/s", "cores": 1, "kind": "reference",
-        "sample": "the first 64 containers (%d particles), compute_all_cells() each, %.1f s" % (m, secs)}
+        "value": m / wall, "unit": "cells/s", "cores": T, "kind": "reference", "cpu": cpu_model(),
+        "sample": "the first %d containers (%d particles), compute_all_cells() each, shared out over %d threads; "
+                  "busiest thread %.2f s of compute, all threads %.2f s" % (len(sample), m, T, wall,
+                                                                            sum(busy.values()))}
+    # clustered: the evaluation draw itself, all cores
+    if "clusters_learned" in out:
+        ids3, xyz3, _ = I.make("clusters", 101)
+        sp = O.make_spec(0, (0, 100, 0, 100, 0, 100), (58, 58, 58))
+        mt = sorted(O.time_reference_mt(sp, ids3, xyz3, None, threads=T) for _ in range(3))[1]
+        out["clusters_learned"]["cpu_baseline"] = {
+            "value": len(ids3) / mt, "unit": "cells/s", "cores": T, "kind": "reference", "cpu": cpu_model(),
+            "sample": "container::compute_all_cells() over the 1,000,000 clustered particles of seed 101 on %d "
+                      "threads (z-slabs), median of 3: %.3f s" % (T, mt)}
 
 
 if __name__ == "__main__":
