@@ -39,6 +39,9 @@ extern "C" {
 #define VGPU_F_LEARNED 4u     /* use the trained candidate predictor if one is loaded */
 #define VGPU_F_CSR 8u         /* compact neighbour lists into loop-order CSR */
 #define VGPU_F_GRAPH 16u      /* batches: capture the compute as a hipGraph once, replay it afterwards */
+#define VGPU_F_GEOMETRY 32u   /* keep every cell's vertices and faces and compute the polyhedron
+                                 statistics (vgpu_get_geometry); implies VGPU_F_NEIGHBORS | VGPU_F_CSR
+                                 and the plain path (single containers only) */
 
 typedef struct vgpu_container vgpu_container;
 
@@ -55,9 +58,21 @@ int vgpu_create(int kind, const double *geom, const int *grid, const int *period
 /* Frees the container and its device memory. */
 int vgpu_destroy(vgpu_container *h);
 
-/* Adds a sphere wall (src/wall.hh:27-30 wall_sphere(xc,yc,zc,rc,w_id),
- * src/container.hh:63-69 add_wall).  Rectangular containers only. */
+/* Adds a wall (src/container.hh:63-69 add_wall); walls cut each cell in the
+ * order they were added.  Rectangular containers only, at most 64.
+ *   type 0 wall_sphere(xc,yc,zc,rc,w_id)            src/wall.hh:27-30
+ *   type 1 wall_plane(xc,yc,zc,ac,w_id)             src/wall.hh:44-47
+ *   type 2 wall_cylinder(xc,yc,zc,xa,ya,za,rc,w_id) src/wall.hh:65-68
+ *   type 3 wall_cone(xc,yc,zc,xa,ya,za,ang,w_id)    src/wall.hh:97-101
+ * params: the constructor's arguments before w_id, in order (4 for the
+ * sphere and the plane, 7 for the cylinder and the cone). */
+int vgpu_add_wall(vgpu_container *h, int type, const double *params, int w_id);
+/* vgpu_add_wall(h, 0, {xc,yc,zc,rc}, w_id). */
 int vgpu_add_wall_sphere(vgpu_container *h, double xc, double yc, double zc, double rc, int w_id);
+/* container_base::point_inside (src/container.cc:484-487): *inside = 1 if
+ * (x,y,z) lies in the box and inside every wall (each wall's point_inside,
+ * src/wall.cc). */
+int vgpu_point_inside(vgpu_container *h, double x, double y, double z, int *inside);
 
 /* Appends n particles from host memory, in call order, exactly as n calls
  * of put(id,x,y,z[,r]) would (src/container.cc:90-115,
@@ -111,6 +126,52 @@ int vgpu_compute_ghost_cells(vgpu_container *h, int n, const double *xyz, const 
  * record a particle_order keeps (src/c_loops.hh:35-75) turned around, for
  * c_loop_order.  Valid after a compute. */
 int vgpu_get_put_index(vgpu_container *h, int *put_index);
+/* The container kind (VGPU_CONTAINER ... VGPU_CONTAINER_PERIODIC_POLY). */
+int vgpu_kind(vgpu_container *h, int *kind);
+/* Bins the particles put so far without computing cells (what the
+ * reference's put() has done by the time a loop reads them); vgpu_compute_all
+ * bins by itself. */
+int vgpu_bin(vgpu_container *h);
+/* After binning: the particle count and, for each stored particle in loop
+ * order, its id, stored position (remapped into the domain along periodic
+ * directions) and radius (default_radius 0.5 for the plain kinds,
+ * src/config.hh:90) -- c_loop_base::pos(pid,x,y,z,r) (src/c_loops.hh:146-153)
+ * for every loop position.  Any pointer may be NULL. */
+int vgpu_get_particles(vgpu_container *h, long long *n, int *id, double *pos, double *r);
+
+/* c_loop_subset (src/c_loops.hh:218-258, src/c_loops.cc:24-150) over a
+ * rectangular container's stored particles, the reference's loop state
+ * machine step for step: mode 0 setup_sphere(v[0], v[1], v[2], r = v[3],
+ * bounds_test), 1 setup_box(v[0..5] = xmin, xmax, ymin, ymax, zmin, zmax,
+ * bounds_test), 2 setup_intbox(iv[0..5] = ai, bi, aj, bj, ak, bk).  For the
+ * visited particles in the loop's order: their c_loop_all position
+ * (index[t], into vgpu_get_results' arrays) and the loop's block coordinates
+ * (ijk_pos[3t..3t+2] = i, j, k as the reference's loop holds them, unwrapped
+ * along periodic directions); up to cap entries, *n = the loop length.  Bins
+ * the particles if needed. */
+int vgpu_loop_subset(vgpu_container *h, int mode, const double *v, const int *iv, int bounds_test, long long *index,
+		     int *ijk_pos, long long cap, long long *n);
+
+/* Text formats, byte for byte the reference's:
+ *   print_custom(format, filename) -- container::print_custom
+ *     (src/container.cc:401-432; the per-cell line of
+ *     voronoicell_base::output_custom, src/cell.cc:2485-2558, every code);
+ *     computes the cells (with VGPU_F_GEOMETRY if a code needs the vertices
+ *     or faces);
+ *   import(filename) -- container::import / container_poly::import
+ *     (src/container.cc:333-376): "id x y z" (radii != 0: "id x y z r")
+ *     records until the end of the file; the records read are put, and a
+ *     record that does not parse returns VGPU_FILE_ERROR (the reference exits);
+ *   draw_particles(filename, pov) -- draw_particles / draw_particles_pov
+ *     (src/container.hh:329-372, container_poly's 551-596);
+ *   draw_cells(filename, pov) -- draw_cells_gnuplot / draw_cells_pov
+ *     (src/container.hh:379-418; voronoicell_base::draw_gnuplot and
+ *     draw_pov, src/cell.cc:1940-1987). */
+int vgpu_print_custom(vgpu_container *h, const char *format, const char *filename);
+int vgpu_import(vgpu_container *h, const char *filename, int radii, long long *n_read);
+int vgpu_draw_particles(vgpu_container *h, const char *filename, int pov);
+int vgpu_draw_cells(vgpu_container *h, const char *filename, int pov);
+
 /* Number of particles stored (the c_loop_all length). */
 int vgpu_num_cells(vgpu_container *h, long long *n);
 
@@ -130,6 +191,27 @@ int vgpu_get_results(vgpu_container *h, int *id, int *ijk, int *q, int *exists, 
  * VGPU_F_LEARNED the list-driven one; 1 order-3 over the first tier's
  * leftovers; 2 wide; 3 huge).  Loop order. */
 int vgpu_get_stats(vgpu_container *h, int *n_planes, int *wl_exit, int *tier);
+
+/* Polyhedron statistics of every cell, after a vgpu_compute_all with
+ * VGPU_F_GEOMETRY, in loop order, each equal bit for bit to what the
+ * reference's voronoicell_neighbor function returns for that cell
+ * (src/cell.cc): vertices() (local coordinates, 3 per vertex; 1791-2216),
+ * vertex_orders() (2191), the edge table ed[i][j] and its back pointers
+ * ed[i][nu[i]+j] (vertex by vertex, nu[i] slots each), face_areas() (1791),
+ * face_perimeters() (2251), normals() (2087, 3 per face), face_orders()
+ * (2309), face_vertices() (2282, per face: the count, then the vertices),
+ * surface_area() (1822), total_edge_distance() (1921), number_of_edges()
+ * (2468), max_radius_squared() (1907).  Per-vertex, per-edge-slot and
+ * face-vertex arrays are indexed by the offsets returned here (n+1 each);
+ * per-face arrays by the neighbour offsets of vgpu_get_results (one
+ * neighbour per face, same order).  Cells that do not exist have none.
+ * Any pointer may be NULL; sizes from vgpu_geometry_sizes. */
+int vgpu_geometry_sizes(vgpu_container *h, long long *n_vertices, long long *n_edge_slots, long long *n_faces,
+			long long *n_face_vertex_entries);
+int vgpu_get_geometry(vgpu_container *h, long long *vert_off, double *vertices, int *vertex_orders, long long *edge_off,
+		      int *edges, int *edge_rel, double *face_areas, double *face_perimeters, double *normals,
+		      int *face_orders, long long *fv_off, int *face_vertices, double *surface_area,
+		      double *total_edge_distance, int *number_of_edges, double *max_radius_squared);
 
 /* Sum of cell volumes in loop order (src/container.cc:451-456
  * sum_cell_volumes()). */

@@ -26,7 +26,7 @@ KIND = {"container": 0, "container_poly": 1, "container_periodic": 2, "container
 class Spec(C.Structure):
     _fields_ = [("kind", C.c_int), ("geom", C.c_double * 6), ("grid", C.c_int * 3),
                 ("periodic", C.c_int * 3), ("init_mem", C.c_int), ("n_spheres", C.c_int),
-                ("spheres", C.POINTER(C.c_double))]
+                ("spheres", C.POINTER(C.c_double)), ("n_walls", C.c_int), ("walls", C.POINTER(C.c_double))]
 
 
 class Out(C.Structure):
@@ -83,7 +83,10 @@ def _ptr(a, t):
     return a.ctypes.data_as(C.POINTER(t))
 
 
-def make_spec(kind, geom, grid, periodic=(0, 0, 0), spheres=None, init_mem=8):
+def make_spec(kind, geom, grid, periodic=(0, 0, 0), spheres=None, init_mem=8, walls=None):
+    """walls: further walls after the spheres, each (type, w_id, *args) with
+    type 0 sphere (xc,yc,zc,rc), 1 plane (xc,yc,zc,ac), 2 cylinder
+    (xc,yc,zc,xa,ya,za,rc), 3 cone (xc,yc,zc,xa,ya,za,ang)."""
     s = Spec()
     s.kind = KIND[kind] if isinstance(kind, str) else int(kind)
     for t in range(6):
@@ -99,7 +102,16 @@ def make_spec(kind, geom, grid, periodic=(0, 0, 0), spheres=None, init_mem=8):
         s.spheres = _ptr(keep, C.c_double)
     else:
         s.n_spheres = 0
-    s._keep = keep
+    wk = None
+    if walls:
+        wk = np.zeros((len(walls), 9))
+        for t, w in enumerate(walls):
+            wk[t, :len(w)] = w
+        s.n_walls = len(walls)
+        s.walls = _ptr(wk, C.c_double)
+    else:
+        s.n_walls = 0
+    s._keep = (keep, wk)
     return s
 
 
@@ -209,6 +221,77 @@ def time_reference_mt(spec, ids, xyz, r=None, threads=None, loop="compute_all_ce
               None if rr is None else _ptr(rr, C.c_double), C.c_int(t), C.c_int(1 if loop == "compute_all_cells" else 2))
 
 
+class Geo(C.Structure):
+    _fields_ = [("vert_off", C.POINTER(C.c_longlong)), ("vertices", C.POINTER(C.c_double)),
+                ("orders", C.POINTER(C.c_int)), ("pts", C.POINTER(C.c_double)), ("vcap", C.c_longlong),
+                ("edge_off", C.POINTER(C.c_longlong)), ("edges", C.POINTER(C.c_int)),
+                ("edge_rel", C.POINTER(C.c_int)), ("ecap", C.c_longlong),
+                ("face_off", C.POINTER(C.c_longlong)), ("face_area", C.POINTER(C.c_double)),
+                ("face_perim", C.POINTER(C.c_double)), ("normal", C.POINTER(C.c_double)),
+                ("face_order", C.POINTER(C.c_int)), ("fcap", C.c_longlong),
+                ("fv_off", C.POINTER(C.c_longlong)), ("face_verts", C.POINTER(C.c_int)), ("fvcap", C.c_longlong),
+                ("surface", C.POINTER(C.c_double)), ("edge_dist", C.POINTER(C.c_double)),
+                ("n_edges", C.POINTER(C.c_int)), ("maxr2", C.POINTER(C.c_double)), ("tol", C.c_double)]
+
+
+def geometry_reference(spec, ids, xyz, r=None):
+    """Every cell's polyhedron statistics from the reference library's own
+    voronoicell_neighbor functions (loop order), with the raw tables: a dict
+    of numpy arrays (None where the library is unavailable)."""
+    lib = _load(REF_SO, "ref_compute_neighbor") and _LIBS[REF_SO][0]
+    if not lib:
+        return None
+    fn = lib.ref_cell_geometry
+    fn.restype = C.c_longlong
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
+    n = len(ids)
+    caps = [max(1, 40 * n), max(1, 120 * n), max(1, 20 * n), max(1, 140 * n)]
+    while True:
+        vc, ec, fc, fvc = caps
+        o = {"vert_off": np.zeros(n + 1, np.int64), "vertices": np.zeros(3 * vc), "orders": np.zeros(vc, np.int32),
+             "pts": np.zeros(3 * vc), "edge_off": np.zeros(n + 1, np.int64), "edges": np.zeros(ec, np.int32),
+             "edge_rel": np.zeros(ec, np.int32), "face_off": np.zeros(n + 1, np.int64),
+             "face_area": np.zeros(fc), "face_perim": np.zeros(fc), "normal": np.zeros(3 * fc),
+             "face_order": np.zeros(fc, np.int32), "fv_off": np.zeros(n + 1, np.int64),
+             "face_verts": np.zeros(fvc, np.int32), "surface": np.zeros(n), "edge_dist": np.zeros(n),
+             "n_edges": np.zeros(n, np.int32), "maxr2": np.zeros(n)}
+        g = Geo()
+        for k, v in o.items():
+            setattr(g, k, _ptr(v, C.c_longlong if v.dtype == np.int64 else
+                             (C.c_int if v.dtype == np.int32 else C.c_double)))
+        g.vcap, g.ecap, g.fcap, g.fvcap = vc, ec, fc, fvc
+        m = fn(C.byref(spec), C.c_int(n), _ptr(ids, C.c_int), _ptr(xyz, C.c_double),
+               None if rr is None else _ptr(rr, C.c_double), C.byref(g))
+        need = [int(o["vert_off"][m]), int(o["edge_off"][m]), int(o["face_off"][m]), int(o["fv_off"][m])]
+        if all(a <= b for a, b in zip(need, caps)):
+            break
+        caps = [max(a, b) + 16 for a, b in zip(need, caps)]
+    m = int(m)
+    for k in ("vert_off", "edge_off", "face_off", "fv_off"):
+        o[k] = o[k][:m + 1]
+    nv, ne, nf, nfv = need
+    o["vertices"] = o["vertices"][:3 * nv].reshape(-1, 3)
+    o["pts"] = o["pts"][:3 * nv].reshape(-1, 3)
+    o["orders"] = o["orders"][:nv]
+    o["edges"], o["edge_rel"] = o["edges"][:ne], o["edge_rel"][:ne]
+    for k in ("face_area", "face_perim", "face_order"):
+        o[k] = o[k][:nf]
+    o["normal"] = o["normal"][:3 * nf].reshape(-1, 3)
+    o["face_verts"] = o["face_verts"][:nfv]
+    for k in ("surface", "edge_dist", "n_edges", "maxr2"):
+        o[k] = o[k][:m]
+    o["tol"] = g.tol
+    return o
+
+
+def tolerance(max_len_sq):
+    """The cells' tolerance, tolerance * max_len_sq (src/config.hh:80,
+    src/cell.cc:26)."""
+    return 10.0 * 2.220446049250313e-16 * max_len_sq
+
+
 def find_reference(spec, ids, xyz, r, q, timed=False):
     """find_voronoi_cell of the reference library for every row of q:
     (found, pid, pos[, seconds of the query loop]); None where the library is
@@ -275,6 +358,61 @@ def print_custom_reference(spec, ids, xyz, r, fmt, filename):
     m = fn(C.byref(spec), C.c_int(len(ids)), _ptr(ids, C.c_int), _ptr(xyz, C.c_double),
            None if rr is None else _ptr(rr, C.c_double), fmt.encode(), filename.encode())
     return m == 0
+
+
+def loop_subset_reference(spec, ids, xyz, r, mode, v=None, iv=None, bounds_test=True):
+    """The reference's c_loop_subset steps: an (m, 6) array of particle id,
+    i, j, k, ijk, q (None where the library is unavailable)."""
+    lib = _load(REF_SO, "ref_compute_neighbor") and _LIBS[REF_SO][0]
+    if not lib:
+        return None
+    fn = lib.ref_loop_subset
+    fn.restype = C.c_longlong
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
+    vv = np.ascontiguousarray(v if v is not None else np.zeros(6), dtype=np.float64)
+    ii = np.ascontiguousarray(iv if iv is not None else np.zeros(6), dtype=np.int32)
+    cap = max(1, len(ids) * 8)
+    while True:
+        out = np.zeros((cap, 6), np.int32)
+        m = fn(C.byref(spec), C.c_int(len(ids)), _ptr(ids, C.c_int), _ptr(xyz, C.c_double),
+               None if rr is None else _ptr(rr, C.c_double), C.c_int(mode), _ptr(vv, C.c_double), _ptr(ii, C.c_int),
+               C.c_int(1 if bounds_test else 0), _ptr(out, C.c_int), C.c_longlong(cap))
+        if m <= cap:
+            return out[:m]
+        cap = int(m)
+
+
+def point_inside_reference(spec, q):
+    """container::point_inside of the reference for every row of q (the
+    spec's box and walls); None where the library is unavailable."""
+    lib = _load(REF_SO, "ref_compute_neighbor") and _LIBS[REF_SO][0]
+    if not lib:
+        return None
+    fn = lib.ref_point_inside
+    fn.restype = C.c_longlong
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(len(q), np.int32)
+    fn(C.byref(spec), C.c_int(len(q)), _ptr(q, C.c_double), _ptr(out, C.c_int))
+    return out.astype(bool)
+
+
+def draw_reference(spec, ids, xyz, r, what, filename):
+    """The reference library's drawing into filename: what is
+    "particles", "particles_pov", "cells_gnuplot" or "cells_pov"; False
+    where the library is unavailable."""
+    lib = _load(REF_SO, "ref_compute_neighbor") and _LIBS[REF_SO][0]
+    if not lib:
+        return False
+    code = {"particles": 1, "particles_pov": 2, "cells_gnuplot": 3, "cells_pov": 4}[what]
+    fn = lib.ref_draw
+    fn.restype = C.c_longlong
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
+    return fn(C.byref(spec), C.c_int(len(ids)), _ptr(ids, C.c_int), _ptr(xyz, C.c_double),
+              None if rr is None else _ptr(rr, C.c_double), C.c_int(code), filename.encode()) == 0
 
 
 def compare(a, b, vol_rtol=1e-12):

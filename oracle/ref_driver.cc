@@ -37,6 +37,9 @@ struct ref_spec {
 	int init_mem;
 	int n_spheres;       // sphere walls (rectangular containers only)
 	const double *spheres;  // n_spheres x {xc,yc,zc,rc,w_id}
+	int n_walls;         // further walls, added after the spheres
+	const double *walls;    // n_walls x {type, w_id, constructor arguments (4 or 7)}: 0 sphere,
+	                        // 1 plane, 2 cylinder, 3 cone (src/wall.hh:27-107)
 };
 
 struct ref_out {
@@ -51,6 +54,37 @@ struct ref_out {
 	int *nbr;            // [nbr_cap]
 	long long nbr_cap;
 	double *pos;         // [3n] stored (possibly remapped) position, or null
+};
+
+// Polyhedron statistics of every cell in loop order, each from the
+// reference's own voronoicell_neighbor functions (src/cell.cc), plus the raw
+// tables (pts, nu, ed with back pointers) they are computed from.  The
+// *_off arrays have n+1 entries; an array is written only up to its cap,
+// and the offsets always count everything (so a caller can retry).
+struct ref_geo {
+	long long *vert_off;   // vertices per cell (vertices(), vertex_orders())
+	double *vertices;      // 3 per vertex, local coordinates
+	int *orders;
+	double *pts;           // 3 per vertex, the raw (doubled) coordinates
+	long long vcap;
+	long long *edge_off;   // edge slots per cell (sum of the orders)
+	int *edges;            // ed[i][j]
+	int *edge_rel;         // ed[i][nu[i]+j]
+	long long ecap;
+	long long *face_off;   // faces per cell
+	double *face_area;
+	double *face_perim;
+	double *normal;        // 3 per face
+	int *face_order;
+	long long fcap;
+	long long *fv_off;     // face_vertices() entries per cell
+	int *face_verts;
+	long long fvcap;
+	double *surface;       // per cell
+	double *edge_dist;
+	int *n_edges;
+	double *maxr2;
+	double tol;            // the cells' tolerance (tolerance * max_len_sq)
 };
 
 }
@@ -121,12 +155,24 @@ void put_all_r(c_class &con, int n, const int *ids, const double *xyz, const dou
 }
 
 template<class c_class>
-void add_spheres(c_class &con, const ref_spec *s, std::vector<wall_sphere*> &keep) {
+void add_spheres(c_class &con, const ref_spec *s, std::vector<wall*> &keep) {
 	for (int w = 0; w < s->n_spheres; w++) {
 		const double *q = s->spheres + 5 * w;
 		wall_sphere *ws = new wall_sphere(q[0], q[1], q[2], q[3], (int)q[4]);
 		keep.push_back(ws);
 		con.add_wall(*ws);
+	}
+	for (int w = 0; w < s->n_walls; w++) {
+		const double *q = s->walls + 9 * w;
+		const int type = (int)q[0], id = (int)q[1];
+		const double *a = q + 2;
+		wall *x;
+		if (type == 0) x = new wall_sphere(a[0], a[1], a[2], a[3], id);
+		else if (type == 1) x = new wall_plane(a[0], a[1], a[2], a[3], id);
+		else if (type == 2) x = new wall_cylinder(a[0], a[1], a[2], a[3], a[4], a[5], a[6], id);
+		else x = new wall_cone(a[0], a[1], a[2], a[3], a[4], a[5], a[6], id);
+		keep.push_back(x);
+		con.add_wall(*x);
 	}
 }
 
@@ -135,7 +181,7 @@ void add_spheres(c_class &con, const ref_spec *s, std::vector<wall_sphere*> &kee
 long long dispatch(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r,
 		   ref_out *o, double *secs, int mode) {
 	const double *g = s->geom;
-	std::vector<wall_sphere*> keep;
+	std::vector<wall*> keep;
 	long long m = 0;
 	if (s->kind == 0) {
 		container con(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
@@ -186,6 +232,47 @@ long long dispatch(const ref_spec *s, int n, const int *ids, const double *xyz, 
 	}
 	for (size_t w = 0; w < keep.size(); w++) delete keep[w];
 	return m;
+}
+
+template<class c_class>
+void text_of(c_class &con, int what, const char *format, const char *filename) {
+	switch (what) {
+	case 0: con.print_custom(format, filename); break;
+	case 1: con.draw_particles(filename); break;
+	case 2: con.draw_particles_pov(filename); break;
+	case 3: con.draw_cells_gnuplot(filename); break;
+	default: con.draw_cells_pov(filename); break;
+	}
+}
+
+long long ref_text(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r, int what,
+		   const char *format, const char *filename) {
+	const double *g = s->geom;
+	std::vector<wall*> keep;
+	if (s->kind == 0) {
+		container con(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
+			      s->periodic[0] != 0, s->periodic[1] != 0, s->periodic[2] != 0, s->init_mem);
+		add_spheres(con, s, keep);
+		put_all(con, n, ids, xyz, r);
+		text_of(con, what, format, filename);
+	} else if (s->kind == 1) {
+		container_poly con(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
+				   s->periodic[0] != 0, s->periodic[1] != 0, s->periodic[2] != 0, s->init_mem);
+		add_spheres(con, s, keep);
+		put_all_r(con, n, ids, xyz, r);
+		text_of(con, what, format, filename);
+	} else if (s->kind == 2) {
+		container_periodic con(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2], s->init_mem);
+		put_all(con, n, ids, xyz, r);
+		text_of(con, what, format, filename);
+	} else {
+		container_periodic_poly con(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
+					    s->init_mem);
+		put_all_r(con, n, ids, xyz, r);
+		text_of(con, what, format, filename);
+	}
+	for (size_t w = 0; w < keep.size(); w++) delete keep[w];
+	return 0;
 }
 
 }  // namespace
@@ -290,24 +377,12 @@ long long ref_compute_ghost_cells(const ref_spec *s, int n, const int *ids, cons
 	return m;
 }
 
-// container(_poly)::print_custom(format, filename) (src/container.cc:401-432).
-// Rectangular kinds only (returns -1).
+// print_custom(format, filename) (src/container.cc:401-432 and the periodic
+// classes'), and the drawings: what 0 print_custom, 1 draw_particles,
+// 2 draw_particles_pov, 3 draw_cells_gnuplot, 4 draw_cells_pov.
 long long ref_print_custom(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r,
 			   const char *format, const char *filename) {
-	const double *g = s->geom;
-	if (s->kind > 1) return -1;
-	if (s->kind == 0) {
-		container con(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
-			      s->periodic[0] != 0, s->periodic[1] != 0, s->periodic[2] != 0, s->init_mem);
-		put_all(con, n, ids, xyz, r);
-		con.print_custom(format, filename);
-	} else {
-		container_poly con(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
-				   s->periodic[0] != 0, s->periodic[1] != 0, s->periodic[2] != 0, s->init_mem);
-		put_all_r(con, n, ids, xyz, r);
-		con.print_custom(format, filename);
-	}
-	return 0;
+	return ref_text(s, n, ids, xyz, r, 0, format, filename);
 }
 
 // Times the voronoicell_neighbor loop with volume/centroid/faces/neighbours.
@@ -322,6 +397,63 @@ double ref_time_neighbor_loop(const ref_spec *s, int n, const int *ids, const do
 }  // extern "C"
 
 namespace {
+
+template<class c_class, class c_loop>
+long long geo_loop(c_class &con, ref_geo *g) {
+	voronoicell_neighbor c(con);
+	c_loop vl(con);
+	std::vector<int> vi;
+	std::vector<double> vd;
+	long long m = 0, nv = 0, ne = 0, nf = 0, nfv = 0;
+	g->tol = c.tol;
+	if (vl.start()) do {
+		g->vert_off[m] = nv; g->edge_off[m] = ne; g->face_off[m] = nf; g->fv_off[m] = nfv;
+		if (con.compute_cell(c, vl)) {
+			c.vertices(vd);
+			for (int i = 0; i < c.p; i++) {
+				if (nv + i < g->vcap) {
+					for (int a = 0; a < 3; a++) {
+						g->vertices[3 * (nv + i) + a] = vd[3 * i + a];
+						g->pts[3 * (nv + i) + a] = c.pts[4 * i + a];
+					}
+					g->orders[nv + i] = c.nu[i];
+				}
+				for (int j = 0; j < c.nu[i]; j++) {
+					if (ne < g->ecap) { g->edges[ne] = c.ed[i][j]; g->edge_rel[ne] = c.ed[i][c.nu[i] + j]; }
+					ne++;
+				}
+			}
+			nv += c.p;
+			std::vector<double> fa, fp, nr;
+			std::vector<int> fo;
+			c.face_areas(fa);
+			c.face_perimeters(fp);
+			c.normals(nr);
+			c.face_orders(fo);
+			for (size_t f = 0; f < fa.size(); f++) {
+				if (nf + (long long)f < g->fcap) {
+					g->face_area[nf + f] = fa[f];
+					g->face_perim[nf + f] = fp[f];
+					g->face_order[nf + f] = fo[f];
+					for (int a = 0; a < 3; a++) g->normal[3 * (nf + f) + a] = nr[3 * f + a];
+				}
+			}
+			nf += (long long)fa.size();
+			c.face_vertices(vi);
+			for (size_t t = 0; t < vi.size(); t++) if (nfv + (long long)t < g->fvcap) g->face_verts[nfv + t] = vi[t];
+			nfv += (long long)vi.size();
+			g->surface[m] = c.surface_area();
+			g->edge_dist[m] = c.total_edge_distance();
+			g->n_edges[m] = c.number_of_edges();
+			g->maxr2[m] = c.max_radius_squared();
+		} else {
+			g->surface[m] = 0; g->edge_dist[m] = 0; g->n_edges[m] = 0; g->maxr2[m] = 0;
+		}
+		m++;
+	} while (vl.inc());
+	g->vert_off[m] = nv; g->edge_off[m] = ne; g->face_off[m] = nf; g->fv_off[m] = nfv;
+	return m;
+}
 
 // ---- Threaded runs: one container copy per thread, each thread computing
 // the cells of its own z-slab of blocks through c_loop_subset::setup_intbox
@@ -422,10 +554,10 @@ void slab_compute(c_class &con, int k0, int k1) {
 
 template<class c_class>
 c_class *make_rect(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r,
-		   std::vector<wall_sphere*> &keep);
+		   std::vector<wall*> &keep);
 template<>
 container *make_rect<container>(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r,
-				std::vector<wall_sphere*> &keep) {
+				std::vector<wall*> &keep) {
 	const double *g = s->geom;
 	container *con = new container(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
 				       s->periodic[0] != 0, s->periodic[1] != 0, s->periodic[2] != 0, s->init_mem);
@@ -435,7 +567,7 @@ container *make_rect<container>(const ref_spec *s, int n, const int *ids, const 
 }
 template<>
 container_poly *make_rect<container_poly>(const ref_spec *s, int n, const int *ids, const double *xyz,
-					  const double *r, std::vector<wall_sphere*> &keep) {
+					  const double *r, std::vector<wall*> &keep) {
 	const double *g = s->geom;
 	container_poly *con = new container_poly(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1],
 						 s->grid[2], s->periodic[0] != 0, s->periodic[1] != 0,
@@ -450,7 +582,7 @@ container_poly *make_rect<container_poly>(const ref_spec *s, int n, const int *i
 template<class c_class>
 long long threaded_rect(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r, ref_out *o,
 			int T, double *secs, int mode) {
-	std::vector<wall_sphere*> keep;
+	std::vector<wall*> keep;
 	// the block occupancy decides the slabs and each slab's first loop index
 	c_class *probe = make_rect<c_class>(s, n, ids, xyz, r, keep);
 	const int nx = probe->nx, ny = probe->ny, nz = probe->nz;
@@ -465,7 +597,7 @@ long long threaded_rect(const ref_spec *s, int n, const int *ids, const double *
 	std::vector<std::vector<long long>> cnts(T);
 	std::vector<c_class*> cons(T, nullptr);
 	cons[0] = probe;
-	std::vector<std::vector<wall_sphere*>> keeps(T);
+	std::vector<std::vector<wall*>> keeps(T);
 	Gate gate;
 	std::vector<double> sinks(T, 0);
 	std::vector<std::thread> th;
@@ -570,6 +702,100 @@ double ref_time_mt(const ref_spec *s, int n, const int *ids, const double *xyz, 
 	else if (s->kind == 2) secs = threaded_prd_time<container_periodic>(s, n, ids, xyz, nullptr, threads);
 	else secs = threaded_prd_time<container_periodic_poly>(s, n, ids, xyz, r, threads);
 	return secs;
+}
+
+
+// The polyhedron statistics of every cell (loop order), rectangular and
+// periodic kinds; returns the loop length.
+long long ref_cell_geometry(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r,
+			    ref_geo *g) {
+	const double *gm = s->geom;
+	std::vector<wall*> keep;
+	long long m = 0;
+	if (s->kind == 0) {
+		container con(gm[0], gm[1], gm[2], gm[3], gm[4], gm[5], s->grid[0], s->grid[1], s->grid[2],
+			      s->periodic[0] != 0, s->periodic[1] != 0, s->periodic[2] != 0, s->init_mem);
+		add_spheres(con, s, keep);
+		put_all(con, n, ids, xyz, r);
+		m = geo_loop<container, c_loop_all>(con, g);
+	} else if (s->kind == 1) {
+		container_poly con(gm[0], gm[1], gm[2], gm[3], gm[4], gm[5], s->grid[0], s->grid[1], s->grid[2],
+				   s->periodic[0] != 0, s->periodic[1] != 0, s->periodic[2] != 0, s->init_mem);
+		add_spheres(con, s, keep);
+		put_all_r(con, n, ids, xyz, r);
+		m = geo_loop<container_poly, c_loop_all>(con, g);
+	} else if (s->kind == 2) {
+		container_periodic con(gm[0], gm[1], gm[2], gm[3], gm[4], gm[5], s->grid[0], s->grid[1], s->grid[2],
+				       s->init_mem);
+		put_all(con, n, ids, xyz, r);
+		m = geo_loop<container_periodic, c_loop_all_periodic>(con, g);
+	} else {
+		container_periodic_poly con(gm[0], gm[1], gm[2], gm[3], gm[4], gm[5], s->grid[0], s->grid[1],
+					    s->grid[2], s->init_mem);
+		put_all_r(con, n, ids, xyz, r);
+		m = geo_loop<container_periodic_poly, c_loop_all_periodic>(con, g);
+	}
+	for (size_t w = 0; w < keep.size(); w++) delete keep[w];
+	return m;
+}
+
+
+// The reference's drawings (what 1..4 as ref_text) into filename.
+long long ref_draw(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r, int what,
+		   const char *filename) {
+	return ref_text(s, n, ids, xyz, r, what, "", filename);
+}
+
+
+// c_loop_subset (src/c_loops.hh:218-258) of the reference over a rectangular
+// container: mode 0 setup_sphere(v0..v3, bounds_test), 1 setup_box(v0..v5,
+// bounds_test), 2 setup_intbox(iv0..iv5); for each step up to cap: the
+// particle id, i, j, k, ijk, q.  Returns the loop length.
+long long ref_loop_subset(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r, int mode,
+			  const double *v, const int *iv, int bounds_test, int *out6, long long cap) {
+	const double *g = s->geom;
+	std::vector<wall*> keep;
+	long long m = 0;
+	auto run = [&](auto &con) {
+		c_loop_subset vl(con);
+		if (mode == 0) vl.setup_sphere(v[0], v[1], v[2], v[3], bounds_test != 0);
+		else if (mode == 1) vl.setup_box(v[0], v[1], v[2], v[3], v[4], v[5], bounds_test != 0);
+		else vl.setup_intbox(iv[0], iv[1], iv[2], iv[3], iv[4], iv[5]);
+		if (vl.start()) do {
+			if (m < cap) {
+				int *o = out6 + 6 * m;
+				o[0] = vl.pid(); o[1] = vl.i; o[2] = vl.j; o[3] = vl.k; o[4] = vl.ijk; o[5] = vl.q;
+			}
+			m++;
+		} while (vl.inc());
+	};
+	if (s->kind == 0) {
+		container con(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
+			      s->periodic[0] != 0, s->periodic[1] != 0, s->periodic[2] != 0, s->init_mem);
+		put_all(con, n, ids, xyz, r);
+		run(con);
+	} else if (s->kind == 1) {
+		container_poly con(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
+				   s->periodic[0] != 0, s->periodic[1] != 0, s->periodic[2] != 0, s->init_mem);
+		put_all_r(con, n, ids, xyz, r);
+		run(con);
+	} else return -1;
+	return m;
+}
+
+
+// container::point_inside (src/container.cc:484-487) of a rectangular
+// container with the spec's walls, for nq points.
+long long ref_point_inside(const ref_spec *s, int nq, const double *q, int *inside) {
+	const double *g = s->geom;
+	std::vector<wall*> keep;
+	container con(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
+		      s->periodic[0] != 0, s->periodic[1] != 0, s->periodic[2] != 0, s->init_mem);
+	add_spheres(con, s, keep);
+	long long k = 0;
+	for (int i = 0; i < nq; i++) { inside[i] = con.point_inside(q[3 * i], q[3 * i + 1], q[3 * i + 2]) ? 1 : 0; k += inside[i]; }
+	for (size_t w = 0; w < keep.size(); w++) delete keep[w];
+	return k;
 }
 
 }

@@ -859,6 +859,8 @@ struct Spec {
 	int init_mem;
 	int n_spheres;
 	const double *spheres;
+	int n_walls;         // plane, cylinder and cone walls are not restated here
+	const double *walls;
 };
 
 struct Out {
@@ -1518,6 +1520,7 @@ extern "C" {
 long long orc_compute_neighbor(const Spec *s, int n, const int *ids, const double *xyz, const double *r,
 			       Out *o, Stats *st) {
 	if (s->kind > 1) return -2;
+	if (s->n_walls > 0) return -3;   // only sphere walls are restated
 	RectContainer con(*s);
 	for (int t = 0; t < n; t++) con.put(ids[t], xyz[3 * t], xyz[3 * t + 1], xyz[3 * t + 2], r ? r[t] : 0.5);
 	Cell c(con.max_len_sq);

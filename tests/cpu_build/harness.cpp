@@ -20,6 +20,7 @@ using namespace vgpu;
 extern "C" {
 struct hx_spec {
 	int kind; double geom[6]; int grid[3]; int periodic[3]; int init_mem; int n_spheres; const double *spheres;
+	int n_walls; const double *walls;   // {type, w_id, constructor arguments}
 };
 struct hx_out {
 	int *id; int *ijk; int *q; int *exists; double *vol; double *cent; int *nfaces;
@@ -157,10 +158,24 @@ long long hx_compute_neighbor(const hx_spec *s, int n, const int *ids, const dou
 	for (int t = 0; t < n; t++) if (blk[t] >= 0) { int d = fill[blk[t]]++; P[d] = rem[t]; pid[d] = ids[t]; }
 	std::vector<double> mrad(4096), walls;
 	worklist_radii(v.boxx, v.boxy, v.boxz, mrad.data());
-	for (int w = 0; w < s->n_spheres; w++) for (int t = 0; t < 5; t++) walls.push_back(s->spheres[5 * w + t]);
+	// device wall records (type, w_id, the constants the wall constructors derive)
+	auto add = [&](int type, int id, const double *a) {
+		double r[VGPU_WALL_STRIDE] = {0};
+		r[0] = type; r[1] = id; r[2] = a[0]; r[3] = a[1]; r[4] = a[2];
+		if (type <= 1) r[5] = a[3];
+		else {
+			r[5] = a[3]; r[6] = a[4]; r[7] = a[5];
+			r[8] = 1 / (a[3] * a[3] + a[4] * a[4] + a[5] * a[5]);
+			if (type == 2) r[9] = a[6];
+			else { r[9] = std::sin(a[6]); r[10] = std::cos(a[6]); r[11] = std::tan(a[6]); }
+		}
+		walls.insert(walls.end(), r, r + VGPU_WALL_STRIDE);
+	};
+	for (int w = 0; w < s->n_spheres; w++) add(0, (int)s->spheres[5 * w + 4], s->spheres + 5 * w);
+	for (int w = 0; w < s->n_walls; w++) add((int)s->walls[9 * w], (int)s->walls[9 * w + 1], s->walls + 9 * w + 2);
 	v.gy = v.ny;
 	v.P = P.data(); v.pid = pid.data(); v.bstart = cnt.data(); v.cstart = cnt.data(); v.mrad = mrad.data(); v.wl = vgpu_worklist_table;
-	v.walls = walls.data(); v.nwalls = s->n_spheres; v.max_radius = maxr;
+	v.walls = walls.data(); v.nwalls = (int)(walls.size() / VGPU_WALL_STRIDE); v.max_radius = maxr;
 
 	std::vector<int> cellblk(m);
 	for (int b = 0; b < nxyz; b++) for (int q = cnt[b]; q < cnt[b + 1]; q++) cellblk[q] = b;
@@ -405,3 +420,46 @@ static long long compute_prd(const hx_spec *s, int n, const int *ids, const doub
 	return bad ? -1 : m;
 }
 
+
+// ---- Polyhedron statistics (geom_dev.h) from given cell tables ----
+#include "../../voro_amd/csrc/geom_dev.h"
+
+extern "C" long long hx_geometry_stats(int ncells, const int *exists, const long long *vert_off, const double *pts,
+				       const int *orders, const long long *edge_off, const int *edges,
+				       const int *edge_rel, const long long *face_off, const long long *fv_off,
+				       double tol, double *vertices, double *face_area, double *face_perim,
+				       double *normal, int *face_order, int *face_verts, double *surface,
+				       double *edge_dist, int *n_edges, double *maxr2) {
+	const long long nv = vert_off[ncells], ne = edge_off[ncells];
+	std::vector<int> vstart(ncells), np(ncells), eoff(nv > 0 ? nv : 1), status(ncells), fo(ncells), vo(ncells),
+		fvo(ncells), eo(ncells);
+	std::vector<unsigned char> nu(nv > 0 ? nv : 1), rel(ne > 0 ? ne : 1);
+	std::vector<short> ed(ne > 0 ? ne : 1);
+	std::vector<double> xyz(3 * (nv > 0 ? nv : 1));
+	for (int c = 0; c < ncells; c++) {
+		vstart[c] = (int)vert_off[c];
+		np[c] = (int)(vert_off[c + 1] - vert_off[c]);
+		status[c] = exists[c] ? ST_EXISTS : 0;
+		fo[c] = (int)face_off[c]; vo[c] = (int)vert_off[c]; fvo[c] = (int)fv_off[c]; eo[c] = (int)edge_off[c];
+		long long e = edge_off[c];
+		for (long long v = vert_off[c]; v < vert_off[c + 1]; v++) { eoff[v] = (int)e; e += orders[v]; }
+	}
+	for (long long v = 0; v < nv; v++) {
+		nu[v] = (unsigned char)orders[v];
+		for (int a = 0; a < 3; a++) xyz[3 * v + a] = pts[3 * v + a];
+	}
+	for (long long t = 0; t < ne; t++) { ed[t] = (short)edges[t]; rel[t] = (unsigned char)edge_rel[t]; }
+	GeoSink g;
+	g.vstart = vstart.data(); g.np = np.data(); g.xyz = xyz.data(); g.nu = nu.data(); g.eoff = eoff.data();
+	g.ed = ed.data(); g.rel = rel.data();
+	GeoStats o;
+	o.status = status.data(); o.face_off = fo.data(); o.vert_off = vo.data(); o.edge_off = eo.data();
+	o.fv_off = fvo.data(); o.tol = tol;
+	o.vertices = vertices; o.face_area = face_area; o.face_perim = face_perim; o.normal = normal;
+	o.face_order = face_order; o.face_verts = face_verts; o.surface = surface; o.edge_dist = edge_dist;
+	o.n_edges = n_edges; o.maxr2 = maxr2;
+	for (int c = 0; c < ncells; c++) cell_geometry_stats(g, o, c);
+	// the tables must come back unchanged (edges restored after marking)
+	for (long long t = 0; t < ne; t++) if (ed[t] != edges[t]) return -1;
+	return ncells;
+}

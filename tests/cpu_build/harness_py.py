@@ -69,3 +69,35 @@ def run(O, spec, ids, xyz, r=None, list_depth=0, wave=False):
     out["wave4_done"] = int(lib.hx_wave4_done())
     out["tiers"] = tc
     return out
+
+
+def geometry_stats(ref_geo, exists, tol):
+    """geom_dev.h's cell_geometry_stats over the tables of a reference dump
+    (oracle.geometry_reference): the statistics the engine's statistics
+    pass computes from the tables it keeps."""
+    lib = C.CDLL(SO)
+    f = lib.hx_geometry_stats
+    f.restype = C.c_longlong
+    g = ref_geo
+    m = len(g["surface"])
+    nv, nf, nfv = len(g["orders"]), len(g["face_order"]), len(g["face_verts"])
+    out = {"vertices": np.zeros((max(nv, 1), 3)), "face_area": np.zeros(max(nf, 1)), "face_perim": np.zeros(max(nf, 1)),
+           "normal": np.zeros((max(nf, 1), 3)), "face_order": np.zeros(max(nf, 1), np.int32),
+           "face_verts": np.zeros(max(nfv, 1), np.int32), "surface": np.zeros(m), "edge_dist": np.zeros(m),
+           "n_edges": np.zeros(m, np.int32), "maxr2": np.zeros(m)}
+    P = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+    ex = np.ascontiguousarray(exists, dtype=np.int32)
+    k = f(m, P(ex, C.c_int), P(g["vert_off"], C.c_longlong), P(np.ascontiguousarray(g["pts"]), C.c_double),
+          P(g["orders"], C.c_int), P(g["edge_off"], C.c_longlong), P(g["edges"], C.c_int), P(g["edge_rel"], C.c_int),
+          P(g["face_off"], C.c_longlong), P(g["fv_off"], C.c_longlong), C.c_double(tol),
+          P(out["vertices"], C.c_double), P(out["face_area"], C.c_double), P(out["face_perim"], C.c_double),
+          P(out["normal"], C.c_double), P(out["face_order"], C.c_int), P(out["face_verts"], C.c_int),
+          P(out["surface"], C.c_double), P(out["edge_dist"], C.c_double), P(out["n_edges"], C.c_int),
+          P(out["maxr2"], C.c_double))
+    if k < 0:
+        raise RuntimeError("statistics changed the tables")
+    out["vertices"], out["normal"] = out["vertices"][:nv], out["normal"][:nf]
+    for key in ("face_area", "face_perim", "face_order"):
+        out[key] = out[key][:nf]
+    out["face_verts"] = out["face_verts"][:nfv]
+    return out

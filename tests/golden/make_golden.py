@@ -15,6 +15,10 @@ Fixtures (numpy .npz, no pickles):
   prd_tri_small.npz        -- container_periodic(10,1.5,10,0.7,-1.2,10), 5^3 blocks,
       1,000 particles drawn in [-2,12)^3.
   worklist_sha256.txt      -- SHA-256 of the reference worklist table values.
+  packs/                   -- the particle packings the reference's examples read
+      (examples/basic/pack_ten_cube, examples/custom/pack_six_cube and
+      pack_six_cube_poly, examples/walls/pack_cylinder and pack_torus),
+      copied byte for byte as input data ("python make_golden.py packs").
 """
 import hashlib
 import os
@@ -76,5 +80,21 @@ def main():
     print("worklist sha256", h)
 
 
+PACKS = ["basic/pack_ten_cube", "custom/pack_six_cube", "custom/pack_six_cube_poly", "walls/pack_cylinder",
+         "walls/pack_torus"]
+
+
+def copy_packs():
+    os.makedirs(os.path.join(HERE, "packs"), exist_ok=True)
+    for rel in PACKS:
+        data = open(os.path.join(O.REF_ROOT, "examples", rel), "rb").read()
+        with open(os.path.join(HERE, "packs", os.path.basename(rel)), "wb") as f:
+            f.write(data)
+        print("copied", rel, len(data), "bytes")
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["packs"]:
+        copy_packs()
+    else:
+        main()

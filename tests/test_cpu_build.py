@@ -198,3 +198,104 @@ def test_wave_cell_128_vertices(oracle_mod, harness):
     assert b["wave2_done"] > 0 and b["wave4_done"] > 0
     assert O.compare(b, a, vol_rtol=0.0) == []
     assert np.array_equal(a["vol"], b["vol"]) and np.array_equal(a["cent"], b["cent"])
+
+
+GEO_CASES = {
+    "uniform": lambda O: (O.make_spec(0, (0, 10, 0, 10, 0, 10), (8, 8, 8)), I.uniform_box(3, 3000, (0, 0, 0), (10, 10, 10)), None),
+    "poly_wall": lambda O: (O.make_spec(1, (0, 10, 0, 10, 0, 10), (7, 7, 7), spheres=[(5, 5, 5, 4.5, -99)]),
+                            I.uniform_box(4, 2000, (0, 0, 0), (10, 10, 10)), I.radii_lognormal(4, 2000)),
+    "periodic_triclinic": lambda O: (O.make_spec(2, (10, 1, 10, 0.5, 0.3, 10), (7, 7, 7)),
+                                     I.uniform_box(5, 2000, (0, 0, 0), (10, 10, 10)), None),
+    "large_radical": lambda O: (O.make_spec(1, (0, 12, 0, 12, 0, 12), (6, 6, 6)),
+                                I.uniform_box(6, 1500, (0, 0, 0), (12, 12, 12)),
+                                np.where(np.arange(1500) % 97 == 0, 2.2, 0.3)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(GEO_CASES))
+def test_geometry_statistics_restatement_matches_reference(oracle_mod, harness, case):
+    """The statistics pass (geom_dev.h's cell_geometry_stats, compiled for
+    the CPU) over the reference's own cell tables gives what the reference's
+    face_areas, face_perimeters, normals, face_orders, face_vertices,
+    surface_area, total_edge_distance, number_of_edges, max_radius_squared
+    and vertices return, bit for bit, and leaves the tables unchanged."""
+    O = oracle_mod
+    if O.reference() is None:
+        pytest.skip("reference library not built")
+    spec, xyz, r = GEO_CASES[case](O)
+    ids = np.arange(len(xyz), dtype=np.int32)
+    g = O.geometry_reference(spec, ids, xyz, r)
+    a = O.run("reference", spec, ids, xyz, r)
+    assert np.array_equal(np.diff(g["face_off"]), a["nfaces"])
+    h = H.geometry_stats(g, a["exists"], g["tol"])
+    for k in ("vertices", "face_area", "face_perim", "normal", "face_order", "face_verts", "surface", "edge_dist",
+              "n_edges", "maxr2"):
+        assert np.array_equal(h[k], g[k]), k
+    if case != "periodic_triclinic":
+        # the tolerance is the reference's 10 eps max_len_sq of the box
+        L = spec.geom[1] - spec.geom[0]
+        assert g["tol"] == O.tolerance(3 * L * L)
+
+
+def test_geometry_statistics_closed_forms(oracle_mod, harness):
+    """One particle in a 2 x 3 x 5 box: six rectangular faces; areas, orders,
+    perimeters, unit normals along the axes and the edge totals follow from
+    the box (a planted mistake in the restatement, such as a dropped factor
+    or a transposed cross product, fails here)."""
+    O = oracle_mod
+    if O.reference() is None:
+        pytest.skip("reference library not built")
+    spec = O.make_spec(0, (0, 2, 0, 3, 0, 5), (1, 1, 1))
+    g = O.geometry_reference(spec, np.array([0], np.int32), np.array([[0.7, 1.1, 2.9]]))
+    h = H.geometry_stats(g, np.array([1]), g["tol"])
+    assert sorted(h["face_area"].tolist()) == sorted([6.0, 6.0, 10.0, 10.0, 15.0, 15.0])
+    assert h["face_order"].tolist() == [4] * 6
+    assert sorted(h["face_perim"].tolist()) == sorted([10.0, 10.0, 14.0, 14.0, 16.0, 16.0])
+    assert h["surface"][0] == 62.0 and h["edge_dist"][0] == 40.0 and h["n_edges"][0] == 12
+    nrm = h["normal"]
+    assert np.allclose(np.abs(nrm).sum(axis=1), 1.0) and np.allclose(np.abs(nrm).max(axis=1), 1.0)
+    assert np.allclose(nrm.sum(axis=0), 0.0)
+    fv = h["face_verts"]
+    assert len(fv) == 6 * 5 and all(fv[5 * t] == 4 for t in range(6))
+
+
+WALL_CASES = {
+    # examples/walls/tetrahedron.cc: four plane walls
+    "tetrahedron": lambda: ((-2, 2, -2, 2, -2, 2), (7, 7, 7),
+                            [(1, -99, 1, 1, 1, 1), (1, -99, -1, -1, 1, 1), (1, -99, 1, -1, -1, 1), (1, -99, -1, 1, -1, 1)],
+                            I.uniform_box(71, 400, (-2, -2, -2), (2, 2, 2))),
+    # examples/walls/frustum.cc: a cone
+    "frustum": lambda: ((-1.2, 1.2, -1.2, 1.2, 0, 1), (14, 14, 7), [(3, -99, 0, 0, 2, 0, 0, -1, np.arctan(0.5))],
+                        np.array([[x, y, z] for z in np.arange(0.1, 1, 0.2) for y in np.arange(-0.85, 1, 0.2)
+                                  for x in np.arange(-0.95, 1, 0.2)])),
+    # examples/walls/cylinder.cc: a cylinder along z, plus a sphere and a tilted plane
+    "cylinder_sphere_plane": lambda: ((-6.5, 6.5, -6.5, 6.5, 0, 18.5), (7, 7, 14),
+                                      [(2, 7, 0, 0, 0, 0, 0, 1, 6), (0, 8, 0, 0, 9, 8.0), (1, 9, 0.1, 0.2, 1.0, 15.0)],
+                                      I.uniform_box(72, 2500, (-6.5, -6.5, 0), (6.5, 6.5, 18.5))),
+    # a tilted cylinder and cone in one container
+    "tilted": lambda: ((0, 10, 0, 10, 0, 10), (6, 6, 6),
+                       [(2, -7, 5, 5, 5, 1, 0.5, 0.2, 4.2), (3, -8, 5, 5, -2, 0.1, 0.2, 1, 0.5)],
+                       I.uniform_box(73, 2000, (0, 0, 0), (10, 10, 10))),
+}
+
+
+@pytest.mark.parametrize("case", sorted(WALL_CASES))
+def test_wall_cuts_on_cpu_match_reference(oracle_mod, harness, case):
+    """Plane, cylinder and cone walls (src/wall.cc:41-120) cut by the
+    device code compiled for the CPU, per-lane and wavefront cells, equal
+    the reference library bit for bit; point_inside selects the same
+    particles."""
+    O = oracle_mod
+    if O.reference() is None:
+        pytest.skip("reference library not built")
+    geom, grid, walls, pts = WALL_CASES[case]()
+    spec = O.make_spec(0, geom, grid, walls=walls)
+    inside = O.point_inside_reference(spec, pts)
+    xyz = pts[inside] if case in ("tetrahedron", "frustum") else pts
+    ids = np.arange(len(xyz), dtype=np.int32)
+    a = O.run("reference", spec, ids, xyz)
+    assert a["exists"].sum() > 0
+    for wave in (False, True):
+        b = H.run(O, spec, ids, xyz, wave=wave)
+        assert O.compare(b, a, vol_rtol=0.0) == [], wave
+        assert np.array_equal(a["vol"], b["vol"])

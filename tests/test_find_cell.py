@@ -152,8 +152,8 @@ FORMAT = "%i %q %v %s %n %c %C 100%% %z"
 @pytest.mark.gpu
 @pytest.mark.parametrize("kind", [0, 1])
 def test_gpu_print_custom_matches_reference_text(oracle_mod, tmp_path, kind):
-    """print_custom's text, byte for byte, for the codes the engine's stored
-    outputs carry; the particles come in through import_()."""
+    """print_custom's text, byte for byte, with the particles read through
+    import_() (which must give the reference's particles exactly)."""
     O = oracle_mod
     _ref_or_skip(O)
     n = 2000
@@ -165,7 +165,7 @@ def test_gpu_print_custom_matches_reference_text(oracle_mod, tmp_path, kind):
         for t in range(n):
             f.write("%d %.17g %.17g %.17g%s\n" % (ids[t], *xyz[t], (" %.17g" % r[t]) if kind else ""))
     want = tmp_path / "ref.txt"
-    fmt = FORMAT if kind else FORMAT + " %r"
+    fmt = FORMAT + " %r"
     assert O.print_custom_reference(O.make_spec(kind, (0, 10, 0, 10, 0, 10), (5, 5, 5)), ids, xyz, r, fmt, str(want))
     cls = voro_amd.container_poly if kind else voro_amd.container
     con = cls(0, 10, 0, 10, 0, 10, 5, 5, 5, False, False, False, 8)
@@ -173,11 +173,18 @@ def test_gpu_print_custom_matches_reference_text(oracle_mod, tmp_path, kind):
     assert con.print_custom(fmt) == want.read_text()
 
 
-def test_print_custom_rejects_geometry_codes():
-    con = voro_amd.container.__new__(voro_amd.container)
-    con.kind = voro_amd.KIND_CONTAINER
-    with pytest.raises(ValueError):
-        voro_amd.container.print_custom(con, "%i %f")
+@pytest.mark.gpu
+def test_gpu_import_stops_at_a_bad_record(tmp_path):
+    """A record that does not parse is an import error; the records before
+    it were put (the reference exits at that point)."""
+    src = tmp_path / "bad.txt"
+    src.write_text("1 0.5 0.5 0.5\n2 1.5 1.5 1.5\n3 x 0 0\n")
+    con = voro_amd.container(0, 4, 0, 4, 0, 4, 2, 2, 2, False, False, False, 8)
+    with pytest.raises(voro_amd.VoroGPUError) as e:
+        con.import_(str(src))
+    assert e.value.status == voro_amd.FILE_ERROR
+    con.compute_all_cells()
+    assert sorted(con.results()["id"].tolist()) == [1, 2]
 
 
 # ---- particle_order / c_loop_order, pre_container --------------------------

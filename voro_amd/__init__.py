@@ -42,6 +42,7 @@ F_STATS = 2
 F_LEARNED = 4
 F_CSR = 8
 F_GRAPH = 16
+F_GEOMETRY = 32
 
 KIND_CONTAINER = 0
 KIND_POLY = 1
@@ -74,6 +75,9 @@ def lib():
         "vgpu_create": (C.c_int, [C.c_int, f64p, i32p, i32p, C.c_int, C.POINTER(p)]),
         "vgpu_destroy": (C.c_int, [p]),
         "vgpu_add_wall_sphere": (C.c_int, [p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]),
+        "vgpu_add_wall": (C.c_int, [p, C.c_int, f64p, C.c_int]),
+        "vgpu_point_inside": (C.c_int, [p, C.c_double, C.c_double, C.c_double, i32p]),
+        "vgpu_loop_subset": (C.c_int, [p, C.c_int, f64p, i32p, C.c_int, i64p, i32p, C.c_longlong, i64p]),
         "vgpu_put": (C.c_int, [p, C.c_int, i32p, f64p, f64p]),
         "vgpu_put_device": (C.c_int, [p, C.c_int, p, p, p]),
         "vgpu_clear": (C.c_int, [p]),
@@ -115,6 +119,16 @@ def lib():
         "vgpu_mlp_loss_grad": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int,
                                          C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "vgpu_mlp_forward": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)]),
+        "vgpu_geometry_sizes": (C.c_int, [p, i64p, i64p, i64p, i64p]),
+        "vgpu_get_geometry": (C.c_int, [p, i64p, f64p, i32p, i64p, i32p, i32p, f64p, f64p, f64p, i32p, i64p, i32p,
+                                        f64p, f64p, i32p, f64p]),
+        "vgpu_kind": (C.c_int, [p, i32p]),
+        "vgpu_bin": (C.c_int, [p]),
+        "vgpu_get_particles": (C.c_int, [p, i64p, i32p, f64p, f64p]),
+        "vgpu_print_custom": (C.c_int, [p, C.c_char_p, C.c_char_p]),
+        "vgpu_import": (C.c_int, [p, C.c_char_p, C.c_int, i64p]),
+        "vgpu_draw_particles": (C.c_int, [p, C.c_char_p, C.c_int]),
+        "vgpu_draw_cells": (C.c_int, [p, C.c_char_p, C.c_int]),
         "vgpu_mlp_loss_grad_noalloc": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
                                                  C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), p]),
     }
@@ -191,19 +205,28 @@ class _Base:
         self._res = None
 
     def add_wall(self, wall):
-        """Adds a wall (wall_sphere only)."""
-        _check(lib().vgpu_add_wall_sphere(self._h, wall.xc, wall.yc, wall.zc, wall.rc, wall.w_id),
-               "vgpu_add_wall_sphere")
+        """add_wall (src/container.hh:63-69): a wall_sphere, wall_plane,
+        wall_cylinder or wall_cone; walls cut each cell in the order added."""
+        a = np.asarray(wall._args, dtype=np.float64)
+        _check(lib().vgpu_add_wall(self._h, wall._type, _p(a, C.c_double), wall.w_id), "vgpu_add_wall")
         self._walls.append(wall)
         self._res = None
+
+    def point_inside(self, x, y, z):
+        """container_base::point_inside (src/container.cc:484-487): inside the
+        box and every wall."""
+        r = C.c_int()
+        _check(lib().vgpu_point_inside(self._h, float(x), float(y), float(z), C.byref(r)), "vgpu_point_inside")
+        return bool(r.value)
 
     # -- compute ---------------------------------------------------------
     def compute_all_cells(self, flags=F_NEIGHBORS, stream=None):
         """Bins the particles and computes every cell (src/container.cc:438-443),
         keeping the results for c_loop_all reads."""
         _check(lib().vgpu_compute_all(self._h, int(flags), stream), "vgpu_compute_all")
-        self._flags = flags
+        self._flags = flags | (F_NEIGHBORS | F_CSR if flags & F_GEOMETRY else 0)
         self._res = None
+        self._geo = None
 
     def sum_cell_volumes(self):
         if self._res is None and not self._computed():
@@ -274,85 +297,110 @@ class _Base:
         _check(lib().vgpu_get_stats(self._h, _p(a, C.c_int), _p(b, C.c_int), _p(t, C.c_int)), "vgpu_get_stats")
         return {"n_planes": a, "wl_exit": b, "tier": t}
 
-    # -- text formats: print_custom and import ---------------------------------
-    _CUSTOM_UNSUPPORTED = set("wpPomgEeFAaftl")
+    # -- polyhedron statistics (VGPU_F_GEOMETRY) --------------------------------
+    def geometry(self):
+        """Every cell's polyhedron statistics after a compute with F_GEOMETRY
+        (computed here if the stored results lack them), in loop order, each
+        the reference's voronoicell_neighbor function's value
+        (src/cell.cc): per-vertex arrays (vertices, orders) by vert_off,
+        per-edge-slot arrays (edges, edge_rel) by edge_off, per-face arrays
+        (face_area, face_perim, normal, face_order) by the results' nbr_off,
+        face_verts by fv_off; per-cell surface, edge_dist, n_edges, maxr2."""
+        if getattr(self, "_geo", None) is not None:
+            return self._geo
+        L = lib()
+        sz = [C.c_longlong() for _ in range(4)]
+        if L.vgpu_geometry_sizes(self._h, *[C.byref(t) for t in sz]) != OK:
+            self.compute_all_cells(F_NEIGHBORS | F_CSR | F_GEOMETRY)
+            _check(L.vgpu_geometry_sizes(self._h, *[C.byref(t) for t in sz]), "vgpu_geometry_sizes")
+        nv, ne, nf, nfv = (t.value for t in sz)
+        n = len(self.results()["id"])
+        g = {"vert_off": np.zeros(n + 1, np.int64), "vertices": np.zeros((max(nv, 1), 3)),
+             "orders": np.zeros(max(nv, 1), np.int32), "edge_off": np.zeros(n + 1, np.int64),
+             "edges": np.zeros(max(ne, 1), np.int32), "edge_rel": np.zeros(max(ne, 1), np.int32),
+             "face_area": np.zeros(max(nf, 1)), "face_perim": np.zeros(max(nf, 1)), "normal": np.zeros((max(nf, 1), 3)),
+             "face_order": np.zeros(max(nf, 1), np.int32), "fv_off": np.zeros(n + 1, np.int64),
+             "face_verts": np.zeros(max(nfv, 1), np.int32), "surface": np.zeros(n), "edge_dist": np.zeros(n),
+             "n_edges": np.zeros(n, np.int32), "maxr2": np.zeros(n)}
+        f64, i32, i64 = C.c_double, C.c_int, C.c_longlong
+        _check(L.vgpu_get_geometry(self._h, _p(g["vert_off"], i64), _p(g["vertices"], f64), _p(g["orders"], i32),
+                                   _p(g["edge_off"], i64), _p(g["edges"], i32), _p(g["edge_rel"], i32),
+                                   _p(g["face_area"], f64), _p(g["face_perim"], f64), _p(g["normal"], f64),
+                                   _p(g["face_order"], i32), _p(g["fv_off"], i64), _p(g["face_verts"], i32),
+                                   _p(g["surface"], f64), _p(g["edge_dist"], f64), _p(g["n_edges"], i32),
+                                   _p(g["maxr2"], f64)), "vgpu_get_geometry")
+        g["vertices"], g["orders"] = g["vertices"][:nv], g["orders"][:nv]
+        g["edges"], g["edge_rel"] = g["edges"][:ne], g["edge_rel"][:ne]
+        for k in ("face_area", "face_perim", "normal", "face_order"):
+            g[k] = g[k][:nf]
+        g["face_verts"] = g["face_verts"][:nfv]
+        self._geo = g
+        return g
 
-    def print_custom(self, format, filename=None, flags=F_NEIGHBORS):
-        """container::print_custom (src/container.cc:401-432, the per-cell line
-        of src/cell.cc:2485-2558) from the stored results: one line per
-        computed cell in loop order.  Codes %i %x %y %z %q %r %s %n %v %c %C;
-        the vertex, edge and face-geometry codes (%w %p %P %o %m %g %E %e %F
-        %A %a %f %t %l) need the cell's vertex tables, which this engine does
-        not keep, and raise ValueError.  Returns the text, or writes it to
-        filename."""
-        codes = [format[k + 1] for k in range(len(format) - 1) if format[k] == "%"]
-        bad = sorted(set(codes) & self._CUSTOM_UNSUPPORTED)
-        if bad:
-            raise ValueError("print_custom: unsupported codes %s" % " ".join("%" + c for c in bad))
-        if "r" in codes and self.kind in (KIND_POLY, KIND_PERIODIC_POLY):
-            raise ValueError("print_custom: %r needs the stored radii, not kept for radical containers")
-        if self._res is None or ("n" in codes and not (self._flags & F_NEIGHBORS)):
-            self.compute_all_cells(flags | (F_NEIGHBORS if "n" in codes else 0))
-        r = self.results()
-        out = []
-        for m in np.nonzero(r["exists"])[0]:
-            x, y, z = (float(t) for t in r["pos"][m])
-            line = []
-            k = 0
-            while k < len(format):
-                ch = format[k]
-                if ch != "%":
-                    line.append(ch)
-                    k += 1
-                    continue
-                k += 1
-                if k >= len(format):
-                    break                   # a trailing '%' prints nothing
-                c = format[k]
-                if c == "i": line.append("%d" % r["id"][m])
-                elif c == "x": line.append("%g" % x)
-                elif c == "y": line.append("%g" % y)
-                elif c == "z": line.append("%g" % z)
-                elif c == "q": line.append("%g %g %g" % (x, y, z))
-                elif c == "r": line.append("%g" % 0.5)
-                elif c == "s": line.append("%d" % r["nfaces"][m])
-                elif c == "n":
-                    line.append(" ".join("%d" % t for t in r["nbr"][r["nbr_off"][m]:r["nbr_off"][m + 1]]))
-                elif c == "v": line.append("%g" % r["vol"][m])
-                elif c == "c": line.append("%g %g %g" % tuple(float(t) for t in r["cent"][m]))
-                elif c == "C":
-                    cx, cy, cz = (float(t) for t in r["cent"][m])
-                    line.append("%g %g %g" % (x + cx, y + cy, z + cz))
-                else: line.append("%" + c)
-                k += 1
-            out.append("".join(line) + "\n")
-        text = "".join(out)
-        if filename is None:
-            return text
-        with open(filename, "w") as f:
-            f.write(text)
+    # -- text formats (src/container.cc:333-432, src/container.hh:325-450) -------
+    def _text(self, fn, *args):
+        import tempfile
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "out.txt")
+            _check(fn(self._h, path.encode(), *args), fn.__name__)
+            with open(path) as f:
+                return f.read()
+
+    def print_custom(self, format, filename=None):
+        """container::print_custom(format, filename) (src/container.cc:401-432):
+        one line per computed cell in loop order, every code of
+        voronoicell_base::output_custom (src/cell.cc:2485-2558), written by
+        the engine library with the reference's printf conversions.  Returns
+        the text, or writes it to filename."""
+        L = lib()
+        self._res = None
+        self._geo = None
+        if filename is not None:
+            _check(L.vgpu_print_custom(self._h, format.encode(), str(filename).encode()), "vgpu_print_custom")
+            self._flags = F_NEIGHBORS | F_CSR
+            return None
+        import tempfile
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "custom.txt")
+            _check(L.vgpu_print_custom(self._h, format.encode(), path.encode()), "vgpu_print_custom")
+            self._flags = F_NEIGHBORS | F_CSR
+            with open(path) as f:
+                return f.read()
 
     def import_(self, filename):
-        """container::import (src/container.cc:333-376): reads "id x y z" lines
-        ("id x y z r" for radical containers) and puts each particle; a line
-        that does not parse ends the file with VoroGPUError(FILE_ERROR), as
-        the reference's "File import error"."""
+        """container::import (src/container.cc:333-376): "id x y z" records
+        ("id x y z r" for radical containers) until the end of the file, each
+        put; a record that does not parse raises VoroGPUError(FILE_ERROR)
+        after the records before it were put (the reference exits)."""
         poly = self.kind in (KIND_POLY, KIND_PERIODIC_POLY)
-        width = 5 if poly else 4
-        ids, xyz, rr = [], [], []
-        with open(filename) as f:
-            tok = f.read().split()
-        for k in range(0, len(tok) - len(tok) % width, width):
-            try:
-                i = int(tok[k]); v = [float(t) for t in tok[k + 1:k + width]]
-            except ValueError:
-                raise VoroGPUError(FILE_ERROR, "import")
-            ids.append(i); xyz.append(v[:3])
-            if poly: rr.append(v[3])
-        if len(tok) % width:
-            raise VoroGPUError(FILE_ERROR, "import")
-        if ids:
-            self.put_array(np.array(ids, np.int32), np.array(xyz, np.float64), np.array(rr) if poly else None)
+        n = C.c_longlong()
+        st = lib().vgpu_import(self._h, str(filename).encode(), 1 if poly else 0, C.byref(n))
+        self._nput = getattr(self, "_nput", 0) + n.value
+        self._res = None
+        _check(st, "import")
+
+    def draw_particles(self, filename=None):
+        """draw_particles (src/container.hh:329-347; radii for container_poly)."""
+        return self._draw(lib().vgpu_draw_particles, 0, filename)
+
+    def draw_particles_pov(self, filename=None):
+        return self._draw(lib().vgpu_draw_particles, 1, filename)
+
+    def draw_cells_gnuplot(self, filename=None):
+        """draw_cells_gnuplot (src/container.hh:379-397)."""
+        return self._draw(lib().vgpu_draw_cells, 0, filename)
+
+    def draw_cells_pov(self, filename=None):
+        """draw_cells_pov (src/container.hh:405-424)."""
+        return self._draw(lib().vgpu_draw_cells, 1, filename)
+
+    def _draw(self, fn, pov, filename):
+        self._res = None
+        self._geo = None
+        if filename is not None:
+            _check(fn(self._h, str(filename).encode(), pov), "draw")
+            return None
+        return self._text(fn, pov)
 
     # -- point location, as container::find_voronoi_cell ----------------------
     def find_voronoi_cells(self, xyz):
@@ -395,7 +443,7 @@ class _Base:
         reference's compute_cell() value (False: the cell was removed)."""
         r = self.results()
         m = vl._m
-        c._set(r, m)
+        c._set(r, m, self._geo if getattr(self, "_geo", None) is not None else None)
         return bool(r["exists"][m])
 
 
@@ -450,9 +498,39 @@ class container_periodic_poly(_Base):
 
 class wall_sphere:
     """wall_sphere(xc,yc,zc,rc,w_id=-99), src/wall.hh:27-30."""
+    _type = 0
 
     def __init__(self, xc, yc, zc, rc, w_id=-99):
         self.xc, self.yc, self.zc, self.rc, self.w_id = float(xc), float(yc), float(zc), float(rc), int(w_id)
+        self._args = (self.xc, self.yc, self.zc, self.rc)
+
+
+class wall_plane:
+    """wall_plane(xc,yc,zc,ac,w_id=-99), src/wall.hh:44-47: the half-space
+    x xc + y yc + z zc < ac."""
+    _type = 1
+
+    def __init__(self, xc, yc, zc, ac, w_id=-99):
+        self._args = (float(xc), float(yc), float(zc), float(ac))
+        self.w_id = int(w_id)
+
+
+class wall_cylinder:
+    """wall_cylinder(xc,yc,zc,xa,ya,za,rc,w_id=-99), src/wall.hh:65-68."""
+    _type = 2
+
+    def __init__(self, xc, yc, zc, xa, ya, za, rc, w_id=-99):
+        self._args = tuple(float(t) for t in (xc, yc, zc, xa, ya, za, rc))
+        self.w_id = int(w_id)
+
+
+class wall_cone:
+    """wall_cone(xc,yc,zc,xa,ya,za,ang,w_id=-99), src/wall.hh:97-101."""
+    _type = 3
+
+    def __init__(self, xc, yc, zc, xa, ya, za, ang, w_id=-99):
+        self._args = tuple(float(t) for t in (xc, yc, zc, xa, ya, za, ang))
+        self.w_id = int(w_id)
 
 
 class particle_order:
@@ -585,16 +663,171 @@ class c_loop_all_periodic(c_loop_all):
     blocks by ascending index, then storage order (src/c_loops.hh:325-393)."""
 
 
+class c_loop_subset:
+    """c_loop_subset (src/c_loops.hh:218-258, src/c_loops.cc:24-150) over a
+    rectangular container: setup_sphere / setup_box / setup_intbox choose the
+    blocks (and, with bounds_test, the particles) exactly as the reference's
+    loop does; start() / inc() then step through them in its order.
+    compute_cell(c, vl) gives the cell c_loop_all computes for the particle
+    (the reference's own compute_cell over a subset loop that wraps a
+    periodic direction passes the loop's unwrapped block index to frac_pos,
+    src/container.hh:227-232, and is not reproduced)."""
+
+    def __init__(self, con):
+        self.con = con
+        self._idx = np.zeros(0, np.int64)
+        self._blk = np.zeros((0, 3), np.int32)
+        self._t = 0
+
+    def _setup(self, mode, v=None, iv=None, bounds_test=True):
+        L = lib()
+        vv = np.ascontiguousarray(v if v is not None else np.zeros(6), dtype=np.float64)
+        ii = np.ascontiguousarray(iv if iv is not None else np.zeros(6), dtype=np.int32)
+        n = C.c_longlong()
+        _check(L.vgpu_loop_subset(self.con._h, mode, _p(vv, C.c_double), _p(ii, C.c_int), 1 if bounds_test else 0,
+                                  None, None, 0, C.byref(n)), "vgpu_loop_subset")
+        idx = np.zeros(max(n.value, 1), np.int64)
+        blk = np.zeros((max(n.value, 1), 3), np.int32)
+        _check(L.vgpu_loop_subset(self.con._h, mode, _p(vv, C.c_double), _p(ii, C.c_int), 1 if bounds_test else 0,
+                                  _p(idx, C.c_longlong), _p(blk, C.c_int), n.value, C.byref(n)), "vgpu_loop_subset")
+        self._idx, self._blk = idx[:n.value], blk[:n.value]
+        self._t = 0
+
+    def setup_sphere(self, vx, vy, vz, r, bounds_test=True):
+        self._setup(0, v=[vx, vy, vz, r], bounds_test=bounds_test)
+
+    def setup_box(self, xmin, xmax, ymin, ymax, zmin, zmax, bounds_test=True):
+        self._setup(1, v=[xmin, xmax, ymin, ymax, zmin, zmax], bounds_test=bounds_test)
+
+    def setup_intbox(self, ai, bi, aj, bj, ak, bk):
+        self._setup(2, iv=[ai, bi, aj, bj, ak, bk])
+
+    def start(self):
+        self._t = 0
+        return len(self._idx) > 0
+
+    def inc(self):
+        self._t += 1
+        return self._t < len(self._idx)
+
+    @property
+    def _m(self):
+        return int(self._idx[self._t])
+
+    def _particles(self):
+        if getattr(self, "_pp", None) is None:
+            L = lib()
+            n = C.c_longlong()
+            _check(L.vgpu_get_particles(self.con._h, C.byref(n), None, None, None), "vgpu_get_particles")
+            ids, pos, r = np.zeros(max(n.value, 1), np.int32), np.zeros((max(n.value, 1), 3)), np.zeros(max(n.value, 1))
+            _check(L.vgpu_get_particles(self.con._h, C.byref(n), _p(ids, C.c_int), _p(pos, C.c_double),
+                                        _p(r, C.c_double)), "vgpu_get_particles")
+            self._pp = (ids, pos, r)
+        return self._pp
+
+    def pid(self):
+        return int(self._particles()[0][self._m])
+
+    def pos(self):
+        return tuple(float(t) for t in self._particles()[1][self._m])
+
+    @property
+    def i(self):
+        return int(self._blk[self._t, 0])
+
+    @property
+    def j(self):
+        return int(self._blk[self._t, 1])
+
+    @property
+    def k(self):
+        return int(self._blk[self._t, 2])
+
+
 class voronoicell_neighbor:
     """Read-only view of one computed cell (src/cell.hh:432-549 outputs)."""
 
     def __init__(self, con=None):
         self._r = None
         self._m = 0
+        self._g = None
 
-    def _set(self, r, m):
+    def _set(self, r, m, g=None):
         self._r = r
         self._m = m
+        self._g = g
+
+    # The polyhedron statistics (src/cell.cc) need a compute with
+    # F_GEOMETRY (container.geometry() does one).
+    def _geo(self):
+        if self._g is None:
+            raise RuntimeError("the container was computed without F_GEOMETRY: call its geometry() first")
+        return self._g
+
+    def _faces(self):
+        s, e = self._r["nbr_off"][self._m], self._r["nbr_off"][self._m + 1]
+        return s, e
+
+    def _verts(self):
+        g = self._geo()
+        return g["vert_off"][self._m], g["vert_off"][self._m + 1]
+
+    def vertices(self, x=None, y=None, z=None):
+        """vertices() (local) or vertices(x,y,z) (global), flattened as the reference's vector."""
+        s, e = self._verts()
+        v = self._geo()["vertices"][s:e]
+        if x is not None:
+            v = v + np.array([x, y, z])
+        return [float(t) for t in v.ravel()]
+
+    def vertex_orders(self):
+        s, e = self._verts()
+        return [int(t) for t in self._geo()["orders"][s:e]]
+
+    def face_areas(self):
+        s, e = self._faces()
+        return [float(t) for t in self._geo()["face_area"][s:e]]
+
+    def face_perimeters(self):
+        s, e = self._faces()
+        return [float(t) for t in self._geo()["face_perim"][s:e]]
+
+    def normals(self):
+        s, e = self._faces()
+        return [float(t) for t in self._geo()["normal"][s:e].ravel()]
+
+    def face_orders(self):
+        s, e = self._faces()
+        return [int(t) for t in self._geo()["face_order"][s:e]]
+
+    def face_freq_table(self):
+        v = []
+        for q in self.face_orders():
+            if q >= len(v):
+                v += [0] * (q + 1 - len(v))
+            v[q] += 1
+        return v
+
+    def face_vertices(self):
+        g = self._geo()
+        return [int(t) for t in g["face_verts"][g["fv_off"][self._m]:g["fv_off"][self._m + 1]]]
+
+    def surface_area(self):
+        return float(self._geo()["surface"][self._m])
+
+    def total_edge_distance(self):
+        return float(self._geo()["edge_dist"][self._m])
+
+    def number_of_edges(self):
+        return int(self._geo()["n_edges"][self._m])
+
+    def max_radius_squared(self):
+        return float(self._geo()["maxr2"][self._m])
+
+    @property
+    def p(self):
+        s, e = self._verts()
+        return int(e - s)
 
     def volume(self):
         return float(self._r["vol"][self._m])

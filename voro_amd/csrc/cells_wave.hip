@@ -18,6 +18,43 @@ struct WSink {
 	VG_INL void put(int i, int v) { if (i < (int)WLayoutT<K>::NBCAP) c->put_nb(i, v); }
 };
 
+// Copies a finished wavefront cell's tables into the geometry arena: each
+// lane writes the vertices it holds (order 3: three edges and back pointers
+// from the topology word); a cell that does not fit is counted in top[2].
+template<int K>
+__device__ __forceinline__ void wave_geo_write(WCellT<K> &c, const GeoSink &g, int cell, bool ok) {
+	const int lane = threadIdx.x;
+	if (!ok) {
+		if (lane == 0) { g.vstart[cell] = 0; g.np[cell] = 0; g.esz[cell] = 0; }
+		return;
+	}
+	const int p = c.p;
+	int vs = 0, es = 0;
+	if (lane == 0) { vs = atomicAdd(&g.top[0], p); es = atomicAdd(&g.top[1], 3 * p); }
+	vs = __builtin_amdgcn_readfirstlane(vs);
+	es = __builtin_amdgcn_readfirstlane(es);
+	if ((long long)vs + p > g.vcap || (long long)es + 3 * p > g.ecap) {
+		if (lane == 0) { g.vstart[cell] = -1; g.np[cell] = 0; g.esz[cell] = 0; atomicAdd(&g.top[2], 1); }
+		return;
+	}
+	if (lane == 0) { g.vstart[cell] = vs; g.np[cell] = p; g.estart[cell] = es; g.esz[cell] = 3 * p; }
+	eachk<K>([&](auto hc) __attribute__((always_inline)) {
+		constexpr int h = decltype(hc)::value;
+		const int v = 64 * h + lane;
+		if (v < p) {
+			const size_t a = (size_t)(vs + v);
+			g.xyz[3 * a] = c.vx[h]; g.xyz[3 * a + 1] = c.vy[h]; g.xyz[3 * a + 2] = c.vz[h];
+			g.nu[a] = 3;
+			g.eoff[a] = es + 3 * v;
+			const unsigned w = c.vt[h];
+			for (int i = 0; i < 3; i++) {
+				g.ed[es + 3 * v + i] = (short)WCellT<K>::twE(w, i);
+				g.rel[es + 3 * v + i] = (unsigned char)WCellT<K>::twR(w, i);
+			}
+		}
+	});
+}
+
 // One cell of the container with view v, block b of the view's numbering:
 // compute_cell, then the outputs (lane 0 writes the scalars, every lane its
 // neighbour ids) or, if the cell overflowed, an entry in out's retry list.
@@ -58,6 +95,7 @@ __device__ __forceinline__ void wave_cell(WCellT<K> &c, const RectView &v, int c
 		}
 		if (PRD && out.touch) out.touch[cell] = (unsigned char)stt.img;
 	}
+	if (out.geo.top) wave_geo_write(c, out.geo, cell, ok);
 	if (out.want_nbrs) {
 		int off = 0;
 		if (lane == 0 && nf > 0) off = atomicAdd(out.arena_top, nf);

@@ -6,7 +6,7 @@
 // 797-944 (compute_min_max_radius), 236-264 (queue/mask helpers), together
 // with the container adapters src/container.hh:192-251 (start cell,
 // frac_pos, region_index), src/rad_option.hh:24-158 (radical policy) and
-// src/wall.cc:28-36 (sphere wall).  The candidate order, the count_list
+// src/wall.cc:28-103 (sphere, plane, cylinder and cone walls).  The candidate order, the count_list
 // schedule on which the maximum vertex radius is refreshed and every
 // floating-point expression are kept as in the reference, so the sequence
 // of real cuts, and hence the cell, is identical.
@@ -49,7 +49,7 @@ struct RectView {
 	const int *cstart;      // offsets of the cells (stored particles) per block, nblocks+1
 	const double *mrad;     // 64x64 worklist radii for this block size
 	const unsigned *wl;     // 64x64 worklist table
-	const double *walls;    // sphere walls: xc,yc,zc,rc,w_id
+	const double *walls;    // walls, VGPU_WALL_STRIDE doubles each (wall_cut_plane)
 	int nwalls;
 	// periodic containers
 	double bxy, bxz, byz;
@@ -61,6 +61,58 @@ struct RectView {
 	// block arrays (bstart and cstart above already point at it)
 	int boff;
 };
+
+// A wall record: type (0 sphere, 1 plane, 2 cylinder, 3 cone), w_id, then
+// the wall's constants as the reference's constructors store them
+// (src/wall.hh:27-115): sphere xc,yc,zc,rc; plane xc,yc,zc,ac; cylinder
+// xc,yc,zc,xa,ya,za,asi,rc; cone xc,yc,zc,xa,ya,za,asi,sang,cang,gra.
+enum { VGPU_WALL_STRIDE = 12 };
+
+// The plane a wall cuts the cell of the particle at (x,y,z) with, as the
+// wall's cut_cell_base computes it (src/wall.cc:28-36 sphere, 50-53 plane,
+// 64-74 cylinder, 88-103 cone): c.nplane(px,py,pz,rs,pid).  False if the
+// wall makes no cut (the particle within 1e-5 of the sphere centre or the
+// axis).
+VG_INL bool wall_cut_plane(const double *wp, double x, double y, double z, double &px, double &py, double &pz,
+			   double &rs, int &pid) {
+	const int type = (int)wp[0];
+	pid = (int)wp[1];
+	const double xc = wp[2], yc = wp[3], zc = wp[4];
+	if (type == 1) {
+		px = xc; py = yc; pz = zc;
+		rs = 2 * (wp[5] - x * xc - y * yc - z * zc);
+		return true;
+	}
+	if (type == 0) {
+		const double xd = x - xc, yd = y - yc, zd = z - zc;
+		double dq = xd * xd + yd * yd + zd * zd;
+		if (dq > 1e-5) {
+			dq = 2 * (sqrt(dq) * wp[5] - dq);
+			px = xd; py = yd; pz = zd; rs = dq;
+			return true;
+		}
+		return false;
+	}
+	const double xa = wp[5], ya = wp[6], za = wp[7], asi = wp[8];
+	double xd = x - xc, yd = y - yc, zd = z - zc, pa = (xd * xa + yd * ya + zd * za) * asi;
+	xd -= xa * pa; yd -= ya * pa; zd -= za * pa;
+	pa = xd * xd + yd * yd + zd * zd;
+	if (!(pa > 1e-5)) return false;
+	if (type == 2) {
+		pa = 2 * (sqrt(pa) * wp[9] - pa);
+		px = xd; py = yd; pz = zd; rs = pa;
+		return true;
+	}
+	const double sang = wp[9], cang = wp[10];
+	pa = 1 / sqrt(pa);
+	const double q = sqrt(asi);
+	const double xf = -sang * q * xa + cang * pa * xd;
+	const double yf = -sang * q * ya + cang * pa * yd;
+	const double zf = -sang * q * za + cang * pa * zd;
+	pa = 2 * (xf * (xc - x) + yf * (yc - y) + zf * (zc - z));
+	px = xf; py = yf; pz = zf; rs = pa;
+	return true;
+}
 
 // Per-cell statistics (labels for the candidate predictor).
 struct CellStats {
@@ -470,16 +522,11 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				continue;
 			}
 			if (stage == 0) {
-				// Sphere walls (src/container.hh:86-89, src/wall.cc:28-36).
+				// Walls in the order they were added (src/container.hh:86-89).
 				if (w < v.nwalls) {
-					const double *wp = v.walls + 5 * w;
+					const double *wp = v.walls + VGPU_WALL_STRIDE * w;
 					w++;
-					double xd = x - wp[0], yd = y - wp[1], zd = z - wp[2], dq = xd * xd + yd * yd + zd * zd;
-					if (dq > 1e-5) {
-						dq = 2 * (sqrt(dq) * wp[3] - dq);
-						px_ = xd; py_ = yd; pz_ = zd; rs = dq; pid = (int)wp[4];
-						have = true;
-					}
+					have = wall_cut_plane(wp, x, y, z, px_, py_, pz_, rs, pid);
 					continue;
 				}
 				stage = 1;
@@ -776,15 +823,10 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, int ci, 
 		bool have = false;
 		while (!have) {
 			if (w < v.nwalls) {
-				// Sphere walls (src/container.hh:86-89, src/wall.cc:28-36).
-				const double *wp = v.walls + 5 * w;
+				// Walls in the order they were added (src/container.hh:86-89).
+				const double *wp = v.walls + VGPU_WALL_STRIDE * w;
 				w++;
-				double xd = x - wp[0], yd = y - wp[1], zd = z - wp[2], dq = xd * xd + yd * yd + zd * zd;
-				if (dq > 1e-5) {
-					dq = 2 * (sqrt(dq) * wp[3] - dq);
-					px_ = xd; py_ = yd; pz_ = zd; rs = dq; pid = (int)wp[4];
-					have = true;
-				}
+				have = wall_cut_plane(wp, x, y, z, px_, py_, pz_, rs, pid);
 				continue;
 			}
 			if constexpr (LaneSteps<CELL>::value) {

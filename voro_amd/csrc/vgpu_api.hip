@@ -129,6 +129,17 @@ struct vgpu_container {
 	std::vector<dbl4> hP;
 	std::vector<int> hpid, hcs;
 	int image_passes = 0;
+	// neighbour arena size that held the last compute (grown when it overflows)
+	long long acap_hint = 0;
+	// polyhedron statistics (VGPU_F_GEOMETRY): the cell tables kept by the
+	// tiers, then the loop-order statistics computed from them
+	DBuf geo_top, geo_vstart, geo_np, geo_estart, geo_esz, geo_xyz, geo_nu, geo_eoff, geo_ed, geo_rel;
+	DBuf go_voff, go_eoff, go_fvoff, go_nv, go_ne, go_nfv;
+	DBuf go_vert, go_ord, go_edges, go_erel, go_farea, go_fperim, go_normal, go_forder, go_fverts;
+	DBuf go_surface, go_edist, go_nedges, go_maxr2;
+	long long geo_vcap_hint = 0, geo_ecap_hint = 0;
+	long long geo_nv = 0, geo_ne = 0, geo_nfv = 0, geo_nf = 0;
+	bool geo_valid = false;
 };
 
 namespace {
@@ -192,7 +203,7 @@ RectView make_view(vgpu_container *h) {
 	v.max_radius = h->max_radius;
 	v.P = h->P.as<dbl4>(); v.pid = h->pid.as<int>(); v.bstart = h->bstart.as<int>(); v.cstart = h->bstart.as<int>();
 	v.mrad = h->d_mrad.as<double>(); v.wl = h->d_wl.as<unsigned>();
-	v.walls = h->d_walls.as<double>(); v.nwalls = (int)(h->walls.size() / 5);
+	v.walls = h->d_walls.as<double>(); v.nwalls = (int)(h->walls.size() / VGPU_WALL_STRIDE);
 	const PrdGeom &pg = h->pg;
 	v.bxy = h->prd ? pg.bxy : 0; v.bxz = h->prd ? pg.bxz : 0; v.byz = h->prd ? pg.byz : 0;
 	v.ey = pg.ey; v.ez = pg.ez; v.wy = pg.wy; v.wz = pg.wz; v.oy = pg.oy; v.oz = pg.oz;
@@ -515,6 +526,55 @@ int run_prd_cells(vgpu_container *h, RectView &v, CellOut out, int nc, int fw, h
 	return VGPU_INTERNAL_ERROR;   // the first-visit order did not settle
 }
 
+// Internal status: a capacity grew, compute again.
+const int VGPU_RETRY = -100;
+
+// The statistics of every cell from the tables the tiers kept: output sizes
+// in loop order, their scans, then one thread per cell.  The face offsets
+// are the neighbour CSR's (h->tmp after the CSR stage).
+int geometry_pass(vgpu_container *h, const GeoSink &g, int nc, hipStream_t s) {
+	const size_t C1 = (size_t)nc + 1;
+	int st;
+	if ((st = h->go_nv.ensure(C1 * 4)) || (st = h->go_ne.ensure(C1 * 4)) || (st = h->go_nfv.ensure(C1 * 4)) ||
+	    (st = h->go_voff.ensure(C1 * 4)) || (st = h->go_eoff.ensure(C1 * 4)) || (st = h->go_fvoff.ensure(C1 * 4)))
+		return st;
+	if (launch_geo_sizes(s, nc, h->status.as<int>(), h->nfaces.as<int>(), g, h->go_nv.as<int>(), h->go_ne.as<int>(),
+			     h->go_nfv.as<int>()))
+		return VGPU_HIP_ERROR;
+	if ((st = scan_excl(h->part, h->go_nv.as<int>(), h->go_voff.as<int>(), nc + 1, s)) ||
+	    (st = scan_excl(h->part, h->go_ne.as<int>(), h->go_eoff.as<int>(), nc + 1, s)) ||
+	    (st = scan_excl(h->part, h->go_nfv.as<int>(), h->go_fvoff.as<int>(), nc + 1, s)))
+		return st;
+	int tot[4];
+	HIPCHK(hipMemcpyAsync(&tot[0], h->go_voff.as<int>() + nc, 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(&tot[1], h->go_eoff.as<int>() + nc, 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(&tot[2], h->go_fvoff.as<int>() + nc, 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(&tot[3], h->tmp.as<int>() + nc, 4, hipMemcpyDeviceToHost, s));   // faces: the CSR total
+	HIPCHK(hipStreamSynchronize(s));
+	const size_t nv = (size_t)std::max(tot[0], 1), ne = (size_t)std::max(tot[1], 1), nfv = (size_t)std::max(tot[2], 1);
+	const size_t nf = (size_t)std::max(tot[3], 1);
+	if ((st = h->go_vert.ensure(nv * 24)) || (st = h->go_ord.ensure(nv * 4)) || (st = h->go_edges.ensure(ne * 4)) ||
+	    (st = h->go_erel.ensure(ne * 4)) || (st = h->go_farea.ensure(nf * 8)) || (st = h->go_fperim.ensure(nf * 8)) ||
+	    (st = h->go_normal.ensure(nf * 24)) || (st = h->go_forder.ensure(nf * 4)) ||
+	    (st = h->go_fverts.ensure(nfv * 4)) || (st = h->go_surface.ensure(C1 * 8)) ||
+	    (st = h->go_edist.ensure(C1 * 8)) || (st = h->go_nedges.ensure(C1 * 4)) || (st = h->go_maxr2.ensure(C1 * 8)))
+		return st;
+	GeoStats o;
+	o.status = h->status.as<int>(); o.face_off = h->tmp.as<int>(); o.vert_off = h->go_voff.as<int>();
+	o.edge_off = h->go_eoff.as<int>(); o.fv_off = h->go_fvoff.as<int>();
+	o.tol = 10.0 * 2.220446049250313e-16 * h->max_len_sq;
+	o.vertices = h->go_vert.as<double>(); o.orders = h->go_ord.as<int>(); o.edges = h->go_edges.as<int>();
+	o.edge_rel = h->go_erel.as<int>(); o.face_area = h->go_farea.as<double>(); o.face_perim = h->go_fperim.as<double>();
+	o.normal = h->go_normal.as<double>(); o.face_order = h->go_forder.as<int>(); o.face_verts = h->go_fverts.as<int>();
+	o.surface = h->go_surface.as<double>(); o.edge_dist = h->go_edist.as<double>(); o.n_edges = h->go_nedges.as<int>();
+	o.maxr2 = h->go_maxr2.as<double>();
+	if (launch_geo_stats(s, nc, g, o)) return VGPU_HIP_ERROR;
+	HIPCHK(hipStreamSynchronize(s));
+	h->geo_nv = tot[0]; h->geo_ne = tot[1]; h->geo_nfv = tot[2]; h->geo_nf = tot[3];
+	h->geo_valid = true;
+	return VGPU_OK;
+}
+
 int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s, bool bin_only = false) {
 	const int nxyz = h->nblocks;
 	int n = h->n_in;
@@ -594,7 +654,8 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s, bool bin_only
 	const size_t C1 = (size_t)std::max(nc, 1);
 	const bool nbrs = (flags & VGPU_F_NEIGHBORS) != 0;
 	// periodic containers recompute the cells near image blocks: room for both
-	long long acap = nbrs ? std::min<long long>(2147483000LL, (h->prd ? 48LL : 24LL) * C1 + 4096) : 1;
+	long long acap = nbrs ? std::min<long long>(2147483000LL, std::max<long long>(h->acap_hint, (h->prd ? 48LL : 24LL) * (long long)C1 + 4096))
+			      : 1;
 	if ((st = h->vol.ensure(C1 * 8)) || (st = h->cent.ensure(C1 * 24)) || (st = h->nfaces.ensure(C1 * 4)) ||
 	    (st = h->status.ensure(C1 * 4)) || (st = h->nbr_start.ensure(C1 * 4)) || (st = h->arena.ensure((size_t)acap * 4)) ||
 	    (st = h->ctr.ensure(16 * 4)) || (st = h->retry0.ensure(C1 * 4)) || (st = h->retry1.ensure(C1 * 4)) ||
@@ -618,10 +679,32 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s, bool bin_only
 	out.st_tier = stats ? h->st_tier.as<signed char>() : nullptr;
 	out.want_nbrs = nbrs ? 1 : 0;
 	out.touch = nullptr;
+	const bool geometry = (flags & VGPU_F_GEOMETRY) != 0 && nbrs && (flags & VGPU_F_CSR);
+	h->geo_valid = false;
+	if (geometry) {
+		// about 27 vertices per cell on random points; grown and computed
+		// again if a cell does not fit
+		const long long vcap = std::min<long long>(2147483000LL,
+			std::max(h->geo_vcap_hint, (h->prd ? 64LL : 40LL) * (long long)C1 + 4096));
+		const long long ecap = std::min<long long>(2147483000LL, std::max(h->geo_ecap_hint, 3 * vcap + 4096));
+		if ((st = h->geo_top.ensure(4 * 4)) || (st = h->geo_vstart.ensure(C1 * 4)) || (st = h->geo_np.ensure(C1 * 4)) ||
+		    (st = h->geo_estart.ensure(C1 * 4)) || (st = h->geo_esz.ensure(C1 * 4)) ||
+		    (st = h->geo_xyz.ensure((size_t)vcap * 24)) || (st = h->geo_nu.ensure((size_t)vcap)) ||
+		    (st = h->geo_eoff.ensure((size_t)vcap * 4)) || (st = h->geo_ed.ensure((size_t)ecap * 2)) ||
+		    (st = h->geo_rel.ensure((size_t)ecap)))
+			return st;
+		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, h->geo_top.as<int>(), 4, 0);
+		GeoSink &g = out.geo;
+		g.top = h->geo_top.as<int>(); g.vcap = (int)vcap; g.ecap = (int)ecap;
+		g.vstart = h->geo_vstart.as<int>(); g.np = h->geo_np.as<int>(); g.estart = h->geo_estart.as<int>();
+		g.esz = h->geo_esz.as<int>(); g.xyz = h->geo_xyz.as<double>(); g.nu = h->geo_nu.as<unsigned char>();
+		g.eoff = h->geo_eoff.as<int>(); g.ed = h->geo_ed.as<short>(); g.rel = h->geo_rel.as<unsigned char>();
+	}
 
 	int fw = h->fast_waves > 0 ? h->fast_waves : default_fast_waves(h->fast_occ);
 	fw = std::max(1, std::min(fw, nblk(std::max(nc, 1), 64)));
-	const bool learned = (flags & VGPU_F_LEARNED) != 0;
+	// the cell tables are kept by the plain tiers only
+	const bool learned = (flags & VGPU_F_LEARNED) != 0 && !geometry;
 	h->last_fallbacks = 0;
 	h->last_entries = 0;
 	h->image_passes = 0;
@@ -658,6 +741,26 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s, bool bin_only
 	int hc[8];
 	HIPCHK(hipMemcpyAsync(hc, ctr, 32, hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
+	// a full neighbour arena: grow it and compute again
+	if (nbrs && (long long)(unsigned)hc[0] > acap) {
+		h->acap_hint = (long long)(unsigned)hc[0] + (long long)(unsigned)hc[0] / 8 + 4096;
+		h->computed = false;
+		return VGPU_RETRY;
+	}
+	h->nbr_total = hc[0];
+	if (geometry && nc > 0) {
+		int gt[4];
+		HIPCHK(hipMemcpyAsync(gt, h->geo_top.p, 16, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipStreamSynchronize(s));
+		if (gt[2] > 0) {
+			// a cell's tables did not fit: room for everything asked for, and more
+			h->geo_vcap_hint = (long long)(unsigned)gt[0] + (long long)(unsigned)gt[0] / 4 + 4096;
+			h->geo_ecap_hint = (long long)(unsigned)gt[1] + (long long)(unsigned)gt[1] / 4 + 4096;
+			h->computed = false;
+			return VGPU_RETRY;
+		}
+		if (hc[3] == 0 && (st = geometry_pass(h, out.geo, nc, s))) return st;
+	}
 	float ms;
 	for (int t = 0; t < 4; t++) {
 		HIPCHK(hipEventElapsedTime(&ms, h->ev[t], h->ev[t + 1]));
@@ -996,7 +1099,7 @@ int vgpu_create(int kind, const double *geom, const int *grid, const int *period
 	}
 	std::vector<double> mrad(4096);
 	worklist_radii(g.boxx, g.boxy, g.boxz, mrad.data());
-	if (h->d_mrad.ensure(4096 * 8) || h->d_wl.ensure(4096 * 4) || h->d_walls.ensure(8 * 5 * 64)) {
+	if (h->d_mrad.ensure(4096 * 8) || h->d_wl.ensure(4096 * 4) || h->d_walls.ensure(8 * VGPU_WALL_STRIDE * 64)) {
 		delete h;
 		return VGPU_MEMORY_ERROR;
 	}
@@ -1028,15 +1131,62 @@ int vgpu_destroy(vgpu_container *h) {
 	return VGPU_OK;
 }
 
-int vgpu_add_wall_sphere(vgpu_container *h, double xc, double yc, double zc, double rc, int w_id) {
-	if (!h || h->prd) return VGPU_INVALID_ARGUMENT;
-	if ((int)h->walls.size() / 5 >= 64) return VGPU_MEMORY_ERROR;
-	double w[5] = {xc, yc, zc, rc, (double)w_id};
-	h->walls.insert(h->walls.end(), w, w + 5);
+int vgpu_add_wall(vgpu_container *h, int type, const double *prm, int w_id) {
+	if (!h || h->prd || !prm || type < 0 || type > 3) return VGPU_INVALID_ARGUMENT;
+	if ((int)(h->walls.size() / VGPU_WALL_STRIDE) >= 64) return VGPU_MEMORY_ERROR;
+	double w[VGPU_WALL_STRIDE] = {0};
+	w[0] = type; w[1] = w_id;
+	w[2] = prm[0]; w[3] = prm[1]; w[4] = prm[2];
+	if (type <= 1) w[5] = prm[3];
+	else {
+		// the constants the reference's constructors derive (src/wall.hh:83-85, 104-107)
+		const double xa = prm[3], ya = prm[4], za = prm[5];
+		w[5] = xa; w[6] = ya; w[7] = za;
+		w[8] = 1 / (xa * xa + ya * ya + za * za);
+		if (type == 2) w[9] = prm[6];
+		else { const double ang = prm[6]; w[9] = sin(ang); w[10] = cos(ang); w[11] = tan(ang); }
+	}
+	h->walls.insert(h->walls.end(), w, w + VGPU_WALL_STRIDE);
 	if (hipMemcpy(h->d_walls.p, h->walls.data(), h->walls.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
 		return VGPU_HIP_ERROR;
 	h->computed = false;
 	h->binned = false;
+	return VGPU_OK;
+}
+
+int vgpu_add_wall_sphere(vgpu_container *h, double xc, double yc, double zc, double rc, int w_id) {
+	const double prm[4] = {xc, yc, zc, rc};
+	return vgpu_add_wall(h, 0, prm, w_id);
+}
+
+// container_base::point_inside (src/container.cc:484-487) with each wall's
+// point_inside (src/wall.cc:17-19, 44-46, 55-60, 77-85).
+int vgpu_point_inside(vgpu_container *h, double x, double y, double z, int *inside) {
+	if (!h || !inside || h->prd) return VGPU_INVALID_ARGUMENT;
+	const RectGeom &g = h->g;
+	*inside = 0;
+	if (x < g.ax || x > g.bx || y < g.ay || y > g.by || z < g.az || z > g.bz) return VGPU_OK;
+	for (size_t o = 0; o < h->walls.size(); o += VGPU_WALL_STRIDE) {
+		const double *w = h->walls.data() + o;
+		const int type = (int)w[0];
+		const double xc = w[2], yc = w[3], zc = w[4];
+		bool in;
+		if (type == 0) in = (x - xc) * (x - xc) + (y - yc) * (y - yc) + (z - zc) * (z - zc) < w[5] * w[5];
+		else if (type == 1) in = x * xc + y * yc + z * zc < w[5];
+		else {
+			double xd = x - xc, yd = y - yc, zd = z - zc;
+			double pa = (xd * w[5] + yd * w[6] + zd * w[7]) * w[8];
+			xd -= w[5] * pa; yd -= w[6] * pa; zd -= w[7] * pa;
+			if (type == 2) in = xd * xd + yd * yd + zd * zd < w[9] * w[9];
+			else {
+				pa *= w[11];
+				if (pa < 0) in = false;
+				else { pa *= pa; in = xd * xd + yd * yd + zd * zd < pa; }
+			}
+		}
+		if (!in) return VGPU_OK;
+	}
+	*inside = 1;
 	return VGPU_OK;
 }
 
@@ -1105,7 +1255,12 @@ int vgpu_clear(vgpu_container *h) {
 int vgpu_compute_all(vgpu_container *h, unsigned flags, void *stream) {
 	if (h && (flags & VGPU_F_LEARNED) && (!h->have_pred || h->prd)) return VGPU_INVALID_ARGUMENT;
 	if (!h) return VGPU_INVALID_ARGUMENT;
-	return run_pipeline(h, flags, (hipStream_t)stream);
+	if (flags & VGPU_F_GEOMETRY) flags |= VGPU_F_NEIGHBORS | VGPU_F_CSR;
+	// a capacity that overflowed (neighbour ids, cell tables) has been grown:
+	// compute again (each retry at least doubles what was missing)
+	int st = VGPU_RETRY;
+	for (int attempt = 0; attempt < 6 && st == VGPU_RETRY; attempt++) st = run_pipeline(h, flags, (hipStream_t)stream);
+	return st == VGPU_RETRY ? VGPU_MEMORY_ERROR : st;
 }
 
 int vgpu_find_voronoi_cells(vgpu_container *h, int n, const double *xyz, int *found, int *pid, double *pos,
@@ -1240,6 +1395,189 @@ int vgpu_get_put_index(vgpu_container *h, int *put_index) {
 	return VGPU_OK;
 }
 
+int vgpu_kind(vgpu_container *h, int *kind) {
+	if (!h || !kind) return VGPU_INVALID_ARGUMENT;
+	*kind = h->kind;
+	return VGPU_OK;
+}
+
+int vgpu_bin(vgpu_container *h) {
+	if (!h) return VGPU_INVALID_ARGUMENT;
+	if (h->binned) return VGPU_OK;
+	return run_pipeline(h, 0, nullptr, true);
+}
+
+int vgpu_get_particles(vgpu_container *h, long long *n, int *id, double *pos, double *r) {
+	if (!h) return VGPU_INVALID_ARGUMENT;
+	if (!h->binned) return VGPU_INTERNAL_ERROR;
+	const int nc = h->ncells;
+	if (n) *n = nc;
+	if (nc == 0 || (!id && !pos && !r)) return VGPU_OK;
+	if (id) HIPCHK(hipMemcpy(id, h->pid.p, (size_t)nc * 4, hipMemcpyDeviceToHost));
+	if (pos || r) {
+		std::vector<dbl4> P(nc);
+		HIPCHK(hipMemcpy(P.data(), h->P.p, (size_t)nc * 32, hipMemcpyDeviceToHost));
+		for (int c = 0; c < nc; c++) {
+			if (pos) { pos[3 * c] = P[c].x; pos[3 * c + 1] = P[c].y; pos[3 * c + 2] = P[c].z; }
+			// the stored radius; default_radius (src/config.hh:90) for the plain kinds
+			if (r) r[c] = h->poly ? P[c].w : 0.5;
+		}
+	}
+	return VGPU_OK;
+}
+
+// c_loop_subset, src/c_loops.hh:218-258 and src/c_loops.cc:24-150: the
+// loop's state machine over the stored block counts and positions, every
+// step as the reference takes it (including its quirks: the first row
+// starts at the wrapped index, later rows at the unwrapped one).
+namespace {
+struct SubsetLoop {
+	enum Mode { no_check, sphere, box };
+	Mode mode = no_check;
+	int nx, ny, nz, nxy, nxyz;
+	double ax, ay, az, sx, sy, sz, xsp, ysp, zsp;
+	bool xper, yper, zper;
+	const int *co;            // particles per block
+	const int *bs;            // first loop index of each block
+	const dbl4 *P;            // stored positions, loop order
+	double px = 0, py = 0, pz = 0, apx = 0, apy = 0, apz = 0;
+	double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+	int ai = 0, bi = 0, aj = 0, bj = 0, ak = 0, bk = 0;
+	int ci = 0, cj = 0, ck = 0, di = 0, dj = 0, dk = 0, inc1 = 0, inc2 = 0;
+	int i = 0, j = 0, k = 0, ijk = 0, q = 0;
+	static int step_mod(int a, int b) { return a >= 0 ? a % b : b - 1 - (b - 1 - a) % b; }
+	static int step_div(int a, int b) { return a >= 0 ? a / b : -1 + (a + 1) / b; }
+	static int step_int(double a) { return a < 0 ? int(a) - 1 : int(a); }
+	void setup_sphere(double vx, double vy, double vz, double r, bool bounds_test) {
+		if (bounds_test) { mode = sphere; v0 = vx; v1 = vy; v2 = vz; v3 = r * r; } else mode = no_check;
+		ai = step_int((vx - ax - r) * xsp);
+		bi = step_int((vx - ax + r) * xsp);
+		aj = step_int((vy - ay - r) * ysp);
+		bj = step_int((vy - ay + r) * ysp);
+		ak = step_int((vz - az - r) * zsp);
+		bk = step_int((vz - az + r) * zsp);
+		setup_common();
+	}
+	void setup_box(double xmin, double xmax, double ymin, double ymax, double zmin, double zmax, bool bounds_test) {
+		if (bounds_test) { mode = box; v0 = xmin; v1 = xmax; v2 = ymin; v3 = ymax; v4 = zmin; v5 = zmax; }
+		else mode = no_check;
+		ai = step_int((xmin - ax) * xsp);
+		bi = step_int((xmax - ax) * xsp);
+		aj = step_int((ymin - ay) * ysp);
+		bj = step_int((ymax - ay) * ysp);
+		ak = step_int((zmin - az) * zsp);
+		bk = step_int((zmax - az) * zsp);
+		setup_common();
+	}
+	void setup_intbox(int ai_, int bi_, int aj_, int bj_, int ak_, int bk_) {
+		ai = ai_; bi = bi_; aj = aj_; bj = bj_; ak = ak_; bk = bk_;
+		mode = no_check;
+		setup_common();
+	}
+	void setup_common() {
+		if (!xper) {
+			if (ai < 0) { ai = 0; if (bi < 0) bi = 0; }
+			if (bi >= nx) { bi = nx - 1; if (ai >= nx) ai = nx - 1; }
+		}
+		if (!yper) {
+			if (aj < 0) { aj = 0; if (bj < 0) bj = 0; }
+			if (bj >= ny) { bj = ny - 1; if (aj >= ny) aj = ny - 1; }
+		}
+		if (!zper) {
+			if (ak < 0) { ak = 0; if (bk < 0) bk = 0; }
+			if (bk >= nz) { bk = nz - 1; if (ak >= nz) ak = nz - 1; }
+		}
+		ci = ai; cj = aj; ck = ak;
+		di = i = step_mod(ci, nx); apx = px = step_div(ci, nx) * sx;
+		dj = j = step_mod(cj, ny); apy = py = step_div(cj, ny) * sy;
+		dk = k = step_mod(ck, nz); apz = pz = step_div(ck, nz) * sz;
+		inc1 = di - step_mod(bi, nx);
+		inc2 = nx * (ny + dj - step_mod(bj, ny)) + inc1;
+		inc1 += nx;
+		ijk = di + nx * (dj + ny * dk);
+		q = 0;
+	}
+	bool out_of_bounds() const {
+		const dbl4 &pp = P[bs[ijk] + q];
+		if (mode == sphere) {
+			const double fx = pp.x + px - v0, fy = pp.y + py - v1, fz = pp.z + pz - v2;
+			return fx * fx + fy * fy + fz * fz > v3;
+		}
+		double f = pp.x + px;
+		if (f < v0 || f > v1) return true;
+		f = pp.y + py;
+		if (f < v2 || f > v3) return true;
+		f = pp.z + pz;
+		return f < v4 || f > v5;
+	}
+	bool next_block() {
+		if (i < bi) {
+			i++;
+			if (ci < nx - 1) { ci++; ijk++; } else { ci = 0; ijk += 1 - nx; px += sx; }
+			return true;
+		} else if (j < bj) {
+			i = ai; ci = di; px = apx; j++;
+			if (cj < ny - 1) { cj++; ijk += inc1; } else { cj = 0; ijk += inc1 - nxy; py += sy; }
+			return true;
+		} else if (k < bk) {
+			i = ai; ci = di; j = aj; cj = dj; px = apx; py = apy; k++;
+			if (ck < nz - 1) { ck++; ijk += inc2; } else { ck = 0; ijk += inc2 - nxyz; pz += sz; }
+			return true;
+		}
+		return false;
+	}
+	bool start() {
+		while (co[ijk] == 0) { if (!next_block()) return false; }
+		while (mode != no_check && out_of_bounds()) {
+			q++;
+			while (q >= co[ijk]) { q = 0; if (!next_block()) return false; }
+		}
+		return true;
+	}
+	bool inc() {
+		do {
+			q++;
+			while (q >= co[ijk]) { q = 0; if (!next_block()) return false; }
+		} while (mode != no_check && out_of_bounds());
+		return true;
+	}
+};
+}  // namespace
+
+int vgpu_loop_subset(vgpu_container *h, int mode, const double *v, const int *iv, int bounds_test, long long *index,
+		     int *ijk_pos, long long cap, long long *n) {
+	if (!h || !n || h->prd || mode < 0 || mode > 2 || (mode < 2 && !v) || (mode == 2 && !iv)) return VGPU_INVALID_ARGUMENT;
+	int st;
+	if (!h->binned && (st = run_pipeline(h, 0, nullptr, true))) return st;
+	const int nb = h->nblocks, nc = h->ncells;
+	std::vector<int> bs(nb + 1), co(nb);
+	std::vector<dbl4> P(std::max(nc, 1));
+	HIPCHK(hipMemcpy(bs.data(), h->bstart.p, (size_t)(nb + 1) * 4, hipMemcpyDeviceToHost));
+	if (nc) HIPCHK(hipMemcpy(P.data(), h->P.p, (size_t)nc * 32, hipMemcpyDeviceToHost));
+	for (int b = 0; b < nb; b++) co[b] = bs[b + 1] - bs[b];
+	const RectGeom &g = h->g;
+	SubsetLoop L;
+	L.nx = g.nx; L.ny = g.ny; L.nz = g.nz; L.nxy = g.nx * g.ny; L.nxyz = L.nxy * g.nz;
+	L.ax = g.ax; L.ay = g.ay; L.az = g.az;
+	L.sx = g.bx - g.ax; L.sy = g.by - g.ay; L.sz = g.bz - g.az;
+	L.xsp = g.xsp; L.ysp = g.ysp; L.zsp = g.zsp;
+	L.xper = g.xper; L.yper = g.yper; L.zper = g.zper;
+	L.co = co.data(); L.bs = bs.data(); L.P = P.data();
+	if (mode == 0) L.setup_sphere(v[0], v[1], v[2], v[3], bounds_test != 0);
+	else if (mode == 1) L.setup_box(v[0], v[1], v[2], v[3], v[4], v[5], bounds_test != 0);
+	else L.setup_intbox(iv[0], iv[1], iv[2], iv[3], iv[4], iv[5]);
+	long long m = 0;
+	if (nc > 0 && L.start()) do {
+		if (m < cap) {
+			if (index) index[m] = bs[L.ijk] + L.q;
+			if (ijk_pos) { ijk_pos[3 * m] = L.i; ijk_pos[3 * m + 1] = L.j; ijk_pos[3 * m + 2] = L.k; }
+		}
+		m++;
+	} while (L.inc());
+	*n = m;
+	return VGPU_OK;
+}
+
 int vgpu_num_cells(vgpu_container *h, long long *n) {
 	if (!h || !n) return VGPU_INVALID_ARGUMENT;
 	if (!h->computed) return VGPU_INVALID_ARGUMENT;
@@ -1304,6 +1642,48 @@ int vgpu_get_results(vgpu_container *h, int *id, int *ijk, int *q, int *exists, 
 			}
 		}
 	}
+	return VGPU_OK;
+}
+
+int vgpu_geometry_sizes(vgpu_container *h, long long *n_vertices, long long *n_edge_slots, long long *n_faces,
+			long long *n_face_vertex_entries) {
+	if (!h) return VGPU_INVALID_ARGUMENT;
+	if (!h->computed || !h->geo_valid) return VGPU_INTERNAL_ERROR;
+	if (n_vertices) *n_vertices = h->geo_nv;
+	if (n_edge_slots) *n_edge_slots = h->geo_ne;
+	if (n_faces) *n_faces = h->geo_nf;
+	if (n_face_vertex_entries) *n_face_vertex_entries = h->geo_nfv;
+	return VGPU_OK;
+}
+
+int vgpu_get_geometry(vgpu_container *h, long long *vert_off, double *vertices, int *vertex_orders, long long *edge_off,
+		      int *edges, int *edge_rel, double *face_areas, double *face_perimeters, double *normals,
+		      int *face_orders, long long *fv_off, int *face_vertices, double *surface_area,
+		      double *total_edge_distance, int *number_of_edges, double *max_radius_squared) {
+	if (!h) return VGPU_INVALID_ARGUMENT;
+	if (!h->computed || !h->geo_valid) return VGPU_INTERNAL_ERROR;
+	const int nc = h->ncells;
+	hipStream_t s = nullptr;
+	const size_t nv = (size_t)h->geo_nv, ne = (size_t)h->geo_ne, nfv = (size_t)h->geo_nfv, nf = (size_t)h->geo_nf;
+	auto offs = [&](DBuf &d, long long *out) -> int {
+		if (!out) return VGPU_OK;
+		std::vector<int> t((size_t)nc + 1);
+		if (hipMemcpy(t.data(), d.p, t.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return VGPU_HIP_ERROR;
+		for (size_t i = 0; i < t.size(); i++) out[i] = t[i];
+		return VGPU_OK;
+	};
+	int st;
+	if ((st = offs(h->go_voff, vert_off)) || (st = offs(h->go_eoff, edge_off)) || (st = offs(h->go_fvoff, fv_off)))
+		return st;
+	struct Cp { void *dst; const DBuf *src; size_t bytes; } cps[] = {
+		{vertices, &h->go_vert, nv * 24}, {vertex_orders, &h->go_ord, nv * 4}, {edges, &h->go_edges, ne * 4},
+		{edge_rel, &h->go_erel, ne * 4}, {face_areas, &h->go_farea, nf * 8}, {face_perimeters, &h->go_fperim, nf * 8},
+		{normals, &h->go_normal, nf * 24}, {face_orders, &h->go_forder, nf * 4}, {face_vertices, &h->go_fverts, nfv * 4},
+		{surface_area, &h->go_surface, (size_t)nc * 8}, {total_edge_distance, &h->go_edist, (size_t)nc * 8},
+		{number_of_edges, &h->go_nedges, (size_t)nc * 4}, {max_radius_squared, &h->go_maxr2, (size_t)nc * 8}};
+	for (const Cp &c : cps)
+		if (c.dst && c.bytes) HIPCHK(hipMemcpyAsync(c.dst, c.src->p, c.bytes, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
 	return VGPU_OK;
 }
 
@@ -1485,7 +1865,7 @@ int vgpu_batch_create(vgpu_container *const *hs, int count, vgpu_batch **out) {
 		if (!hs[c] || hs[c]->kind != kind) return VGPU_INVALID_ARGUMENT;
 		n += hs[c]->n_in;
 		nb += hs[c]->nblocks;
-		nw += (long long)hs[c]->walls.size() / 5;
+		nw += (long long)(hs[c]->walls.size() / VGPU_WALL_STRIDE);
 	}
 	if (n > 2000000000LL / 32 || nb + 1 > 2000000000LL) return VGPU_MEMORY_ERROR;
 	vgpu_batch *b = new vgpu_batch();
@@ -1499,7 +1879,7 @@ int vgpu_batch_create(vgpu_container *const *hs, int count, vgpu_batch **out) {
 	for (int c = 0; c < count; c++) {
 		b->h_boff[c + 1] = b->h_boff[c] + hs[c]->nblocks;
 		b->h_pfirst[c + 1] = b->h_pfirst[c] + hs[c]->n_in;
-		woff[c + 1] = woff[c] + (int)hs[c]->walls.size() / 5;
+		woff[c + 1] = woff[c] + (int)(hs[c]->walls.size() / VGPU_WALL_STRIDE);
 	}
 	int st = VGPU_OK;
 	auto fail = [&](int e) { vgpu_batch_destroy(b); return e; };
@@ -1508,12 +1888,12 @@ int vgpu_batch_create(vgpu_container *const *hs, int count, vgpu_batch **out) {
 	    (st = b->geoms.ensure(count * sizeof(RectGeom))) || (st = b->boff.ensure(C1 * 4)) ||
 	    (st = b->pfirst.ensure(C1 * 4)) || (st = b->pcont.ensure(N1 * 4)) ||
 	    (st = b->mrad.ensure((size_t)count * 4096 * 8)) || (st = b->d_wl.ensure(4096 * 4)) ||
-	    (st = b->walls.ensure((size_t)std::max<long long>(nw, 1) * 5 * 8)) ||
+	    (st = b->walls.ensure((size_t)std::max<long long>(nw, 1) * VGPU_WALL_STRIDE * 8)) ||
 	    (st = b->views.ensure(count * sizeof(RectView))))
 		return fail(st);
 	// particles, in container order
 	std::vector<int> ids((size_t)b->n);
-	std::vector<double> xyz((size_t)b->n * 3), r(b->poly ? (size_t)b->n : 0), wall((size_t)nw * 5);
+	std::vector<double> xyz((size_t)b->n * 3), r(b->poly ? (size_t)b->n : 0), wall((size_t)nw * VGPU_WALL_STRIDE);
 	std::vector<RectGeom> geo(count);
 	for (int c = 0; c < count; c++) {
 		vgpu_container *h = hs[c];
@@ -1528,7 +1908,8 @@ int vgpu_batch_create(vgpu_container *const *hs, int count, vgpu_batch **out) {
 			std::memcpy(xyz.data() + 3 * o, h->h_xyz.data(), m * 24);
 			if (b->poly) std::memcpy(r.data() + o, h->h_r.data(), m * 8);
 		}
-		if (!h->walls.empty()) std::memcpy(wall.data() + (size_t)woff[c] * 5, h->walls.data(), h->walls.size() * 8);
+		if (!h->walls.empty())
+			std::memcpy(wall.data() + (size_t)woff[c] * VGPU_WALL_STRIDE, h->walls.data(), h->walls.size() * 8);
 		geo[c] = h->g;
 	}
 	if ((b->n && (hipMemcpy(b->in_ids.p, ids.data(), (size_t)b->n * 4, hipMemcpyHostToDevice) != hipSuccess ||
@@ -1552,7 +1933,7 @@ int vgpu_batch_create(vgpu_container *const *hs, int count, vgpu_batch **out) {
 		v.P = b->P.as<dbl4>(); v.pid = b->pid.as<int>();
 		v.bstart = b->bstart.as<int>() + b->h_boff[c]; v.cstart = v.bstart;
 		v.mrad = b->mrad.as<double>() + (size_t)c * 4096; v.wl = b->d_wl.as<unsigned>();
-		v.walls = b->walls.as<double>() + (size_t)woff[c] * 5; v.nwalls = woff[c + 1] - woff[c];
+		v.walls = b->walls.as<double>() + (size_t)woff[c] * VGPU_WALL_STRIDE; v.nwalls = woff[c + 1] - woff[c];
 		v.max_radius = 0;
 		v.unit = nullptr; v.unit_p = 0; v.fv = nullptr;
 		v.boff = b->h_boff[c];
@@ -1583,7 +1964,7 @@ int vgpu_batch_destroy(vgpu_batch *b) {
 }
 
 int vgpu_batch_compute(vgpu_batch *b, unsigned flags, void *stream) {
-	if (!b || (flags & VGPU_F_LEARNED)) return VGPU_INVALID_ARGUMENT;
+	if (!b || (flags & (VGPU_F_LEARNED | VGPU_F_GEOMETRY))) return VGPU_INVALID_ARGUMENT;
 	return batch_run(b, flags, (hipStream_t)stream);
 }
 

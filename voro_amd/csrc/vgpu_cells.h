@@ -1,6 +1,7 @@
 // The persistent cell kernel and the per-tier launchers.
 #pragma once
 #include "compute_dev.h"
+#include "geom_dev.h"
 
 namespace vgpu {
 
@@ -23,7 +24,45 @@ struct CellOut {
 	unsigned char *touch; // periodic containers: 1 if the cell read an image block
 	int tier;
 	int want_nbrs;
+	GeoSink geo;          // cell tables for the polyhedron statistics (geo.top null: not kept)
 };
+
+// Vertex order of a per-lane cell: 3 in the order-3 cell, stored in the
+// general one.
+template<class CELL> VG_INL int cell_order(CELL &, int) { return 3; }
+template<class T_> VG_INL int cell_order(VCell<T_, 64> &c, int v) { return (int)c.NU(v); }
+
+// Copies a finished per-lane cell's tables (coordinates, orders, edges and
+// back pointers) into the geometry arena; a cell that does not fit is
+// counted in top[2] (the caller grows the arena and computes again).
+template<class CELL>
+__device__ void geo_write_lane(CELL &c, const GeoSink &g, int cell, bool ok) {
+	if (!ok) { g.vstart[cell] = 0; g.np[cell] = 0; g.esz[cell] = 0; return; }
+	const int p = c.p;
+	int esz = 0;
+	for (int v = 0; v < p; v++) esz += cell_order(c, v);
+	const int vs = atomicAdd(&g.top[0], p), es = atomicAdd(&g.top[1], esz);
+	if ((long long)vs + p > g.vcap || (long long)es + esz > g.ecap) {
+		g.vstart[cell] = -1; g.np[cell] = 0; g.esz[cell] = 0;
+		atomicAdd(&g.top[2], 1);
+		return;
+	}
+	g.vstart[cell] = vs; g.np[cell] = p; g.estart[cell] = es; g.esz[cell] = esz;
+	int e = es;
+	for (int v = 0; v < p; v++) {
+		const int n = cell_order(c, v);
+		g.xyz[3 * (size_t)(vs + v)] = c.X(v);
+		g.xyz[3 * (size_t)(vs + v) + 1] = c.Y(v);
+		g.xyz[3 * (size_t)(vs + v) + 2] = c.Z(v);
+		g.nu[vs + v] = (unsigned char)n;
+		g.eoff[vs + v] = e;
+		for (int i = 0; i < n; i++) {
+			g.ed[e + i] = (short)c.E(v, i);
+			g.rel[e + i] = (unsigned char)c.R(v, i);
+		}
+		e += n;
+	}
+}
 
 // Neighbour ids of one cell go to a scratch row of the lane's slab first,
 // then to the arena once the count is known.
@@ -114,6 +153,7 @@ k_cells(RectView v, int ncells, const int *__restrict__ list, const int *__restr
 						out.st_tier[cell] = (signed char)out.tier;
 					}
 					if (PRD && out.touch) out.touch[cell] = (unsigned char)stt.img;
+					if (out.geo.top) geo_write_lane(c, out.geo, cell, ok);
 				}
 			}
 		}
@@ -155,6 +195,12 @@ int launch_ghost_cells(bool poly, int waves, hipStream_t s, const RectView &v, c
 int launch_cells_wave(int k, bool poly, bool prd, int occ, int waves, hipStream_t s, const RectView &v, int ncells,
 		      const int *list, const int *list_count, const int *cellblk, int *counters, char *ws,
 		      unsigned stamp_base, int chunk, const CellOut &out);
+
+// Polyhedron statistics (geometry.hip): per-cell output sizes (nc + 1
+// entries, the last 0) and the statistics pass.
+int launch_geo_sizes(hipStream_t s, int nc, const int *status, const int *nfaces, const GeoSink &g, int *nv, int *ne,
+		     int *nfv);
+int launch_geo_stats(hipStream_t s, int nc, const GeoSink &g, const GeoStats &o);
 
 // Periodic containers (cells_prd.hip): tier 0 order-3, 1 wide, 2 huge.
 int launch_cells_prd(int tier, bool poly, int waves, hipStream_t s, const RectView &v, int ncells, const int *list,
