@@ -144,6 +144,57 @@ def test_gpu_ghost_cells_match_reference(oracle_mod, case):
         assert np.array_equal(got["nbr"][i, :nf], want["nbr"][i, :nf]), i
 
 
+@pytest.mark.gpu
+def test_gpu_ghost_cells_through_the_huge_tier(oracle_mod):
+    """Ghost points with radii far above the particles': cells of over 1,024
+    vertices, which the wide cell hands on to the huge one, equal the
+    reference's; points outside the box come back with zeroed outputs."""
+    O = oracle_mod
+    _ref_or_skip(O)
+    n = 40000
+    xyz = I.uniform_box(131, n, (0, 0, 0), (34, 34, 34))
+    r = np.full(n, 0.3)
+    ids = np.arange(n, dtype=np.int32)
+    q = np.array([[17.0, 17.0, 17.0], [10.2, 20.5, 15.1], [-3.0, 5.0, 5.0], [12.0, 12.0, 23.0]])
+    qr = np.array([8.5, 7.0, 1.0, 0.3])
+    spec = O.make_spec(1, (0, 34, 0, 34, 0, 34), (20, 20, 20))
+    want = O.ghost_reference(spec, ids, xyz, r, q, qr, max_nbr=4096)
+    assert want["nfaces"].max() > 520          # over 1,024 vertices
+    con = voro_amd.container_poly(0, 34, 0, 34, 0, 34, 20, 20, 20, False, False, False, 8)
+    con.put_array(ids, xyz, r)
+    got = con.compute_ghost_cells(q, qr, max_nbr=4096)
+    assert np.array_equal(got["exists"], want["exists"])
+    assert got["exists"][2] == 0 and got["vol"][2] == 0 and np.all(got["cent"][2] == 0)
+    e = want["exists"] == 1
+    for k in ("vol", "cent", "nfaces"):
+        assert np.array_equal(got[k][e], want[k][e]), k
+    for i in np.nonzero(e)[0]:
+        nf = want["nfaces"][i]
+        assert np.array_equal(got["nbr"][i, :nf], want["nbr"][i, :nf]), i
+
+
+@pytest.mark.gpu
+def test_gpu_find_on_empty_container_and_non_finite_queries(oracle_mod):
+    """No particle: every query is not found (src/container.cc:260-261), at
+    once, on a 100^3-block grid; NaN and infinite query points are not found
+    either."""
+    O = oracle_mod
+    _ref_or_skip(O)
+    con = voro_amd.container(0, 10, 0, 10, 0, 10, 100, 100, 100, False, False, False, 8)
+    f, pid, pos = con.find_voronoi_cells(np.array([[1.0, 2.0, 3.0], [np.nan, 1, 1], [5, 5, 5]]))
+    assert not f.any()
+    xyz = I.uniform_box(132, 5000, (0, 0, 0), (10, 10, 10))
+    ids = np.arange(5000, dtype=np.int32)
+    con2 = voro_amd.container(0, 10, 0, 10, 0, 10, 10, 10, 10, False, True, False, 8)
+    con2.put_array(ids, xyz)
+    q = np.array([[np.nan, 1, 1], [1, np.inf, 1], [2, 2, -np.inf], [4.5, 4.5, 4.5], [1, 1e300, 1]])
+    f2, pid2, pos2 = con2.find_voronoi_cells(q)
+    assert f2.tolist() == [False, False, False, True, False]
+    want = O.find_reference(O.make_spec(0, (0, 10, 0, 10, 0, 10), (10, 10, 10), periodic=(0, 1, 0)), ids, xyz, None,
+                            q[3:4])
+    assert want[0][0] and pid2[3] == want[1][0] and np.array_equal(pos2[3], want[2][0])
+
+
 # ---- print_custom and import ---------------------------------------------
 
 FORMAT = "%i %q %v %s %n %c %C 100%% %z"

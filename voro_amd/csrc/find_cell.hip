@@ -62,7 +62,14 @@ VG_INL bool min_radius_skip(const RectView &v, int di, int dj, int dk, double fx
 template<class CELL, bool POLY>
 VG_FN int find_cell(CELL &c, const RectView &v, double x, double y, double z, unsigned stamp, int &idx, double &rx,
 		    double &ry, double &rz) {
-	// remap, src/container.cc:203-224
+	// remap, src/container.cc:203-224.  A non-finite or huge coordinate
+	// would make the int conversion undefined (on x86-64 the reference's
+	// lands outside the grid): the point is reported as not found.
+	{
+		const double lim = 1073741824.0;
+		const double a = (x - v.ax) * v.xsp, b = (y - v.ay) * v.ysp, c = (z - v.az) * v.zsp;
+		if (!(fabs(a) < lim && fabs(b) < lim && fabs(c) < lim)) return 0;
+	}
 	int ai, aj, ak;
 	int ci = q_step_int((x - v.ax) * v.xsp);
 	if (ci < 0 || ci >= v.nx) {
@@ -246,26 +253,33 @@ k_find_cells(RectView v, int n, const double *__restrict__ xyz, const int *__res
 // 1 / 0 as the reference's return value, -1 if the cell could not be
 // computed here (capacity, or a search that comes back to the point's own
 // block); nbr holds up to max_nbr neighbour ids per point.
-template<bool POLY>
+template<class T_, bool POLY>
 __global__ void __launch_bounds__(64, 2)
 k_ghost_cells(RectView v, RectGeom geo, int n, const double *__restrict__ xyz, const double *__restrict__ rad,
-	      int *__restrict__ exists, double *__restrict__ vol, double *__restrict__ cent, int *__restrict__ nfaces,
-	      int *__restrict__ nbr, int max_nbr, int *__restrict__ counter, int *__restrict__ failed,
+	      const int *__restrict__ list, const int *__restrict__ list_count, int *__restrict__ exists,
+	      double *__restrict__ vol, double *__restrict__ cent, int *__restrict__ nfaces, int *__restrict__ nbr,
+	      int max_nbr, int *__restrict__ counter, int *__restrict__ retry, int *__restrict__ failed,
 	      char *__restrict__ ws, size_t slab, unsigned stamp_base) {
-	typedef typename TierCell<TierWide>::type CELL;
+	typedef typename TierCell<T_>::type CELL;
 	const int lane = threadIdx.x;
 	CELL c;
 	c.setup(ws + (size_t)blockIdx.x * slab, lane, v.max_len_sq);
+	const int total = list_count ? *list_count : n;
 	for (;;) {
 		int base = 0;
 		if (lane == 0) base = atomicAdd(counter, 64);
 		base = __builtin_amdgcn_readfirstlane(base);
-		if (base >= n) break;
-		const int t = base + lane;
-		if (t >= n) continue;
+		if (base >= total) break;
+		const int tt = base + lane;
+		if (tt >= total) continue;
+		const int t = list ? list[tt] : tt;
 		double x = xyz[3 * t], y = xyz[3 * t + 1], z = xyz[3 * t + 2];
 		const int ijk = locate_rect(geo, x, y, z);
-		if (ijk < 0) { exists[t] = 0; nfaces[t] = 0; continue; }
+		if (ijk < 0) {
+			exists[t] = 0; nfaces[t] = 0; vol[t] = 0;
+			cent[3 * t] = cent[3 * t + 1] = cent[3 * t + 2] = 0;
+			continue;
+		}
 		const double r = POLY ? rad[t] : 0.5;
 		const dbl4 gp{x, y, z, r};
 		const double maxr = POLY && r > v.max_radius ? r : v.max_radius;
@@ -282,7 +296,14 @@ k_ghost_cells(RectView v, RectGeom geo, int n, const double *__restrict__ xyz, c
 			c.outputs(cv, cx, cy, cz, nf, sk);
 			if (nf > (int)CELL::L::NBCAP || nf > max_nbr) c.ovf = true;
 		}
-		if (c.ovf) { exists[t] = -1; nfaces[t] = 0; atomicAdd(failed, 1); continue; }
+		if (c.ovf) {
+			exists[t] = -1; nfaces[t] = 0; vol[t] = 0;
+			cent[3 * t] = cent[3 * t + 1] = cent[3 * t + 2] = 0;
+			// the next tier's list, or (last tier) a failure
+			if (retry) retry[atomicAdd(failed, 1)] = t;
+			else atomicAdd(failed, 1);
+			continue;
+		}
 		exists[t] = ok ? 1 : 0;
 		vol[t] = cv;
 		cent[3 * t] = cx; cent[3 * t + 1] = cy; cent[3 * t + 2] = cz;
@@ -313,15 +334,21 @@ int launch_find_cells(int tier, bool poly, int waves, hipStream_t s, const RectV
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 
-int launch_ghost_cells(bool poly, int waves, hipStream_t s, const RectView &v, const RectGeom &geo, int n,
-		       const double *xyz, const double *rad, int *exists, double *vol, double *cent, int *nfaces, int *nbr,
-		       int max_nbr, int *counter, int *failed, char *ws, size_t slab, unsigned stamp_base) {
-	if (poly)
-		hipLaunchKernelGGL((k_ghost_cells<true>), dim3(waves), dim3(64), 0, s, v, geo, n, xyz, rad, exists, vol, cent,
-				   nfaces, nbr, max_nbr, counter, failed, ws, slab, stamp_base);
-	else
-		hipLaunchKernelGGL((k_ghost_cells<false>), dim3(waves), dim3(64), 0, s, v, geo, n, xyz, rad, exists, vol, cent,
-				   nfaces, nbr, max_nbr, counter, failed, ws, slab, stamp_base);
+int launch_ghost_cells(int tier, bool poly, int waves, hipStream_t s, const RectView &v, const RectGeom &geo, int n,
+		       const double *xyz, const double *rad, const int *list, const int *list_count, int *exists,
+		       double *vol, double *cent, int *nfaces, int *nbr, int max_nbr, int *counter, int *retry, int *failed,
+		       char *ws, size_t slab, unsigned stamp_base) {
+#define VGPU_GHOST(T, P)                                                                                          \
+	hipLaunchKernelGGL((k_ghost_cells<T, P>), dim3(waves), dim3(64), 0, s, v, geo, n, xyz, rad, list, list_count,  \
+			   exists, vol, cent, nfaces, nbr, max_nbr, counter, retry, failed, ws, slab, stamp_base)
+	if (tier == 0) {
+		if (poly) VGPU_GHOST(TierWide, true);
+		else VGPU_GHOST(TierWide, false);
+	} else {
+		if (poly) VGPU_GHOST(TierHuge, true);
+		else VGPU_GHOST(TierHuge, false);
+	}
+#undef VGPU_GHOST
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 

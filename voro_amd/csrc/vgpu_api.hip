@@ -1272,6 +1272,12 @@ int vgpu_find_voronoi_cells(vgpu_container *h, int n, const double *xyz, int *fo
 	int st;
 	if (!h->binned && (st = run_pipeline(h, 0, s, true))) return st;
 	const size_t N = (size_t)n;
+	if (h->ncells == 0) {
+		// no particle: no point has a cell (src/container.cc:260-261)
+		if (on_device) HIPCHK(hipMemset(found, 0, N * 4));
+		else std::memset(found, 0, N * 4);
+		return VGPU_OK;
+	}
 	const double *dq = xyz;
 	int *dfound = found, *dpid = pid;
 	double *dpos = pos;
@@ -1365,22 +1371,44 @@ int vgpu_compute_ghost_cells(vgpu_container *h, int n, const double *xyz, const 
 		HIPCHK(hipMemset(h->ws_wide.buf.p, 0, h->ws_wide.buf.n));
 		h->ws_wide.stamp_base = 0;
 	}
+	if ((st = h->fq_retry.ensure(N * 4))) return st;
+	// [0] / [2] point counters of the wide and huge tiers, [1] the wide
+	// tier's retries, [3] points no tier finished
 	int *ctr = h->d_counters.as<int>();
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 8, 0);
 	const RectView v = make_view(h);
-	if (launch_ghost_cells(h->poly, waves, s, v, h->g, n, h->fq_xyz.as<double>(), h->fq_r.as<double>(),
-			       h->fq_found.as<int>(), h->fq_vol.as<double>(), h->fq_pos.as<double>(), h->fq_pid.as<int>(),
-			       h->fq_nbr.as<int>(), max_nbr, ctr, ctr + 1, h->ws_wide.base(), h->ws_wide.slab,
-			       h->ws_wide.stamp_base))
+	if (launch_ghost_cells(0, h->poly, waves, s, v, h->g, n, h->fq_xyz.as<double>(), h->fq_r.as<double>(), nullptr,
+			       nullptr, h->fq_found.as<int>(), h->fq_vol.as<double>(), h->fq_pos.as<double>(),
+			       h->fq_pid.as<int>(), h->fq_nbr.as<int>(), max_nbr, ctr, h->fq_retry.as<int>(), ctr + 1,
+			       h->ws_wide.base(), h->ws_wide.slab, h->ws_wide.stamp_base))
 		return VGPU_HIP_ERROR;
 	h->ws_wide.stamp_base += (unsigned)n + 1u;
+	int retried = 0;
+	HIPCHK(hipMemcpyAsync(&retried, ctr + 1, 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
+	if (retried > 0) {
+		// the points whose cell outgrew the wide cell, with the huge one
+		typedef Layout<TierHuge, 64> LH;
+		const int hw = std::max(1, std::min(4, nblk((long long)retried, 64)));
+		if ((st = h->ws_huge.ensure(LH::bytes, std::max(hw, h->ws_huge.waves)))) return st;
+		if ((unsigned long long)h->ws_huge.stamp_base + N + 2 >= 0xffffffffull) {
+			HIPCHK(hipMemset(h->ws_huge.buf.p, 0, h->ws_huge.buf.n));
+			h->ws_huge.stamp_base = 0;
+		}
+		if (launch_ghost_cells(1, h->poly, hw, s, v, h->g, n, h->fq_xyz.as<double>(), h->fq_r.as<double>(),
+				       h->fq_retry.as<int>(), ctr + 1, h->fq_found.as<int>(), h->fq_vol.as<double>(),
+				       h->fq_pos.as<double>(), h->fq_pid.as<int>(), h->fq_nbr.as<int>(), max_nbr, ctr + 2,
+				       nullptr, ctr + 3, h->ws_huge.base(), h->ws_huge.slab, h->ws_huge.stamp_base))
+			return VGPU_HIP_ERROR;
+		h->ws_huge.stamp_base += (unsigned)n + 1u;
+	}
 	int failed = 0;
 	HIPCHK(hipMemcpyAsync(exists, h->fq_found.p, N * 4, hipMemcpyDeviceToHost, s));
 	HIPCHK(hipMemcpyAsync(vol, h->fq_vol.p, N * 8, hipMemcpyDeviceToHost, s));
 	HIPCHK(hipMemcpyAsync(cent, h->fq_pos.p, N * 24, hipMemcpyDeviceToHost, s));
 	HIPCHK(hipMemcpyAsync(nfaces, h->fq_pid.p, N * 4, hipMemcpyDeviceToHost, s));
 	HIPCHK(hipMemcpyAsync(nbr, h->fq_nbr.p, N * (size_t)max_nbr * 4, hipMemcpyDeviceToHost, s));
-	HIPCHK(hipMemcpyAsync(&failed, ctr + 1, 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(&failed, ctr + 3, 4, hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
 	return failed > 0 ? VGPU_MEMORY_ERROR : VGPU_OK;
 }

@@ -188,10 +188,14 @@ int launch_find_cells(int tier, bool poly, int waves, hipStream_t s, const RectV
 		      const int *list, const int *list_count, int *found, int *pid, double *pos, int *retry,
 		      int *retry_count, int *counter, char *ws, size_t slab, unsigned stamp_base);
 struct RectGeom;
-// compute_ghost_cell for n points (find_cell.hip), general per-lane cells.
-int launch_ghost_cells(bool poly, int waves, hipStream_t s, const RectView &v, const RectGeom &geo, int n,
-		       const double *xyz, const double *rad, int *exists, double *vol, double *cent, int *nfaces, int *nbr,
-		       int max_nbr, int *counter, int *failed, char *ws, size_t slab, unsigned stamp_base);
+// compute_ghost_cell for n points (or the list of them counted at
+// *list_count) (find_cell.hip), general per-lane cells: tier 0 the wide
+// cell (points it cannot finish go to retry, counted at *failed), tier 1 the
+// huge one (retry null: *failed counts the points it cannot finish).
+int launch_ghost_cells(int tier, bool poly, int waves, hipStream_t s, const RectView &v, const RectGeom &geo, int n,
+		       const double *xyz, const double *rad, const int *list, const int *list_count, int *exists,
+		       double *vol, double *cent, int *nfaces, int *nbr, int max_nbr, int *counter, int *retry, int *failed,
+		       char *ws, size_t slab, unsigned stamp_base);
 int launch_cells_wave(int k, bool poly, bool prd, int occ, int waves, hipStream_t s, const RectView &v, int ncells,
 		      const int *list, const int *list_count, const int *cellblk, int *counters, char *ws,
 		      unsigned stamp_base, int chunk, const CellOut &out);
