@@ -337,24 +337,60 @@ def find_cells(out, steps, cpu):
 
 
 def periodic_poly(out, flags, torch, steps):
+    """The radical periodic workload, plain and with the radius-input
+    predictor (trained on the plain path's statistics for seeds 1 and 2,
+    evaluated on seed 42; results checked identical to the plain path's)."""
     import voro_amd
     from voro_amd import inputs as I
-    ids, xyz, r = I.make("periodic_poly", 42)
-    con = voro_amd.container_periodic_poly(60, 5, 60, 3, 4, 60, 74, 74, 74)
-    d_ids = torch.from_numpy(ids).to("cuda:0")
-    d_xyz = torch.from_numpy(xyz).to("cuda:0")
-    d_r = torch.from_numpy(r).to("cuda:0")
-    torch.cuda.synchronize()
-    con.put_device(len(ids), d_ids.data_ptr(), d_xyz.data_ptr(), d_r.data_ptr())
-    con.compute_all_cells(flags)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        con.compute_all_cells(flags)
-    dt = (time.perf_counter() - t0) / steps
+
+    def make(seed):
+        ids, xyz, r = I.make("periodic_poly", seed)
+        c = voro_amd.container_periodic_poly(60, 5, 60, 3, 4, 60, 74, 74, 74)
+        d = [torch.from_numpy(a).to("cuda:0") for a in (ids, xyz, r)]
+        torch.cuda.synchronize()
+        c.put_device(len(ids), d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr())
+        c._keep = d
+        return c
+
+    def timed(c, fl):
+        c.compute_all_cells(fl)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            c.compute_all_cells(fl)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    con = make(42)
+    dt = timed(con, flags)
     res = con.results()
-    out["periodic_poly_2m"] = {"cells_per_s": len(res["id"]) / dt, "ms": dt * 1e3,
+    n = len(res["id"])
+    plain_ms = {k: float(v) for k, v in con.timings().items()}
+    ref = (res["vol"].copy(), res["nbr"].copy())
+    t0 = time.perf_counter()
+    pred = voro_amd.predictor(with_radius=True)
+    for seed in (1, 2):
+        c = make(seed)
+        c.compute_all_cells(flags | voro_amd.F_STATS)
+        pred.add_samples(c)
+        del c
+    loss = pred.train(epochs=300, lr=3e-3, seed=1, fallback=0.02)
+    t_train = time.perf_counter() - t0
+    con.set_predictor(pred)
+    dtl = timed(con, flags | voro_amd.F_LEARNED)
+    r = con.results()
+    ls = con.learned_stats()
+    out["periodic_poly_2m"] = {"cells_per_s": n / dt, "ms": dt * 1e3,
                                "volume_sum_rel_err": abs(float(res["vol"].sum()) - 216000.0) / 216000.0,
-                               "timings_ms": {k: float(v) for k, v in con.timings().items()}}
+                               "timings_ms": plain_ms,
+                               "learned": {
+                                   "cells_per_s": n / dtl, "ms": dtl * 1e3, "speedup_vs_plain": dt / dtl,
+                                   "fallbacks": ls["fallbacks"], "fallback_rate": ls["fallbacks"] / n,
+                                   "given_no_list": ls["routed"], "list_entries": ls["list_entries"],
+                                   "train_seeds": [1, 2], "eval_seed": 42, "train_seconds": t_train,
+                                   "train_loss": loss, "timings_ms": {k: float(v) for k, v in con.timings().items()},
+                                   "identical_to_plain": bool(np.array_equal(r["vol"], ref[0])
+                                                              and np.array_equal(r["nbr"], ref[1]))}}
 
 
 def batch(out, flags, steps):

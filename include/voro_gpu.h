@@ -270,13 +270,22 @@ int vgpu_predictor_train(vgpu_predictor *p, int epochs, float lr, unsigned seed,
 /* Reads (get) and/or replaces (set) the n = vgpu_mlp_num_params() weights. */
 int vgpu_predictor_params(vgpu_predictor *p, float *get, const float *set, int n);
 int vgpu_predictor_margin(vgpu_predictor *p, float *get, const float *set);
+/* Reads and/or replaces the depth from which a cell is given no list (its
+ * predicted depth, margin included, says it reaches the block search, which
+ * no list covers; it takes the plain path at once).  Default 64; 65 or more
+ * gives every cell a list of depth at most 63. */
+int vgpu_predictor_route(vgpu_predictor *p, int *get, const int *set);
 /* Uses p (NULL: none) for VGPU_F_LEARNED computations of h. */
 int vgpu_set_predictor(vgpu_container *h, vgpu_predictor *p);
-/* Cells of the last learned computation that fell back to the plain path,
- * the list entries gathered (including batch padding), and the time from
- * the end of binning to the end of list gathering (features, prediction,
- * gathering; included in vgpu_last_timings' first-tier figure). */
-int vgpu_learned_stats(vgpu_container *h, long long *fallbacks, long long *list_entries, double *gather_ms);
+/* Cells of the last learned computation whose list ended before the cell
+ * was finished (they fell back to the plain path), the list entries
+ * gathered (including batch padding), the time from the end of binning to
+ * the end of list gathering (features, prediction, gathering; included in
+ * vgpu_last_timings' first-tier figure), and the cells given no list because
+ * the block search was predicted (they took the plain path at once).  Any
+ * pointer may be NULL. */
+int vgpu_learned_stats(vgpu_container *h, long long *fallbacks, long long *list_entries, double *gather_ms,
+		       long long *routed);
 /* The network's kernels on host data, for checking them: loss
  * (1/n) sum (y - t)^2 and its gradient, and forward outputs (n x 2). */
 int vgpu_mlp_num_params(void);

@@ -55,7 +55,9 @@ struct Runner {
 
 // The list-driven tier: gather the cell's candidate list at depth G, run
 // compute_cell_list; -1 sends the cell on to the plain tiers.
-template<bool POLY, class CELL = VCell3, size_t BYTES = Layout3<>::lane_bytes>
+// PRD: a periodic container; the gather reads a copy of the view without
+// the first-visit record, the cell records its visits with key0.
+template<bool POLY, class CELL = VCell3, size_t BYTES = Layout3<>::lane_bytes, bool PRD = false>
 struct ListRunner {
 	std::vector<char> slab;
 	CELL c;
@@ -63,11 +65,13 @@ struct ListRunner {
 	std::vector<ListTag> lb;
 	explicit ListRunner(double mls) : slab(BYTES + 2 * 65536, 0) { c.setup(slab.data() + 65536, 0, mls); }
 	int run(const RectView &v, const int *cellblk, int cell, int G, int ijk, int s, double *vol, double *cent, int *nf,
-		std::vector<int> &nb, CellStats &st, long long *fallbacks) {
-		const int n = gather_cell_list<false>(v, cellblk, cell, G, nullptr, nullptr, 1);
+		std::vector<int> &nb, CellStats &st, long long *fallbacks, unsigned long long key0 = 0) {
+		RectView vg = v;
+		vg.fv = nullptr;
+		const int n = gather_cell_list<false, PRD>(vg, cellblk[cell], cell, G, nullptr, nullptr, 1);
 		la.resize(n); lb.resize(n);
-		gather_cell_list<true>(v, cellblk, cell, G, la.data(), lb.data(), 1);
-		int r = compute_cell_list<CELL, POLY>(c, v, ijk, s, 0, 0, 0, la.data(), lb.data(), 1, n, st);
+		gather_cell_list<true, PRD>(vg, cellblk[cell], cell, G, la.data(), lb.data(), 1);
+		int r = compute_cell_list<CELL, POLY, PRD>(c, v, ijk, s, 0, 0, 0, la.data(), lb.data(), 1, n, st, key0);
 		if (r == -1) { (*fallbacks)++; return -1; }
 		if (r < 0 || c.ovf) return -1;
 		if (r == 1) {
@@ -324,6 +328,10 @@ static long long compute_prd(const hx_spec *s, int n, const int *ids, const doub
 	std::vector<unsigned long long> fv(nb, ~0ull), prev;
 	v.fv = fv.data();
 
+	ListRunner<false, VCell3, Layout3<>::lane_bytes, true> lr0(v.max_len_sq);
+	ListRunner<true, VCell3, Layout3<>::lane_bytes, true> lr1(v.max_len_sq);
+	ListRunner<false, WCell, WLayout::bytes, true> lw0(v.max_len_sq);
+	ListRunner<true, WCell, WLayout::bytes, true> lw1(v.max_len_sq);
 	Runner<WCell, WLayout::bytes> rw(v.max_len_sq);
 	Runner<WCell2, WLayoutT<2>::bytes> rw2(v.max_len_sq);
 	Runner<WCell4, WLayoutT<4>::bytes> rw4(v.max_len_sq);
@@ -334,14 +342,26 @@ static long long compute_prd(const hx_spec *s, int n, const int *ids, const doub
 	std::vector<Res> res(m);
 	std::vector<unsigned char> touched(m, 0);
 	bool bad = false;
-	auto one = [&](int cell) {
+	// list: the first pass tries the candidate list first (as the device's
+	// learned tier does); the replays take the plain tiers
+	auto one = [&](int cell, bool list) {
 		const int ijk = cellblk[cell], q = cell - cnt[ijk];
 		const int ci = ijk % g.nx, cj = (ijk / g.nx) % g.oy, ck = ijk / (g.nx * g.oy);
 		Res &R = res[cell];
 		R.vol = 0; R.cent[0] = R.cent[1] = R.cent[2] = 0; R.nf = 0; R.nb.clear(); R.tier = 0;
 		const unsigned long long key = (unsigned long long)cell << 24;
 		int rr = -1;
-		if (use_wave) {
+		if (list && list_depth > 0) {
+			const int *cb = cellblk.data();
+			if (use_wave)
+				rr = poly ? lw1.run(v, cb, cell, list_depth, ijk, q, &R.vol, R.cent, &R.nf, R.nb, R.st, &list_fallbacks, key)
+					  : lw0.run(v, cb, cell, list_depth, ijk, q, &R.vol, R.cent, &R.nf, R.nb, R.st, &list_fallbacks, key);
+			else
+				rr = poly ? lr1.run(v, cb, cell, list_depth, ijk, q, &R.vol, R.cent, &R.nf, R.nb, R.st, &list_fallbacks, key)
+					  : lr0.run(v, cb, cell, list_depth, ijk, q, &R.vol, R.cent, &R.nf, R.nb, R.st, &list_fallbacks, key);
+			if (rr < 0) R.nb.clear();
+		}
+		if (rr < 0 && use_wave) {
 			rr = poly ? rw.run<true, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key)
 				  : rw.run<false, true>(v, ijk, q, ci, cj, ck, &R.vol, R.cent, &R.nf, R.nb, R.st, key);
 			if (rr >= 0 && R.nf > (int)WLayout::NBCAP) rr = -1;
@@ -385,7 +405,7 @@ static long long compute_prd(const hx_spec *s, int n, const int *ids, const doub
 		im.assemble(Pa, ia, ba);
 		v.P = Pa.data(); v.pid = ia.data(); v.bstart = ba.data();
 	}
-	for (int c = 0; c < m; c++) one(c);
+	for (int c = 0; c < m; c++) one(c, true);
 	prd_passes = 1;
 	bool settled = false;
 	if (!prd_replay) settled = true;
@@ -397,7 +417,7 @@ static long long compute_prd(const hx_spec *s, int n, const int *ids, const doub
 		im.assemble(Pa, ia, ba);
 		v.P = Pa.data(); v.pid = ia.data(); v.bstart = ba.data();
 		std::fill(fv.begin(), fv.end(), ~0ull);
-		for (int c = 0; c < m; c++) if (touched[c]) one(c);
+		for (int c = 0; c < m; c++) if (touched[c]) one(c, false);
 		prd_passes = pass + 2;
 		settled = fv == prev;
 	}

@@ -90,6 +90,7 @@ def test_gpu_learned_uniform_is_exact_and_saves_work(oracle_mod):
     assert st["fallbacks"] < 0.1 * n
     # without the network: every list gathered to the full worklist
     pred.margin = 1000.0
+    pred.route = 65
     con.set_predictor(pred)
     con.compute_all_cells(F | voro_amd.F_LEARNED)
     con._res = None
@@ -202,3 +203,94 @@ def test_gpu_learned_radical_per_particle(oracle_mod):
     got = con.results()
     _vs_oracle(oracle_mod, got, con)
     _same(plain, got)
+
+
+# ---- periodic containers: the list kernel records the image blocks' first
+# visits with the keys the plain path gives them ----
+
+PRD_BOX = (30.0, 2.5, 30.0, 1.5, 2.0, 30.0)
+
+
+def _prd_poly(sd, g=37):
+    """The radical triclinic workload at 1/8 the volume and particle count
+    (same density): container_periodic_poly(30,2.5,30,1.5,2,30), 37^3 blocks,
+    250,000 particles."""
+    ids, xyz, r = I.make("periodic_poly", sd, 250000)
+    xyz = xyz * 0.5
+    con = voro_amd.container_periodic_poly(*PRD_BOX, g, g, g)
+    con.put_array(ids, xyz, r)
+    con.oracle_inputs = (3, PRD_BOX, (g, g, g), xyz, r)
+    return con
+
+
+def test_gpu_learned_periodic_poly_vs_reference(oracle_mod):
+    """The radical periodic workload with the radius-input predictor: a
+    250,000-particle sample checked against the reference library cell for
+    cell, and against the plain path bit for bit."""
+    if oracle_mod.reference() is None:
+        pytest.skip("reference library not built")
+    pred = voro_amd.predictor(with_radius=True)
+    for sd in (51, 52):
+        _flush("periodic train seed %d" % sd)
+        c = _prd_poly(sd)
+        c.compute_all_cells(F | voro_amd.F_STATS)
+        pred.add_samples(c)
+    pred.train(epochs=200)
+    con = _prd_poly(53)
+    con.compute_all_cells(F)
+    plain = dict(con.results())
+    con.set_predictor(pred)
+    _flush("periodic learned compute")
+    con.compute_all_cells(F | voro_amd.F_LEARNED)
+    con._res = None
+    got = con.results()
+    _same(plain, got)
+    st = con.learned_stats()
+    n = len(got["id"])
+    _flush("periodic fallbacks %d, given no list %d, of %d; entries %d"
+           % (st["fallbacks"], st["routed"], n, st["list_entries"]))
+    # about 60% of these cells reach the block search, which no list covers:
+    # the predictor sends many of them to the plain path at once
+    assert st["routed"] > 0.25 * n and st["fallbacks"] < 0.4 * n
+    kind, geom, grid, xyz, r = con.oracle_inputs
+    want = oracle_mod.run("reference", oracle_mod.make_spec(kind, geom, grid), np.arange(len(xyz), dtype=np.int32),
+                          xyz, r)
+    assert oracle_mod.compare(got, want, vol_rtol=1e-12) == []
+    assert np.array_equal(got["vol"], want["vol"])
+
+
+@pytest.mark.parametrize("margin", [-1000.0, None, 1000.0], ids=["depth1", "trained", "full"])
+def test_gpu_learned_periodic_depths_match_reference(oracle_mod, margin):
+    """A small sheared periodic container at list depth 1 (every cell falls
+    back), as trained, and at full depth: the same cells and neighbour orders
+    as the reference."""
+    if oracle_mod.reference() is None:
+        pytest.skip("reference library not built")
+    box = (8.0, 1.3, 8.0, -0.9, 2.1, 8.0)
+    n, g = 3000, 7
+
+    def make(sd):
+        u = np.stack([I.uniform(sd, 90 + a, n) for a in range(3)], 1) * 1.3 - 0.15
+        xyz = np.stack([u[:, 0] * box[0] + u[:, 1] * box[1] + u[:, 2] * box[3], u[:, 1] * box[2] + u[:, 2] * box[4],
+                        u[:, 2] * box[5]], 1)
+        con = voro_amd.container_periodic(*box, g, g, g)
+        con.put_array(np.arange(n, dtype=np.int32), xyz)
+        con.oracle_inputs = (2, box, (g, g, g), xyz, None)
+        return con
+    pred, _, _ = _train(make, [61], epochs=100)
+    if margin is not None:
+        pred.margin = margin
+        pred.route = 65
+    con = make(62)
+    con.compute_all_cells(F)
+    plain = dict(con.results())
+    con.set_predictor(pred)
+    con.compute_all_cells(F | voro_amd.F_LEARNED)
+    con._res = None
+    got = con.results()
+    _same(plain, got)
+    kind, geom, grid, xyz, r = con.oracle_inputs
+    want = oracle_mod.run("reference", oracle_mod.make_spec(kind, geom, grid), np.arange(n, dtype=np.int32), xyz, r)
+    assert oracle_mod.compare(got, want, vol_rtol=1e-12) == []
+    if margin is not None and margin < 0:
+        assert con.learned_stats()["fallbacks"] > 0.9 * n

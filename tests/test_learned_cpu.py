@@ -85,3 +85,55 @@ def test_list_cutting_on_cpu_matches_oracle(oracle_mod, harness, case, wave):
     # depth 63 falls back only for cells that reach the block search
     st = harness.run(O, sp, ids, xyz, r)["stats"]
     assert last_fb == int(np.sum(st[:, 1] >= 64))
+
+
+def _para(geom, n, seed, lo=-0.15, hi=1.15):
+    bx, bxy, by, bxz, byz, bz = geom
+    u = np.stack([I.uniform(seed, 80 + a, n) for a in range(3)], 1) * (hi - lo) + lo
+    return np.stack([u[:, 0] * bx + u[:, 1] * bxy + u[:, 2] * bxz, u[:, 1] * by + u[:, 2] * byz, u[:, 2] * bz], 1)
+
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.parametrize("wave", [False, True], ids=["lane", "wave"])
+def test_list_cutting_periodic_matches_golden(oracle_mod, harness, wave):
+    """Periodic containers through the lists: the list cell records each
+    image block's first visit with the key the plain path would give it, so
+    the replayed creation order, and with it every neighbour order, is the
+    reference's (golden vectors written by the reference library)."""
+    O = oracle_mod
+    d = dict(np.load(os.path.join(GOLD, "prd_poly_small.npz"), allow_pickle=False))
+    sp = O.make_spec(int(d["spec_kind"]), d["spec_geom"], d["spec_grid"])
+    for G in (2, 12, 40, 63):
+        b = harness.run(O, sp, d["ids"], d["xyz"], d.get("r"), list_depth=G, wave=wave)
+        assert O.compare(b, d, vol_rtol=0.0) == [], G
+        assert np.array_equal(b["vol"], d["vol"]) and np.array_equal(b["cent"], d["cent"])
+
+
+PRD_CASES = {
+    "triclinic_poly": (3, (10, 1.5, 10, 0.7, -1.2, 10), (5, 5, 5), 800, True),
+    "skew": (3, (5, 7.5, 5, -6.2, 8.1, 5), (4, 4, 4), 600, True),
+    "cubic": (2, (10, 0, 10, 0, 0, 10), (4, 4, 4), 500, False),
+}
+
+
+@pytest.mark.parametrize("wave", [False, True], ids=["lane", "wave"])
+@pytest.mark.parametrize("case", sorted(PRD_CASES))
+def test_list_cutting_periodic_matches_reference(oracle_mod, harness, case, wave):
+    O = oracle_mod
+    if O.reference() is None:
+        pytest.skip("reference library not built")
+    kind, geom, grid, n, poly = PRD_CASES[case]
+    xyz = _para(geom, n, 300 + len(case))
+    r = I.radii_lognormal(9, n) * 0.6 if poly else None
+    ids = np.arange(n, dtype=np.int32)
+    sp = O.make_spec(kind, geom, grid)
+    a = O.run("reference", sp, ids, xyz, r)
+    fb = []
+    for G in (3, 20, 63):
+        b = harness.run(O, sp, ids, xyz, r, list_depth=G, wave=wave)
+        assert O.compare(b, a, vol_rtol=0.0) == [], G
+        assert np.array_equal(a["pos"], b["pos"])
+        fb.append(b["list_fallbacks"])
+    assert fb[0] > 0 and fb == sorted(fb, reverse=True)

@@ -102,8 +102,9 @@ def lib():
         "vgpu_predictor_train": (C.c_int, [p, C.c_int, C.c_float, C.c_uint, C.c_float, C.POINTER(C.c_float)]),
         "vgpu_predictor_params": (C.c_int, [p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int]),
         "vgpu_predictor_margin": (C.c_int, [p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "vgpu_predictor_route": (C.c_int, [p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "vgpu_set_predictor": (C.c_int, [p, p]),
-        "vgpu_learned_stats": (C.c_int, [p, i64p, i64p, f64p]),
+        "vgpu_learned_stats": (C.c_int, [p, i64p, i64p, f64p, i64p]),
         "vgpu_batch_create": (C.c_int, [C.POINTER(p), C.c_int, C.POINTER(p)]),
         "vgpu_batch_destroy": (C.c_int, [p]),
         "vgpu_batch_compute": (C.c_int, [p, C.c_uint, p]),
@@ -245,9 +246,12 @@ class _Base:
         self._pred = pred
 
     def learned_stats(self):
-        fb, ent, gm = C.c_longlong(), C.c_longlong(), C.c_double()
-        _check(lib().vgpu_learned_stats(self._h, C.byref(fb), C.byref(ent), C.byref(gm)), "vgpu_learned_stats")
-        return {"fallbacks": fb.value, "list_entries": ent.value, "gather_ms": gm.value}
+        """fallbacks: cells whose list ended first; routed: cells given no
+        list (block search predicted); list_entries; gather_ms."""
+        fb, ent, gm, ro = C.c_longlong(), C.c_longlong(), C.c_double(), C.c_longlong()
+        _check(lib().vgpu_learned_stats(self._h, C.byref(fb), C.byref(ent), C.byref(gm), C.byref(ro)),
+               "vgpu_learned_stats")
+        return {"fallbacks": fb.value, "list_entries": ent.value, "gather_ms": gm.value, "routed": ro.value}
 
     def set_launch(self, fast_waves=0, fast_occupancy=0):
         _check(lib().vgpu_set_launch(self._h, int(fast_waves), int(fast_occupancy)), "vgpu_set_launch")
@@ -989,6 +993,20 @@ class predictor:
     def margin(self, v):
         m = C.c_float(v)
         _check(lib().vgpu_predictor_margin(self._h, None, C.byref(m)), "vgpu_predictor_margin")
+
+    @property
+    def route(self):
+        """Predicted depths from this one on give a cell no list (the block
+        search is predicted; the cell takes the plain path at once).  64 by
+        default; 65 or more gives every cell a list."""
+        m = C.c_int()
+        _check(lib().vgpu_predictor_route(self._h, C.byref(m), None), "vgpu_predictor_route")
+        return m.value
+
+    @route.setter
+    def route(self, v):
+        m = C.c_int(int(v))
+        _check(lib().vgpu_predictor_route(self._h, None, C.byref(m)), "vgpu_predictor_route")
 
 
 def mlp_num_params():

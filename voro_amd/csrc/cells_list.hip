@@ -59,13 +59,14 @@ __global__ void k_order_scatter(int nc, const int *__restrict__ cellblk, const u
 	if (c < nc) order[base[k] + r] = c;
 }
 
+template<bool PRD>
 __global__ void k_list_count(RectView v, int nc, const int *__restrict__ cellblk, const int *__restrict__ order,
 			     const unsigned char *__restrict__ depth, int per_cell, int *__restrict__ cnt) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= nc) return;
 	const int c = order[i];
 	const int G = depth[per_cell ? c : cellblk[c]];
-	cnt[i] = gather_cell_list<false>(v, cellblk, c, G, nullptr, nullptr, 64);
+	cnt[i] = gather_cell_list<false, PRD>(v, cellblk[c], c, G, nullptr, nullptr, 64);
 }
 
 // Capacity of each batch of 64 cells (the longest list in it); entry nb is
@@ -82,6 +83,7 @@ __global__ void k_batch_max(int nc, const int *__restrict__ cnt, int *__restrict
 
 // csr: position i's entries contiguous from boff[i] (wavefront cells);
 // otherwise interleaved per batch of 64 positions from row boff[i / 64].
+template<bool PRD>
 __global__ void k_list_fill(RectView v, int nc, const int *__restrict__ cellblk, const int *__restrict__ order,
 			    const unsigned char *__restrict__ depth, int per_cell, const int *__restrict__ boff,
 			    int csr, ListEntry *__restrict__ LA, ListTag *__restrict__ LB) {
@@ -91,22 +93,26 @@ __global__ void k_list_fill(RectView v, int nc, const int *__restrict__ cellblk,
 	const int G = depth[per_cell ? c : cellblk[c]];
 	if (csr) {
 		const size_t base = (size_t)boff[i];
-		gather_cell_list<true>(v, cellblk, c, G, LA + base, LB + base, 1);
+		gather_cell_list<true, PRD>(v, cellblk[c], c, G, LA + base, LB + base, 1);
 	} else {
 		const size_t base = (size_t)boff[i >> 6] * 64 + (i & 63);
-		gather_cell_list<true>(v, cellblk, c, G, LA + base, LB + base, 64);
+		gather_cell_list<true, PRD>(v, cellblk[c], c, G, LA + base, LB + base, 64);
 	}
 }
 
 // One wavefront per workgroup, a batch of 64 cells at a time (batches
 // taken in order through an atomic counter).  Cells whose list ends before
 // the radius test stops them, and cells that overflow the order-3 tier, go to
-// the plain path's retry list; *fallbacks counts the former.
-template<bool POLY, int OCC>
+// the plain path's retry list; *fallbacks counts the former, *routed the
+// cells given no list (predicted to reach the block search).  PRD: a
+// periodic container (first visits recorded with the plain path's keys,
+// image reads flagged in out.touch).
+template<bool POLY, bool PRD, int OCC>
 __global__ void __launch_bounds__(64, OCC)
 k_cells_list(RectView v, int nc, const int *__restrict__ cellblk, const int *__restrict__ order, const int *__restrict__ cnt,
 	     const int *__restrict__ boff, const ListEntry *__restrict__ LA, const ListTag *__restrict__ LB,
-	     int *__restrict__ counters, char *__restrict__ ws, int *__restrict__ fallbacks, CellOut out) {
+	     int *__restrict__ counters, char *__restrict__ ws, int *__restrict__ fallbacks, int *__restrict__ routed,
+	     CellOut out) {
 	const int lane = threadIdx.x;
 	VCell3 c;
 	c.setup(ws + (size_t)blockIdx.x * VCell3::L::bytes, lane, v.max_len_sq);
@@ -120,10 +126,11 @@ k_cells_list(RectView v, int nc, const int *__restrict__ cellblk, const int *__r
 		if (pos >= nc) continue;
 		const int cell = order[pos];
 		const int ijk = cellblk[cell];
-		const int s = cell - v.bstart[ijk];
+		const int s = cell - v.cstart[ijk];
 		const size_t base = (size_t)boff[b] * 64 + lane;
 		CellStats stt;
-		const int r = compute_cell_list<VCell3, POLY>(c, v, ijk, s, 0, 0, 0, LA + base, LB + base, 64, cnt[pos], stt);
+		const int r = compute_cell_list<VCell3, POLY, PRD>(c, v, ijk, s, 0, 0, 0, LA + base, LB + base, 64, cnt[pos],
+								   stt, (unsigned long long)cell << 24);
 		double vol = 0, cx = 0, cy = 0, cz = 0;
 		int nf = 0;
 		bool retry = r < 0 || c.ovf;
@@ -134,6 +141,7 @@ k_cells_list(RectView v, int nc, const int *__restrict__ cellblk, const int *__r
 		}
 		if (retry) {
 			if (r == -1) atomicAdd(fallbacks, 1);
+			else if (r == -3) atomicAdd(routed, 1);
 			const int slot_ = atomicAdd(out.retry_count, 1);
 			out.retry[slot_] = cell;
 			out.status[cell] = ST_OVERFLOW;
@@ -156,6 +164,7 @@ k_cells_list(RectView v, int nc, const int *__restrict__ cellblk, const int *__r
 			out.st_exit[cell] = (signed char)stt.wl_exit;
 			out.st_tier[cell] = (signed char)out.tier;
 		}
+		if (PRD && out.touch) out.touch[cell] = (unsigned char)stt.img;
 	}
 }
 
@@ -163,12 +172,12 @@ k_cells_list(RectView v, int nc, const int *__restrict__ cellblk, const int *__r
 // of schedule position pos is contiguous from boff[pos], its entries read
 // with uniform loads.  Positions are taken `chunk` at a time from one
 // counter per XCD share.
-template<bool POLY, int OCC>
+template<bool POLY, bool PRD, int OCC>
 __global__ void __launch_bounds__(64, OCC)
 k_wcells_list(RectView v_, int nc, const int *__restrict__ cellblk, const int *__restrict__ order,
 	      const int *__restrict__ cnt, const int *__restrict__ boff, const ListEntry *__restrict__ LA,
 	      const ListTag *__restrict__ LB, int *__restrict__ counters, char *__restrict__ ws, int chunk,
-	      int *__restrict__ fallbacks, CellOut out) {
+	      int *__restrict__ fallbacks, int *__restrict__ routed, CellOut out) {
 	typedef const __attribute__((address_space(4))) RectView CView;
 	const int lane = threadIdx.x;
 	WCell c;
@@ -192,11 +201,11 @@ k_wcells_list(RectView v_, int nc, const int *__restrict__ cellblk, const int *_
 			for (int pos = lo + base; pos < end; pos++) {
 				const int cell = order[pos];
 				const int ijk = cellblk[cell];
-				const int s = cell - v.bstart[ijk];
+				const int s = cell - v.cstart[ijk];
 				const size_t bs = (size_t)boff[pos];
 				CellStats stt;
-				const int r = compute_cell_list<WCell, POLY>(c, v, ijk, s, 0, 0, 0, LA + bs, LB + bs, 1, cnt[pos],
-									     stt);
+				const int r = compute_cell_list<WCell, POLY, PRD>(c, v, ijk, s, 0, 0, 0, LA + bs, LB + bs, 1,
+										  cnt[pos], stt, (unsigned long long)cell << 24);
 				double vol = 0, cx = 0, cy = 0, cz = 0;
 				int nf = 0;
 				bool retry = r < 0 || c.ovf;
@@ -208,6 +217,7 @@ k_wcells_list(RectView v_, int nc, const int *__restrict__ cellblk, const int *_
 				if (retry) {
 					if (lane == 0) {
 						if (r == -1) atomicAdd(fallbacks, 1);
+						else if (r == -3) atomicAdd(routed, 1);
 						const int slot_ = atomicAdd(out.retry_count, 1);
 						out.retry[slot_] = cell;
 						out.status[cell] = ST_OVERFLOW;
@@ -224,6 +234,7 @@ k_wcells_list(RectView v_, int nc, const int *__restrict__ cellblk, const int *_
 						out.st_exit[cell] = (signed char)stt.wl_exit;
 						out.st_tier[cell] = (signed char)out.tier;
 					}
+					if (PRD && out.touch) out.touch[cell] = (unsigned char)stt.img;
 				}
 				if (out.want_nbrs) {
 					int off = 0;
@@ -241,15 +252,15 @@ k_wcells_list(RectView v_, int nc, const int *__restrict__ cellblk, const int *_
 	}
 }
 
-int launch_cells_wlist(bool poly, int waves, int chunk, hipStream_t s, const RectView &v, int nc, const int *cellblk,
-		       const int *order, const int *cnt, const int *boff, const ListEntry *LA, const ListTag *LB,
-		       int *counters, char *ws, int *fallbacks, const CellOut &out) {
-	if (poly)
-		hipLaunchKernelGGL((k_wcells_list<true, 4>), dim3(waves), dim3(64), 0, s, v, nc, cellblk, order, cnt, boff, LA,
-				   LB, counters, ws, chunk, fallbacks, out);
-	else
-		hipLaunchKernelGGL((k_wcells_list<false, 4>), dim3(waves), dim3(64), 0, s, v, nc, cellblk, order, cnt, boff,
-				   LA, LB, counters, ws, chunk, fallbacks, out);
+int launch_cells_wlist(bool poly, bool prd, int waves, int chunk, hipStream_t s, const RectView &v, int nc,
+		       const int *cellblk, const int *order, const int *cnt, const int *boff, const ListEntry *LA,
+		       const ListTag *LB, int *counters, char *ws, int *fallbacks, int *routed, const CellOut &out) {
+#define VG_WLIST(P, R)                                                                                              \
+	hipLaunchKernelGGL((k_wcells_list<P, R, 4>), dim3(waves), dim3(64), 0, s, v, nc, cellblk, order, cnt, boff, LA, \
+			   LB, counters, ws, chunk, fallbacks, routed, out)
+	if (prd) { if (poly) VG_WLIST(true, true); else VG_WLIST(false, true); }
+	else { if (poly) VG_WLIST(true, false); else VG_WLIST(false, false); }
+#undef VG_WLIST
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 
@@ -262,30 +273,38 @@ int launch_list_order(hipStream_t s, int nc, const int *cellblk, const unsigned 
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 
-int launch_list_gather(hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
+// v.fv must be null: the gather only reads.
+int launch_list_gather(bool prd, hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
 		       const unsigned char *depth, int per_cell, int *cnt, int *bmax) {
 	const int g = (nc + 255) / 256;
-	hipLaunchKernelGGL(k_list_count, dim3(g), dim3(256), 0, s, v, nc, cellblk, order, depth, per_cell, cnt);
+	if (v.fv) return 2;
+	if (prd) hipLaunchKernelGGL(k_list_count<true>, dim3(g), dim3(256), 0, s, v, nc, cellblk, order, depth, per_cell, cnt);
+	else hipLaunchKernelGGL(k_list_count<false>, dim3(g), dim3(256), 0, s, v, nc, cellblk, order, depth, per_cell, cnt);
 	hipLaunchKernelGGL(k_batch_max, dim3(g), dim3(256), 0, s, nc, cnt, bmax);
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 
-int launch_list_fill(hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
+int launch_list_fill(bool prd, hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
 		     const unsigned char *depth, int per_cell, const int *boff, bool csr, ListEntry *LA, ListTag *LB) {
-	hipLaunchKernelGGL(k_list_fill, dim3((nc + 255) / 256), dim3(256), 0, s, v, nc, cellblk, order, depth, per_cell,
-			   boff, csr ? 1 : 0, LA, LB);
+	if (v.fv) return 2;
+	if (prd)
+		hipLaunchKernelGGL(k_list_fill<true>, dim3((nc + 255) / 256), dim3(256), 0, s, v, nc, cellblk, order, depth,
+				   per_cell, boff, csr ? 1 : 0, LA, LB);
+	else
+		hipLaunchKernelGGL(k_list_fill<false>, dim3((nc + 255) / 256), dim3(256), 0, s, v, nc, cellblk, order, depth,
+				   per_cell, boff, csr ? 1 : 0, LA, LB);
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 
-int launch_cells_list(bool poly, int waves, hipStream_t s, const RectView &v, int nc, const int *cellblk,
+int launch_cells_list(bool poly, bool prd, int waves, hipStream_t s, const RectView &v, int nc, const int *cellblk,
 		      const int *order, const int *cnt, const int *boff, const ListEntry *LA, const ListTag *LB,
-		      int *counters, char *ws, int *fallbacks, const CellOut &out) {
-	if (poly)
-		hipLaunchKernelGGL((k_cells_list<true, 2>), dim3(waves), dim3(64), 0, s, v, nc, cellblk, order, cnt, boff, LA,
-				   LB, counters, ws, fallbacks, out);
-	else
-		hipLaunchKernelGGL((k_cells_list<false, 2>), dim3(waves), dim3(64), 0, s, v, nc, cellblk, order, cnt, boff, LA,
-				   LB, counters, ws, fallbacks, out);
+		      int *counters, char *ws, int *fallbacks, int *routed, const CellOut &out) {
+#define VG_LIST(P, R)                                                                                               \
+	hipLaunchKernelGGL((k_cells_list<P, R, 2>), dim3(waves), dim3(64), 0, s, v, nc, cellblk, order, cnt, boff, LA, \
+			   LB, counters, ws, fallbacks, routed, out)
+	if (prd) { if (poly) VG_LIST(true, true); else VG_LIST(false, true); }
+	else { if (poly) VG_LIST(true, false); else VG_LIST(false, false); }
+#undef VG_LIST
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 

@@ -781,22 +781,33 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 // are written by a separate gather pass; the decisions that depend on the
 // shape (count_list refresh, the radius stop, the block skip, the
 // per-particle check) are taken here exactly as compute_cell_rect takes them.
+// A kind-2 tag of a periodic container's list also carries the block
+// (kind = 2 | image << 3 | block << 4): a visit of it is recorded here, in
+// the order and with the key compute_cell_rect records it (the gather must
+// not record visits: the cell may stop before the block).
 struct ListEntry { double x, y, z, w; };
 struct ListTag { int pid, kind; };
 enum { LK_PART = 0, LK_OWN = 1, LK_BLOCK = 2, LK_NOBLOCK = 3, LK_END = 4 };
 
 // compute_cell over a candidate list.  Returns 1 / 0 (the cell exists /
 // was deleted), -1 when the list ends before the radius test stops the
-// cell (the plain path must compute it), -2 on overflow (c.ovf).
-template<class CELL, bool POLY>
+// cell (the plain path must compute it), -2 on overflow (c.ovf), -3 for an
+// empty list (no list was gathered: the plain path computes the cell).
+// key0: the cell's loop position << 24 (periodic first-visit keys).
+template<class CELL, bool POLY, bool PRD = false>
 VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, int ci, int cj, int ck,
 			    const ListEntry *__restrict__ LA, const ListTag *__restrict__ LB, int stride, int nent,
-			    CellStats &stt) {
-	stt.n_planes = 0; stt.wl_exit = 0; stt.phase = 1;
+			    CellStats &stt, unsigned long long key0 = 0) {
+	stt.n_planes = 0; stt.wl_exit = 0; stt.phase = 1; stt.img = 0;
+	if (nent <= 0) return -3;
+	unsigned long long key = key0;
 	const int b0 = v.bstart[ijk];
 	const dbl4 me = v.P[b0 + s];
 	const double x = me.x, y = me.y, z = me.z;
-	{
+	if (PRD) {
+		c.init_unit(v.unit, v.unit_p);
+		if (c.ovf) { stt.n_planes = 0; return -2; }
+	} else {
 		double xa, xb, ya, yb, za, zb;
 		if (v.xper) { xb = 0.5 * (v.bx - v.ax); xa = -xb; } else { xa = v.ax - x; xb = v.bx - x; }
 		if (v.yper) { yb = 0.5 * (v.by - v.ay); ya = -yb; } else { ya = v.ay - y; yb = v.by - y; }
@@ -894,6 +905,15 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, int ci, 
 			if (tg.kind == LK_NOBLOCK) continue;
 			const int cnt = tg.pid;
 			if (rp.ctest(a.y, mrs)) { k += cnt; continue; }   // compute_min_max_radius skip
+			if (PRD) {
+				// region_index of the block (src/container_prd.hh:163-168):
+				// every call takes a key, image blocks record the visit
+				if ((tg.kind >> 3) & 1) {
+					stt.img = 1;
+					if (v.fv) note_first_visit(v.fv + (tg.kind >> 4), key);
+				}
+				key++;
+			}
 			bcheck = rp.ctest(a.z, mrs);
 			left = cnt;
 			if (cnt == 0) bcheck = false;

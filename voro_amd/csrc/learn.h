@@ -42,14 +42,20 @@ int mlp_loss_grad_dev(const float *P, const float *X, const float *T, int n, flo
 		      float *part, int chunks, float *grad, float *loss, hipStream_t s);
 // One Adam step.
 int mlp_adam_dev(float *P, const float *G, float *M, float *V, float lr, int t, hipStream_t s);
-// Depth per sample: min(63, max(1, ceil(64 y0 + margin))); cost (may be
+// Depth per sample: d = max(1, ceil(64 y0 + margin)); 64 (no list: the
+// block search is predicted) where d >= route (route <= 64), else min(d, 63);
+// cost (may be
 // null): the predicted log(1 + nplane calls) in steps of 1/8, 0..63.
-int mlp_depth_dev(const float *P, const float *X, int n, float margin, unsigned char *depth, unsigned char *cost,
-		  hipStream_t s);
+int mlp_depth_dev(const float *P, const float *X, int n, float margin, int route, unsigned char *depth,
+		  unsigned char *cost, hipStream_t s);
 // Raw outputs per sample (n x OUT).
 int mlp_forward_dev(const float *P, const float *X, int n, float *Y, hipStream_t s);
-// Block features (27 neighbour counts / mean; 28th = 0).
-int block_features_dev(int nx, int ny, int nz, const int *bstart, float inv_mean, float *X, hipStream_t s);
+// Block features (27 neighbour counts / mean; 28th = 0) of the blocks of an
+// nx x ny x nz grid.  A periodic container passes its block grid
+// (nx x gy x gz, real rows from j0, k0; one sample per block of it, zero for
+// image blocks) and wrap = true.
+int block_features_dev(int nx, int ny, int nz, const int *bstart, float inv_mean, float *X, hipStream_t s, int gy = 0,
+		       int gz = 0, int j0 = 0, int k0 = 0, bool wrap = false);
 // Per-cell features for a predictor that takes the radius: the cell's block
 // features with input 27 = r / max_r (r read with a stride of 4 doubles).
 int cell_features_dev(int nc, const int *cellblk, const double *r, float inv_maxr, const float *Xb, float *Xc,
@@ -68,6 +74,7 @@ struct vgpu_predictor {
 	std::vector<float> X, T;          // training samples (host)
 	std::vector<float> params;        // NP floats
 	float margin = 0;                 // depth units
+	int route = 64;                   // depths from here on: no list (block search predicted)
 	bool trained = false;
 	long long seen_blocks = 0;
 };

@@ -154,7 +154,7 @@ __global__ void k_adam(float *__restrict__ P, const float *__restrict__ G, float
 	P[p] -= lr * (m / bc1) / (sqrtf(v / bc2) + eps);
 }
 
-__global__ void k_depth(const float *__restrict__ P, const float *__restrict__ X, int n, float margin,
+__global__ void k_depth(const float *__restrict__ P, const float *__restrict__ X, int n, float margin, int route,
 			unsigned char *__restrict__ depth, unsigned char *__restrict__ cost) {
 	__shared__ float W[NP];
 	for (int t = threadIdx.x; t < NP; t += blockDim.x) W[t] = P[t];
@@ -164,9 +164,12 @@ __global__ void k_depth(const float *__restrict__ P, const float *__restrict__ X
 	float x[IN], h1[H], h2[H], y[OUT];
 	for (int k = 0; k < IN; k++) x[k] = X[(size_t)s * IN + k];
 	forward1(W, x, h1, h2, y);
+	// 64: the cell is predicted to go past the worklist into the block
+	// search, which no list covers; it takes the plain path at once
 	float g = ceilf(64.f * y[0] + margin);
 	if (!(g >= 1.f)) g = 1.f;
-	if (g > 63.f) g = 63.f;
+	if (route <= 64 && g >= (float)route) g = 64.f;
+	else if (g > 63.f) g = 63.f;
 	depth[s] = (unsigned char)g;
 	if (cost) {
 		// predicted log(1 + nplane calls) in 1/8 steps, 0..63
@@ -190,20 +193,31 @@ __global__ void k_forward_out(const float *__restrict__ P, const float *__restri
 }
 
 // 27 neighbour-block counts around each block (zero outside the grid).
-__global__ void k_block_features(int nx, int ny, int nz, const int *__restrict__ bstart, float inv_mean,
-				 float *__restrict__ X) {
+// Block features: the particle counts of the 27 blocks around a block,
+// over the mean count.  Blocks are indexed in a grid nx x gy x gz whose
+// real blocks are j0 <= j < j0 + ny, k0 <= k < k0 + nz (a periodic
+// container's image rows around them; plain containers have gy = ny,
+// j0 = 0); wrap: neighbours wrap around the real grid (periodic containers).
+__global__ void k_block_features(int nx, int ny, int nz, int gy, int gz, int j0, int k0, int wrap,
+				 const int *__restrict__ bstart, float inv_mean, float *__restrict__ X) {
 	const int b = blockIdx.x * blockDim.x + threadIdx.x;
-	const int nb = nx * ny * nz;
+	const int nb = nx * gy * gz;
 	if (b >= nb) return;
-	const int i = b % nx, j = (b / nx) % ny, k = b / (nx * ny);
+	const int i = b % nx, j = (b / nx) % gy - j0, k = b / (nx * gy) - k0;
+	const bool real = j >= 0 && j < ny && k >= 0 && k < nz;
 	int t = 0;
 	for (int dk = -1; dk <= 1; dk++)
 		for (int dj = -1; dj <= 1; dj++)
 			for (int di = -1; di <= 1; di++, t++) {
-				const int a = i + di, c = j + dj, e = k + dk;
+				int a = i + di, c = j + dj, e = k + dk;
+				if (wrap) {
+					a = a < 0 ? a + nx : (a >= nx ? a - nx : a);
+					c = c < 0 ? c + ny : (c >= ny ? c - ny : c);
+					e = e < 0 ? e + nz : (e >= nz ? e - nz : e);
+				}
 				float v = 0.f;
-				if (a >= 0 && a < nx && c >= 0 && c < ny && e >= 0 && e < nz) {
-					const int q = a + nx * (c + ny * e);
+				if (real && a >= 0 && a < nx && c >= 0 && c < ny && e >= 0 && e < nz) {
+					const int q = a + nx * ((c + j0) + gy * (e + k0));
 					v = (float)(bstart[q + 1] - bstart[q]) * inv_mean;
 				}
 				X[(size_t)b * IN + t] = v;
@@ -252,10 +266,10 @@ int mlp_adam_dev(float *P, const float *G, float *M, float *V, float lr, int t, 
 	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
 }
 
-int mlp_depth_dev(const float *P, const float *X, int n, float margin, unsigned char *depth, unsigned char *cost,
-		  hipStream_t s) {
+int mlp_depth_dev(const float *P, const float *X, int n, float margin, int route, unsigned char *depth,
+		  unsigned char *cost, hipStream_t s) {
 	if (n <= 0) return VGPU_OK;
-	hipLaunchKernelGGL(k_depth, dim3(nblk(n, 256)), dim3(256), 0, s, P, X, n, margin, depth, cost);
+	hipLaunchKernelGGL(k_depth, dim3(nblk(n, 256)), dim3(256), 0, s, P, X, n, margin, route, depth, cost);
 	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
 }
 
@@ -265,9 +279,12 @@ int mlp_forward_dev(const float *P, const float *X, int n, float *Y, hipStream_t
 	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
 }
 
-int block_features_dev(int nx, int ny, int nz, const int *bstart, float inv_mean, float *X, hipStream_t s) {
-	const int nb = nx * ny * nz;
-	hipLaunchKernelGGL(k_block_features, dim3(nblk(nb, 256)), dim3(256), 0, s, nx, ny, nz, bstart, inv_mean, X);
+int block_features_dev(int nx, int ny, int nz, const int *bstart, float inv_mean, float *X, hipStream_t s, int gy,
+		       int gz, int j0, int k0, bool wrap) {
+	if (gy <= 0) { gy = ny; gz = nz; j0 = k0 = 0; }
+	const int nb = nx * gy * gz;
+	hipLaunchKernelGGL(k_block_features, dim3(nblk(nb, 256)), dim3(256), 0, s, nx, ny, nz, gy, gz, j0, k0,
+			   wrap ? 1 : 0, bstart, inv_mean, X);
 	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
 }
 
@@ -412,6 +429,13 @@ int vgpu_predictor_margin(vgpu_predictor *p, float *get, const float *set) {
 	return VGPU_OK;
 }
 
+int vgpu_predictor_route(vgpu_predictor *p, int *get, const int *set) {
+	if (!p || (set && *set < 1)) return VGPU_INVALID_ARGUMENT;
+	if (set) p->route = *set;
+	if (get) *get = p->route;
+	return VGPU_OK;
+}
+
 // Trains on the collected samples (full-batch Adam, `epochs` steps), then
 // sets the depth margin so that about `fallback` of the samples would have
 // been under-predicted.  Parameters start from a seeded He initialisation
@@ -447,12 +471,20 @@ int vgpu_predictor_train(vgpu_predictor *p, int epochs, float lr, unsigned seed,
 	    hipMemcpy(&lo, loss, sizeof(float), hipMemcpyDeviceToHost) != hipSuccess ||
 	    hipMemcpy(y.data(), Y, sizeof(float) * y.size(), hipMemcpyDeviceToHost) != hipSuccess)
 		return VGPU_HIP_ERROR;
-	// margin: quantile of (label - prediction) in depth units
-	std::vector<float> r(n);
-	for (int t = 0; t < n; t++) r[t] = 64.f * (p->T[(size_t)t * OUT] - y[(size_t)t * OUT]);
-	std::sort(r.begin(), r.end());
-	int q = (int)std::floor((1.0 - fallback) * (n - 1));
-	p->margin = std::max(0.f, r[std::min(std::max(q, 0), n - 1)]);
+	// margin: quantile of (label - prediction) in depth units over the
+	// samples a list can finish (worklist exit below 64); the others reach
+	// the block search whatever the depth
+	std::vector<float> r;
+	r.reserve(n);
+	for (int t = 0; t < n; t++)
+		if (p->T[(size_t)t * OUT] < 1.f) r.push_back(64.f * (p->T[(size_t)t * OUT] - y[(size_t)t * OUT]));
+	if (r.empty()) p->margin = 0.f;
+	else {
+		std::sort(r.begin(), r.end());
+		const int m = (int)r.size();
+		const int q = (int)std::floor((1.0 - fallback) * (m - 1));
+		p->margin = std::max(0.f, r[std::min(std::max(q, 0), m - 1)]);
+	}
 	p->trained = true;
 	if (loss_out) *loss_out = lo;
 	return VGPU_OK;

@@ -8,19 +8,25 @@
 
 namespace vgpu {
 
-// Walks the list of sorted particle `cell` at depth G.  With FILL false it
-// only counts entries; with FILL true it writes entry k at
-// LA[k * stride], LB[k * stride].  Returns the number of entries.
-template<bool FILL>
-VG_FN int gather_cell_list(const RectView &v, const int *__restrict__ cellblk, int cell, int G, ListEntry *LA,
-			   ListTag *LB, int stride) {
-	const int ijk = cellblk[cell];
-	const int b0 = v.bstart[ijk], co0 = v.bstart[ijk + 1] - b0, s = cell - b0;
-	const dbl4 me = v.P[cell];
+// Walks the list of stored particle `cell` (block ijk of view v) at depth
+// G.  With FILL false it only counts entries; with FILL true it writes entry
+// k at LA[k * stride], LB[k * stride].  Returns the number of entries; 0
+// for G >= 64 (no list: the cell takes the plain path).
+// PRD: a periodic container's view (unit-cell start, image regions; v.fv
+// must be null here, the list kernel records the visits).
+template<bool FILL, bool PRD = false>
+VG_FN int gather_cell_list(const RectView &v, int ijk, int cell, int G, ListEntry *LA, ListTag *LB, int stride) {
+	if (G >= 64) return 0;
+	const int b0 = v.bstart[ijk], co0 = v.bstart[ijk + 1] - b0, s = cell - v.cstart[ijk];
+	const dbl4 me = v.P[b0 + s];
 	const double x = me.x, y = me.y, z = me.z;
-	const int ci = ijk % v.nx, cj = (ijk / v.nx) % v.ny, ck = ijk / v.nxy;
-	const int i = v.xper ? v.nx : ci, j = v.yper ? v.ny : cj, k = v.zper ? v.nz : ck;
-	const int disp = ijk - i - v.nx * (j + v.ny * k);
+	const int ci = ijk % v.nx, cj = (ijk / v.nx) % v.gy, ck = ijk / v.nxy;
+	int i, j, k, disp = 0;
+	if (PRD) { i = v.nx; j = v.ey; k = v.ez; }
+	else {
+		i = v.xper ? v.nx : ci; j = v.yper ? v.ny : cj; k = v.zper ? v.nz : ck;
+		disp = ijk - i - v.nx * (j + v.ny * k);
+	}
 	int n = 0;
 	// own block (src/v_compute.cc:307-323)
 	for (int t = 0; t < co0; t++) {
@@ -33,7 +39,9 @@ VG_FN int gather_cell_list(const RectView &v, const int *__restrict__ cellblk, i
 		n++;
 	}
 	// worklist selection (src/v_compute.cc:328-362)
-	const double fx = x - v.ax - v.boxx * ci, fy = y - v.ay - v.boxy * cj, fz = z - v.az - v.boxz * ck;
+	double fx, fy, fz;
+	if (PRD) { fx = x - v.boxx * ci; fy = y - v.boxy * (cj - v.ey); fz = z - v.boxz * (ck - v.ez); }
+	else { fx = x - v.ax - v.boxx * ci; fy = y - v.ay - v.boxy * cj; fz = z - v.az - v.boxz * ck; }
 	double gxs, gys, gzs;
 	unsigned m1, m2;
 	int di = int(fx * v.xsp * VGPU_WL_FGRID), dj = int(fy * v.ysp * VGPU_WL_FGRID), dk = int(fz * v.zsp * VGPU_WL_FGRID);
@@ -69,14 +77,16 @@ VG_FN int gather_cell_list(const RectView &v, const int *__restrict__ cellblk, i
 			continue;
 		}
 		double qx = 0, qy = 0, qz = 0;
-		const int rijk = rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
+		int img = 0;
+		const int rijk = PRD ? prd_region(v, ci, cj, ck, ei, ej, ek, qx, 0, img)
+				     : rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
 		const int r0 = v.bstart[rijk], cnt = v.bstart[rijk + 1] - r0;
 		if (FILL) {
 			double cn, cf;
 			int form;
 			min_max_radius_pair(v, ddi, ddj, ddk, fx, fy, fz, gxs, gys, gzs, cn, cf, form);
 			LA[(size_t)n * stride] = ListEntry{radp[g], cn, cf, 0};
-			LB[(size_t)n * stride] = ListTag{cnt, LK_BLOCK};
+			LB[(size_t)n * stride] = ListTag{cnt, PRD ? (LK_BLOCK | img << 3 | rijk << 4) : LK_BLOCK};
 			const double x2 = x - qx, y2 = y - qy, z2 = z - qz;
 			for (int t = 0; t < cnt; t++) {
 				const dbl4 o = v.P[r0 + t];

@@ -116,8 +116,9 @@ struct vgpu_container {
 	bool have_pred = false;
 	int pred_with_r = 0;
 	float pred_margin = 0;
+	int pred_route = 64;
 	DBuf d_pred, d_feat, d_cfeat, d_depth, d_cost, d_order, d_hist, lcnt, bmax, boff, list_a, list_b;
-	long long last_fallbacks = 0, last_entries = 0;
+	long long last_fallbacks = 0, last_entries = 0, last_routed = 0;
 	Workspace ws_fast, ws_wide, ws_huge, ws_wave;
 	long long nbr_total = 0;
 	bool csr_valid = false;
@@ -297,17 +298,35 @@ int launch_wave(vgpu_container *h, int k, const RectView &v, int nc, const int *
 	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
 }
 
+// The predictor's block features of a container into h->d_feat; returns
+// the number of blocks (samples) through *nsamp.  A periodic container's
+// samples are its whole block grid (image rows included, zero features
+// there), so that a cell's block index is its sample.
+int container_features(vgpu_container *h, int nc, int *nsamp, hipStream_t s) {
+	using namespace vgpu::learn;
+	const int nreal = h->g.nx * h->g.ny * h->g.nz;
+	const int nb = h->prd ? h->nblocks : nreal;
+	int st;
+	if ((st = h->d_feat.ensure((size_t)nb * IN * 4))) return st;
+	const float inv_mean = (float)nreal / (float)std::max(nc, 1);
+	if (h->prd) st = block_features_dev(h->g.nx, h->g.ny, h->g.nz, h->bstart.as<int>(), inv_mean, h->d_feat.as<float>(),
+					    s, h->pg.oy, h->pg.oz, h->pg.ey, h->pg.ez, true);
+	else st = block_features_dev(h->g.nx, h->g.ny, h->g.nz, h->bstart.as<int>(), inv_mean, h->d_feat.as<float>(), s);
+	*nsamp = nb;
+	return st;
+}
+
 // The learned tier: features, depth prediction, list gathering, the
-// list-driven cell kernel.  Cells it does not finish go to retry0.
+// list-driven cell kernel.  Cells it does not finish go to retry0.  v may
+// carry a periodic container's first-visit record (v.fv): the gather reads
+// a copy without it, the list kernel records the visits.
 int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut out, hipStream_t s) {
 	using namespace vgpu::learn;
-	const int nxyz = h->g.nx * h->g.ny * h->g.nz;
-	int st;
+	int st, nxyz = 0;
 	int *ctr = h->ctr.as<int>();
-	if ((st = h->d_feat.ensure((size_t)nxyz * IN * 4))) return st;
-	if ((st = block_features_dev(h->g.nx, h->g.ny, h->g.nz, h->bstart.as<int>(), (float)nxyz / (float)nc,
-				     h->d_feat.as<float>(), s)))
-		return st;
+	if ((st = container_features(h, nc, &nxyz, s))) return st;
+	RectView vg = v;
+	vg.fv = nullptr;
 	const float *X = h->d_feat.as<float>();
 	int ns = nxyz;
 	if (h->pred_with_r) {
@@ -320,7 +339,7 @@ int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut
 		ns = nc;
 	}
 	if ((st = h->d_depth.ensure((size_t)ns)) || (st = h->d_cost.ensure((size_t)ns))) return st;
-	if ((st = mlp_depth_dev(h->d_pred.as<float>(), X, ns, h->pred_margin, h->d_depth.as<unsigned char>(),
+	if ((st = mlp_depth_dev(h->d_pred.as<float>(), X, ns, h->pred_margin, h->pred_route, h->d_depth.as<unsigned char>(),
 				h->d_cost.as<unsigned char>(), s)))
 		return st;
 	const int nb = nblk(nc, 64);
@@ -333,7 +352,7 @@ int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut
 	if (launch_list_order(s, nc, h->cellblk.as<int>(), h->d_cost.as<unsigned char>(), h->pred_with_r,
 			      h->d_hist.as<int>(), h->d_order.as<int>()))
 		return VGPU_HIP_ERROR;
-	if (launch_list_gather(s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(), h->d_depth.as<unsigned char>(),
+	if (launch_list_gather(h->prd, s, vg, nc, h->cellblk.as<int>(), h->d_order.as<int>(), h->d_depth.as<unsigned char>(),
 			       h->pred_with_r, h->lcnt.as<int>(), h->bmax.as<int>()))
 		return VGPU_HIP_ERROR;
 	size_t ent;
@@ -357,7 +376,7 @@ int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut
 		h->last_entries = (long long)rows * 64;
 	}
 	if ((st = h->list_a.ensure(ent * sizeof(ListEntry))) || (st = h->list_b.ensure(ent * sizeof(ListTag)))) return st;
-	if (launch_list_fill(s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(), h->d_depth.as<unsigned char>(),
+	if (launch_list_fill(h->prd, s, vg, nc, h->cellblk.as<int>(), h->d_order.as<int>(), h->d_depth.as<unsigned char>(),
 			     h->pred_with_r, h->boff.as<int>(), csr, h->list_a.as<ListEntry>(), h->list_b.as<ListTag>()))
 		return VGPU_HIP_ERROR;
 	if (!h->ev_list) HIPCHK(hipEventCreate(&h->ev_list));
@@ -370,15 +389,17 @@ int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut
 		const int chunk = nc >= 4 * w ? 4 : 1;
 		w = std::max(1, std::min(w, (nc + chunk - 1) / chunk));
 		if ((st = h->ws_wave.ensure(WLayout::bytes, w))) return st;
-		if (launch_cells_wlist(h->poly, w, chunk, s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(),
+		if (launch_cells_wlist(h->poly, h->prd, w, chunk, s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(),
 				       h->lcnt.as<int>(), h->boff.as<int>(), h->list_a.as<ListEntry>(),
-				       h->list_b.as<ListTag>(), h->d_counters.as<int>(), h->ws_wave.base(), ctr + 5, out))
+				       h->list_b.as<ListTag>(), h->d_counters.as<int>(), h->ws_wave.base(), ctr + 5, ctr + 9,
+				       out))
 			return VGPU_HIP_ERROR;
 	} else {
 		if ((st = h->ws_fast.ensure(VCell3::L::bytes, waves))) return st;
-		if (launch_cells_list(h->poly, waves, s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(),
+		if (launch_cells_list(h->poly, h->prd, waves, s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(),
 				      h->lcnt.as<int>(), h->boff.as<int>(), h->list_a.as<ListEntry>(),
-				      h->list_b.as<ListTag>(), h->d_counters.as<int>(), h->ws_fast.base(), ctr + 5, out))
+				      h->list_b.as<ListTag>(), h->d_counters.as<int>(), h->ws_fast.base(), ctr + 5, ctr + 9,
+				      out))
 			return VGPU_HIP_ERROR;
 	}
 	return hipGetLastError() == hipSuccess ? VGPU_OK : VGPU_HIP_ERROR;
@@ -471,7 +492,10 @@ int upload_blocks(vgpu_container *h, const ImageSet &im, int nc, RectView &v, hi
 // that read an image block are recomputed, recording first visits again.
 // When the recorded order reproduces itself, every cell has been computed
 // against the reference's block arrays.
-int run_prd_cells(vgpu_container *h, RectView &v, CellOut out, int nc, int fw, hipStream_t s) {
+//
+// learned: the first pass takes the learned tier first (its list kernel
+// records first visits with the plain path's keys), the rest the plain tiers.
+int run_prd_cells(vgpu_container *h, RectView &v, CellOut out, int nc, int fw, hipStream_t s, bool learned) {
 	const PrdGeom &g = h->pg;
 	const int nb = h->nblocks;
 	int st;
@@ -491,7 +515,11 @@ int run_prd_cells(vgpu_container *h, RectView &v, CellOut out, int nc, int fw, h
 	hipLaunchKernelGGL(k_fill_u8, dim3(nblk(nc, 256)), dim3(256), 0, s, h->touch.as<unsigned char>(), nc, 0);
 	v.fv = fv;
 	out.touch = h->touch.as<unsigned char>();
-	if ((st = run_tiers(h, v, nc, fw, nullptr, nullptr, out, s, true))) return st;
+	if (learned) {
+		if ((st = run_learned(h, v, nc, fw, out, s))) return st;
+		HIPCHK(hipEventRecord(h->ev[2], s));
+		if ((st = run_tiers(h, v, nc, fw, h->retry0.as<int>(), h->ctr.as<int>() + 4, out, s, false, 1))) return st;
+	} else if ((st = run_tiers(h, v, nc, fw, nullptr, nullptr, out, s, true))) return st;
 	std::vector<unsigned long long> prev(nb), cur(nb);
 	std::vector<unsigned char> t(nc), touched(nc, 0);
 	HIPCHK(hipMemcpyAsync(prev.data(), fv, (size_t)nb * 8, hipMemcpyDeviceToHost, s));
@@ -667,6 +695,7 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s, bool bin_only
 		if ((st = h->st_planes.ensure(C1 * 2)) || (st = h->st_exit.ensure(C1)) || (st = h->st_tier.ensure(C1))) return st;
 	}
 	// [0] arena top [1..2] retry counts of tiers 1, 2 [3] failures [4] learned-tier retries [5] fallbacks
+	// [6..8] wavefront-cell retry counts [9] learned: cells given no list
 	int *ctr = h->ctr.as<int>();
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 16, 0);
 	RectView v = make_view(h);
@@ -707,9 +736,10 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s, bool bin_only
 	const bool learned = (flags & VGPU_F_LEARNED) != 0 && !geometry;
 	h->last_fallbacks = 0;
 	h->last_entries = 0;
+	h->last_routed = 0;
 	h->image_passes = 0;
 	if (nc > 0 && h->prd) {
-		if ((st = run_prd_cells(h, v, out, nc, fw, s))) return st;
+		if ((st = run_prd_cells(h, v, out, nc, fw, s, learned))) return st;
 	} else if (nc > 0 && learned) {
 		// tier 0: candidate lists cut at the predicted worklist depth; the
 		// cells it does not finish (retry0, count ctr[4]) take the plain tiers
@@ -738,8 +768,8 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s, bool bin_only
 		h->csr_valid = true;
 	}
 	HIPCHK(hipEventRecord(h->ev[4], s));
-	int hc[8];
-	HIPCHK(hipMemcpyAsync(hc, ctr, 32, hipMemcpyDeviceToHost, s));
+	int hc[10];
+	HIPCHK(hipMemcpyAsync(hc, ctr, 40, hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
 	// a full neighbour arena: grow it and compute again
 	if (nbrs && (long long)(unsigned)hc[0] > acap) {
@@ -770,6 +800,7 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s, bool bin_only
 	h->gather_ms = 0;
 	if (learned && nc > 0) {
 		h->last_fallbacks = hc[5];
+		h->last_routed = hc[9];
 		HIPCHK(hipEventElapsedTime(&ms, h->ev[1], h->ev_list));
 		h->gather_ms = ms;
 	}
@@ -1253,8 +1284,9 @@ int vgpu_clear(vgpu_container *h) {
 }
 
 int vgpu_compute_all(vgpu_container *h, unsigned flags, void *stream) {
-	if (h && (flags & VGPU_F_LEARNED) && (!h->have_pred || h->prd)) return VGPU_INVALID_ARGUMENT;
-	if (!h) return VGPU_INVALID_ARGUMENT;
+	if (!h || ((flags & VGPU_F_LEARNED) && !h->have_pred)) return VGPU_INVALID_ARGUMENT;
+	// a cell kind = 2 | image << 3 | block << 4 must fit an int
+	if ((flags & VGPU_F_LEARNED) && h->prd && h->nblocks >= (1 << 27)) return VGPU_INVALID_ARGUMENT;
 	if (flags & VGPU_F_GEOMETRY) flags |= VGPU_F_NEIGHBORS | VGPU_F_CSR;
 	// a capacity that overflowed (neighbour ids, cell tables) has been grown:
 	// compute again (each retry at least doubles what was missing)
@@ -1792,13 +1824,16 @@ int vgpu_set_predictor(vgpu_container *h, vgpu_predictor *p) {
 	if (st) return st;
 	HIPCHK(hipMemcpy(h->d_pred.p, p->params.data(), sizeof(float) * vgpu::learn::NP, hipMemcpyHostToDevice));
 	h->pred_margin = p->margin;
+	h->pred_route = p->route;
 	h->pred_with_r = p->with_radius;
 	h->have_pred = true;
 	return VGPU_OK;
 }
 
-int vgpu_learned_stats(vgpu_container *h, long long *fallbacks, long long *list_entries, double *gather_ms) {
+int vgpu_learned_stats(vgpu_container *h, long long *fallbacks, long long *list_entries, double *gather_ms,
+		       long long *routed) {
 	if (!h) return VGPU_INVALID_ARGUMENT;
+	if (routed) *routed = h->last_routed;
 	if (gather_ms) *gather_ms = h->gather_ms;
 	if (fallbacks) *fallbacks = h->last_fallbacks;
 	if (list_entries) *list_entries = h->last_entries;
@@ -1811,15 +1846,12 @@ int vgpu_learned_stats(vgpu_container *h, long long *fallbacks, long long *list_
 int vgpu_predictor_add_samples(vgpu_predictor *p, vgpu_container *h) {
 	using namespace vgpu::learn;
 	if (!p || !h || !h->computed || !(h->last_flags & VGPU_F_STATS)) return VGPU_INVALID_ARGUMENT;
-	if ((p->with_radius && !h->poly) || h->prd) return VGPU_INVALID_ARGUMENT;
-	const int nxyz = h->g.nx * h->g.ny * h->g.nz, nc = h->ncells;
+	if (p->with_radius && !h->poly) return VGPU_INVALID_ARGUMENT;
+	const int nc = h->ncells;
 	if (nc <= 0) return VGPU_OK;
 	hipStream_t s = nullptr;
-	int st;
-	if ((st = h->d_feat.ensure((size_t)nxyz * IN * 4))) return st;
-	if ((st = block_features_dev(h->g.nx, h->g.ny, h->g.nz, h->bstart.as<int>(), (float)nxyz / (float)nc,
-				     h->d_feat.as<float>(), s)))
-		return st;
+	int st, nxyz = 0;
+	if ((st = container_features(h, nc, &nxyz, s))) return st;
 	std::vector<float> X;
 	std::vector<int> lab;
 	int ns;

@@ -212,21 +212,23 @@ int launch_cells_prd(int tier, bool poly, int waves, hipStream_t s, const RectVi
 		     unsigned stamp_base, int per_wave, const CellOut &out);
 
 // Learned path (cells_list.hip): list gathering and the list-driven tier.
+// prd: a periodic container's view (the gather needs v.fv null; the cell
+// kernels record first visits through v.fv and image reads in out.touch).
 int launch_list_order(hipStream_t s, int nc, const int *cellblk, const unsigned char *cost, int per_cell, int *hist,
 		      int *order);
-int launch_list_gather(hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
+int launch_list_gather(bool prd, hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
 		       const unsigned char *depth, int per_cell, int *cnt, int *bmax);
-int launch_list_fill(hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
+int launch_list_fill(bool prd, hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
 		     const unsigned char *depth, int per_cell, const int *boff, bool csr, ListEntry *LA, ListTag *LB);
-int launch_cells_list(bool poly, int waves, hipStream_t s, const RectView &v, int nc, const int *cellblk,
+int launch_cells_list(bool poly, bool prd, int waves, hipStream_t s, const RectView &v, int nc, const int *cellblk,
 		      const int *order, const int *cnt, const int *boff, const ListEntry *LA, const ListTag *LB,
-		      int *counters, char *ws, int *fallbacks, const CellOut &out);
+		      int *counters, char *ws, int *fallbacks, int *routed, const CellOut &out);
 // The list-driven tier with one cell per wavefront (4 waves per SIMD) over
 // lists filled with csr, cells taken `chunk` at a time; ws holds one
 // wavefront-cell slab per wavefront.
-int launch_cells_wlist(bool poly, int waves, int chunk, hipStream_t s, const RectView &v, int nc, const int *cellblk,
-		       const int *order, const int *cnt, const int *boff, const ListEntry *LA, const ListTag *LB,
-		       int *counters, char *ws, int *fallbacks, const CellOut &out);
+int launch_cells_wlist(bool poly, bool prd, int waves, int chunk, hipStream_t s, const RectView &v, int nc,
+		       const int *cellblk, const int *order, const int *cnt, const int *boff, const ListEntry *LA,
+		       const ListTag *LB, int *counters, char *ws, int *fallbacks, int *routed, const CellOut &out);
 
 #define VGPU_CELLS_LAUNCH(TIER, P, O)                                                                      \
 	hipLaunchKernelGGL((k_cells<TIER, P, O>), dim3(waves), dim3(64), 0, s, v, ncells, list, list_count,    \
