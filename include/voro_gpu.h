@@ -323,8 +323,28 @@ int vgpu_batch_create(vgpu_container *const *hs, int count, vgpu_batch **out);
 int vgpu_batch_destroy(vgpu_batch *b);
 /* Computes every container (flags as vgpu_compute_all; VGPU_F_GRAPH
  * captures the whole device sequence as a hipGraph on the first call and
- * replays it on later calls with the same flags). */
+ * replays it on later calls with the same flags; VGPU_F_LEARNED needs a
+ * predictor from vgpu_batch_set_predictor and works with VGPU_F_GRAPH;
+ * VGPU_F_GEOMETRY is not available for batches). */
 int vgpu_batch_compute(vgpu_batch *b, unsigned flags, void *stream);
+/* Uses p (a block predictor, not one that takes the radius; NULL: none) for
+ * VGPU_F_LEARNED computations of b.  Drops a captured graph. */
+int vgpu_batch_set_predictor(vgpu_batch *b, vgpu_predictor *p);
+/* The last learned computation of b: cells whose list ended first, cells
+ * given no list (block search predicted, or their list did not fit), list
+ * entries written, and the entries every list needed (more than the
+ * capacity: the next computation gets room for all of them).  Any pointer
+ * may be NULL. */
+int vgpu_batch_learned_stats(vgpu_batch *b, long long *fallbacks, long long *routed, long long *list_entries,
+			     long long *list_need);
+/* Reads and/or replaces the candidate-list capacity (entries of 40 bytes)
+ * of b's learned computations.  Lists beyond it are not written; their cells
+ * take the plain path with identical results.  Drops a captured graph. */
+int vgpu_batch_list_capacity(vgpu_batch *b, long long *get, const long long *set);
+/* Training samples from the last vgpu_batch_compute of b (run with
+ * VGPU_F_STATS): per block of every container, as vgpu_predictor_add_samples
+ * takes them from one container (a block predictor only). */
+int vgpu_predictor_add_batch_samples(vgpu_predictor *p, vgpu_batch *b);
 /* Replaces all particle positions (same count and order as at creation:
  * n x 3 doubles, host or device memory). */
 int vgpu_batch_set_positions(vgpu_batch *b, const double *xyz, int on_device);

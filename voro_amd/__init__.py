@@ -108,6 +108,10 @@ def lib():
         "vgpu_batch_create": (C.c_int, [C.POINTER(p), C.c_int, C.POINTER(p)]),
         "vgpu_batch_destroy": (C.c_int, [p]),
         "vgpu_batch_compute": (C.c_int, [p, C.c_uint, p]),
+        "vgpu_batch_set_predictor": (C.c_int, [p, p]),
+        "vgpu_batch_learned_stats": (C.c_int, [p, i64p, i64p, i64p, i64p]),
+        "vgpu_batch_list_capacity": (C.c_int, [p, i64p, i64p]),
+        "vgpu_predictor_add_batch_samples": (C.c_int, [p, p]),
         "vgpu_batch_set_positions": (C.c_int, [p, p, C.c_int]),
         "vgpu_batch_num_cells": (C.c_int, [p, C.c_int, i64p]),
         "vgpu_batch_get_results": (C.c_int, [p, C.c_int, i32p, i32p, i32p, i32p, f64p, f64p, f64p, i32p, i64p, i32p,
@@ -876,6 +880,30 @@ class container_batch:
         _check(lib().vgpu_batch_compute(self._h, f, stream), "vgpu_batch_compute")
         self._flags = f
 
+    def set_predictor(self, pred):
+        """Uses a trained block predictor for compute(F_LEARNED | ...)."""
+        _check(lib().vgpu_batch_set_predictor(self._h, pred._h if pred is not None else None),
+               "vgpu_batch_set_predictor")
+        self._pred = pred
+
+    def learned_stats(self):
+        """fallbacks: cells whose list ended first; routed: cells given no
+        list; list_entries written; list_need: entries every list needed."""
+        a = [C.c_longlong() for _ in range(4)]
+        _check(lib().vgpu_batch_learned_stats(self._h, *[C.byref(x) for x in a]), "vgpu_batch_learned_stats")
+        return dict(zip(("fallbacks", "routed", "list_entries", "list_need"), [x.value for x in a]))
+
+    @property
+    def list_capacity(self):
+        v = C.c_longlong()
+        _check(lib().vgpu_batch_list_capacity(self._h, C.byref(v), None), "vgpu_batch_list_capacity")
+        return v.value
+
+    @list_capacity.setter
+    def list_capacity(self, entries):
+        v = C.c_longlong(int(entries))
+        _check(lib().vgpu_batch_list_capacity(self._h, None, C.byref(v)), "vgpu_batch_list_capacity")
+
     def set_positions(self, xyz):
         """New positions for every particle (same count and order)."""
         xyz = np.ascontiguousarray(xyz, dtype=np.float64)
@@ -956,7 +984,11 @@ class predictor:
             self._h = None
 
     def add_samples(self, con):
-        _check(lib().vgpu_predictor_add_samples(self._h, con._h), "vgpu_predictor_add_samples")
+        """Samples from a container's, or a batch's, last compute with F_STATS."""
+        if isinstance(con, container_batch):
+            _check(lib().vgpu_predictor_add_batch_samples(self._h, con._h), "vgpu_predictor_add_batch_samples")
+        else:
+            _check(lib().vgpu_predictor_add_samples(self._h, con._h), "vgpu_predictor_add_samples")
 
     def clear_samples(self):
         _check(lib().vgpu_predictor_clear_samples(self._h), "vgpu_predictor_clear_samples")

@@ -6,6 +6,8 @@
 // lanes read entry k of their lists in one coalesced access.
 #include <hip/hip_runtime.h>
 
+#include "batch_dev.h"
+#include "learn.h"
 #include "list_dev.h"
 #include "vgpu_cells.h"
 #include "wcell_dev.h"
@@ -22,9 +24,11 @@ struct WListSink {
 // (64 cost classes, any order within a class), so that the 64 cells a
 // wavefront cuts in lockstep need similar work.  Position i of the schedule
 // holds cell order[i]; list counts and batches follow the schedule.
-__global__ void k_order_hist(int nc, const int *__restrict__ cellblk, const unsigned char *__restrict__ cost,
-			     int per_cell, int *__restrict__ hist) {
+// ncp (may be null): the cell count on the device, at most nc.
+__global__ void k_order_hist(int nc, const int *__restrict__ ncp, const int *__restrict__ cellblk,
+			     const unsigned char *__restrict__ cost, int per_cell, int *__restrict__ hist) {
 	__shared__ int h[64];
+	if (ncp) nc = min(nc, *ncp);
 	if (threadIdx.x < 64) h[threadIdx.x] = 0;
 	__syncthreads();
 	const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -42,9 +46,11 @@ __global__ void k_order_scan(int *__restrict__ hist) {
 
 // Slots are claimed per workgroup: a class's cells of one workgroup are
 // counted in LDS and take one global atomic together.
-__global__ void k_order_scatter(int nc, const int *__restrict__ cellblk, const unsigned char *__restrict__ cost,
-				int per_cell, int *__restrict__ fill, int *__restrict__ order) {
+__global__ void k_order_scatter(int nc, const int *__restrict__ ncp, const int *__restrict__ cellblk,
+				const unsigned char *__restrict__ cost, int per_cell, int *__restrict__ fill,
+				int *__restrict__ order) {
 	__shared__ int h[64], base[64];
+	if (ncp) nc = min(nc, *ncp);
 	if (threadIdx.x < 64) h[threadIdx.x] = 0;
 	__syncthreads();
 	const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -168,6 +174,60 @@ k_cells_list(RectView v, int nc, const int *__restrict__ cellblk, const int *__r
 	}
 }
 
+// One cell of the list-driven wavefront tier: compute_cell_list over the
+// cell's list (nent entries from LA, LB), then the outputs, or an entry in
+// out's retry list (*fallbacks: the list ended first; *routed: no list).
+template<bool POLY, bool PRD>
+__device__ __forceinline__ void wlist_cell(WCell &c, const RectView &v, int cell, int ijk, const ListEntry *LA,
+					   const ListTag *LB, int nent, int *fallbacks, int *routed, const CellOut &out) {
+	const int lane = threadIdx.x;
+	const int s = cell - v.cstart[ijk];
+	CellStats stt;
+	const int r = compute_cell_list<WCell, POLY, PRD>(c, v, ijk, s, 0, 0, 0, LA, LB, 1, nent, stt,
+							  (unsigned long long)cell << 24);
+	double vol = 0, cx = 0, cy = 0, cz = 0;
+	int nf = 0;
+	bool retry = r < 0 || c.ovf;
+	if (!retry && r == 1) {
+		WListSink sk{&c};
+		c.outputs(vol, cx, cy, cz, nf, sk);
+		if (c.ovf || nf > (int)WLayout::NBCAP) retry = true;
+	}
+	if (retry) {
+		if (lane == 0) {
+			if (r == -1) atomicAdd(fallbacks, 1);
+			else if (r == -3) atomicAdd(routed, 1);
+			const int slot_ = atomicAdd(out.retry_count, 1);
+			out.retry[slot_] = cell;
+			out.status[cell] = ST_OVERFLOW;
+		}
+		return;
+	}
+	if (lane == 0) {
+		out.vol[cell] = vol;
+		out.cent[3 * cell] = cx; out.cent[3 * cell + 1] = cy; out.cent[3 * cell + 2] = cz;
+		out.nfaces[cell] = nf;
+		out.status[cell] = r == 1 ? ST_EXISTS : 0;
+		if (out.st_planes) {
+			out.st_planes[cell] = (short)min(stt.n_planes, 32767);
+			out.st_exit[cell] = (signed char)stt.wl_exit;
+			out.st_tier[cell] = (signed char)out.tier;
+		}
+		if (PRD && out.touch) out.touch[cell] = (unsigned char)stt.img;
+	}
+	if (out.want_nbrs) {
+		int off = 0;
+		if (lane == 0 && nf > 0) off = atomicAdd(out.arena_top, nf);
+		off = __builtin_amdgcn_readfirstlane(off);
+		if (off + nf > out.arena_cap) {
+			if (lane == 0) { out.nbr_start[cell] = -1; atomicAdd(out.fail_count, 1); }
+		} else {
+			if (lane == 0) out.nbr_start[cell] = off;
+			if (lane < nf) out.nbr[off + lane] = c.nb_lane(0);
+		}
+	}
+}
+
 // The same over the lists, one cell per wavefront (wcell_dev.h): the list
 // of schedule position pos is contiguous from boff[pos], its entries read
 // with uniform loads.  Positions are taken `chunk` at a time from one
@@ -201,55 +261,172 @@ k_wcells_list(RectView v_, int nc, const int *__restrict__ cellblk, const int *_
 			for (int pos = lo + base; pos < end; pos++) {
 				const int cell = order[pos];
 				const int ijk = cellblk[cell];
-				const int s = cell - v.cstart[ijk];
 				const size_t bs = (size_t)boff[pos];
-				CellStats stt;
-				const int r = compute_cell_list<WCell, POLY, PRD>(c, v, ijk, s, 0, 0, 0, LA + bs, LB + bs, 1,
-										  cnt[pos], stt, (unsigned long long)cell << 24);
-				double vol = 0, cx = 0, cy = 0, cz = 0;
-				int nf = 0;
-				bool retry = r < 0 || c.ovf;
-				if (!retry && r == 1) {
-					WListSink sk{&c};
-					c.outputs(vol, cx, cy, cz, nf, sk);
-					if (c.ovf || nf > (int)WLayout::NBCAP) retry = true;
-				}
-				if (retry) {
-					if (lane == 0) {
-						if (r == -1) atomicAdd(fallbacks, 1);
-						else if (r == -3) atomicAdd(routed, 1);
-						const int slot_ = atomicAdd(out.retry_count, 1);
-						out.retry[slot_] = cell;
-						out.status[cell] = ST_OVERFLOW;
-					}
-					continue;
-				}
-				if (lane == 0) {
-					out.vol[cell] = vol;
-					out.cent[3 * cell] = cx; out.cent[3 * cell + 1] = cy; out.cent[3 * cell + 2] = cz;
-					out.nfaces[cell] = nf;
-					out.status[cell] = r == 1 ? ST_EXISTS : 0;
-					if (out.st_planes) {
-						out.st_planes[cell] = (short)min(stt.n_planes, 32767);
-						out.st_exit[cell] = (signed char)stt.wl_exit;
-						out.st_tier[cell] = (signed char)out.tier;
-					}
-					if (PRD && out.touch) out.touch[cell] = (unsigned char)stt.img;
-				}
-				if (out.want_nbrs) {
-					int off = 0;
-					if (lane == 0 && nf > 0) off = atomicAdd(out.arena_top, nf);
-					off = __builtin_amdgcn_readfirstlane(off);
-					if (off + nf > out.arena_cap) {
-						if (lane == 0) { out.nbr_start[cell] = -1; atomicAdd(out.fail_count, 1); }
-					} else {
-						if (lane == 0) out.nbr_start[cell] = off;
-						if (lane < nf) out.nbr[off + lane] = c.nb_lane(0);
-					}
-				}
+				wlist_cell<POLY, PRD>(c, v, cell, ijk, LA + bs, LB + bs, cnt[pos], fallbacks, routed, out);
 			}
 		}
 	}
+}
+
+// ---- batches of containers ----
+
+// The container of every block of a batch (blocks of container c are
+// boff[c] .. boff[c+1]-1).
+__global__ void k_batch_bcont(int ncont, const int *__restrict__ boff, int *__restrict__ bcont) {
+	const int c = blockIdx.x;
+	if (c >= ncont) return;
+	for (int t = boff[c] + threadIdx.x; t < boff[c + 1]; t += blockDim.x) bcont[t] = c;
+}
+
+// Block features of every block of a batch: its container's 27 neighbouring
+// block counts over that container's mean count (learn.h's layout; input 27
+// is 0), as a single container's blocks get them.
+__global__ void k_batch_features(int nb, const int *__restrict__ bcont, const RectView *__restrict__ views,
+				 float *__restrict__ X) {
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= nb) return;
+	const RectView &v = views[bcont[q]];
+	const int b = q - v.boff;
+	const int nx = v.nx, ny = v.ny, nz = v.nz, nxyz = nx * ny * nz;
+	const int ncells = v.cstart[nxyz] - v.cstart[0];
+	const float inv_mean = ncells > 0 ? (float)nxyz / (float)ncells : 0.f;
+	const int i = b % nx, j = (b / nx) % ny, k = b / (nx * ny);
+	int t = 0;
+	for (int dk = -1; dk <= 1; dk++)
+		for (int dj = -1; dj <= 1; dj++)
+			for (int di = -1; di <= 1; di++, t++) {
+				const int a = i + di, c = j + dj, e = k + dk;
+				float x = 0.f;
+				if (a >= 0 && a < nx && c >= 0 && c < ny && e >= 0 && e < nz) {
+					const int r = a + nx * (c + ny * e);
+					x = (float)(v.bstart[r + 1] - v.bstart[r]) * inv_mean;
+				}
+				X[(size_t)q * learn::IN + t] = x;
+			}
+	X[(size_t)q * learn::IN + 27] = 0.f;
+}
+
+// Candidate lists of a batch's cells, schedule position i (cell order[i])
+// by thread i, in one pass: count, a wavefront-wide scan, one 64-bit bump
+// of *top per wavefront, then the fill.  A list that would pass `cap`
+// entries, or that is longer than VGPU_LIST_MAX, is not written: its count
+// is set to 0 and the cell takes the plain path (*top still grows by it, so
+// the caller learns how much room everything needed).  loff[i] / lcnt[i]:
+// the list's first entry and length.  Sizes are read on the device, so the
+// sequence needs no host round trip.
+enum { VGPU_LIST_MAX = 8192 };
+__global__ void __launch_bounds__(256)
+k_batch_list_build(const RectView *__restrict__ views, const int *__restrict__ bcont, int n, const int *__restrict__ ncp,
+		   const int *__restrict__ cellblk, const int *__restrict__ order, const unsigned char *__restrict__ depth,
+		   long long cap, unsigned long long *__restrict__ top, int *__restrict__ lcnt, int *__restrict__ loff,
+		   ListEntry *__restrict__ LA, ListTag *__restrict__ LB) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	const int nc = min(n, *ncp);
+	const int lane = threadIdx.x & 63;
+	int cell = 0, gb = 0, cont = 0, cnt = 0;
+	if (i < nc) {
+		cell = order[i];
+		gb = cellblk[cell];
+		cont = bcont[gb];
+		const RectView &v = views[cont];
+		cnt = gather_cell_list<false>(v, gb - v.boff, cell, depth[gb], nullptr, nullptr, 1);
+		if (cnt > VGPU_LIST_MAX) cnt = 0;
+	}
+	// inclusive scan over the wavefront
+	int inc = cnt;
+	for (int o = 1; o < 64; o <<= 1) {
+		const int y = __shfl_up(inc, o);
+		if (lane >= o) inc += y;
+	}
+	const int total = __shfl(inc, 63);
+	unsigned long long base = 0;
+	if (lane == 63 && total > 0) base = atomicAdd(top, (unsigned long long)total);
+	base = __shfl(base, 63);
+	const unsigned long long off = base + (unsigned long long)(inc - cnt);
+	if (i < n) {
+		if (i < nc && cnt > 0 && off + (unsigned long long)cnt <= (unsigned long long)cap) {
+			const RectView &v = views[cont];
+			gather_cell_list<true>(v, gb - v.boff, cell, depth[gb], LA + off, LB + off, 1);
+			lcnt[i] = cnt;
+			loff[i] = (int)off;
+		} else {
+			lcnt[i] = 0;
+			loff[i] = 0;
+		}
+	}
+}
+
+// The list-driven wavefront tier over a batch's schedule (positions
+// 0 .. *ncp-1), each cell with its container's view; positions are taken
+// `chunk` at a time from one counter per XCD share.
+template<bool POLY>
+__global__ void __launch_bounds__(64, 4)
+k_batch_wcells_list(const RectView *__restrict__ views, const int *__restrict__ bcont, const int *__restrict__ ncp,
+		    const int *__restrict__ cellblk, const int *__restrict__ order, const int *__restrict__ lcnt,
+		    const int *__restrict__ loff, const ListEntry *__restrict__ LA, const ListTag *__restrict__ LB,
+		    int *__restrict__ counters, char *__restrict__ ws, size_t slab_bytes, int chunk,
+		    int *__restrict__ fallbacks, int *__restrict__ routed, CellOut out) {
+	const int lane = threadIdx.x;
+	WCell c;
+	c.setup(ws + (size_t)blockIdx.x * slab_bytes, lane, 0.0);
+	const int nc = *ncp;
+	const int nshare = 8;
+	const int my = blockIdx.x % nshare;
+	const int share = (nc + nshare - 1) / nshare;
+	for (int h = 0; h < nshare; h++) {
+		const int sh = (my + h) % nshare;
+		const int lo = sh * share, hi = min(nc, lo + share);
+		for (;;) {
+			int base = 0;
+			if (lane == 0) base = atomicAdd(&counters[sh], chunk);
+			base = __builtin_amdgcn_readfirstlane(base);
+			if (lo + base >= hi) break;
+			const int end = min(hi, lo + base + chunk);
+			for (int pos = lo + base; pos < end; pos++) {
+				const int cell = __builtin_amdgcn_readfirstlane(order[pos]);
+				const int gb = __builtin_amdgcn_readfirstlane(cellblk[cell]);
+				const int cont = __builtin_amdgcn_readfirstlane(bcont[gb]);
+				typedef const __attribute__((address_space(4))) RectView CView;
+				const RectView &v = *(const RectView *)((CView *)views + cont);
+				c.set_tol(v.max_len_sq);
+				const size_t bs = (size_t)(unsigned)loff[pos];
+				wlist_cell<POLY, false>(c, v, cell, gb - v.boff, LA + bs, LB + bs, lcnt[pos], fallbacks, routed,
+							out);
+			}
+		}
+	}
+}
+
+int launch_batch_bcont(hipStream_t s, int ncont, const int *boff, int *bcont) {
+	if (ncont > 0) hipLaunchKernelGGL(k_batch_bcont, dim3(ncont), dim3(256), 0, s, ncont, boff, bcont);
+	return hipGetLastError() == hipSuccess ? 0 : 6;
+}
+
+int launch_batch_features(hipStream_t s, int nb, const int *bcont, const RectView *views, float *X) {
+	if (nb > 0) hipLaunchKernelGGL(k_batch_features, dim3((nb + 255) / 256), dim3(256), 0, s, nb, bcont, views, X);
+	return hipGetLastError() == hipSuccess ? 0 : 6;
+}
+
+int launch_batch_list_build(hipStream_t s, const RectView *views, const int *bcont, int n, const int *ncp,
+			    const int *cellblk, const int *order, const unsigned char *depth, long long cap,
+			    unsigned long long *top, int *lcnt, int *loff, ListEntry *LA, ListTag *LB) {
+	if (n > 0)
+		hipLaunchKernelGGL(k_batch_list_build, dim3((n + 255) / 256), dim3(256), 0, s, views, bcont, n, ncp, cellblk,
+				   order, depth, cap, top, lcnt, loff, LA, LB);
+	return hipGetLastError() == hipSuccess ? 0 : 6;
+}
+
+int launch_batch_wcells_list(bool poly, int waves, int chunk, hipStream_t s, const RectView *views, const int *bcont,
+			     const int *ncp, const int *cellblk, const int *order, const int *lcnt, const int *loff,
+			     const ListEntry *LA, const ListTag *LB, int *counters, char *ws, size_t slab,
+			     int *fallbacks, int *routed, const CellOut &out) {
+	if (poly)
+		hipLaunchKernelGGL((k_batch_wcells_list<true>), dim3(waves), dim3(64), 0, s, views, bcont, ncp, cellblk,
+				   order, lcnt, loff, LA, LB, counters, ws, slab, chunk, fallbacks, routed, out);
+	else
+		hipLaunchKernelGGL((k_batch_wcells_list<false>), dim3(waves), dim3(64), 0, s, views, bcont, ncp, cellblk,
+				   order, lcnt, loff, LA, LB, counters, ws, slab, chunk, fallbacks, routed, out);
+	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 
 int launch_cells_wlist(bool poly, bool prd, int waves, int chunk, hipStream_t s, const RectView &v, int nc,
@@ -265,11 +442,11 @@ int launch_cells_wlist(bool poly, bool prd, int waves, int chunk, hipStream_t s,
 }
 
 int launch_list_order(hipStream_t s, int nc, const int *cellblk, const unsigned char *cost, int per_cell, int *hist,
-		      int *order) {
+		      int *order, const int *ncp) {
 	const int g = (nc + 255) / 256;
-	hipLaunchKernelGGL(k_order_hist, dim3(g), dim3(256), 0, s, nc, cellblk, cost, per_cell, hist);
+	hipLaunchKernelGGL(k_order_hist, dim3(g), dim3(256), 0, s, nc, ncp, cellblk, cost, per_cell, hist);
 	hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(64), 0, s, hist);
-	hipLaunchKernelGGL(k_order_scatter, dim3(g), dim3(256), 0, s, nc, cellblk, cost, per_cell, hist, order);
+	hipLaunchKernelGGL(k_order_scatter, dim3(g), dim3(256), 0, s, nc, ncp, cellblk, cost, per_cell, hist, order);
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 

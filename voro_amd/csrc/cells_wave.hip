@@ -202,6 +202,42 @@ k_batch_wfirst(const RectView *__restrict__ views, const BatchChunk *__restrict_
 	}
 }
 
+// Wavefront cells of a batch over the device list of cells whose count is at
+// *list_count (the learned tier's leftovers), each cell with its container's
+// view (bcont: the container of every block), one cell per counter
+// increment, shared out per XCD as in k_wcells.
+template<bool POLY>
+__global__ void __launch_bounds__(64, 4)
+k_batch_wlist(const RectView *__restrict__ views, const int *__restrict__ bcont, const int *__restrict__ list,
+	      const int *__restrict__ list_count, const int *__restrict__ cellblk, int *__restrict__ counters,
+	      char *__restrict__ ws, size_t slab_bytes, const unsigned *__restrict__ stamp_dev, CellOut out) {
+	const int lane = threadIdx.x;
+	WCell c;
+	c.setup(ws + (size_t)blockIdx.x * slab_bytes, lane, 0.0);
+	const unsigned stamp_base = *stamp_dev;
+	const int total = *list_count;
+	const int nshare = 8;
+	const int my = blockIdx.x % nshare;
+	const int share = (total + nshare - 1) / nshare;
+	for (int h = 0; h < nshare; h++) {
+		const int sh = (my + h) % nshare;
+		const int lo = sh * share, hi = min(total, lo + share);
+		for (;;) {
+			int k = 0;
+			if (lane == 0) k = atomicAdd(&counters[sh], 1);
+			k = __builtin_amdgcn_readfirstlane(k);
+			if (lo + k >= hi) break;
+			const int cell = __builtin_amdgcn_readfirstlane(list[lo + k]);
+			const int gb = __builtin_amdgcn_readfirstlane(cellblk[cell]);
+			const int cont = __builtin_amdgcn_readfirstlane(bcont[gb]);
+			typedef const __attribute__((address_space(4))) RectView CView;
+			const RectView &v = *(const RectView *)((CView *)views + cont);
+			c.set_tol(v.max_len_sq);
+			wave_cell<1, POLY, false>(c, v, cell, gb - v.boff, stamp_base + (unsigned)cell + 1u, out);
+		}
+	}
+}
+
 }  // namespace
 
 #define VGPU_WAVE_LAUNCH(K, P, O, R)                                                                            \
@@ -257,6 +293,18 @@ int launch_batch_wfirst(bool poly, int waves, hipStream_t s, const RectView *vie
 	else
 		hipLaunchKernelGGL((k_batch_wfirst<false>), dim3(waves), dim3(64), 0, s, views, chunks, nchunks, cellblk,
 				   counters, ws, slab, stamp_dev, out);
+	return hipGetLastError() == hipSuccess ? 0 : 6;
+}
+
+int launch_batch_wlist(bool poly, int waves, hipStream_t s, const RectView *views, const int *bcont, const int *list,
+		       const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
+		       const unsigned *stamp_dev, const CellOut &out) {
+	if (poly)
+		hipLaunchKernelGGL((k_batch_wlist<true>), dim3(waves), dim3(64), 0, s, views, bcont, list, list_count, cellblk,
+				   counters, ws, slab, stamp_dev, out);
+	else
+		hipLaunchKernelGGL((k_batch_wlist<false>), dim3(waves), dim3(64), 0, s, views, bcont, list, list_count,
+				   cellblk, counters, ws, slab, stamp_dev, out);
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 

@@ -832,6 +832,15 @@ struct vgpu_batch {
 	DBuf st_planes, st_exit, st_tier;
 	Workspace ws_fast, ws_wide, ws_huge, ws_wave;
 	long long acap = 0;
+	// learned tier: predictor (per block), its outputs, the schedule, the
+	// candidate lists (capacity lcap entries, grown after a compute that
+	// needed more), the container of every block
+	DBuf d_pred, d_feat, d_depth, d_cost, d_hist, d_order, lcnt, loff, list_a, list_b, bcont, retry0, ltop;
+	bool have_pred = false, bcont_ok = false;
+	float pred_margin = 0;
+	int pred_route = 64;
+	long long lcap = 0;
+	long long last_fallbacks = 0, last_routed = 0, last_entries = 0, last_need = 0;
 	// launch configuration (autotuned): persistent wavefronts of the first
 	// tier, its kind (0 wavefront cells, 2..4 one cell per lane at that many
 	// waves per SIMD) and the cells per queue entry
@@ -898,7 +907,29 @@ int batch_prepare(vgpu_batch *b, unsigned flags) {
 	if ((st = b->part.ensure(pb))) return st;
 	if (stats && ((st = b->st_planes.ensure(N1 * 2)) || (st = b->st_exit.ensure(N1)) || (st = b->st_tier.ensure(N1))))
 		return st;
-	const bool wave = b->occ == 0;
+	const bool learned = (flags & VGPU_F_LEARNED) != 0;
+	if (learned || !b->bcont_ok) {
+		if ((st = b->bcont.ensure(NB1 * 4))) return st;
+		if (!b->bcont_ok) {
+			if (launch_batch_bcont(nullptr, b->ncont, b->boff.as<int>(), b->bcont.as<int>())) return VGPU_HIP_ERROR;
+			HIPCHK(hipStreamSynchronize(nullptr));
+			b->bcont_ok = true;
+		}
+	}
+	if (learned) {
+		using namespace vgpu::learn;
+		// about 30 entries per cell at the trained depth on random points
+		if (b->lcap <= 0) b->lcap = std::min<long long>(2147483000LL, 40LL * (long long)N1 + 4096);
+		if (b->list_a.n < (size_t)b->lcap * sizeof(ListEntry) || b->list_b.n < (size_t)b->lcap * sizeof(ListTag))
+			b->drop_graph();   // the list buffers move
+		if ((st = b->d_feat.ensure(NB1 * IN * 4)) || (st = b->d_depth.ensure(NB1)) || (st = b->d_cost.ensure(NB1)) ||
+		    (st = b->d_hist.ensure(64 * 4)) || (st = b->d_order.ensure(N1 * 4)) || (st = b->lcnt.ensure(N1 * 4)) ||
+		    (st = b->loff.ensure(N1 * 4)) || (st = b->retry0.ensure(N1 * 4)) || (st = b->ltop.ensure(8)) ||
+		    (st = b->list_a.ensure((size_t)b->lcap * sizeof(ListEntry))) ||
+		    (st = b->list_b.ensure((size_t)b->lcap * sizeof(ListTag))))
+			return st;
+	}
+	const bool wave = b->occ == 0 || learned;
 	int w = b->waves > 0 ? b->waves : default_fast_waves(wave ? 4 : b->occ);
 	w = std::max(1, std::min(w, (wave ? nblk((long long)N1, b->chunk) : nblk((long long)N1, 64)) + b->ncont));
 	b->waves = w;
@@ -973,7 +1004,50 @@ int batch_enqueue(vgpu_batch *b, unsigned flags, hipStream_t s, bool timing) {
 	// the wide tier's list: the first tier's leftovers, or (wavefront cells
 	// first) those the order-3 per-lane tier leaves (ctr[5])
 	const int *wide_list = b->retry1.as<int>(), *wide_count = ctr + 1;
-	if (b->occ == 0) {
+	if (flags & VGPU_F_LEARNED) {
+		// the learned tier: block features, predicted depths (and costs, the
+		// schedule's order), the candidate lists, the list-driven wavefront
+		// cells; their leftovers (retry0, count ctr[11]; ctr[12] lists that
+		// ended first, ctr[13] cells given no list) take wavefront cells,
+		// theirs (retry1) the per-lane order-3 tier, and so on
+		using namespace vgpu::learn;
+		const int *ncp = b->bstart.as<int>() + nb;
+		if (launch_batch_features(s, nb, b->bcont.as<int>(), b->views.as<RectView>(), b->d_feat.as<float>()))
+			return VGPU_HIP_ERROR;
+		if ((st = mlp_depth_dev(b->d_pred.as<float>(), b->d_feat.as<float>(), nb, b->pred_margin, b->pred_route,
+					b->d_depth.as<unsigned char>(), b->d_cost.as<unsigned char>(), s)))
+			return st;
+		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, b->d_hist.as<int>(), 64, 0);
+		if (n > 0 && launch_list_order(s, n, b->cellblk.as<int>(), b->d_cost.as<unsigned char>(), 0,
+					       b->d_hist.as<int>(), b->d_order.as<int>(), ncp))
+			return VGPU_HIP_ERROR;
+		hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, b->ltop.as<unsigned long long>(), 1, 0ull);
+		if (launch_batch_list_build(s, b->views.as<RectView>(), b->bcont.as<int>(), n, ncp, b->cellblk.as<int>(),
+					    b->d_order.as<int>(), b->d_depth.as<unsigned char>(), b->lcap,
+					    b->ltop.as<unsigned long long>(), b->lcnt.as<int>(), b->loff.as<int>(),
+					    b->list_a.as<ListEntry>(), b->list_b.as<ListTag>()))
+			return VGPU_HIP_ERROR;
+		out.tier = 0; out.retry = b->retry0.as<int>(); out.retry_count = ctr + 11;
+		const int chunk = n >= 4 * b->waves ? 4 : 1;
+		if (launch_batch_wcells_list(b->poly, b->waves, chunk, s, b->views.as<RectView>(), b->bcont.as<int>(), ncp,
+					     b->cellblk.as<int>(), b->d_order.as<int>(), b->lcnt.as<int>(), b->loff.as<int>(),
+					     b->list_a.as<ListEntry>(), b->list_b.as<ListTag>(), cnt8, b->ws_wave.base(),
+					     b->ws_wave.slab, ctr + 12, ctr + 13, out))
+			return VGPU_HIP_ERROR;
+		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, cnt8, 8, 0);
+		out.tier = 1; out.retry = b->retry1.as<int>(); out.retry_count = ctr + 1;
+		if (launch_batch_wlist(b->poly, b->waves, s, b->views.as<RectView>(), b->bcont.as<int>(), b->retry0.as<int>(),
+				       ctr + 11, b->cellblk.as<int>(), cnt8, b->ws_wave.base(), b->ws_wave.slab,
+				       (const unsigned *)(ctr + 8), out))
+			return VGPU_HIP_ERROR;
+		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, cnt8, 8, 0);
+		out.tier = 1; out.retry = b->retry3.as<int>(); out.retry_count = ctr + 5;
+		if (launch_batch_retry(0, b->poly, BATCH_LEFTOVER_WAVES, s, b->views.as<RectView>(), b->boff.as<int>(), nc,
+				       b->retry1.as<int>(), ctr + 1, b->cellblk.as<int>(), cnt8, b->ws_fast.base(),
+				       b->ws_fast.slab, (const unsigned *)(ctr + 8), out))
+			return VGPU_HIP_ERROR;
+		wide_list = b->retry3.as<int>(); wide_count = ctr + 5;
+	} else if (b->occ == 0) {
 		if (launch_batch_wfirst(b->poly, b->waves, s, b->views.as<RectView>(), b->chunks.as<BatchChunk>(),
 					b->nchunks.as<int>(), b->cellblk.as<int>(), cnt8, b->ws_wave.base(), b->ws_wave.slab,
 					(const unsigned *)(ctr + 8), out))
@@ -1048,8 +1122,10 @@ int batch_run(vgpu_batch *b, unsigned flags, hipStream_t s) {
 		if ((st = batch_enqueue(b, flags, s, true))) return st;
 	}
 	b->stamp_used += 3ull * ((unsigned long long)std::max(b->n, 1) + 1) + 3;
-	int hc[8];
-	HIPCHK(hipMemcpyAsync(hc, b->ctr.p, 32, hipMemcpyDeviceToHost, s));
+	int hc[16];
+	unsigned long long need = 0;
+	HIPCHK(hipMemcpyAsync(hc, b->ctr.p, 64, hipMemcpyDeviceToHost, s));
+	if (flags & VGPU_F_LEARNED) HIPCHK(hipMemcpyAsync(&need, b->ltop.p, 8, hipMemcpyDeviceToHost, s));
 	b->h_cfirst.resize(b->ncont + 1);
 	HIPCHK(hipMemcpyAsync(b->h_cfirst.data(), b->cfirst.p, (size_t)(b->ncont + 1) * 4, hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
@@ -1065,6 +1141,17 @@ int batch_run(vgpu_batch *b, unsigned flags, hipStream_t s) {
 	}
 	b->nbr_total = hc[0];
 	b->fails = hc[3];
+	b->last_fallbacks = b->last_routed = b->last_entries = b->last_need = 0;
+	if (flags & VGPU_F_LEARNED) {
+		b->last_fallbacks = hc[12];
+		b->last_routed = hc[13];
+		b->last_need = (long long)need;
+		b->last_entries = std::min((long long)need, b->lcap);
+		// lists that did not fit took the plain path (results are the
+		// same); give the next compute room for all of them
+		if ((long long)need > b->lcap)
+			b->lcap = std::min<long long>(2147483000LL, (long long)need + (long long)need / 8 + 4096);
+	}
 	b->computed = hc[3] == 0;   // no results to read after a failure
 	b->last_flags = flags;
 	return hc[3] > 0 ? VGPU_MEMORY_ERROR : VGPU_OK;
@@ -1905,6 +1992,47 @@ int vgpu_predictor_add_samples(vgpu_predictor *p, vgpu_container *h) {
 	return VGPU_OK;
 }
 
+// Training samples from the last vgpu_batch_compute (run with
+// VGPU_F_STATS): per block of every container, its features and the largest
+// worklist exit and nplane count of its cells (a block predictor only).
+int vgpu_predictor_add_batch_samples(vgpu_predictor *p, vgpu_batch *b) {
+	using namespace vgpu::learn;
+	if (!p || !b || !b->computed || !(b->last_flags & VGPU_F_STATS) || p->with_radius) return VGPU_INVALID_ARGUMENT;
+	const int nb = b->nblocks;
+	if (nb <= 0 || b->n <= 0) return VGPU_OK;
+	int st;
+	if ((st = b->bcont.ensure(((size_t)nb + 1) * 4)) || (st = b->d_feat.ensure(((size_t)nb + 1) * IN * 4))) return st;
+	if (!b->bcont_ok) {
+		if (launch_batch_bcont(nullptr, b->ncont, b->boff.as<int>(), b->bcont.as<int>())) return VGPU_HIP_ERROR;
+		b->bcont_ok = true;
+	}
+	if (launch_batch_features(nullptr, nb, b->bcont.as<int>(), b->views.as<RectView>(), b->d_feat.as<float>()))
+		return VGPU_HIP_ERROR;
+	std::vector<int> bs((size_t)nb + 1);
+	HIPCHK(hipMemcpy(bs.data(), b->bstart.p, bs.size() * 4, hipMemcpyDeviceToHost));
+	const int nc = bs[nb];
+	DBuf d_lab;
+	if ((st = d_lab.ensure((size_t)nb * 8))) return st;
+	if ((st = block_labels_dev(nc, b->cellblk.as<int>(), b->st_exit.as<signed char>(), b->st_planes.as<short>(), nb,
+				   d_lab.as<int>(), nullptr))) {
+		d_lab.release();
+		return st;
+	}
+	std::vector<float> X((size_t)nb * IN);
+	std::vector<int> lab((size_t)2 * nb);
+	const bool ok = hipMemcpy(X.data(), b->d_feat.p, X.size() * 4, hipMemcpyDeviceToHost) == hipSuccess &&
+			hipMemcpy(lab.data(), d_lab.p, lab.size() * 4, hipMemcpyDeviceToHost) == hipSuccess;
+	d_lab.release();
+	if (!ok) return VGPU_HIP_ERROR;
+	for (int q = 0; q < nb; q++) {
+		if (bs[q + 1] == bs[q]) continue;   // only blocks that hold cells carry a label
+		p->X.insert(p->X.end(), &X[(size_t)q * IN], &X[(size_t)q * IN] + IN);
+		p->T.push_back((float)lab[2 * q] / 64.f);
+		p->T.push_back(std::log1p((float)lab[2 * q + 1]) / 8.f);
+	}
+	return VGPU_OK;
+}
+
 int vgpu_predictor_clear_samples(vgpu_predictor *p) {
 	if (!p) return VGPU_INVALID_ARGUMENT;
 	p->X.clear();
@@ -2015,7 +2143,8 @@ int vgpu_batch_destroy(vgpu_batch *b) {
 			&b->in_r, &b->blk, &b->slot, &b->rem, &b->cnt, &b->bstart, &b->tmp, &b->part, &b->P, &b->pid,
 			&b->src, &b->cellblk, &b->vol, &b->cent, &b->nfaces, &b->status, &b->nbr_start, &b->arena,
 			&b->csr_off, &b->csr, &b->ctr, &b->counters, &b->retry1, &b->retry2, &b->retry3, &b->st_planes,
-			&b->st_exit, &b->st_tier};
+			&b->st_exit, &b->st_tier, &b->d_pred, &b->d_feat, &b->d_depth, &b->d_cost, &b->d_hist,
+			&b->d_order, &b->lcnt, &b->loff, &b->list_a, &b->list_b, &b->bcont, &b->retry0, &b->ltop};
 	for (DBuf *x : bufs) x->release();
 	b->ws_fast.buf.release(); b->ws_wide.buf.release(); b->ws_huge.buf.release(); b->ws_wave.buf.release();
 	if (b->events) for (int t = 0; t < 4; t++) (void)hipEventDestroy(b->ev[t]);
@@ -2024,8 +2153,44 @@ int vgpu_batch_destroy(vgpu_batch *b) {
 }
 
 int vgpu_batch_compute(vgpu_batch *b, unsigned flags, void *stream) {
-	if (!b || (flags & (VGPU_F_LEARNED | VGPU_F_GEOMETRY))) return VGPU_INVALID_ARGUMENT;
+	if (!b || (flags & VGPU_F_GEOMETRY) || ((flags & VGPU_F_LEARNED) && !b->have_pred)) return VGPU_INVALID_ARGUMENT;
 	return batch_run(b, flags, (hipStream_t)stream);
+}
+
+int vgpu_batch_set_predictor(vgpu_batch *b, vgpu_predictor *p) {
+	if (!b) return VGPU_INVALID_ARGUMENT;
+	b->drop_graph();   // margin and route are launch arguments of a captured graph
+	if (!p) { b->have_pred = false; return VGPU_OK; }
+	if (!p->trained || p->params.size() != (size_t)vgpu::learn::NP || p->with_radius) return VGPU_INVALID_ARGUMENT;
+	int st = b->d_pred.ensure(sizeof(float) * vgpu::learn::NP);
+	if (st) return st;
+	HIPCHK(hipMemcpy(b->d_pred.p, p->params.data(), sizeof(float) * vgpu::learn::NP, hipMemcpyHostToDevice));
+	b->pred_margin = p->margin;
+	b->pred_route = p->route;
+	b->have_pred = true;
+	return VGPU_OK;
+}
+
+int vgpu_batch_learned_stats(vgpu_batch *b, long long *fallbacks, long long *routed, long long *list_entries,
+			     long long *list_need) {
+	if (!b) return VGPU_INVALID_ARGUMENT;
+	if (fallbacks) *fallbacks = b->last_fallbacks;
+	if (routed) *routed = b->last_routed;
+	if (list_entries) *list_entries = b->last_entries;
+	if (list_need) *list_need = b->last_need;
+	return VGPU_OK;
+}
+
+int vgpu_batch_list_capacity(vgpu_batch *b, long long *get, const long long *set) {
+	if (!b || (set && (*set < 1 || *set > 2147483000LL))) return VGPU_INVALID_ARGUMENT;
+	if (set && *set != b->lcap) {
+		b->drop_graph();
+		b->lcap = *set;
+		b->list_a.release();
+		b->list_b.release();
+	}
+	if (get) *get = b->lcap;
+	return VGPU_OK;
 }
 
 int vgpu_batch_set_positions(vgpu_batch *b, const double *xyz, int on_device) {

@@ -214,8 +214,9 @@ int launch_cells_prd(int tier, bool poly, int waves, hipStream_t s, const RectVi
 // Learned path (cells_list.hip): list gathering and the list-driven tier.
 // prd: a periodic container's view (the gather needs v.fv null; the cell
 // kernels record first visits through v.fv and image reads in out.touch).
+// ncp (may be null): the cell count on the device, at most nc.
 int launch_list_order(hipStream_t s, int nc, const int *cellblk, const unsigned char *cost, int per_cell, int *hist,
-		      int *order);
+		      int *order, const int *ncp = nullptr);
 int launch_list_gather(bool prd, hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
 		       const unsigned char *depth, int per_cell, int *cnt, int *bmax);
 int launch_list_fill(bool prd, hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
