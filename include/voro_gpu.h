@@ -278,8 +278,8 @@ int vgpu_predictor_route(vgpu_predictor *p, int *get, const int *set);
 /* Uses p (NULL: none) for VGPU_F_LEARNED computations of h. */
 int vgpu_set_predictor(vgpu_container *h, vgpu_predictor *p);
 /* Cells of the last learned computation whose list ended before the cell
- * was finished (they fell back to the plain path), the list entries
- * gathered (including batch padding), the time from the end of binning to
+ * was finished (they fell back to the plain path), the list records
+ * gathered (one per worklist step, including batch padding), the time from the end of binning to
  * the end of list gathering (features, prediction, gathering; included in
  * vgpu_last_timings' first-tier figure), and the cells given no list because
  * the block search was predicted (they took the plain path at once).  Any
@@ -332,12 +332,12 @@ int vgpu_batch_compute(vgpu_batch *b, unsigned flags, void *stream);
 int vgpu_batch_set_predictor(vgpu_batch *b, vgpu_predictor *p);
 /* The last learned computation of b: cells whose list ended first, cells
  * given no list (block search predicted, or their list did not fit), list
- * entries written, and the entries every list needed (more than the
+ * records written, and the records every list needed (more than the
  * capacity: the next computation gets room for all of them).  Any pointer
  * may be NULL. */
 int vgpu_batch_learned_stats(vgpu_batch *b, long long *fallbacks, long long *routed, long long *list_entries,
 			     long long *list_need);
-/* Reads and/or replaces the candidate-list capacity (entries of 40 bytes)
+/* Reads and/or replaces the candidate-list capacity (records of 32 bytes)
  * of b's learned computations.  Lists beyond it are not written; their cells
  * take the plain path with identical results.  Drops a captured graph. */
 int vgpu_batch_list_capacity(vgpu_batch *b, long long *get, const long long *set);

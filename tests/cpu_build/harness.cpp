@@ -61,17 +61,15 @@ template<bool POLY, class CELL = VCell3, size_t BYTES = Layout3<>::lane_bytes, b
 struct ListRunner {
 	std::vector<char> slab;
 	CELL c;
-	std::vector<ListEntry> la;
-	std::vector<ListTag> lb;
+	std::vector<ListRec> la;
 	explicit ListRunner(double mls) : slab(BYTES + 2 * 65536, 0) { c.setup(slab.data() + 65536, 0, mls); }
 	int run(const RectView &v, const int *cellblk, int cell, int G, int ijk, int s, double *vol, double *cent, int *nf,
 		std::vector<int> &nb, CellStats &st, long long *fallbacks, unsigned long long key0 = 0) {
 		RectView vg = v;
 		vg.fv = nullptr;
-		const int n = gather_cell_list<false, PRD>(vg, cellblk[cell], cell, G, nullptr, nullptr, 1);
-		la.resize(n); lb.resize(n);
-		gather_cell_list<true, PRD>(vg, cellblk[cell], cell, G, la.data(), lb.data(), 1);
-		int r = compute_cell_list<CELL, POLY, PRD>(c, v, ijk, s, 0, 0, 0, la.data(), lb.data(), 1, n, st, key0);
+		la.resize(64);
+		const int n = gather_cell_list<PRD>(vg, cellblk[cell], cell, G, la.data(), 1);
+		int r = compute_cell_list<CELL, POLY, PRD>(c, v, ijk, s, la.data(), 1, n, st, key0);
 		if (r == -1) { (*fallbacks)++; return -1; }
 		if (r < 0 || c.ovf) return -1;
 		if (r == 1) {

@@ -22,7 +22,8 @@ ARCH = os.environ.get("VGPU_ARCH", "gfx950")
 UNITS = ["vgpu_api.hip", "cells_fast3.hip", "cells_wide.hip", "cells_huge.hip", "cells_prd.hip", "cells_list.hip",
          "cells_batch.hip", "cells_wave.hip", "find_cell.hip", "learn.hip", "geometry.hip", "text_io.hip"]
 HEADERS = ["vgpu_common.h", "cell_dev.h", "compute_dev.h", "vgpu_cells.h", "vgpu_bin.h", "grid_host.h",
-           "worklist_table.h", "cell3_dev.h", "list_dev.h", "learn.h", "prd_host.h", "batch_dev.h", "wcell_dev.h", "geom_dev.h"]
+           "worklist_table.h", "cell3_dev.h", "list_dev.h", "learn.h", "prd_host.h", "batch_dev.h", "wcell_dev.h", "geom_dev.h",
+           "wave_cell_dev.h"]
 
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fno-unroll-loops", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -66,24 +66,21 @@ int launch_batch_retry(int tier, bool poly, int waves, hipStream_t s, const Rect
 		       int ncont, const int *list, const int *list_count, const int *cellblk, int *counters, char *ws,
 		       size_t slab, const unsigned *stamp_dev, const CellOut &out);
 
-// The learned tier of a batch (cells_list.hip, cells_wave.hip): the
-// container of every block; block features; candidate lists built in one
-// pass into a fixed capacity (lists that do not fit give their cells no
-// list; *top: the entries everything needed); the list-driven wavefront
-// tier over the schedule; wavefront cells over its leftovers (a list).
-// Counts are read on the device (ncp: the cells), so the sequence can be
-// captured.
+// The learned tier of a batch (cells_list.hip): the container of every
+// block; block features; candidate lists built in one pass into a fixed
+// capacity (lists that do not fit give their cells no list; *top: the
+// records everything needed); the list-driven wavefront tier over the
+// schedule, whose cells without a usable list continue as plain wavefront
+// cells (block-mask stamps from *stamp_dev).  Counts are read on the device
+// (ncp: the cells), so the sequence can be captured.
 int launch_batch_bcont(hipStream_t s, int ncont, const int *boff, int *bcont);
 int launch_batch_features(hipStream_t s, int nb, const int *bcont, const RectView *views, float *X);
 int launch_batch_list_build(hipStream_t s, const RectView *views, const int *bcont, int n, const int *ncp,
 			    const int *cellblk, const int *order, const unsigned char *depth, long long cap,
-			    unsigned long long *top, int *lcnt, int *loff, ListEntry *LA, ListTag *LB);
+			    unsigned long long *top, int *lcnt, int *loff, ListRec *LA);
 int launch_batch_wcells_list(bool poly, int waves, int chunk, hipStream_t s, const RectView *views, const int *bcont,
 			     const int *ncp, const int *cellblk, const int *order, const int *lcnt, const int *loff,
-			     const ListEntry *LA, const ListTag *LB, int *counters, char *ws, size_t slab,
+			     const ListRec *LA, int *counters, char *ws, size_t slab, const unsigned *stamp_dev,
 			     int *fallbacks, int *routed, const CellOut &out);
-int launch_batch_wlist(bool poly, int waves, hipStream_t s, const RectView *views, const int *bcont, const int *list,
-		       const int *list_count, const int *cellblk, int *counters, char *ws, size_t slab,
-		       const unsigned *stamp_dev, const CellOut &out);
 
 }  // namespace vgpu

@@ -10,6 +10,7 @@
 #include "learn.h"
 #include "list_dev.h"
 #include "vgpu_cells.h"
+#include "wave_cell_dev.h"
 #include "wcell_dev.h"
 
 namespace vgpu {
@@ -72,7 +73,7 @@ __global__ void k_list_count(RectView v, int nc, const int *__restrict__ cellblk
 	if (i >= nc) return;
 	const int c = order[i];
 	const int G = depth[per_cell ? c : cellblk[c]];
-	cnt[i] = gather_cell_list<false, PRD>(v, cellblk[c], c, G, nullptr, nullptr, 64);
+	cnt[i] = list_length(G);
 }
 
 // Capacity of each batch of 64 cells (the longest list in it); entry nb is
@@ -92,17 +93,17 @@ __global__ void k_batch_max(int nc, const int *__restrict__ cnt, int *__restrict
 template<bool PRD>
 __global__ void k_list_fill(RectView v, int nc, const int *__restrict__ cellblk, const int *__restrict__ order,
 			    const unsigned char *__restrict__ depth, int per_cell, const int *__restrict__ boff,
-			    int csr, ListEntry *__restrict__ LA, ListTag *__restrict__ LB) {
+			    int csr, ListRec *__restrict__ LA) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= nc) return;
 	const int c = order[i];
 	const int G = depth[per_cell ? c : cellblk[c]];
 	if (csr) {
 		const size_t base = (size_t)boff[i];
-		gather_cell_list<true, PRD>(v, cellblk[c], c, G, LA + base, LB + base, 1);
+		gather_cell_list<PRD>(v, cellblk[c], c, G, LA + base, 1);
 	} else {
 		const size_t base = (size_t)boff[i >> 6] * 64 + (i & 63);
-		gather_cell_list<true, PRD>(v, cellblk[c], c, G, LA + base, LB + base, 64);
+		gather_cell_list<PRD>(v, cellblk[c], c, G, LA + base, 64);
 	}
 }
 
@@ -116,7 +117,7 @@ __global__ void k_list_fill(RectView v, int nc, const int *__restrict__ cellblk,
 template<bool POLY, bool PRD, int OCC>
 __global__ void __launch_bounds__(64, OCC)
 k_cells_list(RectView v, int nc, const int *__restrict__ cellblk, const int *__restrict__ order, const int *__restrict__ cnt,
-	     const int *__restrict__ boff, const ListEntry *__restrict__ LA, const ListTag *__restrict__ LB,
+	     const int *__restrict__ boff, const ListRec *__restrict__ LA,
 	     int *__restrict__ counters, char *__restrict__ ws, int *__restrict__ fallbacks, int *__restrict__ routed,
 	     CellOut out) {
 	const int lane = threadIdx.x;
@@ -135,7 +136,7 @@ k_cells_list(RectView v, int nc, const int *__restrict__ cellblk, const int *__r
 		const int s = cell - v.cstart[ijk];
 		const size_t base = (size_t)boff[b] * 64 + lane;
 		CellStats stt;
-		const int r = compute_cell_list<VCell3, POLY, PRD>(c, v, ijk, s, 0, 0, 0, LA + base, LB + base, 64, cnt[pos],
+		const int r = compute_cell_list<VCell3, POLY, PRD>(c, v, ijk, s, LA + base, 64, cnt[pos],
 								   stt, (unsigned long long)cell << 24);
 		double vol = 0, cx = 0, cy = 0, cz = 0;
 		int nf = 0;
@@ -175,18 +176,27 @@ k_cells_list(RectView v, int nc, const int *__restrict__ cellblk, const int *__r
 }
 
 // One cell of the list-driven wavefront tier: compute_cell_list over the
-// cell's list (nent entries from LA, LB), then the outputs, or an entry in
-// out's retry list (*fallbacks: the list ended first; *routed: no list).
+// cell's list (nent records from LA), then the outputs, or an entry in
+// out's retry list (overflow).  A cell whose list ends first (*fallbacks) or
+// that has no list (*routed) is computed again at once by the plain
+// algorithm in the same wavefront (block-mask stamp `stamp`), so that the
+// deep cells -- scheduled first -- overlap with the rest instead of running
+// after them.
 template<bool POLY, bool PRD>
-__device__ __forceinline__ void wlist_cell(WCell &c, const RectView &v, int cell, int ijk, const ListEntry *LA,
-					   const ListTag *LB, int nent, int *fallbacks, int *routed, const CellOut &out) {
+__device__ __forceinline__ void wlist_cell(WCell &c, const RectView &v, int cell, int ijk, const ListRec *LA, int nent,
+					   unsigned stamp, int *fallbacks, int *routed, const CellOut &out) {
 	const int lane = threadIdx.x;
 	const int s = cell - v.cstart[ijk];
 	CellStats stt;
-	const int r = compute_cell_list<WCell, POLY, PRD>(c, v, ijk, s, 0, 0, 0, LA, LB, 1, nent, stt,
+	const int r = compute_cell_list<WCell, POLY, PRD>(c, v, ijk, s, LA, 1, nent, stt,
 							  (unsigned long long)cell << 24);
 	double vol = 0, cx = 0, cy = 0, cz = 0;
 	int nf = 0;
+	if (r == -1 || r == -3) {
+		if (lane == 0) atomicAdd(r == -1 ? fallbacks : routed, 1);
+		wave_cell<1, POLY, PRD>(c, v, cell, ijk, stamp, out);
+		return;
+	}
 	bool retry = r < 0 || c.ovf;
 	if (!retry && r == 1) {
 		WListSink sk{&c};
@@ -195,8 +205,6 @@ __device__ __forceinline__ void wlist_cell(WCell &c, const RectView &v, int cell
 	}
 	if (retry) {
 		if (lane == 0) {
-			if (r == -1) atomicAdd(fallbacks, 1);
-			else if (r == -3) atomicAdd(routed, 1);
 			const int slot_ = atomicAdd(out.retry_count, 1);
 			out.retry[slot_] = cell;
 			out.status[cell] = ST_OVERFLOW;
@@ -235,8 +243,8 @@ __device__ __forceinline__ void wlist_cell(WCell &c, const RectView &v, int cell
 template<bool POLY, bool PRD, int OCC>
 __global__ void __launch_bounds__(64, OCC)
 k_wcells_list(RectView v_, int nc, const int *__restrict__ cellblk, const int *__restrict__ order,
-	      const int *__restrict__ cnt, const int *__restrict__ boff, const ListEntry *__restrict__ LA,
-	      const ListTag *__restrict__ LB, int *__restrict__ counters, char *__restrict__ ws, int chunk,
+	      const int *__restrict__ cnt, const int *__restrict__ boff, const ListRec *__restrict__ LA,
+	      int *__restrict__ counters, char *__restrict__ ws, unsigned stamp_base, int chunk,
 	      int *__restrict__ fallbacks, int *__restrict__ routed, CellOut out) {
 	typedef const __attribute__((address_space(4))) RectView CView;
 	const int lane = threadIdx.x;
@@ -262,7 +270,8 @@ k_wcells_list(RectView v_, int nc, const int *__restrict__ cellblk, const int *_
 				const int cell = order[pos];
 				const int ijk = cellblk[cell];
 				const size_t bs = (size_t)boff[pos];
-				wlist_cell<POLY, PRD>(c, v, cell, ijk, LA + bs, LB + bs, cnt[pos], fallbacks, routed, out);
+				wlist_cell<POLY, PRD>(c, v, cell, ijk, LA + bs, cnt[pos], stamp_base + (unsigned)cell + 1u, fallbacks,
+						      routed, out);
 			}
 		}
 	}
@@ -307,30 +316,26 @@ __global__ void k_batch_features(int nb, const int *__restrict__ bcont, const Re
 }
 
 // Candidate lists of a batch's cells, schedule position i (cell order[i])
-// by thread i, in one pass: count, a wavefront-wide scan, one 64-bit bump
-// of *top per wavefront, then the fill.  A list that would pass `cap`
-// entries, or that is longer than VGPU_LIST_MAX, is not written: its count
-// is set to 0 and the cell takes the plain path (*top still grows by it, so
-// the caller learns how much room everything needed).  loff[i] / lcnt[i]:
-// the list's first entry and length.  Sizes are read on the device, so the
+// by thread i, in one pass: the length (depth + 1 records), a
+// wavefront-wide scan, one 64-bit bump of *top per wavefront, then the
+// fill.  A list that would pass `cap` records is not written: its length is
+// set to 0 and the cell takes the plain path (*top still grows by it, so the
+// caller learns how much room everything needed).  loff[i] / lcnt[i]: the
+// list's first record and length.  Sizes are read on the device, so the
 // sequence needs no host round trip.
-enum { VGPU_LIST_MAX = 8192 };
 __global__ void __launch_bounds__(256)
 k_batch_list_build(const RectView *__restrict__ views, const int *__restrict__ bcont, int n, const int *__restrict__ ncp,
 		   const int *__restrict__ cellblk, const int *__restrict__ order, const unsigned char *__restrict__ depth,
 		   long long cap, unsigned long long *__restrict__ top, int *__restrict__ lcnt, int *__restrict__ loff,
-		   ListEntry *__restrict__ LA, ListTag *__restrict__ LB) {
+		   ListRec *__restrict__ LA) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	const int nc = min(n, *ncp);
 	const int lane = threadIdx.x & 63;
-	int cell = 0, gb = 0, cont = 0, cnt = 0;
+	int cell = 0, gb = 0, cnt = 0;
 	if (i < nc) {
 		cell = order[i];
 		gb = cellblk[cell];
-		cont = bcont[gb];
-		const RectView &v = views[cont];
-		cnt = gather_cell_list<false>(v, gb - v.boff, cell, depth[gb], nullptr, nullptr, 1);
-		if (cnt > VGPU_LIST_MAX) cnt = 0;
+		cnt = list_length(depth[gb]);
 	}
 	// inclusive scan over the wavefront
 	int inc = cnt;
@@ -345,8 +350,8 @@ k_batch_list_build(const RectView *__restrict__ views, const int *__restrict__ b
 	const unsigned long long off = base + (unsigned long long)(inc - cnt);
 	if (i < n) {
 		if (i < nc && cnt > 0 && off + (unsigned long long)cnt <= (unsigned long long)cap) {
-			const RectView &v = views[cont];
-			gather_cell_list<true>(v, gb - v.boff, cell, depth[gb], LA + off, LB + off, 1);
+			const RectView &v = views[bcont[gb]];
+			gather_cell_list<false>(v, gb - v.boff, cell, depth[gb], LA + off, 1);
 			lcnt[i] = cnt;
 			loff[i] = (int)off;
 		} else {
@@ -363,12 +368,13 @@ template<bool POLY>
 __global__ void __launch_bounds__(64, 4)
 k_batch_wcells_list(const RectView *__restrict__ views, const int *__restrict__ bcont, const int *__restrict__ ncp,
 		    const int *__restrict__ cellblk, const int *__restrict__ order, const int *__restrict__ lcnt,
-		    const int *__restrict__ loff, const ListEntry *__restrict__ LA, const ListTag *__restrict__ LB,
-		    int *__restrict__ counters, char *__restrict__ ws, size_t slab_bytes, int chunk,
-		    int *__restrict__ fallbacks, int *__restrict__ routed, CellOut out) {
+		    const int *__restrict__ loff, const ListRec *__restrict__ LA,
+		    int *__restrict__ counters, char *__restrict__ ws, size_t slab_bytes, const unsigned *__restrict__ stamp_dev,
+		    int chunk, int *__restrict__ fallbacks, int *__restrict__ routed, CellOut out) {
 	const int lane = threadIdx.x;
 	WCell c;
 	c.setup(ws + (size_t)blockIdx.x * slab_bytes, lane, 0.0);
+	const unsigned stamp_base = *stamp_dev;
 	const int nc = *ncp;
 	const int nshare = 8;
 	const int my = blockIdx.x % nshare;
@@ -390,8 +396,8 @@ k_batch_wcells_list(const RectView *__restrict__ views, const int *__restrict__ 
 				const RectView &v = *(const RectView *)((CView *)views + cont);
 				c.set_tol(v.max_len_sq);
 				const size_t bs = (size_t)(unsigned)loff[pos];
-				wlist_cell<POLY, false>(c, v, cell, gb - v.boff, LA + bs, LB + bs, lcnt[pos], fallbacks, routed,
-							out);
+				wlist_cell<POLY, false>(c, v, cell, gb - v.boff, LA + bs, lcnt[pos], stamp_base + (unsigned)cell + 1u,
+							fallbacks, routed, out);
 			}
 		}
 	}
@@ -409,32 +415,32 @@ int launch_batch_features(hipStream_t s, int nb, const int *bcont, const RectVie
 
 int launch_batch_list_build(hipStream_t s, const RectView *views, const int *bcont, int n, const int *ncp,
 			    const int *cellblk, const int *order, const unsigned char *depth, long long cap,
-			    unsigned long long *top, int *lcnt, int *loff, ListEntry *LA, ListTag *LB) {
+			    unsigned long long *top, int *lcnt, int *loff, ListRec *LA) {
 	if (n > 0)
 		hipLaunchKernelGGL(k_batch_list_build, dim3((n + 255) / 256), dim3(256), 0, s, views, bcont, n, ncp, cellblk,
-				   order, depth, cap, top, lcnt, loff, LA, LB);
+				   order, depth, cap, top, lcnt, loff, LA);
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 
 int launch_batch_wcells_list(bool poly, int waves, int chunk, hipStream_t s, const RectView *views, const int *bcont,
 			     const int *ncp, const int *cellblk, const int *order, const int *lcnt, const int *loff,
-			     const ListEntry *LA, const ListTag *LB, int *counters, char *ws, size_t slab,
+			     const ListRec *LA, int *counters, char *ws, size_t slab, const unsigned *stamp_dev,
 			     int *fallbacks, int *routed, const CellOut &out) {
 	if (poly)
 		hipLaunchKernelGGL((k_batch_wcells_list<true>), dim3(waves), dim3(64), 0, s, views, bcont, ncp, cellblk,
-				   order, lcnt, loff, LA, LB, counters, ws, slab, chunk, fallbacks, routed, out);
+				   order, lcnt, loff, LA, counters, ws, slab, stamp_dev, chunk, fallbacks, routed, out);
 	else
 		hipLaunchKernelGGL((k_batch_wcells_list<false>), dim3(waves), dim3(64), 0, s, views, bcont, ncp, cellblk,
-				   order, lcnt, loff, LA, LB, counters, ws, slab, chunk, fallbacks, routed, out);
+				   order, lcnt, loff, LA, counters, ws, slab, stamp_dev, chunk, fallbacks, routed, out);
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 
 int launch_cells_wlist(bool poly, bool prd, int waves, int chunk, hipStream_t s, const RectView &v, int nc,
-		       const int *cellblk, const int *order, const int *cnt, const int *boff, const ListEntry *LA,
-		       const ListTag *LB, int *counters, char *ws, int *fallbacks, int *routed, const CellOut &out) {
+		       const int *cellblk, const int *order, const int *cnt, const int *boff, const ListRec *LA,
+		       int *counters, char *ws, unsigned stamp_base, int *fallbacks, int *routed, const CellOut &out) {
 #define VG_WLIST(P, R)                                                                                              \
 	hipLaunchKernelGGL((k_wcells_list<P, R, 4>), dim3(waves), dim3(64), 0, s, v, nc, cellblk, order, cnt, boff, LA, \
-			   LB, counters, ws, chunk, fallbacks, routed, out)
+			   counters, ws, stamp_base, chunk, fallbacks, routed, out)
 	if (prd) { if (poly) VG_WLIST(true, true); else VG_WLIST(false, true); }
 	else { if (poly) VG_WLIST(true, false); else VG_WLIST(false, false); }
 #undef VG_WLIST
@@ -462,23 +468,23 @@ int launch_list_gather(bool prd, hipStream_t s, const RectView &v, int nc, const
 }
 
 int launch_list_fill(bool prd, hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
-		     const unsigned char *depth, int per_cell, const int *boff, bool csr, ListEntry *LA, ListTag *LB) {
+		     const unsigned char *depth, int per_cell, const int *boff, bool csr, ListRec *LA) {
 	if (v.fv) return 2;
 	if (prd)
 		hipLaunchKernelGGL(k_list_fill<true>, dim3((nc + 255) / 256), dim3(256), 0, s, v, nc, cellblk, order, depth,
-				   per_cell, boff, csr ? 1 : 0, LA, LB);
+				   per_cell, boff, csr ? 1 : 0, LA);
 	else
 		hipLaunchKernelGGL(k_list_fill<false>, dim3((nc + 255) / 256), dim3(256), 0, s, v, nc, cellblk, order, depth,
-				   per_cell, boff, csr ? 1 : 0, LA, LB);
+				   per_cell, boff, csr ? 1 : 0, LA);
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 
 int launch_cells_list(bool poly, bool prd, int waves, hipStream_t s, const RectView &v, int nc, const int *cellblk,
-		      const int *order, const int *cnt, const int *boff, const ListEntry *LA, const ListTag *LB,
+		      const int *order, const int *cnt, const int *boff, const ListRec *LA,
 		      int *counters, char *ws, int *fallbacks, int *routed, const CellOut &out) {
 #define VG_LIST(P, R)                                                                                               \
 	hipLaunchKernelGGL((k_cells_list<P, R, 2>), dim3(waves), dim3(64), 0, s, v, nc, cellblk, order, cnt, boff, LA, \
-			   LB, counters, ws, fallbacks, routed, out)
+			   counters, ws, fallbacks, routed, out)
 	if (prd) { if (poly) VG_LIST(true, true); else VG_LIST(false, true); }
 	else { if (poly) VG_LIST(true, false); else VG_LIST(false, false); }
 #undef VG_LIST

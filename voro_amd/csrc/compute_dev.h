@@ -767,27 +767,32 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 }
 
 // ---------------------------------------------------------------------------
-// Candidate lists (the learned path).  A list holds, in the reference's
-// candidate order, every candidate of one cell up to a worklist depth G:
-//   kind 1  a particle of the cell's own block        {dx, dy, dz, r}, pid
-//   kind 2  worklist step g -> g+1, block in the grid  {radp[g], crs_near, crs_far}, particle count
-//   kind 0  a particle of that block                   {dx, dy, dz, r}, pid
-//   kind 3  worklist step g -> g+1, block outside      {radp[g]}
-//   kind 4  end of the list at depth G                 {radp[G]}
-// where (dx, dy, dz) is the particle minus the (image-shifted) cell particle
-// and crs_near / crs_far are the two radii compute_min_max_radius derives
-// for the block (src/v_compute.cc:797-944) before and after its mrs test.
-// Everything in an entry is independent of the cell's shape, so the entries
-// are written by a separate gather pass; the decisions that depend on the
-// shape (count_list refresh, the radius stop, the block skip, the
-// per-particle check) are taken here exactly as compute_cell_rect takes them.
-// A kind-2 tag of a periodic container's list also carries the block
-// (kind = 2 | image << 3 | block << 4): a visit of it is recorded here, in
-// the order and with the key compute_cell_rect records it (the gather must
-// not record visits: the cell may stop before the block).
-struct ListEntry { double x, y, z, w; };
-struct ListTag { int pid, kind; };
-enum { LK_PART = 0, LK_OWN = 1, LK_BLOCK = 2, LK_NOBLOCK = 3, LK_END = 4 };
+// Candidate lists (the learned path).  A list holds one record per worklist
+// step of one cell up to a depth G, in the reference's order
+// (src/v_compute.cc:328-431), then an end record:
+//   LK_BLOCK    step g -> g+1, block rijk in the grid: radp[g], crs_near,
+//               crs_far, the image shift of the block
+//   LK_NOBLOCK  step g -> g+1, block outside the grid: radp[g]
+//   LK_END      the end of the list at depth G: radp[G]
+// crs_near / crs_far are the two radii compute_min_max_radius derives for
+// the block (src/v_compute.cc:797-944) before and after its mrs test.  A
+// record is independent of the cell's shape, so the records are written by a
+// separate gather pass; the particles are read from the block arrays when
+// the block is cut, and the decisions that depend on the shape (count_list
+// refresh, the radius stop, the block skip, the per-particle check) are taken
+// here exactly as compute_cell_rect takes them.  info: the kind (bits 0-2),
+// the block is a periodic container's image block (bit 3; its first visit is
+// recorded here, in the order and with the key compute_cell_rect records it:
+// the gather must not record visits, the cell may stop before the block),
+// and the shift of the block's image per axis (bits 4-5 x, 6-7 y, 8-9 z:
+// 0 none, 1 minus one period, 2 plus one period).
+struct ListRec { double rad, cn, cf; int rijk, info; };
+enum { LK_BLOCK = 2, LK_NOBLOCK = 3, LK_END = 4 };
+
+// The record's image shift per axis code: the values rect_region and
+// prd_region give.
+VG_INL double list_shift(int code, double period) { return code == 1 ? -period : (code == 2 ? period : 0.0); }
+VG_INL int list_code(double q) { return q < 0 ? 1 : (q > 0 ? 2 : 0); }
 
 // compute_cell over a candidate list.  Returns 1 / 0 (the cell exists /
 // was deleted), -1 when the list ends before the radius test stops the
@@ -795,13 +800,12 @@ enum { LK_PART = 0, LK_OWN = 1, LK_BLOCK = 2, LK_NOBLOCK = 3, LK_END = 4 };
 // empty list (no list was gathered: the plain path computes the cell).
 // key0: the cell's loop position << 24 (periodic first-visit keys).
 template<class CELL, bool POLY, bool PRD = false>
-VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, int ci, int cj, int ck,
-			    const ListEntry *__restrict__ LA, const ListTag *__restrict__ LB, int stride, int nent,
-			    CellStats &stt, unsigned long long key0 = 0) {
+VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, const ListRec *__restrict__ L, int stride,
+			    int nrec, CellStats &stt, unsigned long long key0 = 0) {
 	stt.n_planes = 0; stt.wl_exit = 0; stt.phase = 1; stt.img = 0;
-	if (nent <= 0) return -3;
+	if (nrec <= 0) return -3;
 	unsigned long long key = key0;
-	const int b0 = v.bstart[ijk];
+	const int b0 = v.bstart[ijk], co0 = v.bstart[ijk + 1] - b0;
 	const dbl4 me = v.P[b0 + s];
 	const double x = me.x, y = me.y, z = me.z;
 	if (PRD) {
@@ -814,19 +818,20 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, int ci, 
 		if (v.zper) { zb = 0.5 * (v.bz - v.az); za = -zb; } else { za = v.az - z; zb = v.bz - z; }
 		c.init_box(xa, xb, ya, yb, za, zb);
 	}
-	(void)ci; (void)cj; (void)ck;
 	RadPolicy<POLY> rp;
 	rp.init(me.w, v.max_radius);
 	double mrs = 0;
-	int next_count = 3, cpi = 0, g = 0, w = 0, k = 0, left = 0;
+	int next_count = 3, cpi = 0, g = 0, w = 0, t = 0, k = 0;
 	bool own_done = false, bcheck = false;
+	int bt = 0, be = 0;              // particle range of the block being cut
+	double x2 = 0, y2 = 0, z2 = 0;   // the cell's particle, shifted by the block's image
 	// upper bound on max_radius_squared(), as in compute_cell_rect
 	double mrs_b = 1.7976931348623157e308;
 	int since = 0;
-	// lane-parallel particle checks (as in compute_cell_rect): entries
-	// cbase + l of a checked block that are candidates, up to cend
+	// lane-parallel particle checks (as in compute_cell_rect): particles
+	// cbase + l of a checked block that are candidates
 	unsigned long long cm = 0;
-	int cbase = 0, cend = 0;
+	int cbase = 0;
 	for (;;) {
 		double px_ = 0, py_ = 0, pz_ = 0, rs = 0;
 		double rs_raw = -1;
@@ -844,83 +849,103 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, int ci, 
 				if (cm) {
 					const int idx = cbase + __builtin_ctzll(cm);
 					cm &= cm - 1;
-					const ListEntry a = LA[(size_t)idx * stride];
-					px_ = a.x; py_ = a.y; pz_ = a.z; pid = LB[(size_t)idx * stride].pid;
+					const dbl4 o = v.P[idx];
+					pid = v.pid[idx];
+					px_ = o.x - x2; py_ = o.y - y2; pz_ = o.z - z2;
 					rs_raw = px_ * px_ + py_ * py_ + pz_ * pz_;
-					rs = rp.scale_checked(rs_raw, a.w);
+					rs = rp.scale_checked(rs_raw, o.w);
 					have = true;
 					continue;
 				}
-				if (k < cend) {
-					const int cnt = cend - k < 64 ? cend - k : 64;
+				if (bcheck && bt < be) {
+					// up to 64 particles of a checked block at once, one per
+					// lane (src/v_compute.cc:402-428; mrs is constant within a block)
+					const int cnt = be - bt < 64 ? be - bt : 64;
 					const int lane = steps_lane();
 					bool pass = false;
 					if (lane < cnt) {
-						const ListEntry a = LA[(size_t)(k + lane) * stride];
-						double r2 = a.x * a.x + a.y * a.y + a.z * a.z;
-						pass = rp.scale_check(r2, mrs, a.w);
+						const dbl4 o = v.P[bt + lane];
+						const double ax = o.x - x2, ay = o.y - y2, az = o.z - z2;
+						double r2 = ax * ax + ay * ay + az * az;
+						pass = rp.scale_check(r2, mrs, o.w);
 					}
 					cm = steps_ballot(pass);
-					cbase = k;
-					k += cnt;
+					cbase = bt;
+					bt += cnt;
 					continue;
 				}
 			}
-			if (k >= nent) { stt.n_planes = c.n_cuts; return -1; }
-			const ListEntry a = LA[(size_t)k * stride];
-			const ListTag tg = LB[(size_t)k * stride];
-			k++;
-			if (tg.kind == LK_OWN) {
-				px_ = a.x; py_ = a.y; pz_ = a.z; pid = tg.pid;
-				rs_raw = px_ * px_ + py_ * py_ + pz_ * pz_;
-				rs = rp.scale(rs_raw, a.w);
-				have = true;
-				continue;
-			}
-			if (tg.kind == LK_PART) {
-				px_ = a.x; py_ = a.y; pz_ = a.z; pid = tg.pid;
+			if (bt < be) {
+				// next particle of the block (src/v_compute.cc:402-428)
+				const dbl4 o = v.P[bt];
+				pid = v.pid[bt];
+				bt++;
+				px_ = o.x - x2; py_ = o.y - y2; pz_ = o.z - z2;
 				rs = px_ * px_ + py_ * py_ + pz_ * pz_;
 				rs_raw = rs;
-				if (!bcheck) { rs = rp.scale(rs, a.w); have = true; }
-				else have = rp.scale_check(rs, mrs, a.w);
-				if (--left == 0) bcheck = false;
+				if (!bcheck) { rs = rp.scale(rs, o.w); have = true; }
+				else have = rp.scale_check(rs, mrs, o.w);
 				continue;
 			}
-			// a worklist step: the own block is done (src/v_compute.cc:328)
-			if (!own_done) { mrs = c.max_radius_squared(); mrs_b = mrs; since = 0; own_done = true; }
+			if (!own_done) {
+				// own block in storage order, skipping the particle itself
+				// (src/v_compute.cc:307-323)
+				if (t < co0) {
+					if (t == s) { t++; continue; }
+					const dbl4 o = v.P[b0 + t];
+					pid = v.pid[b0 + t];
+					t++;
+					px_ = o.x - x; py_ = o.y - y; pz_ = o.z - z;
+					rs_raw = px_ * px_ + py_ * py_ + pz_ * pz_;
+					rs = rp.scale(rs_raw, o.w);
+					have = true;
+					continue;
+				}
+				// worklist selection (src/v_compute.cc:328)
+				mrs = c.max_radius_squared();
+				mrs_b = mrs; since = 0;
+				own_done = true;
+			}
+			if (k >= nrec) { stt.n_planes = c.n_cuts; return -1; }
+			const ListRec a = L[(size_t)k * stride];
+			k++;
 			if (g < 63 && g == next_count) {
 				mrs = c.max_radius_squared();
 				mrs_b = mrs; since = 0;
-				if (cpi != 8) {
-					next_count = count_list(cpi++);
-				}
+				if (cpi != 8) next_count = count_list(cpi++);
 			}
-			if (rp.ctest(a.x, mrs)) { stt.wl_exit = g; stt.n_planes = c.n_cuts; return 1; }
-			if (tg.kind == LK_END) {
+			if (rp.ctest(a.rad, mrs)) { stt.wl_exit = g; stt.n_planes = c.n_cuts; return 1; }
+			const int kind = a.info & 7;
+			if (kind == LK_END) {
 				stt.wl_exit = g >= 63 ? 64 : g;
 				stt.n_planes = c.n_cuts;
 				return -1;
 			}
 			g++;
-			if (tg.kind == LK_NOBLOCK) continue;
-			const int cnt = tg.pid;
-			if (rp.ctest(a.y, mrs)) { k += cnt; continue; }   // compute_min_max_radius skip
+			if (kind == LK_NOBLOCK) continue;
+			if (rp.ctest(a.cn, mrs)) continue;   // compute_min_max_radius skip
 			if (PRD) {
 				// region_index of the block (src/container_prd.hh:163-168):
 				// every call takes a key, image blocks record the visit
-				if ((tg.kind >> 3) & 1) {
+				if ((a.info >> 3) & 1) {
 					stt.img = 1;
-					if (v.fv) note_first_visit(v.fv + (tg.kind >> 4), key);
+					if (v.fv) note_first_visit(v.fv + a.rijk, key);
 				}
 				key++;
 			}
-			bcheck = rp.ctest(a.z, mrs);
-			left = cnt;
-			if (cnt == 0) bcheck = false;
-			if constexpr (LaneSteps<CELL>::value) {
-				// a checked block's particles go through the lane checks
-				if (bcheck) { cend = k + cnt < nent ? k + cnt : nent; left = 0; bcheck = false; }
+			bcheck = rp.ctest(a.cf, mrs);
+			{
+				double qx, qy = 0, qz = 0;
+				if (PRD) qx = list_shift((a.info >> 4) & 3, v.bx);
+				else {
+					qx = list_shift((a.info >> 4) & 3, v.bx - v.ax);
+					qy = list_shift((a.info >> 6) & 3, v.by - v.ay);
+					qz = list_shift((a.info >> 8) & 3, v.bz - v.az);
+				}
+				x2 = x - qx; y2 = y - qy; z2 = z - qz;
 			}
+			bt = v.bstart[a.rijk];
+			be = v.bstart[a.rijk + 1];
 		}
 		if (rs_raw >= 0) {
 			// the exact skip of a plane that provably misses the cell (see

@@ -1,23 +1,24 @@
 // Gathering of candidate lists for the learned path (see compute_dev.h,
 // "Candidate lists").  One cell's list follows the reference's candidate
-// order (src/v_compute.cc:307-431): its own block in storage order without
-// the particle itself, then worklist steps 0 -> 1 ... G-1 -> G, each with the
-// block's particles in storage order, then the end marker at depth G.
+// order (src/v_compute.cc:328-431): one record per worklist step 0 -> 1 ...
+// G-1 -> G, then the end record at depth G.
 #pragma once
 #include "compute_dev.h"
 
 namespace vgpu {
 
-// Walks the list of stored particle `cell` (block ijk of view v) at depth
-// G.  With FILL false it only counts entries; with FILL true it writes entry
-// k at LA[k * stride], LB[k * stride].  Returns the number of entries; 0
-// for G >= 64 (no list: the cell takes the plain path).
-// PRD: a periodic container's view (unit-cell start, image regions; v.fv
-// must be null here, the list kernel records the visits).
-template<bool FILL, bool PRD = false>
-VG_FN int gather_cell_list(const RectView &v, int ijk, int cell, int G, ListEntry *LA, ListTag *LB, int stride) {
+// The number of records of a list of depth G: 0 for G >= 64 (no list: the
+// cell takes the plain path).
+VG_INL int list_length(int G) { return G >= 64 ? 0 : G + 1; }
+
+// Writes the list of stored particle `cell` (block ijk of view v) at depth
+// G, record k at L[k * stride]; returns its length.  PRD: a periodic
+// container's view (unit-cell start, image regions; v.fv must be null here,
+// the list kernel records the visits).
+template<bool PRD = false>
+VG_FN int gather_cell_list(const RectView &v, int ijk, int cell, int G, ListRec *L, int stride) {
 	if (G >= 64) return 0;
-	const int b0 = v.bstart[ijk], co0 = v.bstart[ijk + 1] - b0, s = cell - v.cstart[ijk];
+	const int b0 = v.bstart[ijk], s = cell - v.cstart[ijk];
 	const dbl4 me = v.P[b0 + s];
 	const double x = me.x, y = me.y, z = me.z;
 	const int ci = ijk % v.nx, cj = (ijk / v.nx) % v.gy, ck = ijk / v.nxy;
@@ -26,17 +27,6 @@ VG_FN int gather_cell_list(const RectView &v, int ijk, int cell, int G, ListEntr
 	else {
 		i = v.xper ? v.nx : ci; j = v.yper ? v.ny : cj; k = v.zper ? v.nz : ck;
 		disp = ijk - i - v.nx * (j + v.ny * k);
-	}
-	int n = 0;
-	// own block (src/v_compute.cc:307-323)
-	for (int t = 0; t < co0; t++) {
-		if (t == s) continue;
-		if (FILL) {
-			const dbl4 o = v.P[b0 + t];
-			LA[(size_t)n * stride] = ListEntry{o.x - x, o.y - y, o.z - z, o.w};
-			LB[(size_t)n * stride] = ListTag{v.pid[b0 + t], LK_OWN};
-		}
-		n++;
 	}
 	// worklist selection (src/v_compute.cc:328-362)
 	double fx, fy, fz;
@@ -69,38 +59,21 @@ VG_FN int gather_cell_list(const RectView &v, int ijk, int cell, int G, ListEntr
 		int ddk = (q >> 14) & 127; ddk -= 64;
 		const int ei = ddi + i, ej = ddj + j, ek = ddk + k;
 		if (ei < 0 || ei >= v.hx || ej < 0 || ej >= v.hy || ek < 0 || ek >= v.hz) {
-			if (FILL) {
-				LA[(size_t)n * stride] = ListEntry{radp[g], 0, 0, 0};
-				LB[(size_t)n * stride] = ListTag{0, LK_NOBLOCK};
-			}
-			n++;
+			L[(size_t)g * stride] = ListRec{radp[g], 0, 0, 0, LK_NOBLOCK};
 			continue;
 		}
 		double qx = 0, qy = 0, qz = 0;
 		int img = 0;
 		const int rijk = PRD ? prd_region(v, ci, cj, ck, ei, ej, ek, qx, 0, img)
 				     : rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
-		const int r0 = v.bstart[rijk], cnt = v.bstart[rijk + 1] - r0;
-		if (FILL) {
-			double cn, cf;
-			int form;
-			min_max_radius_pair(v, ddi, ddj, ddk, fx, fy, fz, gxs, gys, gzs, cn, cf, form);
-			LA[(size_t)n * stride] = ListEntry{radp[g], cn, cf, 0};
-			LB[(size_t)n * stride] = ListTag{cnt, PRD ? (LK_BLOCK | img << 3 | rijk << 4) : LK_BLOCK};
-			const double x2 = x - qx, y2 = y - qy, z2 = z - qz;
-			for (int t = 0; t < cnt; t++) {
-				const dbl4 o = v.P[r0 + t];
-				LA[(size_t)(n + 1 + t) * stride] = ListEntry{o.x - x2, o.y - y2, o.z - z2, o.w};
-				LB[(size_t)(n + 1 + t) * stride] = ListTag{v.pid[r0 + t], LK_PART};
-			}
-		}
-		n += 1 + cnt;
+		double cn, cf;
+		int form;
+		min_max_radius_pair(v, ddi, ddj, ddk, fx, fy, fz, gxs, gys, gzs, cn, cf, form);
+		const int info = LK_BLOCK | img << 3 | list_code(qx) << 4 | list_code(qy) << 6 | list_code(qz) << 8;
+		L[(size_t)g * stride] = ListRec{radp[g], cn, cf, rijk, info};
 	}
-	if (FILL) {
-		LA[(size_t)n * stride] = ListEntry{radp[G], 0, 0, 0};
-		LB[(size_t)n * stride] = ListTag{0, LK_END};
-	}
-	return n + 1;
+	L[(size_t)G * stride] = ListRec{radp[G], 0, 0, 0, LK_END};
+	return G + 1;
 }
 
 }  // namespace vgpu

@@ -117,7 +117,7 @@ struct vgpu_container {
 	int pred_with_r = 0;
 	float pred_margin = 0;
 	int pred_route = 64;
-	DBuf d_pred, d_feat, d_cfeat, d_depth, d_cost, d_order, d_hist, lcnt, bmax, boff, list_a, list_b;
+	DBuf d_pred, d_feat, d_cfeat, d_depth, d_cost, d_order, d_hist, lcnt, bmax, boff, list_a;
 	long long last_fallbacks = 0, last_entries = 0, last_routed = 0;
 	Workspace ws_fast, ws_wide, ws_huge, ws_wave;
 	long long nbr_total = 0;
@@ -375,9 +375,9 @@ int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut
 		ent = (size_t)std::max(rows, 1) * 64;
 		h->last_entries = (long long)rows * 64;
 	}
-	if ((st = h->list_a.ensure(ent * sizeof(ListEntry))) || (st = h->list_b.ensure(ent * sizeof(ListTag)))) return st;
+	if ((st = h->list_a.ensure(ent * sizeof(ListRec)))) return st;
 	if (launch_list_fill(h->prd, s, vg, nc, h->cellblk.as<int>(), h->d_order.as<int>(), h->d_depth.as<unsigned char>(),
-			     h->pred_with_r, h->boff.as<int>(), csr, h->list_a.as<ListEntry>(), h->list_b.as<ListTag>()))
+			     h->pred_with_r, h->boff.as<int>(), csr, h->list_a.as<ListRec>()))
 		return VGPU_HIP_ERROR;
 	if (!h->ev_list) HIPCHK(hipEventCreate(&h->ev_list));
 	HIPCHK(hipEventRecord(h->ev_list, s));
@@ -388,17 +388,22 @@ int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut
 		int w = h->wave_waves > 0 ? h->wave_waves : default_fast_waves(4);
 		const int chunk = nc >= 4 * w ? 4 : 1;
 		w = std::max(1, std::min(w, (nc + chunk - 1) / chunk));
-		if ((st = h->ws_wave.ensure(WLayout::bytes, w))) return st;
+		Workspace &ws = h->ws_wave;
+		if ((st = ws.ensure(WLayout::bytes, w))) return st;
+		if ((unsigned long long)ws.stamp_base + (unsigned long long)nc + 2 >= 0xffffffffull) {
+			HIPCHK(hipMemset(ws.buf.p, 0, ws.buf.n));
+			ws.stamp_base = 0;
+		}
 		if (launch_cells_wlist(h->poly, h->prd, w, chunk, s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(),
-				       h->lcnt.as<int>(), h->boff.as<int>(), h->list_a.as<ListEntry>(),
-				       h->list_b.as<ListTag>(), h->d_counters.as<int>(), h->ws_wave.base(), ctr + 5, ctr + 9,
-				       out))
+				       h->lcnt.as<int>(), h->boff.as<int>(), h->list_a.as<ListRec>(),
+				       h->d_counters.as<int>(), ws.base(), ws.stamp_base, ctr + 5, ctr + 9, out))
 			return VGPU_HIP_ERROR;
+		ws.stamp_base += (unsigned)nc + 1u;
 	} else {
 		if ((st = h->ws_fast.ensure(VCell3::L::bytes, waves))) return st;
 		if (launch_cells_list(h->poly, h->prd, waves, s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(),
-				      h->lcnt.as<int>(), h->boff.as<int>(), h->list_a.as<ListEntry>(),
-				      h->list_b.as<ListTag>(), h->d_counters.as<int>(), h->ws_fast.base(), ctr + 5, ctr + 9,
+				      h->lcnt.as<int>(), h->boff.as<int>(), h->list_a.as<ListRec>(),
+				      h->d_counters.as<int>(), h->ws_fast.base(), ctr + 5, ctr + 9,
 				      out))
 			return VGPU_HIP_ERROR;
 	}
@@ -408,8 +413,10 @@ int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut
 // The plain tiers over every cell (list == nullptr) or over a device list
 // of cells whose count is at *list_count: order-3, then wide, then huge
 // over each tier's overflow.
+// from2: the list's cells overflowed the 64-vertex wavefront cell already
+// (the learned tier's leftovers): start with the 128-vertex one.
 int run_tiers(vgpu_container *h, const RectView &v, int nc, int fw, const int *list, const int *list_count,
-	      CellOut out, hipStream_t s, bool mark_ev2, int tier0 = 0) {
+	      CellOut out, hipStream_t s, bool mark_ev2, int tier0 = 0, bool from2 = false) {
 	int st;
 	int *ctr = h->ctr.as<int>();
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr + 1, 2, 0);
@@ -418,11 +425,17 @@ int run_tiers(vgpu_container *h, const RectView &v, int nc, int fw, const int *l
 		// ctr[6]) to the same with up to 128, theirs (ctr[7]) up to 254, and
 		// theirs (ctr[8]) to order-3 per lane
 		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr + 6, 3, 0);
-		out.tier = tier0; out.retry = h->retry3.as<int>(); out.retry_count = ctr + 6;
-		if ((st = launch_wave(h, 1, v, nc, list, list_count, out, s))) return st;
-		if (mark_ev2) HIPCHK(hipEventRecord(h->ev[2], s));
-		out.tier = 1; out.retry = h->retry4.as<int>(); out.retry_count = ctr + 7;
-		if ((st = launch_wave(h, 2, v, nc, h->retry3.as<int>(), ctr + 6, out, s))) return st;
+		if (from2) {
+			out.tier = tier0; out.retry = h->retry4.as<int>(); out.retry_count = ctr + 7;
+			if ((st = launch_wave(h, 2, v, nc, list, list_count, out, s))) return st;
+			if (mark_ev2) HIPCHK(hipEventRecord(h->ev[2], s));
+		} else {
+			out.tier = tier0; out.retry = h->retry3.as<int>(); out.retry_count = ctr + 6;
+			if ((st = launch_wave(h, 1, v, nc, list, list_count, out, s))) return st;
+			if (mark_ev2) HIPCHK(hipEventRecord(h->ev[2], s));
+			out.tier = 1; out.retry = h->retry4.as<int>(); out.retry_count = ctr + 7;
+			if ((st = launch_wave(h, 2, v, nc, h->retry3.as<int>(), ctr + 6, out, s))) return st;
+		}
 		out.retry = h->retry5.as<int>(); out.retry_count = ctr + 8;
 		if ((st = launch_wave(h, 4, v, nc, h->retry4.as<int>(), ctr + 7, out, s))) return st;
 		out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
@@ -518,7 +531,9 @@ int run_prd_cells(vgpu_container *h, RectView &v, CellOut out, int nc, int fw, h
 	if (learned) {
 		if ((st = run_learned(h, v, nc, fw, out, s))) return st;
 		HIPCHK(hipEventRecord(h->ev[2], s));
-		if ((st = run_tiers(h, v, nc, fw, h->retry0.as<int>(), h->ctr.as<int>() + 4, out, s, false, 1))) return st;
+		if ((st = run_tiers(h, v, nc, fw, h->retry0.as<int>(), h->ctr.as<int>() + 4, out, s, false, 1,
+				    h->first_tier == 0)))
+			return st;
 	} else if ((st = run_tiers(h, v, nc, fw, nullptr, nullptr, out, s, true))) return st;
 	std::vector<unsigned long long> prev(nb), cur(nb);
 	std::vector<unsigned char> t(nc), touched(nc, 0);
@@ -745,7 +760,8 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s, bool bin_only
 		// cells it does not finish (retry0, count ctr[4]) take the plain tiers
 		if ((st = run_learned(h, v, nc, fw, out, s))) return st;
 		HIPCHK(hipEventRecord(h->ev[2], s));
-		if ((st = run_tiers(h, v, nc, fw, h->retry0.as<int>(), ctr + 4, out, s, false, 1))) return st;
+		if ((st = run_tiers(h, v, nc, fw, h->retry0.as<int>(), ctr + 4, out, s, false, 1, h->first_tier == 0)))
+			return st;
 	} else if (nc > 0) {
 		// the plain tiers: wavefront cells (or order-3 per lane), wide, huge
 		if ((st = run_tiers(h, v, nc, fw, nullptr, nullptr, out, s, true))) return st;
@@ -835,7 +851,7 @@ struct vgpu_batch {
 	// learned tier: predictor (per block), its outputs, the schedule, the
 	// candidate lists (capacity lcap entries, grown after a compute that
 	// needed more), the container of every block
-	DBuf d_pred, d_feat, d_depth, d_cost, d_hist, d_order, lcnt, loff, list_a, list_b, bcont, retry0, ltop;
+	DBuf d_pred, d_feat, d_depth, d_cost, d_hist, d_order, lcnt, loff, list_a, bcont, ltop;
 	bool have_pred = false, bcont_ok = false;
 	float pred_margin = 0;
 	int pred_route = 64;
@@ -918,15 +934,15 @@ int batch_prepare(vgpu_batch *b, unsigned flags) {
 	}
 	if (learned) {
 		using namespace vgpu::learn;
-		// about 30 entries per cell at the trained depth on random points
-		if (b->lcap <= 0) b->lcap = std::min<long long>(2147483000LL, 40LL * (long long)N1 + 4096);
-		if (b->list_a.n < (size_t)b->lcap * sizeof(ListEntry) || b->list_b.n < (size_t)b->lcap * sizeof(ListTag))
+		// a list holds depth + 1 <= 64 records; trained depths are about 40
+		// worklist steps on random and lattice points
+		if (b->lcap <= 0) b->lcap = std::min<long long>(2147483000LL, 48LL * (long long)N1 + 4096);
+		if (b->list_a.n < (size_t)b->lcap * sizeof(ListRec))
 			b->drop_graph();   // the list buffers move
 		if ((st = b->d_feat.ensure(NB1 * IN * 4)) || (st = b->d_depth.ensure(NB1)) || (st = b->d_cost.ensure(NB1)) ||
 		    (st = b->d_hist.ensure(64 * 4)) || (st = b->d_order.ensure(N1 * 4)) || (st = b->lcnt.ensure(N1 * 4)) ||
-		    (st = b->loff.ensure(N1 * 4)) || (st = b->retry0.ensure(N1 * 4)) || (st = b->ltop.ensure(8)) ||
-		    (st = b->list_a.ensure((size_t)b->lcap * sizeof(ListEntry))) ||
-		    (st = b->list_b.ensure((size_t)b->lcap * sizeof(ListTag))))
+		    (st = b->loff.ensure(N1 * 4)) || (st = b->ltop.ensure(8)) ||
+		    (st = b->list_a.ensure((size_t)b->lcap * sizeof(ListRec))))
 			return st;
 	}
 	const bool wave = b->occ == 0 || learned;
@@ -1007,9 +1023,9 @@ int batch_enqueue(vgpu_batch *b, unsigned flags, hipStream_t s, bool timing) {
 	if (flags & VGPU_F_LEARNED) {
 		// the learned tier: block features, predicted depths (and costs, the
 		// schedule's order), the candidate lists, the list-driven wavefront
-		// cells; their leftovers (retry0, count ctr[11]; ctr[12] lists that
-		// ended first, ctr[13] cells given no list) take wavefront cells,
-		// theirs (retry1) the per-lane order-3 tier, and so on
+		// cells (cells whose list ends first, ctr[12], or that have none,
+		// ctr[13], continue there as plain wavefront cells); overflows
+		// (retry1) take the per-lane order-3 tier, and so on
 		using namespace vgpu::learn;
 		const int *ncp = b->bstart.as<int>() + nb;
 		if (launch_batch_features(s, nb, b->bcont.as<int>(), b->views.as<RectView>(), b->d_feat.as<float>()))
@@ -1025,20 +1041,15 @@ int batch_enqueue(vgpu_batch *b, unsigned flags, hipStream_t s, bool timing) {
 		if (launch_batch_list_build(s, b->views.as<RectView>(), b->bcont.as<int>(), n, ncp, b->cellblk.as<int>(),
 					    b->d_order.as<int>(), b->d_depth.as<unsigned char>(), b->lcap,
 					    b->ltop.as<unsigned long long>(), b->lcnt.as<int>(), b->loff.as<int>(),
-					    b->list_a.as<ListEntry>(), b->list_b.as<ListTag>()))
+					    b->list_a.as<ListRec>()))
 			return VGPU_HIP_ERROR;
-		out.tier = 0; out.retry = b->retry0.as<int>(); out.retry_count = ctr + 11;
+		// overflows (more than 64 vertices) to the per-lane order-3 tier
+		out.tier = 0; out.retry = b->retry1.as<int>(); out.retry_count = ctr + 1;
 		const int chunk = n >= 4 * b->waves ? 4 : 1;
 		if (launch_batch_wcells_list(b->poly, b->waves, chunk, s, b->views.as<RectView>(), b->bcont.as<int>(), ncp,
 					     b->cellblk.as<int>(), b->d_order.as<int>(), b->lcnt.as<int>(), b->loff.as<int>(),
-					     b->list_a.as<ListEntry>(), b->list_b.as<ListTag>(), cnt8, b->ws_wave.base(),
-					     b->ws_wave.slab, ctr + 12, ctr + 13, out))
-			return VGPU_HIP_ERROR;
-		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, cnt8, 8, 0);
-		out.tier = 1; out.retry = b->retry1.as<int>(); out.retry_count = ctr + 1;
-		if (launch_batch_wlist(b->poly, b->waves, s, b->views.as<RectView>(), b->bcont.as<int>(), b->retry0.as<int>(),
-				       ctr + 11, b->cellblk.as<int>(), cnt8, b->ws_wave.base(), b->ws_wave.slab,
-				       (const unsigned *)(ctr + 8), out))
+					     b->list_a.as<ListRec>(), cnt8, b->ws_wave.base(), b->ws_wave.slab,
+					     (const unsigned *)(ctr + 8), ctr + 12, ctr + 13, out))
 			return VGPU_HIP_ERROR;
 		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, cnt8, 8, 0);
 		out.tier = 1; out.retry = b->retry3.as<int>(); out.retry_count = ctr + 5;
@@ -1237,7 +1248,7 @@ int vgpu_destroy(vgpu_container *h) {
 			&h->d_counters, &h->vol, &h->cent, &h->nfaces, &h->status, &h->nbr_start, &h->arena, &h->csr_off,
 			&h->csr, &h->ctr, &h->retry0, &h->retry1, &h->retry2, &h->st_planes, &h->st_exit, &h->st_tier,
 			&h->d_pred, &h->d_feat, &h->d_cfeat, &h->d_depth, &h->d_cost, &h->d_order, &h->d_hist, &h->lcnt,
-			&h->bmax, &h->boff, &h->list_a, &h->list_b, &h->d_unit, &h->Pall, &h->pidall, &h->ball, &h->fv,
+			&h->bmax, &h->boff, &h->list_a, &h->d_unit, &h->Pall, &h->pidall, &h->ball, &h->fv,
 			&h->touch, &h->d_flag, &h->retry3, &h->retry4, &h->retry5, &h->img_p, &h->img_id,
 			&h->img_reg, &h->img_rank, &h->img_bc, &h->img_cnt, &h->fq_xyz, &h->fq_found, &h->fq_pid,
 			&h->fq_pos, &h->fq_retry, &h->fq_retry2, &h->fq_r, &h->fq_vol, &h->fq_nbr};
@@ -2144,7 +2155,7 @@ int vgpu_batch_destroy(vgpu_batch *b) {
 			&b->src, &b->cellblk, &b->vol, &b->cent, &b->nfaces, &b->status, &b->nbr_start, &b->arena,
 			&b->csr_off, &b->csr, &b->ctr, &b->counters, &b->retry1, &b->retry2, &b->retry3, &b->st_planes,
 			&b->st_exit, &b->st_tier, &b->d_pred, &b->d_feat, &b->d_depth, &b->d_cost, &b->d_hist,
-			&b->d_order, &b->lcnt, &b->loff, &b->list_a, &b->list_b, &b->bcont, &b->retry0, &b->ltop};
+			&b->d_order, &b->lcnt, &b->loff, &b->list_a, &b->bcont, &b->ltop};
 	for (DBuf *x : bufs) x->release();
 	b->ws_fast.buf.release(); b->ws_wide.buf.release(); b->ws_huge.buf.release(); b->ws_wave.buf.release();
 	if (b->events) for (int t = 0; t < 4; t++) (void)hipEventDestroy(b->ev[t]);
@@ -2187,7 +2198,6 @@ int vgpu_batch_list_capacity(vgpu_batch *b, long long *get, const long long *set
 		b->drop_graph();
 		b->lcap = *set;
 		b->list_a.release();
-		b->list_b.release();
 	}
 	if (get) *get = b->lcap;
 	return VGPU_OK;

@@ -220,16 +220,18 @@ int launch_list_order(hipStream_t s, int nc, const int *cellblk, const unsigned 
 int launch_list_gather(bool prd, hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
 		       const unsigned char *depth, int per_cell, int *cnt, int *bmax);
 int launch_list_fill(bool prd, hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
-		     const unsigned char *depth, int per_cell, const int *boff, bool csr, ListEntry *LA, ListTag *LB);
+		     const unsigned char *depth, int per_cell, const int *boff, bool csr, ListRec *LA);
 int launch_cells_list(bool poly, bool prd, int waves, hipStream_t s, const RectView &v, int nc, const int *cellblk,
-		      const int *order, const int *cnt, const int *boff, const ListEntry *LA, const ListTag *LB,
+		      const int *order, const int *cnt, const int *boff, const ListRec *LA,
 		      int *counters, char *ws, int *fallbacks, int *routed, const CellOut &out);
 // The list-driven tier with one cell per wavefront (4 waves per SIMD) over
 // lists filled with csr, cells taken `chunk` at a time; ws holds one
-// wavefront-cell slab per wavefront.
+// wavefront-cell slab per wavefront.  Cells without a usable list are
+// computed by the plain algorithm in place (block-mask stamps from
+// stamp_base + cell + 1); only overflows go to out's retry list.
 int launch_cells_wlist(bool poly, bool prd, int waves, int chunk, hipStream_t s, const RectView &v, int nc,
-		       const int *cellblk, const int *order, const int *cnt, const int *boff, const ListEntry *LA,
-		       const ListTag *LB, int *counters, char *ws, int *fallbacks, int *routed, const CellOut &out);
+		       const int *cellblk, const int *order, const int *cnt, const int *boff, const ListRec *LA,
+		       int *counters, char *ws, unsigned stamp_base, int *fallbacks, int *routed, const CellOut &out);
 
 #define VGPU_CELLS_LAUNCH(TIER, P, O)                                                                      \
 	hipLaunchKernelGGL((k_cells<TIER, P, O>), dim3(waves), dim3(64), 0, s, v, ncells, list, list_count,    \
