@@ -408,14 +408,48 @@ def batch(out, flags, steps):
     setup = time.perf_counter() - t0
     tune = voro_amd.container_batch(cons[:256]).autotune(flags)
     b.set_config(0, tune["occupancy"], tune["chunk"])
-    b.compute(flags, graph=True)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        b.compute(flags, graph=True)
-    dt = (time.perf_counter() - t0) / steps
+
+    def timed(fl):
+        b.compute(fl, graph=True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.compute(fl, graph=True)
+        return (time.perf_counter() - t0) / steps
+
+    dt = timed(flags)
     n = sum(len(s[2]) for s in specs)
+    sample = list(range(0, len(specs), 97))
+    ref = [b.results(i)["vol"].copy() for i in sample]
+    ref_nb = [b.results(i)["nbr"].copy() for i in sample]
+    # the learned path: a block predictor trained on another draw of the
+    # workload (512 containers, seed 12), the batch's statistics as samples
+    t0 = time.perf_counter()
+    tcons = []
+    for L, g, pts in I.fcc_batch(12, 512):
+        c = voro_amd.container(0, L, 0, L, 0, L, g, g, g, False, False, False, 8)
+        c.add_wall(voro_amd.wall_sphere(L / 2, L / 2, L / 2, 0.45 * L, -99))
+        c.put_array(np.arange(len(pts), dtype=np.int32), pts)
+        tcons.append(c)
+    tb = voro_amd.container_batch(tcons)
+    tb.compute(flags | voro_amd.F_STATS)
+    pred = voro_amd.predictor()
+    pred.add_samples(tb)
+    del tb, tcons
+    loss = pred.train(epochs=300, lr=3e-3, seed=1, fallback=0.02)
+    t_train = time.perf_counter() - t0
+    b.set_predictor(pred)
+    dtl = timed(flags | voro_amd.F_LEARNED)
+    ls = b.learned_stats()
+    same = all(np.array_equal(b.results(i)["vol"], r) and np.array_equal(b.results(i)["nbr"], q)
+               for i, r, q in zip(sample, ref, ref_nb))
     out["batch_4096"] = {"cells_per_s": n / dt, "ms": dt * 1e3, "containers": len(specs), "particles": n,
-                         "graph": True, "autotuned": tune, "setup_s": setup}
+                         "graph": True, "autotuned": tune, "setup_s": setup,
+                         "learned": {"cells_per_s": n / dtl, "ms": dtl * 1e3, "speedup_vs_plain": dt / dtl,
+                                     "graph": True, "fallbacks": ls["fallbacks"], "fallback_rate": ls["fallbacks"] / n,
+                                     "given_no_list": ls["routed"], "list_records": ls["list_entries"],
+                                     "train": "512 containers of seed 12", "train_seconds": t_train,
+                                     "train_loss": loss,
+                                     "identical_to_plain_on_sampled_containers": bool(same)}}
     return specs
 
 

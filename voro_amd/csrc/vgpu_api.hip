@@ -861,6 +861,7 @@ struct vgpu_batch {
 	// tier, its kind (0 wavefront cells, 2..4 one cell per lane at that many
 	// waves per SIMD) and the cells per queue entry
 	int waves = 0, occ = 2, chunk = 64;
+	int waves_req = 0;               // persistent wavefronts asked for (0: as many as fit); waves: this compute's
 	unsigned long long stamp_used = 0;   // stamps handed out since the workspaces were cleared
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
@@ -946,7 +947,7 @@ int batch_prepare(vgpu_batch *b, unsigned flags) {
 			return st;
 	}
 	const bool wave = b->occ == 0 || learned;
-	int w = b->waves > 0 ? b->waves : default_fast_waves(wave ? 4 : b->occ);
+	int w = b->waves_req > 0 ? b->waves_req : default_fast_waves(wave ? 4 : b->occ);
 	w = std::max(1, std::min(w, (wave ? nblk((long long)N1, b->chunk) : nblk((long long)N1, 64)) + b->ncont));
 	b->waves = w;
 	typedef Layout<TierWide, 64> LW;
@@ -2288,7 +2289,7 @@ int vgpu_batch_config(vgpu_batch *b, int *get3, const int *set3) {
 	if (set3) {
 		if (set3[0] < 0 || set3[1] == 1 || set3[1] < 0 || set3[1] > 4 || set3[2] < 1 || set3[2] > 64)
 			return VGPU_INVALID_ARGUMENT;
-		b->waves = set3[0]; b->occ = set3[1]; b->chunk = set3[2];
+		b->waves_req = set3[0]; b->occ = set3[1]; b->chunk = set3[2];
 		b->drop_graph();
 	}
 	return VGPU_OK;
