@@ -150,6 +150,7 @@ struct WCellT {
 	double vtv[K];           // value for the current plane
 	unsigned vds, vdm;       // stack entries 4 * lane .. 4 * lane + 3
 	int nbv[K];              // output staging: neighbour id number 64 h + `lane`
+	unsigned vrec;           // one vertex per lane: the facet records of create_facet1
 	int lane;
 	__amdgpu_buffer_rsrc_t rs;
 #else
@@ -159,6 +160,7 @@ struct WCellT {
 	double vtv[MAXV];
 	unsigned char vds[MAXS], vdm[MAXS];
 	int nbv[MAXV];
+	unsigned vrec[MAXV];
 	char *wsb;
 #endif
 
@@ -171,6 +173,9 @@ struct WCellT {
 	WMask<K> delm;               // deletion marks
 	WMask<K> flipm;              // tie-search marks (the reference negates back pointers)
 	WMask<K> vis0, vis1, vis2;   // face-walk marks per edge slot
+	// one register set: the live vertices above the plane (value > tol),
+	// below it (< -tol), and below it but within big_tol (plane_values)
+	unsigned long long posm, negm, nearm;
 
 	// Value (never lvalue) selection: a conditional over members would make
 	// the compiler address the cell in memory.
@@ -222,6 +227,13 @@ struct WCellT {
 		vdm = lane == (i >> 2) ? w : vdm;
 	}
 	VG_INL void put_nb(int i, int v) { putk<K>(nbv, i >> 6, lane == (i & 63), v); }
+	VG_INL void put_rec(int j, unsigned r) { vrec = lane == j ? r : vrec; }
+	static VG_INL double bperm64(double a, int addr) {
+		const unsigned long long b = __builtin_bit_cast(unsigned long long, a);
+		const unsigned lo = (unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)(unsigned)b);
+		const unsigned hi = (unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)(unsigned)(b >> 32));
+		return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+	}
 	// neighbour id number h * 64 + `lane`
 	VG_INL int nb_lane(int h = 0) const { return pickk<K>(nbv, h); }
 	// vertex-parallel: bit l set for live vertices h * 64 + l whose predicate holds
@@ -245,6 +257,7 @@ struct WCellT {
 	VG_INL int DM(int i) const { return vdm[i]; }
 	VG_INL void setDM(int i, int v) { vdm[i] = (unsigned char)v; }
 	VG_INL void put_nb(int i, int v) { nbv[i] = v; }
+	VG_INL void put_rec(int j, unsigned r) { vrec[j] = r; }
 	VG_INL int nb_lane(int h = 0) const { return nbv[64 * h]; }
 #endif
 
@@ -420,18 +433,30 @@ struct WCellT {
 				a += vy[h] * py;
 				a += vz[h] * pz - prsq;
 				vtv[h] = a;
-				any = any || live_ballot(!(a < -tol), h) != 0;
+				if constexpr (K == 1) {
+					posm = live_ballot(a > tol);
+					negm = live_ballot(a < -tol);
+					nearm = live_ballot(a < -tol && a > -big_tol);
+					const unsigned long long live = p >= 64 ? ~0ull : (1ull << p) - 1;
+					any = (live & ~negm) != 0;
+				} else any = any || live_ballot(!(a < -tol), h) != 0;
 			}
 		});
 		return any;
 #else
 		bool any = false;
+		posm = negm = nearm = 0;
 		for (int v = 0; v < MAXV; v++) {
 			double a = vx[v] * px;
 			a += vy[v] * py;
 			a += vz[v] * pz - prsq;
 			vtv[v] = a;
 			if (v < p && !(a < -tol)) any = true;
+			if (K == 1 && v < p) {
+				if (a > tol) posm |= 1ull << v;
+				if (a < -tol) negm |= 1ull << v;
+				if (a < -tol && a > -big_tol) nearm |= 1ull << v;
+			}
 		}
 		return any;
 #endif
@@ -661,6 +686,155 @@ struct WCellT {
 		return false;
 	}
 
+	// create_facet for one register set, in two passes.  The walk around
+	// the cut (the reference's loop of src/cell.cc:875-1364, in the same
+	// order) classifies vertices by the masks of plane_values and only
+	// records, per new vertex, its cut edge (positive vertex and slot,
+	// negative vertex and slot) in lane `new vertex - p` of vrec; the cut
+	// slot of the positive vertex takes the new vertex's index (the
+	// reference writes -1 there; both stop the deletion search, which treats
+	// indices >= the old vertex count as cut).  A vector pass then builds
+	// every new vertex at once -- coordinates by the reference's
+	// interpolation, edges to its neighbours on the facet and to the
+	// negative vertex, face ids -- and points the negative vertices' cut
+	// slots at them.  The result is the state the reference's vertex-by-vertex
+	// construction leaves.  A cut edge the walk did not reach (the cell is not
+	// convex to working precision) sets ovf.  Returns true if the cell was
+	// deleted (or on overflow).
+	VG_FN bool create_facet1(int lp, int ls, int us, int pid) {
+		static_assert(K == 1, "one register set");
+		if (!room_v(p)) return true;
+		if (!push_ds(up)) return true;
+		const int p0 = p;
+		unsigned w = TW(up);
+		put_rec(0, (unsigned)up | (unsigned)us << 8 | (unsigned)lp << 16 | (unsigned)ls << 24);
+		w = twSetE(w, us, p0);
+		setTW(up, w);
+		int m = 1, qp = up, qs = cyc_up(us), guard = 0;
+		while (qp != up || qs != us) {
+			if (++guard > 16 * MAXV) { fatal = ovf = true; return true; }
+			const int n = (int)((w >> (8 * qs)) & 0xffu);
+			if (n >= p0) { fatal = ovf = true; return true; }   // -1, or a slot already cut
+			const unsigned long long b = 1ull << n;
+			if (posm & b) {
+				qs = cyc_up(twR(w, qs));
+				qp = n;
+				w = TW(qp);
+				if (!push_ds(qp)) return true;
+			} else if (negm & b) {
+				if (nearm & b) { ovf = true; return true; }   // a kept vertex within big_tol
+				if (!room_v(p0 + m)) return true;
+				put_rec(m, (unsigned)qp | (unsigned)qs << 8 | (unsigned)n << 16 | (unsigned)twR(w, qs) << 24);
+				w = twSetE(w, qs, p0 + m);
+				setTW(qp, w);
+				m++;
+				qs = cyc_up(qs);
+			} else {
+				ovf = true;      // a vertex on the plane: marginal case
+				return true;
+			}
+		}
+		p = p0 + m;
+		build_facet(p0, m, pid);
+		if (!link_cut(p0, m)) { ovf = true; return true; }
+		return false;
+	}
+
+	// The new vertices p0 .. p0+m-1 from their records (see create_facet1):
+	// new vertex j lies on the cut edge of record j, between the facet's
+	// vertices j-1 and j+1 (relations 2 and 0) and joined to the negative
+	// vertex (its relation the negative vertex's slot).
+	VG_FN void build_facet(int p0, int m, int pid) {
+#if VG_WDEV
+		const int j = lane - p0;
+		const bool act = j >= 0 && j < m;
+		const unsigned r = bperm(vrec, (act ? j : 0) << 2);
+		const int qp = (int)(r & 0xffu), qs = (int)((r >> 8) & 3u), lp = (int)((r >> 16) & 0xffu),
+			  ls = (int)(r >> 24);
+		const int aq = qp << 2, al = lp << 2;
+		const double qx = bperm64(vx[0], aq), qy = bperm64(vy[0], aq), qz = bperm64(vz[0], aq);
+		const double qv = bperm64(vtv[0], aq);
+		const double lx = bperm64(vx[0], al), ly = bperm64(vy[0], al), lz = bperm64(vz[0], al);
+		const double lv = bperm64(vtv[0], al);
+		const int na = (int)bperm((unsigned)vna[0], aq), nb = (int)bperm((unsigned)vnb[0], aq),
+			  nc = (int)bperm((unsigned)vnc[0], aq);
+		const int la = (int)bperm((unsigned)vna[0], al), lb = (int)bperm((unsigned)vnb[0], al),
+			  lc = (int)bperm((unsigned)vnc[0], al);
+		if (act) {
+			const double rr = qv / (qv - lv), l1 = 1 - rr;
+			vx[0] = lx * rr + qx * l1;
+			vy[0] = ly * rr + qy * l1;
+			vz[0] = lz * rr + qz * l1;
+			const int prv = p0 + (j == 0 ? m - 1 : j - 1), nxt = p0 + (j + 1 == m ? 0 : j + 1);
+			vt[0] = (unsigned)prv | (unsigned)lp << 8 | (unsigned)nxt << 16 | (2u | (unsigned)ls << 2) << 24;
+			vna[0] = pid;
+			vnb[0] = sel3(qs, na, nb, nc);
+			vnc[0] = sel3(ls, la, lb, lc);
+		}
+#else
+		for (int j = 0; j < m; j++) {
+			const unsigned r = vrec[j];
+			const int qp = (int)(r & 0xffu), qs = (int)((r >> 8) & 3u), lp = (int)((r >> 16) & 0xffu),
+				  ls = (int)(r >> 24);
+			const double qv = vtv[qp], lv = vtv[lp];
+			const double rr = qv / (qv - lv), l1 = 1 - rr;
+			const int v = p0 + j;
+			vx[v] = vx[lp] * rr + vx[qp] * l1;
+			vy[v] = vy[lp] * rr + vy[qp] * l1;
+			vz[v] = vz[lp] * rr + vz[qp] * l1;
+			const int prv = p0 + (j == 0 ? m - 1 : j - 1), nxt = p0 + (j + 1 == m ? 0 : j + 1);
+			vt[v] = (unsigned)prv | (unsigned)lp << 8 | (unsigned)nxt << 16 | (2u | (unsigned)ls << 2) << 24;
+			vn[v][0] = pid;
+			vn[v][1] = vn[qp][qs];
+			vn[v][2] = vn[lp][ls];
+		}
+#endif
+		const unsigned long long nm = (m >= 64 ? ~0ull : (1ull << m) - 1) << p0;
+		delm.w[0] &= ~nm;
+		flipm.w[0] &= ~nm;
+	}
+
+	// Points every cut slot of the negative vertices (below p0) at the new
+	// vertex on that edge: the positive end's slot holds it since the walk.
+	// False if some cut edge was not walked (its count differs from m).
+	VG_FN bool link_cut(int p0, int m) {
+#if VG_WDEV
+		const unsigned w = vt[0];
+		const bool neg = lane < p0 && ((negm >> lane) & 1ull);
+		int cnt = 0;
+		unsigned nw = w;
+		for (int i = 0; i < 3; i++) {
+			const unsigned e = (w >> (8 * i)) & 0xffu;
+			const bool cut = neg && e < (unsigned)p0 && ((posm >> e) & 1ull);
+			const unsigned we = bperm(w, (int)((cut ? e : 0u) << 2));
+			const unsigned back = (w >> (24 + 2 * i)) & 3u;
+			const unsigned nv = (we >> (8 * back)) & 0xffu;
+			if (cut) nw = (nw & ~(0xffu << (8 * i)) & ~(3u << (24 + 2 * i))) | nv << (8 * i) | 1u << (24 + 2 * i);
+			cnt += __builtin_popcountll(__ballot(cut));
+		}
+		vt[0] = nw;
+		return cnt == m;
+#else
+		int cnt = 0;
+		unsigned nw[64];
+		for (int v = 0; v < p0; v++) {
+			const unsigned w = vt[v];
+			nw[v] = w;
+			if (!((negm >> v) & 1ull)) continue;
+			for (int i = 0; i < 3; i++) {
+				const unsigned e = (w >> (8 * i)) & 0xffu;
+				if (!(e < (unsigned)p0 && ((posm >> e) & 1ull))) continue;
+				const unsigned back = (w >> (24 + 2 * i)) & 3u;
+				const unsigned nv = (vt[e] >> (8 * back)) & 0xffu;
+				nw[v] = (nw[v] & ~(0xffu << (8 * i)) & ~(3u << (24 + 2 * i))) | nv << (8 * i) | 1u << (24 + 2 * i);
+				cnt++;
+			}
+		}
+		for (int v = 0; v < p0; v++) vt[v] = nw[v];
+		return cnt == m;
+#endif
+	}
+
 	// src/cell.cc:718-870.  Returns false if the cell was deleted or a
 	// capacity was exceeded (ovf set).
 	VG_FN bool nplane(double x, double y, double z, double rsq, int pid) {
@@ -690,26 +864,44 @@ struct WCellT {
 		}
 		if (ovf) return false;
 		nds = 0;
-		if (create_facet(lp, ls, l, us, u, pid)) return false;
+		const int p0 = p;
+		if constexpr (K == 1) {
+			if (create_facet1(lp, ls, us, pid)) return false;
+		} else {
+			if (create_facet(lp, ls, l, us, u, pid)) return false;
+		}
 		// Delete the cut vertices and everything reachable from them
-		// (src/cell.cc:786-863).
+		// (src/cell.cc:786-863); a cut slot holds -1 or (one register set)
+		// the index of a new vertex, both at least p0.
 		int d = 0;
 		while (d < nds) {
 			j = DS(d);
 			if (!DEL(j)) { setDEL(j); d++; }
 			else setDS(d, DS(--nds));
 		}
-		for (d = 0; d < nds; d++) {
-			cp = DS(d);
-			const unsigned w = TW(cp);
-			for (int e = 0; e < 3; e++) {
-				qp = twE(w, e);
-				if (qp != -1 && !DEL(qp)) {
-					if (!push_ds(qp)) return false;
-					setDEL(qp);
+		bool search = true;
+		if constexpr (K == 1) {
+			// Every cut edge was walked (create_facet1), so a non-cut edge
+			// of a vertex above the plane ends above or on it.  A vertex on
+			// the plane that the walk did not meet is left to the general
+			// tier; otherwise the search only has vertices above the plane
+			// to reach, none once all of them are marked.
+			const unsigned long long low = p0 >= 64 ? ~0ull : (1ull << p0) - 1;
+			if (low & ~posm & ~negm) { ovf = true; return false; }
+			search = __builtin_popcountll(delm.w[0] & low) != __builtin_popcountll(posm);
+		}
+		if (search)
+			for (d = 0; d < nds; d++) {
+				cp = DS(d);
+				const unsigned w = TW(cp);
+				for (int e = 0; e < 3; e++) {
+					qp = twE(w, e);
+					if (qp != -1 && (K > 1 || qp < p0) && !DEL(qp)) {
+						if (!push_ds(qp)) return false;
+						setDEL(qp);
+					}
 				}
 			}
-		}
 		return compact();
 	}
 
