@@ -135,6 +135,12 @@ int launch_cells_wave(int k, bool poly, bool prd, int occ, int waves, hipStream_
 	} else if (occ >= 8) {
 		if (poly) VGPU_WAVE_LAUNCH(1, true, 8, false);
 		else VGPU_WAVE_LAUNCH(1, false, 8, false);
+	} else if (occ == 6) {
+		if (poly) VGPU_WAVE_LAUNCH(1, true, 6, false);
+		else VGPU_WAVE_LAUNCH(1, false, 6, false);
+	} else if (occ == 5) {
+		if (poly) VGPU_WAVE_LAUNCH(1, true, 5, false);
+		else VGPU_WAVE_LAUNCH(1, false, 5, false);
 	} else if (occ <= 2) {
 		if (poly) VGPU_WAVE_LAUNCH(1, true, 2, false);
 		else VGPU_WAVE_LAUNCH(1, false, 2, false);

@@ -1900,7 +1900,7 @@ int vgpu_set_launch(vgpu_container *h, int fast_waves, int fast_occupancy) {
 
 int vgpu_set_first_tier(vgpu_container *h, int tier, int waves, int occupancy) {
 	if (!h || tier < 0 || tier > 1 || waves < 0 || occupancy < 0 || occupancy > 8 || occupancy == 1 ||
-	    (occupancy > 4 && occupancy < 8))
+	    occupancy == 7)
 		return VGPU_INVALID_ARGUMENT;
 	h->first_tier = tier;
 	h->wave_waves = waves;

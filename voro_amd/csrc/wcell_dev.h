@@ -84,6 +84,11 @@ VG_INL T pickk(const T (&a)[K], int h) {
 // a[h] = x where c, every set updated by value arithmetic
 template<int K, class T>
 VG_INL void putk(T (&a)[K], int h, bool c, T x) {
+	if constexpr (K == 1) {      // one set: every valid index is below 64
+		(void)h;
+		a[0] = c ? x : a[0];
+		return;
+	}
 	a[0] = (c && h == 0) ? x : a[0];
 	if (K > 1) a[1] = (c && h == 1) ? x : a[1];
 	if (K > 2) { a[2] = (c && h == 2) ? x : a[2]; a[3] = (c && h == 3) ? x : a[3]; }
@@ -110,6 +115,7 @@ struct WMask {
 	VG_INL bool get(int v) const { return (pickk<K>(w, v >> 6) >> (v & 63)) & 1ull; }
 	VG_INL void set(int v) { set_if(true, v); }
 	VG_INL void unset(int v) {
+		if constexpr (K == 1) { w[0] &= ~(1ull << (v & 63)); return; }
 		const unsigned long long b = 1ull << (v & 63);
 		const int h = v >> 6;
 		w[0] &= h == 0 ? ~b : ~0ull;
@@ -117,6 +123,7 @@ struct WMask {
 		if (K > 2) { w[2] &= h == 2 ? ~b : ~0ull; w[3] &= h == 3 ? ~b : ~0ull; }
 	}
 	VG_INL void toggle(int v) {
+		if constexpr (K == 1) { w[0] ^= 1ull << (v & 63); return; }
 		const unsigned long long b = 1ull << (v & 63);
 		const int h = v >> 6;
 		w[0] ^= h == 0 ? b : 0ull;
@@ -125,6 +132,7 @@ struct WMask {
 	}
 	// set(v) when c
 	VG_INL void set_if(bool c, int v) {
+		if constexpr (K == 1) { w[0] |= c ? 1ull << (v & 63) : 0ull; return; }
 		const unsigned long long b = c ? 1ull << (v & 63) : 0ull;
 		const int h = v >> 6;
 		w[0] |= h == 0 ? b : 0ull;
@@ -151,6 +159,7 @@ struct WCellT {
 	unsigned vds, vdm;       // stack entries 4 * lane .. 4 * lane + 3
 	int nbv[K];              // output staging: neighbour id number 64 h + `lane`
 	unsigned vrec;           // one vertex per lane: the facet records of create_facet1
+	unsigned vdec;           // one vertex per lane: the search decisions of decide_search
 	int lane;
 	__amdgpu_buffer_rsrc_t rs;
 #else
@@ -161,6 +170,7 @@ struct WCellT {
 	unsigned char vds[MAXS], vdm[MAXS];
 	int nbv[MAXV];
 	unsigned vrec[MAXV];
+	unsigned vdec[MAXV];
 	char *wsb;
 #endif
 
@@ -530,6 +540,138 @@ struct WCellT {
 		return true;
 	}
 
+	// The searches' step decisions of every vertex for the current plane
+	// (one register set): for each slot vs the search entered by (-1 for
+	// none; bits 2(vs+1) up, 8 + 2(vs+1) down), the first other slot whose
+	// neighbour's value is greater (upward) or smaller (downward) than the
+	// vertex's -- the comparisons of the reference's loops in
+	// src/cell.cc:480-515 and 597-629 -- or 3 for none; bit 16: a neighbour
+	// index the value test rejects (m_test).
+	VG_FN void decide_search() {
+#if VG_WDEV
+		const unsigned w = vt[0];
+		const double own = vtv[0];
+		double t[3];
+		unsigned bad = 0;
+		for (int i = 0; i < 3; i++) {
+			const unsigned e = (w >> (8 * i)) & 0xffu;
+			bad |= e >= (unsigned)MAXV ? 1u : 0u;
+			t[i] = bperm64(vtv[0], (int)((e & 63u) << 2));
+		}
+		unsigned d = bad << 16;
+		for (int vs = -1; vs < 3; vs++) {
+			unsigned a = 3, b = 3;
+			for (int i = 2; i >= 0; i--) {
+				if (i == vs) continue;
+				if (t[i] > own) a = (unsigned)i;
+				if (own > t[i]) b = (unsigned)i;
+			}
+			d |= a << (2 * (vs + 1)) | b << (8 + 2 * (vs + 1));
+		}
+		vdec = d;
+#else
+		for (int v = 0; v < p; v++) {
+			const unsigned w = vt[v];
+			const double own = vtv[v];
+			double t[3];
+			unsigned bad = 0;
+			for (int i = 0; i < 3; i++) {
+				const unsigned e = (w >> (8 * i)) & 0xffu;
+				bad |= e >= (unsigned)MAXV ? 1u : 0u;
+				t[i] = vtv[e & 63u];
+			}
+			unsigned d = bad << 16;
+			for (int vs = -1; vs < 3; vs++) {
+				unsigned a = 3, b = 3;
+				for (int i = 2; i >= 0; i--) {
+					if (i == vs) continue;
+					if (t[i] > own) a = (unsigned)i;
+					if (own > t[i]) b = (unsigned)i;
+				}
+				d |= a << (2 * (vs + 1)) | b << (8 + 2 * (vs + 1));
+			}
+			vdec[v] = d;
+		}
+#endif
+	}
+#if VG_WDEV
+	VG_INL unsigned DEC(int v) const { return rlu(vdec, v); }
+#else
+	VG_INL unsigned DEC(int v) const { return vdec[v]; }
+#endif
+	// m_test's class from the plane's masks (one register set)
+	VG_INL unsigned cls(int n) const {
+		return ((posm >> n) & 1ull) ? 2u : (((negm >> n) & 1ull) ? 0u : 1u);
+	}
+
+	// search_upward over the decisions of decide_search: a step reads the
+	// vertex's edges and decision word; a vertex with no greater neighbour
+	// takes the reference's tie search (definite_max) with the values.
+	VG_FN bool search_upward1(unsigned &uw, int &lp, int &ls, int &us, double &l, double &u) {
+		int vs = -1, guard = 0;
+		unsigned w = 0;
+		lp = up;
+		for (;;) {
+			w = TW(lp);
+			const unsigned d = DEC(lp);
+			if ((d >> 16) & 1u) { fatal = ovf = true; return false; }
+			ls = (int)((d >> (2 * (vs + 1))) & 3u);
+			if (ls != 3) {
+				up = (int)((w >> (8 * ls)) & 0xffu);
+				uw = cls(up);
+			} else {
+				l = T(lp);
+				const int e0 = twE(w, 0), e1 = twE(w, 1), e2 = twE(w, 2);
+				double ta, tb, tc;
+				const unsigned wa = m_test(e0, ta), wb = m_test(e1, tb), wc = m_test(e2, tc);
+				const int last = vs == 2 ? 1 : 2;
+				up = sel3(last, e0, e1, e2);
+				uw = sel3(last, wa, wb, wc);
+				u = sel3(last, ta, tb, tc);
+				if (definite_max(lp, ls, l, u, uw, ta, tb, tc)) { up = lp; return false; }
+				w = TW(lp);
+			}
+			if (uw != 0) break;
+			if (ovf || ++guard > 4 * MAXV) { fatal = ovf = true; return false; }
+			vs = twR(w, ls); lp = up;
+		}
+		us = twR(w, ls);
+		return true;
+	}
+
+	// search_downward over the decisions of decide_search (definite_min for
+	// a vertex with no smaller neighbour).
+	VG_FN bool search_downward1(unsigned &lw, int &lp, int &ls, int &us, double &l, double &u) {
+		int vs = -1, guard = 0;
+		unsigned w = 0;
+		for (;;) {
+			w = TW(up);
+			const unsigned d = DEC(up);
+			if ((d >> 16) & 1u) { fatal = ovf = true; return false; }
+			us = (int)((d >> (8 + 2 * (vs + 1))) & 3u);
+			if (us != 3) {
+				lp = (int)((w >> (8 * us)) & 0xffu);
+				lw = cls(lp);
+			} else {
+				u = T(up);
+				const int e0 = twE(w, 0), e1 = twE(w, 1), e2 = twE(w, 2);
+				double ta, tb, tc;
+				const unsigned wa = m_test(e0, ta), wb = m_test(e1, tb), wc = m_test(e2, tc);
+				const int last = vs == 2 ? 1 : 2;
+				lp = sel3(last, e0, e1, e2);
+				lw = sel3(last, wa, wb, wc);
+				l = sel3(last, ta, tb, tc);
+				if (definite_min(lp, us, l, u, lw, ta, tb, tc)) return false;
+				w = TW(up);
+			}
+			if (lw != 2) break;
+			if (ovf || ++guard > 4 * MAXV) { fatal = ovf = true; return false; }
+			vs = twR(w, us); up = lp;
+		}
+		ls = twR(w, us);
+		return true;
+	}
+
 	// src/cell.cc:480-515 with definite_max folded in.
 	VG_FN bool search_upward(unsigned &uw, int &lp, int &ls, int &us, double &l, double &u) {
 		int vs = -1, guard = 0;
@@ -851,12 +993,21 @@ struct WCellT {
 		up = 0;
 		px = x; py = y; pz = z; prsq = rsq;
 		if (!plane_values()) return true;   // every vertex below -tol: no cut
+		if constexpr (K == 1) decide_search();
 		uw = m_test(up, u);
 		if (uw == 2) {
-			if (!search_downward(lw, lp, ls, us, l, u)) return false;
+			if constexpr (K == 1) {
+				if (!search_downward1(lw, lp, ls, us, l, u)) return false;
+			} else {
+				if (!search_downward(lw, lp, ls, us, l, u)) return false;
+			}
 			if (lw == 1) { ovf = true; return false; }     // vertex on the plane
 		} else if (uw == 0) {
-			if (!search_upward(uw, lp, ls, us, l, u)) return true;
+			if constexpr (K == 1) {
+				if (!search_upward1(uw, lp, ls, us, l, u)) return true;
+			} else {
+				if (!search_upward(uw, lp, ls, us, l, u)) return true;
+			}
 			if (uw == 1) { ovf = true; return false; }     // vertex on the plane
 		} else {
 			ovf = true;                                      // vertex 0 on the plane
