@@ -259,6 +259,10 @@ class container_periodic : public container_base_gpu {
 				     nullptr) { (void)init_mem; }
 	void put(int n, double x, double y, double z) { put_one(n, x, y, z, nullptr, nullptr); }
 	void put(particle_order &vo, int n, double x, double y, double z) { put_one(n, x, y, z, nullptr, &vo); }
+	// find_voronoi_cell, src/container_prd.cc:303-326
+	bool find_voronoi_cell(double x, double y, double z, double &rx, double &ry, double &rz, int &pid) {
+		return find(x, y, z, rx, ry, rz, pid);
+	}
 };
 
 // container_periodic_poly, src/container_prd.hh:434-441
@@ -270,6 +274,10 @@ class container_periodic_poly : public container_base_gpu {
 				     nullptr) { (void)init_mem; }
 	void put(int n, double x, double y, double z, double r) { put_one(n, x, y, z, &r, nullptr); }
 	void put(particle_order &vo, int n, double x, double y, double z, double r) { put_one(n, x, y, z, &r, &vo); }
+	// find_voronoi_cell, src/container_prd.cc:338-360
+	bool find_voronoi_cell(double x, double y, double z, double &rx, double &ry, double &rz, int &pid) {
+		return find(x, y, z, rx, ry, rz, pid);
+	}
 };
 
 // Shared part of the loops: the loop position m (an index into the

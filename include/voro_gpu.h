@@ -97,27 +97,33 @@ int vgpu_clear(vgpu_container *h);
 int vgpu_compute_all(vgpu_container *h, unsigned flags, void *stream);
 
 /* container::find_voronoi_cell / container_poly::find_voronoi_cell
- * (src/container.cc:237-297) for n query points at once: found[i] = 1 with
- * pid[i] and pos[3i..3i+2] (the particle's position, shifted by whole
- * periods along periodic directions) for the particle whose Voronoi cell
- * contains point i, or 0 (point outside a non-periodic direction, or no
- * particle).  Rectangular containers only (VGPU_INVALID_ARGUMENT for the
- * periodic classes).  Bins the particles if they changed since the last
- * compute.  on_device: the four arrays are device pointers.  Returns
- * VGPU_MEMORY_ERROR (found[i] = -1) if a search outgrew the largest block
- * window (±63 blocks). */
+ * (src/container.cc:237-297) and container_periodic(_poly)::
+ * find_voronoi_cell (src/container_prd.cc:303-360) for n query points at
+ * once, as n calls made one after another: found[i] = 1 with pid[i] and
+ * pos[3i..3i+2] (the particle's position, shifted by whole periods to the
+ * image nearest the point) for the particle whose Voronoi cell contains
+ * point i, or 0 (point outside a non-periodic direction, or no particle).
+ * Periodic classes: the queries create the image blocks they reach, in the
+ * reference's order, on top of those the last compute created.  Bins the
+ * particles if they changed since the last compute.  on_device: the four
+ * arrays are device pointers.  Returns VGPU_MEMORY_ERROR (found[i] = -1) if a
+ * search outgrew the largest block window (±63 blocks). */
 int vgpu_find_voronoi_cells(vgpu_container *h, int n, const double *xyz, int *found, int *pid, double *pos,
 			    int on_device);
 /* container::compute_ghost_cell / container_poly::compute_ghost_cell
- * (src/container.hh:488-498,710-721) for n points (host arrays; r: radii,
- * radical containers only): the cell a particle at each point would have if
- * it were put last in its block.  exists[i]: the reference's return value
- * (0: the point lies outside a non-periodic direction, or its cell vanished);
- * vol, cent (relative to the point), nfaces, and up to max_nbr neighbour ids
- * per point at nbr[i * max_nbr].  Rectangular containers only.  Returns
- * VGPU_MEMORY_ERROR (exists[i] = -1) for a cell over 1,024 vertices or
- * max_nbr faces, or whose search comes back to the point's own block through
- * a periodic image. */
+ * (src/container.hh:488-498,710-721) and container_periodic(_poly)::
+ * compute_ghost_cell (src/container_prd.hh:414-420,635-645) for n points
+ * (host arrays; r: radii, radical containers only): the cell a particle at
+ * each point would have if it were put last in its block.  exists[i]: the
+ * reference's return value (0: the point lies outside a non-periodic
+ * direction, or its cell vanished); vol, cent (relative to the point),
+ * nfaces, and up to max_nbr neighbour ids per point at nbr[i * max_nbr].
+ * Returns VGPU_MEMORY_ERROR, with exists[i] = -1 for the points concerned and
+ * every other point's results complete, for a cell over 4,096 vertices or
+ * max_nbr faces, a search that comes back to the point's own block through a
+ * periodic image, or (periodic classes) one that needs an image block the
+ * last compute or queries did not create (the reference would create it
+ * during the call and copy the point into it, with an id it never sets). */
 int vgpu_compute_ghost_cells(vgpu_container *h, int n, const double *xyz, const double *r, int *exists, double *vol,
 			     double *cent, int *nfaces, int *nbr, int max_nbr);
 /* For each stored particle in loop order, its position in the sequence of

@@ -344,6 +344,41 @@ def ghost_reference(spec, ids, xyz, r, q, qr=None, max_nbr=128):
     return o
 
 
+def prd_queries_reference(spec, ids, xyz, r, q, qr=None, pre=False, ghost=False, max_nbr=128):
+    """The periodic classes' find_voronoi_cell (ghost False: (found, pid,
+    pos)) or compute_ghost_cell (ghost True: dict as ghost_reference) of the
+    reference library for every row of q, made one after another on one
+    container; pre: compute_all_cells() first.  None where the library is
+    unavailable."""
+    lib = _load(REF_SO, "ref_compute_neighbor") and _LIBS[REF_SO][0]
+    if not lib:
+        return None
+    fn = lib.ref_prd_queries
+    fn.restype = C.c_longlong
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+    m = len(q)
+    qq = None if qr is None else np.ascontiguousarray(qr, dtype=np.float64)
+    found = np.zeros(m, np.int32)
+    pid = np.zeros(m, np.int32)
+    pos = np.zeros((m, 3))
+    o = {"vol": np.zeros(m), "cent": np.zeros((m, 3)), "nfaces": np.zeros(m, np.int32),
+         "nbr": np.zeros((m, max_nbr), np.int32)}
+    k = fn(C.byref(spec), C.c_int(len(ids)), _ptr(ids, C.c_int), _ptr(xyz, C.c_double),
+           None if rr is None else _ptr(rr, C.c_double), C.c_int(1 if pre else 0), C.c_int(1 if ghost else 0),
+           C.c_int(m), _ptr(q, C.c_double), None if qq is None else _ptr(qq, C.c_double), _ptr(found, C.c_int),
+           _ptr(pid, C.c_int), _ptr(pos, C.c_double), _ptr(o["vol"], C.c_double), _ptr(o["cent"], C.c_double),
+           _ptr(o["nfaces"], C.c_int), _ptr(o["nbr"], C.c_int), C.c_int(max_nbr))
+    if k < 0:
+        raise RuntimeError("reference periodic queries: periodic containers only")
+    if ghost:
+        o["exists"] = found
+        return o
+    return found.astype(bool), pid, pos
+
+
 def print_custom_reference(spec, ids, xyz, r, fmt, filename):
     """The reference library's print_custom(format, filename) for this
     container; False where the library is unavailable."""

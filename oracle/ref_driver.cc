@@ -275,6 +275,51 @@ long long ref_text(const ref_spec *s, int n, const int *ids, const double *xyz, 
 	return 0;
 }
 
+// The periodic classes' find_voronoi_cell (src/container_prd.cc:303-360) and
+// compute_ghost_cell (src/container_prd.hh:414-420,635-645) for nq points,
+// made one after another on one container; pre != 0: compute_all_cells()
+// first (the image blocks it creates are then the ones the queries start
+// from).  Outputs as ref_find_voronoi_cells / ref_compute_ghost_cells;
+// ghost = 0 finds, 1 ghost cells.  The ghost point is written into its block
+// without growing it (the reference's code), so the caller's init_mem must
+// exceed every block's count.
+inline bool ghost_call(container_periodic &con, voronoicell_neighbor &c, double x, double y, double z, const double *) {
+	return con.compute_ghost_cell(c, x, y, z);
+}
+inline bool ghost_call(container_periodic_poly &con, voronoicell_neighbor &c, double x, double y, double z,
+		       const double *r) {
+	return con.compute_ghost_cell(c, x, y, z, *r);
+}
+
+template<class c_class>
+long long prd_queries_impl(c_class &con, int pre, int ghost, int nq, const double *q, const double *qr, int *found,
+			   int *pid, double *pos, double *vol, double *cent, int *nfaces, int *nbr, int max_nbr,
+			   bool poly) {
+	if (pre) con.compute_all_cells();
+	long long m = 0;
+	std::vector<int> nb;
+	voronoicell_neighbor c(con);
+	for (int i = 0; i < nq; i++) {
+		const double x = q[3 * i], y = q[3 * i + 1], z = q[3 * i + 2];
+		if (!ghost) {
+			found[i] = con.find_voronoi_cell(x, y, z, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], pid[i]) ? 1 : 0;
+			m += found[i];
+			continue;
+		}
+		const bool ok = ghost_call(con, c, x, y, z, poly ? qr + i : 0);
+		found[i] = ok ? 1 : 0;
+		vol[i] = 0; cent[3 * i] = cent[3 * i + 1] = cent[3 * i + 2] = 0; nfaces[i] = 0;
+		if (!ok) continue;
+		m++;
+		vol[i] = c.volume();
+		c.centroid(cent[3 * i], cent[3 * i + 1], cent[3 * i + 2]);
+		nfaces[i] = c.number_of_faces();
+		c.neighbors(nb);
+		for (int f = 0; f < (int)nb.size() && f < max_nbr; f++) nbr[(size_t)i * max_nbr + f] = nb[f];
+	}
+	return m;
+}
+
 }  // namespace
 
 extern "C" {
@@ -380,6 +425,27 @@ long long ref_compute_ghost_cells(const ref_spec *s, int n, const int *ids, cons
 // print_custom(format, filename) (src/container.cc:401-432 and the periodic
 // classes'), and the drawings: what 0 print_custom, 1 draw_particles,
 // 2 draw_particles_pov, 3 draw_cells_gnuplot, 4 draw_cells_pov.
+long long ref_prd_queries(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r, int pre,
+			  int ghost, int nq, const double *q, const double *qr, int *found, int *pid, double *pos,
+			  double *vol, double *cent, int *nfaces, int *nbr, int max_nbr) {
+	const double *g = s->geom;
+	if (s->kind == 2) {
+		container_periodic con(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
+				       s->init_mem);
+		put_all(con, n, ids, xyz, r);
+		return prd_queries_impl(con, pre, ghost, nq, q, qr, found, pid, pos, vol, cent, nfaces, nbr, max_nbr,
+					false);
+	}
+	if (s->kind == 3) {
+		container_periodic_poly con(g[0], g[1], g[2], g[3], g[4], g[5], s->grid[0], s->grid[1], s->grid[2],
+					    s->init_mem);
+		put_all_r(con, n, ids, xyz, r);
+		return prd_queries_impl(con, pre, ghost, nq, q, qr, found, pid, pos, vol, cent, nfaces, nbr, max_nbr,
+					true);
+	}
+	return -1;
+}
+
 long long ref_print_custom(const ref_spec *s, int n, const int *ids, const double *xyz, const double *r,
 			   const char *format, const char *filename) {
 	return ref_text(s, n, ids, xyz, r, 0, format, filename);

@@ -79,15 +79,177 @@ def test_gpu_find_matches_reference(oracle_mod, case):
 
 
 @pytest.mark.gpu
-def test_gpu_find_single_point_and_periodic_class_rejected():
+def test_gpu_find_single_point_and_periodic_class():
     con = voro_amd.container(0, 1, 0, 1, 0, 1, 2, 2, 2, False, False, False, 8)
     con.put_array(np.array([5, 6], np.int32), np.array([[0.25, 0.25, 0.25], [0.75, 0.75, 0.75]]))
     assert con.find_voronoi_cell(0.3, 0.2, 0.1) == (True, (0.25, 0.25, 0.25), 5)
     assert con.find_voronoi_cell(1.5, 0.2, 0.1)[0] is False
+    # one particle in a unit cubic lattice: every point finds it, at the
+    # lattice image nearest the point
     pc = voro_amd.container_periodic(1, 0, 1, 0, 0, 1, 2, 2, 2, 8)
     pc.put_array(np.array([1], np.int32), np.array([[0.5, 0.5, 0.5]]))
-    with pytest.raises(voro_amd.VoroGPUError):
-        pc.find_voronoi_cells(np.zeros((1, 3)))
+    assert pc.find_voronoi_cell(0.1, 0.2, 0.3) == (True, (0.5, 0.5, 0.5), 1)
+    assert pc.find_voronoi_cell(3.2, -1.9, 0.6) == (True, (3.5, -1.5, 0.5), 1)
+
+
+# ---- the periodic classes' find_voronoi_cell and compute_ghost_cell ---------
+
+def _para(geom, n, seed, lo, hi):
+    """n points at lattice coordinates uniform in [lo, hi)^3."""
+    bx, bxy, by, bxz, byz, bz = geom
+    u = np.stack([I.uniform(seed, 90 + a, n) for a in range(3)], 1) * (hi - lo) + lo
+    return np.stack([u[:, 0] * bx + u[:, 1] * bxy + u[:, 2] * bxz, u[:, 1] * by + u[:, 2] * byz, u[:, 2] * bz], 1)
+
+
+def test_reference_periodic_find_matches_brute_force(oracle_mod):
+    """The reference's periodic find_voronoi_cell (the oracle the GPU tests
+    use) is the nearest particle image over the lattice: same id, and the
+    returned position is that image (to rounding: the reference adds the
+    lattice shifts in its own order)."""
+    O = oracle_mod
+    _ref_or_skip(O)
+    geom = (10, 1.5, 10, 0.7, -1.2, 10)
+    bx, bxy, by, bxz, byz, bz = geom
+    n = 300
+    xyz = _para(geom, n, 51, 0.0, 1.0)
+    r = I.radii_lognormal(51, n) * 0.5
+    q = _para(geom, 300, 52, -2.0, 3.0)
+    a = np.array([[bx, 0, 0], [bxy, by, 0], [bxz, byz, bz]])
+    sh = np.array([(i, j, k) for i in range(-4, 5) for j in range(-4, 5) for k in range(-4, 5)], float) @ a
+    img = (xyz[None, :, :] + sh[:, None, :]).reshape(-1, 3)
+    for kind, rr in ((2, None), (3, r)):
+        found, pid, pos = O.prd_queries_reference(O.make_spec(kind, geom, (4, 4, 4), init_mem=64),
+                                                  np.arange(n, dtype=np.int32), xyz, rr, q)
+        assert found.all()
+        d = ((q[:, None, :] - img[None, :, :]) ** 2).sum(-1)
+        if rr is not None:
+            d = d - np.tile(rr, len(sh))[None, :] ** 2
+        best = d.argmin(axis=1)
+        assert np.array_equal(pid, best % n)
+        assert np.allclose(pos, img[best], rtol=0, atol=1e-9)
+
+
+def test_reference_periodic_ghost_equals_appended_particle(oracle_mod):
+    """After compute_all_cells() the reference's periodic ghost cell of a
+    point deep inside a large lattice is the cell the point gets when put as
+    the last particle of a fresh container."""
+    O = oracle_mod
+    _ref_or_skip(O)
+    geom = (30, 4.5, 30, 2.1, -3.6, 30)
+    n = 5000
+    xyz = _para(geom, n, 53, 0.0, 1.0)
+    q = _para(geom, 6, 54, 0.3, 0.7)
+    sp = O.make_spec(2, geom, (8, 8, 8), init_mem=64)
+    g = O.prd_queries_reference(sp, np.arange(n, dtype=np.int32), xyz, None, q, pre=True, ghost=True)
+    assert g["exists"].all()
+    for i in range(len(q)):
+        a = O.run("reference", sp, np.arange(n + 1, dtype=np.int32), np.vstack([xyz, q[i:i + 1]]))
+        m = int(np.nonzero(a["id"] == n)[0][0])
+        assert a["vol"][m] == g["vol"][i] and a["nfaces"][m] == g["nfaces"][i]
+        assert np.array_equal(a["nbr"][a["nbr_off"][m]:a["nbr_off"][m + 1]], g["nbr"][i, :g["nfaces"][i]])
+
+
+PRD_FIND = {
+    # kind, lattice, grid, particles, radical
+    "cubic": (2, (10, 0, 10, 0, 0, 10), (5, 5, 5), 1500, False),
+    "triclinic": (2, (10, 1.5, 10, 0.7, -1.2, 10), (5, 5, 5), 2000, False),
+    "triclinic_poly": (3, (10, 1.5, 10, 0.7, -1.2, 10), (5, 5, 5), 2000, True),
+    "skew_poly": (3, (5, 7.5, 5, -6.2, 8.1, 5), (4, 4, 4), 600, True),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pre", [False, True], ids=["fresh", "after_compute"])
+@pytest.mark.parametrize("case", sorted(PRD_FIND))
+def test_gpu_periodic_find_matches_reference(oracle_mod, case, pre):
+    """Queries over several periods, as one call each on the reference
+    (which creates the image blocks they reach), on a fresh container and
+    after compute_all_cells(): found, id and returned position bit for bit."""
+    O = oracle_mod
+    _ref_or_skip(O)
+    kind, geom, grid, n, poly = PRD_FIND[case]
+    xyz = _para(geom, n, 61, 0.0, 1.0)
+    r = I.radii_lognormal(61, n) * 0.5 if poly else None
+    q = _para(geom, 6000, 62, -2.5, 3.5)
+    ids = np.arange(n, dtype=np.int32) * 2 + 11
+    print("periodic find case %s pre=%s n=%d queries=%d" % (case, pre, n, len(q)), flush=True)
+    want = O.prd_queries_reference(O.make_spec(kind, geom, grid, init_mem=64), ids, xyz, r, q, pre=pre)
+    cls = voro_amd.container_periodic_poly if poly else voro_amd.container_periodic
+    con = cls(*geom, *grid, 8)
+    con.put_array(ids, xyz, r)
+    if pre:
+        con.compute_all_cells()
+    got = con.find_voronoi_cells(q)
+    assert want[0].all() and got[0].all()
+    assert np.array_equal(got[1], want[1])
+    assert np.array_equal(got[2], want[2])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pre", [False, True], ids=["fresh", "after_compute"])
+def test_gpu_periodic_find_lattice_ties(oracle_mod, pre):
+    """A simple cubic lattice queried at its cell corners, edge and face
+    centres: every answer is a tie between 8, 4 or 2 particles, which the
+    reference breaks by scan order, i.e. by the order in which the image
+    blocks were created; the engine's answers equal the reference's."""
+    O = oracle_mod
+    _ref_or_skip(O)
+    m = 6
+    g = (np.arange(m) + 0.5)
+    xyz = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    ids = np.arange(len(xyz), dtype=np.int32)
+    h = np.arange(-2 * m, 2 * m + 1) * 0.5
+    q = np.stack(np.meshgrid(h, h, h, indexing="ij"), -1).reshape(-1, 3)
+    q = q[(np.abs(q * 2) % 2 == 0).sum(1) >= 1]          # at least one integer coordinate: a tie
+    geom = (m, 0, m, 0, 0, m)
+    want = O.prd_queries_reference(O.make_spec(2, geom, (3, 3, 3), init_mem=64), ids, xyz, None, q, pre=pre)
+    con = voro_amd.container_periodic(*geom, 3, 3, 3, 8)
+    con.put_array(ids, xyz)
+    if pre:
+        con.compute_all_cells()
+    got = con.find_voronoi_cells(q)
+    assert want[0].all() and got[0].all()
+    assert np.array_equal(got[1], want[1])
+    assert np.array_equal(got[2], want[2])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["triclinic", "triclinic_poly"])
+def test_gpu_periodic_ghost_cells_match_reference(oracle_mod, case):
+    """Ghost cells after compute_all_cells(): a point is computed against the
+    image blocks the compute created, bit for bit as the reference's; points
+    whose search would need another image block (one the reference would
+    create during the call, copying the point into it) come back with
+    exists = -1 and every other point's results complete."""
+    O = oracle_mod
+    _ref_or_skip(O)
+    kind, geom, grid, n, poly = PRD_FIND[case]
+    geom = tuple(3 * t for t in geom)
+    grid = (12, 12, 12)
+    n = 27 * n
+    xyz = _para(geom, n, 71, 0.0, 1.0)
+    r = I.radii_lognormal(71, n) * 0.5 if poly else None
+    q = _para(geom, 2000, 72, -1.0, 2.0)
+    qr = I.radii_lognormal(72, 2000) * 0.5 if poly else None
+    ids = np.arange(n, dtype=np.int32) * 3 + 1
+    print("periodic ghost case %s n=%d points=%d" % (case, n, len(q)), flush=True)
+    want = O.prd_queries_reference(O.make_spec(kind, geom, grid, init_mem=64), ids, xyz, r, q, qr, pre=True,
+                                   ghost=True)
+    cls = voro_amd.container_periodic_poly if poly else voro_amd.container_periodic
+    con = cls(*geom, *grid, 8)
+    con.put_array(ids, xyz, r)
+    con.compute_all_cells()
+    got = con.compute_ghost_cells(q, qr)
+    ok = got["exists"] >= 0
+    print("ghosts computed %d of %d" % (ok.sum(), len(q)), flush=True)
+    assert ok.mean() > 0.9
+    assert np.array_equal(got["exists"][ok], want["exists"][ok])
+    e = ok & (want["exists"] == 1)
+    for k in ("vol", "cent", "nfaces"):
+        assert np.array_equal(got[k][e], want[k][e]), k
+    for i in np.nonzero(e)[0]:
+        nf = want["nfaces"][i]
+        assert np.array_equal(got["nbr"][i, :nf], want["nbr"][i, :nf]), i
 
 
 # ---- compute_ghost_cell ----------------------------------------------------

@@ -412,9 +412,11 @@ class _Base:
 
     # -- point location, as container::find_voronoi_cell ----------------------
     def find_voronoi_cells(self, xyz):
-        """find_voronoi_cell (src/container.cc:237-297) for every row of xyz
-        (rectangular containers): returns (found, pid, pos) arrays; found[i]
-        is False where the point lies outside a non-periodic direction."""
+        """find_voronoi_cell (src/container.cc:237-297; the periodic classes'
+        src/container_prd.cc:303-360) for every row of xyz: returns (found,
+        pid, pos) arrays; found[i] is False where the point lies outside a
+        non-periodic direction.  pos is the particle's position shifted by
+        whole periods to the image nearest the point."""
         q = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         n = len(q)
         found = np.zeros(n, np.int32)
@@ -430,19 +432,26 @@ class _Base:
         return bool(f[0]), tuple(float(t) for t in pos[0]), int(pid[0])
 
     def compute_ghost_cells(self, xyz, r=None, max_nbr=128):
-        """compute_ghost_cell (src/container.hh:488-498,710-721) for every row
-        of xyz (rectangular containers; r: radii for radical containers):
-        dict of exists, vol, cent (relative to the point), nfaces and nbr
-        (max_nbr ids per point)."""
+        """compute_ghost_cell (src/container.hh:488-498,710-721; the periodic
+        classes' src/container_prd.hh:414-420,635-645) for every row of xyz
+        (r: radii for radical containers): dict of exists, vol, cent
+        (relative to the point), nfaces and nbr (max_nbr ids per point).
+        exists[i] is -1 for a point this engine does not compute: a cell over
+        the largest cell's capacity or max_nbr faces, a search that comes back
+        to the point's own block through a periodic image, or (periodic
+        classes) one that needs an image block the reference would create
+        during the call."""
         q = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         m = len(q)
         rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
         o = {"exists": np.zeros(m, np.int32), "vol": np.zeros(m), "cent": np.zeros((m, 3)),
              "nfaces": np.zeros(m, np.int32), "nbr": np.zeros((m, max_nbr), np.int32)}
-        _check(lib().vgpu_compute_ghost_cells(self._h, m, _p(q, C.c_double), None if rr is None else _p(rr, C.c_double),
-                                              _p(o["exists"], C.c_int), _p(o["vol"], C.c_double),
-                                              _p(o["cent"], C.c_double), _p(o["nfaces"], C.c_int),
-                                              _p(o["nbr"], C.c_int), int(max_nbr)), "vgpu_compute_ghost_cells")
+        st = lib().vgpu_compute_ghost_cells(self._h, m, _p(q, C.c_double), None if rr is None else _p(rr, C.c_double),
+                                            _p(o["exists"], C.c_int), _p(o["vol"], C.c_double),
+                                            _p(o["cent"], C.c_double), _p(o["nfaces"], C.c_int),
+                                            _p(o["nbr"], C.c_int), int(max_nbr))
+        if st != MEMORY_ERROR:      # per-point failures are marked exists = -1
+            _check(st, "vgpu_compute_ghost_cells")
         return o
 
     # -- per-cell access, as voro_compute::compute_cell(c, vl) ----------------

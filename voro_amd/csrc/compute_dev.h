@@ -57,6 +57,9 @@ struct RectView {
 	const UnitVert *unit;   // the unit Voronoi cell
 	int unit_p;
 	unsigned long long *fv; // first visit of each block (null: not recorded)
+	// ghost cells of a periodic container: the blocks they may read (null:
+	// any); a search that reaches another block is given up (overflow)
+	const unsigned char *img_ok;
 	// batches of containers: this container's first block in the batch's
 	// block arrays (bstart and cstart above already point at it)
 	int boff;
@@ -404,7 +407,9 @@ struct CandFetch {
 // point that is not stored, as if put last in block ijk (s = the block's
 // count) with the container's maximum radius ghost_maxr; a block search that
 // comes back to block ijk (where the reference would also meet the point's
-// own periodic image) ends as an overflow.
+// own periodic image) ends as an overflow, and so does one that reaches a
+// block v.img_ok excludes (periodic containers: an image block the reference
+// would create during the call, copying the point into it).
 template<class CELL, bool POLY, bool PRD = false>
 VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci, int cj, int ck,
 			     unsigned stamp, CellStats &stt, unsigned long long key0 = 0, const dbl4 *ghost = nullptr,
@@ -585,7 +590,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 					ek = (int)((q >> 14) & 127) - 64 + k;
 					const int rijk = PRD ? prd_region(v, ci, cj, ck, ei, ej, ek, qx, key++, stt.img)
 							     : rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
-					if (ghost && rijk == ijk) { c.ovf = true; stt.n_planes = c.n_cuts; return false; }
+					if (ghost && (rijk == ijk || (PRD && v.img_ok && !v.img_ok[rijk]))) { c.ovf = true; stt.n_planes = c.n_cuts; return false; }
 					cf.range(v, g1, rijk, bt, be);
 					cf.block(bt, be);
 					x2 = x - qx; y2 = y - qy; z2 = z - qz;
@@ -684,7 +689,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				{
 					const int rijk = PRD ? prd_region(v, ci, cj, ck, ei, ej, ek, qx, key++, stt.img)
 							     : rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
-					if (ghost && rijk == ijk) { c.ovf = true; stt.n_planes = c.n_cuts; return false; }
+					if (ghost && (rijk == ijk || (PRD && v.img_ok && !v.img_ok[rijk]))) { c.ovf = true; stt.n_planes = c.n_cuts; return false; }
 					cf.range(v, g, rijk, bt, be);
 					cf.block(bt, be);
 					x2 = x - qx; y2 = y - qy; z2 = z - qz;
@@ -737,7 +742,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 			{
 				const int rijk = PRD ? prd_region(v, ci, cj, ck, ei, ej, ek, qx, key++, stt.img)
 						     : rect_region(v, ci, cj, ck, ei, ej, ek, qx, qy, qz, disp);
-				if (ghost && rijk == ijk) { c.ovf = true; stt.n_planes = c.n_cuts; return false; }
+				if (ghost && (rijk == ijk || (PRD && v.img_ok && !v.img_ok[rijk]))) { c.ovf = true; stt.n_planes = c.n_cuts; return false; }
 				bt = v.bstart[rijk]; be = v.bstart[rijk + 1];
 				cf.block(bt, be);
 				x2 = x - qx; y2 = y - qy; z2 = z - qz;

@@ -30,6 +30,10 @@ namespace {
 struct DBuf {
 	void *p = nullptr;
 	size_t n = 0;
+	DBuf() = default;
+	DBuf(const DBuf &) = delete;
+	DBuf &operator=(const DBuf &) = delete;
+	~DBuf() { release(); }
 	int ensure(size_t bytes) {
 		if (bytes <= n) return VGPU_OK;
 		if (p) (void)hipFree(p);
@@ -111,6 +115,7 @@ struct vgpu_container {
 	DBuf retry0, retry1, retry2, retry3, retry4, retry5;
 	DBuf fq_xyz, fq_found, fq_pid, fq_pos, fq_retry, fq_retry2;   // find_voronoi_cell queries
 	DBuf fq_r, fq_vol, fq_nbr;                                   // ghost cells
+	DBuf fq_touch, fq_list, img_okb;                             // periodic queries: image reads, reruns
 	DBuf st_planes, st_exit, st_tier;
 	// learned path
 	bool have_pred = false;
@@ -130,6 +135,9 @@ struct vgpu_container {
 	std::vector<dbl4> hP;
 	std::vector<int> hpid, hcs;
 	int image_passes = 0;
+	// the image blocks the reference would hold after the last compute and
+	// queries since the binning, in their order of creation
+	std::vector<int> prd_order;
 	// neighbour arena size that held the last compute (grown when it overflows)
 	long long acap_hint = 0;
 	// polyhedron statistics (VGPU_F_GEOMETRY): the cell tables kept by the
@@ -210,6 +218,7 @@ RectView make_view(vgpu_container *h) {
 	v.ey = pg.ey; v.ez = pg.ez; v.wy = pg.wy; v.wz = pg.wz; v.oy = pg.oy; v.oz = pg.oz;
 	v.unit = h->d_unit.as<UnitVert>(); v.unit_p = h->prd ? (int)pg.unit.size() : 0;
 	v.fv = nullptr;
+	v.img_ok = nullptr;
 	v.boff = 0;
 	return v;
 }
@@ -549,7 +558,8 @@ int run_prd_cells(vgpu_container *h, RectView &v, CellOut out, int nc, int fw, h
 		}
 		if (list.empty()) return VGPU_OK;   // no cell read an image block
 		ImageSet im(g, h->hP.data(), h->hpid.data(), h->hcs.data());
-		for (int b : prd_visit_order(g, prev.data())) im.create_block(b);
+		h->prd_order = prd_visit_order(g, prev.data());
+		for (int b : h->prd_order) im.create_block(b);
 		if (im.late) return VGPU_INTERNAL_ERROR;
 		if ((st = upload_blocks(h, im, nc, v, s))) return st;
 		int *ctr = h->ctr.as<int>();
@@ -566,6 +576,7 @@ int run_prd_cells(vgpu_container *h, RectView &v, CellOut out, int nc, int fw, h
 		if (cur == prev) return VGPU_OK;
 		prev.swap(cur);
 	}
+	h->prd_order.clear();
 	return VGPU_INTERNAL_ERROR;   // the first-visit order did not settle
 }
 
@@ -691,6 +702,7 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s, bool bin_only
 	if (dup) { h->computed = false; return VGPU_DUPLICATE; }
 	HIPCHK(hipEventRecord(h->ev[1], s));
 	h->binned = true;
+	h->prd_order.clear();
 	if (bin_only) return VGPU_OK;
 
 	// outputs
@@ -1252,7 +1264,8 @@ int vgpu_destroy(vgpu_container *h) {
 			&h->bmax, &h->boff, &h->list_a, &h->d_unit, &h->Pall, &h->pidall, &h->ball, &h->fv,
 			&h->touch, &h->d_flag, &h->retry3, &h->retry4, &h->retry5, &h->img_p, &h->img_id,
 			&h->img_reg, &h->img_rank, &h->img_bc, &h->img_cnt, &h->fq_xyz, &h->fq_found, &h->fq_pid,
-			&h->fq_pos, &h->fq_retry, &h->fq_retry2, &h->fq_r, &h->fq_vol, &h->fq_nbr};
+			&h->fq_pos, &h->fq_retry, &h->fq_retry2, &h->fq_r, &h->fq_vol, &h->fq_nbr, &h->fq_touch,
+			&h->fq_list, &h->img_okb};
 	for (DBuf *b : bufs) b->release();
 	h->ws_fast.buf.release(); h->ws_wide.buf.release(); h->ws_huge.buf.release(); h->ws_wave.buf.release();
 	if (h->events) for (int t = 0; t < 5; t++) (void)hipEventDestroy(h->ev[t]);
@@ -1394,10 +1407,151 @@ int vgpu_compute_all(vgpu_container *h, unsigned flags, void *stream) {
 	return st == VGPU_RETRY ? VGPU_MEMORY_ERROR : st;
 }
 
+namespace {
+
+// The three find tiers (order-3, wide and huge mask windows) over n queries,
+// or over the device list of them counted at *list_count; `failed` counts
+// the queries no window held.  Periodic containers: touch as
+// launch_find_cells.
+int find_tiers(vgpu_container *h, const RectView &v, int n, const double *dq, const int *list, const int *list_count,
+	       int *dfound, int *dpid, double *dpos, unsigned char *touch, hipStream_t s, int &failed) {
+	const size_t N = (size_t)n;
+	int st;
+	failed = 0;
+	if ((st = h->fq_retry.ensure(N * 4)) || (st = h->fq_retry2.ensure(N * 4)) || (st = h->d_counters.ensure(64 * 4)))
+		return st;
+	typedef Layout<TierWide, 64> LW;
+	typedef Layout<TierHuge, 64> LH;
+	const int w0 = std::max(1, std::min(default_fast_waves(2), nblk((long long)n, 64)));
+	if ((st = h->ws_fast.ensure(VCell3::L::bytes, std::max(w0, h->ws_fast.waves))) ||
+	    (st = h->ws_wide.ensure(LW::bytes, std::max(64, h->ws_wide.waves))) ||
+	    (st = h->ws_huge.ensure(LH::bytes, std::max(4, h->ws_huge.waves))))
+		return st;
+	Workspace *wss[3] = {&h->ws_fast, &h->ws_wide, &h->ws_huge};
+	for (Workspace *x : wss)
+		if ((unsigned long long)x->stamp_base + N + 2 >= 0xffffffffull) {
+			HIPCHK(hipMemset(x->buf.p, 0, x->buf.n));
+			x->stamp_base = 0;
+		}
+	// [0] / [2] / [4] query counters of the three tiers, [1] / [3] their retry
+	// counts, [5] queries no window held ([8] a caller's list count)
+	int *ctr = h->d_counters.as<int>();
+	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 8, 0);
+	const bool prd = h->prd;
+	if (launch_find_cells(0, h->poly, prd, w0, s, v, n, dq, list, list_count, dfound, dpid, dpos,
+			      h->fq_retry.as<int>(), ctr + 1, ctr, h->ws_fast.base(), h->ws_fast.slab,
+			      h->ws_fast.stamp_base, touch))
+		return VGPU_HIP_ERROR;
+	h->ws_fast.stamp_base += (unsigned)n + 1u;
+	// the few searches that left the order-3 window, with the wide one, then
+	// the huge one (each launched only if it has queries)
+	int retried = 0;
+	HIPCHK(hipMemcpyAsync(&retried, ctr + 1, 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
+	if (retried > 0) {
+		if (launch_find_cells(1, h->poly, prd, 64, s, v, n, dq, h->fq_retry.as<int>(), ctr + 1, dfound, dpid, dpos,
+				      h->fq_retry2.as<int>(), ctr + 3, ctr + 2, h->ws_wide.base(), h->ws_wide.slab,
+				      h->ws_wide.stamp_base, touch))
+			return VGPU_HIP_ERROR;
+		h->ws_wide.stamp_base += (unsigned)n + 1u;
+		HIPCHK(hipMemcpyAsync(&retried, ctr + 3, 4, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipStreamSynchronize(s));
+	}
+	if (retried > 0) {
+		if (launch_find_cells(2, h->poly, prd, 4, s, v, n, dq, h->fq_retry2.as<int>(), ctr + 3, dfound, dpid, dpos,
+				      nullptr, ctr + 5, ctr + 4, h->ws_huge.base(), h->ws_huge.slab, h->ws_huge.stamp_base,
+				      touch))
+			return VGPU_HIP_ERROR;
+		h->ws_huge.stamp_base += (unsigned)n + 1u;
+		HIPCHK(hipMemcpyAsync(&failed, ctr + 5, 4, hipMemcpyDeviceToHost, s));
+	}
+	HIPCHK(hipStreamSynchronize(s));
+	return VGPU_OK;
+}
+
+// Copies the binned primary blocks to the host (the image creation reads
+// them there).
+int prd_host_blocks(vgpu_container *h, hipStream_t s) {
+	const int nc = h->ncells, nb = h->nblocks;
+	h->hP.resize(std::max(nc, 1)); h->hpid.resize(std::max(nc, 1)); h->hcs.resize(nb + 1);
+	if (nc) {
+		HIPCHK(hipMemcpyAsync(h->hP.data(), h->P.p, (size_t)nc * sizeof(dbl4), hipMemcpyDeviceToHost, s));
+		HIPCHK(hipMemcpyAsync(h->hpid.data(), h->pid.p, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
+	}
+	HIPCHK(hipMemcpyAsync(h->hcs.data(), h->bstart.p, (size_t)(nb + 1) * 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
+	return VGPU_OK;
+}
+
+// find_voronoi_cell on a periodic container.  The reference's queries create
+// the image blocks they reach, one query after another, on top of those the
+// last compute_all_cells (and earlier queries) created; the particle order
+// inside an image block is its order of creation.  As for the cells
+// (run_prd_cells), the order is found by a fixed point: the queries run
+// against the blocks created so far plus those near the primary domain,
+// recording first visits; the blocks they reached are then created in the
+// order of those visits and the queries that read an image block run again,
+// until the order reproduces itself.
+int find_prd(vgpu_container *h, int n, const double *dq, int *dfound, int *dpid, double *dpos, hipStream_t s,
+	     int &failed) {
+	const PrdGeom &g = h->pg;
+	const int nb = h->nblocks, nc = h->ncells;
+	const size_t N = (size_t)n;
+	int st;
+	if ((st = prd_host_blocks(h, s))) return st;
+	std::vector<unsigned char> prior(nb, 0);
+	for (int b : h->prd_order) prior[b] = 1;
+	std::vector<int> order = h->prd_order;
+	for (int b : prd_margin_images(g, PRD_FIRST_MARGIN))
+		if (!prior[b]) order.push_back(b);
+	if ((st = h->fv.ensure((size_t)nb * 8)) || (st = h->fq_touch.ensure(N)) || (st = h->fq_list.ensure(N * 4)) ||
+	    (st = h->d_counters.ensure(64 * 4)))
+		return st;
+	RectView v = make_view(h);
+	unsigned long long *fv = h->fv.as<unsigned long long>();
+	unsigned char *touch = h->fq_touch.as<unsigned char>();
+	int *lc = h->d_counters.as<int>() + 8;
+	const int *list = nullptr, *list_count = nullptr;
+	std::vector<unsigned long long> keys(nb);
+	std::vector<unsigned char> t(N);
+	for (int pass = 0; pass < 8; pass++) {
+		ImageSet im(g, h->hP.data(), h->hpid.data(), h->hcs.data());
+		for (int b : order) im.create_block(b);
+		if (im.late) return VGPU_INTERNAL_ERROR;
+		if ((st = upload_blocks(h, im, nc, v, s))) return st;
+		v.fv = fv;
+		hipLaunchKernelGGL(k_fill_u64, dim3(nblk(nb, 256)), dim3(256), 0, s, fv, nb, ~0ull);
+		hipLaunchKernelGGL(k_fill_u8, dim3(nblk(n, 256)), dim3(256), 0, s, touch, n, 0);
+		if ((st = find_tiers(h, v, n, dq, list, list_count, dfound, dpid, dpos, touch, s, failed))) return st;
+		HIPCHK(hipMemcpyAsync(keys.data(), fv, (size_t)nb * 8, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipMemcpyAsync(t.data(), touch, N, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipStreamSynchronize(s));
+		std::vector<int> rerun;
+		for (int q = 0; q < n; q++)
+			if (t[q]) rerun.push_back(q);
+		if (rerun.empty()) return VGPU_OK;   // no query read an image block: none is created
+		std::vector<int> next = h->prd_order;
+		for (int b : prd_visit_order(g, keys.data()))
+			if (!prior[b]) next.push_back(b);
+		if (next == order) {
+			h->prd_order.swap(next);
+			return VGPU_OK;
+		}
+		order.swap(next);
+		const int m = (int)rerun.size();
+		HIPCHK(hipMemcpyAsync(h->fq_list.p, rerun.data(), (size_t)m * 4, hipMemcpyHostToDevice, s));
+		HIPCHK(hipMemcpyAsync(lc, &m, 4, hipMemcpyHostToDevice, s));
+		list = h->fq_list.as<int>();
+		list_count = lc;
+	}
+	return VGPU_INTERNAL_ERROR;   // the creation order did not settle
+}
+
+}  // namespace
+
 int vgpu_find_voronoi_cells(vgpu_container *h, int n, const double *xyz, int *found, int *pid, double *pos,
 			    int on_device) {
 	if (!h || n < 0 || (n > 0 && (!xyz || !found || !pid || !pos))) return VGPU_INVALID_ARGUMENT;
-	if (h->prd) return VGPU_INVALID_ARGUMENT;       // rectangular containers only
 	if (n == 0) return VGPU_OK;
 	hipStream_t s = nullptr;
 	int st;
@@ -1420,61 +1574,16 @@ int vgpu_find_voronoi_cells(vgpu_container *h, int n, const double *xyz, int *fo
 		dq = h->fq_xyz.as<double>(); dfound = h->fq_found.as<int>(); dpid = h->fq_pid.as<int>();
 		dpos = h->fq_pos.as<double>();
 	}
-	if ((st = h->fq_retry.ensure(N * 4)) || (st = h->fq_retry2.ensure(N * 4)) || (st = h->d_counters.ensure(64 * 4)))
-		return st;
-	typedef Layout<TierWide, 64> LW;
-	typedef Layout<TierHuge, 64> LH;
-	const int w0 = std::max(1, std::min(default_fast_waves(2), nblk((long long)n, 64)));
-	if ((st = h->ws_fast.ensure(VCell3::L::bytes, std::max(w0, h->ws_fast.waves))) ||
-	    (st = h->ws_wide.ensure(LW::bytes, std::max(64, h->ws_wide.waves))) ||
-	    (st = h->ws_huge.ensure(LH::bytes, std::max(4, h->ws_huge.waves))))
-		return st;
-	Workspace *wss[3] = {&h->ws_fast, &h->ws_wide, &h->ws_huge};
-	for (Workspace *x : wss)
-		if ((unsigned long long)x->stamp_base + N + 2 >= 0xffffffffull) {
-			HIPCHK(hipMemset(x->buf.p, 0, x->buf.n));
-			x->stamp_base = 0;
-		}
-	// [0] / [2] / [4] query counters of the three tiers, [1] / [3] their retry
-	// counts, [5] queries no window held
-	int *ctr = h->d_counters.as<int>();
-	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 8, 0);
-	const RectView v = make_view(h);
-	if (launch_find_cells(0, h->poly, w0, s, v, n, dq, nullptr, nullptr, dfound, dpid, dpos, h->fq_retry.as<int>(),
-			      ctr + 1, ctr, h->ws_fast.base(), h->ws_fast.slab, h->ws_fast.stamp_base))
-		return VGPU_HIP_ERROR;
-	h->ws_fast.stamp_base += (unsigned)n + 1u;
-	// the few searches that left the order-3 window, with the wide one, then
-	// the huge one (each launched only if it has queries)
-	int retried = 0;
-	HIPCHK(hipMemcpyAsync(&retried, ctr + 1, 4, hipMemcpyDeviceToHost, s));
-	HIPCHK(hipStreamSynchronize(s));
-	if (retried > 0) {
-		if (launch_find_cells(1, h->poly, 64, s, v, n, dq, h->fq_retry.as<int>(), ctr + 1, dfound, dpid, dpos,
-				      h->fq_retry2.as<int>(), ctr + 3, ctr + 2, h->ws_wide.base(), h->ws_wide.slab,
-				      h->ws_wide.stamp_base))
-			return VGPU_HIP_ERROR;
-		h->ws_wide.stamp_base += (unsigned)n + 1u;
-		HIPCHK(hipMemcpyAsync(&retried, ctr + 3, 4, hipMemcpyDeviceToHost, s));
-		HIPCHK(hipStreamSynchronize(s));
-	}
-	if (retried > 0) {
-		if (launch_find_cells(2, h->poly, 4, s, v, n, dq, h->fq_retry2.as<int>(), ctr + 3, dfound, dpid, dpos,
-				      nullptr, ctr + 5, ctr + 4, h->ws_huge.base(), h->ws_huge.slab, h->ws_huge.stamp_base))
-			return VGPU_HIP_ERROR;
-		h->ws_huge.stamp_base += (unsigned)n + 1u;
-	}
+	int failed = 0;
+	if (h->prd) st = find_prd(h, n, dq, dfound, dpid, dpos, s, failed);
+	else st = find_tiers(h, make_view(h), n, dq, nullptr, nullptr, dfound, dpid, dpos, nullptr, s, failed);
+	if (st) return st;
 	if (!on_device) {
 		HIPCHK(hipMemcpyAsync(found, dfound, N * 4, hipMemcpyDeviceToHost, s));
 		HIPCHK(hipMemcpyAsync(pid, dpid, N * 4, hipMemcpyDeviceToHost, s));
 		HIPCHK(hipMemcpyAsync(pos, dpos, N * 24, hipMemcpyDeviceToHost, s));
 	}
 	HIPCHK(hipStreamSynchronize(s));
-	int failed = 0;
-	if (retried > 0) {
-		HIPCHK(hipMemcpyAsync(&failed, ctr + 5, 4, hipMemcpyDeviceToHost, s));
-		HIPCHK(hipStreamSynchronize(s));
-	}
 	return failed > 0 ? VGPU_MEMORY_ERROR : VGPU_OK;
 }
 
@@ -1483,12 +1592,29 @@ int vgpu_compute_ghost_cells(vgpu_container *h, int n, const double *xyz, const 
 	if (!h || n < 0 || max_nbr < 1 || (n > 0 && (!xyz || !exists || !vol || !cent || !nfaces || !nbr)) ||
 	    (h->poly && n > 0 && !r))
 		return VGPU_INVALID_ARGUMENT;
-	if (h->prd) return VGPU_INVALID_ARGUMENT;       // rectangular containers only
 	if (n == 0) return VGPU_OK;
 	hipStream_t s = nullptr;
 	int st;
 	if (!h->binned && (st = run_pipeline(h, 0, s, true))) return st;
 	const size_t N = (size_t)n;
+	RectView v = make_view(h);
+	if (h->prd) {
+		// The block arrays as the reference holds them before the call: the
+		// primary blocks and the image blocks created by the last compute and
+		// queries.  A ghost point's search that needs another image block is
+		// given up: the reference would create that block during the call and
+		// copy the point itself into it, with an id it never sets.
+		if ((st = prd_host_blocks(h, s))) return st;
+		const int nb = h->nblocks;
+		ImageSet im(h->pg, h->hP.data(), h->hpid.data(), h->hcs.data());
+		std::vector<unsigned char> ok(nb, 0);
+		for (int b = 0; b < nb; b++) ok[b] = h->pg.primary(b) ? 1 : 0;
+		for (int b : h->prd_order) { im.create_block(b); ok[b] = 1; }
+		if (im.late) return VGPU_INTERNAL_ERROR;
+		if ((st = upload_blocks(h, im, h->ncells, v, s)) || (st = h->img_okb.ensure((size_t)nb))) return st;
+		HIPCHK(hipMemcpy(h->img_okb.p, ok.data(), (size_t)nb, hipMemcpyHostToDevice));
+		v.img_ok = h->img_okb.as<unsigned char>();
+	}
 	if ((st = h->fq_xyz.ensure(N * 24)) || (st = h->fq_r.ensure(N * 8)) || (st = h->fq_found.ensure(N * 4)) ||
 	    (st = h->fq_vol.ensure(N * 8)) || (st = h->fq_pos.ensure(N * 24)) || (st = h->fq_pid.ensure(N * 4)) ||
 	    (st = h->fq_nbr.ensure(N * (size_t)max_nbr * 4)) || (st = h->d_counters.ensure(64 * 4)))
@@ -1507,8 +1633,7 @@ int vgpu_compute_ghost_cells(vgpu_container *h, int n, const double *xyz, const 
 	// tier's retries, [3] points no tier finished
 	int *ctr = h->d_counters.as<int>();
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 8, 0);
-	const RectView v = make_view(h);
-	if (launch_ghost_cells(0, h->poly, waves, s, v, h->g, n, h->fq_xyz.as<double>(), h->fq_r.as<double>(), nullptr,
+	if (launch_ghost_cells(0, h->poly, h->prd, waves, s, v, h->g, n, h->fq_xyz.as<double>(), h->fq_r.as<double>(), nullptr,
 			       nullptr, h->fq_found.as<int>(), h->fq_vol.as<double>(), h->fq_pos.as<double>(),
 			       h->fq_pid.as<int>(), h->fq_nbr.as<int>(), max_nbr, ctr, h->fq_retry.as<int>(), ctr + 1,
 			       h->ws_wide.base(), h->ws_wide.slab, h->ws_wide.stamp_base))
@@ -1526,7 +1651,7 @@ int vgpu_compute_ghost_cells(vgpu_container *h, int n, const double *xyz, const 
 			HIPCHK(hipMemset(h->ws_huge.buf.p, 0, h->ws_huge.buf.n));
 			h->ws_huge.stamp_base = 0;
 		}
-		if (launch_ghost_cells(1, h->poly, hw, s, v, h->g, n, h->fq_xyz.as<double>(), h->fq_r.as<double>(),
+		if (launch_ghost_cells(1, h->poly, h->prd, hw, s, v, h->g, n, h->fq_xyz.as<double>(), h->fq_r.as<double>(),
 				       h->fq_retry.as<int>(), ctr + 1, h->fq_found.as<int>(), h->fq_vol.as<double>(),
 				       h->fq_pos.as<double>(), h->fq_pid.as<int>(), h->fq_nbr.as<int>(), max_nbr, ctr + 2,
 				       nullptr, ctr + 3, h->ws_huge.base(), h->ws_huge.slab, h->ws_huge.stamp_base))

@@ -183,19 +183,24 @@ int launch_cells_huge(bool poly, int occ, int waves, hipStream_t s, const RectVi
 // out.retry.
 // find_voronoi_cell for n query points (find_cell.hip): tiers 0, 1, 2 search
 // with the order-3, wide and huge cells' mask windows and send queries that
-// leave the window to `retry` (none: found[q] = -1).
-int launch_find_cells(int tier, bool poly, int waves, hipStream_t s, const RectView &v, int n, const double *xyz,
-		      const int *list, const int *list_count, int *found, int *pid, double *pos, int *retry,
-		      int *retry_count, int *counter, char *ws, size_t slab, unsigned stamp_base);
+// leave the window to `retry` (none: found[q] = -1).  prd: a periodic
+// container (touch[q] = 1 when query q read an image block; first visits in
+// v.fv).
+int launch_find_cells(int tier, bool poly, bool prd, int waves, hipStream_t s, const RectView &v, int n,
+		      const double *xyz, const int *list, const int *list_count, int *found, int *pid, double *pos,
+		      int *retry, int *retry_count, int *counter, char *ws, size_t slab, unsigned stamp_base,
+		      unsigned char *touch);
 struct RectGeom;
 // compute_ghost_cell for n points (or the list of them counted at
 // *list_count) (find_cell.hip), general per-lane cells: tier 0 the wide
 // cell (points it cannot finish go to retry, counted at *failed), tier 1 the
 // huge one (retry null: *failed counts the points it cannot finish).
-int launch_ghost_cells(int tier, bool poly, int waves, hipStream_t s, const RectView &v, const RectGeom &geo, int n,
-		       const double *xyz, const double *rad, const int *list, const int *list_count, int *exists,
-		       double *vol, double *cent, int *nfaces, int *nbr, int max_nbr, int *counter, int *retry, int *failed,
-		       char *ws, size_t slab, unsigned stamp_base);
+// prd: a periodic container (the searches read v's block arrays and stop at
+// the blocks v.img_ok excludes).
+int launch_ghost_cells(int tier, bool poly, bool prd, int waves, hipStream_t s, const RectView &v,
+		       const RectGeom &geo, int n, const double *xyz, const double *rad, const int *list,
+		       const int *list_count, int *exists, double *vol, double *cent, int *nfaces, int *nbr, int max_nbr,
+		       int *counter, int *retry, int *failed, char *ws, size_t slab, unsigned stamp_base);
 int launch_cells_wave(int k, bool poly, bool prd, int occ, int waves, hipStream_t s, const RectView &v, int ncells,
 		      const int *list, const int *list_count, const int *cellblk, int *counters, char *ws,
 		      unsigned stamp_base, int chunk, const CellOut &out);
