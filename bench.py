@@ -146,13 +146,21 @@ def main():
     con.set_first_tier(1)
     dt_lane, lane_tier_ms, _ = timed(flags)
     con.set_first_tier(0)
+    # the project's autotuner picks the plain path's launch configuration on
+    # this container (untimed setup), then the learned path's depth margin
+    wl = "uniform %d particles, %d^3 blocks" % (args.n, G)
+    tune_plain = con.autotune(flags, reps=2, config=wl)
     dt_plain, plain_tier_ms, _ = timed(flags)
+    plain_occ = tune_plain["best"]["occupancy"]
+    plain_kernel = ("k_wcells<1, mono, %d>" % plain_occ if tune_plain["best"]["first_tier"] == "wavefront"
+                    else "k_cells<Tier3, mono, %d>" % plain_occ)
     r = con.results()
     ncells = len(r["id"])
     nnbr = len(r["nbr"])
     plain_cells_s = ncells * args.steps / dt_plain
     plain_ref = (r["vol"].copy(), r["nbr"].copy())
     con.set_predictor(pred)
+    tune_learned = con.autotune(flags | voro_amd.F_LEARNED, reps=2, config=wl + ", learned")
     dt, tier_ms, gather_ms = timed(flags | voro_amd.F_LEARNED)
     ls = con.learned_stats()
     r = con.results()
@@ -205,15 +213,15 @@ def main():
                    "n_particles": int(args.n), "grid": [G, G, G], "path": "learned",
                    "fast_occupancy": args.occ, "fast_waves": args.waves},
         "plain": {"value": plain_cells_s, "ms_per_step": dt_plain / args.steps * 1e3, "tier_ms": plain_tier_ms,
-                  "first_tier": "one cell per wavefront (k_wcells<1>)",
+                  "first_tier": plain_kernel, "autotune": tune_plain,
                   "per_lane_first_tier": {"value": ncells * args.steps / dt_lane,
                                           "ms_per_step": dt_lane / args.steps * 1e3, "tier_ms": lane_tier_ms},
-                  "roofline_fp64": {"kernel": "k_wcells<1, mono, 4>", "kernel_ms": plain_tier_ms,
+                  "roofline_fp64": {"kernel": plain_kernel, "kernel_ms": plain_tier_ms,
                                     "achieved": wave_fp64, "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
                                     "frac": wave_fp64 / MI355X_FP64_TFLOPS},
                   "roofline_hbm": {"achieved": bytes_launch / wave_s / 1e9, "peak": MI355X_HBM_GBS, "unit": "GB/s",
                                    "frac": bytes_launch / wave_s / 1e9 / MI355X_HBM_GBS, "traffic": wave_traffic}},
-        "learned": {"speedup_vs_plain": cells_s / plain_cells_s, "fallbacks": ls["fallbacks"],
+        "learned": {"speedup_vs_plain": cells_s / plain_cells_s, "autotune": tune_learned, "fallbacks": ls["fallbacks"],
                     "fallback_rate": ls["fallbacks"] / max(ncells, 1), "list_entries": ls["list_entries"],
                     "gather_ms": gather_ms, "tier_ms": tier_ms, "identical_to_plain": identical,
                     "train_seconds": t_train, "train_loss": train_loss, "train_seeds": list(args.train_seeds)},

@@ -242,6 +242,27 @@ int vgpu_set_launch(vgpu_container *h, int fast_waves, int fast_occupancy);
  * register budget in waves per SIMD (2, 3, 4 or 8; 0 = keep). */
 int vgpu_set_first_tier(vgpu_container *h, int tier, int waves, int occupancy);
 
+/* Autotuner for one container (no counterpart in the reference): times
+ * vgpu_compute_all(h, flags) `reps` times for each launch configuration and
+ * keeps the fastest on the handle.  The configurations are the first tier's
+ * cells per wavefront (one wavefront cell, vertex tables in registers, at 3,
+ * 4, 5 or 6 waves per SIMD -- periodic containers: 4 -- or 64 order-3 cells
+ * per wavefront, one per lane, at 2, 3 or 4) and, with VGPU_F_LEARNED, the
+ * candidate-list length (the predictor's depth margin, -2 .. +4 worklist
+ * steps around the trained one).  Results do not depend on the choice.
+ * trials[0] is the chosen configuration, then the others in the order tried
+ * (up to max_trials); *n_trials = configurations tried.  flags may not hold
+ * VGPU_F_GEOMETRY. */
+typedef struct {
+	int first_tier;       /* 0 one cell per wavefront, 1 one cell per lane */
+	int occupancy;        /* register budget, waves per SIMD */
+	int waves;            /* persistent wavefronts */
+	float depth_offset;   /* learned path: added to the trained depth margin */
+	double ms;            /* fastest of the timed computes */
+} vgpu_tune_trial;
+int vgpu_autotune(vgpu_container *h, unsigned flags, int reps, vgpu_tune_trial *trials, int max_trials,
+		  int *n_trials);
+
 /* Time of the last vgpu_compute_all, split by stage, in milliseconds, from
  * HIP events: [0] binning, [1] fast tier, [2] wide+huge tiers, [3] outputs. */
 int vgpu_last_timings(vgpu_container *h, double *ms4);

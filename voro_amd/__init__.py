@@ -37,6 +37,12 @@ HIP_ERROR = 6
 DUPLICATE = 7
 NO_DEVICE = 8
 
+class TuneTrial(C.Structure):
+    """vgpu_tune_trial: one configuration the autotuner timed."""
+    _fields_ = [("first_tier", C.c_int), ("occupancy", C.c_int), ("waves", C.c_int),
+                ("depth_offset", C.c_float), ("ms", C.c_double)]
+
+
 F_NEIGHBORS = 1
 F_STATS = 2
 F_LEARNED = 4
@@ -119,6 +125,7 @@ def lib():
         "vgpu_batch_sum_volumes": (C.c_int, [p, f64p]),
         "vgpu_batch_config": (C.c_int, [p, i32p, i32p]),
         "vgpu_batch_autotune": (C.c_int, [p, C.c_uint, C.c_int, i32p, f64p]),
+        "vgpu_autotune": (C.c_int, [p, C.c_uint, C.c_int, C.POINTER(TuneTrial), C.c_int, i32p]),
         "vgpu_batch_last_timings": (C.c_int, [p, f64p]),
         "vgpu_mlp_num_params": (C.c_int, []),
         "vgpu_mlp_loss_grad": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int,
@@ -269,6 +276,26 @@ class _Base:
         a = np.zeros(4)
         _check(lib().vgpu_last_timings(self._h, _p(a, C.c_double)), "vgpu_last_timings")
         return {"bin_ms": a[0], "fast_ms": a[1], "retry_ms": a[2], "out_ms": a[3]}
+
+    def autotune(self, flags=F_NEIGHBORS | F_CSR, reps=2, path=None, config=None):
+        """The project's autotuner for this container (vgpu_autotune): times
+        each launch configuration -- the first tier's cells per wavefront
+        and register budget and, with F_LEARNED, the candidate-list length
+        (depth margin) -- on the particles put, and keeps the fastest.
+        Returns {"config", "best", "trials"}; with `path`, also writes that
+        table as JSON (`config` names the workload in it)."""
+        tr = (TuneTrial * 16)()
+        n = C.c_int()
+        _check(lib().vgpu_autotune(self._h, int(flags), int(reps), tr, 16, C.byref(n)), "vgpu_autotune")
+        rows = [{"first_tier": "wavefront" if t.first_tier == 0 else "lane", "cells_per_wavefront":
+                 1 if t.first_tier == 0 else 64, "occupancy": t.occupancy, "waves": t.waves,
+                 "depth_offset": t.depth_offset, "ms": t.ms} for t in tr[:min(n.value, 16)]]
+        out = {"config": config, "flags": int(flags), "reps": int(reps), "best": rows[0], "trials": rows}
+        if path is not None:
+            import json
+            with open(path, "w") as f:
+                json.dump(out, f, indent=1)
+        return out
 
     def results(self):
         """All stored results as numpy arrays in c_loop_all order."""

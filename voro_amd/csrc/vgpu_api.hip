@@ -2033,6 +2033,70 @@ int vgpu_set_first_tier(vgpu_container *h, int tier, int waves, int occupancy) {
 	return VGPU_OK;
 }
 
+int vgpu_autotune(vgpu_container *h, unsigned flags, int reps, vgpu_tune_trial *trials, int max_trials,
+		  int *n_trials) {
+	if (!h || reps < 1 || max_trials < 0 || (max_trials > 0 && !trials) || (flags & VGPU_F_GEOMETRY))
+		return VGPU_INVALID_ARGUMENT;
+	const bool learned = (flags & VGPU_F_LEARNED) != 0;
+	if (learned && !h->have_pred) return VGPU_INVALID_ARGUMENT;
+	// Candidates: the first tier (wavefront cells at each compiled register
+	// budget; periodic containers have one) or one order-3 cell per lane at
+	// each budget; with the learned path, the depth margin around the
+	// trained one as well.  Each is timed `reps` times (fastest kept).
+	std::vector<vgpu_tune_trial> cand;
+	const int wave_occs_rect[] = {3, 4, 5, 6};
+	const int lane_occs[] = {2, 3, 4};
+	if (h->prd) cand.push_back(vgpu_tune_trial{0, 4, 0, 0.f, 0.0});
+	else
+		for (int o : wave_occs_rect) cand.push_back(vgpu_tune_trial{0, o, 0, 0.f, 0.0});
+	if (!learned)
+		for (int o : lane_occs) cand.push_back(vgpu_tune_trial{1, o, 0, 0.f, 0.0});
+	if (learned) {
+		const float offs[] = {-2.f, -1.f, 1.f, 2.f, 4.f};
+		for (float d : offs) cand.push_back(vgpu_tune_trial{0, 4, 0, d, 0.0});
+	}
+	const int keep_tier = h->first_tier, keep_wocc = h->wave_occ, keep_focc = h->fast_occ;
+	const float keep_margin = h->pred_margin;
+	double best = 1e300;
+	int bi = -1;
+	for (size_t t = 0; t < cand.size(); t++) {
+		vgpu_tune_trial &c = cand[t];
+		h->first_tier = c.first_tier;
+		if (c.first_tier == 0) h->wave_occ = c.occupancy;
+		else h->fast_occ = c.occupancy;
+		h->pred_margin = std::max(0.f, keep_margin + c.depth_offset);
+		c.depth_offset = h->pred_margin - keep_margin;
+		c.waves = c.first_tier == 0 ? (h->wave_waves > 0 ? h->wave_waves : default_fast_waves(c.occupancy))
+					    : (h->fast_waves > 0 ? h->fast_waves : default_fast_waves(c.occupancy));
+		double ms = 1e300;
+		for (int r = 0; r < reps; r++) {
+			const int st = vgpu_compute_all(h, flags, nullptr);
+			if (st) {
+				h->first_tier = keep_tier; h->wave_occ = keep_wocc; h->fast_occ = keep_focc;
+				h->pred_margin = keep_margin;
+				return st;
+			}
+			double tot = 0;
+			for (int k = 0; k < 4; k++) tot += h->timings[k];
+			ms = std::min(ms, tot);
+		}
+		c.ms = ms;
+		if (ms < best) { best = ms; bi = (int)t; }
+	}
+	const vgpu_tune_trial &b = cand[bi];
+	h->first_tier = b.first_tier;
+	h->wave_occ = b.first_tier == 0 ? b.occupancy : keep_wocc;
+	h->fast_occ = b.first_tier == 1 ? b.occupancy : keep_focc;
+	h->pred_margin = keep_margin + b.depth_offset;
+	// the best first, then the rest in the order tried
+	int m = 0;
+	if (max_trials > 0) trials[m++] = b;
+	for (size_t t = 0; t < cand.size() && m < max_trials; t++)
+		if ((int)t != bi) trials[m++] = cand[t];
+	if (n_trials) *n_trials = (int)cand.size();
+	return VGPU_OK;
+}
+
 int vgpu_last_timings(vgpu_container *h, double *ms4) {
 	if (!h || !ms4) return VGPU_INVALID_ARGUMENT;
 	for (int t = 0; t < 4; t++) ms4[t] = h->timings[t];
