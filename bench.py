@@ -370,6 +370,7 @@ def periodic_poly(out, flags, torch, steps):
         return (time.perf_counter() - t0) / steps
 
     con = make(42)
+    tune_plain = con.autotune(flags, reps=1, config="periodic radical 2M")
     dt = timed(con, flags)
     res = con.results()
     n = len(res["id"])
@@ -385,13 +386,15 @@ def periodic_poly(out, flags, torch, steps):
     loss = pred.train(epochs=300, lr=3e-3, seed=1, fallback=0.02)
     t_train = time.perf_counter() - t0
     con.set_predictor(pred)
+    tune_learned = con.autotune(flags | voro_amd.F_LEARNED, reps=1, config="periodic radical 2M, learned")
     dtl = timed(con, flags | voro_amd.F_LEARNED)
     r = con.results()
     ls = con.learned_stats()
     out["periodic_poly_2m"] = {"cells_per_s": n / dt, "ms": dt * 1e3,
                                "volume_sum_rel_err": abs(float(res["vol"].sum()) - 216000.0) / 216000.0,
-                               "timings_ms": plain_ms,
+                               "timings_ms": plain_ms, "autotune_best": tune_plain["best"],
                                "learned": {
+                                   "autotune_best": tune_learned["best"],
                                    "cells_per_s": n / dtl, "ms": dtl * 1e3, "speedup_vs_plain": dt / dtl,
                                    "fallbacks": ls["fallbacks"], "fallback_rate": ls["fallbacks"] / n,
                                    "given_no_list": ls["routed"], "list_entries": ls["list_entries"],
@@ -489,18 +492,25 @@ def clusters_learned(out, flags, torch):
     loss = pred.train(epochs=400, lr=3e-3, seed=1, fallback=0.02)
     t_train = time.perf_counter() - t0
     con = make(101)
-    con.compute_all_cells(flags)
-    t0 = time.perf_counter()
-    con.compute_all_cells(flags)
-    dt_plain = time.perf_counter() - t0
+    reps = 3
+
+    def timed(fl):
+        con.compute_all_cells(fl)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            con.compute_all_cells(fl)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    tune_plain = con.autotune(flags, reps=1, config="clustered 1M")
+    dt_plain = timed(flags)
     plain_ms = {k: float(v) for k, v in con.timings().items()}
     ref = con.results()
     ref = (ref["vol"].copy(), ref["nbr"].copy())
     con.set_predictor(pred)
-    con.compute_all_cells(flags | voro_amd.F_LEARNED)
-    t0 = time.perf_counter()
-    con.compute_all_cells(flags | voro_amd.F_LEARNED)
-    dt = time.perf_counter() - t0
+    tune_learned = con.autotune(flags | voro_amd.F_LEARNED, reps=1, config="clustered 1M, learned")
+    dt = timed(flags | voro_amd.F_LEARNED)
     r = con.results()
     n = len(r["id"])
     ls = con.learned_stats()
@@ -509,6 +519,7 @@ def clusters_learned(out, flags, torch):
         "fallbacks": ls["fallbacks"], "fallback_rate": ls["fallbacks"] / n, "train_seeds": list(range(1, 11)),
         "eval_seed": 101, "train_seconds": t_train, "train_loss": loss, "plain_timings_ms": plain_ms,
         "learned_timings_ms": {k: float(v) for k, v in con.timings().items()},
+        "autotune_best": {"plain": tune_plain["best"], "learned": tune_learned["best"]},
         "identical_to_plain": bool(np.array_equal(r["vol"], ref[0]) and np.array_equal(r["nbr"], ref[1]))}
 
 

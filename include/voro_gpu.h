@@ -239,17 +239,20 @@ int vgpu_set_launch(vgpu_container *h, int fast_waves, int fast_occupancy);
  * lane.  Cells the first tier cannot finish go on to the per-lane tiers, so
  * results do not depend on the choice.  waves: persistent wavefronts of the
  * wavefront tier (0 = 4 x occupancy x compute units); occupancy: its
- * register budget in waves per SIMD (2, 3, 4 or 8; 0 = keep). */
+ * register budget in waves per SIMD (2, 3, 4, 5, 6 or 8, default 5, the
+ * learned path's list kernel 4, 5 or 6; periodic containers 4, 5 or 6 and
+ * their list kernel 4; 0 = keep). */
 int vgpu_set_first_tier(vgpu_container *h, int tier, int waves, int occupancy);
 
 /* Autotuner for one container (no counterpart in the reference): times
  * vgpu_compute_all(h, flags) `reps` times for each launch configuration and
  * keeps the fastest on the handle.  The configurations are the first tier's
  * cells per wavefront (one wavefront cell, vertex tables in registers, at 3,
- * 4, 5 or 6 waves per SIMD -- periodic containers: 4 -- or 64 order-3 cells
+ * 4, 5 or 6 waves per SIMD -- periodic containers: 4, 5 or 6 -- or 64 order-3 cells
  * per wavefront, one per lane, at 2, 3 or 4) and, with VGPU_F_LEARNED, the
  * candidate-list length (the predictor's depth margin, -2 .. +4 worklist
- * steps around the trained one).  Results do not depend on the choice.
+ * steps around the trained one, at the best register budget).  Results do
+ * not depend on the choice.
  * trials[0] is the chosen configuration, then the others in the order tried
  * (up to max_trials); *n_trials = configurations tried.  flags may not hold
  * VGPU_F_GEOMETRY. */

@@ -435,14 +435,17 @@ int launch_batch_wcells_list(bool poly, int waves, int chunk, hipStream_t s, con
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 
-int launch_cells_wlist(bool poly, bool prd, int waves, int chunk, hipStream_t s, const RectView &v, int nc,
+int launch_cells_wlist(bool poly, bool prd, int occ, int waves, int chunk, hipStream_t s, const RectView &v, int nc,
 		       const int *cellblk, const int *order, const int *cnt, const int *boff, const ListRec *LA,
 		       int *counters, char *ws, unsigned stamp_base, int *fallbacks, int *routed, const CellOut &out) {
-#define VG_WLIST(P, R)                                                                                              \
-	hipLaunchKernelGGL((k_wcells_list<P, R, 4>), dim3(waves), dim3(64), 0, s, v, nc, cellblk, order, cnt, boff, LA, \
+#define VG_WLIST(P, R, O)                                                                                           \
+	hipLaunchKernelGGL((k_wcells_list<P, R, O>), dim3(waves), dim3(64), 0, s, v, nc, cellblk, order, cnt, boff, LA, \
 			   counters, ws, stamp_base, chunk, fallbacks, routed, out)
-	if (prd) { if (poly) VG_WLIST(true, true); else VG_WLIST(false, true); }
-	else { if (poly) VG_WLIST(true, false); else VG_WLIST(false, false); }
+	// 4, 5 or 6 waves per SIMD for rectangular containers, 4 for periodic ones
+	if (prd) { if (poly) VG_WLIST(true, true, 4); else VG_WLIST(false, true, 4); }
+	else if (occ == 5) { if (poly) VG_WLIST(true, false, 5); else VG_WLIST(false, false, 5); }
+	else if (occ >= 6) { if (poly) VG_WLIST(true, false, 6); else VG_WLIST(false, false, 6); }
+	else { if (poly) VG_WLIST(true, false, 4); else VG_WLIST(false, false, 4); }
 #undef VG_WLIST
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }

@@ -130,8 +130,17 @@ int launch_cells_wave(int k, bool poly, bool prd, int occ, int waves, hipStream_
 			else VGPU_WAVE_LAUNCH(2, false, 2, false);
 		}
 	} else if (prd) {
-		if (poly) VGPU_WAVE_LAUNCH(1, true, 4, true);
-		else VGPU_WAVE_LAUNCH(1, false, 4, true);
+		// periodic containers: 4, 5 or 6 waves per SIMD
+		if (occ == 5) {
+			if (poly) VGPU_WAVE_LAUNCH(1, true, 5, true);
+			else VGPU_WAVE_LAUNCH(1, false, 5, true);
+		} else if (occ >= 6) {
+			if (poly) VGPU_WAVE_LAUNCH(1, true, 6, true);
+			else VGPU_WAVE_LAUNCH(1, false, 6, true);
+		} else {
+			if (poly) VGPU_WAVE_LAUNCH(1, true, 4, true);
+			else VGPU_WAVE_LAUNCH(1, false, 4, true);
+		}
 	} else if (occ >= 8) {
 		if (poly) VGPU_WAVE_LAUNCH(1, true, 8, false);
 		else VGPU_WAVE_LAUNCH(1, false, 8, false);

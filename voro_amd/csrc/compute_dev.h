@@ -416,7 +416,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 			     double ghost_maxr = 0) {
 	double x, y, z, qx = 0, qy = 0, qz = 0;
 	int i, j, k, di, dj, dk, ei, ej, ek, f = 0, g = 0, disp = 0;
-	double fx = 0, fy = 0, fz = 0, gxs = 0, gys = 0, gzs = 0;
+	double fx0 = 0, fy0 = 0, fz0 = 0, gxs0 = 0, gys0 = 0, gzs0 = 0;
 	unsigned q, m1 = 0, m2 = 0;
 	unsigned long long key = key0;
 	stt.n_planes = 0; stt.wl_exit = 0; stt.phase = 1; stt.img = 0;
@@ -456,7 +456,24 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 	bool marking = false;
 	int bt = 0, be = 0;           // particle range of the block being cut
 	bool bcheck = false;          // per-particle radius check for this block
-	double x2 = 0, y2 = 0, z2 = 0;
+	// the particle shifted by the current block's periodic image (the
+	// particle itself until the worklist starts)
+	double x2 = x, y2 = y, z2 = z;
+	// The particle's own position is read again where it is needed after
+	// the cuts have started (block changes, worklist selection) instead of
+	// being held across them: wavefront cells run short of scalar registers.
+	const int me_idx = b0 + s;
+	auto reload = [&]() {
+		if constexpr (LaneSteps<CELL>::value) {
+			if (ghost) { x = ghost->x; y = ghost->y; z = ghost->z; }
+			else {
+				int q_ = me_idx;
+				asm volatile("" : "+s"(q_));
+				const dbl4 m_ = v.P[q_];
+				x = m_.x; y = m_.y; z = m_.z;
+			}
+		}
+	};
 
 	// Upper bound on the cell's current max_radius_squared() (the cell only
 	// shrinks), refreshed with the reference's mrs and after every 16
@@ -474,6 +491,19 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 	unsigned long long cm = 0;
 	int cbase = 0;
 
+	// frac_pos and the squared distances to the far block faces
+	// (src/v_compute.cc:328-362), recomputed from the particle where the
+	// wavefront cell needs them rather than held across the cuts.
+	auto frac = [&](double &fx_, double &fy_, double &fz_, double &gx_, double &gy_, double &gz_) {
+		reload();
+		if (PRD) { fx_ = x - v.boxx * ci; fy_ = y - v.boxy * (cj - v.ey); fz_ = z - v.boxz * (ck - v.ez); }
+		else { fx_ = x - v.ax - v.boxx * ci; fy_ = y - v.ay - v.boxy * cj; fz_ = z - v.az - v.boxz * ck; }
+		gx_ = (m1 & 127u) ? fx_ : v.boxx - fx_;
+		gy_ = (m1 & (127u << 7)) ? fy_ : v.boxy - fy_;
+		gz_ = (m1 & (127u << 14)) ? fz_ : v.boxz - fz_;
+		gx_ *= gx_; gy_ *= gy_; gz_ *= gz_;
+	};
+	constexpr bool refrac = LaneSteps<CELL>::value;
 	for (;;) {
 		double px_ = 0, py_ = 0, pz_ = 0, rs = 0;
 		double rs_raw = -1;           // squared distance of a particle candidate (walls: -1)
@@ -531,7 +561,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				if (w < v.nwalls) {
 					const double *wp = v.walls + VGPU_WALL_STRIDE * w;
 					w++;
-					have = wall_cut_plane(wp, x, y, z, px_, py_, pz_, rs, pid);
+					have = wall_cut_plane(wp, x2, y2, z2, px_, py_, pz_, rs, pid);
 					continue;
 				}
 				stage = 1;
@@ -545,7 +575,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 					dbl4 o;
 					cf.part(v, b0 + t, o, pid);
 					t++;
-					px_ = o.x - x; py_ = o.y - y; pz_ = o.z - z;
+					px_ = o.x - x2; py_ = o.y - y2; pz_ = o.z - z2;
 					rs_raw = px_ * px_ + py_ * py_ + pz_ * pz_;
 					rs = rp.scale(rs_raw, o.w);
 					have = true;
@@ -554,6 +584,8 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				// Worklist selection (src/v_compute.cc:328-362).
 				mrs = c.max_radius_squared();
 				mrs_b = mrs; since = 0;
+				double fx, fy, fz, gxs, gys, gzs;
+				reload();
 				if (PRD) { fx = x - v.boxx * ci; fy = y - v.boxy * (cj - v.ey); fz = z - v.boxz * (ck - v.ez); }
 				else { fx = x - v.ax - v.boxx * ci; fy = y - v.ay - v.boxy * cj; fz = z - v.az - v.boxz * ck; }
 				di = int(fx * v.xsp * VGPU_WL_FGRID); dj = int(fy * v.ysp * VGPU_WL_FGRID); dk = int(fz * v.zsp * VGPU_WL_FGRID);
@@ -570,6 +602,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 					m1 |= (127 << 14) + (3 << 27); m2 |= (1 << 14) + (1 << 27); dk = VGPU_WL_FGRID - 1 - dk; if (dk < 0) dk = 0;
 				} else gzs = v.boxz - fz;
 				gxs *= gxs; gys *= gys; gzs *= gzs;
+				if constexpr (!refrac) { fx0 = fx; fy0 = fy; fz0 = fz; gxs0 = gxs; gys0 = gys; gzs0 = gzs; }
 				const int wli = di + VGPU_WL_HGRID * (dj + VGPU_WL_HGRID * dk);
 				radp = v.mrad + wli * 64;
 				e = v.wl + wli * 64;
@@ -593,6 +626,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 					if (ghost && (rijk == ijk || (PRD && v.img_ok && !v.img_ok[rijk]))) { c.ovf = true; stt.n_planes = c.n_cuts; return false; }
 					cf.range(v, g1, rijk, bt, be);
 					cf.block(bt, be);
+					reload();
 					x2 = x - qx; y2 = y - qy; z2 = z - qz;
 					bcheck = ((bmask >> l) & 1ull) != 0;
 					continue;
@@ -620,6 +654,8 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				const int gE = (next_count > g && next_count < 63) ? next_count : 63;
 				const int n = gE - g;
 				const int lane = steps_lane();
+				double fx = fx0, fy = fy0, fz = fz0, gxs = gxs0, gys = gys0, gzs = gzs0;
+				if constexpr (refrac) frac(fx, fy, fz, gxs, gys, gzs);
 				const int gg = g + lane;
 				bool stop = false, acc = false, bc = false, mk = false;
 				if (lane < n) {
@@ -684,7 +720,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				ei = di + i; if (ei < 0 || ei >= v.hx) continue;
 				ej = dj + j; if (ej < 0 || ej >= v.hy) continue;
 				ek = dk + k; if (ek < 0 || ek >= v.hz) continue;
-				if (min_max_radius<POLY>(v, rp, di, dj, dk, fx, fy, fz, gxs, gys, gzs, crs, mrs)) continue;
+				if (min_max_radius<POLY>(v, rp, di, dj, dk, fx0, fy0, fz0, gxs0, gys0, gzs0, crs, mrs)) continue;
 				if (marking) p2_cut |= 1ull << g;
 				{
 					const int rijk = PRD ? prd_region(v, ci, cj, ck, ei, ej, ek, qx, key++, stt.img)
@@ -692,6 +728,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 					if (ghost && (rijk == ijk || (PRD && v.img_ok && !v.img_ok[rijk]))) { c.ovf = true; stt.n_planes = c.n_cuts; return false; }
 					cf.range(v, g, rijk, bt, be);
 					cf.block(bt, be);
+					reload();
 					x2 = x - qx; y2 = y - qy; z2 = z - qz;
 					bcheck = rp.ctest(crs, mrs);
 				}
@@ -731,6 +768,9 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 			// stage 3: breadth-first over queued blocks (src/v_compute.cc:534-618).
 			if (!bs.pop(ei, ej, ek)) { stt.n_planes = c.n_cuts; return true; }
 			{
+				double fx = fx0, fy = fy0, fz = fz0, gxs = gxs0, gys = gys0, gzs = gzs0;
+				if constexpr (refrac) frac(fx, fy, fz, gxs, gys, gzs);
+				(void)gxs; (void)gys; (void)gzs;
 				const double xlo = (ei - i) * v.boxx - fx, xhi = xlo + v.boxx;
 				const double ylo = (ej - j) * v.boxy - fy, yhi = ylo + v.boxy;
 				const double zlo = (ek - k) * v.boxz - fz, zhi = zlo + v.boxz;
@@ -745,6 +785,7 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				if (ghost && (rijk == ijk || (PRD && v.img_ok && !v.img_ok[rijk]))) { c.ovf = true; stt.n_planes = c.n_cuts; return false; }
 				bt = v.bstart[rijk]; be = v.bstart[rijk + 1];
 				cf.block(bt, be);
+				reload();
 				x2 = x - qx; y2 = y - qy; z2 = z - qz;
 				bcheck = false;
 			}
@@ -812,7 +853,7 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, const Li
 	unsigned long long key = key0;
 	const int b0 = v.bstart[ijk], co0 = v.bstart[ijk + 1] - b0;
 	const dbl4 me = v.P[b0 + s];
-	const double x = me.x, y = me.y, z = me.z;
+	double x = me.x, y = me.y, z = me.z;
 	if (PRD) {
 		c.init_unit(v.unit, v.unit_p);
 		if (c.ovf) { stt.n_planes = 0; return -2; }
@@ -829,7 +870,19 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, const Li
 	int next_count = 3, cpi = 0, g = 0, w = 0, t = 0, k = 0;
 	bool own_done = false, bcheck = false;
 	int bt = 0, be = 0;              // particle range of the block being cut
-	double x2 = 0, y2 = 0, z2 = 0;   // the cell's particle, shifted by the block's image
+	// the cell's particle, shifted by the block's image (the particle itself
+	// before the first list block); the particle is read again at each block
+	// change rather than held across the cuts (as in compute_cell_rect)
+	double x2 = x, y2 = y, z2 = z;
+	const int me_idx = b0 + s;
+	auto reload = [&]() {
+		if constexpr (LaneSteps<CELL>::value) {
+			int q_ = me_idx;
+			asm volatile("" : "+s"(q_));
+			const dbl4 m_ = v.P[q_];
+			x = m_.x; y = m_.y; z = m_.z;
+		}
+	};
 	// upper bound on max_radius_squared(), as in compute_cell_rect
 	double mrs_b = 1.7976931348623157e308;
 	int since = 0;
@@ -847,7 +900,7 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, const Li
 				// Walls in the order they were added (src/container.hh:86-89).
 				const double *wp = v.walls + VGPU_WALL_STRIDE * w;
 				w++;
-				have = wall_cut_plane(wp, x, y, z, px_, py_, pz_, rs, pid);
+				have = wall_cut_plane(wp, x2, y2, z2, px_, py_, pz_, rs, pid);
 				continue;
 			}
 			if constexpr (LaneSteps<CELL>::value) {
@@ -900,7 +953,7 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, const Li
 					const dbl4 o = v.P[b0 + t];
 					pid = v.pid[b0 + t];
 					t++;
-					px_ = o.x - x; py_ = o.y - y; pz_ = o.z - z;
+					px_ = o.x - x2; py_ = o.y - y2; pz_ = o.z - z2;
 					rs_raw = px_ * px_ + py_ * py_ + pz_ * pz_;
 					rs = rp.scale(rs_raw, o.w);
 					have = true;
@@ -947,6 +1000,7 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, const Li
 					qy = list_shift((a.info >> 6) & 3, v.by - v.ay);
 					qz = list_shift((a.info >> 8) & 3, v.bz - v.az);
 				}
+				reload();
 				x2 = x - qx; y2 = y - qy; z2 = z - qz;
 			}
 			bt = v.bstart[a.rijk];

@@ -101,7 +101,7 @@ struct vgpu_container {
 	int fast_occ = 2;
 	int first_tier = 0;      // 0: wavefront cells, 1: order-3 cells, one per lane
 	int wave_waves = 0;
-	int wave_occ = 4;
+	int wave_occ = 5;
 	hipEvent_t ev[5];
 	hipEvent_t ev_list = nullptr;   // learned tier: lists gathered, cell kernel starts
 	double gather_ms = 0;
@@ -279,7 +279,8 @@ int default_fast_waves(int occ) {
 // leftovers go to out.retry.
 int launch_wave(vgpu_container *h, int k, const RectView &v, int nc, const int *list, const int *list_count,
 		CellOut out, hipStream_t s) {
-	const int occ = k > 1 ? 2 : h->wave_occ;
+	// periodic containers: compiled for 4, 5 and 6 waves per SIMD
+	const int occ = k > 1 ? 2 : (h->prd ? std::min(6, std::max(4, h->wave_occ)) : h->wave_occ);
 	int waves = (h->wave_waves > 0 && k == 1) ? h->wave_waves : default_fast_waves(occ);
 	int cnt = nc;
 	if (list_count) {
@@ -394,7 +395,10 @@ int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut
 	out.tier = 0; out.retry = h->retry0.as<int>(); out.retry_count = ctr + 4;
 	if (h->first_tier == 0) {
 		// one cell per wavefront
-		int w = h->wave_waves > 0 ? h->wave_waves : default_fast_waves(4);
+		// the list kernel at the wavefront tier's register budget (4, 5 or
+		// 6 waves per SIMD; periodic containers 4)
+		const int locc = h->prd ? 4 : (h->wave_occ >= 6 ? 6 : (h->wave_occ == 5 ? 5 : 4));
+		int w = h->wave_waves > 0 ? h->wave_waves : default_fast_waves(locc);
 		const int chunk = nc >= 4 * w ? 4 : 1;
 		w = std::max(1, std::min(w, (nc + chunk - 1) / chunk));
 		Workspace &ws = h->ws_wave;
@@ -403,7 +407,7 @@ int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut
 			HIPCHK(hipMemset(ws.buf.p, 0, ws.buf.n));
 			ws.stamp_base = 0;
 		}
-		if (launch_cells_wlist(h->poly, h->prd, w, chunk, s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(),
+		if (launch_cells_wlist(h->poly, h->prd, locc, w, chunk, s, v, nc, h->cellblk.as<int>(), h->d_order.as<int>(),
 				       h->lcnt.as<int>(), h->boff.as<int>(), h->list_a.as<ListRec>(),
 				       h->d_counters.as<int>(), ws.base(), ws.stamp_base, ctr + 5, ctr + 9, out))
 			return VGPU_HIP_ERROR;
@@ -2039,28 +2043,12 @@ int vgpu_autotune(vgpu_container *h, unsigned flags, int reps, vgpu_tune_trial *
 		return VGPU_INVALID_ARGUMENT;
 	const bool learned = (flags & VGPU_F_LEARNED) != 0;
 	if (learned && !h->have_pred) return VGPU_INVALID_ARGUMENT;
-	// Candidates: the first tier (wavefront cells at each compiled register
-	// budget; periodic containers have one) or one order-3 cell per lane at
-	// each budget; with the learned path, the depth margin around the
-	// trained one as well.  Each is timed `reps` times (fastest kept).
-	std::vector<vgpu_tune_trial> cand;
-	const int wave_occs_rect[] = {3, 4, 5, 6};
-	const int lane_occs[] = {2, 3, 4};
-	if (h->prd) cand.push_back(vgpu_tune_trial{0, 4, 0, 0.f, 0.0});
-	else
-		for (int o : wave_occs_rect) cand.push_back(vgpu_tune_trial{0, o, 0, 0.f, 0.0});
-	if (!learned)
-		for (int o : lane_occs) cand.push_back(vgpu_tune_trial{1, o, 0, 0.f, 0.0});
-	if (learned) {
-		const float offs[] = {-2.f, -1.f, 1.f, 2.f, 4.f};
-		for (float d : offs) cand.push_back(vgpu_tune_trial{0, 4, 0, d, 0.0});
-	}
 	const int keep_tier = h->first_tier, keep_wocc = h->wave_occ, keep_focc = h->fast_occ;
 	const float keep_margin = h->pred_margin;
-	double best = 1e300;
-	int bi = -1;
-	for (size_t t = 0; t < cand.size(); t++) {
-		vgpu_tune_trial &c = cand[t];
+	std::vector<vgpu_tune_trial> cand;
+	// Times configuration c `reps` times (fastest kept); false on an error.
+	int st = VGPU_OK;
+	auto trial = [&](vgpu_tune_trial c) -> bool {
 		h->first_tier = c.first_tier;
 		if (c.first_tier == 0) h->wave_occ = c.occupancy;
 		else h->fast_occ = c.occupancy;
@@ -2070,19 +2058,44 @@ int vgpu_autotune(vgpu_container *h, unsigned flags, int reps, vgpu_tune_trial *
 					    : (h->fast_waves > 0 ? h->fast_waves : default_fast_waves(c.occupancy));
 		double ms = 1e300;
 		for (int r = 0; r < reps; r++) {
-			const int st = vgpu_compute_all(h, flags, nullptr);
-			if (st) {
-				h->first_tier = keep_tier; h->wave_occ = keep_wocc; h->fast_occ = keep_focc;
-				h->pred_margin = keep_margin;
-				return st;
-			}
+			if ((st = vgpu_compute_all(h, flags, nullptr))) return false;
 			double tot = 0;
 			for (int k = 0; k < 4; k++) tot += h->timings[k];
 			ms = std::min(ms, tot);
 		}
 		c.ms = ms;
-		if (ms < best) { best = ms; bi = (int)t; }
+		cand.push_back(c);
+		return true;
+	};
+	auto best_of = [&]() {
+		int bi = 0;
+		for (size_t t = 1; t < cand.size(); t++)
+			if (cand[t].ms < cand[bi].ms) bi = (int)t;
+		return bi;
+	};
+	// Stage 1: the first tier -- one wavefront cell (vertex tables in
+	// registers) at each compiled register budget (periodic containers: 4, 5
+	// and 6 waves per SIMD) or, on the plain path, 64 order-3 cells per
+	// wavefront at each budget.
+	// Stage 2, learned path: the depth margin around the trained one at the
+	// best budget.
+	bool ok = true;
+	for (int o : {3, 4, 5, 6})
+		if (ok && !(h->prd && o == 3)) ok = trial(vgpu_tune_trial{0, o, 0, 0.f, 0.0});
+	if (!learned)
+		for (int o : {2, 3, 4})
+			if (ok) ok = trial(vgpu_tune_trial{1, o, 0, 0.f, 0.0});
+	if (ok && learned) {
+		const vgpu_tune_trial b = cand[best_of()];
+		for (float d : {-2.f, -1.f, 1.f, 2.f, 4.f})
+			if (ok) ok = trial(vgpu_tune_trial{b.first_tier, b.occupancy, 0, d, 0.0});
 	}
+	if (!ok) {
+		h->first_tier = keep_tier; h->wave_occ = keep_wocc; h->fast_occ = keep_focc;
+		h->pred_margin = keep_margin;
+		return st;
+	}
+	const int bi = best_of();
 	const vgpu_tune_trial &b = cand[bi];
 	h->first_tier = b.first_tier;
 	h->wave_occ = b.first_tier == 0 ? b.occupancy : keep_wocc;

@@ -234,7 +234,7 @@ int launch_cells_list(bool poly, bool prd, int waves, hipStream_t s, const RectV
 // wavefront-cell slab per wavefront.  Cells without a usable list are
 // computed by the plain algorithm in place (block-mask stamps from
 // stamp_base + cell + 1); only overflows go to out's retry list.
-int launch_cells_wlist(bool poly, bool prd, int waves, int chunk, hipStream_t s, const RectView &v, int nc,
+int launch_cells_wlist(bool poly, bool prd, int occ, int waves, int chunk, hipStream_t s, const RectView &v, int nc,
 		       const int *cellblk, const int *order, const int *cnt, const int *boff, const ListRec *LA,
 		       int *counters, char *ws, unsigned stamp_base, int *fallbacks, int *routed, const CellOut &out);
 
