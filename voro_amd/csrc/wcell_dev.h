@@ -225,15 +225,17 @@ struct WCellT {
 		putk<K>(vna, v >> 6, m, a); putk<K>(vnb, v >> 6, m, b); putk<K>(vnc, v >> 6, m, c);
 	}
 	VG_INL int DS(int i) const { return (int)((rlu(vds, i >> 2) >> (8 * (i & 3))) & 0xffu); }
+	// a stack write changes one byte of lane i >> 2's word, in that lane's
+	// own register (no lane read)
 	VG_INL void setDS(int i, int v) {
 		const unsigned sh = 8u * (unsigned)(i & 3);
-		const unsigned w = (rlu(vds, i >> 2) & ~(0xffu << sh)) | ((unsigned)v & 0xffu) << sh;
+		const unsigned w = (vds & ~(0xffu << sh)) | ((unsigned)v & 0xffu) << sh;
 		vds = lane == (i >> 2) ? w : vds;
 	}
 	VG_INL int DM(int i) const { return (int)((rlu(vdm, i >> 2) >> (8 * (i & 3))) & 0xffu); }
 	VG_INL void setDM(int i, int v) {
 		const unsigned sh = 8u * (unsigned)(i & 3);
-		const unsigned w = (rlu(vdm, i >> 2) & ~(0xffu << sh)) | ((unsigned)v & 0xffu) << sh;
+		const unsigned w = (vdm & ~(0xffu << sh)) | ((unsigned)v & 0xffu) << sh;
 		vdm = lane == (i >> 2) ? w : vdm;
 	}
 	VG_INL void put_nb(int i, int v) { putk<K>(nbv, i >> 6, lane == (i & 63), v); }
@@ -313,8 +315,28 @@ struct WCellT {
 	}
 	VG_INL int E(int v, int i) const { return twE(TW(v), i); }
 	VG_INL int R(int v, int i) const { return twR(TW(v), i); }
+#if VG_WDEV
+	// edge and relation updates made in the vertex's own lane (no lane read)
+	VG_INL void setE(int v, int i, int e) {
+		const int h = v >> 6;
+		const bool me = lane == (v & 63);
+		eachk<K>([&](auto hc) __attribute__((always_inline)) {
+			constexpr int q = decltype(hc)::value;
+			vt[q] = (me && h == q) ? twSetE(vt[q], i, e) : vt[q];
+		});
+	}
+	VG_INL void setER(int v, int i, int e, int r) {
+		const int h = v >> 6;
+		const bool me = lane == (v & 63);
+		eachk<K>([&](auto hc) __attribute__((always_inline)) {
+			constexpr int q = decltype(hc)::value;
+			vt[q] = (me && h == q) ? twSetR(twSetE(vt[q], i, e), i, r) : vt[q];
+		});
+	}
+#else
 	VG_INL void setE(int v, int i, int e) { setTW(v, twSetE(TW(v), i, e)); }
 	VG_INL void setER(int v, int i, int e, int r) { setTW(v, twSetR(twSetE(TW(v), i, e), i, r)); }
+#endif
 	VG_INL bool DEL(int v) const { return delm.get(v); }
 	VG_INL void setDEL(int v) { delm.set(v); }
 	VG_INL bool FL(int v) const { return flipm.get(v); }
@@ -1082,6 +1104,7 @@ struct WCellT {
 			return true;
 		}
 #if VG_WDEV
+		if constexpr (K == 1) return compact1();
 		int org[K], inv[K];
 		eachk<K>([&](auto hc) __attribute__((always_inline)) {
 			constexpr int h = decltype(hc)::value;
@@ -1122,6 +1145,46 @@ struct WCellT {
 	}
 
 #if VG_WDEV
+	// compact() for one register set with the replay's scalar chain kept
+	// short: the last live vertex below p is found with a bit scan of the
+	// deletion marks, the first 16 delete-stack entries are read into scalar
+	// registers at once, and a moved vertex's original index is read back
+	// from `org` only when its slot was itself filled earlier in the replay
+	// (otherwise it is the slot).  The moves, and so the permutation, are
+	// the same as the vertex-by-vertex loop's.
+	VG_FN bool compact1() {
+		int org[1] = {lane}, inv[1] = {lane};
+		unsigned long long dm = delm.w[0], fm = flipm.w[0], tgt = 0;
+		const unsigned d0 = rlu(vds, 0), d1 = rlu(vds, 1), d2 = rlu(vds, 2), d3 = rlu(vds, 3);
+		bool moved = false;
+		up = 0;
+		while (nds > 0) {
+			const unsigned long long below = p >= 64 ? ~0ull : (1ull << p) - 1;
+			const unsigned long long live = ~dm & below;
+			if (live == 0) { fatal = ovf = true; return false; }
+			p = 63 - __builtin_clzll(live);
+			--nds;
+			if (nds < 16) {
+				const unsigned wd = nds < 4 ? d0 : (nds < 8 ? d1 : (nds < 12 ? d2 : d3));
+				up = (int)((wd >> (8 * (nds & 3))) & 0xffu);
+			} else up = DS(nds);
+			if (up < p) {
+				// slot p's vertex moves to slot up
+				const int o = ((tgt >> p) & 1ull) ? rli(org[0], p) : p;
+				org[0] = lane == up ? o : org[0];
+				inv[0] = lane == o ? up : inv[0];
+				dm &= ~(1ull << up);
+				fm &= ~(1ull << up);
+				tgt |= 1ull << up;
+				moved = true;
+			} else up = p++;
+		}
+		delm.w[0] = dm;
+		flipm.w[0] = fm;
+		if (moved) permute(org, inv);
+		return true;
+	}
+
 	static VG_INL unsigned bperm(unsigned v, int addr) {
 		return (unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)v);
 	}
