@@ -175,11 +175,19 @@ def main():
     # and of the plain path's wavefront-cell kernel
     wave_s = plain_tier_ms / 1e3
     wave_fp64 = ncells * FLOP_PER_CELL / wave_s / 1e12
-    def traffic_of(name):
+    # the calibrated FETCH_SIZE + WRITE_SIZE of each kernel variant, measured
+    # with rocprofv3 on this workload (profiles/); the variant the autotuner
+    # chose is looked up
+    def traffic_of(name, kernel):
         tpath = os.path.join(ROOT, "profiles", name)
-        return json.load(open(tpath)).get("bytes_per_launch") if os.path.exists(tpath) else None
-    traffic = traffic_of("traffic_k_wcells_list_uniform1m.json")
-    wave_traffic = traffic_of("traffic_k_wcells_uniform1m.json")
+        if not os.path.exists(tpath):
+            return None
+        return json.load(open(tpath)).get("kernels", {}).get(kernel, {}).get("bytes_per_launch")
+    lt = tune_learned["best"]
+    list_occ = 4 if lt["occupancy"] <= 4 else min(6, lt["occupancy"])
+    list_kernel = "k_wcells_list<mono, %d>" % list_occ
+    traffic = traffic_of("traffic_k_wcells_list_uniform1m.json", list_kernel)
+    wave_traffic = traffic_of("traffic_k_wcells_uniform1m.json", plain_kernel)
 
     cpu = None
     if not args.no_cpu:
@@ -227,7 +235,7 @@ def main():
                     "train_seconds": t_train, "train_loss": train_loss, "train_seeds": list(args.train_seeds)},
         "roofline": {"bound": "mfma", "achieved": fp64_tf, "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": fp64_tf / MI355X_FP64_TFLOPS, "traffic": traffic,
-                     "kernel": "k_wcells_list<mono, 4>", "kernel_ms": kern_s * 1e3,
+                     "kernel": list_kernel, "kernel_ms": kern_s * 1e3,
                      "compute": "FP64 vector arithmetic, no matrix instructions; the peak is MI355X's FP64 rate",
                      "limiter": "scalar-instruction issue (see the instruction-mix profile)",
                      "flop_per_cell": FLOP_PER_CELL},

@@ -246,7 +246,9 @@ int vgpu_set_first_tier(vgpu_container *h, int tier, int waves, int occupancy);
 
 /* Autotuner for one container (no counterpart in the reference): times
  * vgpu_compute_all(h, flags) `reps` times for each launch configuration and
- * keeps the fastest on the handle.  The configurations are the first tier's
+ * keeps the fastest on the handle (of configurations within 1% of the
+ * fastest, the one with the fewest waves per SIMD: it spills least to
+ * memory).  The configurations are the first tier's
  * cells per wavefront (one wavefront cell, vertex tables in registers, at 3,
  * 4, 5 or 6 waves per SIMD -- periodic containers: 4, 5 or 6 -- or 64 order-3 cells
  * per wavefront, one per lane, at 2, 3 or 4) and, with VGPU_F_LEARNED, the

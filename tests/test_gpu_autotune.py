@@ -31,7 +31,7 @@ def test_gpu_autotune_plain_keeps_results_and_writes_table(oracle_mod, tmp_path)
     kinds = {(r["first_tier"], r["occupancy"]) for r in t["trials"]}
     assert kinds == {("wavefront", 3), ("wavefront", 4), ("wavefront", 5), ("wavefront", 6),
                      ("lane", 2), ("lane", 3), ("lane", 4)}
-    assert t["best"]["ms"] == min(r["ms"] for r in t["trials"])
+    assert t["best"]["ms"] <= 1.01 * min(r["ms"] for r in t["trials"])
     assert json.load(open(path))["best"] == t["best"]
     # the kept configuration computes the oracle's cells
     con.compute_all_cells(F)

@@ -2067,10 +2067,18 @@ int vgpu_autotune(vgpu_container *h, unsigned flags, int reps, vgpu_tune_trial *
 		cand.push_back(c);
 		return true;
 	};
+	// The fastest, except that a configuration within 1% of it with fewer
+	// waves per SIMD is preferred: more waves hold fewer registers each and
+	// spill more to memory, for gains that small.
 	auto best_of = [&]() {
-		int bi = 0;
+		int f = 0;
 		for (size_t t = 1; t < cand.size(); t++)
-			if (cand[t].ms < cand[bi].ms) bi = (int)t;
+			if (cand[t].ms < cand[f].ms) f = (int)t;
+		int bi = f;
+		for (size_t t = 0; t < cand.size(); t++)
+			if (cand[t].ms <= 1.01 * cand[f].ms && cand[t].first_tier == cand[f].first_tier &&
+			    cand[t].depth_offset == cand[f].depth_offset && cand[t].occupancy < cand[bi].occupancy)
+				bi = (int)t;
 		return bi;
 	};
 	// Stage 1: the first tier -- one wavefront cell (vertex tables in
