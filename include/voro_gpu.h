@@ -240,8 +240,8 @@ int vgpu_set_launch(vgpu_container *h, int fast_waves, int fast_occupancy);
  * results do not depend on the choice.  waves: persistent wavefronts of the
  * wavefront tier (0 = 4 x occupancy x compute units); occupancy: its
  * register budget in waves per SIMD (2, 3, 4, 5, 6 or 8, default 5, the
- * learned path's list kernel 4, 5 or 6; periodic containers 4, 5 or 6 and
- * their list kernel 4; 0 = keep). */
+ * learned path's list kernel and periodic containers 4, 5 or 6; 0 =
+ * keep). */
 int vgpu_set_first_tier(vgpu_container *h, int tier, int waves, int occupancy);
 
 /* Autotuner for one container (no counterpart in the reference): times

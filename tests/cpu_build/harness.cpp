@@ -53,8 +53,14 @@ struct Runner {
 	}
 };
 
+// 1: a wavefront cell whose list ends is continued from the list's last
+// step by compute_cell_rect (as the device's list tier does); 0: -1 sends
+// it on to the plain tiers, which compute it again.
+static int list_resume = 1;
+
 // The list-driven tier: gather the cell's candidate list at depth G, run
-// compute_cell_list; -1 sends the cell on to the plain tiers.
+// compute_cell_list; -1 sends the cell on to the plain tiers (the order-3
+// cell) or continues it (the wavefront cell, list_resume).
 // PRD: a periodic container; the gather reads a copy of the view without
 // the first-visit record, the cell records its visits with key0.
 template<bool POLY, class CELL = VCell3, size_t BYTES = Layout3<>::lane_bytes, bool PRD = false>
@@ -62,6 +68,7 @@ struct ListRunner {
 	std::vector<char> slab;
 	CELL c;
 	std::vector<ListRec> la;
+	unsigned stamp = 0;
 	explicit ListRunner(double mls) : slab(BYTES + 2 * 65536, 0) { c.setup(slab.data() + 65536, 0, mls); }
 	int run(const RectView &v, const int *cellblk, int cell, int G, int ijk, int s, double *vol, double *cent, int *nf,
 		std::vector<int> &nb, CellStats &st, long long *fallbacks, unsigned long long key0 = 0) {
@@ -69,8 +76,18 @@ struct ListRunner {
 		vg.fv = nullptr;
 		la.resize(64);
 		const int n = gather_cell_list<PRD>(vg, cellblk[cell], cell, G, la.data(), 1);
-		int r = compute_cell_list<CELL, POLY, PRD>(c, v, ijk, s, la.data(), 1, n, st, key0);
-		if (r == -1) { (*fallbacks)++; return -1; }
+		ListResume rs;
+		int r = compute_cell_list<CELL, POLY, PRD>(c, v, ijk, s, la.data(), 1, n, st, key0, &rs);
+		if (r == -1) {
+			(*fallbacks)++;
+			if (!std::is_same<CELL, WCell>::value || !list_resume) return -1;
+			list_resume_fill<POLY, PRD>(v, ijk, s, la.data(), 1, n, key0, rs);
+			const int ci = ijk % v.nx, cj = (ijk / v.nx) % v.gy, ck = ijk / v.nxy;
+			const bool ok = compute_cell_rect<CELL, POLY, PRD>(c, v, ijk, s, ci, cj, ck, ++stamp, st, key0, nullptr, 0,
+									   &rs);
+			if (c.ovf) return -1;
+			r = ok ? 1 : 0;
+		}
 		if (r < 0 || c.ovf) return -1;
 		if (r == 1) {
 			VecSink sk{&nb};
@@ -91,6 +108,9 @@ static int list_depth = 0;
 static long long list_fallbacks = 0;
 // G > 0: first try every cell through a candidate list of worklist depth G.
 void hx_use_list(int G) { list_depth = G; list_fallbacks = 0; }
+// on: wavefront cells continue where their list ended (default); off: they
+// are computed again by the plain tiers.
+void hx_list_resume(int on) { list_resume = on; }
 long long hx_list_fallbacks(void) { return list_fallbacks; }
 
 static int use_wave = 0;

@@ -16,16 +16,18 @@ def build():
                     "-o", SO, os.path.join(HERE, "harness.cpp")], check=True)
 
 
-def run(O, spec, ids, xyz, r=None, list_depth=0, wave=False):
+def run(O, spec, ids, xyz, r=None, list_depth=0, wave=False, resume=True):
     """Same result dict as oracle.run, plus per-cell stats and tier counts.
     list_depth > 0 computes every cell through a candidate list of that
     worklist depth first (the learned path's kernel), the rest on the plain
     tiers; res["list_fallbacks"] counts the cells whose list was too short.
     wave: every cell is first tried by the wavefront-cell tiers;
     res["wave_done"] / res["wave2_done"] / res["wave4_done"] count the cells
-    the 64-, 128- and 254-vertex ones finished."""
+    the 64-, 128- and 254-vertex ones finished.  resume: a wavefront cell
+    whose list ends is continued from there (else computed again)."""
     lib = C.CDLL(SO)
     lib.hx_use_list(int(list_depth))
+    lib.hx_list_resume(1 if resume else 0)
     lib.hx_use_wave(1 if wave else 0)
     lib.hx_list_fallbacks.restype = C.c_longlong
     lib.hx_wave_done.restype = C.c_longlong

@@ -87,6 +87,29 @@ def test_list_cutting_on_cpu_matches_oracle(oracle_mod, harness, case, wave):
     assert last_fb == int(np.sum(st[:, 1] >= 64))
 
 
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_list_resume_matches_the_plain_cell(oracle_mod, harness, case):
+    """A wavefront cell whose list ends is continued from the list's last
+    step instead of being computed again: the continued cell makes the same
+    nplane calls as the plain cell (plane count and worklist exit equal), is
+    in the same phase where the plain cell passed the list depth, and gives
+    the same results as a cell computed again."""
+    O = oracle_mod
+    kind, geom, grid, sph, xyz, r, per = CASES[case]()
+    ids = np.arange(len(xyz), dtype=np.int32)
+    sp = O.make_spec(kind, geom, grid, periodic=per, spheres=sph)
+    plain = harness.run(O, sp, ids, xyz, r, wave=True)
+    for G in (1, 3, 8, 30, 63):
+        a = harness.run(O, sp, ids, xyz, r, list_depth=G, wave=True, resume=False)
+        b = harness.run(O, sp, ids, xyz, r, list_depth=G, wave=True, resume=True)
+        assert b["list_fallbacks"] == a["list_fallbacks"] > 0 or G == 63
+        assert O.compare(b, a, vol_rtol=0.0) == [], G
+        assert np.array_equal(a["vol"], b["vol"]) and np.array_equal(a["cent"], b["cent"])
+        assert np.array_equal(b["stats"][:, :2], plain["stats"][:, :2]), G
+        deep = plain["stats"][:, 1] > G
+        assert np.array_equal(b["stats"][deep, 2], plain["stats"][deep, 2]), G
+
+
 def _para(geom, n, seed, lo=-0.15, hi=1.15):
     bx, bxy, by, bxz, byz, bz = geom
     u = np.stack([I.uniform(seed, 80 + a, n) for a in range(3)], 1) * (hi - lo) + lo

@@ -177,24 +177,27 @@ k_cells_list(RectView v, int nc, const int *__restrict__ cellblk, const int *__r
 
 // One cell of the list-driven wavefront tier: compute_cell_list over the
 // cell's list (nent records from LA), then the outputs, or an entry in
-// out's retry list (overflow).  A cell whose list ends first (*fallbacks) or
-// that has no list (*routed) is computed again at once by the plain
-// algorithm in the same wavefront (block-mask stamp `stamp`), so that the
-// deep cells -- scheduled first -- overlap with the rest instead of running
-// after them.
+// out's retry list (overflow).  A cell whose list ends first (*fallbacks) is
+// continued from the list's last step, and a cell that has no list
+// (*routed) computed, by the plain algorithm at once in the same wavefront
+// (block-mask stamp `stamp`), so that the deep cells -- scheduled first --
+// overlap with the rest instead of running after them.  res: the wavefront's
+// LDS record of where the list ended.
 template<bool POLY, bool PRD>
 __device__ __forceinline__ void wlist_cell(WCell &c, const RectView &v, int cell, int ijk, const ListRec *LA, int nent,
-					   unsigned stamp, int *fallbacks, int *routed, const CellOut &out) {
+					   unsigned stamp, int *fallbacks, int *routed, const CellOut &out, ListResume &res) {
 	const int lane = threadIdx.x;
 	const int s = cell - v.cstart[ijk];
 	CellStats stt;
 	const int r = compute_cell_list<WCell, POLY, PRD>(c, v, ijk, s, LA, 1, nent, stt,
-							  (unsigned long long)cell << 24);
+							  (unsigned long long)cell << 24, &res);
 	double vol = 0, cx = 0, cy = 0, cz = 0;
 	int nf = 0;
 	if (r == -1 || r == -3) {
 		if (lane == 0) atomicAdd(r == -1 ? fallbacks : routed, 1);
-		wave_cell<1, POLY, PRD>(c, v, cell, ijk, stamp, out);
+		if (r == -1) list_resume_fill<POLY, PRD>(v, ijk, s, LA, 1, nent, (unsigned long long)cell << 24, res);
+		else res.g = -1;
+		wave_cell<1, POLY, PRD>(c, v, cell, ijk, stamp, out, &res);
 		return;
 	}
 	bool retry = r < 0 || c.ovf;
@@ -248,6 +251,7 @@ k_wcells_list(RectView v_, int nc, const int *__restrict__ cellblk, const int *_
 	      int *__restrict__ fallbacks, int *__restrict__ routed, CellOut out) {
 	typedef const __attribute__((address_space(4))) RectView CView;
 	const int lane = threadIdx.x;
+	__shared__ ListResume res;
 	WCell c;
 	c.setup(ws + (size_t)blockIdx.x * WLayout::bytes, lane, v_.max_len_sq);
 	const int nshare = 8;
@@ -271,7 +275,7 @@ k_wcells_list(RectView v_, int nc, const int *__restrict__ cellblk, const int *_
 				const int ijk = cellblk[cell];
 				const size_t bs = (size_t)boff[pos];
 				wlist_cell<POLY, PRD>(c, v, cell, ijk, LA + bs, cnt[pos], stamp_base + (unsigned)cell + 1u, fallbacks,
-						      routed, out);
+						      routed, out, res);
 			}
 		}
 	}
@@ -372,6 +376,7 @@ k_batch_wcells_list(const RectView *__restrict__ views, const int *__restrict__ 
 		    int *__restrict__ counters, char *__restrict__ ws, size_t slab_bytes, const unsigned *__restrict__ stamp_dev,
 		    int chunk, int *__restrict__ fallbacks, int *__restrict__ routed, CellOut out) {
 	const int lane = threadIdx.x;
+	__shared__ ListResume res;
 	WCell c;
 	c.setup(ws + (size_t)blockIdx.x * slab_bytes, lane, 0.0);
 	const unsigned stamp_base = *stamp_dev;
@@ -397,7 +402,7 @@ k_batch_wcells_list(const RectView *__restrict__ views, const int *__restrict__ 
 				c.set_tol(v.max_len_sq);
 				const size_t bs = (size_t)(unsigned)loff[pos];
 				wlist_cell<POLY, false>(c, v, cell, gb - v.boff, LA + bs, lcnt[pos], stamp_base + (unsigned)cell + 1u,
-							fallbacks, routed, out);
+							fallbacks, routed, out, res);
 			}
 		}
 	}
@@ -441,9 +446,12 @@ int launch_cells_wlist(bool poly, bool prd, int occ, int waves, int chunk, hipSt
 #define VG_WLIST(P, R, O)                                                                                           \
 	hipLaunchKernelGGL((k_wcells_list<P, R, O>), dim3(waves), dim3(64), 0, s, v, nc, cellblk, order, cnt, boff, LA, \
 			   counters, ws, stamp_base, chunk, fallbacks, routed, out)
-	// 4, 5 or 6 waves per SIMD for rectangular containers, 4 for periodic ones
-	if (prd) { if (poly) VG_WLIST(true, true, 4); else VG_WLIST(false, true, 4); }
-	else if (occ == 5) { if (poly) VG_WLIST(true, false, 5); else VG_WLIST(false, false, 5); }
+	// 4, 5 or 6 waves per SIMD
+	if (prd) {
+		if (occ == 5) { if (poly) VG_WLIST(true, true, 5); else VG_WLIST(false, true, 5); }
+		else if (occ >= 6) { if (poly) VG_WLIST(true, true, 6); else VG_WLIST(false, true, 6); }
+		else { if (poly) VG_WLIST(true, true, 4); else VG_WLIST(false, true, 4); }
+	} else if (occ == 5) { if (poly) VG_WLIST(true, false, 5); else VG_WLIST(false, false, 5); }
 	else if (occ >= 6) { if (poly) VG_WLIST(true, false, 6); else VG_WLIST(false, false, 6); }
 	else { if (poly) VG_WLIST(true, false, 4); else VG_WLIST(false, false, 4); }
 #undef VG_WLIST

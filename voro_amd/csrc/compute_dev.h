@@ -362,10 +362,24 @@ template<class C> struct LaneSteps<C, std::void_t<decltype(C::lane_parallel)>> {
 #if VG_WDEV
 VG_INL unsigned long long steps_ballot(bool b) { return __ballot(b); }
 VG_INL int steps_lane() { return (int)threadIdx.x; }
+// a value every lane holds alike, read back from memory into a scalar register
+VG_INL int steps_uniform(int a) { return __builtin_amdgcn_readfirstlane(a); }
 #else
 VG_INL unsigned long long steps_ballot(bool) { return 0; }
 VG_INL int steps_lane() { return 0; }
+VG_INL int steps_uniform(int a) { return a; }
 #endif
+VG_INL unsigned long long steps_uniform64(unsigned long long a) {
+	return (unsigned long long)(unsigned)steps_uniform((int)(unsigned)a)
+	       | ((unsigned long long)(unsigned)steps_uniform((int)(unsigned)(a >> 32)) << 32);
+}
+VG_INL double steps_uniform_d(double a) {
+	unsigned long long b;
+	__builtin_memcpy(&b, &a, 8);
+	b = steps_uniform64(b);
+	__builtin_memcpy(&a, &b, 8);
+	return a;
+}
 
 // Candidate data of compute_cell_rect: the worklist entries and radii, the
 // particle range of a worklist block, and the particles themselves, each
@@ -385,6 +399,24 @@ struct CandFetch {
 	VG_INL void part(const RectView &v, int idx, dbl4 &o, int &pid) { o = v.P[idx]; pid = v.pid[idx]; }
 };
 
+
+// Where a learned-path list ended (compute_cell_list returning -1): the
+// cell holds every cut of worklist steps 0 .. g-1, and the plain candidate
+// loop continues at the top of step g, before its radius refresh, with the
+// list loop's radius state (g < 0: nothing to continue).  acc: bit s+1 for
+// each step s whose block was cut (not skipped by its distance), from which
+// the phase-2 bookkeeping is rebuilt; img, key: the periodic image-read
+// flag and the next first-visit key.  The list loop records only mh, the
+// radius bound at the worklist start (mh[0]) and after each refresh
+// (mh[1..9]); list_resume_fill derives the rest from the records.  A
+// wavefront cell keeps the record in LDS, read where the worklist starts, so
+// that it holds no registers across the cuts.
+struct ListResume {
+	int g, next_count, cpi, img;
+	double mrs;
+	unsigned long long acc, key;
+	double mh[10];
+};
 
 // compute_cell (src/v_compute.cc:291-621) for a rectangular container or,
 // with PRD, a periodic one (start cell, frac_pos and region_index of
@@ -413,27 +445,30 @@ struct CandFetch {
 template<class CELL, bool POLY, bool PRD = false>
 VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci, int cj, int ck,
 			     unsigned stamp, CellStats &stt, unsigned long long key0 = 0, const dbl4 *ghost = nullptr,
-			     double ghost_maxr = 0) {
+			     double ghost_maxr = 0, const ListResume *resume = nullptr) {
 	double x, y, z, qx = 0, qy = 0, qz = 0;
 	int i, j, k, di, dj, dk, ei, ej, ek, f = 0, g = 0, disp = 0;
 	double fx0 = 0, fy0 = 0, fz0 = 0, gxs0 = 0, gys0 = 0, gzs0 = 0;
 	unsigned q, m1 = 0, m2 = 0;
 	unsigned long long key = key0;
-	stt.n_planes = 0; stt.wl_exit = 0; stt.phase = 1; stt.img = 0;
+	const bool resumed = resume && steps_uniform(resume->g) >= 0;
+	stt.n_planes = 0; stt.wl_exit = 0; stt.phase = 1; stt.img = resumed ? steps_uniform(resume->img) : 0;
 
 	const int b0 = v.bstart[ijk], co0 = v.bstart[ijk + 1] - b0;
 	const dbl4 me = ghost ? *ghost : v.P[b0 + s];
 	x = me.x; y = me.y; z = me.z;
 	if (PRD) {
-		c.init_unit(v.unit, v.unit_p);
-		if (c.ovf) return false;
+		if (!resumed) {
+			c.init_unit(v.unit, v.unit_p);
+			if (c.ovf) return false;
+		}
 		i = v.nx; j = v.ey; k = v.ez;
 	} else {
 		double xa, xb, ya, yb, za, zb;
 		if (v.xper) { xb = 0.5 * (v.bx - v.ax); xa = -xb; i = v.nx; } else { xa = v.ax - x; xb = v.bx - x; i = ci; }
 		if (v.yper) { yb = 0.5 * (v.by - v.ay); ya = -yb; j = v.ny; } else { ya = v.ay - y; yb = v.by - y; j = cj; }
 		if (v.zper) { zb = 0.5 * (v.bz - v.az); za = -zb; k = v.nz; } else { za = v.az - z; zb = v.bz - z; k = ck; }
-		c.init_box(xa, xb, ya, yb, za, zb);
+		if (!resumed) c.init_box(xa, xb, ya, yb, za, zb);
 		disp = ijk - i - v.nx * (j + v.ny * k);
 	}
 	RadPolicy<POLY> rp;
@@ -451,8 +486,9 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 	const double *radp = v.mrad;
 	const unsigned *e = v.wl;
 	// Generator state: 0 walls, 1 own block, 2 worklist, 4 replay of the
-	// phase-2 bookkeeping, 3 block search.
-	int stage = 0, w = 0, t = 0;
+	// phase-2 bookkeeping, 3 block search.  A continued cell has its walls
+	// and own block cut already.
+	int stage = resumed ? 1 : 0, w = resumed ? v.nwalls : 0, t = resumed ? co0 : 0;
 	bool marking = false;
 	int bt = 0, be = 0;           // particle range of the block being cut
 	bool bcheck = false;          // per-particle radius check for this block
@@ -504,6 +540,33 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 		gx_ *= gx_; gy_ *= gy_; gz_ *= gz_;
 	};
 	constexpr bool refrac = LaneSteps<CELL>::value;
+	// the worklist of the particle's sub-block (src/v_compute.cc:328-362)
+	auto select_worklist = [&]() {
+		double fx, fy, fz, gxs, gys, gzs;
+		reload();
+		if (PRD) { fx = x - v.boxx * ci; fy = y - v.boxy * (cj - v.ey); fz = z - v.boxz * (ck - v.ez); }
+		else { fx = x - v.ax - v.boxx * ci; fy = y - v.ay - v.boxy * cj; fz = z - v.az - v.boxz * ck; }
+		di = int(fx * v.xsp * VGPU_WL_FGRID); dj = int(fy * v.ysp * VGPU_WL_FGRID); dk = int(fz * v.zsp * VGPU_WL_FGRID);
+		if (di >= VGPU_WL_HGRID) {
+			gxs = fx;
+			m1 = 127 + (3 << 21); m2 = 1 + (1 << 21); di = VGPU_WL_FGRID - 1 - di; if (di < 0) di = 0;
+		} else { m1 = m2 = 0; gxs = v.boxx - fx; }
+		if (dj >= VGPU_WL_HGRID) {
+			gys = fy;
+			m1 |= (127 << 7) + (3 << 24); m2 |= (1 << 7) + (1 << 24); dj = VGPU_WL_FGRID - 1 - dj; if (dj < 0) dj = 0;
+		} else gys = v.boxy - fy;
+		if (dk >= VGPU_WL_HGRID) {
+			gzs = fz;
+			m1 |= (127 << 14) + (3 << 27); m2 |= (1 << 14) + (1 << 27); dk = VGPU_WL_FGRID - 1 - dk; if (dk < 0) dk = 0;
+		} else gzs = v.boxz - fz;
+		gxs *= gxs; gys *= gys; gzs *= gzs;
+		if constexpr (!refrac) { fx0 = fx; fy0 = fy; fz0 = fz; gxs0 = gxs; gys0 = gys; gzs0 = gzs; }
+		const int wli = di + VGPU_WL_HGRID * (dj + VGPU_WL_HGRID * dk);
+		radp = v.mrad + wli * 64;
+		e = v.wl + wli * 64;
+		cf.wl_begin(v, radp, e, i, j, k, m1, m2, ci, cj, ck, disp);
+		f = cf.ent(e, 0); g = 0;
+	};
 	for (;;) {
 		double px_ = 0, py_ = 0, pz_ = 0, rs = 0;
 		double rs_raw = -1;           // squared distance of a particle candidate (walls: -1)
@@ -584,30 +647,23 @@ VG_FN bool compute_cell_rect(CELL &c, const RectView &v, int ijk, int s, int ci,
 				// Worklist selection (src/v_compute.cc:328-362).
 				mrs = c.max_radius_squared();
 				mrs_b = mrs; since = 0;
-				double fx, fy, fz, gxs, gys, gzs;
-				reload();
-				if (PRD) { fx = x - v.boxx * ci; fy = y - v.boxy * (cj - v.ey); fz = z - v.boxz * (ck - v.ez); }
-				else { fx = x - v.ax - v.boxx * ci; fy = y - v.ay - v.boxy * cj; fz = z - v.az - v.boxz * ck; }
-				di = int(fx * v.xsp * VGPU_WL_FGRID); dj = int(fy * v.ysp * VGPU_WL_FGRID); dk = int(fz * v.zsp * VGPU_WL_FGRID);
-				if (di >= VGPU_WL_HGRID) {
-					gxs = fx;
-					m1 = 127 + (3 << 21); m2 = 1 + (1 << 21); di = VGPU_WL_FGRID - 1 - di; if (di < 0) di = 0;
-				} else { m1 = m2 = 0; gxs = v.boxx - fx; }
-				if (dj >= VGPU_WL_HGRID) {
-					gys = fy;
-					m1 |= (127 << 7) + (3 << 24); m2 |= (1 << 7) + (1 << 24); dj = VGPU_WL_FGRID - 1 - dj; if (dj < 0) dj = 0;
-				} else gys = v.boxy - fy;
-				if (dk >= VGPU_WL_HGRID) {
-					gzs = fz;
-					m1 |= (127 << 14) + (3 << 27); m2 |= (1 << 14) + (1 << 27); dk = VGPU_WL_FGRID - 1 - dk; if (dk < 0) dk = 0;
-				} else gzs = v.boxz - fz;
-				gxs *= gxs; gys *= gys; gzs *= gzs;
-				if constexpr (!refrac) { fx0 = fx; fy0 = fy; fz0 = fz; gxs0 = gxs; gys0 = gys; gzs0 = gzs; }
-				const int wli = di + VGPU_WL_HGRID * (dj + VGPU_WL_HGRID * dk);
-				radp = v.mrad + wli * 64;
-				e = v.wl + wli * 64;
-				cf.wl_begin(v, radp, e, i, j, k, m1, m2, ci, cj, ck, disp);
-				f = cf.ent(e, 0); g = 0;
+				select_worklist();
+				if (resume) {
+					const int gr = steps_uniform(resume->g);
+					if (gr >= 0) {
+						// continue a list that ended: the worklist starts
+						// again at the top of step gr
+						g = gr; mrs = steps_uniform_d(resume->mrs); mrs_b = mrs;
+						next_count = steps_uniform(resume->next_count);
+						cpi = steps_uniform(resume->cpi);
+						key = steps_uniform64(resume->key);
+						const int s1 = f > 1 ? f : 1;     // the step at whose top phase 2 starts
+						if (g > s1) {
+							marking = true; stt.phase = 2; g2_first = s1 + 1;
+							p2_cut = steps_uniform64(resume->acc) & ~((2ull << s1) - 1);   // bits s1+1 ..
+						}
+					}
+				}
 				stage = 2;
 				continue;
 			}
@@ -842,12 +898,14 @@ VG_INL int list_code(double q) { return q < 0 ? 1 : (q > 0 ? 2 : 0); }
 
 // compute_cell over a candidate list.  Returns 1 / 0 (the cell exists /
 // was deleted), -1 when the list ends before the radius test stops the
-// cell (the plain path must compute it), -2 on overflow (c.ovf), -3 for an
+// cell (the plain path must finish it), -2 on overflow (c.ovf), -3 for an
 // empty list (no list was gathered: the plain path computes the cell).
-// key0: the cell's loop position << 24 (periodic first-visit keys).
+// key0: the cell's loop position << 24 (periodic first-visit keys).  res:
+// where a list that ended left off (-1), for compute_cell_rect to continue
+// the cell from there instead of computing it again.
 template<class CELL, bool POLY, bool PRD = false>
 VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, const ListRec *__restrict__ L, int stride,
-			    int nrec, CellStats &stt, unsigned long long key0 = 0) {
+			    int nrec, CellStats &stt, unsigned long long key0 = 0, ListResume *res = nullptr) {
 	stt.n_planes = 0; stt.wl_exit = 0; stt.phase = 1; stt.img = 0;
 	if (nrec <= 0) return -3;
 	unsigned long long key = key0;
@@ -963,6 +1021,7 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, const Li
 				mrs = c.max_radius_squared();
 				mrs_b = mrs; since = 0;
 				own_done = true;
+				if (res) res->mh[0] = mrs;
 			}
 			if (k >= nrec) { stt.n_planes = c.n_cuts; return -1; }
 			const ListRec a = L[(size_t)k * stride];
@@ -970,6 +1029,7 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, const Li
 			if (g < 63 && g == next_count) {
 				mrs = c.max_radius_squared();
 				mrs_b = mrs; since = 0;
+				if (res) res->mh[cpi + 1] = mrs;
 				if (cpi != 8) next_count = count_list(cpi++);
 			}
 			if (rp.ctest(a.rad, mrs)) { stt.wl_exit = g; stt.n_planes = c.n_cuts; return 1; }
@@ -1014,6 +1074,38 @@ VG_FN int compute_cell_list(CELL &c, const RectView &v, int ijk, int s, const Li
 		}
 		if (!c.nplane(px_, py_, pz_, rs, pid)) { stt.n_planes = c.n_cuts; return c.ovf ? -2 : 0; }
 	}
+}
+
+// The rest of a ListResume after compute_cell_list returned -1 over the
+// nrec records of L (mh filled by that call): the records are one per
+// worklist step, then the end record if the list has one, so the list ended
+// at the top of step g; the loop's refresh schedule, block skips and
+// first-visit keys are replayed from the records with the recorded radius
+// bounds.
+template<bool POLY, bool PRD>
+VG_FN void list_resume_fill(const RectView &v, int ijk, int s, const ListRec *__restrict__ L, int stride, int nrec,
+			    unsigned long long key0, ListResume &res) {
+	const int g = (L[(size_t)(nrec - 1) * stride].info & 7) == LK_END ? nrec - 1 : nrec;
+	RadPolicy<POLY> rp;
+	rp.init(v.P[v.bstart[ijk] + s].w, v.max_radius);
+	int nc = 3, cp = 0, img = 0;
+	double mrs = steps_uniform_d(res.mh[0]);
+	unsigned long long acc = 0, key = key0;
+	for (int t = 0; t < g; t++) {
+		if (t < 63 && t == nc) {
+			mrs = steps_uniform_d(res.mh[cp + 1]);
+			if (cp != 8) nc = count_list(cp++);
+		}
+		const ListRec a = L[(size_t)t * stride];
+		if ((a.info & 7) != LK_BLOCK || rp.ctest(a.cn, mrs)) continue;
+		acc |= 1ull << (t + 1);
+		if (PRD) {
+			if ((a.info >> 3) & 1) img = 1;
+			key++;
+		}
+	}
+	res.g = g; res.next_count = nc; res.cpi = cp; res.img = img;
+	res.mrs = mrs; res.acc = acc; res.key = key;
 }
 
 }  // namespace vgpu

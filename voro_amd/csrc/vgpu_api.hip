@@ -396,8 +396,8 @@ int run_learned(vgpu_container *h, const RectView &v, int nc, int waves, CellOut
 	if (h->first_tier == 0) {
 		// one cell per wavefront
 		// the list kernel at the wavefront tier's register budget (4, 5 or
-		// 6 waves per SIMD; periodic containers 4)
-		const int locc = h->prd ? 4 : (h->wave_occ >= 6 ? 6 : (h->wave_occ == 5 ? 5 : 4));
+		// 6 waves per SIMD)
+		const int locc = h->wave_occ >= 6 ? 6 : (h->wave_occ == 5 ? 5 : 4);
 		int w = h->wave_waves > 0 ? h->wave_waves : default_fast_waves(locc);
 		const int chunk = nc >= 4 * w ? 4 : 1;
 		w = std::max(1, std::min(w, (nc + chunk - 1) / chunk));

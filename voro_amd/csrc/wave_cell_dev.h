@@ -57,16 +57,17 @@ __device__ __forceinline__ void wave_geo_write(WCellT<K> &c, const GeoSink &g, i
 // One cell of the container with view v, block b of the view's numbering:
 // compute_cell, then the outputs (lane 0 writes the scalars, every lane its
 // neighbour ids) or, if the cell overflowed, an entry in out's retry list.
+// resume: the cell is continued where a candidate list ended.
 template<int K, bool POLY, bool PRD>
 __device__ __forceinline__ void wave_cell(WCellT<K> &c, const RectView &v, int cell, int b, unsigned stamp,
-					  const CellOut &out) {
+					  const CellOut &out, const ListResume *resume = nullptr) {
 	typedef WLayoutT<K> L;
 	const int lane = threadIdx.x;
 	const int s = cell - v.cstart[b];
 	const int ci = b % v.nx, cj = (b / v.nx) % v.gy, ck = b / v.nxy;
 	CellStats stt;
 	const bool ok = compute_cell_rect<WCellT<K>, POLY, PRD>(c, v, b, s, ci, cj, ck, stamp, stt,
-							  (unsigned long long)cell << 24);
+							  (unsigned long long)cell << 24, nullptr, 0, resume);
 	double vol = 0, cx = 0, cy = 0, cz = 0;
 	int nf = 0;
 	if (!c.ovf && ok) {
