@@ -390,8 +390,9 @@ int vgpu_batch_get_results(vgpu_batch *b, int index, int *id, int *ijk, int *q, 
 int vgpu_batch_sum_volumes(vgpu_batch *b, double *out);
 /* Launch configuration {persistent wavefronts of the first tier (0 = as
  * many as fit), first tier (2..4: one cell per lane at that many waves per
- * SIMD, default 2; 0: one cell per wavefront), cells per queue entry (1..64,
- * default 64)}; get and/or set.  Results do not depend on it. */
+ * SIMD, default 2; 0, 5, 6: one cell per wavefront at 4, 5 or 6 waves per
+ * SIMD, which the learned path's list tier also takes), cells per queue
+ * entry (1..64, default 64)}; get and/or set.  Results do not depend on it. */
 int vgpu_batch_config(vgpu_batch *b, int *get3, const int *set3);
 /* Times every configuration on this batch (reps computes each) and keeps
  * the fastest; best3 / best_ms receive it and its time. */

@@ -104,7 +104,7 @@ def test_gpu_batch_first_tiers_and_leftovers_match_oracle(oracle_mod):
                           np.arange(len(xyz), dtype=np.int32), xyz, r))
     assert max(int(w["nfaces"].max()) for w in want) > 66     # cells over 128 vertices
     b = voro_amd.container_batch(cons)
-    for cfg in ((0, 0, 16), (0, 0, 1), (0, 2, 64)):
+    for cfg in ((0, 0, 16), (0, 0, 1), (0, 5, 16), (0, 6, 4), (0, 2, 64)):
         _flush("batch first tier config %s" % (cfg,))
         b.set_config(*cfg)
         b.compute()

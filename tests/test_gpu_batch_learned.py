@@ -67,6 +67,12 @@ def test_gpu_batch_learned_matches_plain_and_oracle(oracle_mod):
     st = b.learned_stats()
     _flush("batch learned stats %s" % st)
     assert st["fallbacks"] < 0.2 * n and st["list_entries"] > 0 and st["list_need"] <= b.list_capacity
+    # the list tier at 5 and 6 waves per SIMD
+    for occ in (5, 6):
+        _flush("batch learned: list tier at %d waves per SIMD" % occ)
+        b.set_config(0, occ, 16)
+        b.compute(F | voro_amd.F_LEARNED)
+        _same(plain, _all(b, len(specs)))
     for i in (0, 7, 31, 63):
         L, g, xyz = specs[i]
         sp = O.make_spec(0, (0, L, 0, L, 0, L), (g, g, g), spheres=[[L / 2, L / 2, L / 2, 0.45 * L, -99]])

@@ -981,9 +981,9 @@ class container_batch:
 
     def set_config(self, waves=0, occupancy=2, chunk=64):
         """occupancy 2..4: one cell per lane at that many waves per SIMD
-        (the default first tier); 0: one cell per wavefront.  waves: the
-        first tier's persistent wavefronts (0 = as many as fit); chunk:
-        cells per queue entry."""
+        (the default first tier); 0, 5, 6: one cell per wavefront at 4, 5 or
+        6 waves per SIMD.  waves: the first tier's persistent wavefronts
+        (0 = as many as fit); chunk: cells per queue entry."""
         a = np.array([waves, occupancy, chunk], np.int32)
         _check(lib().vgpu_batch_config(self._h, None, _p(a, C.c_int)), "vgpu_batch_config")
 

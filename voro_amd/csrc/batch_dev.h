@@ -58,8 +58,9 @@ int launch_batch_chunks(hipStream_t s, int ncont, int ch, const int *cfirst, con
 int launch_batch_first(bool poly, int occ, int waves, hipStream_t s, const RectView *views, const BatchChunk *chunks,
 		       const int *nchunks, const int *cellblk, int *counters, char *ws, size_t slab,
 		       const unsigned *stamp_dev, const CellOut &out);
-// Wavefront-cell first tier over the chunk queue (cells_wave.hip).
-int launch_batch_wfirst(bool poly, int waves, hipStream_t s, const RectView *views, const BatchChunk *chunks,
+// Wavefront-cell first tier over the chunk queue (cells_wave.hip), occ
+// (4, 5 or 6) waves per SIMD.
+int launch_batch_wfirst(bool poly, int occ, int waves, hipStream_t s, const RectView *views, const BatchChunk *chunks,
 			const int *nchunks, const int *cellblk, int *counters, char *ws, size_t slab,
 			const unsigned *stamp_dev, const CellOut &out);
 int launch_batch_retry(int tier, bool poly, int waves, hipStream_t s, const RectView *views, const int *boff,
@@ -78,9 +79,9 @@ int launch_batch_features(hipStream_t s, int nb, const int *bcont, const RectVie
 int launch_batch_list_build(hipStream_t s, const RectView *views, const int *bcont, int n, const int *ncp,
 			    const int *cellblk, const int *order, const unsigned char *depth, long long cap,
 			    unsigned long long *top, int *lcnt, int *loff, ListRec *LA);
-int launch_batch_wcells_list(bool poly, int waves, int chunk, hipStream_t s, const RectView *views, const int *bcont,
-			     const int *ncp, const int *cellblk, const int *order, const int *lcnt, const int *loff,
-			     const ListRec *LA, int *counters, char *ws, size_t slab, const unsigned *stamp_dev,
-			     int *fallbacks, int *routed, const CellOut &out);
+int launch_batch_wcells_list(bool poly, int occ, int waves, int chunk, hipStream_t s, const RectView *views,
+			     const int *bcont, const int *ncp, const int *cellblk, const int *order, const int *lcnt,
+			     const int *loff, const ListRec *LA, int *counters, char *ws, size_t slab,
+			     const unsigned *stamp_dev, int *fallbacks, int *routed, const CellOut &out);
 
 }  // namespace vgpu

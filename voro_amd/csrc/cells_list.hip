@@ -367,9 +367,10 @@ k_batch_list_build(const RectView *__restrict__ views, const int *__restrict__ b
 
 // The list-driven wavefront tier over a batch's schedule (positions
 // 0 .. *ncp-1), each cell with its container's view; positions are taken
-// `chunk` at a time from one counter per XCD share.
-template<bool POLY>
-__global__ void __launch_bounds__(64, 4)
+// `chunk` at a time from one counter per XCD share.  OCC waves per SIMD (4,
+// 5 or 6).
+template<bool POLY, int OCC>
+__global__ void __launch_bounds__(64, OCC)
 k_batch_wcells_list(const RectView *__restrict__ views, const int *__restrict__ bcont, const int *__restrict__ ncp,
 		    const int *__restrict__ cellblk, const int *__restrict__ order, const int *__restrict__ lcnt,
 		    const int *__restrict__ loff, const ListRec *__restrict__ LA,
@@ -427,16 +428,17 @@ int launch_batch_list_build(hipStream_t s, const RectView *views, const int *bco
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 
-int launch_batch_wcells_list(bool poly, int waves, int chunk, hipStream_t s, const RectView *views, const int *bcont,
-			     const int *ncp, const int *cellblk, const int *order, const int *lcnt, const int *loff,
-			     const ListRec *LA, int *counters, char *ws, size_t slab, const unsigned *stamp_dev,
-			     int *fallbacks, int *routed, const CellOut &out) {
-	if (poly)
-		hipLaunchKernelGGL((k_batch_wcells_list<true>), dim3(waves), dim3(64), 0, s, views, bcont, ncp, cellblk,
-				   order, lcnt, loff, LA, counters, ws, slab, stamp_dev, chunk, fallbacks, routed, out);
-	else
-		hipLaunchKernelGGL((k_batch_wcells_list<false>), dim3(waves), dim3(64), 0, s, views, bcont, ncp, cellblk,
-				   order, lcnt, loff, LA, counters, ws, slab, stamp_dev, chunk, fallbacks, routed, out);
+int launch_batch_wcells_list(bool poly, int occ, int waves, int chunk, hipStream_t s, const RectView *views,
+			     const int *bcont, const int *ncp, const int *cellblk, const int *order, const int *lcnt,
+			     const int *loff, const ListRec *LA, int *counters, char *ws, size_t slab,
+			     const unsigned *stamp_dev, int *fallbacks, int *routed, const CellOut &out) {
+#define VG_BWLIST(P, O)                                                                                          \
+	hipLaunchKernelGGL((k_batch_wcells_list<P, O>), dim3(waves), dim3(64), 0, s, views, bcont, ncp, cellblk, order, \
+			   lcnt, loff, LA, counters, ws, slab, stamp_dev, chunk, fallbacks, routed, out)
+	if (occ >= 6) { if (poly) VG_BWLIST(true, 6); else VG_BWLIST(false, 6); }
+	else if (occ == 5) { if (poly) VG_BWLIST(true, 5); else VG_BWLIST(false, 5); }
+	else { if (poly) VG_BWLIST(true, 4); else VG_BWLIST(false, 4); }
+#undef VG_BWLIST
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 

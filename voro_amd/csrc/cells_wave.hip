@@ -61,9 +61,10 @@ k_wcells(RectView v_, int ncells, const int *__restrict__ list, const int *__res
 // 64 consecutive cells of one container): a wavefront takes one entry at a
 // time and computes its cells one after another as wavefront cells (up to 64
 // vertices).  The queue is split into one contiguous share per XCD, as in
-// k_wcells; cells it does not finish go to out's retry list.
-template<bool POLY>
-__global__ void __launch_bounds__(64, 4)
+// k_wcells; cells it does not finish go to out's retry list.  OCC waves per
+// SIMD (4, 5 or 6).
+template<bool POLY, int OCC>
+__global__ void __launch_bounds__(64, OCC)
 k_batch_wfirst(const RectView *__restrict__ views, const BatchChunk *__restrict__ chunks,
 	       const int *__restrict__ nchunks, const int *__restrict__ cellblk, int *__restrict__ counters,
 	       char *__restrict__ ws, size_t slab_bytes, const unsigned *__restrict__ stamp_dev, CellOut out) {
@@ -163,15 +164,23 @@ int launch_cells_wave(int k, bool poly, bool prd, int occ, int waves, hipStream_
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 
-int launch_batch_wfirst(bool poly, int waves, hipStream_t s, const RectView *views, const BatchChunk *chunks,
+#define VGPU_BATCH_WAVE_LAUNCH(P, O)                                                                       \
+	hipLaunchKernelGGL((k_batch_wfirst<P, O>), dim3(waves), dim3(64), 0, s, views, chunks, nchunks, cellblk, \
+			   counters, ws, slab, stamp_dev, out)
+
+int launch_batch_wfirst(bool poly, int occ, int waves, hipStream_t s, const RectView *views, const BatchChunk *chunks,
 			const int *nchunks, const int *cellblk, int *counters, char *ws, size_t slab,
 			const unsigned *stamp_dev, const CellOut &out) {
-	if (poly)
-		hipLaunchKernelGGL((k_batch_wfirst<true>), dim3(waves), dim3(64), 0, s, views, chunks, nchunks, cellblk,
-				   counters, ws, slab, stamp_dev, out);
-	else
-		hipLaunchKernelGGL((k_batch_wfirst<false>), dim3(waves), dim3(64), 0, s, views, chunks, nchunks, cellblk,
-				   counters, ws, slab, stamp_dev, out);
+	if (occ >= 6) {
+		if (poly) VGPU_BATCH_WAVE_LAUNCH(true, 6);
+		else VGPU_BATCH_WAVE_LAUNCH(false, 6);
+	} else if (occ == 5) {
+		if (poly) VGPU_BATCH_WAVE_LAUNCH(true, 5);
+		else VGPU_BATCH_WAVE_LAUNCH(false, 5);
+	} else {
+		if (poly) VGPU_BATCH_WAVE_LAUNCH(true, 4);
+		else VGPU_BATCH_WAVE_LAUNCH(false, 4);
+	}
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 

@@ -899,6 +899,10 @@ struct vgpu_batch {
 
 namespace {
 
+// Waves per SIMD of a batch's wavefront-cell tiers: first tier 5 or 6 asks
+// for that many, the other settings take 4.
+int batch_wave_occ(const vgpu_batch *b) { return b->occ >= 5 ? b->occ : 4; }
+
 __global__ void k_batch_pcont(int ncont, const int *__restrict__ pfirst, int *__restrict__ pcont) {
 	const int c = blockIdx.x;
 	if (c >= ncont) return;
@@ -962,8 +966,8 @@ int batch_prepare(vgpu_batch *b, unsigned flags) {
 		    (st = b->list_a.ensure((size_t)b->lcap * sizeof(ListRec))))
 			return st;
 	}
-	const bool wave = b->occ == 0 || learned;
-	int w = b->waves_req > 0 ? b->waves_req : default_fast_waves(wave ? 4 : b->occ);
+	const bool wave = b->occ == 0 || b->occ >= 5 || learned;
+	int w = b->waves_req > 0 ? b->waves_req : default_fast_waves(wave ? batch_wave_occ(b) : b->occ);
 	w = std::max(1, std::min(w, (wave ? nblk((long long)N1, b->chunk) : nblk((long long)N1, 64)) + b->ncont));
 	b->waves = w;
 	typedef Layout<TierWide, 64> LW;
@@ -1063,7 +1067,7 @@ int batch_enqueue(vgpu_batch *b, unsigned flags, hipStream_t s, bool timing) {
 		// overflows (more than 64 vertices) to the per-lane order-3 tier
 		out.tier = 0; out.retry = b->retry1.as<int>(); out.retry_count = ctr + 1;
 		const int chunk = n >= 4 * b->waves ? 4 : 1;
-		if (launch_batch_wcells_list(b->poly, b->waves, chunk, s, b->views.as<RectView>(), b->bcont.as<int>(), ncp,
+		if (launch_batch_wcells_list(b->poly, batch_wave_occ(b), b->waves, chunk, s, b->views.as<RectView>(), b->bcont.as<int>(), ncp,
 					     b->cellblk.as<int>(), b->d_order.as<int>(), b->lcnt.as<int>(), b->loff.as<int>(),
 					     b->list_a.as<ListRec>(), cnt8, b->ws_wave.base(), b->ws_wave.slab,
 					     (const unsigned *)(ctr + 8), ctr + 12, ctr + 13, out))
@@ -1075,8 +1079,8 @@ int batch_enqueue(vgpu_batch *b, unsigned flags, hipStream_t s, bool timing) {
 				       b->ws_fast.slab, (const unsigned *)(ctr + 8), out))
 			return VGPU_HIP_ERROR;
 		wide_list = b->retry3.as<int>(); wide_count = ctr + 5;
-	} else if (b->occ == 0) {
-		if (launch_batch_wfirst(b->poly, b->waves, s, b->views.as<RectView>(), b->chunks.as<BatchChunk>(),
+	} else if (b->occ == 0 || b->occ >= 5) {
+		if (launch_batch_wfirst(b->poly, batch_wave_occ(b), b->waves, s, b->views.as<RectView>(), b->chunks.as<BatchChunk>(),
 					b->nchunks.as<int>(), b->cellblk.as<int>(), cnt8, b->ws_wave.base(), b->ws_wave.slab,
 					(const unsigned *)(ctr + 8), out))
 			return VGPU_HIP_ERROR;
@@ -2497,7 +2501,7 @@ int vgpu_batch_config(vgpu_batch *b, int *get3, const int *set3) {
 	if (!b) return VGPU_INVALID_ARGUMENT;
 	if (get3) { get3[0] = b->waves; get3[1] = b->occ; get3[2] = b->chunk; }
 	if (set3) {
-		if (set3[0] < 0 || set3[1] == 1 || set3[1] < 0 || set3[1] > 4 || set3[2] < 1 || set3[2] > 64)
+		if (set3[0] < 0 || set3[1] == 1 || set3[1] < 0 || set3[1] > 6 || set3[2] < 1 || set3[2] > 64)
 			return VGPU_INVALID_ARGUMENT;
 		b->waves_req = set3[0]; b->occ = set3[1]; b->chunk = set3[2];
 		b->drop_graph();
@@ -2508,16 +2512,17 @@ int vgpu_batch_config(vgpu_batch *b, int *get3, const int *set3) {
 // The project's autotuner for batches: times the compute (eager, `reps`
 // runs each, best kept) for the per-lane first tier at every register budget
 // (2, 3, 4 waves per SIMD) with 64 or 32 cells per queue entry and for the
-// wavefront-cell first tier (16 cells per entry), each with as many
-// persistent wavefronts as fit, and keeps the fastest.  Results do not depend
+// wavefront-cell first tier at 4, 5 and 6 waves per SIMD (16 cells per
+// entry), each with as many persistent wavefronts as fit, and keeps the
+// fastest.  Results do not depend
 // on the choice.  (On FCC-like containers, whose cells are alike, the
 // per-lane tier wins; on uneven cells the wavefront cells.)
 int vgpu_batch_autotune(vgpu_batch *b, unsigned flags, int reps, int *best3, double *best_ms) {
 	if (!b || reps < 1 || (flags & (VGPU_F_LEARNED | VGPU_F_GRAPH))) return VGPU_INVALID_ARGUMENT;
-	const int cands[7][2] = {{2, 64}, {2, 32}, {3, 64}, {3, 32}, {4, 64}, {4, 32}, {0, 16}};
+	const int cands[9][2] = {{2, 64}, {2, 32}, {3, 64}, {3, 32}, {4, 64}, {4, 32}, {0, 16}, {5, 16}, {6, 16}};
 	double best = 1e300;
 	int bo = 2, bc = 64;
-	for (int oi = 0; oi < 7; oi++) {
+	for (int oi = 0; oi < 9; oi++) {
 		const int cfg[3] = {0, cands[oi][0], cands[oi][1]};
 		int st = vgpu_batch_config(b, nullptr, cfg);
 		if (st) return st;
@@ -2531,7 +2536,7 @@ int vgpu_batch_autotune(vgpu_batch *b, unsigned flags, int reps, int *best3, dou
 	const int cfg[3] = {0, bo, bc};
 	int st = vgpu_batch_config(b, nullptr, cfg);
 	if (st) return st;
-	if (best3) { best3[0] = default_fast_waves(bo == 0 ? 4 : bo); best3[1] = bo; best3[2] = bc; }
+	if (best3) { best3[0] = default_fast_waves(bo == 0 ? 4 : bo); best3[1] = bo; best3[2] = bc; }   // 5, 6: wavefront cells
 	if (best_ms) *best_ms = best;
 	return VGPU_OK;
 }
