@@ -124,11 +124,11 @@ int launch_cells_wave(int k, bool poly, bool prd, int occ, int waves, hipStream_
 	} else if (k == 2) {
 		// the overflow tier: few cells, registers for two vertex sets
 		if (prd) {
-			if (poly) VGPU_WAVE_LAUNCH(2, true, 2, true);
-			else VGPU_WAVE_LAUNCH(2, false, 2, true);
+			if (poly) VGPU_WAVE_LAUNCH(2, true, VG_K2_OCC, true);
+			else VGPU_WAVE_LAUNCH(2, false, VG_K2_OCC, true);
 		} else {
-			if (poly) VGPU_WAVE_LAUNCH(2, true, 2, false);
-			else VGPU_WAVE_LAUNCH(2, false, 2, false);
+			if (poly) VGPU_WAVE_LAUNCH(2, true, VG_K2_OCC, false);
+			else VGPU_WAVE_LAUNCH(2, false, VG_K2_OCC, false);
 		}
 	} else if (prd) {
 		// periodic containers: 4, 5 or 6 waves per SIMD

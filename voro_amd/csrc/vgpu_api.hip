@@ -280,7 +280,7 @@ int default_fast_waves(int occ) {
 int launch_wave(vgpu_container *h, int k, const RectView &v, int nc, const int *list, const int *list_count,
 		CellOut out, hipStream_t s) {
 	// periodic containers: compiled for 4, 5 and 6 waves per SIMD
-	const int occ = k > 1 ? 2 : (h->prd ? std::min(6, std::max(4, h->wave_occ)) : h->wave_occ);
+	const int occ = k == 2 ? VG_K2_OCC : (k > 1 ? 2 : (h->prd ? std::min(6, std::max(4, h->wave_occ)) : h->wave_occ));
 	int waves = (h->wave_waves > 0 && k == 1) ? h->wave_waves : default_fast_waves(occ);
 	int cnt = nc;
 	if (list_count) {

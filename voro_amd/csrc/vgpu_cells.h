@@ -201,6 +201,14 @@ int launch_ghost_cells(int tier, bool poly, bool prd, int waves, hipStream_t s, 
 		       const RectGeom &geo, int n, const double *xyz, const double *rad, const int *list,
 		       const int *list_count, int *exists, double *vol, double *cent, int *nfaces, int *nbr, int max_nbr,
 		       int *counter, int *retry, int *failed, char *ws, size_t slab, unsigned stamp_base);
+// Waves per SIMD of the 128-vertex wavefront tier (k = 2): four, whose
+// register spills cost less than the latency two waves leave exposed
+// (periodic radical 2M, ~44k cells over 64 vertices: the tiers after the
+// first take 135 instead of 154 ms per compute; 3, 5, 6 waves: 141, 135,
+// 144 ms).
+#ifndef VG_K2_OCC
+#define VG_K2_OCC 4
+#endif
 int launch_cells_wave(int k, bool poly, bool prd, int occ, int waves, hipStream_t s, const RectView &v, int ncells,
 		      const int *list, const int *list_count, const int *cellblk, int *counters, char *ws,
 		      unsigned stamp_base, int chunk, const CellOut &out);
