@@ -149,6 +149,15 @@ struct ImageSet {
 	bool late = false;   // a particle reached a block that was already complete
 	ImageSet(const PrdGeom &g_, const dbl4 *P_, const int *pid_, const int *cstart_)
 		: g(g_), P(P_), pid(pid_), cstart(cstart_), img(g_.nblocks(), 0), created(g_.nblocks(), 0) {}
+	// Starts again with no image block over new primary blocks, keeping the
+	// lists' storage (a periodic compute builds several sets).
+	void reset(const dbl4 *P_, const int *pid_, const int *cstart_) {
+		P = P_; pid = pid_; cstart = cstart_;
+		reg.clear(); ip.clear(); iid.clear();
+		std::fill(img.begin(), img.end(), (unsigned char)0);
+		std::fill(created.begin(), created.end(), (unsigned char)0);
+		late = false;
+	}
 
 	int co(int f) const { return cstart[f + 1] - cstart[f]; }
 	double px(int f, int l) const { return P[cstart[f] + l].x; }

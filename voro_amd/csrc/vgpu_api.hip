@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -44,6 +45,28 @@ struct DBuf {
 		return VGPU_OK;
 	}
 	void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+	template<class T> T *as() const { return (T *)p; }
+};
+
+// Grow-only pinned host buffer (device-to-host copies at full speed, no
+// per-call allocation or zeroing).
+struct HBuf {
+	void *p = nullptr;
+	size_t n = 0;
+	HBuf() = default;
+	HBuf(const HBuf &) = delete;
+	HBuf &operator=(const HBuf &) = delete;
+	~HBuf() { release(); }
+	int ensure(size_t bytes) {
+		if (bytes <= n) return VGPU_OK;
+		release();
+		if (bytes == 0) return VGPU_OK;
+		if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) { p = nullptr; return VGPU_MEMORY_ERROR; }
+		n = bytes;
+		return VGPU_OK;
+	}
+	void release() { if (p) (void)hipHostFree(p); p = nullptr; n = 0; }
+	void swap(HBuf &o) { std::swap(p, o.p); std::swap(n, o.n); }
 	template<class T> T *as() const { return (T *)p; }
 };
 
@@ -132,8 +155,16 @@ struct vgpu_container {
 	DBuf d_unit, Pall, pidall, ball, fv, touch, d_flag;
 	DBuf img_p, img_id, img_reg, img_rank, img_bc, img_cnt;   // image particles for the device assembly
 	std::vector<int> h_rank;                                   // per-block creation counters (host, kept zero)
-	std::vector<dbl4> hP;
-	std::vector<int> hpid, hcs;
+	// host copies of the primary blocks (the image creation reads them), the
+	// first-visit keys of two passes and the image-read flags (pinned)
+	HBuf hP, hpid, hcs, h_prev, h_cur, h_touch;
+	std::vector<unsigned char> h_touched;
+	std::unique_ptr<ImageSet> h_images;   // the image blocks being built (storage kept)
+	ImageSet &image_set(const dbl4 *P_, const int *pid_, const int *cstart_) {
+		if (!h_images) h_images.reset(new ImageSet(pg, P_, pid_, cstart_));
+		else h_images->reset(P_, pid_, cstart_);
+		return *h_images;
+	}
 	int image_passes = 0;
 	// the image blocks the reference would hold after the last compute and
 	// queries since the binning, in their order of creation
@@ -525,13 +556,18 @@ int run_prd_cells(vgpu_container *h, RectView &v, CellOut out, int nc, int fw, h
 	const PrdGeom &g = h->pg;
 	const int nb = h->nblocks;
 	int st;
-	h->hP.resize(nc); h->hpid.resize(nc); h->hcs.resize(nb + 1);
-	HIPCHK(hipMemcpyAsync(h->hP.data(), h->P.p, (size_t)nc * sizeof(dbl4), hipMemcpyDeviceToHost, s));
-	HIPCHK(hipMemcpyAsync(h->hpid.data(), h->pid.p, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
-	HIPCHK(hipMemcpyAsync(h->hcs.data(), h->bstart.p, (size_t)(nb + 1) * 4, hipMemcpyDeviceToHost, s));
+	if ((st = h->hP.ensure((size_t)std::max(nc, 1) * sizeof(dbl4))) || (st = h->hpid.ensure((size_t)std::max(nc, 1) * 4)) ||
+	    (st = h->hcs.ensure((size_t)(nb + 1) * 4)) || (st = h->h_prev.ensure((size_t)nb * 8)) ||
+	    (st = h->h_cur.ensure((size_t)nb * 8)) || (st = h->h_touch.ensure((size_t)std::max(nc, 1))))
+		return st;
+	const dbl4 *hP = h->hP.as<dbl4>();
+	const int *hpid = h->hpid.as<int>(), *hcs = h->hcs.as<int>();
+	HIPCHK(hipMemcpyAsync(h->hP.p, h->P.p, (size_t)nc * sizeof(dbl4), hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(h->hpid.p, h->pid.p, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(h->hcs.p, h->bstart.p, (size_t)(nb + 1) * 4, hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
 	{
-		ImageSet im(g, h->hP.data(), h->hpid.data(), h->hcs.data());
+		ImageSet &im = h->image_set(hP, hpid, hcs);
 		for (int b : prd_margin_images(g, PRD_FIRST_MARGIN)) im.create_block(b);
 		if ((st = upload_blocks(h, im, nc, v, s))) return st;
 	}
@@ -548,11 +584,12 @@ int run_prd_cells(vgpu_container *h, RectView &v, CellOut out, int nc, int fw, h
 				    h->first_tier == 0)))
 			return st;
 	} else if ((st = run_tiers(h, v, nc, fw, nullptr, nullptr, out, s, true))) return st;
-	std::vector<unsigned long long> prev(nb), cur(nb);
-	std::vector<unsigned char> t(nc), touched(nc, 0);
-	HIPCHK(hipMemcpyAsync(prev.data(), fv, (size_t)nb * 8, hipMemcpyDeviceToHost, s));
-	HIPCHK(hipMemcpyAsync(t.data(), h->touch.p, (size_t)nc, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(h->h_prev.p, fv, (size_t)nb * 8, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(h->h_touch.p, h->touch.p, (size_t)nc, hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
+	const unsigned char *t = h->h_touch.as<unsigned char>();
+	std::vector<unsigned char> &touched = h->h_touched;
+	touched.assign(nc, 0);
 	h->image_passes = 1;
 	for (int pass = 0; pass < 8; pass++) {
 		std::vector<int> list;
@@ -561,8 +598,8 @@ int run_prd_cells(vgpu_container *h, RectView &v, CellOut out, int nc, int fw, h
 			if (touched[c]) list.push_back(c);
 		}
 		if (list.empty()) return VGPU_OK;   // no cell read an image block
-		ImageSet im(g, h->hP.data(), h->hpid.data(), h->hcs.data());
-		h->prd_order = prd_visit_order(g, prev.data());
+		ImageSet &im = h->image_set(hP, hpid, hcs);
+		h->prd_order = prd_visit_order(g, h->h_prev.as<unsigned long long>());
 		for (int b : h->prd_order) im.create_block(b);
 		if (im.late) return VGPU_INTERNAL_ERROR;
 		if ((st = upload_blocks(h, im, nc, v, s))) return st;
@@ -573,12 +610,12 @@ int run_prd_cells(vgpu_container *h, RectView &v, CellOut out, int nc, int fw, h
 		hipLaunchKernelGGL(k_fill_u64, dim3(nblk(nb, 256)), dim3(256), 0, s, fv, nb, ~0ull);
 		hipLaunchKernelGGL(k_fill_u8, dim3(nblk(nc, 256)), dim3(256), 0, s, h->touch.as<unsigned char>(), nc, 0);
 		if ((st = run_tiers(h, v, nc, fw, h->retry0.as<int>(), ctr + 4, out, s, false))) return st;
-		HIPCHK(hipMemcpyAsync(cur.data(), fv, (size_t)nb * 8, hipMemcpyDeviceToHost, s));
-		HIPCHK(hipMemcpyAsync(t.data(), h->touch.p, (size_t)nc, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipMemcpyAsync(h->h_cur.p, fv, (size_t)nb * 8, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipMemcpyAsync(h->h_touch.p, h->touch.p, (size_t)nc, hipMemcpyDeviceToHost, s));
 		HIPCHK(hipStreamSynchronize(s));
 		h->image_passes = pass + 2;
-		if (cur == prev) return VGPU_OK;
-		prev.swap(cur);
+		if (std::memcmp(h->h_cur.p, h->h_prev.p, (size_t)nb * 8) == 0) return VGPU_OK;
+		h->h_prev.swap(h->h_cur);
 	}
 	h->prd_order.clear();
 	return VGPU_INTERNAL_ERROR;   // the first-visit order did not settle
@@ -1481,12 +1518,15 @@ int find_tiers(vgpu_container *h, const RectView &v, int n, const double *dq, co
 // them there).
 int prd_host_blocks(vgpu_container *h, hipStream_t s) {
 	const int nc = h->ncells, nb = h->nblocks;
-	h->hP.resize(std::max(nc, 1)); h->hpid.resize(std::max(nc, 1)); h->hcs.resize(nb + 1);
+	int st;
+	if ((st = h->hP.ensure((size_t)std::max(nc, 1) * sizeof(dbl4))) || (st = h->hpid.ensure((size_t)std::max(nc, 1) * 4)) ||
+	    (st = h->hcs.ensure((size_t)(nb + 1) * 4)))
+		return st;
 	if (nc) {
-		HIPCHK(hipMemcpyAsync(h->hP.data(), h->P.p, (size_t)nc * sizeof(dbl4), hipMemcpyDeviceToHost, s));
-		HIPCHK(hipMemcpyAsync(h->hpid.data(), h->pid.p, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipMemcpyAsync(h->hP.p, h->P.p, (size_t)nc * sizeof(dbl4), hipMemcpyDeviceToHost, s));
+		HIPCHK(hipMemcpyAsync(h->hpid.p, h->pid.p, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
 	}
-	HIPCHK(hipMemcpyAsync(h->hcs.data(), h->bstart.p, (size_t)(nb + 1) * 4, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(h->hcs.p, h->bstart.p, (size_t)(nb + 1) * 4, hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
 	return VGPU_OK;
 }
@@ -1523,7 +1563,7 @@ int find_prd(vgpu_container *h, int n, const double *dq, int *dfound, int *dpid,
 	std::vector<unsigned long long> keys(nb);
 	std::vector<unsigned char> t(N);
 	for (int pass = 0; pass < 8; pass++) {
-		ImageSet im(g, h->hP.data(), h->hpid.data(), h->hcs.data());
+		ImageSet im(g, h->hP.as<dbl4>(), h->hpid.as<int>(), h->hcs.as<int>());
 		for (int b : order) im.create_block(b);
 		if (im.late) return VGPU_INTERNAL_ERROR;
 		if ((st = upload_blocks(h, im, nc, v, s))) return st;
@@ -1614,7 +1654,7 @@ int vgpu_compute_ghost_cells(vgpu_container *h, int n, const double *xyz, const 
 		// copy the point itself into it, with an id it never sets.
 		if ((st = prd_host_blocks(h, s))) return st;
 		const int nb = h->nblocks;
-		ImageSet im(h->pg, h->hP.data(), h->hpid.data(), h->hcs.data());
+		ImageSet im(h->pg, h->hP.as<dbl4>(), h->hpid.as<int>(), h->hcs.as<int>());
 		std::vector<unsigned char> ok(nb, 0);
 		for (int b = 0; b < nb; b++) ok[b] = h->pg.primary(b) ? 1 : 0;
 		for (int b : h->prd_order) { im.create_block(b); ok[b] = 1; }
