@@ -155,6 +155,28 @@ def test_gpu_first_tiers_agree(oracle_mod, case):
         assert np.array_equal(a["vol"], b["vol"]) and np.array_equal(a["cent"], b["cent"])
 
 
+def test_gpu_far_block_searches_take_the_wide_window_wavefront_tier(oracle_mod):
+    """Isolated points beside a dense cluster, with small blocks: their cells'
+    block searches reach well beyond the +-6-block window of the 64- and
+    128-vertex wavefront tiers.  The 254-vertex tier, whose window reaches
+    +-31 blocks, finishes such a cell (stats tier 1), bit-exact; none reaches
+    the per-lane tiers."""
+    O = oracle_mod
+    rng = np.random.default_rng(21)
+    dense = rng.uniform(0, 8, (12000, 3))
+    far = rng.uniform(0, 40, (80, 3))
+    xyz = np.concatenate([dense, far])
+    ids = np.arange(len(xyz), dtype=np.int32)
+    sp = (0, (0, 40, 0, 40, 0, 40), (40, 40, 40))
+    _flush("far block searches n=%d" % len(xyz))
+    a = O.run("restatement", O.make_spec(*sp), ids, xyz, stats=True)
+    b = engine_run(*sp, ids, xyz, stats=True)
+    assert O.compare(b, a, vol_rtol=VOL_RTOL) == []
+    assert np.array_equal(a["vol"], b["vol"]) and np.array_equal(a["cent"], b["cent"])
+    assert (b["st_wl_exit"] >= 64).sum() >= 100 and (b["st_tier"] == 1).sum() >= 100
+    assert (b["st_tier"] >= 2).sum() == 0
+
+
 def test_gpu_large_cells_take_the_wide_wavefront_tiers(oracle_mod):
     """Radical cells of a few large particles among small ones have more
     than 64 (and more than 128) vertices: the 128- and 254-vertex wavefront

@@ -135,7 +135,7 @@ struct vgpu_container {
 	DBuf blk, slot, rem, cnt, bstart, tmp, part, P, pid, src, cellblk;
 	DBuf d_mrad, d_wl, d_walls, d_maxr, d_counters;
 	DBuf vol, cent, nfaces, status, nbr_start, arena, csr_off, csr, ctr;
-	DBuf retry0, retry1, retry2, retry3, retry4, retry5;
+	DBuf retry0, retry1, retry2, retry3, retry4;
 	DBuf fq_xyz, fq_found, fq_pid, fq_pos, fq_retry, fq_retry2;   // find_voronoi_cell queries
 	DBuf fq_r, fq_vol, fq_nbr;                                   // ghost cells
 	DBuf fq_touch, fq_list, img_okb;                             // periodic queries: image reads, reruns
@@ -147,7 +147,7 @@ struct vgpu_container {
 	int pred_route = 64;
 	DBuf d_pred, d_feat, d_cfeat, d_depth, d_cost, d_order, d_hist, lcnt, bmax, boff, list_a;
 	long long last_fallbacks = 0, last_entries = 0, last_routed = 0;
-	Workspace ws_fast, ws_wide, ws_huge, ws_wave;
+	Workspace ws_fast, ws_wide, ws_huge, ws_wave, ws_wave4;
 	long long nbr_total = 0;
 	bool csr_valid = false;
 	// periodic containers: unit cell, block arrays with image blocks, first
@@ -323,8 +323,9 @@ int launch_wave(vgpu_container *h, int k, const RectView &v, int nc, const int *
 	// every wavefront that way
 	const int chunk = cnt >= 4 * waves ? 4 : 1;
 	waves = std::max(1, std::min(waves, (cnt + chunk - 1) / chunk));
-	Workspace &ws = h->ws_wave;
-	int st = ws.ensure(WLayout::bytes, waves);
+	// the 254-vertex tier's slabs hold a wider block-search window
+	Workspace &ws = k == 4 ? h->ws_wave4 : h->ws_wave;
+	int st = ws.ensure(k == 4 ? WLayoutT<4>::bytes : WLayout::bytes, waves);
 	if (st) return st;
 	if ((unsigned long long)ws.stamp_base + (unsigned long long)nc + 2 >= 0xffffffffull) {
 		HIPCHK(hipMemset(ws.buf.p, 0, ws.buf.n));
@@ -467,7 +468,7 @@ int run_tiers(vgpu_container *h, const RectView &v, int nc, int fw, const int *l
 	if (h->first_tier == 0) {
 		// one cell per wavefront, up to 64 vertices; leftovers (count at
 		// ctr[6]) to the same with up to 128, theirs (ctr[7]) up to 254, and
-		// theirs (ctr[8]) to order-3 per lane
+		// theirs (ctr[1]) to the wide per-lane tier
 		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr + 6, 3, 0);
 		if (from2) {
 			out.tier = tier0; out.retry = h->retry4.as<int>(); out.retry_count = ctr + 7;
@@ -480,10 +481,11 @@ int run_tiers(vgpu_container *h, const RectView &v, int nc, int fw, const int *l
 			out.tier = 1; out.retry = h->retry4.as<int>(); out.retry_count = ctr + 7;
 			if ((st = launch_wave(h, 2, v, nc, h->retry3.as<int>(), ctr + 6, out, s))) return st;
 		}
-		out.retry = h->retry5.as<int>(); out.retry_count = ctr + 8;
-		if ((st = launch_wave(h, 4, v, nc, h->retry4.as<int>(), ctr + 7, out, s))) return st;
+		// the 254-vertex tier's leftovers go to the wide per-lane tier: the
+		// order-3 per-lane tier's capacities are all smaller, so it finishes
+		// none of them
 		out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
-		if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, h->retry5.as<int>(), ctr + 8, out, s))) return st;
+		if ((st = launch_wave(h, 4, v, nc, h->retry4.as<int>(), ctr + 7, out, s))) return st;
 	} else {
 		out.tier = tier0; out.retry = h->retry1.as<int>(); out.retry_count = ctr + 1;
 		if ((st = launch_tier<Tier3>(h, h->ws_fast, fw, v, nc, list, list_count, out, s))) return st;
@@ -756,14 +758,14 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s, bool bin_only
 	    (st = h->status.ensure(C1 * 4)) || (st = h->nbr_start.ensure(C1 * 4)) || (st = h->arena.ensure((size_t)acap * 4)) ||
 	    (st = h->ctr.ensure(16 * 4)) || (st = h->retry0.ensure(C1 * 4)) || (st = h->retry1.ensure(C1 * 4)) ||
 	    (st = h->retry2.ensure(C1 * 4)) || (st = h->retry3.ensure(C1 * 4)) ||
-	    (st = h->retry4.ensure(C1 * 4)) || (st = h->retry5.ensure(C1 * 4)))
+	    (st = h->retry4.ensure(C1 * 4)))
 		return st;
 	const bool stats = (flags & VGPU_F_STATS) != 0;
 	if (stats) {
 		if ((st = h->st_planes.ensure(C1 * 2)) || (st = h->st_exit.ensure(C1)) || (st = h->st_tier.ensure(C1))) return st;
 	}
 	// [0] arena top [1..2] retry counts of tiers 1, 2 [3] failures [4] learned-tier retries [5] fallbacks
-	// [6..8] wavefront-cell retry counts [9] learned: cells given no list
+	// [6..7] wavefront-cell retry counts [9] learned: cells given no list
 	int *ctr = h->ctr.as<int>();
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 16, 0);
 	RectView v = make_view(h);
@@ -1307,12 +1309,13 @@ int vgpu_destroy(vgpu_container *h) {
 			&h->csr, &h->ctr, &h->retry0, &h->retry1, &h->retry2, &h->st_planes, &h->st_exit, &h->st_tier,
 			&h->d_pred, &h->d_feat, &h->d_cfeat, &h->d_depth, &h->d_cost, &h->d_order, &h->d_hist, &h->lcnt,
 			&h->bmax, &h->boff, &h->list_a, &h->d_unit, &h->Pall, &h->pidall, &h->ball, &h->fv,
-			&h->touch, &h->d_flag, &h->retry3, &h->retry4, &h->retry5, &h->img_p, &h->img_id,
+			&h->touch, &h->d_flag, &h->retry3, &h->retry4, &h->img_p, &h->img_id,
 			&h->img_reg, &h->img_rank, &h->img_bc, &h->img_cnt, &h->fq_xyz, &h->fq_found, &h->fq_pid,
 			&h->fq_pos, &h->fq_retry, &h->fq_retry2, &h->fq_r, &h->fq_vol, &h->fq_nbr, &h->fq_touch,
 			&h->fq_list, &h->img_okb};
 	for (DBuf *b : bufs) b->release();
 	h->ws_fast.buf.release(); h->ws_wide.buf.release(); h->ws_huge.buf.release(); h->ws_wave.buf.release();
+	h->ws_wave4.buf.release();
 	if (h->events) for (int t = 0; t < 5; t++) (void)hipEventDestroy(h->ev[t]);
 	if (h->ev_list) (void)hipEventDestroy(h->ev_list);
 	delete h;

@@ -42,10 +42,13 @@
 namespace vgpu {
 
 // Vertex indices are bytes (0xff stands for the reference's -1), so four
-// vertices per lane hold up to 254.
+// vertices per lane hold up to 254.  The last wavefront tier (K = 4) also
+// takes the cells whose block search leaves the others' window of +-6
+// blocks: its window reaches +-31 blocks (the wide per-lane tier's), its
+// ring 8,192 blocks.
 template<int K>
 struct WLayoutT {
-	enum { MAXV = K < 4 ? 64 * K : 254, MAXS = 256, W = 6, MAXQ = 256, MAXF = 64 * K };
+	enum { MAXV = K < 4 ? 64 * K : 254, MAXS = 256, W = K < 4 ? 6 : 31, MAXQ = K < 4 ? 256 : 8192, MAXF = 64 * K };
 	enum { WD = 2 * W + 1, WBITS = WD * WD * WD, WWORDS = (WBITS + 31) / 32 };
 	// per-wavefront global slab: block-search window and ring only
 	static constexpr unsigned o_bm = 0;                          // WWORDS
