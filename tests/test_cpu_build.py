@@ -200,6 +200,24 @@ def test_wave_cell_128_vertices(oracle_mod, harness):
     assert np.array_equal(a["vol"], b["vol"]) and np.array_equal(a["cent"], b["cent"])
 
 
+def test_wave_cell_far_block_search(oracle_mod, harness):
+    """Isolated points beside a dense cluster, with small blocks: block
+    searches reach beyond the +-6-block window of the 64- and 128-vertex
+    wavefront cells; the 254-vertex one, whose window reaches +-31 blocks,
+    finishes them, so no cell goes on to the per-lane tiers (bit for bit)."""
+    O = oracle_mod
+    rng = np.random.default_rng(21)
+    xyz = np.concatenate([rng.uniform(0, 8, (12000, 3)), rng.uniform(0, 40, (80, 3))])
+    ids = np.arange(len(xyz), dtype=np.int32)
+    sp = O.make_spec(0, (0, 40, 0, 40, 0, 40), (40, 40, 40))
+    a = O.run("restatement", sp, ids, xyz)
+    b = harness.run(O, sp, ids, xyz, wave=True)
+    assert b["wave4_done"] >= 100
+    assert b["wave_done"] + b["wave2_done"] + b["wave4_done"] == len(xyz)
+    assert O.compare(b, a, vol_rtol=0.0) == []
+    assert np.array_equal(a["vol"], b["vol"]) and np.array_equal(a["cent"], b["cent"])
+
+
 GEO_CASES = {
     "uniform": lambda O: (O.make_spec(0, (0, 10, 0, 10, 0, 10), (8, 8, 8)), I.uniform_box(3, 3000, (0, 0, 0), (10, 10, 10)), None),
     "poly_wall": lambda O: (O.make_spec(1, (0, 10, 0, 10, 0, 10), (7, 7, 7), spheres=[(5, 5, 5, 4.5, -99)]),
