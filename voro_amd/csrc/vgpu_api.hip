@@ -157,8 +157,8 @@ struct vgpu_container {
 	std::vector<int> h_rank;                                   // per-block creation counters (host, kept zero)
 	// host copies of the primary blocks (the image creation reads them), the
 	// first-visit keys of two passes and the image-read flags (pinned)
-	HBuf hP, hpid, hcs, h_prev, h_cur, h_touch;
-	std::vector<unsigned char> h_touched;
+	HBuf hP, hpid, hcs, h_vkeys, h_vblks;
+	DBuf d_prev, d_vkeys, d_vblks, d_tacc;   // device side of the replay (keys of the last pass, visited image blocks, touched cells)
 	std::unique_ptr<ImageSet> h_images;   // the image blocks being built (storage kept)
 	ImageSet &image_set(const dbl4 *P_, const int *pid_, const int *cstart_) {
 		if (!h_images) h_images.reset(new ImageSet(pg, P_, pid_, cstart_));
@@ -559,8 +559,8 @@ int run_prd_cells(vgpu_container *h, RectView &v, CellOut out, int nc, int fw, h
 	const int nb = h->nblocks;
 	int st;
 	if ((st = h->hP.ensure((size_t)std::max(nc, 1) * sizeof(dbl4))) || (st = h->hpid.ensure((size_t)std::max(nc, 1) * 4)) ||
-	    (st = h->hcs.ensure((size_t)(nb + 1) * 4)) || (st = h->h_prev.ensure((size_t)nb * 8)) ||
-	    (st = h->h_cur.ensure((size_t)nb * 8)) || (st = h->h_touch.ensure((size_t)std::max(nc, 1))))
+	    (st = h->hcs.ensure((size_t)(nb + 1) * 4)) || (st = h->d_prev.ensure((size_t)nb * 8)) ||
+	    (st = h->d_tacc.ensure((size_t)std::max(nc, 1))))
 		return st;
 	const dbl4 *hP = h->hP.as<dbl4>();
 	const int *hpid = h->hpid.as<int>(), *hcs = h->hcs.as<int>();
@@ -586,38 +586,73 @@ int run_prd_cells(vgpu_container *h, RectView &v, CellOut out, int nc, int fw, h
 				    h->first_tier == 0)))
 			return st;
 	} else if ((st = run_tiers(h, v, nc, fw, nullptr, nullptr, out, s, true))) return st;
-	HIPCHK(hipMemcpyAsync(h->h_prev.p, fv, (size_t)nb * 8, hipMemcpyDeviceToHost, s));
-	HIPCHK(hipMemcpyAsync(h->h_touch.p, h->touch.p, (size_t)nc, hipMemcpyDeviceToHost, s));
-	HIPCHK(hipStreamSynchronize(s));
-	const unsigned char *t = h->h_touch.as<unsigned char>();
-	std::vector<unsigned char> &touched = h->h_touched;
-	touched.assign(nc, 0);
+	// The bookkeeping between passes stays on the device: the cells touched
+	// so far (list for the next pass), the visited image blocks with their
+	// keys (the host sorts them into the creation order), and whether a
+	// pass's keys equal the previous one's.
+	int *ctr = h->ctr.as<int>();
+	unsigned char *tacc = h->d_tacc.as<unsigned char>();
+	hipLaunchKernelGGL(k_fill_u8, dim3(nblk(nc, 256)), dim3(256), 0, s, tacc, nc, 0);
 	h->image_passes = 1;
 	for (int pass = 0; pass < 8; pass++) {
-		std::vector<int> list;
-		for (int c = 0; c < nc; c++) {
-			touched[c] |= t[c];
-			if (touched[c]) list.push_back(c);
+		// ctr[4]: cells for the next pass, ctr[10]: visited image blocks,
+		// ctr[11]: the keys differ (ctr[5] keeps the learned tier's fallbacks)
+		int cnt[2] = {0, 0};
+		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr + 4, 1, 0);
+		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr + 10, 2, 0);
+		hipLaunchKernelGGL(k_prd_touched, dim3(nblk(nc, 256)), dim3(256), 0, s, nc, h->touch.as<unsigned char>(), tacc,
+				   h->retry0.as<int>(), ctr + 4);
+		int cap = (int)(h->d_vblks.n / 4);
+		hipLaunchKernelGGL(k_prd_visited, dim3(nblk(nb, 256)), dim3(256), 0, s, nb, fv, g.nx, g.oy, g.ey, g.wy, g.ez,
+				   g.wz, h->d_vkeys.as<unsigned long long>(), h->d_vblks.as<int>(), cap, ctr + 10);
+		HIPCHK(hipMemcpyAsync(&cnt[0], ctr + 4, 4, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipMemcpyAsync(&cnt[1], ctr + 10, 4, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipStreamSynchronize(s));
+		if (cnt[0] == 0) return VGPU_OK;   // no cell read an image block
+		const int nv = cnt[1];
+		if (nv > cap) {
+			// more visited image blocks than the buffers hold: grow, list again
+			if ((st = h->d_vkeys.ensure((size_t)nv * 8)) || (st = h->d_vblks.ensure((size_t)nv * 4))) return st;
+			cap = nv;
+			hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr + 10, 1, 0);
+			hipLaunchKernelGGL(k_prd_visited, dim3(nblk(nb, 256)), dim3(256), 0, s, nb, fv, g.nx, g.oy, g.ey, g.wy,
+					   g.ez, g.wz, h->d_vkeys.as<unsigned long long>(), h->d_vblks.as<int>(), cap, ctr + 10);
 		}
-		if (list.empty()) return VGPU_OK;   // no cell read an image block
+		if ((st = h->h_vkeys.ensure((size_t)std::max(nv, 1) * 8)) || (st = h->h_vblks.ensure((size_t)std::max(nv, 1) * 4)))
+			return st;
+		if (nv) {
+			HIPCHK(hipMemcpyAsync(h->h_vkeys.p, h->d_vkeys.p, (size_t)nv * 8, hipMemcpyDeviceToHost, s));
+			HIPCHK(hipMemcpyAsync(h->h_vblks.p, h->d_vblks.p, (size_t)nv * 4, hipMemcpyDeviceToHost, s));
+		}
+		// this pass's keys, to compare the next pass's with
+		HIPCHK(hipMemcpyAsync(h->d_prev.p, fv, (size_t)nb * 8, hipMemcpyDeviceToDevice, s));
+		HIPCHK(hipStreamSynchronize(s));
+		// the creation order: visited image blocks by first-visit key (keys
+		// are distinct: each names one visit of one cell)
+		{
+			const unsigned long long *kk = h->h_vkeys.as<unsigned long long>();
+			const int *bb = h->h_vblks.as<int>();
+			std::vector<int> idx(nv);
+			for (int q = 0; q < nv; q++) idx[q] = q;
+			std::sort(idx.begin(), idx.end(), [&](int a, int b) { return kk[a] < kk[b]; });
+			h->prd_order.resize(nv);
+			for (int q = 0; q < nv; q++) h->prd_order[q] = bb[idx[q]];
+		}
 		ImageSet &im = h->image_set(hP, hpid, hcs);
-		h->prd_order = prd_visit_order(g, h->h_prev.as<unsigned long long>());
 		for (int b : h->prd_order) im.create_block(b);
 		if (im.late) return VGPU_INTERNAL_ERROR;
 		if ((st = upload_blocks(h, im, nc, v, s))) return st;
-		int *ctr = h->ctr.as<int>();
-		const int m = (int)list.size();
-		HIPCHK(hipMemcpyAsync(h->retry0.p, list.data(), (size_t)m * 4, hipMemcpyHostToDevice, s));
-		HIPCHK(hipMemcpyAsync(ctr + 4, &m, 4, hipMemcpyHostToDevice, s));
 		hipLaunchKernelGGL(k_fill_u64, dim3(nblk(nb, 256)), dim3(256), 0, s, fv, nb, ~0ull);
 		hipLaunchKernelGGL(k_fill_u8, dim3(nblk(nc, 256)), dim3(256), 0, s, h->touch.as<unsigned char>(), nc, 0);
 		if ((st = run_tiers(h, v, nc, fw, h->retry0.as<int>(), ctr + 4, out, s, false))) return st;
-		HIPCHK(hipMemcpyAsync(h->h_cur.p, fv, (size_t)nb * 8, hipMemcpyDeviceToHost, s));
-		HIPCHK(hipMemcpyAsync(h->h_touch.p, h->touch.p, (size_t)nc, hipMemcpyDeviceToHost, s));
+		int differ = 0;
+		hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr + 11, 1, 0);
+		hipLaunchKernelGGL(k_differ_u64, dim3(nblk(nb, 256)), dim3(256), 0, s, nb, fv,
+				   h->d_prev.as<unsigned long long>(), ctr + 11);
+		HIPCHK(hipMemcpyAsync(&differ, ctr + 11, 4, hipMemcpyDeviceToHost, s));
 		HIPCHK(hipStreamSynchronize(s));
 		h->image_passes = pass + 2;
-		if (std::memcmp(h->h_cur.p, h->h_prev.p, (size_t)nb * 8) == 0) return VGPU_OK;
-		h->h_prev.swap(h->h_cur);
+		if (!differ) return VGPU_OK;
 	}
 	h->prd_order.clear();
 	return VGPU_INTERNAL_ERROR;   // the first-visit order did not settle
@@ -766,6 +801,7 @@ int run_pipeline(vgpu_container *h, unsigned flags, hipStream_t s, bool bin_only
 	}
 	// [0] arena top [1..2] retry counts of tiers 1, 2 [3] failures [4] learned-tier retries [5] fallbacks
 	// [6..7] wavefront-cell retry counts [9] learned: cells given no list
+	// [10..11] periodic replay: visited image blocks, keys differ
 	int *ctr = h->ctr.as<int>();
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, ctr, 16, 0);
 	RectView v = make_view(h);
@@ -1312,7 +1348,7 @@ int vgpu_destroy(vgpu_container *h) {
 			&h->touch, &h->d_flag, &h->retry3, &h->retry4, &h->img_p, &h->img_id,
 			&h->img_reg, &h->img_rank, &h->img_bc, &h->img_cnt, &h->fq_xyz, &h->fq_found, &h->fq_pid,
 			&h->fq_pos, &h->fq_retry, &h->fq_retry2, &h->fq_r, &h->fq_vol, &h->fq_nbr, &h->fq_touch,
-			&h->fq_list, &h->img_okb};
+			&h->fq_list, &h->img_okb, &h->d_prev, &h->d_vkeys, &h->d_vblks, &h->d_tacc};
 	for (DBuf *b : bufs) b->release();
 	h->ws_fast.buf.release(); h->ws_wide.buf.release(); h->ws_huge.buf.release(); h->ws_wave.buf.release();
 	h->ws_wave4.buf.release();

@@ -144,6 +144,39 @@ __global__ void k_fill_u8(unsigned char *p, int n, unsigned char v) {
 	if (t < n) p[t] = v;
 }
 
+// Periodic containers' replay bookkeeping.  The image blocks (rows outside
+// [ey, wy) x [ez, wz)) that some cell visited, with their first-visit keys,
+// in any order (the host sorts them by key); entries beyond cap are only
+// counted.
+__global__ void k_prd_visited(int nb, const unsigned long long *__restrict__ fv, int nx, int oy, int ey, int wy,
+			      int ez, int wz, unsigned long long *__restrict__ keys, int *__restrict__ blks, int cap,
+			      int *__restrict__ count) {
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= nb) return;
+	const unsigned long long f = fv[b];
+	if (f == ~0ull) return;
+	const int j = (b / nx) % oy, k = b / (nx * oy);
+	if (j >= ey && j < wy && k >= ez && k < wz) return;   // a primary block
+	const int i = atomicAdd(count, 1);
+	if (i < cap) { keys[i] = f; blks[i] = b; }
+}
+// The cells that read an image block in any pass so far (acc |= touch), as
+// a list in any order.
+__global__ void k_prd_touched(int nc, const unsigned char *__restrict__ touch, unsigned char *__restrict__ acc,
+			      int *__restrict__ list, int *__restrict__ count) {
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= nc) return;
+	const unsigned char a = acc[c] | touch[c];
+	acc[c] = a;
+	if (a) list[atomicAdd(count, 1)] = c;
+}
+// *flag = 1 if a and b differ anywhere.
+__global__ void k_differ_u64(int n, const unsigned long long *__restrict__ a, const unsigned long long *__restrict__ b,
+			     int *__restrict__ flag) {
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < n && a[t] != b[t]) *flag = 1;
+}
+
 // Exclusive scan, tiles of 1024 ints by 256 threads; tile totals to part.
 __global__ void k_scan_tiles(const int *__restrict__ in, int *__restrict__ out, int n, int *__restrict__ part) {
 	__shared__ int sh[256];
