@@ -469,11 +469,15 @@ struct WCellT {
 				a += vz[h] * pz - prsq;
 				vtv[h] = a;
 				if constexpr (K == 1) {
-					posm = live_ballot(a > tol);
+					// a plane that leaves every vertex below -tol (most planes)
+					// needs no other mask
 					negm = live_ballot(a < -tol);
-					nearm = live_ballot(a < -tol && a > -big_tol);
 					const unsigned long long live = p >= 64 ? ~0ull : (1ull << p) - 1;
 					any = (live & ~negm) != 0;
+					if (any) {
+						posm = live_ballot(a > tol);
+						nearm = live_ballot(a < -tol && a > -big_tol);
+					}
 				} else any = any || live_ballot(!(a < -tol), h) != 0;
 			}
 		});
