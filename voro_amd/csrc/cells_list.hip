@@ -107,6 +107,23 @@ __global__ void k_list_fill(RectView v, int nc, const int *__restrict__ cellblk,
 	}
 }
 
+// The contiguous lists (wavefront cells): one wavefront per cell, lane g
+// writing record g, so that a list is written in full cache lines (one
+// thread per cell wrote each 32-byte record on its own line: 4.7 GB of
+// writes for 1.7 GB of records).  Four cells per workgroup.
+template<bool PRD>
+__global__ void __launch_bounds__(256) k_list_fill_lanes(RectView v, int nc, const int *__restrict__ cellblk,
+							  const int *__restrict__ order, const unsigned char *__restrict__ depth,
+							  int per_cell, const int *__restrict__ boff, ListRec *__restrict__ LA) {
+	const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), g = (int)(threadIdx.x & 63);
+	if (i >= nc) return;
+	const int c = order[i];
+	const int G = depth[per_cell ? c : cellblk[c]];
+	if (G >= 64 || g > G) return;
+	const ListCtx x = list_ctx<PRD>(v, cellblk[c], c);
+	LA[(size_t)boff[i] + g] = list_record<PRD>(v, x, g, G);
+}
+
 // One wavefront per workgroup, a batch of 64 cells at a time (batches
 // taken in order through an atomic counter).  Cells whose list ends before
 // the radius test stops them, and cells that overflow the order-3 tier, go to
@@ -483,12 +500,20 @@ int launch_list_gather(bool prd, hipStream_t s, const RectView &v, int nc, const
 int launch_list_fill(bool prd, hipStream_t s, const RectView &v, int nc, const int *cellblk, const int *order,
 		     const unsigned char *depth, int per_cell, const int *boff, bool csr, ListRec *LA) {
 	if (v.fv) return 2;
-	if (prd)
+	if (csr) {
+		const unsigned blocks = (unsigned)(((long long)nc + 3) / 4);
+		if (prd)
+			hipLaunchKernelGGL(k_list_fill_lanes<true>, dim3(blocks), dim3(256), 0, s, v, nc, cellblk, order, depth,
+					   per_cell, boff, LA);
+		else
+			hipLaunchKernelGGL(k_list_fill_lanes<false>, dim3(blocks), dim3(256), 0, s, v, nc, cellblk, order, depth,
+					   per_cell, boff, LA);
+	} else if (prd)
 		hipLaunchKernelGGL(k_list_fill<true>, dim3((nc + 255) / 256), dim3(256), 0, s, v, nc, cellblk, order, depth,
-				   per_cell, boff, csr ? 1 : 0, LA);
+				   per_cell, boff, 0, LA);
 	else
 		hipLaunchKernelGGL(k_list_fill<false>, dim3((nc + 255) / 256), dim3(256), 0, s, v, nc, cellblk, order, depth,
-				   per_cell, boff, csr ? 1 : 0, LA);
+				   per_cell, boff, 0, LA);
 	return hipGetLastError() == hipSuccess ? 0 : 6;
 }
 
