@@ -339,9 +339,10 @@ __global__ void k_batch_features(int nb, const int *__restrict__ bcont, const Re
 // Candidate lists of a batch's cells, schedule position i (cell order[i])
 // by thread i, in one pass: the length (depth + 1 records), a
 // wavefront-wide scan, one 64-bit bump of *top per wavefront, then the
-// fill.  A list that would pass `cap` records is not written: its length is
-// set to 0 and the cell takes the plain path (*top still grows by it, so the
-// caller learns how much room everything needed).  loff[i] / lcnt[i]: the
+// fill, one list at a time by the whole wavefront.  A list that would
+// pass `cap` records is not written: its length is set to 0 and the cell
+// takes the plain path (*top still grows by it, so the caller learns how
+// much room everything needed).  loff[i] / lcnt[i]: the
 // list's first record and length.  Sizes are read on the device, so the
 // sequence needs no host round trip.
 __global__ void __launch_bounds__(256)
@@ -369,15 +370,22 @@ k_batch_list_build(const RectView *__restrict__ views, const int *__restrict__ b
 	if (lane == 63 && total > 0) base = atomicAdd(top, (unsigned long long)total);
 	base = __shfl(base, 63);
 	const unsigned long long off = base + (unsigned long long)(inc - cnt);
+	const bool ok = i < nc && cnt > 0 && off + (unsigned long long)cnt <= (unsigned long long)cap;
 	if (i < n) {
-		if (i < nc && cnt > 0 && off + (unsigned long long)cnt <= (unsigned long long)cap) {
-			const RectView &v = views[bcont[gb]];
-			gather_cell_list<false>(v, gb - v.boff, cell, depth[gb], LA + off, 1);
-			lcnt[i] = cnt;
-			loff[i] = (int)off;
-		} else {
-			lcnt[i] = 0;
-			loff[i] = 0;
+		lcnt[i] = ok ? cnt : 0;
+		loff[i] = ok ? (int)off : 0;
+	}
+	// The records: each list written by the whole wavefront, lane g writing
+	// record g, so that every list is one run of full cache lines.
+	for (unsigned long long m = __ballot(ok); m; m &= m - 1) {
+		const int src = __builtin_ctzll(m);
+		const int cj = __shfl(cell, src), gbj = __shfl(gb, src), G = __shfl(cnt, src) - 1;
+		const unsigned long long oj = (unsigned long long)(unsigned)__shfl((int)(unsigned)off, src) |
+					      ((unsigned long long)(unsigned)__shfl((int)(unsigned)(off >> 32), src) << 32);
+		if (lane <= G) {
+			const RectView &v = views[bcont[gbj]];
+			const ListCtx x = list_ctx<false>(v, gbj - v.boff, cj);
+			LA[oj + lane] = list_record<false>(v, x, lane, G);
 		}
 	}
 }
